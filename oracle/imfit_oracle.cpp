@@ -1,0 +1,1218 @@
+// imfit_oracle.cpp -- CPU FP64 restatement of imfit's model-image hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the GPU library in
+// imfit_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg load it.  The product path never calls it.
+//
+// Every function restates the reference algorithm it cites (paths relative to the
+// reference tree).  The arithmetic order of the reference is kept where it decides
+// rounding (Kahan component sums, sub-pixel loops, weight formulas), and the file
+// is compiled with -ffp-contract=off, like the reference's x86-64 -O3 build (no
+// FMA contraction without -march).
+//
+// Pinning (what checks each part; see tests/test_oracle.py):
+//   * GetValue of Sersic/Exponential/Gaussian/Moffat/FlatSky: unit_tests/unittest_funcs.t.h
+//     known answers; Calculate_bn against the reference's own helper_funcs.cpp built
+//     into oracle/_ref.
+//   * model images: tests/sersic+exp_orig.fits, tests/biggertest_orig.fits (float32).
+//   * chi^2 / Cash: imfit_textout1, imfit_textout3, imfit_textout4c, imfit_textout4d.
+//   * convolution: tests/convolve_out_n5419.fits, exact direct convolution, numpy.fft.
+//   * oversampled regions: tests/oversampled_orig.fits, tests/oversampled2both_orig.fits;
+//     DownsampleAndReplace against the reference's downsample.cpp in oracle/_ref.
+//   * mp_enorm: against the reference's mp_enorm.cpp in oracle/_ref.
+//   * QAGS (ExponentialDisk3D): scipy.integrate.quad (QUADPACK dqagse, the algorithm
+//     GSL's qags reimplements) fixtures and closed forms.
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cfloat>
+#include <string>
+#include <vector>
+#include <complex>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using std::vector;
+typedef std::complex<double> cplx;
+
+namespace {
+
+const double DEG2RAD = 0.017453292519943295;   // function_objects/func_sersic.cpp:65
+const int SUBSAMPLE_R = 10;                    // function_objects/func_sersic.cpp:67
+const double LOG_SMALL_VALUE = 1.0e-25;        // core/model_object.cpp:86
+const int WEIGHTS_ARE_SIGMAS = 100, WEIGHTS_ARE_VARIANCES = 110, WEIGHTS_ARE_WEIGHTS = 120;
+const int MASK_ZERO_IS_GOOD = 10, MASK_ZERO_IS_BAD = 20;   // core/definitions.h:76-82
+
+// Calculate_bn (function_objects/helper_funcs.cpp:14-29): Ciotti & Bertin (1999)
+// asymptotic series for n > 0.36, MacArthur et al. (2003) polynomial below.
+double Calculate_bn(double n) {
+  double n2 = n * n;
+  if (n > 0.36)
+    return 2 * n - 0.333333333333333 + 0.009876543209876543 / n + 0.0018028610621203215 / n2 +
+           0.00011409410586365319 / (n2 * n) - 7.1510122958919723e-05 / (n2 * n2);
+  return 0.01945 + -0.8902 * n + 10.95 * n2 + -19.67 * n2 * n + 13.43 * n2 * n2;
+}
+
+// ---------------------------------------------------------------- function objects
+struct Func {
+  double x0 = 0, y0 = 0;
+  bool doSubsampling = true;
+  virtual ~Func() {}
+  virtual int NParams() const = 0;
+  virtual void Setup(const double* p, int off, double xc, double yc) = 0;
+  virtual double GetValue(double x, double y) = 0;
+};
+
+// Shared structure of the 2-D elliptical functions' GetValue: rotate into the
+// component frame, scale y by 1/q, pick the GALFIT-style sub-pixel count from r and
+// average an nSub x nSub grid whose first sample sits at x - 0.5 + 0.5/nSub,
+// x outer / y inner (function_objects/func_sersic.cpp:150-188,
+// func_exp.cpp:134-172, func_gaussian.cpp:142-178, func_moffat.cpp:152-188).
+template <class F>
+double EllipticalGetValue(F& f, double x, double y) {
+  double x_diff = x - f.x0, y_diff = y - f.y0;
+  double xp = x_diff * f.cosPA + y_diff * f.sinPA;
+  double yp_scaled = (-x_diff * f.sinPA + y_diff * f.cosPA) / f.q;
+  double r = sqrt(xp * xp + yp_scaled * yp_scaled);
+  int nSub = f.CalculateSubsamples(r);
+  if (nSub > 1) {
+    double deltaSubpix = 1.0 / nSub;
+    double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+    double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+    double theSum = 0.0;
+    for (int ii = 0; ii < nSub; ii++) {
+      double x_ii = x_sub_start + ii * deltaSubpix;
+      for (int jj = 0; jj < nSub; jj++) {
+        double y_ii = y_sub_start + jj * deltaSubpix;
+        x_diff = x_ii - f.x0;
+        y_diff = y_ii - f.y0;
+        xp = x_diff * f.cosPA + y_diff * f.sinPA;
+        yp_scaled = (-x_diff * f.sinPA + y_diff * f.cosPA) / f.q;
+        r = sqrt(xp * xp + yp_scaled * yp_scaled);
+        theSum += f.CalculateIntensity(r);
+      }
+    }
+    return theSum / (nSub * nSub);
+  }
+  return f.CalculateIntensity(r);
+}
+
+// GALFIT sub-pixel rule (func_sersic.cpp:197-213 with rLim=4, scale=2*SUBSAMPLE_R;
+// func_exp.cpp:181-197 rLim=3; func_gaussian.cpp:187-203 scale 4*SUBSAMPLE_R;
+// func_moffat.cpp:200-216 scale 4*SUBSAMPLE_R).
+inline int GalfitSubsamples(bool on, double r, double size, double smallScale, double rLim) {
+  int nSamples = 1;
+  if (on && r < 10.0) {
+    if (size <= 1.0 && r <= 1.0)
+      nSamples = std::min(100, (int)(smallScale / size));
+    else if (r <= rLim)
+      nSamples = 2 * SUBSAMPLE_R;
+    else
+      nSamples = std::min(100, (int)(2 * SUBSAMPLE_R / r));
+  }
+  return nSamples;
+}
+
+struct Sersic : Func {   // function_objects/func_sersic.cpp:109-213
+  double PA, ell, n, I_e, r_e, q, PA_rad, cosPA, sinPA, bn, invn;
+  int NParams() const override { return 5; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; ell = p[o + 1]; n = p[o + 2]; I_e = p[o + 3]; r_e = p[o + 4];
+    q = 1.0 - ell;
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    bn = Calculate_bn(n);
+    invn = 1.0 / n;
+  }
+  double CalculateIntensity(double r) const { return I_e * exp(-bn * (pow((r / r_e), invn) - 1.0)); }
+  int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, r_e, 2 * SUBSAMPLE_R, 4.0); }
+  double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+struct Exponential : Func {   // function_objects/func_exp.cpp:104-197
+  double PA, ell, I_0, h, q, PA_rad, cosPA, sinPA;
+  int NParams() const override { return 4; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; ell = p[o + 1]; I_0 = p[o + 2]; h = p[o + 3];
+    q = 1.0 - ell;
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+  }
+  double CalculateIntensity(double r) const { return I_0 * exp(-r / h); }
+  int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, h, 2 * SUBSAMPLE_R, 3.0); }
+  double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+struct Gaussian : Func {   // function_objects/func_gaussian.cpp:102-203
+  double PA, ell, I_0, sigma, q, PA_rad, cosPA, sinPA, twosigma_squared;
+  int NParams() const override { return 4; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; ell = p[o + 1]; I_0 = p[o + 2]; sigma = p[o + 3];
+    q = 1.0 - ell;
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    twosigma_squared = 2.0 * sigma * sigma;
+  }
+  double CalculateIntensity(double r) const {
+    double r_squared = r * r;
+    return I_0 * exp(-r_squared / twosigma_squared);
+  }
+  int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, sigma, 4 * SUBSAMPLE_R, 3.0); }
+  double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+struct Moffat : Func {   // function_objects/func_moffat.cpp:111-216
+  double PA, ell, I_0, fwhm, beta, q, PA_rad, cosPA, sinPA, alpha;
+  int NParams() const override { return 5; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; ell = p[o + 1]; I_0 = p[o + 2]; fwhm = p[o + 3]; beta = p[o + 4];
+    q = 1.0 - ell;
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    double exponent = pow(2.0, 1.0 / beta);
+    alpha = 0.5 * fwhm / sqrt(exponent - 1.0);
+  }
+  double CalculateIntensity(double r) const {
+    double scaledR = r / alpha;
+    double denominator = pow((1.0 + scaledR * scaledR), beta);
+    return I_0 / denominator;
+  }
+  int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, alpha, 4 * SUBSAMPLE_R, 3.0); }
+  double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+struct FlatSky : Func {   // function_objects/func_flatsky.cpp:85-98
+  double I_sky;
+  int NParams() const override { return 1; }
+  void Setup(const double* p, int o, double xc, double yc) override { x0 = xc; y0 = yc; I_sky = p[o]; }
+  double GetValue(double, double) override { return I_sky; }
+};
+
+// ------------------------------------------------------------------ QAGS restatement
+// QUADPACK QAGS as reimplemented by GSL 2.5 (gsl_integration_qags, called from
+// function_objects/integrator.cpp:42-55 with epsabs=0, epsrel=1e-6, limit=1000):
+// 21-point Gauss-Kronrod on the initial interval, then repeated bisection of the
+// interval with the largest error estimate, with Wynn's epsilon-algorithm
+// extrapolation over the sequence of area estimates (Piessens et al. 1983,
+// "QUADPACK", ch. 2-3; routines DQAGSE, DQK21, DQPSRT, DQELG).
+// GSL is un-vendored (CHANGELOG.md:225); this is restated from the published algorithm.
+const double XGK21[11] = {0.995657163025808080735527280689003, 0.973906528517171720077964012084452,
+                          0.930157491355708226001207180059508, 0.865063366688984510732096688423493,
+                          0.780817726586416897063717578345042, 0.679409568299024406234327365114874,
+                          0.562757134668604683339000099272694, 0.433395394129247190799265943165784,
+                          0.294392862701460198131126603103866, 0.148874338981631210884826001129720,
+                          0.000000000000000000000000000000000};
+const double WG10[5] = {0.066671344308688137593568809893332, 0.149451349150580593145776339657697,
+                        0.219086362515982043995534934228163, 0.269266719309996355091226921569469,
+                        0.295524224714752870173892994651338};
+const double WGK21[11] = {0.011694638867371874278064396062192, 0.032558162307964727478818972459390,
+                          0.054755896574351996031381300244580, 0.075039674810919952767043140916190,
+                          0.093125454583697605535065465083366, 0.109387158802297641899210590325805,
+                          0.123491976262065851077600525478480, 0.134709217311473325928054001771707,
+                          0.142775938577060080797094273138717, 0.147739104901338491374841515972068,
+                          0.149445554002916905664936468389821};
+
+template <class Fn>
+double qk21(Fn& f, double a, double b, double* abserr, double* resabs, double* resasc) {
+  const double center = 0.5 * (a + b), half_length = 0.5 * (b - a);
+  const double abs_half_length = fabs(half_length);
+  const double f_center = f(center);
+  double result_gauss = 0.0, result_kronrod = f_center * WGK21[10];
+  double result_abs = fabs(result_kronrod), result_asc, mean, err;
+  double fv1[10], fv2[10];
+  for (int j = 0; j < 5; j++) {   // Gauss nodes are the odd Kronrod indices
+    const int jtw = 2 * j + 1;
+    const double absc = half_length * XGK21[jtw];
+    const double fval1 = f(center - absc), fval2 = f(center + absc);
+    const double fsum = fval1 + fval2;
+    fv1[jtw] = fval1; fv2[jtw] = fval2;
+    result_gauss += WG10[j] * fsum;
+    result_kronrod += WGK21[jtw] * fsum;
+    result_abs += WGK21[jtw] * (fabs(fval1) + fabs(fval2));
+  }
+  for (int j = 0; j < 5; j++) {
+    const int jtwm1 = 2 * j;
+    const double absc = half_length * XGK21[jtwm1];
+    const double fval1 = f(center - absc), fval2 = f(center + absc);
+    fv1[jtwm1] = fval1; fv2[jtwm1] = fval2;
+    result_kronrod += WGK21[jtwm1] * (fval1 + fval2);
+    result_abs += WGK21[jtwm1] * (fabs(fval1) + fabs(fval2));
+  }
+  mean = result_kronrod * 0.5;
+  result_asc = WGK21[10] * fabs(f_center - mean);
+  for (int j = 0; j < 10; j++)
+    result_asc += WGK21[j] * (fabs(fv1[j] - mean) + fabs(fv2[j] - mean));
+  err = (result_kronrod - result_gauss) * half_length;
+  result_kronrod *= half_length;
+  result_abs *= abs_half_length;
+  result_asc *= abs_half_length;
+  // DQK21 rescaling of the error estimate
+  err = fabs(err);
+  if (result_asc != 0 && err != 0) {
+    double scale = pow((200 * err / result_asc), 1.5);
+    err = (scale < 1) ? result_asc * scale : result_asc;
+  }
+  if (result_abs > DBL_MIN / (50 * DBL_EPSILON)) {
+    double min_err = 50 * DBL_EPSILON * result_abs;
+    if (min_err > err) err = min_err;
+  }
+  *abserr = err; *resabs = result_abs; *resasc = result_asc;
+  return result_kronrod;
+}
+
+struct QagsWork {   // DQAGSE's alist/blist/rlist/elist/iord/level arrays
+  int limit, size = 0, nrmax = 0, i = 0, maximum_level = 0;
+  vector<double> alist, blist, rlist, elist;
+  vector<int> order, level;
+  explicit QagsWork(int lim) : limit(lim), alist(lim), blist(lim), rlist(lim), elist(lim), order(lim), level(lim) {}
+  void init(double a, double b, double r, double e) {
+    size = 1; nrmax = 0; i = 0; maximum_level = 0;
+    alist[0] = a; blist[0] = b; rlist[0] = r; elist[0] = e; order[0] = 0; level[0] = 0;
+  }
+  // DQPSRT: keep `order` sorted by decreasing error after inserting the new pieces
+  void qpsrt() {
+    const int last = size - 1;
+    int i_nrmax = nrmax, i_maxerr = order[i_nrmax];
+    if (last < 2) {   // two intervals: the larger-error piece already sits at index 0
+      order[0] = 0;
+      order[1] = 1;
+      i = i_maxerr;
+      return;
+    }
+    double errmax = elist[i_maxerr];
+    while (i_nrmax > 0 && errmax > elist[order[i_nrmax - 1]]) {
+      order[i_nrmax] = order[i_nrmax - 1];
+      i_nrmax--;
+    }
+    int top = (last < (limit / 2 + 2)) ? last : limit - last + 1;
+    int idx = i_nrmax + 1;
+    while (idx < top && errmax < elist[order[idx]]) {
+      order[idx - 1] = order[idx];
+      idx++;
+    }
+    order[idx - 1] = i_maxerr;
+    double errmin = elist[last];
+    int k = top - 1;
+    while (k > idx - 2 && errmin >= elist[order[k]]) {
+      order[k + 1] = order[k];
+      k--;
+    }
+    order[k + 1] = last;
+    i = order[i_nrmax];
+    nrmax = i_nrmax;
+  }
+  void update(double a1, double b1, double area1, double error1, double a2, double b2, double area2, double error2) {
+    const int i_max = i, i_new = size, new_level = level[i_max] + 1;
+    if (error2 > error1) {
+      alist[i_max] = a2; rlist[i_max] = area2; elist[i_max] = error2; level[i_max] = new_level;
+      alist[i_new] = a1; blist[i_new] = b1; rlist[i_new] = area1; elist[i_new] = error1; level[i_new] = new_level;
+    } else {
+      blist[i_max] = b1; rlist[i_max] = area1; elist[i_max] = error1; level[i_max] = new_level;
+      alist[i_new] = a2; blist[i_new] = b2; rlist[i_new] = area2; elist[i_new] = error2; level[i_new] = new_level;
+    }
+    size++;
+    if (new_level > maximum_level) maximum_level = new_level;
+    qpsrt();
+  }
+  double sum_results() const {
+    double s = 0;
+    for (int k = 0; k < size; k++) s += rlist[k];
+    return s;
+  }
+  bool large_interval() const { return level[i] < maximum_level; }
+  void reset_nrmax() { nrmax = 0; i = order[0]; }
+  bool increase_nrmax() {
+    int id = nrmax, jupbnd;
+    const int last = size - 1;
+    jupbnd = (last > (1 + limit / 2)) ? limit + 1 - last : last;
+    for (int k = id; k <= jupbnd; k++) {
+      int i_max = order[nrmax];
+      i = i_max;
+      if (level[i_max] < maximum_level) return true;
+      nrmax++;
+    }
+    return false;
+  }
+};
+
+struct ExtrapTable {   // DQELG's epsilon table
+  int n = 0, nres = 0;
+  double rlist2[52], res3la[3];
+  void append(double y) {
+    if (n < 52) { rlist2[n] = y; n++; }
+  }
+  // Wynn's epsilon algorithm (DQELG)
+  void qelg(double* result, double* abserr) {
+    double* epstab = rlist2;
+    const int n0 = n - 1;
+    const double current = epstab[n0];
+    double absolute = DBL_MAX, relative = 5 * DBL_EPSILON * fabs(current);
+    const int newelm = n0 / 2, n_orig = n0;
+    int n_final = n0;
+    const int nres_orig = nres;
+    *result = current;
+    *abserr = DBL_MAX;
+    if (n0 < 2) {
+      *result = current;
+      *abserr = std::max(absolute, relative);
+      return;
+    }
+    epstab[n0 + 2] = epstab[n0];
+    epstab[n0] = DBL_MAX;
+    for (int i = 0; i < newelm; i++) {
+      double res = epstab[n0 - 2 * i + 2];
+      double e0 = epstab[n0 - 2 * i - 2], e1 = epstab[n0 - 2 * i - 1], e2 = res;
+      double e1abs = fabs(e1), delta2 = e2 - e1, err2 = fabs(delta2);
+      double tol2 = std::max(fabs(e2), e1abs) * DBL_EPSILON;
+      double delta3 = e1 - e0, err3 = fabs(delta3);
+      double tol3 = std::max(e1abs, fabs(e0)) * DBL_EPSILON;
+      double e3, delta1, err1, tol1, ss;
+      if (err2 <= tol2 && err3 <= tol3) {
+        *result = res;
+        absolute = err2 + err3;
+        relative = 5 * DBL_EPSILON * fabs(res);
+        *abserr = std::max(absolute, relative);
+        return;
+      }
+      e3 = epstab[n0 - 2 * i];
+      epstab[n0 - 2 * i] = e1;
+      delta1 = e1 - e3;
+      err1 = fabs(delta1);
+      tol1 = std::max(e1abs, fabs(e3)) * DBL_EPSILON;
+      if (err1 <= tol1 || err2 <= tol2 || err3 <= tol3) {
+        n_final = 2 * i;
+        break;
+      }
+      ss = (1 / delta1 + 1 / delta2) - 1 / delta3;
+      if (fabs(ss * e1) <= 0.0001) {
+        n_final = 2 * i;
+        break;
+      }
+      res = e1 + 1 / ss;
+      epstab[n0 - 2 * i] = res;
+      {
+        const double error = err2 + fabs(res - e2) + err3;
+        if (error <= *abserr) {
+          *abserr = error;
+          *result = res;
+        }
+      }
+    }
+    {
+      const int limexp = 50 - 1;
+      if (n_final == limexp) n_final = 2 * (limexp / 2);
+    }
+    if (n_orig % 2 == 1) {
+      for (int i = 0; i <= newelm; i++) epstab[1 + i * 2] = epstab[i * 2 + 3];
+    } else {
+      for (int i = 0; i <= newelm; i++) epstab[i * 2] = epstab[i * 2 + 2];
+    }
+    if (n_orig != n_final) {
+      for (int i = 0; i <= n_final; i++) epstab[i] = epstab[n_orig - n_final + i];
+    }
+    n = n_final + 1;
+    if (nres_orig < 3) {
+      res3la[nres_orig] = *result;
+      *abserr = DBL_MAX;
+    } else {
+      *abserr = (fabs(*result - res3la[2]) + fabs(*result - res3la[1]) + fabs(*result - res3la[0]));
+      res3la[0] = res3la[1];
+      res3la[1] = res3la[2];
+      res3la[2] = *result;
+    }
+    nres = nres_orig + 1;
+    *abserr = std::max(*abserr, 5 * DBL_EPSILON * fabs(*result));
+  }
+};
+
+inline bool subinterval_too_small(double a1, double a2, double b2) {
+  const double e = DBL_EPSILON, u = DBL_MIN;
+  double tmp = (1 + 100 * e) * (fabs(a2) + 1000 * u);
+  return fabs(a1) <= tmp && fabs(b2) <= tmp;
+}
+
+template <class Fn>
+double qags(Fn& f, double a, double b, double epsabs, double epsrel, int limit, double* abserr_out, int* neval) {
+  QagsWork w(limit);
+  ExtrapTable table;
+  double area, errsum, res_ext, err_ext, result0, abserr0, resabs0, resasc0, tolerance;
+  double ertest = 0, error_over_large_intervals = 0, reseps = 0, abseps = 0, correc = 0;
+  int ktmin = 0, roundoff_type1 = 0, roundoff_type2 = 0, roundoff_type3 = 0;
+  int error_type = 0, error_type2 = 0, iteration = 0;
+  bool positive_integrand, extrapolate = false, disallow_extrapolation = false;
+  *neval = 21;
+  result0 = qk21(f, a, b, &abserr0, &resabs0, &resasc0);
+  tolerance = std::max(epsabs, epsrel * fabs(result0));
+  const double round_off = 50 * DBL_EPSILON * resabs0;
+  if (abserr0 <= round_off && abserr0 > tolerance) { *abserr_out = abserr0; return result0; }
+  if ((abserr0 <= tolerance && abserr0 != resasc0) || abserr0 == 0.0) { *abserr_out = abserr0; return result0; }
+  if (limit == 1) { *abserr_out = abserr0; return result0; }
+  w.init(a, b, result0, abserr0);
+  table.append(result0);
+  area = result0;
+  errsum = abserr0;
+  res_ext = result0;
+  err_ext = DBL_MAX;
+  positive_integrand = fabs(result0) >= (1 - 50 * DBL_EPSILON) * resabs0;
+  iteration = 1;
+  do {
+    const int current_level = w.level[w.i] + 1;
+    const double a_i = w.alist[w.i], b_i = w.blist[w.i], r_i = w.rlist[w.i], e_i = w.elist[w.i];
+    const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
+    double error1, error2, resabs1, resabs2, resasc1, resasc2;
+    iteration++;
+    const double area1 = qk21(f, a1, b1, &error1, &resabs1, &resasc1);
+    const double area2 = qk21(f, a2, b2, &error2, &resabs2, &resasc2);
+    *neval += 42;
+    const double area12 = area1 + area2, error12 = error1 + error2;
+    const double last_e_i = e_i;
+    errsum = errsum + error12 - e_i;
+    area = area + area12 - r_i;
+    tolerance = std::max(epsabs, epsrel * fabs(area));
+    if (resasc1 != error1 && resasc2 != error2) {
+      const double delta = r_i - area12;
+      if (fabs(delta) <= 1.0e-5 * fabs(area12) && error12 >= 0.99 * e_i) {
+        if (!extrapolate) roundoff_type1++;
+        else roundoff_type2++;
+      }
+      if (iteration > 10 && error12 > e_i) roundoff_type3++;
+    }
+    if (roundoff_type1 + roundoff_type2 >= 10 || roundoff_type3 >= 20) error_type = 2;
+    if (roundoff_type2 >= 5) error_type2 = 1;
+    if (subinterval_too_small(a1, a2, b2)) error_type = 4;
+    w.update(a1, b1, area1, error1, a2, b2, area2, error2);
+    if (errsum <= tolerance) goto compute_result;
+    if (error_type) break;
+    if (iteration >= limit - 1) { error_type = 1; break; }
+    if (iteration == 2) {
+      error_over_large_intervals = errsum;
+      ertest = tolerance;
+      table.append(area);
+      continue;
+    }
+    if (disallow_extrapolation) continue;
+    error_over_large_intervals += -last_e_i;
+    if (current_level < w.maximum_level) error_over_large_intervals += error12;
+    if (!extrapolate) {
+      if (w.large_interval()) continue;
+      extrapolate = true;
+      w.nrmax = 1;
+    }
+    if (!error_type2 && error_over_large_intervals > ertest) {
+      if (w.increase_nrmax()) continue;
+    }
+    table.append(area);
+    table.qelg(&reseps, &abseps);
+    ktmin++;
+    if (ktmin > 5 && err_ext < 0.001 * errsum) error_type = 5;
+    if (abseps < err_ext) {
+      ktmin = 0;
+      err_ext = abseps;
+      res_ext = reseps;
+      correc = error_over_large_intervals;
+      ertest = std::max(epsabs, epsrel * fabs(reseps));
+      if (err_ext <= ertest) break;
+    }
+    if (table.n == 1) disallow_extrapolation = true;
+    if (error_type == 5) break;
+    w.reset_nrmax();
+    extrapolate = false;
+    error_over_large_intervals = errsum;
+  } while (iteration < limit);
+
+  *abserr_out = err_ext;
+  if (err_ext == DBL_MAX) goto compute_result;
+  if (error_type || error_type2) {
+    if (error_type2) err_ext += correc;
+    if (error_type == 0) error_type = 3;
+    if (res_ext != 0.0 && area != 0.0) {
+      if (err_ext / fabs(res_ext) > errsum / fabs(area)) goto compute_result;
+    } else if (err_ext > errsum) {
+      goto compute_result;
+    } else if (area == 0.0) {
+      *abserr_out = err_ext;
+      return res_ext;
+    }
+  }
+  *abserr_out = err_ext;
+  return res_ext;
+compute_result:
+  *abserr_out = errsum;
+  return w.sum_results();
+}
+
+// ExponentialDisk3D (function_objects/func_expdisk3d.cpp:149-268): line-of-sight
+// integral of a 3-D exponential disk with sech^(2/n) vertical profile, integrated
+// over s in [-20h, +20h] with QAGS at epsrel 1e-6 (integrator.cpp:42-55).
+struct ExpDisk3D : Func {
+  double PA, inclination, J_0, h, n, z_0, PA_rad, cosPA, sinPA, inc_rad, cosInc, sinInc;
+  double alpha, scaledZ0, two_to_alpha;
+  ExpDisk3D() { doSubsampling = false; }
+  int NParams() const override { return 6; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; inclination = p[o + 1]; J_0 = p[o + 2]; h = p[o + 3]; n = p[o + 4]; z_0 = p[o + 5];
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    inc_rad = inclination * DEG2RAD;
+    cosInc = cos(inc_rad); sinInc = sin(inc_rad);
+    alpha = 2.0 / n;
+    scaledZ0 = alpha * z_0;
+    two_to_alpha = pow(2.0, alpha);
+  }
+  struct Density {   // LuminosityDensity (func_expdisk3d.cpp:232-268); COSH_LIMIT = 100
+    double x_d0, y_d0, z_d0, cosInc, sinInc, J_0, h, z_0, scaledZ0, two_to_alpha, alpha;
+    double operator()(double s) const {
+      double y_d = y_d0 + s * sinInc;
+      double z_d = z_d0 - s * cosInc;
+      double R = sqrt(x_d0 * x_d0 + y_d * y_d);
+      double z = fabs(z_d);
+      double verticalScaling;
+      if ((z / scaledZ0) > 100.0)
+        verticalScaling = two_to_alpha * exp(-z / z_0);
+      else {
+        double sech = 1.0 / cosh(z / scaledZ0);
+        verticalScaling = pow(sech, alpha);
+      }
+      return J_0 * exp(-R / h) * verticalScaling;
+    }
+  };
+  Density Make(double x, double y) const {
+    double x_diff = x - x0, y_diff = y - y0;
+    double xp = x_diff * cosPA + y_diff * sinPA;
+    double yp = -x_diff * sinPA + y_diff * cosPA;
+    return Density{xp, yp * cosInc, yp * sinInc, cosInc, sinInc, J_0, h, z_0, scaledZ0, two_to_alpha, alpha};
+  }
+  double GetValue(double x, double y) override {
+    Density F = Make(x, y);
+    double integLimit = 20 * h, err;
+    int neval;
+    return qags(F, -integLimit, integLimit, 0.0, 1.0e-6, 1000, &err, &neval);
+  }
+};
+
+Func* MakeFunc(const char* name) {   // core/add_functions.cpp:163-385 (name -> class)
+  std::string s(name);
+  if (s == "Sersic") return new Sersic();
+  if (s == "Exponential") return new Exponential();
+  if (s == "Gaussian") return new Gaussian();
+  if (s == "Moffat") return new Moffat();
+  if (s == "FlatSky") return new FlatSky();
+  if (s == "ExponentialDisk3D") return new ExpDisk3D();
+  return nullptr;
+}
+
+// ------------------------------------------------------------------ CPU FFT
+// Mixed-radix (2,3,4,5,7) Stockham complex FFT, used where the reference calls
+// FFTW (core/convolver.cpp:231-282,365,395).  Sizes are padded to the next
+// 2^a 3^b 5^c 7^d length, which leaves the *linear* convolution unchanged.
+int NextSmooth(int n) {
+  for (int m = std::max(n, 1);; m++) {
+    int k = m;
+    for (int p : {2, 3, 5, 7})
+      while (k % p == 0) k /= p;
+    if (k == 1) return m;
+  }
+}
+
+struct FFT1D {
+  int N = 0;
+  vector<int> radices;
+  vector<cplx> tw;   // exp(-2 pi i k / N)
+  void plan(int n) {
+    N = n;
+    radices.clear();
+    int k = n;
+    while (k % 4 == 0) { radices.push_back(4); k /= 4; }
+    for (int p : {2, 3, 5, 7})
+      while (k % p == 0) { radices.push_back(p); k /= p; }
+    if (k != 1) { fprintf(stderr, "oracle FFT: unsupported length %d\n", n); abort(); }
+    tw.resize(n);
+    for (int j = 0; j < n; j++) {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+      tw[j] = cplx((double)cosl(ang), (double)sinl(ang));
+    }
+  }
+  // x: N values at stride `stride`; work: N scratch; sign -1 forward, +1 inverse.
+  void run(cplx* x, int stride, cplx* work, int sign) const {
+    vector<cplx> buf(N);
+    cplx* src = work;
+    cplx* dst = buf.data();
+    for (int j = 0; j < N; j++) src[j] = x[(long)j * stride];
+    int Ns = 1;
+    for (int R : radices) {
+      const int nb = N / R;
+      for (int j = 0; j < nb; j++) {
+        const int k = j % Ns;
+        cplx v[7];
+        for (int r = 0; r < R; r++) {
+          cplx t = tw[(long)k * r * (N / (Ns * R)) % N];
+          if (sign > 0) t = std::conj(t);
+          v[r] = src[j + r * nb] * t;
+        }
+        cplx o[7];
+        for (int m = 0; m < R; m++) {
+          cplx s = 0;
+          for (int r = 0; r < R; r++) {
+            cplx t = tw[(long)((m * r) % R) * (N / R)];
+            if (sign > 0) t = std::conj(t);
+            s += v[r] * t;
+          }
+          o[m] = s;
+        }
+        const int base = (j / Ns) * Ns * R + k;
+        for (int m = 0; m < R; m++) dst[base + m * Ns] = o[m];
+      }
+      std::swap(src, dst);
+      Ns *= R;
+    }
+    for (int j = 0; j < N; j++) x[(long)j * stride] = src[j];
+  }
+};
+
+void FFT2D(vector<cplx>& a, int nr, int nc, const FFT1D& fr, const FFT1D& fc, int sign, int nthreads) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int i = 0; i < nr; i++) {
+    vector<cplx> work(nc);
+    fc.run(&a[(long)i * nc], 1, work.data(), sign);
+  }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int j = 0; j < nc; j++) {
+    vector<cplx> work(nr);
+    fr.run(&a[j], nc, work.data(), sign);
+  }
+}
+
+// ------------------------------------------------------------------ Convolver
+// core/convolver.cpp: SetupPSF (175-184), DoFullSetup (204-331), ConvolveImage
+// (340-414), ShiftAndWrapPSF (421-441).  mode 0 = FFT (padded to a smooth length
+// >= nImage + nPSF - 1), mode 1 = exact direct summation of the same linear
+// convolution (the definition both FFT codes must reproduce).
+struct Convolver {
+  int ncPsf = 0, nrPsf = 0, ncImg = 0, nrImg = 0, ncPad = 0, nrPad = 0, mode = 0, nthreads = 1;
+  vector<double> psf;   // normalised copy
+  vector<cplx> psfFFT;
+  FFT1D fr, fc;
+  int Setup(const double* psfIn, int nc, int nr, bool normalize, int ncImage, int nrImage, int convMode, int nthr) {
+    ncPsf = nc; nrPsf = nr; ncImg = ncImage; nrImg = nrImage; mode = convMode; nthreads = nthr;
+    psf.assign(psfIn, psfIn + (long)nc * nr);
+    if (normalize) {   // Kahan sum, then divide (core/convolver.cpp:296-311)
+      double psfSum = 0.0, storedError = 0.0;
+      for (size_t k = 0; k < psf.size(); k++) {
+        double adjustedVal = psf[k] - storedError;
+        double tempSum = psfSum + adjustedVal;
+        storedError = (tempSum - psfSum) - adjustedVal;
+        psfSum = tempSum;
+      }
+      for (size_t k = 0; k < psf.size(); k++) psf[k] = psf[k] / psfSum;
+    }
+    if (mode == 0) {
+      ncPad = NextSmooth(ncImg + ncPsf - 1);
+      nrPad = NextSmooth(nrImg + nrPsf - 1);
+      fr.plan(nrPad);
+      fc.plan(ncPad);
+      psfFFT.assign((long)ncPad * nrPad, 0.0);
+      const int cx = ncPsf / 2, cy = nrPsf / 2;   // ShiftAndWrapPSF
+      for (int i = 0; i < nrPsf; i++)
+        for (int j = 0; j < ncPsf; j++) {
+          int destCol = (ncPad - cx + j) % ncPad, destRow = (nrPad - cy + i) % nrPad;
+          psfFFT[(long)destRow * ncPad + destCol] = psf[(long)i * ncPsf + j];
+        }
+      FFT2D(psfFFT, nrPad, ncPad, fr, fc, -1, nthreads);
+    }
+    return 0;
+  }
+  void Convolve(double* img) {
+    const int cx = ncPsf / 2, cy = nrPsf / 2;
+    if (mode == 1) {
+      vector<double> out((long)ncImg * nrImg);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+      for (int i = 0; i < nrImg; i++)
+        for (int j = 0; j < ncImg; j++) {
+          double s = 0.0;
+          for (int a = 0; a < nrPsf; a++) {
+            int ii = i - a + cy;
+            if (ii < 0 || ii >= nrImg) continue;
+            for (int b = 0; b < ncPsf; b++) {
+              int jj = j - b + cx;
+              if (jj < 0 || jj >= ncImg) continue;
+              s += psf[(long)a * ncPsf + b] * img[(long)ii * ncImg + jj];
+            }
+          }
+          out[(long)i * ncImg + j] = s;
+        }
+      std::copy(out.begin(), out.end(), img);
+      return;
+    }
+    vector<cplx> a((long)ncPad * nrPad, 0.0);
+    for (int i = 0; i < nrImg; i++)
+      for (int j = 0; j < ncImg; j++) a[(long)i * ncPad + j] = img[(long)i * ncImg + j];
+    FFT2D(a, nrPad, ncPad, fr, fc, -1, nthreads);
+    for (size_t z = 0; z < a.size(); z++) {   // complex multiply (core/convolver.cpp:372-379)
+      double ar = a[z].real(), bi = a[z].imag(), c = psfFFT[z].real(), d = psfFFT[z].imag();
+      a[z] = cplx(ar * c - bi * d, bi * c + ar * d);
+    }
+    FFT2D(a, nrPad, ncPad, fr, fc, +1, nthreads);
+    const double rescale = 1.0 / ((double)ncPad * (double)nrPad);
+    for (int i = 0; i < nrImg; i++)
+      for (int j = 0; j < ncImg; j++) img[(long)i * ncImg + j] = rescale * a[(long)i * ncPad + j].real();
+  }
+};
+
+// DownsampleAndReplace (core/downsample.cpp:54-109), including the reference's use
+// of nOversampCols for the sub-region row count (downsample.cpp:79).
+void DownsampleAndReplace(const double* osImg, int nOsCols, int nOsRows, int nOsPsfCols, int nOsPsfRows,
+                          double* mainImg, int nMainCols, int nMainRows, int nMainPsfCols, int nMainPsfRows,
+                          int startX, int startY, int scale) {
+  (void)nOsRows; (void)nMainRows;
+  const int j1 = startX - 1 + nMainPsfCols, i1 = startY - 1 + nMainPsfRows;
+  const int nCols_sub = (int)((nOsCols - 2 * nOsPsfCols) / scale);
+  const int nRows_sub = (int)((nOsCols - 2 * nOsPsfRows) / scale);
+  const double area = scale * scale;
+  for (int i = i1; i < i1 + nRows_sub; i++)
+    for (int j = j1; j < j1 + nCols_sub; j++) {
+      const int jj1 = (j - j1) * scale + nOsPsfCols, ii1 = (i - i1) * scale + nOsPsfRows;
+      double binned = 0.0;
+      for (int ii = ii1; ii < ii1 + scale; ii++)
+        for (int jj = jj1; jj < jj1 + scale; jj++) binned += osImg[(long)ii * nOsCols + jj];
+      mainImg[(long)i * nMainCols + j] = binned / area;
+    }
+}
+
+// mp_enorm (core/mp_enorm.cpp:54-122): MINPACK scaled Euclidean norm.
+double mp_enorm(long n, const double* x) {
+  const double RDWARF = 3.834e-20, RGIANT = 1.304e19;
+  double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0, temp;
+  const double agiant = RGIANT / (double)n;
+  for (long i = 0; i < n; i++) {
+    double xabs = fabs(x[i]);
+    if (xabs > RDWARF && xabs < agiant) { s2 += xabs * xabs; continue; }
+    if (xabs > RDWARF) {
+      if (xabs > x1max) { temp = x1max / xabs; s1 = 1.0 + s1 * temp * temp; x1max = xabs; }
+      else { temp = xabs / x1max; s1 += temp * temp; }
+      continue;
+    }
+    if (xabs > x3max) { temp = x3max / xabs; s3 = 1.0 + s3 * temp * temp; x3max = xabs; }
+    else if (xabs != 0.0) { temp = xabs / x3max; s3 += temp * temp; }
+  }
+  if (s1 != 0.0) return x1max * sqrt(s1 + (s2 / x1max) / x1max);
+  if (s2 != 0.0) {
+    if (s2 >= x3max) temp = s2 * (1.0 + (x3max / s2) * (x3max * s3));
+    else temp = x3max * ((s2 / x3max) + (x3max * s3));
+    return sqrt(temp);
+  }
+  return x3max * sqrt(s3);
+}
+
+// ------------------------------------------------------------------ ModelObject
+struct OsampRegion {   // core/oversampled_region.cpp:207-430
+  int x1, y1, nBaseCols, nBaseRows, scale, ncPsf, nrPsf, nCols, nRows;
+  double subpixFrac, startOffset;
+  Convolver conv;
+  vector<double> model;
+};
+
+struct Model {
+  vector<Func*> funcs;
+  vector<int> fsetStart, paramSizes;
+  int nParamsTot = 0, nthreads = 1, convMode = 0;
+  bool doSubsampling = true;
+  // image geometry (core/model_object.cpp:488-527)
+  int nDataCols = 0, nDataRows = 0, nModelCols = 0, nModelRows = 0, nPsfCols = 0, nPsfRows = 0;
+  long nDataVals = 0, nModelVals = 0, nValid = 0;
+  bool doConvolution = false, setupDone = false, dataSet = false;
+  vector<double> psfIn;
+  bool psfNormalize = true;
+  Convolver conv;
+  vector<OsampRegion*> regions;
+  // data and weights (core/model_object.cpp:462-819)
+  vector<double> data, weight, mask, extraCash, model, deviates;
+  double gain = 1, readNoise = 0, expTime = 1, originalSky = 0, effGain = 1, rnAdu2 = 0;
+  int nCombined = 1;
+  bool maskExists = false, externalErrors = false, weightSet = false;
+  bool useCash = false, poissonMLR = false, modelErrors = false, dataErrors = true;
+  ~Model() {
+    for (Func* f : funcs) delete f;
+    for (OsampRegion* r : regions) delete r;
+  }
+
+  int SetupModelImage(int nc, int nr) {   // core/model_object.cpp:488-527
+    nDataCols = nc; nDataRows = nr;
+    nDataVals = (long)nc * nr;
+    if (doConvolution) {
+      nModelCols = nc + 2 * nPsfCols;
+      nModelRows = nr + 2 * nPsfRows;
+      conv.Setup(psfIn.data(), nPsfCols, nPsfRows, psfNormalize, nModelCols, nModelRows, convMode, nthreads);
+    } else {
+      nModelCols = nc;
+      nModelRows = nr;
+    }
+    nModelVals = (long)nModelCols * nModelRows;
+    model.assign(nModelVals, 0.0);
+    setupDone = true;
+    return 0;
+  }
+
+  // CreateModelImage (core/model_object.cpp:1103-1229)
+  void CreateModelImage(const double* params) {
+    int offset = 0;
+    double x0 = 0, y0 = 0;
+    for (size_t n = 0; n < funcs.size(); n++) {
+      if (fsetStart[n]) { x0 = params[offset]; y0 = params[offset + 1]; offset += 2; }
+      funcs[n]->Setup(params, offset, x0, y0);
+      offset += paramSizes[n];
+    }
+    const int nF = (int)funcs.size();
+#pragma omp parallel for schedule(static, 10) num_threads(nthreads)
+    for (long k = 0; k < nModelVals; k++) {
+      long j = k % nModelCols, i = k / nModelCols;
+      double y = (double)(i - nPsfRows + 1), x = (double)(j - nPsfCols + 1);
+      double newValSum = 0.0, storedError = 0.0;
+      for (int n = 0; n < nF; n++) {   // Kahan summation over components
+        double adjVal = funcs[n]->GetValue(x, y) - storedError;
+        double tempSum = newValSum + adjVal;
+        storedError = (tempSum - newValSum) - adjVal;
+        newValSum = tempSum;
+      }
+      model[i * nModelCols + j] = newValSum;
+    }
+    if (doConvolution) conv.Convolve(model.data());
+    for (OsampRegion* R : regions) ComputeRegionAndDownsample(*R);
+  }
+
+  // OversampledRegion::ComputeRegionAndDownsample (core/oversampled_region.cpp:284-430)
+  void ComputeRegionAndDownsample(OsampRegion& R) {
+    const long nVals = (long)R.nCols * R.nRows;
+    const int nF = (int)funcs.size();
+#pragma omp parallel for schedule(static, 10) num_threads(nthreads)
+    for (long k = 0; k < nVals; k++) {
+      int j = (int)(k % R.nCols), i = (int)(k / R.nCols);
+      double y = R.y1 + R.startOffset + (i - R.nrPsf) * R.subpixFrac;
+      double x = R.x1 + R.startOffset + (j - R.ncPsf) * R.subpixFrac;
+      double newValSum = 0.0, storedError = 0.0;
+      for (int n = 0; n < nF; n++) {
+        double adjVal = funcs[n]->GetValue(x, y) - storedError;
+        double tempSum = newValSum + adjVal;
+        storedError = (tempSum - newValSum) - adjVal;
+        newValSum = tempSum;
+      }
+      R.model[(long)i * R.nCols + j] = newValSum;
+    }
+    R.conv.Convolve(R.model.data());
+    DownsampleAndReplace(R.model.data(), R.nCols, R.nRows, R.ncPsf, R.nrPsf, model.data(), nModelCols, nModelRows,
+                         nPsfCols, nPsfRows, R.x1, R.y1, R.scale);
+  }
+
+  long ModelIndex(long z) const {   // data pixel -> model pixel (core/model_object.cpp:1457-1459)
+    if (!doConvolution) return z;
+    long r = z / nDataCols, c = z - r * nDataCols;
+    return (long)nModelCols * (nPsfRows + r) + nPsfCols + c;
+  }
+
+  void UpdateWeightVector() {   // core/model_object.cpp:1345-1381
+    for (long z = 0; z < nDataVals; z++) {
+      if (!maskExists || mask[z] > 0) {
+        double totalFlux = model[ModelIndex(z)] + originalSky;
+        double noise_squared = totalFlux / effGain + nCombined * rnAdu2;
+        weight[z] = 1.0 / sqrt(noise_squared);
+      }
+    }
+  }
+
+  double PoissonMLRDeviate(long i, long im) const {   // core/model_object.cpp:1386-1401
+    double modVal = effGain * (model[im] + originalSky);
+    double dataVal = effGain * (data[i] + originalSky);
+    double logModel = (modVal <= 0) ? LOG_SMALL_VALUE : log(modVal);
+    return sqrt(2.0 * weight[i] * fabs(modVal - dataVal * logModel + extraCash[i]));
+  }
+
+  void ComputeDeviates(double* y, const double* params) {   // core/model_object.cpp:1413-1496
+    CreateModelImage(params);
+    if (modelErrors) UpdateWeightVector();
+    for (long z = 0; z < nDataVals; z++) {
+      long zm = ModelIndex(z);
+      y[z] = poissonMLR ? PoissonMLRDeviate(z, zm) : weight[z] * (data[z] - model[zm]);
+    }
+  }
+
+  double ChiSquared(const double* params) {   // core/model_object.cpp:1633-1689
+    deviates.resize(nDataVals);
+    CreateModelImage(params);
+    if (modelErrors) UpdateWeightVector();
+    for (long z = 0; z < nDataVals; z++) deviates[z] = weight[z] * (data[z] - model[ModelIndex(z)]);
+    double chi = mp_enorm(nDataVals, deviates.data());
+    return chi * chi;
+  }
+
+  double CashStatistic(const double* params) {   // core/model_object.cpp:1703-1775
+    CreateModelImage(params);
+    double cashStat = 0.0;
+    for (long z = 0; z < nDataVals; z++) {
+      double modVal = effGain * (model[ModelIndex(z)] + originalSky);
+      double dataVal = effGain * (data[z] + originalSky);
+      double logModel = (modVal <= 0) ? LOG_SMALL_VALUE : log(modVal);
+      cashStat += weight[z] * (modVal - dataVal * logModel + extraCash[z]);
+    }
+    return 2.0 * cashStat;
+  }
+
+  double GetFitStatistic(const double* params) {   // core/model_object.cpp:1620-1626
+    return useCash ? CashStatistic(params) : ChiSquared(params);
+  }
+
+  int FinalSetupForFitting() {   // core/model_object.cpp:967-1079
+    int status = 0;
+    if (!maskExists) {
+      mask.assign(nDataVals, 1.0);
+      maskExists = true;
+    }
+    for (long z = 0; z < nDataVals; z++)
+      if (mask[z] > 0.0 && !std::isfinite(data[z])) { mask[z] = 0.0; nValid--; }
+    if (!useCash && dataErrors && !externalErrors) {   // GenerateErrorVector (642-673)
+      weight.assign(nDataVals, 0.0);
+      for (long z = 0; z < nDataVals; z++) {
+        double totalFlux = data[z] + originalSky;
+        if (totalFlux < 0.0) totalFlux = 0.0;
+        double noise_squared = totalFlux / effGain + nCombined * rnAdu2;
+        weight[z] = 1.0 / sqrt(noise_squared);
+      }
+      weightSet = true;
+    }
+    if (useCash && poissonMLR) {   // GenerateExtraCashTerms (695-709)
+      for (long z = 0; z < nDataVals; z++) {
+        double dataVal = effGain * (data[z] + originalSky);
+        extraCash[z] = (dataVal <= 0.0) ? 0.0 : dataVal * log(dataVal) - dataVal;
+      }
+    }
+    if (externalErrors) {
+      for (long z = 0; z < nDataVals; z++)
+        if (mask[z] > 0.0 && !std::isfinite(weight[z])) { mask[z] = 0.0; weight[z] = 0.0; nValid--; }
+    }
+    if (weightSet) {   // ApplyMask (core/model_object.cpp:796-816)
+      for (long z = 0; z < nDataVals; z++) {
+        double v = mask[z] * weight[z];
+        if (!std::isfinite(v) && mask[z] == 0.0) v = 0.0;
+        weight[z] = v;
+      }
+    }
+    for (long z = 0; z < nDataVals; z++)   // CheckWeightVector (2527-2563)
+      if (mask[z] > 0.0 && (!std::isfinite(weight[z]) || weight[z] < 0.0)) status = -1;
+    for (long z = 0; z < nDataVals; z++) {   // VetDataVector (2500-2519)
+      if (!std::isfinite(data[z])) {
+        if (mask[z] > 0.0) status = (status == 0) ? -2 : status;
+        else data[z] = 0.0;
+      }
+    }
+    if (nValid < 1) status = -3;
+    return status;
+  }
+};
+
+}  // namespace
+
+// ======================================================================= C API
+extern "C" {
+
+typedef Model orc_model;
+
+orc_model* orc_create(void) { return new Model(); }
+void orc_destroy(orc_model* m) { delete m; }
+int orc_set_threads(orc_model* m, int n) { m->nthreads = n < 1 ? 1 : n; return 0; }
+int orc_set_conv_mode(orc_model* m, int mode) { m->convMode = mode; return 0; }
+
+int orc_add_function(orc_model* m, const char* name, int newSet) {
+  Func* f = MakeFunc(name);
+  if (!f) return -1;
+  if (m->funcs.empty()) newSet = 1;
+  f->doSubsampling = (f->doSubsampling && m->doSubsampling);
+  m->funcs.push_back(f);
+  m->fsetStart.push_back(newSet ? 1 : 0);
+  m->paramSizes.push_back(f->NParams());
+  m->nParamsTot += f->NParams() + (newSet ? 2 : 0);
+  return 0;
+}
+int orc_set_subsampling(orc_model* m, int on) {
+  m->doSubsampling = on != 0;
+  for (Func* f : m->funcs)
+    if (!dynamic_cast<ExpDisk3D*>(f)) f->doSubsampling = (on != 0);
+  return 0;
+}
+int orc_get_nparams(orc_model* m) { return m->nParamsTot; }
+
+// AddPSFVector (core/model_object.cpp:827-876)
+int orc_add_psf(orc_model* m, const double* psf, int nc, int nr, int normalize) {
+  double sum = 0.0;
+  for (long i = 0; i < (long)nc * nr; i++) {
+    if (!std::isfinite(psf[i])) return -1;
+    sum += psf[i];
+  }
+  if (normalize && sum <= 0.0) return -1;
+  m->psfIn.assign(psf, psf + (long)nc * nr);
+  m->psfNormalize = normalize != 0;
+  m->nPsfCols = nc; m->nPsfRows = nr;
+  m->doConvolution = true;
+  return m->setupDone ? -1 : 0;
+}
+
+int orc_setup_model_image(orc_model* m, int nc, int nr) { return m->SetupModelImage(nc, nr); }
+
+int orc_add_image_data(orc_model* m, const double* data, int nc, int nr) {   // 462-477
+  m->data.assign(data, data + (long)nc * nr);
+  m->nValid = (long)nc * nr;
+  m->dataSet = true;
+  m->weight.assign((long)nc * nr, 1.0);
+  m->extraCash.assign((long)nc * nr, 0.0);
+  return m->SetupModelImage(nc, nr);
+}
+
+int orc_add_image_characteristics(orc_model* m, double gain, double rn, double expTime, int nComb, double sky) {
+  m->gain = gain; m->readNoise = rn; m->expTime = expTime; m->nCombined = nComb; m->originalSky = sky;
+  m->effGain = gain * expTime * nComb;   // core/model_object.cpp:532-547
+  m->rnAdu2 = rn * rn / (m->effGain * m->effGain);
+  return 0;
+}
+
+int orc_add_error_vector(orc_model* m, const double* err, long n, int inputType) {   // 573-619
+  if (n != m->nDataVals) return -1;
+  m->weight.assign(err, err + n);
+  for (long z = 0; z < n; z++) {
+    switch (inputType) {
+      case WEIGHTS_ARE_SIGMAS: m->weight[z] = 1.0 / m->weight[z]; break;
+      case WEIGHTS_ARE_VARIANCES: m->weight[z] = 1.0 / sqrt(m->weight[z]); break;
+      case WEIGHTS_ARE_WEIGHTS: m->weight[z] = sqrt(m->weight[z]); break;
+      default: return -1;
+    }
+  }
+  m->externalErrors = true;
+  m->weightSet = true;
+  return 0;
+}
+
+int orc_add_mask_vector(orc_model* m, const double* mask, long n, int inputType) {   // 721-776
+  if (n != m->nDataVals) return -1;
+  m->mask.assign(mask, mask + n);
+  m->nValid = 0;
+  for (long z = 0; z < n; z++) {
+    double v = m->mask[z];
+    bool bad = (inputType == MASK_ZERO_IS_GOOD) ? (!std::isfinite(v) || v > 0.0)
+                                                : (!std::isfinite(v) || v < 1.0);
+    if (inputType != MASK_ZERO_IS_GOOD && inputType != MASK_ZERO_IS_BAD) return -1;
+    m->mask[z] = bad ? 0.0 : 1.0;
+    if (!bad) m->nValid++;
+  }
+  m->maskExists = true;
+  return 0;
+}
+
+// AddOversampledPsfInfo (core/model_object.cpp:881-955) + OversampledRegion::SetupModelImage
+int orc_add_oversampled_psf(orc_model* m, const double* psf, int nc, int nr, int scale, int x1, int x2, int y1,
+                            int y2, int normalize) {
+  for (long i = 0; i < (long)nc * nr; i++)
+    if (!std::isfinite(psf[i])) return -1;
+  if (x1 < 1) x1 = 1;
+  if (y1 < 1) y1 = 1;
+  if (x2 > m->nDataCols) x2 = m->nDataCols;
+  if (y2 > m->nDataRows) y2 = m->nDataRows;
+  OsampRegion* R = new OsampRegion();
+  R->x1 = x1; R->y1 = y1;
+  R->nBaseCols = x2 - x1 + 1; R->nBaseRows = y2 - y1 + 1;
+  R->scale = scale; R->ncPsf = nc; R->nrPsf = nr;
+  R->subpixFrac = 1.0 / scale;
+  R->startOffset = 0.5 * R->subpixFrac - 0.5;
+  R->nCols = R->nBaseCols * scale + 2 * nc;
+  R->nRows = R->nBaseRows * scale + 2 * nr;
+  R->model.assign((long)R->nCols * R->nRows, 0.0);
+  R->conv.Setup(psf, nc, nr, normalize != 0, R->nCols, R->nRows, m->convMode, m->nthreads);
+  m->regions.push_back(R);
+  return 0;
+}
+
+int orc_use_cash_statistic(orc_model* m) {   // 1549-1590
+  m->useCash = true;
+  m->weight.assign(m->nDataVals, 1.0);
+  m->extraCash.assign(m->nDataVals, 0.0);
+  m->weightSet = true;
+  return 0;
+}
+int orc_use_poisson_mlr(orc_model* m) { m->poissonMLR = true; return orc_use_cash_statistic(m); }
+int orc_use_model_errors(orc_model* m) {   // 1500-1535
+  m->modelErrors = true;
+  m->dataErrors = false;
+  m->weight.assign(m->nDataVals, 1.0);
+  m->weightSet = true;
+  return 0;
+}
+int orc_final_setup(orc_model* m) { return m->FinalSetupForFitting(); }
+long orc_get_nvalid(orc_model* m) { return m->nValid; }
+
+int orc_create_model_image(orc_model* m, const double* params) { m->CreateModelImage(params); return 0; }
+int orc_get_model_image(orc_model* m, double* out) {   // GetModelImageVector (1310-1335)
+  for (long z = 0; z < m->nDataVals; z++) out[z] = m->model[m->ModelIndex(z)];
+  return 0;
+}
+int orc_get_expanded_model_image(orc_model* m, double* out) {
+  std::copy(m->model.begin(), m->model.end(), out);
+  return 0;
+}
+int orc_get_weights(orc_model* m, double* out) { std::copy(m->weight.begin(), m->weight.end(), out); return 0; }
+int orc_compute_deviates(orc_model* m, const double* params, double* y) { m->ComputeDeviates(y, params); return 0; }
+int orc_fit_statistic(orc_model* m, const double* params, double* out) { *out = m->GetFitStatistic(params); return 0; }
+
+// ---- single-function and primitive entry points for unit-level pins
+double orc_func_value(const char* name, const double* params, double x0, double y0, double x, double y,
+                      int subsampling) {
+  Func* f = MakeFunc(name);
+  if (!f) return NAN;
+  if (!dynamic_cast<ExpDisk3D*>(f)) f->doSubsampling = subsampling != 0;
+  f->Setup(params, 0, x0, y0);
+  double v = f->GetValue(x, y);
+  delete f;
+  return v;
+}
+double orc_calculate_bn(double n) { return Calculate_bn(n); }
+double orc_enorm(long n, const double* x) { return mp_enorm(n, x); }
+void orc_downsample_and_replace(const double* osImg, int nOsCols, int nOsRows, int nOsPsfCols, int nOsPsfRows,
+                                double* mainImg, int nMainCols, int nMainRows, int nMainPsfCols, int nMainPsfRows,
+                                int startX, int startY, int scale) {
+  DownsampleAndReplace(osImg, nOsCols, nOsRows, nOsPsfCols, nOsPsfRows, mainImg, nMainCols, nMainRows,
+                       nMainPsfCols, nMainPsfRows, startX, startY, scale);
+}
+// psfconvolve equivalent (extra/psfconvolve_main.cpp:102-125): convolve `img` in place.
+int orc_convolve(double* img, int nc, int nr, const double* psf, int pc, int pr, int normalize, int mode) {
+  Convolver c;
+  c.Setup(psf, pc, pr, normalize != 0, nc, nr, mode, 1);
+  c.Convolve(img);
+  return 0;
+}
+// QAGS on the ExponentialDisk3D line-of-sight integrand at one pixel.
+double orc_expdisk3d_qags(const double* params6, double x0, double y0, double x, double y, double epsrel,
+                          double* abserr, int* neval) {
+  ExpDisk3D f;
+  f.Setup(params6, 0, x0, y0);
+  ExpDisk3D::Density F = f.Make(x, y);
+  double L = 20 * f.h;
+  return qags(F, -L, L, 0.0, epsrel, 1000, abserr, neval);
+}
+// Generic QAGS on exp(-c|s|) * (1 + s^2)^-1 style test integrands (for pinning qags alone).
+double orc_qags_test(int which, double a, double b, double epsrel, double* abserr, int* neval) {
+  auto f0 = [](double s) { return exp(-fabs(s)); };                 // kink at 0
+  auto f1 = [](double s) { return 1.0 / (1.0 + s * s); };           // smooth
+  auto f2 = [](double s) { return s > 0 ? 1.0 / sqrt(s) : 0.0; };   // end-point singularity
+  auto f3 = [](double s) { return log(s) * sqrt(s); };
+  switch (which) {
+    case 0: return qags(f0, a, b, 0.0, epsrel, 1000, abserr, neval);
+    case 1: return qags(f1, a, b, 0.0, epsrel, 1000, abserr, neval);
+    case 2: return qags(f2, a, b, 0.0, epsrel, 1000, abserr, neval);
+    default: return qags(f3, a, b, 0.0, epsrel, 1000, abserr, neval);
+  }
+}
+
+}  // extern "C"

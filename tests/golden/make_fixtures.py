@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Extract the reference's own regression-test inputs and outputs into .npz fixtures.
+
+Runs only in the build container, where the reference tree is mounted read-only at
+/root/reference.  It reads FITS *data files* (images, masks, PSFs and archived
+outputs) with a minimal FITS reader written here (astropy is not available), and
+copies the archived numeric results (chi^2 / Cash values, iteration counts) from
+the reference's printouts.  Nothing from the reference is imported or executed.
+
+Each fixture records which reference test produced it:
+
+  expdisk32      do_imfit_tests:63   -> tests/imfit_reference/linux_x86_64/imfit_textout1
+  ic3478_64      do_imfit_tests:68   -> tests/imfit_reference/linux_x86_64/imfit_textout3
+  n3073          do_imfit_tests:96-99 -> imfit_textout4, imfit_textout4c, imfit_textout4d
+  sersic_exp512  do_makeimage_tests:51,155 -> tests/sersic+exp_orig.fits
+  biggertest64   do_makeimage_tests:48 -> tests/biggertest_orig.fits
+  oversampled    do_makeimage_tests:68,70,199,215 -> tests/oversampled_orig.fits,
+                 tests/oversampled2both_orig.fits
+  conv_n5419     do_psfconvolve_tests:11,60 -> tests/convolve_out_n5419.fits
+  conv_5x5       do_psfconvolve_tests:8-9 -> tests/psfconvolve_out{1,2}.txt (padded PSF)
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def read_fits(path):
+    """Primary-HDU image of a FITS file as a 2-D float64 array, shape (NAXIS2, NAXIS1).
+
+    FITS stores the first axis (columns, x) fastest, so the C-order reshape gives
+    row-major [row][col] exactly as imfit's pixel vectors (core/image_io.cpp:284).
+    """
+    with open(path, "rb") as f:
+        raw = f.read()
+    hdr = {}
+    pos = 0
+    done = False
+    while not done:
+        block = raw[pos:pos + 2880]
+        pos += 2880
+        for k in range(36):
+            card = block[80 * k:80 * (k + 1)].decode("ascii")
+            key = card[:8].strip()
+            if key == "END":
+                done = True
+                break
+            if card[8:10] == "= ":
+                val = card[10:].split("/")[0].strip()
+                hdr[key] = val
+    bitpix = int(hdr["BITPIX"])
+    naxis = int(hdr["NAXIS"])
+    dims = [int(hdr["NAXIS%d" % (i + 1)]) for i in range(naxis)]
+    dtype = {-32: ">f4", -64: ">f8", 8: ">u1", 16: ">i2", 32: ">i4"}[bitpix]
+    n = int(np.prod(dims))
+    data = np.frombuffer(raw, dtype=dtype, count=n, offset=pos)
+    native = data.astype(data.dtype.newbyteorder("="))
+    arr = native.reshape(dims[::-1])
+    bscale = float(hdr.get("BSCALE", "1"))
+    bzero = float(hdr.get("BZERO", "0"))
+    if bscale != 1.0 or bzero != 0.0:
+        arr = arr.astype(np.float64) * bscale + bzero
+    return arr
+
+
+def grab(path, pattern, cast=float):
+    with open(path) as f:
+        txt = f.read()
+    m = re.search(pattern, txt)
+    if m is None:
+        raise RuntimeError("pattern %r not found in %s" % (pattern, path))
+    return cast(m.group(1))
+
+
+def t(p):
+    return os.path.join(REF, "tests", p)
+
+
+def main():
+    lin = "imfit_reference/linux_x86_64/"
+    # --- textout1: 32x32 Exponential, user noise image, no subsampling
+    np.savez_compressed(
+        os.path.join(OUT, "expdisk32.npz"),
+        data=read_fits(t("testimage_expdisk32.fits")),
+        noise=read_fits(t("uniform_image32.fits")),
+        initial_chi2=grab(t(lin + "imfit_textout1"), r"INITIAL CHI\^2 = ([0-9.]+)"),
+        final_chi2=grab(t(lin + "imfit_textout1"), r"CHI-SQUARE = ([0-9.]+)"),
+        niter=grab(t(lin + "imfit_textout1"), r"NITER = (\d+)", int),
+        nfev=grab(t(lin + "imfit_textout1"), r"NFEV = (\d+)", int),
+    )
+    # --- textout3: 64x64 Sersic, data-based errors
+    np.savez_compressed(
+        os.path.join(OUT, "ic3478_64.npz"),
+        data=read_fits(t("ic3478rss_64x64.fits")),
+        initial_chi2=grab(t(lin + "imfit_textout3"), r"INITIAL CHI\^2 = ([0-9.]+)"),
+        final_chi2=grab(t(lin + "imfit_textout3"), r"CHI-SQUARE = ([0-9.]+)"),
+        niter=grab(t(lin + "imfit_textout3"), r"NITER = (\d+)", int),
+        nfev=grab(t(lin + "imfit_textout3"), r"NFEV = (\d+)", int),
+        best=np.array([32.9439, 34.0933, 18.2612, 0.235989, 2.40028, 20.0094, 60.7613]),
+    )
+    # --- textout4/4c/4d: 150x200 Sersic+Exp, 35x35 PSF, mask
+    np.savez_compressed(
+        os.path.join(OUT, "n3073.npz"),
+        data=read_fits(t("n3073rss_small.fits")),
+        mask=read_fits(t("n3073rss_small_mask.fits")),
+        psf=read_fits(t("psf_moffat_35_n4699z.fits")),
+        chi2=grab(t("imfit_reference/imfit_textout4c"), r"CHI-SQUARE = ([0-9.]+)"),
+        cash=grab(t("imfit_reference/imfit_textout4d"), r"CASH STATISTIC = (-?[0-9.]+)"),
+        lm_final_chi2=grab(t(lin + "imfit_textout4"), r"CHI-SQUARE = ([0-9.]+)"),
+        lm_niter=grab(t(lin + "imfit_textout4"), r"NITER = (\d+)", int),
+        lm_nfev=grab(t(lin + "imfit_textout4"), r"NFEV = (\d+)", int),
+        best=np.array([100.7073, 97.0877, 111.919, 0.184262, 3.2805, 78.2972, 26.7976,
+                       51.724, 0.35226, 39.7753, 53.6058]),
+    )
+    # --- makeimage goldens (float32 images written by SaveVectorAsImage)
+    np.savez_compressed(os.path.join(OUT, "sersic_exp512.npz"),
+                        image=read_fits(t("sersic+exp_orig.fits")))
+    np.savez_compressed(os.path.join(OUT, "biggertest64.npz"),
+                        image=read_fits(t("biggertest_orig.fits")))
+    np.savez_compressed(
+        os.path.join(OUT, "oversampled.npz"),
+        image=read_fits(t("oversampled_orig.fits")),
+        image2=read_fits(t("oversampled2both_orig.fits")),
+        psf=read_fits(t("psf_standard.fits")),
+        psf_osamp=read_fits(t("psf_oversamp.fits")),
+    )
+    # --- psfconvolve goldens
+    np.savez_compressed(
+        os.path.join(OUT, "conv_n5419.npz"),
+        image=read_fits(t("n5419f555w_pc_derot_cutout.fits")),
+        psf=read_fits(t("n5419_sin100_psf_pc2scale.fits")),
+        out=read_fits(t("convolve_out_n5419.fits")),
+    )
+
+    def printed_block(path, title):
+        # psfconvolve --printimages prints each image as rows of %f values after a
+        # title line; return the numeric block that follows `title`.
+        rows, grab_rows = [], False
+        with open(path) as f:
+            for line in f:
+                if title in line:
+                    grab_rows, rows = True, []
+                    continue
+                if grab_rows:
+                    parts = line.split()
+                    if not parts:
+                        if rows:
+                            break
+                        continue
+                    rows.append([float(v) for v in parts])
+        return np.array(rows)
+
+    # tests/psfconvolve_out2.txt: 5x5 Gaussian PSF, normalised then shifted and
+    # wrapped into the 9x9 padded array (core/convolver.cpp:296-311,421-441).
+    np.savez_compressed(
+        os.path.join(OUT, "conv_5x5.npz"),
+        gauss=read_fits(t("gaussian_5x5.fits")),
+        delta=read_fits(t("deltafn_5x5.fits")),
+        padded_psf_gauss=printed_block(t("psfconvolve_out2.txt"),
+                                       "The whole padded, normalized PSF image"),
+        padded_psf_delta=printed_block(t("psfconvolve_out1.txt"),
+                                       "The whole padded, normalized PSF image"),
+    )
+    for fn in sorted(os.listdir(OUT)):
+        if fn.endswith(".npz"):
+            print("%-22s %8d bytes" % (fn, os.path.getsize(os.path.join(OUT, fn))))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,210 @@
+"""Pins of the CPU oracle against the reference's own golden vectors (CPU only).
+
+Known answers, archived chi^2/Cash values and archived images all come from the
+reference's tests (cited per test); the reference's standalone helpers are also
+compiled from source into oracle/_ref and compared directly where they build.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import cases
+
+DELTA = 1.0e-9   # unit_tests/unittest_funcs.t.h:37
+
+
+# ---------------------------------------------------------------- single-pixel known answers
+def test_exponential_known_answers():   # unit_tests/unittest_funcs.t.h:261-281
+    p = [90.0, 0.0, 1.0, 10.0]
+    v1 = math.exp(-1.0 / 10.0)
+    assert abs(oracle.func_value("Exponential", p, 10, 10, 10, 10) - 1.0) < DELTA
+    for x, y in [(11, 10), (10, 11), (9, 10), (10, 9)]:
+        assert abs(oracle.func_value("Exponential", p, 10, 10, x, y) - v1) < DELTA
+
+
+def test_sersic_known_answers():   # unit_tests/unittest_funcs.t.h:397-419
+    p = [90.0, 0.0, 2.0, 1.0, 10.0]
+    assert abs(oracle.func_value("Sersic", p, 10, 10, 10, 10) - 39.333062332325284) < DELTA
+    assert abs(oracle.func_value("Sersic", p, 10, 10, 11, 10) - 12.315472433581958) < DELTA
+    for x, y in [(20, 10), (10, 20), (0, 10), (10, 0)]:
+        assert abs(oracle.func_value("Sersic", p, 10, 10, x, y) - 1.0) < DELTA
+
+
+def test_gaussian_known_answers():   # unit_tests/unittest_funcs.t.h:542-568
+    p = [90.0, 0.0, 1.0, 10.0]
+    assert abs(oracle.func_value("Gaussian", p, 10, 10, 10, 10) - 1.0) < DELTA
+    for x, y in [(11, 10), (9, 10), (10, 9)]:
+        assert abs(oracle.func_value("Gaussian", p, 10, 10, x, y) - math.exp(-1.0 / 200.0)) < DELTA
+    for x, y in [(20, 10), (10, 20), (0, 10), (10, 0)]:
+        assert abs(oracle.func_value("Gaussian", p, 10, 10, x, y) - math.exp(-0.5)) < DELTA
+
+
+def test_moffat_known_answers():   # unit_tests/unittest_funcs.t.h:975-1033
+    for p, r1, rf in [([90.0, 0.0, 1.0, 10.0, 3.0], 0.96944697430705418, 0.11784501202946467),
+                      ([90.0, 0.0, 1.0, 10.0, 1.0], 0.96153846153846145, 0.20000000000000001)]:
+        assert abs(oracle.func_value("Moffat", p, 10, 10, 10, 10) - 1.0) < DELTA
+        for x, y in [(11, 10), (9, 10), (10, 9)]:
+            assert abs(oracle.func_value("Moffat", p, 10, 10, x, y) - r1) < DELTA
+        for x, y in [(20, 10), (10, 20), (0, 10), (10, 0)]:
+            assert abs(oracle.func_value("Moffat", p, 10, 10, x, y) - rf) < DELTA
+    p = [90.0, 0.5, 1.0, 10.0, 1.0]
+    for x, y in [(11, 10), (9, 10), (10, 9.5), (10, 10.5)]:
+        assert abs(oracle.func_value("Moffat", p, 10, 10, x, y) - 0.96153846153846145) < DELTA
+
+
+def test_flatsky_known_answers():   # unit_tests/unittest_funcs.t.h:874-905
+    for v in (1.0, -1.0):
+        for x, y in [(10, 10), (11, 10), (9, 10), (10, 9)]:
+            assert oracle.func_value("FlatSky", [v], 10, 10, x, y) == v
+
+
+def test_subsampled_value_is_grid_mean():
+    """With subsampling on, GetValue is the mean of the closed-form intensity over
+    an nSub x nSub grid of sub-pixel centres (func_sersic.cpp:163-183); brute force
+    it in numpy for a pixel near the centre (r <= 4 -> nSub = 20)."""
+    PA, ell, n, Ie, re = 30.0, 0.3, 1.7, 5.0, 8.0
+    x0, y0, x, y = 20.3, 19.6, 21.0, 18.0
+    bn = oracle.calculate_bn(n)
+    k = (np.arange(20) + 0.5) / 20 - 0.5
+    xs, ys = np.meshgrid(x + k, y + k, indexing="ij")
+    t = math.radians(PA + 90.0)
+    xp = (xs - x0) * math.cos(t) + (ys - y0) * math.sin(t)
+    yp = (-(xs - x0) * math.sin(t) + (ys - y0) * math.cos(t)) / (1 - ell)
+    r = np.sqrt(xp ** 2 + yp ** 2)
+    ref = np.mean(Ie * np.exp(-bn * ((r / re) ** (1 / n) - 1)))
+    got = oracle.func_value("Sersic", [PA, ell, n, Ie, re], x0, y0, x, y, subsampling=True)
+    assert abs(got - ref) <= 1e-13 * ref
+    # and the subsampled value differs measurably from the point value at this radius
+    point = oracle.func_value("Sersic", [PA, ell, n, Ie, re], x0, y0, x, y, subsampling=False)
+    assert abs(point - ref) > 1e-6 * ref
+
+
+# ---------------------------------------------------------------- reference helpers built from source
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="oracle/_ref not built (reference tree absent)")
+def test_calculate_bn_matches_reference_source():
+    R = oracle.ref_lib()
+    for n in np.concatenate([np.linspace(0.05, 0.36, 17), np.linspace(0.3601, 12, 211)]):
+        assert oracle.calculate_bn(n) == R._Z12Calculate_bnd(n)
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="oracle/_ref not built (reference tree absent)")
+def test_enorm_matches_reference_source():
+    R = oracle.ref_lib()
+    rng = np.random.default_rng(7)
+    for scale in (1.0, 1e-21, 1e18, 1e-300):
+        for n in (1, 7, 1000):
+            x = rng.standard_normal(n) * scale
+            x[::5] = 0.0
+            got = oracle.enorm(x)
+            ref = R._Z8mp_enormiPKd(n, x.ctypes.data_as(oracle._dp))
+            assert got == ref
+    x = np.array([1e20, 3.0, 1e-25, 0.0, -2e19])   # all three buckets at once
+    assert oracle.enorm(x) == R._Z8mp_enormiPKd(x.size, x.ctypes.data_as(oracle._dp))
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="oracle/_ref not built (reference tree absent)")
+def test_downsample_matches_reference_source():
+    R = oracle.ref_lib()
+    rng = np.random.default_rng(3)
+    s, npc, nmc = 3, 7, 5
+    os_img = rng.random((10 * s + 2 * npc, 10 * s + 2 * npc))
+    main_a = rng.random((40 + 2 * nmc, 40 + 2 * nmc))
+    main_b = main_a.copy()
+    args = (os_img.shape[1], os_img.shape[0], npc, npc)
+    oracle.lib().orc_downsample_and_replace(os_img.ctypes.data_as(oracle._dp), *args,
+                                            main_a.ctypes.data_as(oracle._dp), main_a.shape[1], main_a.shape[0],
+                                            nmc, nmc, 12, 9, s)
+    R._Z20DownsampleAndReplacePKdiiiiPdiiiiiiii(os_img.ctypes.data_as(oracle._dp), *args,
+                                                main_b.ctypes.data_as(oracle._dp), main_b.shape[1],
+                                                main_b.shape[0], nmc, nmc, 12, 9, s, 0)
+    assert np.array_equal(main_a, main_b)
+    # closed form: each replaced pixel is the mean of its s x s block
+    blk = os_img[npc:npc + 10 * s, npc:npc + 10 * s].reshape(10, s, 10, s).mean(axis=(1, 3))
+    assert np.allclose(main_a[9 - 1 + nmc:9 - 1 + nmc + 10, 12 - 1 + nmc:12 - 1 + nmc + 10], blk, rtol=1e-15)
+
+
+# ---------------------------------------------------------------- archived fit statistics
+@pytest.mark.parametrize("mk", [cases.textout1, cases.textout3,
+                                lambda: cases.n3073("chi2"), lambda: cases.n3073("cash")])
+def test_fit_statistic_goldens(mk):
+    case, ref = mk()
+    m = case.build(oracle.OracleModel(threads=4))
+    got = m.GetFitStatistic(case.params)
+    # printouts carry 6 decimals: |d| <= 5e-7 absolute covers the rounding of the print
+    assert abs(got - ref) <= 5e-7 + 1e-13 * abs(ref), (got, ref)
+
+
+def test_direct_and_fft_convolution_agree_on_n3073():
+    case, ref = cases.n3073()
+    a = case.build(oracle.OracleModel(threads=4, conv_mode=0)).GetFitStatistic(case.params)
+    b = case.build(oracle.OracleModel(threads=4, conv_mode=1)).GetFitStatistic(case.params)
+    assert abs(a - b) <= 1e-10 * abs(b)
+
+
+# ---------------------------------------------------------------- archived images (float32 FITS)
+@pytest.mark.parametrize("mk", [cases.sersic_exp512, cases.biggertest64])
+def test_model_image_goldens(mk):
+    case, ref = mk()
+    m = case.build(oracle.OracleModel(threads=8))
+    m.CreateModelImage(case.params)
+    assert cases.float32_ulps(m.GetModelImageVector(), ref) == 0
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_oversampled_goldens(two):
+    # do_makeimage_tests:190-199 compares only pixels > 1e-7 (FFT noise floor)
+    case, ref = cases.oversampled(two)
+    m = case.build(oracle.OracleModel(threads=8))
+    m.CreateModelImage(case.params)
+    img = m.GetModelImageVector()
+    sel = ref > 1e-7
+    assert cases.float32_ulps(img[sel], ref[sel]) == 0
+    assert np.abs(img - ref).max() < 1e-6
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_psfconvolve_golden(mode):   # do_psfconvolve_tests:11,60
+    d = cases.golden("conv_n5419")
+    out = oracle.convolve(d["image"], d["psf"], True, mode)
+    assert cases.float32_ulps(out, d["out"]) == 0
+
+
+def test_padded_psf_layout_golden():
+    """ShiftAndWrapPSF puts PSF pixel (cy,cx) at padded index (0,0) and wraps the
+    rest (core/convolver.cpp:421-441); tests/psfconvolve_out2.txt prints it for the
+    5x5 Gaussian.  Reproduce the printed 9x9 array from the definition."""
+    d = cases.golden("conv_5x5")
+    g = d["gauss"] / d["gauss"].sum()
+    pad = np.zeros((9, 9))
+    for i in range(5):
+        for j in range(5):
+            pad[(9 - 2 + i) % 9, (9 - 2 + j) % 9] = g[i, j]
+    assert np.allclose(pad, d["padded_psf_gauss"], atol=5e-7)
+
+
+@pytest.mark.parametrize("shape,psf_shape", [((17, 23), (5, 5)), ((16, 9), (4, 6)), ((31, 30), (7, 3))])
+def test_convolution_definition(shape, psf_shape):
+    """FFT and direct oracle convolutions equal the zero-padded linear convolution
+    with the PSF centre at (nr/2, nc/2), computed independently with numpy.fft."""
+    rng = np.random.default_rng(11)
+    img = rng.random(shape)
+    psf = rng.random(psf_shape)
+    psfn = psf / psf.sum()
+    nr, nc = shape[0] + psf_shape[0] - 1, shape[1] + psf_shape[1] - 1
+    full = np.fft.irfft2(np.fft.rfft2(img, (nr, nc)) * np.fft.rfft2(psfn, (nr, nc)), (nr, nc))
+    cy, cx = psf_shape[0] // 2, psf_shape[1] // 2
+    ref = full[cy:cy + shape[0], cx:cx + shape[1]]
+    for mode in (0, 1):
+        out = oracle.convolve(img, psf, True, mode)
+        assert np.allclose(out, ref, rtol=0, atol=1e-14 * np.abs(ref).max())
+
+
+def test_planted_mistake_is_caught():
+    """A transposed PSF (a plausible indexing slip) must fail the n5419 golden."""
+    d = cases.golden("conv_n5419")
+    out = oracle.convolve(d["image"], np.ascontiguousarray(d["psf"].T), True, 0)
+    if np.array_equal(d["psf"], d["psf"].T):
+        pytest.skip("PSF is symmetric")
+    assert cases.float32_ulps(out, d["out"]) > 0
