@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the model-image hot path on one MI355X.
+
+Workload: the 1024^2 single-image configuration -- Sersic (n=4, r_e=60, ell=0.2) +
+Exponential (h=120, ell=0.5) + FlatSky, 51^2 Moffat PSF, data pixels 493-532
+oversampled 5x with a 255^2 Moffat PSF, Poisson noise (gain 4, seed 2).  One step
+is one model + chi^2 evaluation (GetFitStatistic): evaluate the model with
+sub-pixel integration, convolve it (and the oversampled region) with the PSF,
+form the weighted deviates against the data and reduce chi^2 in FP64.  Data,
+weights and PSF spectra are resident in HBM before timing starts.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the FP64 CPU
+oracle of the same evaluation, timed on this host's cores on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "model+χ² evals/s and LM fits/s, adjoint+batched vs plain GPU at equal χ²"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def device_sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    if hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def fft_flops(n):
+    return 5.0 * n * math.log2(n)
+
+
+def kernel_model(w, prec_bytes):
+    """Algorithmic work of the main-grid kernels of one evaluation.
+
+    Work counts follow the data-layout of the path: the main grid is evaluated on
+    the data region plus a PSF halo (W+P-1 columns, H+P-1 rows); two real half-images
+    share one complex plane of Nr x Nc, padded to 2^a 3^b 5^c.
+    """
+    W, H, P = w.width, w.height, w.psf.shape[0]
+    Lc, Hh = W + P - 1, (H + 1) // 2
+    Lr = Hh + P - 1
+
+    def smooth(n):
+        while True:
+            k = n
+            for p in (2, 3, 5):
+                while k % p == 0:
+                    k //= p
+            if k == 1:
+                return n
+            n += 1
+    Nc, Nr = smooth(Lc), smooth(Lr)
+    cb = 2 * prec_bytes   # complex element
+    return {
+        # spectrum rows written by the fused evaluate + row-FFT kernel
+        "k_eval_rowfft": {"bytes": Lr * Nc * cb, "flops": Lr * fft_flops(Nc)},
+        # column FFT in, multiply by the PSF spectrum, column FFT back
+        "k_colconv": {"bytes": (Lr * Nc + Nr * Nc + Hh * Nc) * cb,
+                      "flops": Nc * 2 * fft_flops(Nr) + 6 * Nr * Nc},
+        # spectrum rows + data + weights in
+        "k_rowifft_stat": {"bytes": Hh * Nc * cb + W * H * 2 * prec_bytes, "flops": Hh * fft_flops(Nc)},
+    }
+
+
+def cpu_baseline(w, seconds):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    o = w.setup_fit(oracle.OracleModel(threads=threads))
+    p = w.start_params()
+    t0 = time.perf_counter()
+    o.GetFitStatistic(p)   # warm-up (first call touches every buffer)
+    first = time.perf_counter() - t0
+    n, t0 = 0, time.perf_counter()
+    while n < 20 and (time.perf_counter() - t0) < seconds:
+        o.GetFitStatistic(p)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": "%d FP64 GetFitStatistic calls of the same 1024^2 workload after 1 warm-up (%.1f s); CPU "
+                      "oracle restatement of the reference path with its own mixed-radix FFT (no FFTW on this "
+                      "host), OpenMP schedule(static,10) pixel loops" % (n, first)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU sharding of the path is design-only "
+              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import imfit_amd
+    from imfit_amd import workloads
+
+    log("building the 1024^2 workload (PSFs and data on the GPU)")
+    w = workloads.cfg2()
+    prec = imfit_amd.PRECISION_FP32 if args.precision == "fp32" else imfit_amd.PRECISION_FP64
+    m = w.fit_model(prec)
+    p = w.start_params()
+    for _ in range(args.warmup):
+        m.GetFitStatistic(p)
+    device_sync()
+    m.SetKernelTiming(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stat = m.GetFitStatistic(p)
+    device_sync()
+    dt = time.perf_counter() - t0
+    kt = m.GetKernelTiming()
+    m.SetKernelTiming(False)
+    ms_per_step = 1e3 * dt / args.steps
+    value = args.steps / dt
+    log("chi2 = %.10e, %.4f ms/eval, %.1f evals/s" % (stat, ms_per_step, value))
+    for k, (tot, n) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+        log("  %-22s %9.2f us avg  x%d" % (k, 1e3 * tot / n, n))
+
+    # FP64 parity-mode rate, reported alongside
+    m64 = w.fit_model(imfit_amd.PRECISION_FP64)
+    for _ in range(3):
+        m64.GetFitStatistic(p)
+    device_sync()
+    t1 = time.perf_counter()
+    n64 = max(10, args.steps // 4)
+    for _ in range(n64):
+        m64.GetFitStatistic(p)
+    device_sync()
+    fp64_rate = n64 / (time.perf_counter() - t1)
+
+    # roofline of the dominant kernel
+    prec_bytes = 4 if args.precision == "fp32" else 8
+    km = kernel_model(w, prec_bytes)
+    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
+    roof = None
+    if dom in km:
+        ach = km[dom]["bytes"] / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                "traffic": None, "kernel": dom, "avg_us": avg_s * 1e6}
+    else:
+        roof = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": dom, "avg_us": avg_s * 1e6}
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU oracle baseline")
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        log("cpu: %.3f evals/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    out = {"metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+           "config": {"workload": w.name, "image": "1024x1024", "psf": "51x51 Moffat",
+                      "oversampled_region": "40x40 at 5x, 255x255 PSF",
+                      "precision": "%s pixels and FFT, FP64 deviate reduction" % args.precision,
+                      "fp64_evals_per_s": fp64_rate,
+                      "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

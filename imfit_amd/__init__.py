@@ -1,0 +1,217 @@
+"""imfit_amd -- MI355X model-image library for imfit's ModelObject hot path.
+
+`ModelObjectGPU` is a thin ctypes binding of the C ABI in include/imfit_gpu.h
+(libimfit_gpu.so, built in-tree by imfit_amd/build.py).  Its method names are
+the reference ModelObject's (core/model_object.h:38-347), so callers written
+against ModelObject -- and the parity tests -- read the same.  There is no CPU
+fallback: if the HIP library is missing this module raises on import.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libimfit_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "imfit_gpu.h")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("imfit_amd: %s is missing; build it with `python imfit_amd/build.py`" % LIB_PATH)
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+WEIGHTS_ARE_SIGMAS, WEIGHTS_ARE_VARIANCES, WEIGHTS_ARE_WEIGHTS = 100, 110, 120
+MASK_ZERO_IS_GOOD, MASK_ZERO_IS_BAD = 10, 20
+PRECISION_FP64, PRECISION_FP32 = 0, 1
+
+_SIGS = {
+    "ifg_create": ([C.c_int], _vp),
+    "ifg_destroy": ([_vp], None),
+    "ifg_last_error": ([_vp], C.c_char_p),
+    "ifg_set_precision": ([_vp, C.c_int], C.c_int),
+    "ifg_add_function": ([_vp, C.c_char_p, C.c_int], C.c_int),
+    "ifg_set_subsampling": ([_vp, C.c_int], C.c_int),
+    "ifg_get_nparams": ([_vp], C.c_int),
+    "ifg_add_psf": ([_vp, _dp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "ifg_setup_model_image": ([_vp, C.c_int, C.c_int], C.c_int),
+    "ifg_add_image_data": ([_vp, _dp, C.c_int, C.c_int], C.c_int),
+    "ifg_add_image_characteristics": ([_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double], C.c_int),
+    "ifg_add_error_vector": ([_vp, _dp, C.c_long, C.c_int], C.c_int),
+    "ifg_add_mask_vector": ([_vp, _dp, C.c_long, C.c_int], C.c_int),
+    "ifg_add_oversampled_psf": ([_vp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+                                C.c_int),
+    "ifg_use_cash_statistic": ([_vp], C.c_int),
+    "ifg_use_poisson_mlr": ([_vp], C.c_int),
+    "ifg_use_model_errors": ([_vp], C.c_int),
+    "ifg_final_setup": ([_vp], C.c_int),
+    "ifg_which_fit_statistic": ([_vp], C.c_int),
+    "ifg_get_ndata": ([_vp], C.c_long),
+    "ifg_get_nvalid": ([_vp], C.c_long),
+    "ifg_create_model_image": ([_vp, _dp], C.c_int),
+    "ifg_get_model_image": ([_vp, _dp], C.c_int),
+    "ifg_compute_deviates": ([_vp, _dp, _dp], C.c_int),
+    "ifg_fit_statistic": ([_vp, _dp, _dp], C.c_int),
+    "ifg_fit_statistic_batch": ([_vp, _dp, C.c_int, _dp], C.c_int),
+    "ifg_set_kernel_timing": ([_vp, C.c_int], C.c_int),
+    "ifg_get_kernel_timing": ([_vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_long)], C.c_int),
+}
+
+_LIB = C.CDLL(LIB_PATH)
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(_LIB, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+
+def lib():
+    return _LIB
+
+
+def header_symbols():
+    """Names of the entry points declared in include/imfit_gpu.h."""
+    with open(HEADER) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(ifg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class IfgError(RuntimeError):
+    pass
+
+
+class ModelObjectGPU:
+    """ModelObject on one MI355X: setup and evaluation through the C ABI."""
+
+    def __init__(self, device=0, precision=PRECISION_FP64):
+        self.h = _LIB.ifg_create(device)
+        if not self.h:
+            raise IfgError("ifg_create failed")
+        self._chk(_LIB.ifg_set_precision(self.h, precision), "SetPrecision")
+        self.ncols = self.nrows = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            _LIB.ifg_destroy(h)
+            self.h = None
+
+    def _chk(self, st, what):
+        if st < 0:
+            raise IfgError("%s failed (%d): %s" % (what, st, _LIB.ifg_last_error(self.h).decode()))
+        return st
+
+    # ---- setup (same call order as core/setup_model_object.cpp:58-153)
+    def AddFunction(self, name, new_set=False):
+        return self._chk(_LIB.ifg_add_function(self.h, name.encode(), int(new_set)), "AddFunction")
+
+    def SetSubsampling(self, on):
+        return self._chk(_LIB.ifg_set_subsampling(self.h, int(on)), "SetSubsampling")
+
+    def GetNParams(self):
+        return _LIB.ifg_get_nparams(self.h)
+
+    def AddPSFVector(self, psf, normalize=True):
+        psf = np.ascontiguousarray(psf, dtype=np.float64)
+        return self._chk(_LIB.ifg_add_psf(self.h, _ptr(psf), psf.shape[1], psf.shape[0], int(normalize)),
+                         "AddPSFVector")
+
+    def SetupModelImage(self, ncols, nrows):
+        self.ncols, self.nrows = ncols, nrows
+        return self._chk(_LIB.ifg_setup_model_image(self.h, ncols, nrows), "SetupModelImage")
+
+    def AddImageDataVector(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        self.nrows, self.ncols = data.shape
+        return self._chk(_LIB.ifg_add_image_data(self.h, _ptr(data), self.ncols, self.nrows), "AddImageDataVector")
+
+    def AddImageCharacteristics(self, gain, read_noise, exp_time=1.0, n_combined=1, original_sky=0.0):
+        return self._chk(_LIB.ifg_add_image_characteristics(self.h, gain, read_noise, exp_time, n_combined,
+                                                            original_sky), "AddImageCharacteristics")
+
+    def AddErrorVector(self, err, input_type=WEIGHTS_ARE_SIGMAS):
+        err = np.ascontiguousarray(err, dtype=np.float64)
+        return self._chk(_LIB.ifg_add_error_vector(self.h, _ptr(err), err.size, input_type), "AddErrorVector")
+
+    def AddMaskVector(self, mask, input_type=MASK_ZERO_IS_GOOD):
+        mask = np.ascontiguousarray(mask, dtype=np.float64)
+        return self._chk(_LIB.ifg_add_mask_vector(self.h, _ptr(mask), mask.size, input_type), "AddMaskVector")
+
+    def AddOversampledPsf(self, psf, scale, x1, x2, y1, y2, normalize=True):
+        psf = np.ascontiguousarray(psf, dtype=np.float64)
+        return self._chk(_LIB.ifg_add_oversampled_psf(self.h, _ptr(psf), psf.shape[1], psf.shape[0], scale, x1, x2,
+                                                      y1, y2, int(normalize)), "AddOversampledPsf")
+
+    def UseCashStatistic(self):
+        return self._chk(_LIB.ifg_use_cash_statistic(self.h), "UseCashStatistic")
+
+    def UsePoissonMLR(self):
+        return self._chk(_LIB.ifg_use_poisson_mlr(self.h), "UsePoissonMLR")
+
+    def UseModelErrors(self):
+        return self._chk(_LIB.ifg_use_model_errors(self.h), "UseModelErrors")
+
+    def FinalSetupForFitting(self):
+        return self._chk(_LIB.ifg_final_setup(self.h), "FinalSetupForFitting")
+
+    def WhichFitStatistic(self):
+        return _LIB.ifg_which_fit_statistic(self.h)
+
+    def GetNDataValues(self):
+        return _LIB.ifg_get_ndata(self.h)
+
+    def GetNValidPixels(self):
+        return _LIB.ifg_get_nvalid(self.h)
+
+    # ---- hot path
+    def CreateModelImage(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(_LIB.ifg_create_model_image(self.h, _ptr(p)), "CreateModelImage")
+
+    def GetModelImageVector(self):
+        out = np.empty((self.nrows, self.ncols))
+        self._chk(_LIB.ifg_get_model_image(self.h, _ptr(out)), "GetModelImageVector")
+        return out
+
+    def ComputeDeviates(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        out = np.empty(self.nrows * self.ncols)
+        self._chk(_LIB.ifg_compute_deviates(self.h, _ptr(p), _ptr(out)), "ComputeDeviates")
+        return out
+
+    def GetFitStatistic(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        out = np.zeros(1)
+        self._chk(_LIB.ifg_fit_statistic(self.h, _ptr(p), _ptr(out)), "GetFitStatistic")
+        return float(out[0])
+
+    def GetFitStatisticBatch(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        if p.ndim != 2:
+            raise ValueError("params must be (n_vectors, nParams)")
+        out = np.zeros(p.shape[0])
+        self._chk(_LIB.ifg_fit_statistic_batch(self.h, _ptr(p), p.shape[0], _ptr(out)), "GetFitStatisticBatch")
+        return out
+
+    # ---- instrumentation
+    def SetKernelTiming(self, on):
+        return self._chk(_LIB.ifg_set_kernel_timing(self.h, int(on)), "SetKernelTiming")
+
+    def GetKernelTiming(self):
+        """{kernel name: (total ms, launches)} recorded with HIP events on the handle's stream."""
+        out = {}
+        i = 0
+        while True:
+            name = C.create_string_buffer(64)
+            ms = np.zeros(1)
+            n = C.c_long(0)
+            cnt = _LIB.ifg_get_kernel_timing(self.h, i, name, 64, _ptr(ms), C.byref(n))
+            if cnt <= i:
+                break
+            out[name.value.decode()] = (float(ms[0]), int(n.value))
+            i += 1
+        return out

@@ -1,0 +1,1117 @@
+// ifg_api.cpp -- host side of the C ABI declared in include/imfit_gpu.h.
+//
+// An ifg_model mirrors one reference ModelObject: it records the setup calls
+// (functions, PSF, data, errors, mask, oversampled regions, statistic), applies
+// FinalSetupForFitting's weight/mask rules on the host once
+// (core/model_object.cpp:967-1079), uploads data and weights to HBM, and then
+// runs every evaluation on the device.  Per evaluation the host only computes the
+// FunctionObject::Setup constants (a few cos/sin/pow per component, with the C
+// library so the values equal the reference's) and the sub-pixel boxes, and
+// uploads them with one small copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/imfit_gpu.h"
+#include "ifg_internal.h"
+#include "ifg_launch.h"
+
+using namespace ifg;
+
+namespace {
+
+const double DEG2RAD = 0.017453292519943295;   // function_objects/func_sersic.cpp:65
+const double LOG2E = 1.4426950408889634073599246810019;
+
+// Calculate_bn (function_objects/helper_funcs.cpp:14-29)
+double Calculate_bn(double n) {
+  double n2 = n * n;
+  if (n > 0.36)
+    return 2 * n - 0.333333333333333 + 0.009876543209876543 / n + 0.0018028610621203215 / n2 +
+           0.00011409410586365319 / (n2 * n) - 7.1510122958919723e-05 / (n2 * n2);
+  return 0.01945 + -0.8902 * n + 10.95 * n2 + -19.67 * n2 * n + 13.43 * n2 * n2;
+}
+
+int NParamsOf(int type) {
+  switch (type) {
+    case F_SERSIC: return 5;
+    case F_EXP: return 4;
+    case F_FLATSKY: return 1;
+    case F_GAUSS: return 4;
+    case F_MOFFAT: return 5;
+    case F_EXPDISK3D: return 6;
+  }
+  return 0;
+}
+
+int TypeOf(const char* name) {
+  std::string s(name ? name : "");
+  if (s == "Sersic") return F_SERSIC;
+  if (s == "Exponential") return F_EXP;
+  if (s == "FlatSky") return F_FLATSKY;
+  if (s == "Gaussian") return F_GAUSS;
+  if (s == "Moffat") return F_MOFFAT;
+  if (s == "ExponentialDisk3D") return F_EXPDISK3D;
+  return 0;
+}
+
+bool Subsampling(int type) { return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT; }
+
+// FunctionObject::Setup for one component (func_sersic.cpp:109-127, func_exp.cpp:104-118,
+// func_gaussian.cpp:104-121, func_moffat.cpp:111-132, func_flatsky.cpp:85-90).
+void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
+  std::memset(&R, 0, sizeof(R));
+  double* v = R.v;
+  v[0] = x0;
+  v[1] = y0;
+  if (type == F_FLATSKY) {
+    v[5] = p[0];
+    return;
+  }
+  const double PA = p[0], ell = p[1];
+  const double q = 1.0 - ell;
+  const double PA_rad = (PA + 90.0) * DEG2RAD;
+  v[2] = cos(PA_rad);
+  v[3] = sin(PA_rad);
+  v[4] = q;
+  v[15] = 1.0 / q;
+  switch (type) {
+    case F_SERSIC: {
+      const double n = p[2], I_e = p[3], r_e = p[4];
+      const double bn = Calculate_bn(n);
+      v[5] = I_e; v[6] = r_e; v[7] = bn; v[8] = 1.0 / n;
+      v[12] = 1.0 / (r_e * r_e); v[13] = 0.5 / n; v[14] = -bn * LOG2E;
+      break;
+    }
+    case F_EXP: {
+      const double I_0 = p[2], h = p[3];
+      v[5] = I_0; v[6] = h;
+      v[12] = -LOG2E / h;
+      break;
+    }
+    case F_GAUSS: {
+      const double I_0 = p[2], sigma = p[3];
+      v[5] = I_0; v[6] = sigma; v[7] = 2.0 * sigma * sigma;
+      v[12] = -LOG2E / v[7];
+      break;
+    }
+    case F_MOFFAT: {
+      const double I_0 = p[2], fwhm = p[3], beta = p[4];
+      const double exponent = pow(2.0, 1.0 / beta);
+      const double alpha = 0.5 * fwhm / sqrt(exponent - 1.0);
+      v[5] = I_0; v[6] = alpha; v[7] = beta;
+      v[12] = 1.0 / (alpha * alpha); v[13] = -beta;
+      break;
+    }
+  }
+}
+
+int NextSmooth(int n) {   // smallest 2^a 3^b 5^c >= n
+  for (int m = std::max(n, 1);; m++) {
+    int k = m;
+    for (int p : {2, 3, 5})
+      while (k % p == 0) k /= p;
+    if (k == 1) return m;
+  }
+}
+
+FFTPlan MakePlan(int N) {
+  FFTPlan pl;
+  std::memset(&pl, 0, sizeof(pl));
+  pl.N = N;
+  int k = N;
+  while (k % 4 == 0 && pl.nst < IFG_MAXSTAGE) { pl.rad[pl.nst++] = 4; k /= 4; }
+  for (int p : {2, 3, 5})
+    while (k % p == 0 && pl.nst < IFG_MAXSTAGE) { pl.rad[pl.nst++] = p; k /= p; }
+  return pl;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  template <typename T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct FFTLen {
+  FFTPlan plan;
+  DevBuf tw64, tw32;
+};
+
+// One convolution geometry: a grid (main image or oversampled region), a PSF, and
+// whether one image is split into two halves (single evaluation) or two parameter
+// vectors share a complex plane (batches).
+struct ConvGrid {
+  bool ready = false;
+  GridGeom g;
+  FFTLen* fr = nullptr;   // column transforms (length Nr)
+  FFTLen* fc = nullptr;   // row transforms (length Nc)
+  DevBuf khat64, khat32;
+  int rowsK1 = 1, rowsK3 = 1, C64 = 1, C32 = 1;
+};
+
+struct Region {
+  int x1, y1, nbc, nbr, scale, pc, pr, rowsReplaced, ovOff;
+  std::vector<double> psf;   // normalised
+  ConvGrid conv[2];          // [0] single (halves), [1] batch (pairs)
+};
+
+// Kahan-normalised PSF (core/convolver.cpp:296-311)
+std::vector<double> NormalizedPSF(const double* psf, long n, bool normalize) {
+  std::vector<double> out(psf, psf + n);
+  if (!normalize) return out;
+  double psfSum = 0.0, storedError = 0.0;
+  for (long k = 0; k < n; k++) {
+    double adjustedVal = out[k] - storedError;
+    double tempSum = psfSum + adjustedVal;
+    storedError = (tempSum - psfSum) - adjustedVal;
+    psfSum = tempSum;
+  }
+  for (long k = 0; k < n; k++) out[k] = out[k] / psfSum;
+  return out;
+}
+
+}  // namespace
+
+struct ifg_model {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int precision = IFG_PRECISION_FP64;
+  // model definition
+  std::vector<int> types, fsetStart;
+  int nParamsTot = 0;
+  bool doSubsampling = true;
+  // PSF
+  bool doConv = false;
+  std::vector<double> psf;   // normalised
+  int Pc = 0, Pr = 0;
+  // image
+  bool setupDone = false, dataSet = false, finalDone = false;
+  int W = 0, H = 0;
+  long nData = 0, nValid = 0;
+  std::vector<double> data, weight, mask, extra;
+  double gain = 1, readNoise = 0, expTime = 1, originalSky = 0, effGain = 1, rnAdu2 = 0;
+  int nCombined = 1;
+  bool maskExists = false, externalErrors = false, weightSet = false;
+  bool useCash = false, poissonMLR = false, modelErrors = false, dataErrors = true;
+  std::vector<std::unique_ptr<Region>> regions;
+  // device state
+  bool devReady = false;
+  std::map<int, std::unique_ptr<FFTLen>> ffts;
+  ConvGrid mainConv[2];
+  DevBuf data_d, weight_d, extra_d;
+  DevBuf recs_d, boxes_d, tail_d, pairs_d, slots_d, spec_d, partials_d, out_d, dev_d, model_d, osub_d, ovals_d;
+  std::vector<unsigned char> stage;   // host staging (pinned)
+  void* pinned = nullptr;
+  size_t pinnedBytes = 0;
+  std::vector<double> lastParams;
+  bool modelValid = false;
+  // optional per-kernel timing with HIP events on this handle's stream
+  bool timing = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
+  std::vector<hipEvent_t> eventPool;
+  std::map<std::string, std::pair<double, long>> kernelTimes;   // name -> (total ms, launches)
+
+  ~ifg_model() {
+    for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
+    for (auto& pe : pendingEvents) {
+      (void)hipEventDestroy(pe.second.first);
+      (void)hipEventDestroy(pe.second.second);
+    }
+    if (pinned) (void)hipHostFree(pinned);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+  int hipfail(hipError_t e, const char* where) {
+    err = std::string(where) + ": " + hipGetErrorString(e);
+    return IFG_ERR_HIP;
+  }
+  int nComp() const { return (int)types.size(); }
+  int statMode() const {
+    if (!dataSet) return S_NONE;
+    if (useCash) return poissonMLR ? S_MLR : S_CASH;
+    return S_CHI2;
+  }
+};
+
+#define HIPCHK(expr, where)                          \
+  do {                                               \
+    hipError_t _e = (expr);                          \
+    if (_e != hipSuccess) return m->hipfail(_e, where); \
+  } while (0)
+
+namespace {
+
+hipEvent_t TakeEvent(ifg_model* m) {
+  if (!m->eventPool.empty()) {
+    hipEvent_t e = m->eventPool.back();
+    m->eventPool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Collect the events of the finished evaluation into per-kernel totals.
+void HarvestTimes(ifg_model* m) {
+  for (auto& pe : m->pendingEvents) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+      auto& t = m->kernelTimes[pe.first];
+      t.first += ms;
+      t.second += 1;
+    }
+    m->eventPool.push_back(pe.second.first);
+    m->eventPool.push_back(pe.second.second);
+  }
+  m->pendingEvents.clear();
+}
+
+// Launch `expr` (a hipError_t-returning launcher call), bracketed by events when
+// the handle's kernel timing is on.
+#define TIMED(name, expr, where)                                              \
+  do {                                                                        \
+    hipEvent_t _e0 = nullptr, _e1 = nullptr;                                  \
+    if (m->timing) {                                                          \
+      _e0 = TakeEvent(m);                                                     \
+      _e1 = TakeEvent(m);                                                     \
+      (void)hipEventRecord(_e0, m->stream);                                   \
+    }                                                                         \
+    hipError_t _err = (expr);                                                 \
+    if (m->timing) {                                                          \
+      (void)hipEventRecord(_e1, m->stream);                                   \
+      m->pendingEvents.push_back({name, {_e0, _e1}});                         \
+    }                                                                         \
+    if (_err != hipSuccess) return m->hipfail(_err, where);                   \
+  } while (0)
+
+FFTLen* GetFFT(ifg_model* m, int N, int* status) {
+  auto it = m->ffts.find(N);
+  if (it != m->ffts.end()) return it->second.get();
+  std::unique_ptr<FFTLen> f(new FFTLen());
+  f->plan = MakePlan(N);
+  std::vector<cpx<double>> t64(N);
+  std::vector<cpx<float>> t32(N);
+  for (int j = 0; j < N; j++) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)N;
+    t64[j] = {(double)cosl(ang), (double)sinl(ang)};
+    t32[j] = {(float)t64[j].x, (float)t64[j].y};
+  }
+  hipError_t e = f->tw64.ensure(N * sizeof(cpx<double>));
+  if (e == hipSuccess) e = f->tw32.ensure(N * sizeof(cpx<float>));
+  if (e == hipSuccess) e = hipMemcpy(f->tw64.p, t64.data(), N * sizeof(cpx<double>), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(f->tw32.p, t32.data(), N * sizeof(cpx<float>), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    *status = m->hipfail(e, "FFT twiddles");
+    return nullptr;
+  }
+  FFTLen* raw = f.get();
+  m->ffts[N] = std::move(f);
+  return raw;
+}
+
+int RowsPerBlock(int N, size_t elem) {
+  int rows = std::max(1, 1024 / N);
+  while (rows > 1 && 2 * (size_t)rows * N * elem > 48 * 1024) rows--;
+  return rows;
+}
+
+int ColsPerBlock(int Nr, size_t elem) {
+  const size_t budget = 56 * 1024;
+  int C = (int)(budget / (2 * (size_t)(Nr + 1) * elem));
+  return std::max(1, std::min(16, C));
+}
+
+// Build one convolution geometry and its PSF spectrum (scaled by 1/(Nr*Nc)),
+// computed in FP64 on the device with the same FFT kernels, FP32 copy derived.
+int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double>& psfn, int Pc, int Pr, bool halves,
+              double xa, double ya, double frac) {
+  GridGeom& g = cg.g;
+  std::memset(&g, 0, sizeof(g));
+  const int cx = Pc / 2, cy = Pr / 2;
+  g.W = W;
+  g.H = H;
+  g.hl = Pc - 1 - cx;
+  g.hb = Pr - 1 - cy;
+  g.Hh = halves ? (H + 1) / 2 : H;
+  g.Lc = W + Pc - 1;
+  g.Lr = g.Hh + Pr - 1;
+  g.Nc = NextSmooth(g.Lc);
+  g.Nr = NextSmooth(g.Lr);
+  g.xa = xa;
+  g.ya = ya;
+  g.frac = frac;
+  if (g.Nc > 4096 || g.Nr > 4096)
+    return m->fail(IFG_ERR_UNSUPPORTED, "image too large for the in-LDS FFT (padded side > 4096)");
+  int st = 0;
+  cg.fr = GetFFT(m, g.Nr, &st);
+  if (!cg.fr) return st;
+  cg.fc = GetFFT(m, g.Nc, &st);
+  if (!cg.fc) return st;
+  const long NN = (long)g.Nr * g.Nc;
+  std::vector<cpx<double>> k(NN, cpx<double>{0, 0});
+  const double scale = 1.0 / ((double)g.Nr * (double)g.Nc);
+  for (int i = 0; i < Pr; i++)
+    for (int j = 0; j < Pc; j++) {   // ShiftAndWrapPSF (core/convolver.cpp:421-441) on the Nr x Nc plane
+      const int kr = ((i - cy) % g.Nr + g.Nr) % g.Nr, kc = ((j - cx) % g.Nc + g.Nc) % g.Nc;
+      k[(long)kr * g.Nc + kc].x = psfn[(long)i * Pc + j] * scale;
+    }
+  HIPCHK(cg.khat64.ensure(NN * sizeof(cpx<double>)), "alloc PSF spectrum");
+  HIPCHK(cg.khat32.ensure(NN * sizeof(cpx<float>)), "alloc PSF spectrum");
+  HIPCHK(hipMemcpy(cg.khat64.p, k.data(), NN * sizeof(cpx<double>), hipMemcpyHostToDevice), "upload PSF");
+  const int rows = RowsPerBlock(g.Nc, sizeof(cpx<double>));
+  HIPCHK(launch_rowfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fc->plan,
+                               cg.fc->tw64.as<cpx<double>>(), +1, rows),
+         "PSF row FFT");
+  HIPCHK(launch_colconv<double>(m->stream, g, cg.khat64.as<cpx<double>>(), 0, 1, nullptr, cg.fr->plan,
+                                cg.fr->tw64.as<cpx<double>>(), ColsPerBlock(g.Nr, sizeof(cpx<double>)), 1),
+         "PSF column FFT");
+  HIPCHK(hipStreamSynchronize(m->stream), "PSF spectrum");
+  HIPCHK(hipMemcpy(k.data(), cg.khat64.p, NN * sizeof(cpx<double>), hipMemcpyDeviceToHost), "PSF spectrum D2H");
+  std::vector<cpx<float>> k32(NN);
+  for (long z = 0; z < NN; z++) k32[z] = {(float)k[z].x, (float)k[z].y};
+  HIPCHK(hipMemcpy(cg.khat32.p, k32.data(), NN * sizeof(cpx<float>), hipMemcpyHostToDevice), "PSF spectrum FP32");
+  cg.rowsK1 = RowsPerBlock(g.Nc, m->precision == IFG_PRECISION_FP64 ? sizeof(cpx<double>) : sizeof(cpx<float>));
+  cg.rowsK3 = cg.rowsK1;
+  cg.C64 = ColsPerBlock(g.Nr, sizeof(cpx<double>));
+  cg.C32 = ColsPerBlock(g.Nr, sizeof(cpx<float>));
+  cg.ready = true;
+  return IFG_OK;
+}
+
+// Grid geometry of a grid without PSF (no convolution): only W, H and the mapping.
+GridGeom PlainGeom(int W, int H, double xa, double ya, double frac) {
+  GridGeom g;
+  std::memset(&g, 0, sizeof(g));
+  g.W = W;
+  g.H = H;
+  g.Hh = H;
+  g.xa = xa;
+  g.ya = ya;
+  g.frac = frac;
+  return g;
+}
+
+// Sub-pixel boxes of one parameter vector on one grid: every component that may
+// subsample contributes the grid pixels within 10 pixels of its centre (r < 10
+// implies |x - x0| < 10 and |y - y0| < 10 since q <= 1), plus one pixel of margin.
+// Identical boxes are merged; beyond IFG_MAXBOX the boxes collapse to their union.
+int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cLo, int cHi, int rLo, int rHi,
+              Box* out) {
+  struct B { int c0, c1, r0, r1; };
+  std::vector<B> bs;
+  if (m->doSubsampling) {
+    for (int k = 0; k < m->nComp(); k++) {
+      if (!Subsampling(m->types[k])) continue;
+      const double x0 = recs[k].v[0], y0 = recs[k].v[1];
+      if (!std::isfinite(x0) || !std::isfinite(y0)) continue;
+      double fc0 = std::floor((x0 - 10.0 - g.xa) / g.frac) - 1, fc1 = std::ceil((x0 + 10.0 - g.xa) / g.frac) + 1;
+      double fr0 = std::floor((y0 - 10.0 - g.ya) / g.frac) - 1, fr1 = std::ceil((y0 + 10.0 - g.ya) / g.frac) + 1;
+      fc0 = std::max(fc0, (double)cLo); fc1 = std::min(fc1, (double)cHi);
+      fr0 = std::max(fr0, (double)rLo); fr1 = std::min(fr1, (double)rHi);
+      if (fc0 > fc1 || fr0 > fr1) continue;
+      B b = {(int)fc0, (int)fc1, (int)fr0, (int)fr1};
+      bool dup = false;
+      for (auto& o : bs) dup = dup || (o.c0 == b.c0 && o.c1 == b.c1 && o.r0 == b.r0 && o.r1 == b.r1);
+      if (!dup) bs.push_back(b);
+    }
+  }
+  if ((int)bs.size() > IFG_MAXBOX) {
+    B u = bs[0];
+    for (auto& o : bs) {
+      u.c0 = std::min(u.c0, o.c0); u.c1 = std::max(u.c1, o.c1);
+      u.r0 = std::min(u.r0, o.r0); u.r1 = std::max(u.r1, o.r1);
+    }
+    bs.assign(1, u);
+  }
+  int off = 0;
+  for (int b = 0; b < IFG_MAXBOX; b++) {
+    Box& o = out[b];
+    std::memset(&o, 0, sizeof(o));
+    if (b < (int)bs.size()) {
+      o.c0 = bs[b].c0;
+      o.r0 = bs[b].r0;
+      o.w = bs[b].c1 - bs[b].c0 + 1;
+      o.h = bs[b].r1 - bs[b].r0 + 1;
+      o.off = off;
+      off += o.w * o.h;
+    }
+  }
+  return off;
+}
+
+int EnsurePinned(ifg_model* m, size_t bytes) {
+  if (bytes <= m->pinnedBytes) return IFG_OK;
+  if (m->pinned) (void)hipHostFree(m->pinned);
+  m->pinned = nullptr;
+  m->pinnedBytes = 0;
+  HIPCHK(hipHostMalloc(&m->pinned, bytes, hipHostMallocDefault), "pinned staging");
+  m->pinnedBytes = bytes;
+  return IFG_OK;
+}
+
+template <typename T>
+int UploadImageArrays(ifg_model* m) {
+  const long n = m->nData;
+  std::vector<T> tmp(n);
+  HIPCHK(m->data_d.ensure(n * sizeof(T)), "alloc data");
+  HIPCHK(m->weight_d.ensure(n * sizeof(T)), "alloc weights");
+  HIPCHK(m->extra_d.ensure(n * sizeof(T)), "alloc extra terms");
+  for (long z = 0; z < n; z++) tmp[z] = (T)m->data[z];
+  HIPCHK(hipMemcpy(m->data_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload data");
+  for (long z = 0; z < n; z++) tmp[z] = (T)m->weight[z];
+  HIPCHK(hipMemcpy(m->weight_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload weights");
+  for (long z = 0; z < n; z++) tmp[z] = (T)m->extra[z];
+  HIPCHK(hipMemcpy(m->extra_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload extra terms");
+  return IFG_OK;
+}
+
+int PrepareDevice(ifg_model* m) {
+  if (m->devReady) return IFG_OK;
+  if (!m->setupDone) return m->fail(IFG_ERR_STATE, "no image: call ifg_add_image_data or ifg_setup_model_image");
+  if (m->types.empty()) return m->fail(IFG_ERR_STATE, "no functions added");
+  for (int t : m->types)
+    if (t == F_EXPDISK3D) return m->fail(IFG_ERR_UNSUPPORTED, "ExponentialDisk3D is not implemented on the GPU yet");
+  if (m->dataSet) {
+    if (!m->finalDone) return m->fail(IFG_ERR_STATE, "call ifg_final_setup before evaluating against data");
+    int st = (m->precision == IFG_PRECISION_FP64) ? UploadImageArrays<double>(m) : UploadImageArrays<float>(m);
+    if (st < 0) return st;
+  }
+  int ovOff = 0;
+  for (auto& R : m->regions) {
+    R->ovOff = ovOff;
+    ovOff += R->nbc * R->rowsReplaced;
+  }
+  m->devReady = true;
+  return IFG_OK;
+}
+
+int EnsureMainConv(ifg_model* m, int which) {
+  ConvGrid& cg = m->mainConv[which];
+  if (cg.ready) return IFG_OK;
+  return BuildConv(m, cg, m->W, m->H, m->psf, m->Pc, m->Pr, which == 0, 1.0, 1.0, 1.0);
+}
+
+int EnsureRegionConv(ifg_model* m, Region& R, int which) {
+  ConvGrid& cg = R.conv[which];
+  if (cg.ready) return IFG_OK;
+  const double frac = 1.0 / R.scale;
+  const double off = 0.5 * frac - 0.5;   // core/oversampled_region.cpp:223-226
+  return BuildConv(m, cg, R.nbc * R.scale, R.nbr * R.scale, R.psf, R.pc, R.pr, which == 0, (double)R.x1 + off,
+                   (double)R.y1 + off, frac);
+}
+
+// Evaluate nVec parameter vectors.  outStat[nVec] (device results copied to host),
+// devOut[nVec*nData] and modelOut[nVec*nData] are optional host buffers.
+template <typename T>
+int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, double* devOut, double* modelOut) {
+  int st = PrepareDevice(m);
+  if (st < 0) return st;
+  const int nC = m->nComp(), nP = m->nParamsTot, nR = (int)m->regions.size();
+  const bool single = (nVec == 1);
+  const int which = single ? 0 : 1;
+  if (m->doConv) {
+    if ((st = EnsureMainConv(m, which)) < 0) return st;
+  }
+  for (auto& R : m->regions)
+    if ((st = EnsureRegionConv(m, *R, which)) < 0) return st;
+
+  // ---- host: Setup constants and sub-pixel boxes for every vector
+  const size_t recBytes = (size_t)nVec * nC * sizeof(CompRec);
+  const int nGrids = 1 + nR;
+  const size_t boxBytes = (size_t)nGrids * nVec * IFG_MAXBOX * sizeof(Box);
+  if ((st = EnsurePinned(m, recBytes + boxBytes + 4096)) < 0) return st;
+  CompRec* recs = reinterpret_cast<CompRec*>(m->pinned);
+  Box* boxes = reinterpret_cast<Box*>(reinterpret_cast<char*>(m->pinned) + recBytes);
+  std::vector<int> tailStride(nGrids, 0);
+  std::vector<GridGeom> geoms(nGrids);
+  geoms[0] = m->doConv ? m->mainConv[which].g : PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
+  for (int k = 0; k < nR; k++) geoms[1 + k] = m->regions[k]->conv[which].g;
+  for (int v = 0; v < nVec; v++) {
+    const double* p = params + (long)v * nP;
+    int offset = 0;
+    double x0 = 0, y0 = 0;
+    for (int n = 0; n < nC; n++) {   // core/model_object.cpp:1126-1135
+      if (m->fsetStart[n]) {
+        x0 = p[offset];
+        y0 = p[offset + 1];
+        offset += 2;
+      }
+      SetupRec(m->types[n], p + offset, x0, y0, recs[(long)v * nC + n]);
+      offset += NParamsOf(m->types[n]);
+    }
+    for (int gi = 0; gi < nGrids; gi++) {
+      const GridGeom& g = geoms[gi];
+      // evaluated extent: plane columns [-hl, Lc - hl), rows [-hb, H + Pr) (halves may run past H)
+      const bool conv = g.Lc > 0;
+      const int cLo = -g.hl, cHi = conv ? g.Lc - g.hl - 1 : g.W - 1;
+      const int rLo = -g.hb, rHi = conv ? g.H + (g.Lr - g.Hh) : g.H - 1;
+      const int area = MakeBoxes(m, recs + (long)v * nC, g, cLo, cHi, rLo, rHi,
+                                 boxes + ((long)gi * nVec + v) * IFG_MAXBOX);
+      tailStride[gi] = std::max(tailStride[gi], area);
+    }
+  }
+  std::vector<long> tailOff(nGrids + 1, 0);
+  for (int gi = 0; gi < nGrids; gi++) tailOff[gi + 1] = tailOff[gi] + (long)tailStride[gi] * nVec;
+
+  HIPCHK(m->recs_d.ensure(recBytes), "alloc records");
+  HIPCHK(m->boxes_d.ensure(boxBytes), "alloc boxes");
+  HIPCHK(m->tail_d.ensure(std::max<long>(1, tailOff[nGrids]) * sizeof(double)), "alloc tail");
+  HIPCHK(hipMemcpyAsync(m->recs_d.p, recs, recBytes, hipMemcpyHostToDevice, m->stream), "upload records");
+  HIPCHK(hipMemcpyAsync(m->boxes_d.p, boxes, boxBytes, hipMemcpyHostToDevice, m->stream), "upload boxes");
+
+  auto evalArgs = [&](int gi, int v0) {
+    EvalArgs ea;
+    std::memset(&ea, 0, sizeof(ea));
+    ea.nComp = nC;
+    for (int n = 0; n < nC; n++) ea.types[n] = m->types[n];
+    ea.recs = m->recs_d.as<CompRec>() + (long)v0 * nC;
+    ea.boxes = m->boxes_d.as<Box>() + ((long)gi * nVec + v0) * IFG_MAXBOX;
+    ea.tail = m->tail_d.as<double>() + tailOff[gi] + (long)v0 * tailStride[gi];
+    ea.tailStride = tailStride[gi];
+    return ea;
+  };
+
+  // ---- chunking of the batch so that the complex planes stay within budget
+  size_t pairBytes = 0;
+  if (m->doConv) pairBytes = (size_t)geoms[0].Nr * geoms[0].Nc * sizeof(cpx<T>);
+  for (int k = 0; k < nR; k++)
+    pairBytes = std::max(pairBytes, (size_t)geoms[1 + k].Nr * geoms[1 + k].Nc * sizeof(cpx<T>));
+  const int nPairsAll = single ? 1 : (nVec + 1) / 2;
+  int chunkPairs = nPairsAll;
+  if (pairBytes > 0) chunkPairs = (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / pairBytes));
+  const int chunkVecs = single ? 1 : 2 * chunkPairs;
+
+  // buffers
+  const long nData = m->nData;
+  HIPCHK(m->spec_d.ensure(std::max<size_t>(1, pairBytes * chunkPairs)), "alloc planes");
+  HIPCHK(m->out_d.ensure(sizeof(double) * nVec), "alloc results");
+  if (devOut) HIPCHK(m->dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
+  if (modelOut) HIPCHK(m->model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
+  long osubStride = 0;
+  int ovStride = 0;
+  for (auto& R : m->regions) {
+    osubStride = std::max(osubStride, (long)R->nbc * R->scale * R->nbr * R->scale);
+    ovStride += R->nbc * R->rowsReplaced;
+  }
+  if (nR) {
+    HIPCHK(m->osub_d.ensure(sizeof(double) * osubStride * chunkVecs), "alloc region sub-image");
+    HIPCHK(m->ovals_d.ensure(sizeof(double) * ovStride * chunkVecs), "alloc region values");
+  }
+  // slots per plane of the main statistic pass
+  int nSlots = 1, pixPerBlock = 1024;
+  const GridGeom& gm = geoms[0];
+  if (m->doConv) nSlots = (gm.Hh + m->mainConv[which].rowsK3 - 1) / m->mainConv[which].rowsK3;
+  else nSlots = (int)(((long)m->W * m->H + pixPerBlock - 1) / pixPerBlock);
+  const int nPartPairs = m->doConv ? (single ? 1 : chunkPairs) : chunkVecs;
+  HIPCHK(m->partials_d.ensure(sizeof(double) * 5 * 2 * (size_t)nSlots * nPartPairs), "alloc partials");
+  HIPCHK(m->pairs_d.ensure(sizeof(int4) * std::max(chunkPairs, 1) * nGrids), "alloc pairs");
+  HIPCHK(m->slots_d.ensure(sizeof(int2) * chunkVecs), "alloc slots");
+
+  const int mode = m->statMode();
+  const double agiant = 1.304e19 / (double)std::max<long>(nData, 1);
+  const int subsampling = m->doSubsampling ? 1 : 0;
+  hipStream_t s = m->stream;
+
+  // pairs and vector->slot maps; they outlive the asynchronous copies (synchronised
+  // before the next chunk overwrites them and at the end of the call)
+  std::vector<int4> pairs;
+  std::vector<int2> slots;
+  for (int v0 = 0; v0 < nVec; v0 += chunkVecs) {
+    const int nv = std::min(chunkVecs, nVec - v0);
+    const int np = single ? 1 : (nv + 1) / 2;
+    // one pair list per grid: a single image is split into its top and bottom
+    // halves, whose row offset depends on the grid's own height
+    pairs.assign((size_t)np * nGrids, make_int4(0, 0, -1, 0));
+    slots.assign(nv, make_int2(0, 0));
+    for (int gi = 0; gi < nGrids; gi++) {
+      int4* pg = pairs.data() + (size_t)gi * np;
+      if (single) pg[0] = make_int4(0, 0, 0, (geoms[gi].H + 1) / 2);
+      else
+        for (int p = 0; p < np; p++) pg[p] = make_int4(2 * p, 0, (2 * p + 1 < nv) ? 2 * p + 1 : -1, 0);
+    }
+    if (single) slots[0] = make_int2(0, 3);
+    else
+      for (int v = 0; v < nv; v++) slots[v] = make_int2(v / 2, 1 << (v & 1));
+    if (!m->doConv)
+      for (int v = 0; v < nv; v++) slots[v] = make_int2(v, 1);
+    HIPCHK(hipMemcpyAsync(m->pairs_d.p, pairs.data(), sizeof(int4) * np * nGrids, hipMemcpyHostToDevice, s),
+           "pairs");
+    HIPCHK(hipMemcpyAsync(m->slots_d.p, slots.data(), sizeof(int2) * nv, hipMemcpyHostToDevice, s), "slots");
+
+    // 1. sub-pixel tails of every grid
+    for (int gi = 0; gi < nGrids; gi++)
+      if (tailStride[gi] > 0) TIMED(gi == 0 ? "k_tail" : "k_tail_region", launch_tail<T>(s, evalArgs(gi, v0), geoms[gi], nv, subsampling), "k_tail");
+
+    // 2. oversampled regions: evaluate, convolve with their PSF, block-average
+    StatArgs<T> sa;
+    std::memset(&sa, 0, sizeof(sa));
+    for (int k = 0; k < nR; k++) {
+      Region& R = *m->regions[k];
+      ConvGrid& cg = R.conv[which];
+      const GridGeom& g = cg.g;
+      const long pstride = (long)g.Nr * g.Nc;
+      const cpx<T>* tw_c = (sizeof(T) == 8) ? (const cpx<T>*)cg.fc->tw64.p : (const cpx<T>*)cg.fc->tw32.p;
+      const cpx<T>* tw_r = (sizeof(T) == 8) ? (const cpx<T>*)cg.fr->tw64.p : (const cpx<T>*)cg.fr->tw32.p;
+      const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
+      const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
+      const int4* pairsR = m->pairs_d.as<int4>() + (size_t)(1 + k) * np;
+      TIMED("k_eval_rowfft_region",
+            launch_eval_rowfft<T>(s, evalArgs(1 + k, v0), g, pairsR, np, m->spec_d.as<cpx<T>>(), pstride,
+                                  cg.fc->plan, tw_c, cg.rowsK1),
+            "region k_eval_rowfft");
+      TIMED("k_colconv_region", launch_colconv<T>(s, g, m->spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
+            "region k_colconv");
+      StatArgs<T> so;
+      std::memset(&so, 0, sizeof(so));
+      so.mode = S_NONE;
+      so.W = g.W;
+      so.H = g.H;
+      so.nData = (long)g.W * g.H;
+      so.modelOut = m->osub_d.as<double>();
+      TIMED("k_rowifft_region",
+            launch_rowifft_stat<T>(s, g, pairsR, np, m->spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, so,
+                                   cg.rowsK3, agiant),
+            "region k_rowifft");
+      TIMED("k_downsample",
+            launch_downsample(s, m->osub_d.as<double>(), g.W, (long)g.W * g.H, m->ovals_d.as<double>(), ovStride,
+                              R.ovOff, R.nbc, R.rowsReplaced, R.scale, nv),
+            "k_downsample");
+      RegionOut ro;
+      std::memset(&ro, 0, sizeof(ro));
+      ro.c0 = R.x1 - 1;
+      ro.r0 = R.y1 - 1;
+      ro.w = R.nbc;
+      ro.h = R.rowsReplaced;
+      ro.off = R.ovOff;
+      sa.regs[k] = ro;
+    }
+    sa.nReg = nR;
+    sa.ovals = m->ovals_d.as<double>();
+    sa.ovStride = ovStride;
+
+    // 3. main grid + statistic
+    sa.mode = mode;
+    sa.modelErrors = m->modelErrors ? 1 : 0;
+    sa.W = m->W;
+    sa.H = m->H;
+    sa.nData = nData;
+    sa.data = m->dataSet ? m->data_d.as<T>() : nullptr;
+    sa.weight = m->dataSet ? m->weight_d.as<T>() : nullptr;
+    sa.extra = (m->dataSet && m->poissonMLR) ? m->extra_d.as<T>() : nullptr;
+    sa.effGain = m->effGain;
+    sa.sky = m->originalSky;
+    sa.rnTerm = m->nCombined * m->rnAdu2;
+    sa.devOut = devOut ? m->dev_d.as<double>() + (long)v0 * nData : nullptr;
+    sa.modelOut = modelOut ? m->model_d.as<double>() + (long)v0 * nData : nullptr;
+    sa.partials = m->partials_d.as<double>();
+    sa.nSlots = nSlots;
+    if (m->doConv) {
+      ConvGrid& cg = m->mainConv[which];
+      const GridGeom& g = cg.g;
+      const long pstride = (long)g.Nr * g.Nc;
+      const cpx<T>* tw_c = (sizeof(T) == 8) ? (const cpx<T>*)cg.fc->tw64.p : (const cpx<T>*)cg.fc->tw32.p;
+      const cpx<T>* tw_r = (sizeof(T) == 8) ? (const cpx<T>*)cg.fr->tw64.p : (const cpx<T>*)cg.fr->tw32.p;
+      const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
+      const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
+      TIMED("k_eval_rowfft",
+            launch_eval_rowfft<T>(s, evalArgs(0, v0), g, m->pairs_d.as<int4>(), np, m->spec_d.as<cpx<T>>(), pstride,
+                                  cg.fc->plan, tw_c, cg.rowsK1),
+            "k_eval_rowfft");
+      TIMED("k_colconv", launch_colconv<T>(s, g, m->spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
+            "k_colconv");
+      TIMED("k_rowifft_stat",
+            launch_rowifft_stat<T>(s, g, m->pairs_d.as<int4>(), np, m->spec_d.as<cpx<T>>(), pstride, cg.fc->plan,
+                                   tw_c, sa, cg.rowsK3, agiant),
+            "k_rowifft_stat");
+    } else {
+      TIMED("k_eval_stat", launch_eval_stat<T>(s, evalArgs(0, v0), geoms[0], sa, nv, pixPerBlock, agiant),
+            "k_eval_stat");
+    }
+    if (mode != S_NONE && outStat)
+      TIMED("k_final",
+            launch_final(s, m->partials_d.as<double>(), nSlots, m->slots_d.as<int2>(), nv, mode,
+                         m->out_d.as<double>() + v0),
+            "k_final");
+    if (v0 + chunkVecs < nVec) {   // pairs/slots staging reuse
+      HIPCHK(hipStreamSynchronize(s), "chunk");
+      if (m->timing) HarvestTimes(m);
+    }
+  }
+  if (mode != S_NONE && outStat)
+    HIPCHK(hipMemcpyAsync(outStat, m->out_d.p, sizeof(double) * nVec, hipMemcpyDeviceToHost, s), "results D2H");
+  if (devOut)
+    HIPCHK(hipMemcpyAsync(devOut, m->dev_d.p, sizeof(double) * nData * nVec, hipMemcpyDeviceToHost, s), "dev D2H");
+  if (modelOut)
+    HIPCHK(hipMemcpyAsync(modelOut, m->model_d.p, sizeof(double) * nData * nVec, hipMemcpyDeviceToHost, s),
+           "model D2H");
+  HIPCHK(hipStreamSynchronize(s), "evaluation");
+  if (m->timing) HarvestTimes(m);
+  return IFG_OK;
+}
+
+int Run(ifg_model* m, const double* params, int nVec, double* outStat, double* devOut, double* modelOut) {
+  if (!params || nVec < 1) return m->fail(IFG_ERR_ARG, "null parameters or n_vectors < 1");
+  HIPCHK(hipSetDevice(m->device), "hipSetDevice");
+  if (m->precision == IFG_PRECISION_FP64) return Evaluate<double>(m, params, nVec, outStat, devOut, modelOut);
+  return Evaluate<float>(m, params, nVec, outStat, devOut, modelOut);
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+ifg_model* ifg_create(int device) {
+  ifg_model* m = new ifg_model();
+  m->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
+    m->err = "no HIP device / stream";
+    m->stream = nullptr;
+  }
+  return m;
+}
+
+void ifg_destroy(ifg_model* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  delete m;
+}
+
+const char* ifg_last_error(const ifg_model* m) { return m ? m->err.c_str() : "null handle"; }
+
+int ifg_set_precision(ifg_model* m, int precision) {
+  if (precision != IFG_PRECISION_FP64 && precision != IFG_PRECISION_FP32)
+    return m->fail(IFG_ERR_ARG, "unknown precision");
+  if (m->devReady && precision != m->precision) return m->fail(IFG_ERR_STATE, "precision set after first evaluation");
+  m->precision = precision;
+  return IFG_OK;
+}
+
+int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set) {
+  const int t = TypeOf(name);
+  if (!t) return m->fail(IFG_ERR_ARG, std::string("unknown function ") + (name ? name : "(null)"));
+  if (m->nComp() >= IFG_MAXCOMP) return m->fail(IFG_ERR_UNSUPPORTED, "too many components");
+  if (m->devReady) return m->fail(IFG_ERR_STATE, "function added after first evaluation");
+  const int newSet = (m->types.empty() || starts_new_function_set) ? 1 : 0;
+  m->types.push_back(t);
+  m->fsetStart.push_back(newSet);
+  m->nParamsTot += NParamsOf(t) + (newSet ? 2 : 0);
+  return IFG_OK;
+}
+
+int ifg_set_subsampling(ifg_model* m, int do_subsampling) {
+  m->doSubsampling = do_subsampling != 0;
+  return IFG_OK;
+}
+
+int ifg_get_nparams(const ifg_model* m) { return m->nParamsTot; }
+
+int ifg_add_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int normalize) {
+  if (!psf || n_columns < 1 || n_rows < 1) return m->fail(IFG_ERR_ARG, "bad PSF dimensions");
+  const long n = (long)n_columns * n_rows;
+  double sum = 0.0;
+  for (long i = 0; i < n; i++) {   // core/model_object.cpp:839-856
+    if (!std::isfinite(psf[i])) return m->fail(IFG_ERR_ARG, "PSF image has one or more non-finite values");
+    sum += psf[i];
+  }
+  if (normalize && sum <= 0.0) return m->fail(IFG_ERR_ARG, "sum of PSF pixel values is <= 0");
+  if (m->setupDone) return m->fail(IFG_ERR_STATE, "PSF added after SetupModelImage");
+  m->psf = NormalizedPSF(psf, n, normalize != 0);
+  m->Pc = n_columns;
+  m->Pr = n_rows;
+  m->doConv = true;
+  return IFG_OK;
+}
+
+int ifg_setup_model_image(ifg_model* m, int n_columns, int n_rows) {
+  if (n_columns < 1 || n_rows < 1) return m->fail(IFG_ERR_ARG, "bad image size");
+  if (m->setupDone) return m->fail(IFG_ERR_STATE, "image already set up");
+  m->W = n_columns;
+  m->H = n_rows;
+  m->nData = (long)n_columns * n_rows;
+  m->setupDone = true;
+  return IFG_OK;
+}
+
+int ifg_add_image_data(ifg_model* m, const double* data, int n_columns, int n_rows) {
+  if (!data) return m->fail(IFG_ERR_ARG, "null data");
+  int st = ifg_setup_model_image(m, n_columns, n_rows);
+  if (st < 0) return st;
+  m->data.assign(data, data + m->nData);
+  m->nValid = m->nData;
+  m->dataSet = true;
+  m->weight.assign(m->nData, 1.0);
+  m->extra.assign(m->nData, 0.0);
+  return IFG_OK;
+}
+
+int ifg_add_image_characteristics(ifg_model* m, double gain, double read_noise, double exp_time, int n_combined,
+                                  double original_sky) {
+  if (!(gain > 0.0) || !(read_noise >= 0.0) || !(exp_time > 0.0) || n_combined < 1 || !(original_sky >= 0.0))
+    return m->fail(IFG_ERR_ARG, "bad image characteristics");   // core/model_object.cpp:535-537 asserts
+  m->gain = gain;
+  m->readNoise = read_noise;
+  m->expTime = exp_time;
+  m->nCombined = n_combined;
+  m->originalSky = original_sky;
+  m->effGain = gain * exp_time * n_combined;
+  m->rnAdu2 = read_noise * read_noise / (m->effGain * m->effGain);
+  return IFG_OK;
+}
+
+int ifg_add_error_vector(ifg_model* m, const double* errors, long n_values, int input_type) {
+  if (!m->dataSet || n_values != m->nData || !errors) return m->fail(IFG_ERR_ARG, "error image size mismatch");
+  std::vector<double> w(errors, errors + n_values);
+  for (long z = 0; z < n_values; z++) {   // core/model_object.cpp:597-617
+    switch (input_type) {
+      case IFG_WEIGHTS_ARE_SIGMAS: w[z] = 1.0 / w[z]; break;
+      case IFG_WEIGHTS_ARE_VARIANCES: w[z] = 1.0 / sqrt(w[z]); break;
+      case IFG_WEIGHTS_ARE_WEIGHTS: w[z] = sqrt(w[z]); break;
+      default: return m->fail(IFG_ERR_ARG, "unknown error-image input type");
+    }
+  }
+  m->weight.swap(w);
+  m->externalErrors = true;
+  m->weightSet = true;
+  return IFG_OK;
+}
+
+int ifg_add_mask_vector(ifg_model* m, const double* mask, long n_values, int input_type) {
+  if (!m->dataSet || n_values != m->nData || !mask) return m->fail(IFG_ERR_ARG, "mask size mismatch");
+  if (input_type != IFG_MASK_ZERO_IS_GOOD && input_type != IFG_MASK_ZERO_IS_BAD)
+    return m->fail(IFG_ERR_ARG, "unknown mask input type");
+  m->mask.assign(n_values, 0.0);
+  m->nValid = 0;
+  for (long z = 0; z < n_values; z++) {   // core/model_object.cpp:743-771
+    const double v = mask[z];
+    const bool bad = (input_type == IFG_MASK_ZERO_IS_GOOD) ? (!std::isfinite(v) || v > 0.0)
+                                                           : (!std::isfinite(v) || v < 1.0);
+    m->mask[z] = bad ? 0.0 : 1.0;
+    if (!bad) m->nValid++;
+  }
+  m->maskExists = true;
+  return IFG_OK;
+}
+
+int ifg_add_oversampled_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int scale, int x1, int x2,
+                            int y1, int y2, int normalize) {
+  if (!m->setupDone) return m->fail(IFG_ERR_STATE, "oversampled PSF added before the image");
+  if (!psf || n_columns < 1 || n_rows < 1 || scale < 1 || !(x1 < x2) || !(y1 < y2))
+    return m->fail(IFG_ERR_ARG, "bad oversampled-PSF arguments");
+  if ((int)m->regions.size() >= IFG_MAXREG) return m->fail(IFG_ERR_UNSUPPORTED, "too many oversampled regions");
+  const long n = (long)n_columns * n_rows;
+  for (long i = 0; i < n; i++)
+    if (!std::isfinite(psf[i])) return m->fail(IFG_ERR_ARG, "oversampled PSF has non-finite values");
+  if (x1 < 1) x1 = 1;   // core/model_object.cpp:911-919
+  if (y1 < 1) y1 = 1;
+  if (x2 > m->W) x2 = m->W;
+  if (y2 > m->H) y2 = m->H;
+  if (x2 < x1 || y2 < y1) return m->fail(IFG_ERR_ARG, "oversampled region outside the image");
+  std::unique_ptr<Region> R(new Region());
+  R->x1 = x1;
+  R->y1 = y1;
+  R->nbc = x2 - x1 + 1;
+  R->nbr = y2 - y1 + 1;
+  R->scale = scale;
+  R->pc = n_columns;
+  R->pr = n_rows;
+  R->psf = NormalizedPSF(psf, n, normalize != 0);
+  // DownsampleAndReplace derives the replaced row count from the column count
+  // (core/downsample.cpp:78-79); keep that, but only where it stays inside the region.
+  const int nOsCols = R->nbc * scale + 2 * n_columns;
+  R->rowsReplaced = (nOsCols - 2 * n_rows) / scale;
+  if (R->rowsReplaced > R->nbr)
+    return m->fail(IFG_ERR_UNSUPPORTED,
+                   "oversampled region wider than tall with a square-ish PSF: the reference's row count would read "
+                   "outside the region");
+  R->ovOff = 0;
+  m->regions.push_back(std::move(R));
+  return IFG_OK;
+}
+
+int ifg_use_cash_statistic(ifg_model* m) {
+  m->useCash = true;
+  m->weight.assign(m->nData, 1.0);
+  m->extra.assign(m->nData, 0.0);
+  m->weightSet = true;
+  return IFG_OK;
+}
+
+int ifg_use_poisson_mlr(ifg_model* m) {
+  m->poissonMLR = true;
+  return ifg_use_cash_statistic(m);
+}
+
+int ifg_use_model_errors(ifg_model* m) {
+  m->modelErrors = true;
+  m->dataErrors = false;
+  m->weight.assign(m->nData, 1.0);
+  m->weightSet = true;
+  return IFG_OK;
+}
+
+int ifg_final_setup(ifg_model* m) {   // core/model_object.cpp:967-1079
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  const long n = m->nData;
+  int status = IFG_OK;
+  if (!m->maskExists) {
+    m->mask.assign(n, 1.0);
+    m->maskExists = true;
+  }
+  for (long z = 0; z < n; z++)
+    if (m->mask[z] > 0.0 && !std::isfinite(m->data[z])) {
+      m->mask[z] = 0.0;
+      m->nValid--;
+    }
+  if (!m->useCash && m->dataErrors && !m->externalErrors) {   // GenerateErrorVector (642-673)
+    m->weight.assign(n, 0.0);
+    for (long z = 0; z < n; z++) {
+      double totalFlux = m->data[z] + m->originalSky;
+      if (totalFlux < 0.0) totalFlux = 0.0;
+      double noise_squared = totalFlux / m->effGain + m->nCombined * m->rnAdu2;
+      m->weight[z] = 1.0 / sqrt(noise_squared);
+    }
+    m->weightSet = true;
+  }
+  if (m->useCash && m->poissonMLR) {   // GenerateExtraCashTerms (695-709)
+    for (long z = 0; z < n; z++) {
+      double dataVal = m->effGain * (m->data[z] + m->originalSky);
+      m->extra[z] = (dataVal <= 0.0) ? 0.0 : dataVal * log(dataVal) - dataVal;
+    }
+  }
+  if (m->externalErrors) {
+    for (long z = 0; z < n; z++)
+      if (m->mask[z] > 0.0 && !std::isfinite(m->weight[z])) {
+        m->mask[z] = 0.0;
+        m->weight[z] = 0.0;
+        m->nValid--;
+      }
+  }
+  if (m->weightSet) {   // ApplyMask (796-816)
+    for (long z = 0; z < n; z++) {
+      double v = m->mask[z] * m->weight[z];
+      if (!std::isfinite(v) && m->mask[z] == 0.0) v = 0.0;
+      m->weight[z] = v;
+    }
+  }
+  for (long z = 0; z < n; z++)   // CheckWeightVector (2527-2563)
+    if (m->mask[z] > 0.0 && (!std::isfinite(m->weight[z]) || m->weight[z] < 0.0)) {
+      status = m->fail(IFG_ERR_DATA, "bad values in the weight vector");
+      break;
+    }
+  bool badData = false;
+  for (long z = 0; z < n; z++) {   // VetDataVector (2500-2519)
+    if (!std::isfinite(m->data[z])) {
+      if (m->mask[z] > 0.0) badData = true;
+      else m->data[z] = 0.0;
+    }
+  }
+  if (badData && status == IFG_OK) status = m->fail(IFG_ERR_DATA, "non-finite unmasked data values");
+  if (m->nValid < 1 && status == IFG_OK) status = m->fail(IFG_ERR_DATA, "no valid data values");
+  m->finalDone = true;
+  m->devReady = false;
+  return status;
+}
+
+int ifg_which_fit_statistic(const ifg_model* m) {
+  if (m->useCash) return m->poissonMLR ? IFG_FITSTAT_POISSON_MLR : IFG_FITSTAT_CASH;
+  return IFG_FITSTAT_CHISQUARE;
+}
+
+long ifg_get_ndata(const ifg_model* m) { return m->nData; }
+long ifg_get_nvalid(const ifg_model* m) { return m->nValid; }
+
+int ifg_create_model_image(ifg_model* m, const double* params) {
+  m->lastParams.assign(params, params + m->nParamsTot);
+  std::vector<double> model(m->nData);
+  int st = Run(m, params, 1, nullptr, nullptr, model.data());
+  if (st < 0) return st;
+  m->modelValid = true;
+  m->stage.resize(sizeof(double) * m->nData);
+  std::memcpy(m->stage.data(), model.data(), sizeof(double) * m->nData);
+  return IFG_OK;
+}
+
+int ifg_get_model_image(ifg_model* m, double* out) {
+  if (!out) return m->fail(IFG_ERR_ARG, "null output");
+  if (!m->modelValid) {
+    if (m->lastParams.empty()) return m->fail(IFG_ERR_STATE, "no model computed yet");
+    std::vector<double> p = m->lastParams;
+    int st = ifg_create_model_image(m, p.data());
+    if (st < 0) return st;
+  }
+  std::memcpy(out, m->stage.data(), sizeof(double) * m->nData);
+  return IFG_OK;
+}
+
+int ifg_compute_deviates(ifg_model* m, const double* params, double* y) {
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!y) return m->fail(IFG_ERR_ARG, "null output");
+  m->lastParams.assign(params, params + m->nParamsTot);
+  m->modelValid = false;
+  return Run(m, params, 1, nullptr, y, nullptr);
+}
+
+int ifg_fit_statistic(ifg_model* m, const double* params, double* out) {
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!out) return m->fail(IFG_ERR_ARG, "null output");
+  m->lastParams.assign(params, params + m->nParamsTot);
+  m->modelValid = false;
+  return Run(m, params, 1, out, nullptr, nullptr);
+}
+
+int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, double* out) {
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!out) return m->fail(IFG_ERR_ARG, "null output");
+  m->modelValid = false;
+  return Run(m, params, n_vectors, out, nullptr, nullptr);
+}
+
+int ifg_set_kernel_timing(ifg_model* m, int on) {
+  m->timing = on != 0;
+  m->kernelTimes.clear();
+  return IFG_OK;
+}
+
+int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_len, double* total_ms, long* launches) {
+  if (index < 0 || index >= (int)m->kernelTimes.size()) return 0;
+  auto it = m->kernelTimes.begin();
+  std::advance(it, index);
+  if (name && name_len > 0) {
+    std::strncpy(name, it->first.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (total_ms) *total_ms = it->second.first;
+  if (launches) *launches = it->second.second;
+  return (int)m->kernelTimes.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// ifg_internal.h -- structures shared by the host side (ifg_api.cpp) and the
+// gfx950 kernels (ifg_kernels.hip) of the model-image library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define IFG_MAXCOMP 16   // components per model
+#define IFG_MAXBOX 4     // sub-pixel ("tail") boxes per parameter vector and grid
+#define IFG_MAXREG 4     // oversampled regions per model
+#define IFG_MAXSTAGE 16  // FFT stages
+
+namespace ifg {
+
+enum FuncType { F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6 };
+
+enum StatMode { S_NONE = 0, S_CHI2 = 1, S_CASH = 2, S_MLR = 3 };
+
+template <typename T>
+struct cpx {
+  T x, y;
+};
+
+// Per-component constants of FunctionObject::Setup, computed on the host with the
+// C library's cos/sin/pow so that the device sees the reference's exact values.
+//   v[0] x0  v[1] y0  v[2] cosPA  v[3] sinPA  v[4] q
+//   Sersic:      v[5] I_e  v[6] r_e  v[7] bn   v[8] 1/n
+//   Exponential: v[5] I_0  v[6] h
+//   Gaussian:    v[5] I_0  v[6] sigma  v[7] 2 sigma^2
+//   Moffat:      v[5] I_0  v[6] alpha  v[7] beta
+//   FlatSky:     v[5] I_sky
+//   FP32 helpers: v[12..15] (per type, see host setup)
+struct CompRec {
+  double v[16];
+};
+
+// Rectangle of grid pixels (column c0.., row r0..) that may need sub-pixel
+// integration for some component; its values are computed by the tail kernel and
+// stored at tail[vec * tailStride + off + (r - r0) * w + (c - c0)].  w == 0: unused.
+struct Box {
+  int c0, r0, w, h, off, pad0, pad1, pad2;
+};
+
+// A grid of model pixels: grid pixel (c, r) sits at x = xa + c*frac, y = ya + r*frac
+// (main image: xa = ya = 1, frac = 1, i.e. x = column + 1 as in
+// core/model_object.cpp:1156-1160; oversampled region: core/oversampled_region.cpp:309-311).
+// Convolution planes hold rows [-hb, Hh + Pr - 1 - hb) of each plane and columns
+// [-hl, W + Pc - 1 - hl), zero-padded to Nr x Nc.
+struct GridGeom {
+  int W, H;     // output extent in grid pixels
+  int hl, hb;   // halo left / top (Pc-1-Pc/2, Pr-1-Pr/2)
+  int Lc, Lr;   // evaluated plane extent
+  int Nc, Nr;   // FFT lengths
+  int Hh;       // output rows per plane
+  int pad;
+  double xa, ya, frac;
+};
+
+struct FFTPlan {
+  int N, nst;
+  int rad[IFG_MAXSTAGE];
+};
+
+struct EvalArgs {
+  int nComp;
+  int types[IFG_MAXCOMP];
+  const CompRec* recs;   // [vec * nComp + k]
+  const Box* boxes;      // [vec * IFG_MAXBOX + b]
+  const double* tail;    // tail values (see Box)
+  int tailStride;
+};
+
+// An oversampled region as seen by the main grid: data pixels [c0, c0+w) x [r0, r0+h)
+// are replaced by ovals[vec * ovStride + off + (r-r0)*w + (c-c0)] (core/downsample.cpp:96-104).
+struct RegionOut {
+  int c0, r0, w, h, off, pad0, pad1, pad2;
+};
+
+template <typename T>
+struct StatArgs {
+  int mode;           // StatMode
+  int modelErrors;    // weights from the model (UpdateWeightVector)
+  int W, H;           // data extent
+  long nData;
+  const T* data;      // [nData]
+  const T* weight;    // [nData], mask folded in (1/sigma, or 1/0 for Cash / model errors)
+  const T* extra;     // [nData] extra Cash terms (MLR), may be null
+  double effGain, sky, rnTerm;   // effective gain, original sky, nCombined * readNoise_adu^2
+  double* devOut;     // [vec * nData] or null
+  double* modelOut;   // [vec * nData] or null
+  double* partials;   // [pair][plane][slot] x 5 doubles
+  int nSlots;         // slots per plane
+  int nReg;
+  RegionOut regs[IFG_MAXREG];
+  const double* ovals;   // oversampled-region replacements
+  int ovStride;
+};
+
+}  // namespace ifg

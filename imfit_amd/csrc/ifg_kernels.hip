@@ -1,0 +1,763 @@
+// ifg_kernels.hip -- gfx950 kernels of the model-image path.
+//
+// Pipeline for one PSF-convolved evaluation (main grid or one oversampled region):
+//   k_tail          one wavefront per pixel near a component centre: sub-pixel
+//                   integration spread over the 64 lanes (func_sersic.cpp:150-213 etc.)
+//   k_eval_rowfft   evaluates the model (Kahan over components, core/model_object.cpp:1155-1174)
+//                   for two real planes packed as one complex plane, straight into LDS,
+//                   and runs the forward row FFT there
+//   k_colconv       column FFT, multiply by the PSF spectrum, inverse column FFT, in LDS
+//   k_rowifft_stat  inverse row FFT, then the deviates / fit-statistic terms of the
+//                   data pixels (core/model_object.cpp:1413-1496,1633-1775) with
+//                   per-block partial sums
+//   k_final         per-vector reduction of the partials: mp_enorm^2 or 2*Cash sum
+// Without a PSF, k_eval_stat does evaluation + statistic in one pass.
+//
+// The real->complex packing: since the PSF is real, conv(a + i b) = conv(a) + i conv(b),
+// so one complex FFT convolution carries two real images (two parameter vectors, or
+// the top and bottom halves of one image).  Linear convolution only needs the data
+// region plus a PSF halo, so planes are (data + P - 1) wide, padded to 2^a 3^b 5^c.
+//
+// FP64 kernels are compiled with -ffp-contract=off so that coordinates, radii and
+// sub-pixel counts follow the reference's rounding exactly.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include "ifg_internal.h"
+
+namespace ifg {
+
+// ------------------------------------------------------------------ complex helpers
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i*dir  (dir = +1 forward: -i ; dir = -1 inverse: +i)
+template <typename T>
+__device__ __forceinline__ cpx<T> mul_mi(cpx<T> a, int dir) {
+  return dir > 0 ? cpx<T>{a.y, -a.x} : cpx<T>{-a.y, a.x};
+}
+
+// R-point DFTs; dir = +1 forward (exp(-2 pi i/R)), -1 inverse.
+template <typename T>
+__device__ __forceinline__ void dft2(cpx<T>* a) {
+  cpx<T> t = a[0];
+  a[0] = cadd(t, a[1]);
+  a[1] = csub(t, a[1]);
+}
+template <typename T>
+__device__ __forceinline__ void dft4(cpx<T>* a, int dir) {
+  cpx<T> s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+  cpx<T> s13 = cadd(a[1], a[3]), d13 = mul_mi(csub(a[1], a[3]), dir);
+  a[0] = cadd(s02, s13);
+  a[2] = csub(s02, s13);
+  a[1] = cadd(d02, d13);
+  a[3] = csub(d02, d13);
+}
+template <typename T>
+__device__ __forceinline__ void dft3(cpx<T>* a, int dir) {
+  const T S3 = (T)0.86602540378443864676372317075294;
+  cpx<T> t1 = cadd(a[1], a[2]);
+  cpx<T> t2 = {a[0].x - (T)0.5 * t1.x, a[0].y - (T)0.5 * t1.y};
+  cpx<T> t3 = mul_mi(cscale(csub(a[1], a[2]), S3), dir);
+  a[0] = cadd(a[0], t1);
+  a[1] = cadd(t2, t3);
+  a[2] = csub(t2, t3);
+}
+template <typename T>
+__device__ __forceinline__ void dft5(cpx<T>* a, int dir) {
+  const T C1 = (T)0.30901699437494742410229341718282, C2 = (T)-0.80901699437494742410229341718282;
+  const T S1 = (T)0.95105651629515357211643933337938, S2 = (T)0.58778525229247312916870595463907;
+  cpx<T> b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+  cpx<T> d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+  cpx<T> p1 = {a[0].x + C1 * b1.x + C2 * b2.x, a[0].y + C1 * b1.y + C2 * b2.y};
+  cpx<T> p2 = {a[0].x + C2 * b1.x + C1 * b2.x, a[0].y + C2 * b1.y + C1 * b2.y};
+  cpx<T> q1 = mul_mi(cpx<T>{S1 * d1.x + S2 * d2.x, S1 * d1.y + S2 * d2.y}, dir);
+  cpx<T> q2 = mul_mi(cpx<T>{S2 * d1.x - S1 * d2.x, S2 * d1.y - S1 * d2.y}, dir);
+  a[0] = cadd(a[0], cadd(b1, b2));
+  a[1] = cadd(p1, q1);
+  a[4] = csub(p1, q1);
+  a[2] = cadd(p2, q2);
+  a[3] = csub(p2, q2);
+}
+
+// One Stockham autosort stage of radix R over `nseq` sequences of length N
+// (sequence s at src + s*ld): butterfly j of sub-transform size Ns reads
+// x[j + r*N/R], twiddles by w^(k*r) with k = j mod Ns, and writes the R outputs
+// to ((j-k)*R + k) + m*Ns.  After all stages the output is in natural order.
+template <typename T, int R>
+__device__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ dst, int N, int Ns, int nseq,
+                          int ld, const cpx<T>* __restrict__ tw, int dir) {
+  const int nb = N / R;
+  const int total = nb * nseq;
+  const int twstep = N / (Ns * R);
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    const int seq = b / nb;
+    const int j = b - seq * nb;
+    const int k = j % Ns;
+    const cpx<T>* s = src + (long)seq * ld;
+    cpx<T> a[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) a[r] = s[j + r * nb];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        cpx<T> w = tw[k * r * twstep];
+        if (dir < 0) w.y = -w.y;
+        a[r] = cmul(a[r], w);
+      }
+    }
+    if (R == 2) dft2(a);
+    if (R == 3) dft3(a, dir);
+    if (R == 4) dft4(a, dir);
+    if (R == 5) dft5(a, dir);
+    cpx<T>* d = dst + (long)seq * ld + (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r * Ns] = a[r];
+  }
+}
+
+// Full FFT of nseq sequences held in LDS buffer A (B is scratch); returns the
+// buffer that holds the result.  Ends with a barrier.
+template <typename T>
+__device__ cpx<T>* fft_lds(cpx<T>* A, cpx<T>* B, const FFTPlan& pl, int nseq, int ld, const cpx<T>* tw,
+                           int dir) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nst; s++) {
+    const int R = pl.rad[s];
+    if (R == 4) fft_stage<T, 4>(A, B, pl.N, Ns, nseq, ld, tw, dir);
+    else if (R == 2) fft_stage<T, 2>(A, B, pl.N, Ns, nseq, ld, tw, dir);
+    else if (R == 3) fft_stage<T, 3>(A, B, pl.N, Ns, nseq, ld, tw, dir);
+    else fft_stage<T, 5>(A, B, pl.N, Ns, nseq, ld, tw, dir);
+    __syncthreads();
+    cpx<T>* t = A;
+    A = B;
+    B = t;
+    Ns *= R;
+  }
+  return A;
+}
+
+// ------------------------------------------------------------------ image functions
+// Point intensity at radius r (CalculateIntensity of func_sersic.cpp:135-141,
+// func_exp.cpp:127-130, func_gaussian.cpp:123-128, func_moffat.cpp:140-147).
+__device__ __forceinline__ double intensity_f64(int type, const double* v, double r) {
+  switch (type) {
+    case F_SERSIC: return v[5] * exp(-v[7] * (pow((r / v[6]), v[8]) - 1.0));
+    case F_EXP: return v[5] * exp(-r / v[6]);
+    case F_GAUSS: {
+      double r_squared = r * r;
+      return v[5] * exp(-r_squared / v[7]);
+    }
+    case F_MOFFAT: {
+      double scaledR = r / v[6];
+      double denominator = pow((1.0 + scaledR * scaledR), v[7]);
+      return (v[5] / denominator);
+    }
+    default: return v[5];
+  }
+}
+
+// FP32 intensity from the squared radius, with hardware exp2/log2 (v_exp_f32, v_log_f32).
+__device__ __forceinline__ float intensity_f32_r2(int type, const double* v, float r2) {
+  switch (type) {
+    case F_SERSIC: {
+      float p = __builtin_amdgcn_exp2f((float)v[13] * __builtin_amdgcn_logf(r2 * (float)v[12]));
+      return (float)v[5] * __builtin_amdgcn_exp2f((float)v[14] * (p - 1.0f));
+    }
+    case F_EXP: return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * __builtin_sqrtf(r2));
+    case F_GAUSS: return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * r2);
+    case F_MOFFAT:
+      return (float)v[5] * __builtin_amdgcn_exp2f((float)v[13] * __builtin_amdgcn_logf(1.0f + r2 * (float)v[12]));
+    default: return (float)v[5];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T intensity_r(int type, const double* v, double r);
+template <>
+__device__ __forceinline__ double intensity_r<double>(int type, const double* v, double r) {
+  return intensity_f64(type, v, r);
+}
+template <>
+__device__ __forceinline__ float intensity_r<float>(int type, const double* v, double r) {
+  return intensity_f32_r2(type, v, (float)(r * r));
+}
+
+// Elliptical radius in the component frame (func_sersic.cpp:159-162).
+__device__ __forceinline__ double ell_radius(const double* v, double x, double y) {
+  double x_diff = x - v[0];
+  double y_diff = y - v[1];
+  double xp = x_diff * v[2] + y_diff * v[3];
+  double yp_scaled = (-x_diff * v[3] + y_diff * v[2]) / v[4];
+  return sqrt(xp * xp + yp_scaled * yp_scaled);
+}
+
+// Point value without sub-pixel integration (bulk pixels).
+template <typename T>
+__device__ __forceinline__ T comp_point(int type, const double* v, double x, double y);
+template <>
+__device__ __forceinline__ double comp_point<double>(int type, const double* v, double x, double y) {
+  if (type == F_FLATSKY) return v[5];
+  return intensity_f64(type, v, ell_radius(v, x, y));
+}
+template <>
+__device__ __forceinline__ float comp_point<float>(int type, const double* v, double x, double y) {
+  if (type == F_FLATSKY) return (float)v[5];
+  float xd = (float)(x - v[0]), yd = (float)(y - v[1]);
+  float c = (float)v[2], s = (float)v[3];
+  float xp = xd * c + yd * s;
+  float yp = (yd * c - xd * s) * (float)v[15];
+  return intensity_f32_r2(type, v, xp * xp + yp * yp);
+}
+
+// GALFIT sub-pixel count (func_sersic.cpp:197-213; func_exp.cpp:181-197;
+// func_gaussian.cpp:187-203; func_moffat.cpp:200-216).  v[6] is r_e / h / sigma / alpha.
+__device__ __forceinline__ int n_subsamples(int type, const double* v, double r, int on) {
+  if (!on || !(r < 10.0)) return 1;
+  double size = v[6];
+  double small = (type == F_GAUSS || type == F_MOFFAT) ? 40.0 : 20.0;
+  double rlim = (type == F_SERSIC) ? 4.0 : 3.0;
+  if (type != F_SERSIC && type != F_EXP && type != F_GAUSS && type != F_MOFFAT) return 1;
+  if ((size <= 1.0) && (r <= 1.0)) return min(100, (int)(small / size));
+  if (r <= rlim) return 20;
+  return min(100, (int)(20 / r));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------ model value per grid pixel
+template <typename T>
+__device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, int vec, int r, int c) {
+  const Box* bx = ea.boxes + vec * IFG_MAXBOX;
+#pragma unroll
+  for (int b = 0; b < IFG_MAXBOX; b++) {
+    const Box B = bx[b];
+    const int dc = c - B.c0, dr = r - B.r0;
+    if (B.w > 0 && (unsigned)dc < (unsigned)B.w && (unsigned)dr < (unsigned)B.h)
+      return (T)ea.tail[(long)vec * ea.tailStride + B.off + dr * B.w + dc];
+  }
+  const double x = g.xa + (double)c * g.frac;
+  const double y = g.ya + (double)r * g.frac;
+  const CompRec* rec = ea.recs + (long)vec * ea.nComp;
+  T newValSum = 0, storedError = 0;
+  for (int n = 0; n < ea.nComp; n++) {   // Kahan summation (core/model_object.cpp:1164-1172)
+    T adjVal = comp_point<T>(ea.types[n], rec[n].v, x, y) - storedError;
+    T tempSum = newValSum + adjVal;
+    storedError = (tempSum - newValSum) - adjVal;
+    newValSum = tempSum;
+  }
+  return newValSum;
+}
+
+// One wavefront per box pixel: the full Kahan sum over components, with each
+// component's nSub x nSub sub-pixel average spread across the 64 lanes.  Sample
+// coordinates follow func_sersic.cpp:165-181 exactly (FP64); only the order of the
+// final sum differs (lane partials + xor butterfly, identical in every lane).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail(EvalArgs ea, GridGeom g, int nVec, int subsampling) {
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int vec = wave / ea.tailStride;
+  const int loc = wave - vec * ea.tailStride;
+  if (vec >= nVec) return;
+  const Box* bx = ea.boxes + vec * IFG_MAXBOX;
+  int c = 0, r = 0;
+  bool found = false;
+  for (int b = 0; b < IFG_MAXBOX; b++) {
+    const Box B = bx[b];
+    if (B.w > 0 && loc >= B.off && loc < B.off + B.w * B.h) {
+      c = B.c0 + (loc - B.off) % B.w;
+      r = B.r0 + (loc - B.off) / B.w;
+      found = true;
+      break;
+    }
+  }
+  if (!found) return;
+  const double x = g.xa + (double)c * g.frac;
+  const double y = g.ya + (double)r * g.frac;
+  const CompRec* rec = ea.recs + (long)vec * ea.nComp;
+  double newValSum = 0.0, storedError = 0.0;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int type = ea.types[n];
+    const double* v = rec[n].v;
+    double val;
+    if (type == F_FLATSKY) {
+      val = v[5];
+    } else {
+      const double rad = ell_radius(v, x, y);
+      const int nSub = n_subsamples(type, v, rad, subsampling);
+      if (nSub > 1) {
+        const double deltaSubpix = 1.0 / nSub;
+        const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+        const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+        const int n2 = nSub * nSub;
+        double part = 0.0;
+        for (int s = lane; s < n2; s += 64) {
+          const int ii = s / nSub, jj = s - ii * nSub;
+          const double x_ii = x_sub_start + ii * deltaSubpix;
+          const double y_ii = y_sub_start + jj * deltaSubpix;
+          part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
+        }
+        val = wave_sum(part) / (nSub * nSub);
+      } else {
+        val = (double)intensity_r<T>(type, v, rad);
+      }
+    }
+    double adjVal = val - storedError;
+    double tempSum = newValSum + adjVal;
+    storedError = (tempSum - newValSum) - adjVal;
+    newValSum = tempSum;
+  }
+  if (lane == 0) const_cast<double*>(ea.tail)[(long)vec * ea.tailStride + loc] = newValSum;
+}
+
+// ------------------------------------------------------------------ evaluation + row FFT
+template <typename T>
+__global__ __launch_bounds__(256) void k_eval_rowfft(EvalArgs ea, GridGeom g, const int4* __restrict__ pairs,
+                                                     cpx<T>* __restrict__ spec, long pairStride, FFTPlan pl,
+                                                     const cpx<T>* __restrict__ tw, int rowsPerBlock) {
+  extern __shared__ char smem_raw[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + rowsPerBlock * g.Nc;
+  const int4 pr = pairs[blockIdx.y];
+  const int ir0 = blockIdx.x * rowsPerBlock;
+  const int total = rowsPerBlock * g.Nc;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int rr = e / g.Nc, j = e - rr * g.Nc, ir = ir0 + rr;
+    T va = 0, vb = 0;
+    if (ir < g.Lr && j < g.Lc) {
+      const int c = j - g.hl;
+      va = model_value<T>(ea, g, pr.x, pr.y + ir - g.hb, c);
+      if (pr.z >= 0) vb = model_value<T>(ea, g, pr.z, pr.w + ir - g.hb, c);
+    }
+    A[e] = cpx<T>{va, vb};
+  }
+  __syncthreads();
+  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, tw, +1);
+  cpx<T>* S = spec + (long)blockIdx.y * pairStride;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int rr = e / g.Nc, ir = ir0 + rr;
+    if (ir < g.Lr) S[(long)ir * g.Nc + (e - rr * g.Nc)] = out[e];
+  }
+}
+
+// Generic in-place row FFT (PSF spectrum setup).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, FFTPlan pl, const cpx<T>* tw,
+                                                int dir, int rowsPerBlock) {
+  extern __shared__ char smem_raw[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + rowsPerBlock * Nc;
+  const int ir0 = blockIdx.x * rowsPerBlock;
+  const int total = rowsPerBlock * Nc;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int ir = ir0 + e / Nc;
+    A[e] = (ir < nRows) ? buf[(long)ir0 * Nc + e] : cpx<T>{0, 0};
+  }
+  __syncthreads();
+  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, Nc, tw, dir);
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int ir = ir0 + e / Nc;
+    if (ir < nRows) buf[(long)ir0 * Nc + e] = out[e];
+  }
+}
+
+// ------------------------------------------------------------------ column pass
+// mode 0: forward column FFT of rows [0, Lr) (rest zero), multiply by the PSF
+//         spectrum (already scaled by 1/(Nr*Nc)), inverse column FFT, store rows
+//         [hb, hb + Hh);  mode 1: forward column FFT of all rows, store all rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_colconv(GridGeom g, cpx<T>* __restrict__ spec, long pairStride,
+                                                 const cpx<T>* __restrict__ khat, FFTPlan pl,
+                                                 const cpx<T>* __restrict__ tw, int C, int ld, int mode) {
+  extern __shared__ char smem_raw[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + C * ld;
+  const int c0 = blockIdx.x * C;
+  cpx<T>* S = spec + (long)blockIdx.y * pairStride;
+  const int Nr = g.Nr, Nc = g.Nc;
+  const int rowsIn = (mode == 0) ? g.Lr : Nr;
+  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+    const int i = e / C, cc = e - i * C, col = c0 + cc;
+    cpx<T> v = {0, 0};
+    if (i < rowsIn && col < Nc) v = S[(long)i * Nc + col];
+    A[cc * ld + i] = v;
+  }
+  __syncthreads();
+  cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, tw, +1);
+  cpx<T>* Y = (X == A) ? B : A;
+  if (mode == 0) {
+    for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+      const int cc = e / Nr, i = e - cc * Nr, col = c0 + cc;
+      if (col < Nc) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
+    }
+    __syncthreads();
+    X = fft_lds<T>(X, Y, pl, C, ld, tw, -1);
+    for (int e = threadIdx.x; e < g.Hh * C; e += blockDim.x) {
+      const int i = g.hb + e / C, cc = e % C, col = c0 + cc;
+      if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
+    }
+  } else {
+    for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+      const int i = e / C, cc = e - i * C, col = c0 + cc;
+      if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ statistic terms
+// MINPACK enorm accumulator (core/mp_enorm.cpp:54-122) in a mergeable form:
+// s2 = plain sum of squares of intermediate values; (s1, x1max) and (s3, x3max)
+// the scaled sums of the large and small values.
+struct ENorm {
+  double s1, x1max, s2, s3, x3max;
+};
+__device__ __forceinline__ void en_init(ENorm& e) { e.s1 = e.x1max = e.s2 = e.s3 = e.x3max = 0.0; }
+__device__ __forceinline__ void en_add(ENorm& e, double x, double agiant) {
+  const double RDWARF = 3.834e-20;
+  double xabs = fabs(x);
+  if (xabs > RDWARF && xabs < agiant) {
+    e.s2 += xabs * xabs;
+  } else if (xabs > RDWARF) {
+    if (xabs > e.x1max) {
+      double t = e.x1max / xabs;
+      e.s1 = 1.0 + e.s1 * t * t;
+      e.x1max = xabs;
+    } else {
+      double t = xabs / e.x1max;
+      e.s1 += t * t;
+    }
+  } else if (xabs > e.x3max) {
+    double t = e.x3max / xabs;
+    e.s3 = 1.0 + e.s3 * t * t;
+    e.x3max = xabs;
+  } else if (xabs != 0.0) {
+    double t = xabs / e.x3max;
+    e.s3 += t * t;
+  }
+}
+__device__ __forceinline__ void en_merge(ENorm& a, const ENorm& b) {
+  a.s2 += b.s2;
+  if (b.x1max > 0.0) {
+    if (b.x1max > a.x1max) {
+      double t = a.x1max / b.x1max;
+      a.s1 = b.s1 + a.s1 * t * t;
+      a.x1max = b.x1max;
+    } else {
+      double t = b.x1max / a.x1max;
+      a.s1 += b.s1 * t * t;
+    }
+  }
+  if (b.x3max > 0.0) {
+    if (b.x3max > a.x3max) {
+      double t = a.x3max / b.x3max;
+      a.s3 = b.s3 + a.s3 * t * t;
+      a.x3max = b.x3max;
+    } else {
+      double t = b.x3max / a.x3max;
+      a.s3 += b.s3 * t * t;
+    }
+  }
+}
+
+// Model value m at data pixel (r, c) of vector vec -> deviate / statistic term.
+template <typename T>
+__device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r, int c, double m, ENorm& acc,
+                                           double agiant) {
+  for (int k = 0; k < sa.nReg; k++) {   // oversampled regions replace main pixels (core/downsample.cpp:104)
+    const RegionOut& R = sa.regs[k];
+    const int dc = c - R.c0, dr = r - R.r0;
+    if ((unsigned)dc < (unsigned)R.w && (unsigned)dr < (unsigned)R.h)
+      m = sa.ovals[(long)vec * sa.ovStride + R.off + dr * R.w + dc];
+  }
+  const long z = (long)r * sa.W + c;
+  if (sa.modelOut) sa.modelOut[(long)vec * sa.nData + z] = m;
+  if (sa.mode == S_NONE) return;
+  const double d = (double)sa.data[z];
+  double w = (double)sa.weight[z];
+  if (sa.modelErrors && w != 0.0) {   // UpdateWeightVector (core/model_object.cpp:1345-1381)
+    double totalFlux = m + sa.sky;
+    double noise_squared = totalFlux / sa.effGain + sa.rnTerm;
+    w = 1.0 / sqrt(noise_squared);
+  }
+  if (sa.mode == S_CHI2) {
+    const double dev = w * (d - m);
+    en_add(acc, dev, agiant);
+    if (sa.devOut) sa.devOut[(long)vec * sa.nData + z] = dev;
+  } else {   // Cash / Poisson-MLR (core/model_object.cpp:1386-1401,1703-1775)
+    const double modVal = sa.effGain * (m + sa.sky);
+    const double dataVal = sa.effGain * (d + sa.sky);
+    const double logModel = (modVal <= 0) ? 1.0e-25 : log(modVal);
+    const double extraTerms = sa.extra ? (double)sa.extra[z] : 0.0;
+    const double term = modVal - dataVal * logModel + extraTerms;
+    acc.s2 += w * term;
+    if (sa.devOut)
+      sa.devOut[(long)vec * sa.nData + z] =
+          (sa.mode == S_MLR) ? sqrt(2.0 * w * fabs(term)) : w * (d - m);
+  }
+}
+
+// Block-wide merge of the enorm state; the result is valid in thread 0.
+__device__ ENorm block_reduce(ENorm e) {
+  __shared__ double red[5][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // deterministic: xor butterfly inside the wave (both partners merge lower-with-
+  // upper, so they agree bitwise), then the waves in order
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    ENorm o;
+    o.s1 = __shfl_xor(e.s1, m, 64);
+    o.x1max = __shfl_xor(e.x1max, m, 64);
+    o.s2 = __shfl_xor(e.s2, m, 64);
+    o.s3 = __shfl_xor(e.s3, m, 64);
+    o.x3max = __shfl_xor(e.x3max, m, 64);
+    if (lane & m) {
+      ENorm t = o;
+      en_merge(t, e);
+      e = t;
+    } else {
+      en_merge(e, o);
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    red[0][wv] = e.s1; red[1][wv] = e.x1max; red[2][wv] = e.s2; red[3][wv] = e.s3; red[4][wv] = e.x3max;
+  }
+  __syncthreads();
+  ENorm t = e;
+  if (threadIdx.x == 0) {
+    t = ENorm{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0]};
+    const int nw = blockDim.x >> 6;
+    for (int k = 1; k < nw; k++) {
+      ENorm o = {red[0][k], red[1][k], red[2][k], red[3][k], red[4][k]};
+      en_merge(t, o);
+    }
+  }
+  return t;
+}
+
+__device__ void block_reduce_store(ENorm e, double* dst) {
+  ENorm t = block_reduce(e);
+  if (threadIdx.x == 0) {
+    dst[0] = t.s1; dst[1] = t.x1max; dst[2] = t.s2; dst[3] = t.s3; dst[4] = t.x3max;
+  }
+}
+
+// ------------------------------------------------------------------ inverse row FFT + statistic
+template <typename T>
+__global__ __launch_bounds__(256) void k_rowifft_stat(GridGeom g, const int4* __restrict__ pairs,
+                                                      const cpx<T>* __restrict__ spec, long pairStride, FFTPlan pl,
+                                                      const cpx<T>* __restrict__ tw, StatArgs<T> sa,
+                                                      int rowsPerBlock, double agiant) {
+  extern __shared__ char smem_raw[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + rowsPerBlock * g.Nc;
+  const int4 pr = pairs[blockIdx.y];
+  const int io0 = blockIdx.x * rowsPerBlock;
+  const int total = rowsPerBlock * g.Nc;
+  const cpx<T>* S = spec + (long)blockIdx.y * pairStride;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int rr = e / g.Nc, io = io0 + rr;
+    A[e] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + (e - rr * g.Nc)] : cpx<T>{0, 0};
+  }
+  __syncthreads();
+  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, tw, -1);
+  ENorm ea, eb;
+  en_init(ea);
+  en_init(eb);
+  const int nOut = rowsPerBlock * g.W;
+  for (int e = threadIdx.x; e < nOut; e += blockDim.x) {
+    const int rr = e / g.W, c = e - rr * g.W, io = io0 + rr;
+    if (io >= g.Hh) continue;
+    const cpx<T> v = out[rr * g.Nc + c + g.hl];
+    const int ra = pr.y + io;
+    if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)v.x, ea, agiant);
+    if (pr.z >= 0) {
+      const int rb = pr.w + io;
+      if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)v.y, eb, agiant);
+    }
+  }
+  if (sa.partials) {
+    double* base = sa.partials + ((long)blockIdx.y * 2 * sa.nSlots) * 5;
+    block_reduce_store(ea, base + (long)blockIdx.x * 5);
+    block_reduce_store(eb, base + ((long)sa.nSlots + blockIdx.x) * 5);
+  }
+}
+
+// ------------------------------------------------------------------ no-PSF: evaluation + statistic
+template <typename T>
+__global__ __launch_bounds__(256) void k_eval_stat(EvalArgs ea, GridGeom g, StatArgs<T> sa, int pixPerBlock,
+                                                   double agiant) {
+  const int vec = blockIdx.y;
+  const long p0 = (long)blockIdx.x * pixPerBlock;
+  const long npix = (long)g.W * g.H;
+  ENorm acc;
+  en_init(acc);
+  for (int e = threadIdx.x; e < pixPerBlock; e += blockDim.x) {
+    const long p = p0 + e;
+    if (p >= npix) break;
+    const int r = (int)(p / g.W), c = (int)(p - (long)r * g.W);
+    const double m = (double)model_value<T>(ea, g, vec, r, c);
+    stat_pixel<T>(sa, vec, r, c, m, acc, agiant);
+  }
+  if (sa.partials) {
+    // pair = vec, plane 0
+    double* base = sa.partials + ((long)vec * 2 * sa.nSlots) * 5;
+    block_reduce_store(acc, base + (long)blockIdx.x * 5);
+  }
+}
+
+// ------------------------------------------------------------------ oversampled-region block average
+// DownsampleAndReplace's s x s box mean (core/downsample.cpp:84-106), rows outer,
+// columns inner, one output pixel per thread.
+__global__ void k_downsample(const double* __restrict__ osub, int Wos, long osubStride, double* __restrict__ ovals,
+                             int ovStride, int off, int nColsSub, int nRowsSub, int scale, int nVec) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vec = blockIdx.y;
+  if (t >= nColsSub * nRowsSub || vec >= nVec) return;
+  const int bi = t / nColsSub, bj = t - bi * nColsSub;
+  const double* src = osub + (long)vec * osubStride;
+  double binned = 0.0;
+  for (int ii = bi * scale; ii < bi * scale + scale; ii++)
+    for (int jj = bj * scale; jj < bj * scale + scale; jj++) binned += src[(long)ii * Wos + jj];
+  const double area = scale * scale;
+  ovals[(long)vec * ovStride + off + t] = binned / area;
+}
+
+// ------------------------------------------------------------------ final per-vector reduction
+// vecSlots[v] = (pair, planeMask): the partial slots that belong to vector v.
+__global__ __launch_bounds__(256) void k_final(const double* __restrict__ partials, int nSlots,
+                                               const int2* __restrict__ vecSlots, int mode, double* out) {
+  const int v = blockIdx.x;
+  const int2 vs = vecSlots[v];
+  ENorm e;
+  en_init(e);
+  for (int plane = 0; plane < 2; plane++) {
+    if (!(vs.y & (1 << plane))) continue;
+    const double* base = partials + ((long)vs.x * 2 + plane) * nSlots * 5;
+    for (int s = threadIdx.x; s < nSlots; s += blockDim.x) {
+      ENorm o = {base[s * 5], base[s * 5 + 1], base[s * 5 + 2], base[s * 5 + 3], base[s * 5 + 4]};
+      en_merge(e, o);
+    }
+  }
+  ENorm t = block_reduce(e);
+  if (threadIdx.x == 0) {
+    double ans;
+    if (mode == S_CHI2) {   // final step of mp_enorm (core/mp_enorm.cpp:104-121), squared
+      if (t.s1 != 0.0) {
+        ans = t.x1max * sqrt(t.s1 + (t.s2 / t.x1max) / t.x1max);
+      } else if (t.s2 != 0.0) {
+        double temp;
+        if (t.s2 >= t.x3max) temp = t.s2 * (1.0 + (t.x3max / t.s2) * (t.x3max * t.s3));
+        else temp = t.x3max * ((t.s2 / t.x3max) + (t.x3max * t.s3));
+        ans = sqrt(temp);
+      } else {
+        ans = t.x3max * sqrt(t.s3);
+      }
+      out[v] = ans * ans;
+    } else {
+      out[v] = 2.0 * t.s2;
+    }
+  }
+}
+
+}  // namespace ifg
+
+// ====================================================================== launchers
+namespace ifg {
+
+static size_t smem_rows(int rows, int N, size_t elem) { return 2 * (size_t)rows * N * elem; }
+
+template <typename T>
+hipError_t launch_tail(hipStream_t s, const EvalArgs& ea, const GridGeom& g, int nVec, int sub) {
+  const long waves = (long)nVec * ea.tailStride;
+  if (waves <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tail<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, ea, g, nVec, sub);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_eval_rowfft(hipStream_t s, const EvalArgs& ea, const GridGeom& g, const int4* pairs, int nPairs,
+                              cpx<T>* spec, long pairStride, const FFTPlan& pl, const cpx<T>* tw, int rows) {
+  dim3 grid((g.Lr + rows - 1) / rows, nPairs);
+  hipLaunchKernelGGL(k_eval_rowfft<T>, grid, dim3(256), smem_rows(rows, g.Nc, sizeof(cpx<T>)), s, ea, g, pairs,
+                     spec, pairStride, pl, tw, rows);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
+                         int dir, int rows) {
+  hipLaunchKernelGGL(k_rowfft<T>, dim3((nRows + rows - 1) / rows), dim3(256), smem_rows(rows, Nc, sizeof(cpx<T>)), s,
+                     buf, nRows, Nc, pl, tw, dir, rows);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_colconv(hipStream_t s, const GridGeom& g, cpx<T>* spec, long pairStride, int nPairs,
+                          const cpx<T>* khat, const FFTPlan& pl, const cpx<T>* tw, int C, int mode) {
+  const int ld = g.Nr + 1;
+  dim3 grid((g.Nc + C - 1) / C, nPairs);
+  hipLaunchKernelGGL(k_colconv<T>, grid, dim3(256), 2 * (size_t)C * ld * sizeof(cpx<T>), s, g, spec, pairStride,
+                     khat, pl, tw, C, ld, mode);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rowifft_stat(hipStream_t s, const GridGeom& g, const int4* pairs, int nPairs, const cpx<T>* spec,
+                               long pairStride, const FFTPlan& pl, const cpx<T>* tw, const StatArgs<T>& sa, int rows,
+                               double agiant) {
+  dim3 grid((g.Hh + rows - 1) / rows, nPairs);
+  hipLaunchKernelGGL(k_rowifft_stat<T>, grid, dim3(256), smem_rows(rows, g.Nc, sizeof(cpx<T>)), s, g, pairs, spec,
+                     pairStride, pl, tw, sa, rows, agiant);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_eval_stat(hipStream_t s, const EvalArgs& ea, const GridGeom& g, const StatArgs<T>& sa, int nVec,
+                            int pixPerBlock, double agiant) {
+  const long npix = (long)g.W * g.H;
+  dim3 grid((unsigned)((npix + pixPerBlock - 1) / pixPerBlock), nVec);
+  hipLaunchKernelGGL(k_eval_stat<T>, grid, dim3(256), 0, s, ea, g, sa, pixPerBlock, agiant);
+  return hipGetLastError();
+}
+
+hipError_t launch_downsample(hipStream_t s, const double* osub, int Wos, long osubStride, double* ovals,
+                             int ovStride, int off, int nColsSub, int nRowsSub, int scale, int nVec) {
+  const int n = nColsSub * nRowsSub;
+  hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256, nVec), dim3(256), 0, s, osub, Wos, osubStride, ovals,
+                     ovStride, off, nColsSub, nRowsSub, scale, nVec);
+  return hipGetLastError();
+}
+
+hipError_t launch_final(hipStream_t s, const double* partials, int nSlots, const int2* vecSlots, int nVec, int mode,
+                        double* out) {
+  hipLaunchKernelGGL(k_final, dim3(nVec), dim3(256), 0, s, partials, nSlots, vecSlots, mode, out);
+  return hipGetLastError();
+}
+
+#define IFG_INST(T)                                                                                                 \
+  template hipError_t launch_tail<T>(hipStream_t, const EvalArgs&, const GridGeom&, int, int);                     \
+  template hipError_t launch_eval_rowfft<T>(hipStream_t, const EvalArgs&, const GridGeom&, const int4*, int,        \
+                                            cpx<T>*, long, const FFTPlan&, const cpx<T>*, int);                      \
+  template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
+  template hipError_t launch_colconv<T>(hipStream_t, const GridGeom&, cpx<T>*, long, int, const cpx<T>*,          \
+                                        const FFTPlan&, const cpx<T>*, int, int);                                   \
+  template hipError_t launch_rowifft_stat<T>(hipStream_t, const GridGeom&, const int4*, int, const cpx<T>*, long,   \
+                                             const FFTPlan&, const cpx<T>*, const StatArgs<T>&, int, double);       \
+  template hipError_t launch_eval_stat<T>(hipStream_t, const EvalArgs&, const GridGeom&, const StatArgs<T>&, int,  \
+                                          int, double);
+IFG_INST(double)
+IFG_INST(float)
+
+}  // namespace ifg

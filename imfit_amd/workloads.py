@@ -1,0 +1,119 @@
+"""Synthetic workloads of the benchmark configurations, built on the GPU library.
+
+Each workload is a ModelObject call sequence (functions, PSF, data, oversampled
+regions, image characteristics, statistic) plus truth and start parameters.
+Data images are made the way the benchmark plan prescribes: the noise-free model
+T (computed here by the GPU library in FP64 make-image mode), electrons
+e ~ Poisson(gain*T), read noise r ~ N(0, rn) electrons, data = (e + r)/gain,
+with numpy's PCG64 generator seeded per configuration.  PSFs are Moffat
+functions evaluated by the same library (sub-pixel integration on) at the pixel
+centres of an odd grid centred on pixel (P+1)/2, normalised to sum 1.
+"""
+import numpy as np
+
+from . import ModelObjectGPU, PRECISION_FP64
+
+
+class Workload:
+    def __init__(self, name, funcs, truth, width, height, psf=None, osamp=(), gain=4.0, read_noise=0.0,
+                 noise_seed=0, subsampling=True):
+        self.name = name
+        self.funcs = list(funcs)
+        self.truth = np.asarray(truth, dtype=np.float64)
+        self.width, self.height = width, height
+        self.psf = psf
+        self.osamp = list(osamp)   # (psf, scale, x1, x2, y1, y2)
+        self.gain, self.read_noise = gain, read_noise
+        self.noise_seed = noise_seed
+        self.subsampling = subsampling
+        self.data = None
+
+    # -- the reference's setup order (core/setup_model_object.cpp:58-153)
+    def _define(self, m):
+        for name, new in self.funcs:
+            m.AddFunction(name, new)
+        m.SetSubsampling(self.subsampling)
+        if self.psf is not None:
+            m.AddPSFVector(self.psf)
+
+    def _regions(self, m):
+        for (psf, scale, x1, x2, y1, y2) in self.osamp:
+            m.AddOversampledPsf(psf, scale, x1, x2, y1, y2)
+
+    def model_image(self, params=None):
+        m = ModelObjectGPU(0, PRECISION_FP64)
+        self._define(m)
+        m.SetupModelImage(self.width, self.height)
+        self._regions(m)
+        m.CreateModelImage(self.truth if params is None else params)
+        return m.GetModelImageVector()
+
+    def make_data(self):
+        if self.data is None:
+            T = self.model_image()
+            rng = np.random.default_rng(self.noise_seed)
+            e = rng.poisson(np.clip(self.gain * T, 0, None)).astype(np.float64)
+            if self.read_noise > 0:
+                e += rng.normal(0.0, self.read_noise, size=e.shape)
+            self.data = e / self.gain
+        return self.data
+
+    def fit_model(self, precision=PRECISION_FP64):
+        """A GPU ModelObject set up for fitting this workload's data."""
+        return self.setup_fit(ModelObjectGPU(0, precision))
+
+    def setup_fit(self, m):
+        """Run the fitting setup sequence on any ModelObject-like `m`: data-based chi^2
+        weights from GAIN and READNOISE (ORIGINAL_SKY 0, EXPTIME 1, NCOMBINED 1)."""
+        data = self.make_data()
+        self._define(m)
+        m.AddImageDataVector(data)
+        m.AddImageCharacteristics(self.gain, self.read_noise, 1.0, 1, 0.0)
+        self._regions(m)
+        m.FinalSetupForFitting()
+        return m
+
+    def start_params(self, seed=1, frac=0.2):
+        rng = np.random.default_rng(seed)
+        return self.truth * rng.uniform(1 - frac, 1 + frac, size=self.truth.size)
+
+
+def moffat_psf(size, fwhm, beta):
+    m = ModelObjectGPU(0, PRECISION_FP64)
+    m.AddFunction("Moffat", True)
+    m.SetupModelImage(size, size)
+    c = (size + 1) / 2.0
+    m.CreateModelImage(np.array([c, c, 0.0, 0.0, 1.0, fwhm, beta]))
+    p = m.GetModelImageVector()
+    return p / p.sum()
+
+
+def cfg1():
+    """256^2 Sersic + Exponential + FlatSky (sky 100), gain 4.5, read noise 0.5, no PSF."""
+    truth = [129.5, 128.2, 20.0, 0.25, 2.5, 15.0, 20.0, 20.0, 0.4, 100.0, 40.0, 100.0]
+    return Workload("cfg1_256_sersic_exp_sky", [("Sersic", True), ("Exponential", False), ("FlatSky", False)],
+                    truth, 256, 256, gain=4.5, read_noise=0.5, noise_seed=0)
+
+
+def cfg2():
+    """1024^2 Sersic(n=4, r_e=60, ell=0.2) + Exponential(h=120, ell=0.5) + FlatSky,
+    51^2 Moffat PSF (FWHM 4, beta 3), data pixels 493-532 oversampled 5x with a
+    255^2 Moffat (FWHM 20 sub-pixels, beta 3), gain 4, noise seed 2."""
+    truth = [512.5, 512.5, 20.0, 0.2, 4.0, 15.0, 60.0, 20.0, 0.5, 100.0, 120.0, 100.0]
+    psf = moffat_psf(51, 4.0, 3.0)
+    psf_os = moffat_psf(255, 20.0, 3.0)
+    return Workload("cfg2_1024_sersic_exp_sky_psf51_osamp40x5", [("Sersic", True), ("Exponential", False),
+                                                                 ("FlatSky", False)],
+                    truth, 1024, 1024, psf=psf, osamp=[(psf_os, 5, 493, 532, 493, 532)], gain=4.0, noise_seed=2)
+
+
+def cfg3_population(n=4096, seed=3):
+    """512^2 Sersic + Exponential + FlatSky with a 35^2 Moffat PSF (FWHM 3, beta 2.5),
+    and n parameter vectors uniform within +-30% of cfg1's truth."""
+    base = cfg1()
+    psf = moffat_psf(35, 3.0, 2.5)
+    w = Workload("cfg3_512_population", base.funcs, base.truth, 512, 512, psf=psf, gain=4.5, read_noise=0.5,
+                 noise_seed=3)
+    rng = np.random.default_rng(seed)
+    pop = base.truth * rng.uniform(0.7, 1.3, size=(n, base.truth.size))
+    return w, pop

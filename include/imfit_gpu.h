@@ -1,0 +1,136 @@
+/* imfit_gpu.h -- C ABI of the MI355X (gfx950) model-image library.
+ *
+ * One `ifg_model` plays the role of one reference ModelObject
+ * (core/model_object.h:38-347): the setup calls mirror ModelObject's setup
+ * methods in the same order (core/setup_model_object.cpp:58-153: PSF, data,
+ * oversampled regions, mask, statistic, FinalSetupForFitting), and the
+ * evaluation calls replace its hot entry points (CreateModelImage,
+ * ComputeDeviates, GetFitStatistic).  Plain pointers and sizes only; every call
+ * returns an int status, negative on error (message via ifg_last_error), like
+ * the reference's int-status setup methods (e.g. core/model_object.cpp:517-523,
+ * 839-856).  Caller buffers are copied, never modified.
+ *
+ * Pixel layout everywhere: row-major, x (column) fastest, as the reference's
+ * pixel vectors.  Parameter vectors use the reference layout: X0,Y0 at the start
+ * of each function set, then each component's parameters in order
+ * (core/model_object.cpp:1126-1135).
+ *
+ * Threading: one host thread per ifg_model at a time (the reference ModelObject
+ * is not re-entrant either).  All device work of a handle runs on its own HIP
+ * stream.
+ */
+#ifndef IMFIT_GPU_H
+#define IMFIT_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ifg_model ifg_model;
+
+/* status codes */
+#define IFG_OK 0
+#define IFG_ERR_ARG (-1)         /* bad argument (size mismatch, non-finite PSF, ...) */
+#define IFG_ERR_STATE (-2)       /* call out of order (e.g. PSF after SetupModelImage) */
+#define IFG_ERR_DATA (-3)        /* FinalSetupForFitting found bad data / weights */
+#define IFG_ERR_HIP (-4)         /* HIP runtime error (message has the HIP error string) */
+#define IFG_ERR_UNSUPPORTED (-5) /* configuration outside what the library implements */
+
+/* input-type codes: same values as core/definitions.h:75-82 */
+#define IFG_WEIGHTS_ARE_SIGMAS 100
+#define IFG_WEIGHTS_ARE_VARIANCES 110
+#define IFG_WEIGHTS_ARE_WEIGHTS 120
+#define IFG_MASK_ZERO_IS_GOOD 10
+#define IFG_MASK_ZERO_IS_BAD 20
+
+/* fit statistics: same values as core/definitions.h:54-59 */
+#define IFG_FITSTAT_CHISQUARE 1
+#define IFG_FITSTAT_CASH 2
+#define IFG_FITSTAT_POISSON_MLR 3
+
+/* pixel arithmetic: FP64 (parity mode) or FP32 pixels + FFT with FP64 reductions */
+#define IFG_PRECISION_FP64 0
+#define IFG_PRECISION_FP32 1
+
+/* ---- lifetime ------------------------------------------------------------- */
+/* ModelObject::ModelObject (core/model_object.cpp:96-140); device = HIP ordinal */
+ifg_model* ifg_create(int device);
+/* ModelObject::~ModelObject (core/model_object.cpp:180-232) */
+void ifg_destroy(ifg_model* m);
+/* last error message of this handle ("" if none) */
+const char* ifg_last_error(const ifg_model* m);
+/* pixel arithmetic (IFG_PRECISION_*); must precede the first evaluation */
+int ifg_set_precision(ifg_model* m, int precision);
+
+/* ---- model definition ----------------------------------------------------- */
+/* AddFunction + DefineFunctionSets (core/model_object.cpp:277-340; name -> class
+ * as in core/add_functions.cpp:163-385).  Supported names: "Sersic",
+ * "Exponential", "FlatSky", "Gaussian", "Moffat", "ExponentialDisk3D". */
+int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set);
+/* FunctionObject::SetSubsampling for every component (imfit --no-subsampling) */
+int ifg_set_subsampling(ifg_model* m, int do_subsampling);
+/* GetNParams (core/model_object.cpp:2398) */
+int ifg_get_nparams(const ifg_model* m);
+
+/* ---- image setup ---------------------------------------------------------- */
+/* AddPSFVector (core/model_object.cpp:827-876): must precede image setup */
+int ifg_add_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int normalize);
+/* SetupModelImage (core/model_object.cpp:488-527): make-image mode, no data */
+int ifg_setup_model_image(ifg_model* m, int n_columns, int n_rows);
+/* AddImageDataVector (core/model_object.cpp:462-477) */
+int ifg_add_image_data(ifg_model* m, const double* data, int n_columns, int n_rows);
+/* AddImageCharacteristics (core/model_object.cpp:532-547) */
+int ifg_add_image_characteristics(ifg_model* m, double gain, double read_noise, double exp_time, int n_combined,
+                                  double original_sky);
+/* AddErrorVector (core/model_object.cpp:573-619) */
+int ifg_add_error_vector(ifg_model* m, const double* errors, long n_values, int input_type);
+/* AddMaskVector (core/model_object.cpp:721-776) */
+int ifg_add_mask_vector(ifg_model* m, const double* mask, long n_values, int input_type);
+/* AddOversampledPsfInfo (core/model_object.cpp:881-955) for one region
+ * x1:x2, y1:y2 (1-based, inclusive, data-image coordinates) */
+int ifg_add_oversampled_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int scale, int x1, int x2,
+                            int y1, int y2, int normalize);
+/* UseCashStatistic / UsePoissonMLR / UseModelErrors (core/model_object.cpp:1500-1598) */
+int ifg_use_cash_statistic(ifg_model* m);
+int ifg_use_poisson_mlr(ifg_model* m);
+int ifg_use_model_errors(ifg_model* m);
+/* FinalSetupForFitting (core/model_object.cpp:967-1079); returns 0, or
+ * IFG_ERR_DATA for bad weights (-1 in the reference), non-finite unmasked data
+ * (-2) or no valid pixels (-3) */
+int ifg_final_setup(ifg_model* m);
+/* WhichFitStatistic (core/model_object.cpp:1601-1617), non-verbose form */
+int ifg_which_fit_statistic(const ifg_model* m);
+/* GetNDataValues / GetNValidPixels (core/model_object.cpp:2405-2420) */
+long ifg_get_ndata(const ifg_model* m);
+long ifg_get_nvalid(const ifg_model* m);
+
+/* ---- hot path ------------------------------------------------------------- */
+/* CreateModelImage (core/model_object.cpp:1103-1229) */
+int ifg_create_model_image(ifg_model* m, const double* params);
+/* GetModelImageVector (core/model_object.cpp:1310-1335): data-size image of the
+ * last CreateModelImage / ComputeDeviates / GetFitStatistic call */
+int ifg_get_model_image(ifg_model* m, double* out);
+/* ComputeDeviates (core/model_object.cpp:1413-1496): y[nData] */
+int ifg_compute_deviates(ifg_model* m, const double* params, double* y);
+/* GetFitStatistic (core/model_object.cpp:1620-1626): chi^2, Cash or Poisson-MLR */
+int ifg_fit_statistic(ifg_model* m, const double* params, double* out);
+/* Population form of GetFitStatistic for the callers that evaluate many
+ * parameter vectors per step: DE candidates (solvers/DESolver.cpp:222-252),
+ * DREAM chains (cdream/dream.cpp:268-300), LM finite-difference columns
+ * (solvers/mpfit.cpp:1241-1343).  params: n_vectors x nParams, row-major. */
+int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, double* out);
+
+/* ---- instrumentation (no counterpart in the reference, whose only timing is
+ * wall-clock, core/makeimage_main.cpp:351-363) -------------------------------- */
+/* Record HIP events around every kernel launch of this handle (on its stream);
+ * switching on clears the totals. */
+int ifg_set_kernel_timing(ifg_model* m, int on);
+/* Per-kernel totals: fills name / total milliseconds / launch count of entry
+ * `index` and returns the number of entries (0 when index is out of range). */
+int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_len, double* total_ms,
+                          long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMFIT_GPU_H */
