@@ -54,9 +54,32 @@ _SIGS = {
     "ifg_compute_deviates": ([_vp, _dp, _dp], C.c_int),
     "ifg_fit_statistic": ([_vp, _dp, _dp], C.c_int),
     "ifg_fit_statistic_batch": ([_vp, _dp, C.c_int, _dp], C.c_int),
+    "ifg_jacobian": ([_vp, _dp, C.c_int, C.POINTER(C.c_int), _dp, _dp], C.c_int),
+    "ifg_normal_equations": ([_vp, _dp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp], C.c_int),
+    "ifg_fit_statistic_gradient": ([_vp, _dp, _dp, _dp], C.c_int),
     "ifg_set_kernel_timing": ([_vp, C.c_int], C.c_int),
+    "ifg_set_graph_mode": ([_vp, C.c_int], C.c_int),
     "ifg_get_kernel_timing": ([_vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_long)], C.c_int),
 }
+
+class ParamInfo(C.Structure):
+    """ifg_param_info: mpfit's fixed / limited / limits per parameter."""
+    _fields_ = [("fixed", C.c_int), ("limited", C.c_int * 2), ("limits", C.c_double * 2)]
+
+
+class LMConfig(C.Structure):
+    _fields_ = [("ftol", C.c_double), ("xtol", C.c_double), ("gtol", C.c_double), ("maxiter", C.c_int),
+                ("maxfev", C.c_int)]
+
+
+class LMResult(C.Structure):
+    _fields_ = [("bestnorm", C.c_double), ("orignorm", C.c_double), ("status", C.c_int), ("niter", C.c_int),
+                ("nfev", C.c_int), ("nfree", C.c_int), ("npegged", C.c_int)]
+
+
+_SIGS["ifg_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.c_int, C.POINTER(LMResult)],
+                       C.c_int)
+_SIGS["ifg_fit_lm_normal"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
 
 _LIB = C.CDLL(LIB_PATH)
 for _name, (_args, _res) in _SIGS.items():
@@ -197,9 +220,83 @@ class ModelObjectGPU:
         self._chk(_LIB.ifg_fit_statistic_batch(self.h, _ptr(p), p.shape[0], _ptr(out)), "GetFitStatisticBatch")
         return out
 
+    # ---- derivatives
+    def _ifree(self, ifree):
+        if ifree is None:
+            ifree = np.arange(self.GetNParams())
+        return np.ascontiguousarray(ifree, dtype=np.int32)
+
+    def Jacobian(self, params, ifree=None):
+        """(deviates, J) with J[:, j] = d(deviate)/d(param ifree[j]) (nData x nFree)."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        fr = self._ifree(ifree)
+        n = self.nrows * self.ncols
+        fvec = np.empty(n)
+        fjac = np.empty((fr.size, n))
+        self._chk(_LIB.ifg_jacobian(self.h, _ptr(p), fr.size, fr.ctypes.data_as(C.POINTER(C.c_int)), _ptr(fvec),
+                                    _ptr(fjac)), "Jacobian")
+        return fvec, fjac.T
+
+    def NormalEquations(self, params, ifree=None):
+        """(J^T J, J^T r, statistic) reduced on the GPU."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        fr = self._ifree(ifree)
+        jtj = np.empty((fr.size, fr.size))
+        jtr = np.empty(fr.size)
+        stat = np.zeros(1)
+        self._chk(_LIB.ifg_normal_equations(self.h, _ptr(p), fr.size, fr.ctypes.data_as(C.POINTER(C.c_int)),
+                                            _ptr(jtj), _ptr(jtr), _ptr(stat)), "NormalEquations")
+        return jtj, jtr, float(stat[0])
+
+    def GetFitStatisticGradient(self, params):
+        """(gradient of GetFitStatistic w.r.t. every parameter, statistic)."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        g = np.empty(self.GetNParams())
+        stat = np.zeros(1)
+        self._chk(_LIB.ifg_fit_statistic_gradient(self.h, _ptr(p), _ptr(g), _ptr(stat)), "GetFitStatisticGradient")
+        return g, float(stat[0])
+
+    # ---- Levenberg-Marquardt fits
+    @staticmethod
+    def _pinfo(limits, fixed, n):
+        if limits is None and fixed is None:
+            return None
+        arr = (ParamInfo * n)()
+        for i in range(n):
+            if fixed is not None and fixed[i]:
+                arr[i].fixed = 1
+            if limits is not None and limits[i] is not None:
+                lo, hi = limits[i]
+                if lo is not None:
+                    arr[i].limited[0], arr[i].limits[0] = 1, lo
+                if hi is not None:
+                    arr[i].limited[1], arr[i].limits[1] = 1, hi
+        return arr
+
+    def FitLM(self, params, limits=None, fixed=None, ftol=1e-8, xtol=0.0, maxiter=1000, jacobian="fd",
+              method="mpfit"):
+        """Levenberg-Marquardt fit.  method "mpfit": MINPACK LMDIF as imfit runs it, with
+        jacobian "fd" (one GPU evaluation per free parameter) or "analytic"; method
+        "normal": LM on GPU-reduced normal equations.  Returns (params, LMResult)."""
+        x = np.array(params, dtype=np.float64)
+        pinfo = self._pinfo(limits, fixed, x.size)
+        cfg = LMConfig(ftol, xtol, 0.0, maxiter, 0)
+        res = LMResult()
+        if method == "normal":
+            st = _LIB.ifg_fit_lm_normal(self.h, _ptr(x), pinfo, C.byref(cfg), C.byref(res))
+        else:
+            mode = 1 if jacobian == "analytic" else 0
+            st = _LIB.ifg_fit_lm(self.h, _ptr(x), pinfo, C.byref(cfg), mode, C.byref(res))
+        if st < 0 and st > -16:
+            self._chk(st, "FitLM")
+        return x, res
+
     # ---- instrumentation
     def SetKernelTiming(self, on):
         return self._chk(_LIB.ifg_set_kernel_timing(self.h, int(on)), "SetKernelTiming")
+
+    def SetGraphMode(self, on):
+        return self._chk(_LIB.ifg_set_graph_mode(self.h, int(on)), "SetGraphMode")
 
     def GetKernelTiming(self):
         """{kernel name: (total ms, launches)} recorded with HIP events on the handle's stream."""

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -37,6 +38,15 @@ double Calculate_bn(double n) {
     return 2 * n - 0.333333333333333 + 0.009876543209876543 / n + 0.0018028610621203215 / n2 +
            0.00011409410586365319 / (n2 * n) - 7.1510122958919723e-05 / (n2 * n2);
   return 0.01945 + -0.8902 * n + 10.95 * n2 + -19.67 * n2 * n + 13.43 * n2 * n2;
+}
+
+// d(b_n)/dn of Calculate_bn, for the analytic Sersic derivative
+double Calculate_dbn_dn(double n) {
+  double n2 = n * n;
+  if (n > 0.36)
+    return 2.0 - 0.009876543209876543 / n2 - 2.0 * 0.0018028610621203215 / (n2 * n) -
+           3.0 * 0.00011409410586365319 / (n2 * n2) + 4.0 * 7.1510122958919723e-05 / (n2 * n2 * n);
+  return -0.8902 + 2.0 * 10.95 * n + 3.0 * -19.67 * n2 + 4.0 * 13.43 * n2 * n;
 }
 
 int NParamsOf(int type) {
@@ -87,6 +97,7 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
       const double n = p[2], I_e = p[3], r_e = p[4];
       const double bn = Calculate_bn(n);
       v[5] = I_e; v[6] = r_e; v[7] = bn; v[8] = 1.0 / n;
+      v[9] = n; v[10] = Calculate_dbn_dn(n);
       v[12] = 1.0 / (r_e * r_e); v[13] = 0.5 / n; v[14] = -bn * LOG2E;
       break;
     }
@@ -107,6 +118,7 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
       const double exponent = pow(2.0, 1.0 / beta);
       const double alpha = 0.5 * fwhm / sqrt(exponent - 1.0);
       v[5] = I_0; v[6] = alpha; v[7] = beta;
+      v[9] = fwhm; v[10] = alpha * exponent * log(2.0) / (2.0 * beta * beta * (exponent - 1.0));   // d alpha/d beta
       v[12] = 1.0 / (alpha * alpha); v[13] = -beta;
       break;
     }
@@ -178,6 +190,33 @@ struct Region {
   ConvGrid conv[2];          // [0] single (halves), [1] batch (pairs)
 };
 
+struct Plan {
+  int nVec = 0, flags = 0, which = 0, nGrids = 0, chunkVecs = 1, chunkPairs = 1, nChunks = 1, npc = 1;
+  std::vector<int> freeIdx;   // derivative plans: the parameter of each plane
+  DevBuf normPart_d;
+  bool single = true;
+  std::vector<GridGeom> geoms;
+  std::vector<int> strideMax;
+  std::vector<long> tailOff;
+  size_t recBytes = 0, boxBytes = 0, stageBytes = 0;
+  void* pinnedStage = nullptr;
+  void* pinnedOut = nullptr;
+  DevBuf stage_d, tail_d, pairs_d, slots_d, spec_d, partials_d, out_d, dev_d, model_d, osub_d, ovals_d;
+  int nSlots = 1, pixPerBlock = 1024, ovStride = 0;
+  long osubStride = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Plan() = default;
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+  ~Plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (pinnedStage) (void)hipHostFree(pinnedStage);
+    if (pinnedOut) (void)hipHostFree(pinnedOut);
+  }
+};
+
 // Kahan-normalised PSF (core/convolver.cpp:296-311)
 std::vector<double> NormalizedPSF(const double* psf, long n, bool normalize) {
   std::vector<double> out(psf, psf + n);
@@ -223,15 +262,16 @@ struct ifg_model {
   std::map<int, std::unique_ptr<FFTLen>> ffts;
   ConvGrid mainConv[2];
   DevBuf data_d, weight_d, extra_d;
-  DevBuf recs_d, boxes_d, tail_d, pairs_d, slots_d, spec_d, partials_d, out_d, dev_d, model_d, osub_d, ovals_d;
-  std::vector<unsigned char> stage;   // host staging (pinned)
-  void* pinned = nullptr;
-  size_t pinnedBytes = 0;
+  DevBuf devkeep_d, dfac_d;   // forward-pass outputs consumed by derivative passes
+  std::vector<std::unique_ptr<Plan>> plans;
+  bool useGraphs = true;
+  std::vector<double> modelHost;   // last CreateModelImage result
   std::vector<double> lastParams;
   bool modelValid = false;
   // optional per-kernel timing with HIP events on this handle's stream
   bool timing = false;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>>* curEvents = nullptr;
   std::vector<hipEvent_t> eventPool;
   std::map<std::string, std::pair<double, long>> kernelTimes;   // name -> (total ms, launches)
 
@@ -241,7 +281,7 @@ struct ifg_model {
       (void)hipEventDestroy(pe.second.first);
       (void)hipEventDestroy(pe.second.second);
     }
-    if (pinned) (void)hipHostFree(pinned);
+    plans.clear();
     if (stream) (void)hipStreamDestroy(stream);
   }
   int fail(int code, const std::string& msg) {
@@ -307,7 +347,7 @@ void HarvestTimes(ifg_model* m) {
     hipError_t _err = (expr);                                                 \
     if (m->timing) {                                                          \
       (void)hipEventRecord(_e1, m->stream);                                   \
-      m->pendingEvents.push_back({name, {_e0, _e1}});                         \
+      m->curEvents->push_back({name, {_e0, _e1}});                            \
     }                                                                         \
     if (_err != hipSuccess) return m->hipfail(_err, where);                   \
   } while (0)
@@ -467,16 +507,6 @@ int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cL
   return off;
 }
 
-int EnsurePinned(ifg_model* m, size_t bytes) {
-  if (bytes <= m->pinnedBytes) return IFG_OK;
-  if (m->pinned) (void)hipHostFree(m->pinned);
-  m->pinned = nullptr;
-  m->pinnedBytes = 0;
-  HIPCHK(hipHostMalloc(&m->pinned, bytes, hipHostMallocDefault), "pinned staging");
-  m->pinnedBytes = bytes;
-  return IFG_OK;
-}
-
 template <typename T>
 int UploadImageArrays(ifg_model* m) {
   const long n = m->nData;
@@ -528,32 +558,141 @@ int EnsureRegionConv(ifg_model* m, Region& R, int which) {
                    (double)R.y1 + off, frac);
 }
 
-// Evaluate nVec parameter vectors.  outStat[nVec] (device results copied to host),
-// devOut[nVec*nData] and modelOut[nVec*nData] are optional host buffers.
-template <typename T>
-int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, double* devOut, double* modelOut) {
-  int st = PrepareDevice(m);
-  if (st < 0) return st;
-  const int nC = m->nComp(), nP = m->nParamsTot, nR = (int)m->regions.size();
-  const bool single = (nVec == 1);
-  const int which = single ? 0 : 1;
-  if (m->doConv) {
-    if ((st = EnsureMainConv(m, which)) < 0) return st;
-  }
-  for (auto& R : m->regions)
-    if ((st = EnsureRegionConv(m, *R, which)) < 0) return st;
+// ---------------------------------------------------------------- evaluation plans
+// A plan fixes everything about an evaluation that does not depend on the
+// parameter values: grids, FFT geometry, pairing of planes, buffer sizes and the
+// launch shapes.  Per call only the Setup constants, the sub-pixel boxes and their
+// pixel counts change; they are staged in pinned memory and reach the device in one
+// copy.  The whole sequence (copy, kernels, result copy) is captured once as a HIP
+// graph and replayed.
+// PF_DEVKEEP / PF_DFAC1 / PF_DFAC2: forward pass that leaves deviates and
+// d(deviate)/dm (1) or d(statistic)/dm (2) in the model's device buffers for a
+// following derivative pass.  PF_JAC / PF_GRAD: derivative pass (planes are
+// d(model)/d(p_j)) producing Jacobian columns + normal equations, or the gradient.
+enum PlanFlags {
+  PF_STAT = 1, PF_DEV = 2, PF_MODEL = 4, PF_DEVKEEP = 8, PF_DFAC1 = 16, PF_DFAC2 = 32, PF_JAC = 64, PF_GRAD = 128
+};
+inline bool IsDeriv(int flags) { return (flags & (PF_JAC | PF_GRAD)) != 0; }
 
-  // ---- host: Setup constants and sub-pixel boxes for every vector
-  const size_t recBytes = (size_t)nVec * nC * sizeof(CompRec);
-  const int nGrids = 1 + nR;
-  const size_t boxBytes = (size_t)nGrids * nVec * IFG_MAXBOX * sizeof(Box);
-  if ((st = EnsurePinned(m, recBytes + boxBytes + 4096)) < 0) return st;
-  CompRec* recs = reinterpret_cast<CompRec*>(m->pinned);
-  Box* boxes = reinterpret_cast<Box*>(reinterpret_cast<char*>(m->pinned) + recBytes);
-  std::vector<int> tailStride(nGrids, 0);
-  std::vector<GridGeom> geoms(nGrids);
-  geoms[0] = m->doConv ? m->mainConv[which].g : PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
-  for (int k = 0; k < nR; k++) geoms[1 + k] = m->regions[k]->conv[which].g;
+int BoxAreaMax(const GridGeom& g) {
+  const int side = (int)std::ceil(20.0 / g.frac) + 5;
+  return side * side;
+}
+
+template <typename T>
+int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>& minStride) {
+  const int nC = m->nComp(), nR = (int)m->regions.size();
+  int st;
+  P.nVec = nVec;
+  P.flags = flags;
+  P.single = (nVec == 1);
+  P.which = P.single ? 0 : 1;
+  if (m->doConv && (st = EnsureMainConv(m, P.which)) < 0) return st;
+  for (auto& R : m->regions)
+    if ((st = EnsureRegionConv(m, *R, P.which)) < 0) return st;
+  P.nGrids = 1 + nR;
+  P.geoms.assign(P.nGrids, GridGeom());
+  P.geoms[0] = m->doConv ? m->mainConv[P.which].g : PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
+  for (int k = 0; k < nR; k++) P.geoms[1 + k] = m->regions[k]->conv[P.which].g;
+  P.strideMax.assign(P.nGrids, 0);
+  P.tailOff.assign(P.nGrids + 1, 0);
+  for (int gi = 0; gi < P.nGrids; gi++) {
+    int sm = m->doSubsampling ? IFG_MAXBOX * BoxAreaMax(P.geoms[gi]) : 0;
+    if (gi < (int)minStride.size()) sm = std::max(sm, minStride[gi]);
+    P.strideMax[gi] = sm;
+    P.tailOff[gi + 1] = P.tailOff[gi] + (long)sm * nVec;
+  }
+  // staging layout: recs | boxes | tail pixel counts (derivative plans: one vector)
+  const int nRecVec = IsDeriv(flags) ? 1 : nVec;
+  P.recBytes = (size_t)nRecVec * nC * sizeof(CompRec);
+  P.boxBytes = (size_t)P.nGrids * nRecVec * IFG_MAXBOX * sizeof(Box);
+  P.stageBytes = P.recBytes + P.boxBytes + 64 * sizeof(int);
+  HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
+  HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * nVec, hipHostMallocDefault), "pinned results");
+  HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
+  HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
+
+  // chunks of the batch, so that the complex planes stay within ~1 GiB
+  size_t pairBytes = 0;
+  if (m->doConv) pairBytes = (size_t)P.geoms[0].Nr * P.geoms[0].Nc * sizeof(cpx<T>);
+  for (int k = 0; k < nR; k++)
+    pairBytes = std::max(pairBytes, (size_t)P.geoms[1 + k].Nr * P.geoms[1 + k].Nc * sizeof(cpx<T>));
+  const int nPairsAll = P.single ? 1 : (nVec + 1) / 2;
+  P.chunkPairs = nPairsAll;
+  if (pairBytes > 0)
+    P.chunkPairs = (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / pairBytes));
+  P.chunkVecs = P.single ? 1 : 2 * P.chunkPairs;
+  P.nChunks = (nVec + P.chunkVecs - 1) / P.chunkVecs;
+  const long nData = m->nData;
+  HIPCHK(P.spec_d.ensure(std::max<size_t>(1, pairBytes * P.chunkPairs)), "alloc planes");
+  const int nFree = nVec;
+  const int nNormal = nFree * (nFree + 1) / 2 + nFree;
+  HIPCHK(P.out_d.ensure(sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0)), "alloc results");
+  if (flags & PF_JAC) {
+    HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nFree), "alloc Jacobian columns");
+    HIPCHK(P.normPart_d.ensure(sizeof(double) * normal_partials_size(nData, nFree)), "alloc normal partials");
+    if (P.pinnedOut) (void)hipHostFree(P.pinnedOut);
+    P.pinnedOut = nullptr;
+    HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * nNormal, hipHostMallocDefault), "pinned results");
+  }
+  if (flags & (PF_DEVKEEP | PF_DFAC1 | PF_DFAC2)) {
+    HIPCHK(m->devkeep_d.ensure(sizeof(double) * nData), "alloc kept deviates");
+    HIPCHK(m->dfac_d.ensure(sizeof(double) * nData), "alloc dfac");
+  }
+  if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
+  if (flags & PF_MODEL) HIPCHK(P.model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
+  P.osubStride = 0;
+  P.ovStride = 0;
+  for (auto& R : m->regions) {
+    P.osubStride = std::max(P.osubStride, (long)R->nbc * R->scale * R->nbr * R->scale);
+    P.ovStride += R->nbc * R->rowsReplaced;
+  }
+  if (nR) {
+    HIPCHK(P.osub_d.ensure(sizeof(double) * P.osubStride * P.chunkVecs), "alloc region sub-image");
+    HIPCHK(P.ovals_d.ensure(sizeof(double) * P.ovStride * P.chunkVecs), "alloc region values");
+  }
+  P.pixPerBlock = 1024;
+  const GridGeom& gm = P.geoms[0];
+  if (m->doConv) P.nSlots = (gm.Hh + m->mainConv[P.which].rowsK3 - 1) / m->mainConv[P.which].rowsK3;
+  else P.nSlots = (int)(((long)m->W * m->H + P.pixPerBlock - 1) / P.pixPerBlock);
+  const int nPartPairs = m->doConv ? (P.single ? 1 : P.chunkPairs) : P.chunkVecs;
+  HIPCHK(P.partials_d.ensure(sizeof(double) * 5 * 2 * (size_t)P.nSlots * nPartPairs), "alloc partials");
+
+  // static pair lists (per chunk, per grid) and vector -> partial-slot maps
+  const int npc = P.single ? 1 : P.chunkPairs;
+  std::vector<int4> pairs((size_t)P.nChunks * P.nGrids * npc, make_int4(0, 0, -1, 0));
+  std::vector<int2> slots((size_t)P.nChunks * P.chunkVecs, make_int2(0, 0));
+  for (int c = 0; c < P.nChunks; c++) {
+    const int v0 = c * P.chunkVecs, nv = std::min(P.chunkVecs, nVec - v0);
+    for (int gi = 0; gi < P.nGrids; gi++) {
+      int4* pg = pairs.data() + ((size_t)c * P.nGrids + gi) * npc;
+      if (P.single) pg[0] = make_int4(0, 0, 0, (P.geoms[gi].H + 1) / 2);
+      else
+        for (int p = 0; p < (nv + 1) / 2; p++) pg[p] = make_int4(2 * p, 0, (2 * p + 1 < nv) ? 2 * p + 1 : -1, 0);
+    }
+    int2* sl = slots.data() + (size_t)c * P.chunkVecs;
+    for (int v = 0; v < nv; v++) {
+      if (!m->doConv) sl[v] = make_int2(v, 1);
+      else if (P.single) sl[v] = make_int2(0, 3);
+      else sl[v] = make_int2(v / 2, 1 << (v & 1));
+    }
+  }
+  HIPCHK(P.pairs_d.ensure(sizeof(int4) * pairs.size()), "alloc pairs");
+  HIPCHK(P.slots_d.ensure(sizeof(int2) * slots.size()), "alloc slots");
+  HIPCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
+  HIPCHK(hipMemcpy(P.slots_d.p, slots.data(), sizeof(int2) * slots.size(), hipMemcpyHostToDevice), "slots");
+  P.npc = npc;
+  return IFG_OK;
+}
+
+// Host part of a call: Setup constants, boxes and their pixel counts into the
+// pinned staging area.  Returns 1 if a box layout needs a larger plan.
+int StageCall(ifg_model* m, Plan& P, const double* params, std::vector<int>& need) {
+  const int nC = m->nComp(), nP = m->nParamsTot, nVec = IsDeriv(P.flags) ? 1 : P.nVec;
+  CompRec* recs = reinterpret_cast<CompRec*>(P.pinnedStage);
+  Box* boxes = reinterpret_cast<Box*>(reinterpret_cast<char*>(P.pinnedStage) + P.recBytes);
+  int* used = reinterpret_cast<int*>(reinterpret_cast<char*>(P.pinnedStage) + P.recBytes + P.boxBytes);
+  need.assign(P.nGrids, 0);
   for (int v = 0; v < nVec; v++) {
     const double* p = params + (long)v * nP;
     int offset = 0;
@@ -567,108 +706,88 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       SetupRec(m->types[n], p + offset, x0, y0, recs[(long)v * nC + n]);
       offset += NParamsOf(m->types[n]);
     }
-    for (int gi = 0; gi < nGrids; gi++) {
-      const GridGeom& g = geoms[gi];
+    for (int gi = 0; gi < P.nGrids; gi++) {
+      const GridGeom& g = P.geoms[gi];
       // evaluated extent: plane columns [-hl, Lc - hl), rows [-hb, H + Pr) (halves may run past H)
       const bool conv = g.Lc > 0;
       const int cLo = -g.hl, cHi = conv ? g.Lc - g.hl - 1 : g.W - 1;
       const int rLo = -g.hb, rHi = conv ? g.H + (g.Lr - g.Hh) : g.H - 1;
       const int area = MakeBoxes(m, recs + (long)v * nC, g, cLo, cHi, rLo, rHi,
                                  boxes + ((long)gi * nVec + v) * IFG_MAXBOX);
-      tailStride[gi] = std::max(tailStride[gi], area);
+      need[gi] = std::max(need[gi], area);
     }
   }
-  std::vector<long> tailOff(nGrids + 1, 0);
-  for (int gi = 0; gi < nGrids; gi++) tailOff[gi + 1] = tailOff[gi] + (long)tailStride[gi] * nVec;
+  bool grow = false;
+  for (int gi = 0; gi < P.nGrids; gi++) {
+    used[gi] = need[gi];
+    grow = grow || need[gi] > P.strideMax[gi];
+  }
+  return grow ? 1 : 0;
+}
 
-  HIPCHK(m->recs_d.ensure(recBytes), "alloc records");
-  HIPCHK(m->boxes_d.ensure(boxBytes), "alloc boxes");
-  HIPCHK(m->tail_d.ensure(std::max<long>(1, tailOff[nGrids]) * sizeof(double)), "alloc tail");
-  HIPCHK(hipMemcpyAsync(m->recs_d.p, recs, recBytes, hipMemcpyHostToDevice, m->stream), "upload records");
-  HIPCHK(hipMemcpyAsync(m->boxes_d.p, boxes, boxBytes, hipMemcpyHostToDevice, m->stream), "upload boxes");
-
+// Enqueue the whole evaluation on the handle's stream (captured into the graph).
+template <typename T>
+int Enqueue(ifg_model* m, Plan& P) {
+  const int nC = m->nComp(), nR = (int)m->regions.size(), nVec = P.nVec;
+  const long nData = m->nData;
+  hipStream_t s = m->stream;
+  HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
+  const CompRec* recs_d = reinterpret_cast<const CompRec*>(P.stage_d.p);
+  const Box* boxes_d = reinterpret_cast<const Box*>(P.stage_d.as<char>() + P.recBytes);
+  const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.recBytes + P.boxBytes);
+  const bool deriv = IsDeriv(P.flags);
   auto evalArgs = [&](int gi, int v0) {
     EvalArgs ea;
     std::memset(&ea, 0, sizeof(ea));
     ea.nComp = nC;
     for (int n = 0; n < nC; n++) ea.types[n] = m->types[n];
-    ea.recs = m->recs_d.as<CompRec>() + (long)v0 * nC;
-    ea.boxes = m->boxes_d.as<Box>() + ((long)gi * nVec + v0) * IFG_MAXBOX;
-    ea.tail = m->tail_d.as<double>() + tailOff[gi] + (long)v0 * tailStride[gi];
-    ea.tailStride = tailStride[gi];
+    ea.tail = P.tail_d.as<double>() + P.tailOff[gi] + (long)v0 * P.strideMax[gi];
+    ea.tailStride = P.strideMax[gi];
+    ea.tailUsed = used_d + gi;
+    if (deriv) {
+      ea.recs = recs_d;
+      ea.recStride = 0;
+      ea.boxes = boxes_d + (long)gi * IFG_MAXBOX;
+      ea.boxStride = 0;
+      ea.deriv = 1;
+      ea.nFree = std::min(P.chunkVecs, nVec - v0);
+      for (int k = 0; k < ea.nFree; k++) ea.freeIdx[k] = P.freeIdx[v0 + k];
+      int off = 0, xy = 0;
+      for (int n = 0; n < nC; n++) {
+        if (m->fsetStart[n]) {
+          xy = off;
+          off += 2;
+        }
+        ea.compXY[n] = xy;
+        ea.compFirst[n] = off;
+        ea.compNP[n] = NParamsOf(m->types[n]);
+        off += ea.compNP[n];
+      }
+    } else {
+      ea.recs = recs_d + (long)v0 * nC;
+      ea.recStride = nC;
+      ea.boxes = boxes_d + ((long)gi * nVec + v0) * IFG_MAXBOX;
+      ea.boxStride = IFG_MAXBOX;
+    }
     return ea;
   };
-
-  // ---- chunking of the batch so that the complex planes stay within budget
-  size_t pairBytes = 0;
-  if (m->doConv) pairBytes = (size_t)geoms[0].Nr * geoms[0].Nc * sizeof(cpx<T>);
-  for (int k = 0; k < nR; k++)
-    pairBytes = std::max(pairBytes, (size_t)geoms[1 + k].Nr * geoms[1 + k].Nc * sizeof(cpx<T>));
-  const int nPairsAll = single ? 1 : (nVec + 1) / 2;
-  int chunkPairs = nPairsAll;
-  if (pairBytes > 0) chunkPairs = (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / pairBytes));
-  const int chunkVecs = single ? 1 : 2 * chunkPairs;
-
-  // buffers
-  const long nData = m->nData;
-  HIPCHK(m->spec_d.ensure(std::max<size_t>(1, pairBytes * chunkPairs)), "alloc planes");
-  HIPCHK(m->out_d.ensure(sizeof(double) * nVec), "alloc results");
-  if (devOut) HIPCHK(m->dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
-  if (modelOut) HIPCHK(m->model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
-  long osubStride = 0;
-  int ovStride = 0;
-  for (auto& R : m->regions) {
-    osubStride = std::max(osubStride, (long)R->nbc * R->scale * R->nbr * R->scale);
-    ovStride += R->nbc * R->rowsReplaced;
-  }
-  if (nR) {
-    HIPCHK(m->osub_d.ensure(sizeof(double) * osubStride * chunkVecs), "alloc region sub-image");
-    HIPCHK(m->ovals_d.ensure(sizeof(double) * ovStride * chunkVecs), "alloc region values");
-  }
-  // slots per plane of the main statistic pass
-  int nSlots = 1, pixPerBlock = 1024;
-  const GridGeom& gm = geoms[0];
-  if (m->doConv) nSlots = (gm.Hh + m->mainConv[which].rowsK3 - 1) / m->mainConv[which].rowsK3;
-  else nSlots = (int)(((long)m->W * m->H + pixPerBlock - 1) / pixPerBlock);
-  const int nPartPairs = m->doConv ? (single ? 1 : chunkPairs) : chunkVecs;
-  HIPCHK(m->partials_d.ensure(sizeof(double) * 5 * 2 * (size_t)nSlots * nPartPairs), "alloc partials");
-  HIPCHK(m->pairs_d.ensure(sizeof(int4) * std::max(chunkPairs, 1) * nGrids), "alloc pairs");
-  HIPCHK(m->slots_d.ensure(sizeof(int2) * chunkVecs), "alloc slots");
-
-  const int mode = m->statMode();
+  const int mode = (P.flags & PF_JAC) ? S_JAC : (P.flags & PF_GRAD) ? S_GRAD : m->statMode();
   const double agiant = 1.304e19 / (double)std::max<long>(nData, 1);
   const int subsampling = m->doSubsampling ? 1 : 0;
-  hipStream_t s = m->stream;
+  const int which = P.which;
 
-  // pairs and vector->slot maps; they outlive the asynchronous copies (synchronised
-  // before the next chunk overwrites them and at the end of the call)
-  std::vector<int4> pairs;
-  std::vector<int2> slots;
-  for (int v0 = 0; v0 < nVec; v0 += chunkVecs) {
-    const int nv = std::min(chunkVecs, nVec - v0);
-    const int np = single ? 1 : (nv + 1) / 2;
-    // one pair list per grid: a single image is split into its top and bottom
-    // halves, whose row offset depends on the grid's own height
-    pairs.assign((size_t)np * nGrids, make_int4(0, 0, -1, 0));
-    slots.assign(nv, make_int2(0, 0));
-    for (int gi = 0; gi < nGrids; gi++) {
-      int4* pg = pairs.data() + (size_t)gi * np;
-      if (single) pg[0] = make_int4(0, 0, 0, (geoms[gi].H + 1) / 2);
-      else
-        for (int p = 0; p < np; p++) pg[p] = make_int4(2 * p, 0, (2 * p + 1 < nv) ? 2 * p + 1 : -1, 0);
-    }
-    if (single) slots[0] = make_int2(0, 3);
-    else
-      for (int v = 0; v < nv; v++) slots[v] = make_int2(v / 2, 1 << (v & 1));
-    if (!m->doConv)
-      for (int v = 0; v < nv; v++) slots[v] = make_int2(v, 1);
-    HIPCHK(hipMemcpyAsync(m->pairs_d.p, pairs.data(), sizeof(int4) * np * nGrids, hipMemcpyHostToDevice, s),
-           "pairs");
-    HIPCHK(hipMemcpyAsync(m->slots_d.p, slots.data(), sizeof(int2) * nv, hipMemcpyHostToDevice, s), "slots");
+  for (int c = 0; c < P.nChunks; c++) {
+    const int v0 = c * P.chunkVecs;
+    const int nv = std::min(P.chunkVecs, nVec - v0);
+    const int np = P.single ? 1 : (nv + 1) / 2;
+    const int4* pairsC = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
+    const int2* slotsC = P.slots_d.as<int2>() + (size_t)c * P.chunkVecs;
 
     // 1. sub-pixel tails of every grid
-    for (int gi = 0; gi < nGrids; gi++)
-      if (tailStride[gi] > 0) TIMED(gi == 0 ? "k_tail" : "k_tail_region", launch_tail<T>(s, evalArgs(gi, v0), geoms[gi], nv, subsampling), "k_tail");
+    for (int gi = 0; gi < P.nGrids; gi++)
+      if (P.strideMax[gi] > 0)
+        TIMED(gi == 0 ? "k_tail" : "k_tail_region", launch_tail<T>(s, evalArgs(gi, v0), P.geoms[gi], nv, subsampling),
+              "k_tail");
 
     // 2. oversampled regions: evaluate, convolve with their PSF, block-average
     StatArgs<T> sa;
@@ -682,12 +801,12 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       const cpx<T>* tw_r = (sizeof(T) == 8) ? (const cpx<T>*)cg.fr->tw64.p : (const cpx<T>*)cg.fr->tw32.p;
       const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
       const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
-      const int4* pairsR = m->pairs_d.as<int4>() + (size_t)(1 + k) * np;
+      const int4* pairsR = pairsC + (size_t)(1 + k) * P.npc;
       TIMED("k_eval_rowfft_region",
-            launch_eval_rowfft<T>(s, evalArgs(1 + k, v0), g, pairsR, np, m->spec_d.as<cpx<T>>(), pstride,
-                                  cg.fc->plan, tw_c, cg.rowsK1),
+            launch_eval_rowfft<T>(s, evalArgs(1 + k, v0), g, pairsR, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan,
+                                  tw_c, cg.rowsK1),
             "region k_eval_rowfft");
-      TIMED("k_colconv_region", launch_colconv<T>(s, g, m->spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
+      TIMED("k_colconv_region", launch_colconv<T>(s, g, P.spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
             "region k_colconv");
       StatArgs<T> so;
       std::memset(&so, 0, sizeof(so));
@@ -695,13 +814,13 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       so.W = g.W;
       so.H = g.H;
       so.nData = (long)g.W * g.H;
-      so.modelOut = m->osub_d.as<double>();
+      so.modelOut = P.osub_d.as<double>();
       TIMED("k_rowifft_region",
-            launch_rowifft_stat<T>(s, g, pairsR, np, m->spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, so,
-                                   cg.rowsK3, agiant),
+            launch_rowifft_stat<T>(s, g, pairsR, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, so, cg.rowsK3,
+                                   agiant),
             "region k_rowifft");
       TIMED("k_downsample",
-            launch_downsample(s, m->osub_d.as<double>(), g.W, (long)g.W * g.H, m->ovals_d.as<double>(), ovStride,
+            launch_downsample(s, P.osub_d.as<double>(), g.W, (long)g.W * g.H, P.ovals_d.as<double>(), P.ovStride,
                               R.ovOff, R.nbc, R.rowsReplaced, R.scale, nv),
             "k_downsample");
       RegionOut ro;
@@ -714,8 +833,8 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       sa.regs[k] = ro;
     }
     sa.nReg = nR;
-    sa.ovals = m->ovals_d.as<double>();
-    sa.ovStride = ovStride;
+    sa.ovals = P.ovals_d.as<double>();
+    sa.ovStride = P.ovStride;
 
     // 3. main grid + statistic
     sa.mode = mode;
@@ -729,10 +848,16 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
     sa.effGain = m->effGain;
     sa.sky = m->originalSky;
     sa.rnTerm = m->nCombined * m->rnAdu2;
-    sa.devOut = devOut ? m->dev_d.as<double>() + (long)v0 * nData : nullptr;
-    sa.modelOut = modelOut ? m->model_d.as<double>() + (long)v0 * nData : nullptr;
-    sa.partials = m->partials_d.as<double>();
-    sa.nSlots = nSlots;
+    sa.devOut = (P.flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
+    if (P.flags & PF_DEVKEEP) sa.devOut = m->devkeep_d.as<double>();
+    sa.modelOut = (P.flags & PF_MODEL) ? P.model_d.as<double>() + (long)v0 * nData : nullptr;
+    if (P.flags & (PF_DFAC1 | PF_DFAC2)) {
+      sa.dfacOut = m->dfac_d.as<double>();
+      sa.dfacKind = (P.flags & PF_DFAC2) ? 2 : 1;
+    }
+    if (deriv) sa.dfac = m->dfac_d.as<double>();
+    sa.partials = P.partials_d.as<double>();
+    sa.nSlots = P.nSlots;
     if (m->doConv) {
       ConvGrid& cg = m->mainConv[which];
       const GridGeom& g = cg.g;
@@ -742,37 +867,113 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
       const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
       TIMED("k_eval_rowfft",
-            launch_eval_rowfft<T>(s, evalArgs(0, v0), g, m->pairs_d.as<int4>(), np, m->spec_d.as<cpx<T>>(), pstride,
-                                  cg.fc->plan, tw_c, cg.rowsK1),
+            launch_eval_rowfft<T>(s, evalArgs(0, v0), g, pairsC, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c,
+                                  cg.rowsK1),
             "k_eval_rowfft");
-      TIMED("k_colconv", launch_colconv<T>(s, g, m->spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
+      TIMED("k_colconv", launch_colconv<T>(s, g, P.spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
             "k_colconv");
       TIMED("k_rowifft_stat",
-            launch_rowifft_stat<T>(s, g, m->pairs_d.as<int4>(), np, m->spec_d.as<cpx<T>>(), pstride, cg.fc->plan,
-                                   tw_c, sa, cg.rowsK3, agiant),
+            launch_rowifft_stat<T>(s, g, pairsC, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, sa, cg.rowsK3,
+                                   agiant),
             "k_rowifft_stat");
     } else {
-      TIMED("k_eval_stat", launch_eval_stat<T>(s, evalArgs(0, v0), geoms[0], sa, nv, pixPerBlock, agiant),
+      TIMED("k_eval_stat", launch_eval_stat<T>(s, evalArgs(0, v0), P.geoms[0], sa, nv, P.pixPerBlock, agiant),
             "k_eval_stat");
     }
-    if (mode != S_NONE && outStat)
+    if (mode != S_NONE && mode != S_JAC && (P.flags & PF_STAT))
       TIMED("k_final",
-            launch_final(s, m->partials_d.as<double>(), nSlots, m->slots_d.as<int2>(), nv, mode,
-                         m->out_d.as<double>() + v0),
+            launch_final(s, P.partials_d.as<double>(), P.nSlots, slotsC, nv, mode, P.out_d.as<double>() + v0),
             "k_final");
-    if (v0 + chunkVecs < nVec) {   // pairs/slots staging reuse
-      HIPCHK(hipStreamSynchronize(s), "chunk");
-      if (m->timing) HarvestTimes(m);
-    }
   }
-  if (mode != S_NONE && outStat)
-    HIPCHK(hipMemcpyAsync(outStat, m->out_d.p, sizeof(double) * nVec, hipMemcpyDeviceToHost, s), "results D2H");
-  if (devOut)
-    HIPCHK(hipMemcpyAsync(devOut, m->dev_d.p, sizeof(double) * nData * nVec, hipMemcpyDeviceToHost, s), "dev D2H");
+  if (P.flags & PF_JAC) {
+    const int nEnt = nVec * (nVec + 1) / 2 + nVec;
+    TIMED("k_normal",
+          launch_normal(s, P.dev_d.as<double>(), m->devkeep_d.as<double>(), nData, nVec, P.normPart_d.as<double>(),
+                        P.out_d.as<double>()),
+          "k_normal");
+    HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nEnt, hipMemcpyDeviceToHost, s), "normal D2H");
+  } else if (mode != S_NONE && (P.flags & PF_STAT)) {
+    HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nVec, hipMemcpyDeviceToHost, s), "results D2H");
+  }
+  return IFG_OK;
+}
+
+Plan* FindPlan(ifg_model* m, int nVec, int flags, const std::vector<int>& freeIdx) {
+  for (auto& p : m->plans)
+    if (p->nVec == nVec && p->flags == flags && p->freeIdx == freeIdx) return p.get();
+  return nullptr;
+}
+
+// Evaluate nVec parameter vectors.  outStat[nVec], devOut[nVec*nData] and
+// modelOut[nVec*nData] are optional host buffers.
+template <typename T>
+int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, double* devOut, double* modelOut,
+             int extraFlags = 0, const std::vector<int>& freeIdx = std::vector<int>(), double* normalOut = nullptr) {
+  int st = PrepareDevice(m);
+  if (st < 0) return st;
+  const int flags = (outStat ? PF_STAT : 0) | (devOut ? PF_DEV : 0) | (modelOut ? PF_MODEL : 0) | extraFlags;
+  Plan* P = FindPlan(m, nVec, flags, freeIdx);
+  std::vector<int> need;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!P) {
+      if (m->plans.size() >= 8) m->plans.erase(m->plans.begin());   // bounded plan cache
+      m->plans.emplace_back(new Plan());
+      P = m->plans.back().get();
+      P->freeIdx = freeIdx;
+      if ((st = BuildPlan<T>(m, *P, nVec, flags, need)) < 0) {
+        m->plans.pop_back();
+        return st;
+      }
+    }
+    if (StageCall(m, *P, params, need) == 0) break;
+    // a box layout larger than the plan's tail stride: rebuild with room for it
+    for (size_t k = 0; k < m->plans.size(); k++)
+      if (m->plans[k].get() == P) {
+        m->plans.erase(m->plans.begin() + k);
+        break;
+      }
+    P = nullptr;
+  }
+  if (!P) return m->fail(IFG_ERR_UNSUPPORTED, "sub-pixel box layout does not fit");
+  hipStream_t s = m->stream;
+  // Kernel timing brackets every launch with events, which stream capture does not
+  // take, so timed evaluations launch eagerly.
+  if (m->useGraphs && !m->timing) {
+    if (!P->exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+      st = Enqueue<T>(m, *P);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (st < 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      HIPCHK(e, "end capture");
+      P->graph = graph;
+      HIPCHK(hipGraphInstantiate(&P->exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    }
+    HIPCHK(hipGraphLaunch(P->exec, s), "graph launch");
+  } else {
+    if ((st = Enqueue<T>(m, *P)) < 0) return st;
+  }
+  if (devOut && !(flags & PF_JAC))
+    HIPCHK(hipMemcpyAsync(devOut, (flags & PF_DEVKEEP) ? m->devkeep_d.p : P->dev_d.p,
+                          sizeof(double) * m->nData * nVec, hipMemcpyDeviceToHost, s),
+           "deviates D2H");
   if (modelOut)
-    HIPCHK(hipMemcpyAsync(modelOut, m->model_d.p, sizeof(double) * nData * nVec, hipMemcpyDeviceToHost, s),
+    HIPCHK(hipMemcpyAsync(modelOut, P->model_d.p, sizeof(double) * m->nData * nVec, hipMemcpyDeviceToHost, s),
            "model D2H");
+  if (flags & PF_JAC) {
+    if (devOut)   // the Jacobian columns (mpfit fjac layout: column j at j * nData)
+      HIPCHK(hipMemcpyAsync(devOut, P->dev_d.p, sizeof(double) * m->nData * nVec, hipMemcpyDeviceToHost, s),
+             "Jacobian D2H");
+  }
   HIPCHK(hipStreamSynchronize(s), "evaluation");
+  if (flags & PF_JAC) {
+    if (normalOut) std::memcpy(normalOut, P->pinnedOut, sizeof(double) * (nVec * (nVec + 1) / 2 + nVec));
+  } else if (outStat) {
+    std::memcpy(outStat, P->pinnedOut, sizeof(double) * nVec);
+  }
   if (m->timing) HarvestTimes(m);
   return IFG_OK;
 }
@@ -780,8 +981,48 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
 int Run(ifg_model* m, const double* params, int nVec, double* outStat, double* devOut, double* modelOut) {
   if (!params || nVec < 1) return m->fail(IFG_ERR_ARG, "null parameters or n_vectors < 1");
   HIPCHK(hipSetDevice(m->device), "hipSetDevice");
+  (void)hipGetLastError();   // launch checks below must not see an unrelated earlier error
   if (m->precision == IFG_PRECISION_FP64) return Evaluate<double>(m, params, nVec, outStat, devOut, modelOut);
   return Evaluate<float>(m, params, nVec, outStat, devOut, modelOut);
+}
+
+// Forward pass (deviates + d/dm factors kept on the device), then a derivative pass.
+template <typename T>
+int RunDeriv(ifg_model* m, const double* params, const std::vector<int>& freeIdx, int kind, double* stat,
+             double* fvec, double* fjac, double* normal, double* grad) {
+  int st;
+  const int fwdFlags = PF_DEVKEEP | (kind == PF_GRAD ? PF_DFAC2 : PF_DFAC1);
+  double s0 = 0.0;
+  if ((st = Evaluate<T>(m, params, 1, &s0, fvec, nullptr, fwdFlags)) < 0) return st;
+  if (stat) *stat = s0;
+  const int nFree = (int)freeIdx.size();
+  if (kind == PF_JAC) return Evaluate<T>(m, params, nFree, nullptr, fjac, nullptr, PF_JAC, freeIdx, normal);
+  return Evaluate<T>(m, params, nFree, grad, nullptr, nullptr, PF_GRAD, freeIdx);
+}
+
+int CheckFree(ifg_model* m, int n_free, const int* ifree, std::vector<int>& out) {
+  if (n_free < 1 || n_free > IFG_MAXFREE) return m->fail(IFG_ERR_ARG, "n_free out of range (1..48)");
+  out.assign(n_free, 0);
+  for (int j = 0; j < n_free; j++) {
+    out[j] = ifree ? ifree[j] : j;
+    if (out[j] < 0 || out[j] >= m->nParamsTot) return m->fail(IFG_ERR_ARG, "free-parameter index out of range");
+  }
+  for (int t : m->types)
+    if (t == F_EXPDISK3D) return m->fail(IFG_ERR_UNSUPPORTED, "analytic derivatives of ExponentialDisk3D");
+  return IFG_OK;
+}
+
+int RunDerivDispatch(ifg_model* m, const double* params, const std::vector<int>& freeIdx, int kind, double* stat,
+                     double* fvec, double* fjac, double* normal, double* grad) {
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!params) return m->fail(IFG_ERR_ARG, "null parameters");
+  HIPCHK(hipSetDevice(m->device), "hipSetDevice");
+  (void)hipGetLastError();
+  m->lastParams.assign(params, params + m->nParamsTot);
+  m->modelValid = false;
+  if (m->precision == IFG_PRECISION_FP64)
+    return RunDeriv<double>(m, params, freeIdx, kind, stat, fvec, fjac, normal, grad);
+  return RunDeriv<float>(m, params, freeIdx, kind, stat, fvec, fjac, normal, grad);
 }
 
 }  // namespace
@@ -792,6 +1033,9 @@ extern "C" {
 ifg_model* ifg_create(int device) {
   ifg_model* m = new ifg_model();
   m->device = device;
+  m->curEvents = &m->pendingEvents;
+  const char* ng = getenv("IFG_NO_GRAPH");
+  if (ng && ng[0] == '1') m->useGraphs = false;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
     m->err = "no HIP device / stream";
     m->stream = nullptr;
@@ -1038,6 +1282,7 @@ int ifg_final_setup(ifg_model* m) {   // core/model_object.cpp:967-1079
   if (m->nValid < 1 && status == IFG_OK) status = m->fail(IFG_ERR_DATA, "no valid data values");
   m->finalDone = true;
   m->devReady = false;
+  m->plans.clear();
   return status;
 }
 
@@ -1055,8 +1300,7 @@ int ifg_create_model_image(ifg_model* m, const double* params) {
   int st = Run(m, params, 1, nullptr, nullptr, model.data());
   if (st < 0) return st;
   m->modelValid = true;
-  m->stage.resize(sizeof(double) * m->nData);
-  std::memcpy(m->stage.data(), model.data(), sizeof(double) * m->nData);
+  m->modelHost.swap(model);
   return IFG_OK;
 }
 
@@ -1068,7 +1312,7 @@ int ifg_get_model_image(ifg_model* m, double* out) {
     int st = ifg_create_model_image(m, p.data());
     if (st < 0) return st;
   }
-  std::memcpy(out, m->stage.data(), sizeof(double) * m->nData);
+  std::memcpy(out, m->modelHost.data(), sizeof(double) * m->nData);
   return IFG_OK;
 }
 
@@ -1095,9 +1339,51 @@ int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, d
   return Run(m, params, n_vectors, out, nullptr, nullptr);
 }
 
+int ifg_jacobian(ifg_model* m, const double* params, int n_free, const int* ifree, double* fvec, double* fjac) {
+  std::vector<int> fr;
+  int st = CheckFree(m, n_free, ifree, fr);
+  if (st < 0) return st;
+  if (!fjac) return m->fail(IFG_ERR_ARG, "null fjac");
+  return RunDerivDispatch(m, params, fr, PF_JAC, nullptr, fvec, fjac, nullptr, nullptr);
+}
+
+int ifg_normal_equations(ifg_model* m, const double* params, int n_free, const int* ifree, double* jtj, double* jtr,
+                         double* stat) {
+  std::vector<int> fr;
+  int st = CheckFree(m, n_free, ifree, fr);
+  if (st < 0) return st;
+  std::vector<double> packed(n_free * (n_free + 1) / 2 + n_free);
+  st = RunDerivDispatch(m, params, fr, PF_JAC, stat, nullptr, nullptr, packed.data(), nullptr);
+  if (st < 0) return st;
+  int e = 0;
+  for (int a = 0; a < n_free; a++)
+    for (int b = a; b < n_free; b++, e++) {
+      if (jtj) {
+        jtj[a * n_free + b] = packed[e];
+        jtj[b * n_free + a] = packed[e];
+      }
+    }
+  if (jtr)
+    for (int a = 0; a < n_free; a++) jtr[a] = packed[e + a];
+  return IFG_OK;
+}
+
+int ifg_fit_statistic_gradient(ifg_model* m, const double* params, double* grad, double* stat) {
+  std::vector<int> fr;
+  int st = CheckFree(m, m->nParamsTot, nullptr, fr);
+  if (st < 0) return st;
+  if (!grad) return m->fail(IFG_ERR_ARG, "null gradient");
+  return RunDerivDispatch(m, params, fr, PF_GRAD, stat, nullptr, nullptr, nullptr, grad);
+}
+
 int ifg_set_kernel_timing(ifg_model* m, int on) {
   m->timing = on != 0;
   m->kernelTimes.clear();
+  return IFG_OK;
+}
+
+int ifg_set_graph_mode(ifg_model* m, int on) {
+  m->useGraphs = on != 0;
   return IFG_OK;
 }
 

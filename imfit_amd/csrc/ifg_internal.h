@@ -7,12 +7,14 @@
 #define IFG_MAXBOX 4     // sub-pixel ("tail") boxes per parameter vector and grid
 #define IFG_MAXREG 4     // oversampled regions per model
 #define IFG_MAXSTAGE 16  // FFT stages
+#define IFG_MAXFREE 48   // parameters of a derivative pass
 
 namespace ifg {
 
 enum FuncType { F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6 };
 
-enum StatMode { S_NONE = 0, S_CHI2 = 1, S_CASH = 2, S_MLR = 3 };
+// S_JAC: write Jacobian columns dfac * dm/dp_j;  S_GRAD: accumulate sum(gfac * dm/dp_j)
+enum StatMode { S_NONE = 0, S_CHI2 = 1, S_CASH = 2, S_MLR = 3, S_JAC = 4, S_GRAD = 5 };
 
 template <typename T>
 struct cpx {
@@ -59,13 +61,23 @@ struct FFTPlan {
   int rad[IFG_MAXSTAGE];
 };
 
+// Model (or derivative) evaluation on a grid.  In derivative mode, plane/vector
+// index v holds d(model)/d(parameter freeIdx[v]) for one shared parameter vector
+// (recStride = boxStride = 0).
 struct EvalArgs {
   int nComp;
   int types[IFG_MAXCOMP];
-  const CompRec* recs;   // [vec * nComp + k]
-  const Box* boxes;      // [vec * IFG_MAXBOX + b]
+  const CompRec* recs;   // [vec * recStride + k]
+  const Box* boxes;      // [vec * boxStride + b]
   const double* tail;    // tail values (see Box)
-  int tailStride;
+  int tailStride;        // layout stride of `tail` per vector (fixed per plan)
+  const int* tailUsed;   // device: pixels of boxes per vector in this call (<= tailStride)
+  int recStride, boxStride;
+  int deriv, nFree;
+  int freeIdx[IFG_MAXFREE];     // global parameter index of each derivative plane
+  int compFirst[IFG_MAXCOMP];   // global index of each component's first own parameter
+  int compXY[IFG_MAXCOMP];      // global index of X0 of each component's function set
+  int compNP[IFG_MAXCOMP];      // own parameter count of each component
 };
 
 // An oversampled region as seen by the main grid: data pixels [c0, c0+w) x [r0, r0+h)
@@ -84,7 +96,10 @@ struct StatArgs {
   const T* weight;    // [nData], mask folded in (1/sigma, or 1/0 for Cash / model errors)
   const T* extra;     // [nData] extra Cash terms (MLR), may be null
   double effGain, sky, rnTerm;   // effective gain, original sky, nCombined * readNoise_adu^2
-  double* devOut;     // [vec * nData] or null
+  double* devOut;     // [vec * nData] or null (S_JAC: Jacobian columns)
+  double* dfacOut;    // [nData] d(deviate)/d(model) or d(statistic)/d(model), or null
+  int dfacKind;       // 1: d(deviate)/dm   2: d(statistic)/dm
+  const double* dfac; // S_JAC / S_GRAD input factor per data pixel
   double* modelOut;   // [vec * nData] or null
   double* partials;   // [pair][plane][slot] x 5 doubles
   int nSlots;         // slots per plane

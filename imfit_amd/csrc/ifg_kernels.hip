@@ -21,6 +21,7 @@
 // FP64 kernels are compiled with -ffp-contract=off so that coordinates, radii and
 // sub-pixel counts follow the reference's rounding exactly.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include "ifg_internal.h"
 
@@ -235,10 +236,101 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// ------------------------------------------------------------------ analytic derivatives
+// d(point intensity)/d(parameter) of one component at (x, y).  slot 0 = X0,
+// slot 1 = Y0, slot 2+k = the component's own parameter k in the reference order
+// (Sersic: PA ell n I_e r_e; Exponential: PA ell I_0 h; Gaussian: PA ell I_0 sigma;
+// Moffat: PA ell I_0 fwhm beta; FlatSky: I_sky).  Geometry enters through the
+// elliptical radius r(x0, y0, PA, ell); at r = 0 the radial terms are taken as 0.
+__device__ double comp_grad(int type, const double* v, double x, double y, int slot) {
+  if (type == F_FLATSKY) return slot == 2 ? 1.0 : 0.0;
+  const double DEG2RAD = 0.017453292519943295;
+  const double c = v[2], s = v[3], q = v[4];
+  const double x_diff = x - v[0], y_diff = y - v[1];
+  const double xp = x_diff * c + y_diff * s;
+  const double yp = (-x_diff * s + y_diff * c) / q;
+  const double r = sqrt(xp * xp + yp * yp);
+  double I, dIdr, dloc = 0.0;
+  switch (type) {
+    case F_SERSIC: {   // I = I_e exp(-bn (u - 1)), u = (r/r_e)^(1/n)
+      const double Ie = v[5], re = v[6], bn = v[7], invn = v[8];
+      const double u = pow(r / re, invn);
+      const double e = exp(-bn * (u - 1.0));
+      I = Ie * e;
+      dIdr = (r > 0.0) ? -I * bn * u * invn / r : 0.0;
+      if (slot == 4) {   // n: bn(n) and u(n)
+        const double dbn = v[10];
+        const double dudn = (r > 0.0) ? -u * log(r / re) * invn * invn : 0.0;
+        dloc = I * (-dbn * (u - 1.0) - bn * dudn);
+      } else if (slot == 5) dloc = e;
+      else if (slot == 6) dloc = I * bn * u * invn / re;
+      break;
+    }
+    case F_EXP: {
+      const double I0 = v[5], h = v[6];
+      const double e = exp(-r / h);
+      I = I0 * e;
+      dIdr = -I / h;
+      if (slot == 4) dloc = e;
+      else if (slot == 5) dloc = I * r / (h * h);
+      break;
+    }
+    case F_GAUSS: {
+      const double I0 = v[5], sig = v[6];
+      const double e = exp(-(r * r) / v[7]);
+      I = I0 * e;
+      dIdr = -I * r / (sig * sig);
+      if (slot == 4) dloc = e;
+      else if (slot == 5) dloc = I * r * r / (sig * sig * sig);
+      break;
+    }
+    case F_MOFFAT: {   // I = I_0 t^-beta, t = 1 + (r/alpha)^2
+      const double I0 = v[5], alpha = v[6], beta = v[7], fwhm = v[9], dalpha_dbeta = v[10];
+      const double t = 1.0 + (r / alpha) * (r / alpha);
+      const double e = pow(t, -beta);
+      I = I0 * e;
+      dIdr = -2.0 * beta * I * r / (alpha * alpha * t);
+      const double dIdalpha = 2.0 * beta * I * r * r / (alpha * alpha * alpha * t);
+      if (slot == 4) dloc = e;
+      else if (slot == 5) dloc = dIdalpha * alpha / fwhm;
+      else if (slot == 6) dloc = -I * log(t) + dIdalpha * dalpha_dbeta;
+      break;
+    }
+    default: return 0.0;
+  }
+  if (slot >= 4) return dloc;
+  if (!(r > 0.0)) return 0.0;
+  double drdp;
+  if (slot == 0) drdp = (-xp * c + yp * s / q) / r;
+  else if (slot == 1) drdp = (-xp * s - yp * c / q) / r;
+  else if (slot == 2) drdp = DEG2RAD * xp * yp * (q - 1.0 / q) / r;
+  else drdp = yp * yp / (q * r);
+  return dIdr * drdp;
+}
+
+// Which slot of component n the global parameter gidx is, or -1.
+__device__ __forceinline__ int grad_slot(const EvalArgs& ea, int n, int gidx) {
+  if (gidx == ea.compXY[n]) return 0;
+  if (gidx == ea.compXY[n] + 1) return 1;
+  const int k = gidx - ea.compFirst[n];
+  return (k >= 0 && k < ea.compNP[n]) ? 2 + k : -1;
+}
+
+// d(model)/d(parameter freeIdx[j]) at a grid pixel, without sub-pixel integration.
+__device__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
+  const int gidx = ea.freeIdx[j];
+  double sum = 0.0;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int slot = grad_slot(ea, n, gidx);
+    if (slot >= 0) sum += comp_grad(ea.types[n], ea.recs[n].v, x, y, slot);
+  }
+  return sum;
+}
+
 // ------------------------------------------------------------------ model value per grid pixel
 template <typename T>
 __device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, int vec, int r, int c) {
-  const Box* bx = ea.boxes + vec * IFG_MAXBOX;
+  const Box* bx = ea.boxes + vec * ea.boxStride;
 #pragma unroll
   for (int b = 0; b < IFG_MAXBOX; b++) {
     const Box B = bx[b];
@@ -248,7 +340,8 @@ __device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, 
   }
   const double x = g.xa + (double)c * g.frac;
   const double y = g.ya + (double)r * g.frac;
-  const CompRec* rec = ea.recs + (long)vec * ea.nComp;
+  if (ea.deriv) return (T)deriv_point(ea, vec, x, y);
+  const CompRec* rec = ea.recs + (long)vec * ea.recStride;
   T newValSum = 0, storedError = 0;
   for (int n = 0; n < ea.nComp; n++) {   // Kahan summation (core/model_object.cpp:1164-1172)
     T adjVal = comp_point<T>(ea.types[n], rec[n].v, x, y) - storedError;
@@ -265,60 +358,93 @@ __device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, 
 // final sum differs (lane partials + xor butterfly, identical in every lane).
 template <typename T>
 __global__ __launch_bounds__(256) void k_tail(EvalArgs ea, GridGeom g, int nVec, int subsampling) {
-  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int vec = wave / ea.tailStride;
-  const int loc = wave - vec * ea.tailStride;
-  if (vec >= nVec) return;
-  const Box* bx = ea.boxes + vec * IFG_MAXBOX;
-  int c = 0, r = 0;
-  bool found = false;
-  for (int b = 0; b < IFG_MAXBOX; b++) {
-    const Box B = bx[b];
-    if (B.w > 0 && loc >= B.off && loc < B.off + B.w * B.h) {
-      c = B.c0 + (loc - B.off) % B.w;
-      r = B.r0 + (loc - B.off) / B.w;
-      found = true;
-      break;
-    }
-  }
-  if (!found) return;
-  const double x = g.xa + (double)c * g.frac;
-  const double y = g.ya + (double)r * g.frac;
-  const CompRec* rec = ea.recs + (long)vec * ea.nComp;
-  double newValSum = 0.0, storedError = 0.0;
-  for (int n = 0; n < ea.nComp; n++) {
-    const int type = ea.types[n];
-    const double* v = rec[n].v;
-    double val;
-    if (type == F_FLATSKY) {
-      val = v[5];
-    } else {
-      const double rad = ell_radius(v, x, y);
-      const int nSub = n_subsamples(type, v, rad, subsampling);
-      if (nSub > 1) {
-        const double deltaSubpix = 1.0 / nSub;
-        const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
-        const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
-        const int n2 = nSub * nSub;
-        double part = 0.0;
-        for (int s = lane; s < n2; s += 64) {
-          const int ii = s / nSub, jj = s - ii * nSub;
-          const double x_ii = x_sub_start + ii * deltaSubpix;
-          const double y_ii = y_sub_start + jj * deltaSubpix;
-          part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
-        }
-        val = wave_sum(part) / (nSub * nSub);
-      } else {
-        val = (double)intensity_r<T>(type, v, rad);
+  const int used = *ea.tailUsed;
+  const long total = (long)nVec * used;
+  const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
+  // grid-stride over (vector, box pixel): the launch shape is fixed, the work count
+  // comes from device memory, so the launch can be replayed from a graph
+  for (long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < total; wave += nWaves) {
+    const int vec = (int)(wave / used);
+    const int loc = (int)(wave - (long)vec * used);
+    const Box* bx = ea.boxes + vec * ea.boxStride;
+    int c = 0, r = 0;
+    bool found = false;
+    for (int b = 0; b < IFG_MAXBOX; b++) {
+      const Box B = bx[b];
+      if (B.w > 0 && loc >= B.off && loc < B.off + B.w * B.h) {
+        c = B.c0 + (loc - B.off) % B.w;
+        r = B.r0 + (loc - B.off) / B.w;
+        found = true;
+        break;
       }
     }
-    double adjVal = val - storedError;
-    double tempSum = newValSum + adjVal;
-    storedError = (tempSum - newValSum) - adjVal;
-    newValSum = tempSum;
+    if (!found) continue;
+    const double x = g.xa + (double)c * g.frac;
+    const double y = g.ya + (double)r * g.frac;
+    const CompRec* rec = ea.recs + (long)vec * ea.recStride;
+    if (ea.deriv) {
+      // derivative of the sub-pixel mean = mean of the derivative over the same samples
+      const int gidx = ea.freeIdx[vec];
+      double dsum = 0.0;
+      for (int n = 0; n < ea.nComp; n++) {
+        const int slot = grad_slot(ea, n, gidx);
+        if (slot < 0) continue;
+        const int type = ea.types[n];
+        const double* v = rec[n].v;
+        const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, ell_radius(v, x, y), subsampling);
+        if (nSub > 1) {
+          const double deltaSubpix = 1.0 / nSub;
+          const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+          const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+          const int n2 = nSub * nSub;
+          double part = 0.0;
+          for (int s = lane; s < n2; s += 64) {
+            const int ii = s / nSub, jj = s - ii * nSub;
+            part += comp_grad(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, slot);
+          }
+          dsum += wave_sum(part) / (nSub * nSub);
+        } else {
+          dsum += comp_grad(type, v, x, y, slot);
+        }
+      }
+      if (lane == 0) const_cast<double*>(ea.tail)[(long)vec * ea.tailStride + loc] = dsum;
+      continue;
+    }
+    double newValSum = 0.0, storedError = 0.0;
+    for (int n = 0; n < ea.nComp; n++) {
+      const int type = ea.types[n];
+      const double* v = rec[n].v;
+      double val;
+      if (type == F_FLATSKY) {
+        val = v[5];
+      } else {
+        const double rad = ell_radius(v, x, y);
+        const int nSub = n_subsamples(type, v, rad, subsampling);
+        if (nSub > 1) {
+          const double deltaSubpix = 1.0 / nSub;
+          const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+          const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+          const int n2 = nSub * nSub;
+          double part = 0.0;
+          for (int s = lane; s < n2; s += 64) {
+            const int ii = s / nSub, jj = s - ii * nSub;
+            const double x_ii = x_sub_start + ii * deltaSubpix;
+            const double y_ii = y_sub_start + jj * deltaSubpix;
+            part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
+          }
+          val = wave_sum(part) / (nSub * nSub);
+        } else {
+          val = (double)intensity_r<T>(type, v, rad);
+        }
+      }
+      double adjVal = val - storedError;
+      double tempSum = newValSum + adjVal;
+      storedError = (tempSum - newValSum) - adjVal;
+      newValSum = tempSum;
+    }
+    if (lane == 0) const_cast<double*>(ea.tail)[(long)vec * ea.tailStride + loc] = newValSum;
   }
-  if (lane == 0) const_cast<double*>(ea.tail)[(long)vec * ea.tailStride + loc] = newValSum;
 }
 
 // ------------------------------------------------------------------ evaluation + row FFT
@@ -483,6 +609,14 @@ __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r
   const long z = (long)r * sa.W + c;
   if (sa.modelOut) sa.modelOut[(long)vec * sa.nData + z] = m;
   if (sa.mode == S_NONE) return;
+  if (sa.mode == S_JAC) {   // m is d(model)/d(p_vec) here
+    sa.devOut[(long)vec * sa.nData + z] = sa.dfac[z] * m;
+    return;
+  }
+  if (sa.mode == S_GRAD) {
+    acc.s2 += sa.dfac[z] * m;
+    return;
+  }
   const double d = (double)sa.data[z];
   double w = (double)sa.weight[z];
   if (sa.modelErrors && w != 0.0) {   // UpdateWeightVector (core/model_object.cpp:1345-1381)
@@ -494,6 +628,12 @@ __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r
     const double dev = w * (d - m);
     en_add(acc, dev, agiant);
     if (sa.devOut) sa.devOut[(long)vec * sa.nData + z] = dev;
+    if (sa.dfacOut) {
+      // d(dev)/dm = -w, plus the model-error weight's own dependence on m
+      double f = -w;
+      if (sa.modelErrors && w != 0.0) f += -0.5 * w * w * w / sa.effGain * (d - m);
+      sa.dfacOut[z] = (sa.dfacKind == 2) ? 2.0 * dev * f : f;
+    }
   } else {   // Cash / Poisson-MLR (core/model_object.cpp:1386-1401,1703-1775)
     const double modVal = sa.effGain * (m + sa.sky);
     const double dataVal = sa.effGain * (d + sa.sky);
@@ -501,9 +641,17 @@ __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r
     const double extraTerms = sa.extra ? (double)sa.extra[z] : 0.0;
     const double term = modVal - dataVal * logModel + extraTerms;
     acc.s2 += w * term;
-    if (sa.devOut)
-      sa.devOut[(long)vec * sa.nData + z] =
-          (sa.mode == S_MLR) ? sqrt(2.0 * w * fabs(term)) : w * (d - m);
+    const double dev = (sa.mode == S_MLR) ? sqrt(2.0 * w * fabs(term)) : w * (d - m);
+    if (sa.devOut) sa.devOut[(long)vec * sa.nData + z] = dev;
+    if (sa.dfacOut) {
+      // d(term)/dm = g (1 - D/M) (the log term is constant where M <= 0)
+      const double dterm = (modVal <= 0) ? sa.effGain : sa.effGain * (1.0 - dataVal / modVal);
+      double f;
+      if (sa.dfacKind == 2) f = 2.0 * w * dterm;   // d(2 sum w term)/dm
+      else if (sa.mode == S_MLR) f = (dev > 0.0) ? w * (term >= 0.0 ? 1.0 : -1.0) * dterm / dev : 0.0;
+      else f = -w;
+      sa.dfacOut[z] = f;
+    }
   }
 }
 
@@ -665,10 +813,68 @@ __global__ __launch_bounds__(256) void k_final(const double* __restrict__ partia
         ans = t.x3max * sqrt(t.s3);
       }
       out[v] = ans * ans;
+    } else if (mode == S_GRAD) {
+      out[v] = t.s2;
     } else {
       out[v] = 2.0 * t.s2;
     }
   }
+}
+
+// ------------------------------------------------------------------ normal equations
+// J^T J (upper triangle, row-major packed) and J^T r from Jacobian columns
+// jcol[j * nData + i] and deviates r[i]: each block stages a tile of TP pixels of
+// every column in LDS, each thread forms whole dot products over the tile, and
+// k_normal_final sums the per-block partials in a fixed order.
+__global__ __launch_bounds__(256) void k_normal_tile(const double* __restrict__ jcol, const double* __restrict__ r,
+                                                     long nData, int nFree, int TP, double* __restrict__ partials) {
+  extern __shared__ char smem_raw[];
+  double* J = reinterpret_cast<double*>(smem_raw);   // [nFree + 1][TP]
+  const long p0 = (long)blockIdx.x * TP;
+  const int nT = nFree * (nFree + 1) / 2;
+  for (int e = threadIdx.x; e < (nFree + 1) * TP; e += blockDim.x) {
+    const int j = e / TP, p = e - j * TP;
+    const long i = p0 + p;
+    double v = 0.0;
+    if (i < nData) v = (j < nFree) ? jcol[(long)j * nData + i] : r[i];
+    J[e] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nT + nFree; e += blockDim.x) {
+    int a, b;
+    if (e < nT) {   // e -> (a, b), a <= b, row-major packed upper triangle
+      a = 0;
+      int rem = e;
+      while (rem >= nFree - a) {
+        rem -= nFree - a;
+        a++;
+      }
+      b = a + rem;
+    } else {
+      a = e - nT;
+      b = nFree;   // the deviates row
+    }
+    const double* x = J + (long)a * TP;
+    const double* y = J + (long)b * TP;
+    double acc = 0.0;
+    for (int p = 0; p < TP; p++) acc += x[p] * y[p];
+    partials[(long)blockIdx.x * (nT + nFree) + e] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_normal_final(const double* __restrict__ partials, int nBlocks, int nEnt,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nBlocks; b += blockDim.x) acc += partials[(long)b * nEnt + e];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[e] = red[0];
 }
 
 }  // namespace ifg
@@ -680,9 +886,10 @@ static size_t smem_rows(int rows, int N, size_t elem) { return 2 * (size_t)rows 
 
 template <typename T>
 hipError_t launch_tail(hipStream_t s, const EvalArgs& ea, const GridGeom& g, int nVec, int sub) {
-  const long waves = (long)nVec * ea.tailStride;
+  const long waves = (long)nVec * ea.tailStride;   // upper bound of the work count
   if (waves <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tail<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, ea, g, nVec, sub);
+  const long blocks = std::min<long>((waves + 3) / 4, 2048);
+  hipLaunchKernelGGL(k_tail<T>, dim3((unsigned)blocks), dim3(256), 0, s, ea, g, nVec, sub);
   return hipGetLastError();
 }
 
@@ -738,6 +945,24 @@ hipError_t launch_downsample(hipStream_t s, const double* osub, int Wos, long os
   hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256, nVec), dim3(256), 0, s, osub, Wos, osubStride, ovals,
                      ovStride, off, nColsSub, nRowsSub, scale, nVec);
   return hipGetLastError();
+}
+
+hipError_t launch_normal(hipStream_t s, const double* jcol, const double* r, long nData, int nFree, double* partials,
+                         double* out) {
+  const int TP = (nFree <= 16) ? 256 : 128;
+  const int nBlocks = (int)((nData + TP - 1) / TP);
+  const int nEnt = nFree * (nFree + 1) / 2 + nFree;
+  hipLaunchKernelGGL(k_normal_tile, dim3(nBlocks), dim3(256), (size_t)(nFree + 1) * TP * sizeof(double), s, jcol, r,
+                     nData, nFree, TP, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_normal_final, dim3(nEnt), dim3(256), 0, s, partials, nBlocks, nEnt, out);
+  return hipGetLastError();
+}
+
+long normal_partials_size(long nData, int nFree) {
+  const int TP = (nFree <= 16) ? 256 : 128;
+  return ((nData + TP - 1) / TP) * (long)(nFree * (nFree + 1) / 2 + nFree);
 }
 
 hipError_t launch_final(hipStream_t s, const double* partials, int nSlots, const int2* vecSlots, int nVec, int mode,

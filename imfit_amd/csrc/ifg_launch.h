@@ -23,6 +23,9 @@ hipError_t launch_eval_stat(hipStream_t s, const EvalArgs& ea, const GridGeom& g
                             int pixPerBlock, double agiant);
 hipError_t launch_downsample(hipStream_t s, const double* osub, int Wos, long osubStride, double* ovals,
                              int ovStride, int off, int nColsSub, int nRowsSub, int scale, int nVec);
+hipError_t launch_normal(hipStream_t s, const double* jcol, const double* r, long nData, int nFree, double* partials,
+                         double* out);
+long normal_partials_size(long nData, int nFree);
 hipError_t launch_final(hipStream_t s, const double* partials, int nSlots, const int2* vecSlots, int nVec, int mode,
                         double* out);
 }  // namespace ifg

@@ -1,0 +1,627 @@
+// ifg_lm.cpp -- Levenberg-Marquardt callers of the GPU model (host code).
+//
+// ifg_fit_lm: MINPACK-1 LMDIF as imfit's mpfit runs it (solvers/mpfit.cpp:352-1097):
+// QR factorisation with column pivoting (mp_qrfac, mpfit.cpp:1361-1556), the
+// Levenberg-Marquardt parameter by Newton iteration on the trust-region
+// constraint (mp_lmpar, 1783-2070), the constrained least-squares solve by Givens
+// rotations (mp_qrsolv, 1561-1780), mpfit's handling of parameter limits (pegging
+// and shortening steps that would leave the box, 574-598, 763-806), and its
+// convergence tests (941-987).  The Jacobian comes either from one-sided finite
+// differences with one model evaluation per free parameter (mp_fdjac2,
+// 1102-1356; every evaluation is a GPU ComputeDeviates), or from the GPU's
+// analytic Jacobian (mpfit's user-derivative hook, side == 3, 1208-1232).
+//
+// ifg_fit_lm_normal: Levenberg-Marquardt on the Gauss-Newton normal equations,
+// with J^T J and J^T r reduced on the GPU in FP64 (no per-pixel data crosses the
+// bus); the n x n damped system is solved on the host by Cholesky factorisation.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/imfit_gpu.h"
+
+namespace {
+
+const double MACHEP0 = 2.2204460e-16;   // solvers/mpfit.h:132
+const double DWARF = 2.2250739e-308;
+
+double enorm(int n, const double* x) {   // MINPACK scaled 2-norm (core/mp_enorm.cpp:54-122)
+  const double RDWARF = 3.834e-20, RGIANT = 1.304e19;
+  double s1 = 0, s2 = 0, s3 = 0, x1max = 0, x3max = 0, temp;
+  const double agiant = RGIANT / (double)n;
+  for (int i = 0; i < n; i++) {
+    double xabs = fabs(x[i]);
+    if (xabs > RDWARF && xabs < agiant) {
+      s2 += xabs * xabs;
+    } else if (xabs > RDWARF) {
+      if (xabs > x1max) { temp = x1max / xabs; s1 = 1.0 + s1 * temp * temp; x1max = xabs; }
+      else { temp = xabs / x1max; s1 += temp * temp; }
+    } else if (xabs > x3max) { temp = x3max / xabs; s3 = 1.0 + s3 * temp * temp; x3max = xabs; }
+    else if (xabs != 0.0) { temp = xabs / x3max; s3 += temp * temp; }
+  }
+  if (s1 != 0.0) return x1max * sqrt(s1 + (s2 / x1max) / x1max);
+  if (s2 != 0.0) {
+    temp = (s2 >= x3max) ? s2 * (1.0 + (x3max / s2) * (x3max * s3)) : x3max * ((s2 / x3max) + (x3max * s3));
+    return sqrt(temp);
+  }
+  return x3max * sqrt(s3);
+}
+
+// Householder QR of the m x n column-major matrix a (leading dimension m), with
+// column pivoting; on return the upper triangle of a holds R without its diagonal,
+// rdiag the diagonal of R, acnorm the original column norms, ipvt the permutation.
+void qrfac(int m, int n, double* a, int* ipvt, double* rdiag, double* acnorm, double* wa) {
+  for (int j = 0; j < n; j++) {
+    acnorm[j] = enorm(m, a + (long)j * m);
+    rdiag[j] = acnorm[j];
+    wa[j] = rdiag[j];
+    ipvt[j] = j;
+  }
+  const int minmn = std::min(m, n);
+  for (int j = 0; j < minmn; j++) {
+    int kmax = j;   // bring the column of largest remaining norm to position j
+    for (int k = j; k < n; k++)
+      if (rdiag[k] > rdiag[kmax]) kmax = k;
+    if (kmax != j) {
+      double* cj = a + (long)j * m;
+      double* ck = a + (long)kmax * m;
+      for (int i = 0; i < m; i++) std::swap(cj[i], ck[i]);
+      rdiag[kmax] = rdiag[j];
+      wa[kmax] = wa[j];
+      std::swap(ipvt[j], ipvt[kmax]);
+    }
+    double* cj = a + (long)j * m;
+    double ajnorm = enorm(m - j, cj + j);
+    if (ajnorm != 0.0) {
+      if (cj[j] < 0.0) ajnorm = -ajnorm;
+      for (int i = j; i < m; i++) cj[i] /= ajnorm;
+      cj[j] += 1.0;
+      for (int k = j + 1; k < n; k++) {   // apply the reflection to the remaining columns
+        double* ck = a + (long)k * m;
+        double sum = 0.0;
+        for (int i = j; i < m; i++) sum += cj[i] * ck[i];
+        const double temp = sum / cj[j];
+        for (int i = j; i < m; i++) ck[i] -= temp * cj[i];
+        if (rdiag[k] != 0.0) {   // downdate the column norm, recompute if it lost accuracy
+          double t = ck[j] / rdiag[k];
+          t = std::max(0.0, 1.0 - t * t);
+          rdiag[k] *= sqrt(t);
+          const double t2 = rdiag[k] / wa[k];
+          if (0.05 * t2 * t2 <= MACHEP0) {
+            rdiag[k] = enorm(m - j - 1, ck + j + 1);
+            wa[k] = rdiag[k];
+          }
+        }
+      }
+    }
+    rdiag[j] = -ajnorm;
+  }
+}
+
+// Solve min || [A; D] x - [b; 0] || given the QR factors of A (R in the upper
+// triangle of r, leading dimension ldr) and qtb = Q^T b, with Givens rotations
+// eliminating the diagonal D (indexed through ipvt).  sdiag receives the diagonal
+// of the triangular factor S.
+void qrsolv(int n, double* r, int ldr, const int* ipvt, const double* diag, const double* qtb, double* x,
+            double* sdiag, double* wa) {
+  for (int j = 0; j < n; j++) {   // copy R to the lower triangle, save its diagonal
+    for (int i = j; i < n; i++) r[i + (long)ldr * j] = r[j + (long)ldr * i];
+    x[j] = r[j + (long)ldr * j];
+    wa[j] = qtb[j];
+  }
+  for (int j = 0; j < n; j++) {
+    const int l = ipvt[j];
+    if (diag[l] != 0.0) {
+      for (int k = j; k < n; k++) sdiag[k] = 0.0;
+      sdiag[j] = diag[l];
+      double qtbpj = 0.0;
+      for (int k = j; k < n; k++) {
+        if (sdiag[k] == 0.0) continue;
+        const long kk = k + (long)ldr * k;
+        double cs, sn;
+        if (fabs(r[kk]) < fabs(sdiag[k])) {
+          const double cotan = r[kk] / sdiag[k];
+          sn = 0.5 / sqrt(0.25 + 0.25 * cotan * cotan);
+          cs = sn * cotan;
+        } else {
+          const double tn = sdiag[k] / r[kk];
+          cs = 0.5 / sqrt(0.25 + 0.25 * tn * tn);
+          sn = cs * tn;
+        }
+        r[kk] = cs * r[kk] + sn * sdiag[k];
+        const double temp = cs * wa[k] + sn * qtbpj;
+        qtbpj = -sn * wa[k] + cs * qtbpj;
+        wa[k] = temp;
+        for (int i = k + 1; i < n; i++) {
+          const long ik = i + (long)ldr * k;
+          const double t = cs * r[ik] + sn * sdiag[i];
+          sdiag[i] = -sn * r[ik] + cs * sdiag[i];
+          r[ik] = t;
+        }
+      }
+    }
+    const long kk = j + (long)ldr * j;
+    sdiag[j] = r[kk];
+    r[kk] = x[j];
+  }
+  int nsing = n;   // back substitution on the non-singular part
+  for (int j = 0; j < n; j++) {
+    if (sdiag[j] == 0.0 && nsing == n) nsing = j;
+    if (nsing < n) wa[j] = 0.0;
+  }
+  for (int k = 0; k < nsing; k++) {
+    const int j = nsing - k - 1;
+    double sum = 0.0;
+    for (int i = j + 1; i < nsing; i++) sum += r[i + (long)ldr * j] * wa[i];
+    wa[j] = (wa[j] - sum) / sdiag[j];
+  }
+  for (int j = 0; j < n; j++) x[ipvt[j]] = wa[j];
+}
+
+// Levenberg-Marquardt parameter: find par >= 0 such that the step x solving the
+// damped problem satisfies | ||D x|| - delta | <= 0.1 delta (or par = 0 if the
+// Gauss-Newton step already lies inside), by safeguarded Newton iteration between
+// the bounds parl and paru.  diag is indexed by the free-parameter index.
+void lmpar(int n, double* r, int ldr, const int* ipvt, const double* diag, const double* qtb, double delta,
+           double* par, double* x, double* sdiag, double* wa1, double* wa2) {
+  int nsing = n;
+  for (int j = 0; j < n; j++) {   // Gauss-Newton direction
+    wa1[j] = qtb[j];
+    if (r[j + (long)ldr * j] == 0.0 && nsing == n) nsing = j;
+    if (nsing < n) wa1[j] = 0.0;
+  }
+  for (int k = 0; k < nsing; k++) {
+    const int j = nsing - k - 1;
+    wa1[j] = wa1[j] / r[j + (long)ldr * j];
+    const double temp = wa1[j];
+    for (int i = 0; i < j; i++) wa1[i] -= r[i + (long)ldr * j] * temp;
+  }
+  for (int j = 0; j < n; j++) x[ipvt[j]] = wa1[j];
+  int iter = 0;
+  for (int j = 0; j < n; j++) wa2[j] = diag[j] * x[j];
+  double dxnorm = enorm(n, wa2);
+  double fp = dxnorm - delta;
+  if (fp <= 0.1 * delta) {
+    *par = 0.0;
+    return;
+  }
+  double parl = 0.0;   // lower bound from the Newton step (full rank only)
+  if (nsing >= n) {
+    for (int j = 0; j < n; j++) {
+      const int l = ipvt[j];
+      wa1[j] = diag[l] * (wa2[l] / dxnorm);
+    }
+    for (int j = 0; j < n; j++) {
+      double sum = 0.0;
+      for (int i = 0; i < j; i++) sum += r[i + (long)ldr * j] * wa1[i];
+      wa1[j] = (wa1[j] - sum) / r[j + (long)ldr * j];
+    }
+    const double temp = enorm(n, wa1);
+    parl = ((fp / delta) / temp) / temp;
+  }
+  for (int j = 0; j < n; j++) {   // upper bound from the gradient
+    double sum = 0.0;
+    for (int i = 0; i <= j; i++) sum += r[i + (long)ldr * j] * qtb[i];
+    wa1[j] = sum / diag[ipvt[j]];
+  }
+  const double gnorm = enorm(n, wa1);
+  double paru = gnorm / delta;
+  if (paru == 0.0) paru = DWARF / std::min(delta, 0.1);
+  *par = std::max(*par, parl);
+  *par = std::min(*par, paru);
+  if (*par == 0.0) *par = gnorm / dxnorm;
+  for (;;) {
+    iter++;
+    if (*par == 0.0) *par = std::max(DWARF, 0.001 * paru);
+    const double temp = sqrt(*par);
+    for (int j = 0; j < n; j++) wa1[j] = temp * diag[j];
+    qrsolv(n, r, ldr, ipvt, wa1, qtb, x, sdiag, wa2);
+    for (int j = 0; j < n; j++) wa2[j] = diag[j] * x[j];
+    dxnorm = enorm(n, wa2);
+    const double fpold = fp;
+    fp = dxnorm - delta;
+    if (fabs(fp) <= 0.1 * delta || (parl == 0.0 && fp <= fpold && fpold < 0.0) || iter == 10) break;
+    for (int j = 0; j < n; j++) {   // Newton correction
+      const int l = ipvt[j];
+      wa1[j] = diag[l] * (wa2[l] / dxnorm);
+    }
+    for (int j = 0; j < n; j++) {
+      wa1[j] = wa1[j] / sdiag[j];
+      const double t = wa1[j];
+      for (int i = j + 1; i < n; i++) wa1[i] -= r[i + (long)ldr * j] * t;
+    }
+    const double tn = enorm(n, wa1);
+    const double parc = ((fp / delta) / tn) / tn;
+    if (fp > 0.0) parl = std::max(parl, *par);
+    if (fp < 0.0) paru = std::min(paru, *par);
+    *par = std::max(parl, *par + parc);
+  }
+  if (iter == 0) *par = 0.0;
+}
+
+struct Limits {
+  std::vector<int> ifree, qllim, qulim;
+  std::vector<double> llim, ulim;
+  bool any = false;
+};
+
+int SetupLimits(int npar, const double* x, const ifg_param_info* info, Limits& L) {
+  for (int i = 0; i < npar; i++)
+    if (!info || !info[i].fixed) L.ifree.push_back(i);
+  const int nfree = (int)L.ifree.size();
+  if (nfree == 0) return IFG_LM_ERR_NFREE;
+  L.qllim.assign(nfree, 0);
+  L.qulim.assign(nfree, 0);
+  L.llim.assign(nfree, 0.0);
+  L.ulim.assign(nfree, 0.0);
+  if (!info) return 0;
+  for (int i = 0; i < npar; i++) {   // mpfit.cpp:455-470
+    if (!info[i].fixed && info[i].limited[0] && info[i].limited[1] && info[i].limits[0] >= info[i].limits[1])
+      return IFG_LM_ERR_BOUNDS;
+    if ((info[i].limited[0] && x[i] < info[i].limits[0]) || (info[i].limited[1] && x[i] > info[i].limits[1]))
+      return IFG_LM_ERR_INITBOUNDS;
+  }
+  for (int j = 0; j < nfree; j++) {
+    const ifg_param_info& p = info[L.ifree[j]];
+    L.qllim[j] = p.limited[0];
+    L.qulim[j] = p.limited[1];
+    L.llim[j] = p.limits[0];
+    L.ulim[j] = p.limits[1];
+    if (L.qllim[j] || L.qulim[j]) L.any = true;
+  }
+  return 0;
+}
+
+// Shorten a step so it stays inside the limits (mpfit.cpp:763-806); returns alpha.
+double LimitStep(const Limits& L, const double* x, double* wa1, double* wa2, int nfree) {
+  double alpha = 1.0;
+  if (!L.any) {
+    for (int j = 0; j < nfree; j++) wa2[j] = x[j] + wa1[j];
+    return alpha;
+  }
+  for (int j = 0; j < nfree; j++) {
+    const bool lpegged = L.qllim[j] && (x[j] <= L.llim[j]);
+    const bool upegged = L.qulim[j] && (x[j] >= L.ulim[j]);
+    const bool dwa1 = fabs(wa1[j]) > MACHEP0;
+    if (lpegged && wa1[j] < 0) wa1[j] = 0;
+    if (upegged && wa1[j] > 0) wa1[j] = 0;
+    if (dwa1 && L.qllim[j] && (x[j] + wa1[j]) < L.llim[j]) alpha = std::min(alpha, (L.llim[j] - x[j]) / wa1[j]);
+    if (dwa1 && L.qulim[j] && (x[j] + wa1[j]) > L.ulim[j]) alpha = std::min(alpha, (L.ulim[j] - x[j]) / wa1[j]);
+  }
+  for (int j = 0; j < nfree; j++) {
+    wa1[j] = wa1[j] * alpha;
+    wa2[j] = x[j] + wa1[j];
+    const double sgnu = (L.ulim[j] >= 0) ? 1.0 : -1.0, sgnl = (L.llim[j] >= 0) ? 1.0 : -1.0;
+    const double ulim1 = L.ulim[j] * (1 - sgnu * MACHEP0) - ((L.ulim[j] == 0) ? MACHEP0 : 0);
+    const double llim1 = L.llim[j] * (1 + sgnl * MACHEP0) + ((L.llim[j] == 0) ? MACHEP0 : 0);
+    if (L.qulim[j] && wa2[j] >= ulim1) wa2[j] = L.ulim[j];
+    if (L.qllim[j] && wa2[j] <= llim1) wa2[j] = L.llim[j];
+  }
+  return alpha;
+}
+
+// pegged parameters whose gradient points out of the box get a zero column (mpfit.cpp:574-598)
+void PegColumns(const Limits& L, const double* x, const double* fvec, double* fjac, long m, int nfree) {
+  if (!L.any) return;
+  for (int j = 0; j < nfree; j++) {
+    const bool lpegged = L.qllim[j] && (x[j] == L.llim[j]);
+    const bool upegged = L.qulim[j] && (x[j] == L.ulim[j]);
+    if (!(lpegged || upegged)) continue;
+    double sum = 0;
+    for (long i = 0; i < m; i++) sum += fvec[i] * fjac[(long)j * m + i];
+    if ((lpegged && sum > 0) || (upegged && sum < 0))
+      for (long i = 0; i < m; i++) fjac[(long)j * m + i] = 0;
+  }
+}
+
+void Finish(ifg_lm_result* res, int info, int iter, int nfev, int nfree, double fnorm, double fnorm1,
+            double orignorm, const Limits& L, const double* x, const ifg_param_info* pinfo, int npar,
+            const double* xall) {
+  if (!res) return;
+  res->status = info;
+  res->niter = iter;
+  res->nfev = nfev;
+  res->nfree = nfree;
+  const double bn = std::max(fnorm, fnorm1);
+  res->bestnorm = bn * bn;
+  res->orignorm = orignorm;
+  int npegged = 0;
+  if (pinfo)
+    for (int i = 0; i < npar; i++)
+      if ((pinfo[i].limited[0] && pinfo[i].limits[0] == xall[i]) ||
+          (pinfo[i].limited[1] && pinfo[i].limits[1] == xall[i]))
+        npegged++;
+  res->npegged = npegged;
+  (void)L;
+  (void)x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+               int jacobian_mode, ifg_lm_result* res) {
+  const int npar = ifg_get_nparams(model);
+  const long m = ifg_get_ndata(model);
+  double ftol = 1e-10, xtol = 1e-10, gtol = 1e-10, stepfactor = 100.0;   // mpfit.cpp:377-388
+  int maxiter = 200, maxfev = 0;
+  if (cfg) {
+    if (cfg->ftol > 0) ftol = cfg->ftol;
+    if (cfg->xtol > 0) xtol = cfg->xtol;
+    if (cfg->gtol > 0) gtol = cfg->gtol;
+    if (cfg->maxiter > 0) maxiter = cfg->maxiter;
+    maxfev = cfg->maxfev;
+  }
+  Limits L;
+  int st = SetupLimits(npar, xall, pinfo, L);
+  if (st) return st;
+  const int nfree = (int)L.ifree.size();
+  if (m < nfree) return IFG_LM_ERR_DOF;
+  std::vector<double> fvec(m), qtf(nfree), x(nfree), xnew(xall, xall + npar), fjac((size_t)m * nfree);
+  std::vector<double> diag(nfree), wa1(nfree), wa2(nfree), wa3(nfree), wa4(m);
+  std::vector<int> ipvt(nfree);
+  int nfev = 0, iter = 1, info = 0;
+  if ((st = ifg_compute_deviates(model, xall, fvec.data())) < 0) return st;
+  nfev++;
+  double fnorm = enorm((int)m, fvec.data()), fnorm1 = -1.0, xnorm = -1.0, delta = 0.0, par = 0.0;
+  const double orignorm = fnorm * fnorm;
+  double actred = 0, prered = 0, ratio = 0, gnorm = 0;
+  for (int j = 0; j < nfree; j++) x[j] = xall[L.ifree[j]];
+  const double eps = sqrt(std::max(MACHEP0, MACHEP0));   // epsfcn = MP_MACHEP0
+  for (;;) {   // outer loop: Jacobian at the current point
+    for (int j = 0; j < nfree; j++) xnew[L.ifree[j]] = x[j];
+    if (jacobian_mode == IFG_LM_JACOBIAN_ANALYTIC) {
+      if ((st = ifg_jacobian(model, xnew.data(), nfree, L.ifree.data(), nullptr, fjac.data())) < 0) return st;
+      nfev++;
+    } else {   // mp_fdjac2 one-sided differences (mpfit.cpp:1273-1300)
+      for (int j = 0; j < nfree; j++) {
+        const double temp = xnew[L.ifree[j]];
+        double h = eps * fabs(temp);
+        if (h == 0.0) h = eps;
+        if (L.qulim[j] && temp > L.ulim[j] - h) h = -h;
+        xnew[L.ifree[j]] = temp + h;
+        if ((st = ifg_compute_deviates(model, xnew.data(), wa4.data())) < 0) return st;
+        nfev++;
+        xnew[L.ifree[j]] = temp;
+        double* col = fjac.data() + (size_t)j * m;
+        for (long i = 0; i < m; i++) col[i] = (wa4[i] - fvec[i]) / h;
+      }
+    }
+    PegColumns(L, x.data(), fvec.data(), fjac.data(), m, nfree);
+    qrfac((int)m, nfree, fjac.data(), ipvt.data(), wa1.data(), wa2.data(), wa3.data());
+    if (iter == 1) {   // scale by the initial column norms; initial step bound
+      for (int j = 0; j < nfree; j++) diag[j] = (wa2[j] == 0.0) ? 1.0 : wa2[j];
+      for (int j = 0; j < nfree; j++) wa3[j] = diag[j] * x[j];
+      xnorm = enorm(nfree, wa3.data());
+      delta = stepfactor * xnorm;
+      if (delta == 0.0) delta = stepfactor;
+    }
+    for (long i = 0; i < m; i++) wa4[i] = fvec[i];   // Q^T fvec
+    for (int j = 0; j < nfree; j++) {
+      double* cj = fjac.data() + (size_t)j * m;
+      if (cj[j] != 0.0) {
+        double sum = 0.0;
+        for (long i = j; i < m; i++) sum += cj[i] * wa4[i];
+        const double temp = -sum / cj[j];
+        for (long i = j; i < m; i++) wa4[i] += cj[i] * temp;
+      }
+      cj[j] = wa1[j];
+      qtf[j] = wa4[j];
+    }
+    for (int j = 0; j < nfree; j++)
+      for (int i = 0; i < nfree; i++)
+        if (!std::isfinite(fjac[(size_t)j * m + i])) {
+          info = IFG_LM_ERR_NAN;
+          goto done;
+        }
+    gnorm = 0.0;   // scaled gradient norm
+    if (fnorm != 0.0)
+      for (int j = 0; j < nfree; j++) {
+        const int l = ipvt[j];
+        if (wa2[l] != 0.0) {
+          double sum = 0.0;
+          for (int i = 0; i <= j; i++) sum += fjac[(size_t)j * m + i] * (qtf[i] / fnorm);
+          gnorm = std::max(gnorm, fabs(sum / wa2[l]));
+        }
+      }
+    if (gnorm <= gtol) info = IFG_LM_OK_DIR;
+    if (info != 0) goto done;
+    for (int j = 0; j < nfree; j++) diag[j] = std::max(diag[j], wa2[j]);
+    for (;;) {   // inner loop: trial steps until one reduces the norm enough
+      lmpar(nfree, fjac.data(), (int)m, ipvt.data(), diag.data(), qtf.data(), delta, &par, wa1.data(), wa2.data(),
+            wa3.data(), wa4.data());
+      for (int j = 0; j < nfree; j++) wa1[j] = -wa1[j];
+      const double alpha = LimitStep(L, x.data(), wa1.data(), wa2.data(), nfree);
+      for (int j = 0; j < nfree; j++) wa3[j] = diag[j] * wa1[j];
+      const double pnorm = enorm(nfree, wa3.data());
+      if (iter == 1) delta = std::min(delta, pnorm);
+      for (int j = 0; j < nfree; j++) xnew[L.ifree[j]] = wa2[j];
+      if ((st = ifg_compute_deviates(model, xnew.data(), wa4.data())) < 0) return st;
+      nfev++;
+      fnorm1 = enorm((int)m, wa4.data());
+      actred = -1.0;
+      if (0.1 * fnorm1 < fnorm) {
+        const double t = fnorm1 / fnorm;
+        actred = 1.0 - t * t;
+      }
+      for (int j = 0; j < nfree; j++) {   // R P^T p
+        wa3[j] = 0.0;
+        const double temp = wa1[ipvt[j]];
+        for (int i = 0; i <= j; i++) wa3[i] += fjac[(size_t)j * m + i] * temp;
+      }
+      const double temp1 = enorm(nfree, wa3.data()) * alpha / fnorm;
+      const double temp2 = (sqrt(alpha * par) * pnorm) / fnorm;
+      prered = temp1 * temp1 + (temp2 * temp2) / 0.5;
+      const double dirder = -(temp1 * temp1 + temp2 * temp2);
+      ratio = (prered != 0.0) ? actred / prered : 0.0;
+      if (ratio <= 0.25) {   // step-bound update (mpfit.cpp:880-895)
+        double temp = (actred >= 0.0) ? 0.5 : 0.5 * dirder / (dirder + 0.5 * actred);
+        if (0.1 * fnorm1 >= fnorm || temp < 0.1) temp = 0.1;
+        delta = temp * std::min(delta, pnorm / 0.1);
+        par = par / temp;
+      } else if (par == 0.0 || ratio >= 0.75) {
+        delta = pnorm / 0.5;
+        par = 0.5 * par;
+      }
+      if (ratio >= 0.0001) {   // successful iteration
+        for (int j = 0; j < nfree; j++) {
+          x[j] = wa2[j];
+          wa2[j] = diag[j] * x[j];
+        }
+        for (long i = 0; i < m; i++) fvec[i] = wa4[i];
+        xnorm = enorm(nfree, wa2.data());
+        fnorm = fnorm1;
+        iter++;
+      }
+      if (fabs(actred) <= ftol && prered <= ftol && 0.5 * ratio <= 1.0) info = IFG_LM_OK_CHI;
+      if (delta <= xtol * xnorm) info = IFG_LM_OK_PAR;
+      if (fabs(actred) <= ftol && prered <= ftol && 0.5 * ratio <= 1.0 && info == 2) info = IFG_LM_OK_BOTH;
+      if (info != 0) goto done;
+      if (maxfev > 0 && nfev >= maxfev) info = IFG_LM_MAXITER;
+      if (iter >= maxiter) info = IFG_LM_MAXITER;
+      if (fabs(actred) <= MACHEP0 && prered <= MACHEP0 && 0.5 * ratio <= 1.0) info = IFG_LM_FTOL;
+      if (delta <= MACHEP0 * xnorm) info = IFG_LM_XTOL;
+      if (gnorm <= MACHEP0) info = IFG_LM_GTOL;
+      if (info != 0) goto done;
+      if (ratio >= 0.0001) break;
+    }
+  }
+done:
+  for (int j = 0; j < nfree; j++) xall[L.ifree[j]] = x[j];
+  Finish(res, info, iter, nfev, nfree, fnorm, fnorm1, orignorm, L, x.data(), pinfo, npar, xall);
+  return info;
+}
+
+// Levenberg-Marquardt on the normal equations, with J^T J and J^T r from the GPU.
+// The damped system (J^T J + lambda diag(J^T J)) p = -J^T r is solved by Cholesky;
+// lambda falls by 10 after an accepted step and rises by 10 after a rejected one.
+// Steps are shortened at the parameter limits as in mpfit, and limits whose
+// gradient points outwards pin their parameter.  Convergence: a relative chi^2
+// decrease below ftol on an accepted step, or a relative step below xtol.
+int ifg_fit_lm_normal(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+                      ifg_lm_result* res) {
+  const int npar = ifg_get_nparams(model);
+  double ftol = 1e-10, xtol = 1e-10;
+  int maxiter = 200;
+  if (cfg) {
+    if (cfg->ftol > 0) ftol = cfg->ftol;
+    if (cfg->xtol > 0) xtol = cfg->xtol;
+    if (cfg->maxiter > 0) maxiter = cfg->maxiter;
+  }
+  Limits L;
+  int st = SetupLimits(npar, xall, pinfo, L);
+  if (st) return st;
+  const int n = (int)L.ifree.size();
+  std::vector<double> jtj((size_t)n * n), jtr(n), A((size_t)n * n), p(n), x(n), wa2(n), xnew(xall, xall + npar);
+  for (int j = 0; j < n; j++) x[j] = xall[L.ifree[j]];
+  double chi2 = 0.0, orig = 0.0, lambda = 1e-3;
+  int nfev = 0, iter = 0, info = 0;
+  bool needJ = true;
+  while (info == 0) {
+    if (needJ) {
+      for (int j = 0; j < n; j++) xnew[L.ifree[j]] = x[j];
+      if ((st = ifg_normal_equations(model, xnew.data(), n, L.ifree.data(), jtj.data(), jtr.data(), &chi2)) < 0)
+        return st;
+      nfev++;
+      if (iter == 0) orig = chi2;
+      needJ = false;
+      iter++;
+      if (iter > maxiter) {
+        info = IFG_LM_MAXITER;
+        break;
+      }
+    }
+    // pin pegged parameters whose descent direction leaves the box
+    std::vector<char> pinned(n, 0);
+    if (L.any)
+      for (int j = 0; j < n; j++) {
+        const bool lp = L.qllim[j] && x[j] <= L.llim[j], up = L.qulim[j] && x[j] >= L.ulim[j];
+        if ((lp && jtr[j] > 0) || (up && jtr[j] < 0)) pinned[j] = 1;   // -J^T r points outwards
+      }
+    for (int a = 0; a < n; a++)
+      for (int b = 0; b < n; b++) {
+        double v = jtj[(size_t)a * n + b];
+        if (a == b) v += lambda * std::max(jtj[(size_t)a * n + a], 1e-300);
+        if (pinned[a] || pinned[b]) v = (a == b) ? 1.0 : 0.0;
+        A[(size_t)a * n + b] = v;
+      }
+    for (int a = 0; a < n; a++) p[a] = pinned[a] ? 0.0 : -jtr[a];
+    bool ok = true;   // Cholesky A = L L^T in place (lower triangle)
+    for (int j = 0; j < n && ok; j++) {
+      double d = A[(size_t)j * n + j];
+      for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+      if (!(d > 0.0)) {
+        ok = false;
+        break;
+      }
+      d = sqrt(d);
+      A[(size_t)j * n + j] = d;
+      for (int i = j + 1; i < n; i++) {
+        double s = A[(size_t)i * n + j];
+        for (int k = 0; k < j; k++) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+        A[(size_t)i * n + j] = s / d;
+      }
+    }
+    if (!ok) {
+      lambda *= 10.0;
+      if (lambda > 1e16) {
+        info = IFG_LM_GTOL;
+        break;
+      }
+      continue;
+    }
+    for (int i = 0; i < n; i++) {   // forward / back substitution
+      double s = p[i];
+      for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * p[k];
+      p[i] = s / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double s = p[i];
+      for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * p[k];
+      p[i] = s / A[(size_t)i * n + i];
+    }
+    LimitStep(L, x.data(), p.data(), wa2.data(), n);
+    for (int j = 0; j < n; j++) xnew[L.ifree[j]] = wa2[j];
+    double chi2new = 0.0;
+    if ((st = ifg_fit_statistic(model, xnew.data(), &chi2new)) < 0) return st;
+    nfev++;
+    double pn = 0.0, xn = 0.0;
+    for (int j = 0; j < n; j++) {
+      pn += p[j] * p[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
+      xn += x[j] * x[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
+    }
+    if (std::isfinite(chi2new) && chi2new < chi2) {
+      const double rel = (chi2 - chi2new) / chi2;
+      for (int j = 0; j < n; j++) x[j] = wa2[j];
+      chi2 = chi2new;
+      lambda = std::max(lambda * 0.1, 1e-12);
+      needJ = true;
+      if (rel <= ftol) info = IFG_LM_OK_CHI;
+      else if (sqrt(pn) <= xtol * sqrt(xn)) info = IFG_LM_OK_PAR;
+    } else {
+      lambda *= 10.0;
+      if (sqrt(pn) <= MACHEP0 * sqrt(xn) || lambda > 1e16) info = IFG_LM_OK_PAR;
+    }
+  }
+  for (int j = 0; j < n; j++) xall[L.ifree[j]] = x[j];
+  if (res) {
+    res->status = info;
+    res->niter = iter;
+    res->nfev = nfev;
+    res->nfree = n;
+    res->bestnorm = chi2;
+    res->orignorm = orig;
+    int npegged = 0;
+    if (pinfo)
+      for (int i = 0; i < npar; i++)
+        if ((pinfo[i].limited[0] && pinfo[i].limits[0] == xall[i]) ||
+            (pinfo[i].limited[1] && pinfo[i].limits[1] == xall[i]))
+          npegged++;
+    res->npegged = npegged;
+  }
+  return info;
+}
+
+}  // extern "C"

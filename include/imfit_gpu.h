@@ -120,11 +120,78 @@ int ifg_fit_statistic(ifg_model* m, const double* params, double* out);
  * (solvers/mpfit.cpp:1241-1343).  params: n_vectors x nParams, row-major. */
 int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, double* out);
 
+/* ---- derivatives (no counterpart in the reference: mpfit differentiates by
+ * finite differences, solvers/mpfit.cpp:1102-1356) ----------------------------
+ * Exact derivatives of the deviates w.r.t. the parameters: the analytic
+ * derivative of every image function (averaged over the same sub-pixel samples
+ * as the value), pushed through the PSF convolution and the oversampled
+ * regions by linearity.  ifree[j] lists the parameter of column j (NULL: the
+ * first n_free parameters); n_free <= 48. */
+/* Jacobian for mpfit's user-derivative hook (mp_par side == 3,
+ * solvers/mpfit.cpp:1208-1232): fjac column j at fjac + j*nData holds
+ * d(deviate)/d(param ifree[j]); fvec (nData, may be NULL) gets the deviates. */
+int ifg_jacobian(ifg_model* m, const double* params, int n_free, const int* ifree, double* fvec, double* fjac);
+/* Gauss-Newton normal equations reduced on the GPU in FP64: jtj = J^T J
+ * (n_free x n_free, row-major), jtr = J^T r with r the deviates, stat = the
+ * fit statistic at params. */
+int ifg_normal_equations(ifg_model* m, const double* params, int n_free, const int* ifree, double* jtj, double* jtr,
+                         double* stat);
+/* Gradient of GetFitStatistic w.r.t. all nParams parameters (and the statistic). */
+int ifg_fit_statistic_gradient(ifg_model* m, const double* params, double* grad, double* stat);
+
+/* ---- Levenberg-Marquardt callers ----------------------------------------------
+ * Parameter constraints: the fixed / limited / limits fields of mpfit's mp_par
+ * (core/param_struct.h:21-32).  NULL: every parameter free and unbounded. */
+typedef struct {
+  int fixed;
+  int limited[2];
+  double limits[2];
+} ifg_param_info;
+/* Tolerances (<= 0: mpfit's defaults 1e-10, solvers/mpfit.cpp:377-388; imfit passes
+ * ftol = 1e-8 and maxiter = 1000, solvers/levmar_fit.cpp:38,99-101). */
+typedef struct {
+  double ftol, xtol, gtol;
+  int maxiter, maxfev;
+} ifg_lm_config;
+/* mp_result subset (solvers/mpfit.h:152-166) */
+typedef struct {
+  double bestnorm, orignorm;
+  int status, niter, nfev, nfree, npegged;
+} ifg_lm_result;
+/* status values: mpfit's (solvers/mpfit.h:109-129) */
+#define IFG_LM_OK_CHI 1
+#define IFG_LM_OK_PAR 2
+#define IFG_LM_OK_BOTH 3
+#define IFG_LM_OK_DIR 4
+#define IFG_LM_MAXITER 5
+#define IFG_LM_FTOL 6
+#define IFG_LM_XTOL 7
+#define IFG_LM_GTOL 8
+#define IFG_LM_ERR_NAN (-16)
+#define IFG_LM_ERR_NFREE (-19)
+#define IFG_LM_ERR_INITBOUNDS (-21)
+#define IFG_LM_ERR_BOUNDS (-22)
+#define IFG_LM_ERR_DOF (-24)
+#define IFG_LM_JACOBIAN_FD 0        /* one GPU ComputeDeviates per free parameter (mp_fdjac2) */
+#define IFG_LM_JACOBIAN_ANALYTIC 1  /* ifg_jacobian (mpfit's side == 3 hook) */
+/* mpfit (solvers/mpfit.cpp:352-1097) on the GPU model: MINPACK LMDIF with QR on
+ * the host; params (nParams) is updated in place; returns the mpfit status. */
+int ifg_fit_lm(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+               int jacobian_mode, ifg_lm_result* res);
+/* Levenberg-Marquardt on the Gauss-Newton normal equations J^T J, J^T r reduced
+ * on the GPU in FP64 (ifg_normal_equations); no counterpart in the reference. */
+int ifg_fit_lm_normal(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+                      ifg_lm_result* res);
+
 /* ---- instrumentation (no counterpart in the reference, whose only timing is
  * wall-clock, core/makeimage_main.cpp:351-363) -------------------------------- */
 /* Record HIP events around every kernel launch of this handle (on its stream);
  * switching on clears the totals. */
 int ifg_set_kernel_timing(ifg_model* m, int on);
+/* Replay each evaluation shape from a captured HIP graph (default on; the
+ * environment variable IFG_NO_GRAPH=1 switches the default off).  Off: launch
+ * the same kernels eagerly every call. */
+int ifg_set_graph_mode(ifg_model* m, int on);
 /* Per-kernel totals: fills name / total milliseconds / launch count of entry
  * `index` and returns the number of entries (0 when index is out of range). */
 int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_len, double* total_ms,

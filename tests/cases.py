@@ -119,3 +119,15 @@ def float32_ulps(a, ref):
     ai = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
     ri = np.asarray(ref, np.float32).view(np.int32).astype(np.int64)
     return int(np.abs(ai - ri).max()) if ai.size else 0
+
+
+# Parameter limits of the reference's fit configurations
+LIMITS = {
+    # tests/imfit_reference/config_imfit_expdisk32.dat
+    "textout1": [(14, 16), (14, 16), (0, 180), (0, 1), (0, 1e6), (0, 1000)],
+    # tests/imfit_reference/imfit_config_ic3478_64x64b.dat
+    "textout3": [(1, 64), (1, 64), (0, 90), (0, 1), (0, 5), (0, 500), (0, 100)],
+    # tests/imfit_reference/imfit_config_n3073.dat
+    "n3073": [(90, 110), (90, 110), (0, 180), (0, 1), (0, 5), (0, 500), (0, 100), (0, 90), (0, 1), (0, 600),
+              (0, 1000)],
+}

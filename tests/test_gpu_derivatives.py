@@ -1,0 +1,126 @@
+"""Analytic derivatives on the GPU against finite differences of the CPU oracle.
+
+The sub-pixel sample count is a step function of the parameters (it jumps where
+int(20/r) changes), so per-pixel finite differences can straddle a jump; columns
+are therefore compared through their Euclidean norm: ||J_gpu - J_fd|| <= tol ||J_fd||.
+"""
+import numpy as np
+import pytest
+
+import cases
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_model(precision=0):
+    import imfit_amd
+    return imfit_amd.ModelObjectGPU(0, precision)
+
+
+def announce(msg):
+    print("[gpu-test]", msg, flush=True)
+
+
+def fd_jacobian(case, params, ifree, rel=1e-6):
+    """Central differences of the oracle's deviates, with steps h and h/2.  Pixels
+    where the two disagree straddle a jump of the sub-pixel count within the step;
+    they are flagged (mask False) instead of being compared."""
+    o = case.build(oracle.OracleModel(threads=8))
+    cols, masks = [], []
+    for j in ifree:
+        est = []
+        for h in (rel * max(abs(params[j]), 1.0), 0.5 * rel * max(abs(params[j]), 1.0)):
+            pp, pm = params.copy(), params.copy()
+            pp[j] += h
+            pm[j] -= h
+            est.append((o.ComputeDeviates(pp) - o.ComputeDeviates(pm)) / (2 * h))
+        scale = np.abs(est[0]).max() + 1e-300
+        masks.append(np.abs(est[0] - est[1]) <= 1e-6 * scale + 1e-4 * np.abs(est[0]))
+        cols.append(est[1])
+    return np.array(cols).T, np.array(masks).T
+
+
+def osamp_fit_case():
+    """The oversampled-PSF makeimage case turned into a fit: data = the oracle model
+    at shifted parameters, Poisson-like weights (gain 100, as imfit_textout4e)."""
+    case, _ = cases.oversampled(False)
+    o = case.build(oracle.OracleModel(threads=8))
+    truth = case.params.copy()
+    truth[0] += 0.3
+    truth[5] *= 1.1
+    o.CreateModelImage(truth)
+    data = o.GetModelImageVector() + 5.0
+    c = cases.Case(case.funcs + [("FlatSky", False)], list(case.params) + [4.0], data=data, psf=case.psf,
+                   osamp=case.osamp, chars=(100.0, 1.0, 1.0, 1, 0.0))
+    return c
+
+
+@pytest.mark.parametrize("which", ["n3073", "textout3", "osamp"])
+def test_jacobian_vs_central_differences(which):
+    announce("jacobian %s" % which)
+    if which == "n3073":
+        case, _ = cases.n3073()
+    elif which == "textout3":
+        case, _ = cases.textout3()
+    else:
+        case = osamp_fit_case()
+    p = case.params.astype(float)
+    g = case.build(gpu_model())
+    ifree = np.arange(p.size)
+    fvec, J = g.Jacobian(p, ifree)
+    o = case.build(oracle.OracleModel(threads=8))
+    assert np.allclose(fvec, o.ComputeDeviates(p), rtol=1e-12, atol=1e-12 * np.abs(fvec).max())
+    Jfd, ok = fd_jacobian(case, p, ifree)
+    for j in ifree:
+        sel = ok[:, j]
+        assert sel.mean() >= 0.999, (j, sel.mean())
+        num = np.linalg.norm(J[sel, j] - Jfd[sel, j])
+        den = np.linalg.norm(Jfd[sel, j])
+        assert num <= 2e-5 * den + 1e-12, (j, num, den, int((~sel).sum()))
+
+
+def test_normal_equations_match_jacobian():
+    announce("normal equations n3073")
+    case, _ = cases.n3073()
+    p = case.params.astype(float)
+    g = case.build(gpu_model())
+    ifree = np.array([0, 1, 3, 4, 5, 6, 8, 9, 10])
+    fvec, J = g.Jacobian(p, ifree)
+    jtj, jtr, chi2 = g.NormalEquations(p, ifree)
+    assert np.allclose(jtj, J.T @ J, rtol=1e-11, atol=1e-11 * np.abs(jtj).max())
+    assert np.allclose(jtr, J.T @ fvec, rtol=1e-11, atol=1e-11 * np.abs(jtr).max())
+    assert abs(chi2 - float(fvec @ fvec)) <= 1e-11 * chi2
+    # J is exactly linear in 1/weight: spot-check the chi^2 Gauss-Newton identity
+    # grad(chi^2) = 2 J^T r against the gradient entry point
+    grad, stat = g.GetFitStatisticGradient(p)
+    assert np.allclose(grad[ifree], 2 * jtr, rtol=1e-10, atol=1e-10 * np.abs(grad).max())
+    assert abs(stat - chi2) <= 1e-12 * chi2
+
+
+@pytest.mark.parametrize("stat", ["cash", "mlr"])
+def test_gradient_vs_central_differences(stat):
+    announce("gradient %s" % stat)
+    case, _ = cases.n3073(stat)
+    p = case.params.astype(float)
+    g = case.build(gpu_model())
+    grad, s0 = g.GetFitStatisticGradient(p)
+    o = case.build(oracle.OracleModel(threads=8))
+    assert abs(s0 - o.GetFitStatistic(p)) <= 1e-12 * abs(s0)
+    for j in range(p.size):
+        h = 1e-6 * max(abs(p[j]), 1.0)
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        fd = (o.GetFitStatistic(pp) - o.GetFitStatistic(pm)) / (2 * h)
+        assert abs(grad[j] - fd) <= 1e-4 * abs(fd) + 1e-6 * np.abs(grad).max(), (j, grad[j], fd)
+
+
+def test_fp32_jacobian_close_to_fp64():
+    announce("fp32 normal equations")
+    case, _ = cases.n3073()
+    p = case.params.astype(float)
+    a = case.build(gpu_model(0)).NormalEquations(p)
+    b = case.build(gpu_model(1)).NormalEquations(p)
+    assert np.allclose(b[0], a[0], rtol=1e-4, atol=1e-5 * np.abs(a[0]).max())
+    assert np.allclose(b[1], a[1], rtol=1e-3, atol=1e-4 * np.abs(a[1]).max())
