@@ -31,6 +31,22 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
 
+def copy_calibration(mb):
+    """A device-to-device copy of `mb` MiB (run by the runtime's blit kernel), whose
+    FETCH_SIZE / WRITE_SIZE counters calibrate the ones of our kernels."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d MiB device-to-device" % mb)
+
+
 def device_sync():
     hip = ctypes.CDLL("libamdhip64.so")
     if hip.hipDeviceSynchronize() != 0:
@@ -97,6 +113,31 @@ def cpu_baseline(w, seconds):
                       "host), OpenMP schedule(static,10) pixel loops" % (n, first)}
 
 
+def lm_benchmark(w, repeats):
+    import imfit_amd
+    m = w.fit_model(imfit_amd.PRECISION_FP64)
+    x0, lim = w.lm_start(), w.lm_limits()
+    log("LM: mpfit with finite-difference Jacobian (GPU evaluations, host QR)")
+    t0 = time.perf_counter()
+    x_fd, r_fd = m.FitLM(x0, limits=lim, ftol=1e-10, jacobian="fd")
+    t_fd = time.perf_counter() - t0
+    c_fd = m.GetFitStatistic(x_fd)
+    log("  %.3f s, status %d, niter %d, nfev %d, chi2 %.10f" % (t_fd, r_fd.status, r_fd.niter, r_fd.nfev, c_fd))
+    log("LM: normal equations with the analytic GPU Jacobian")
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        x_ne, r_ne = m.FitLM(x0, limits=lim, ftol=1e-12, method="normal")
+        ts.append(time.perf_counter() - t0)
+    c_ne = m.GetFitStatistic(x_ne)
+    t_ne = float(np.median(ts))
+    log("  %.4f s (median of %d), status %d, niter %d, chi2 %.10f" % (t_ne, repeats, r_ne.status, r_ne.niter, c_ne))
+    return {"fits_per_s_fd_mpfit": 1.0 / t_fd, "fits_per_s_normal_analytic": 1.0 / t_ne,
+            "speedup": t_fd / t_ne, "chi2_fd": c_fd, "chi2_normal": c_ne,
+            "chi2_rel_diff": (c_ne - c_fd) / c_fd, "niter_fd": r_fd.niter, "nfev_fd": r_fd.nfev,
+            "niter_normal": r_ne.niter, "start": "truth x U(0.9,1.1), centre +-5 px (seed 1)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +146,9 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lm", action="store_true")
+    ap.add_argument("--lm-repeats", type=int, default=3)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU sharding of the path is design-only "
@@ -174,6 +218,15 @@ def main():
         roof = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom, "avg_us": avg_s * 1e6}
 
+    # LM fits/s on the same workload: mpfit (MINPACK LMDIF) with a finite-difference
+    # Jacobian -- one GPU ComputeDeviates per free parameter, QR on the host, as the
+    # reference's caller does -- versus LM on the normal equations with the GPU's
+    # analytic Jacobian (J^T J, J^T r reduced on the device).  Both in FP64, compared
+    # at the chi^2 they reach.
+    lm = None
+    if not args.no_lm:
+        lm = lm_benchmark(w, args.lm_repeats)
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle baseline")
@@ -188,8 +241,10 @@ def main():
                       "precision": "%s pixels and FFT, FP64 deviate reduction" % args.precision,
                       "fp64_evals_per_s": fp64_rate, "ms_per_step_with_kernel_events": ms_timed,
                       "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm}
     print(json.dumps(out))
+    if args.copy_calibration_mb > 0:
+        copy_calibration(args.copy_calibration_mb)
 
 
 if __name__ == "__main__":

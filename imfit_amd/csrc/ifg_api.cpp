@@ -201,7 +201,7 @@ struct Plan {
   size_t recBytes = 0, boxBytes = 0, stageBytes = 0;
   void* pinnedStage = nullptr;
   void* pinnedOut = nullptr;
-  DevBuf stage_d, tail_d, pairs_d, slots_d, spec_d, partials_d, out_d, dev_d, model_d, osub_d, ovals_d;
+  DevBuf stage_d, tail_d, pairs_d, slots_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, osub_d, ovals_d;
   int nSlots = 1, pixPerBlock = 1024, ovStride = 0;
   long osubStride = 0;
   hipGraph_t graph = nullptr;
@@ -237,6 +237,8 @@ std::vector<double> NormalizedPSF(const double* psf, long n, bool normalize) {
 struct ifg_model {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // oversampled regions run here, overlapped with the main grid
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   std::string err;
   int precision = IFG_PRECISION_FP64;
   // model definition
@@ -282,6 +284,9 @@ struct ifg_model {
       (void)hipEventDestroy(pe.second.second);
     }
     plans.clear();
+    if (evFork) (void)hipEventDestroy(evFork);
+    if (evJoin) (void)hipEventDestroy(evJoin);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int fail(int code, const std::string& msg) {
@@ -336,21 +341,22 @@ void HarvestTimes(ifg_model* m) {
 
 // Launch `expr` (a hipError_t-returning launcher call), bracketed by events when
 // the handle's kernel timing is on.
-#define TIMED(name, expr, where)                                              \
+#define TIMED_ON(strm, name, expr, where)                                     \
   do {                                                                        \
     hipEvent_t _e0 = nullptr, _e1 = nullptr;                                  \
     if (m->timing) {                                                          \
       _e0 = TakeEvent(m);                                                     \
       _e1 = TakeEvent(m);                                                     \
-      (void)hipEventRecord(_e0, m->stream);                                   \
+      (void)hipEventRecord(_e0, strm);                                        \
     }                                                                         \
     hipError_t _err = (expr);                                                 \
     if (m->timing) {                                                          \
-      (void)hipEventRecord(_e1, m->stream);                                   \
+      (void)hipEventRecord(_e1, strm);                                        \
       m->curEvents->push_back({name, {_e0, _e1}});                            \
     }                                                                         \
     if (_err != hipSuccess) return m->hipfail(_err, where);                   \
   } while (0)
+#define TIMED(name, expr, where) TIMED_ON(m->stream, name, expr, where)
 
 FFTLen* GetFFT(ifg_model* m, int N, int* status) {
   auto it = m->ffts.find(N);
@@ -377,16 +383,21 @@ FFTLen* GetFFT(ifg_model* m, int N, int* status) {
   return raw;
 }
 
+// Rows per block of the row-FFT kernels: about 1024 points per block, within
+// 48 KiB of LDS for the two ping-pong buffers plus the twiddle table.
 int RowsPerBlock(int N, size_t elem) {
   int rows = std::max(1, 1024 / N);
-  while (rows > 1 && 2 * (size_t)rows * N * elem > 48 * 1024) rows--;
+  while (rows > 1 && (2 * (size_t)rows + 1) * N * elem > 48 * 1024) rows--;
   return rows;
 }
 
+// Columns per block of the column pass: a few columns each, so that the grid has
+// several blocks per CU (the planes fit in the last-level cache, which absorbs the
+// narrow column segments).
 int ColsPerBlock(int Nr, size_t elem) {
   const size_t budget = 56 * 1024;
-  int C = (int)(budget / (2 * (size_t)(Nr + 1) * elem));
-  return std::max(1, std::min(16, C));
+  int C = (int)((budget - (size_t)Nr * elem) / (2 * (size_t)(Nr + 1) * elem));
+  return std::max(1, std::min(4, C));
 }
 
 // Build one convolution geometry and its PSF spectrum (scaled by 1/(Nr*Nc)),
@@ -408,8 +419,14 @@ int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double
   g.xa = xa;
   g.ya = ya;
   g.frac = frac;
-  if (g.Nc > 4096 || g.Nr > 4096)
-    return m->fail(IFG_ERR_UNSUPPORTED, "image too large for the in-LDS FFT (padded side > 4096)");
+  int maxLds = 65536;
+  (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device);
+  // the PSF spectrum is always transformed in FP64: one row (two buffers + twiddles)
+  // or one column tile must fit in a workgroup's LDS
+  const size_t need = std::max((size_t)3 * g.Nc, (size_t)(2 * (g.Nr + 1) + g.Nr)) * sizeof(cpx<double>);
+  if (need > (size_t)maxLds)
+    return m->fail(IFG_ERR_UNSUPPORTED, "padded image side too large for the in-LDS FFT (needs " +
+                                            std::to_string(need) + " bytes of LDS per workgroup)");
   int st = 0;
   cg.fr = GetFFT(m, g.Nr, &st);
   if (!cg.fr) return st;
@@ -613,10 +630,11 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
 
   // chunks of the batch, so that the complex planes stay within ~1 GiB
-  size_t pairBytes = 0;
+  size_t pairBytes = 0, pairBytesR = 0;
   if (m->doConv) pairBytes = (size_t)P.geoms[0].Nr * P.geoms[0].Nc * sizeof(cpx<T>);
   for (int k = 0; k < nR; k++)
-    pairBytes = std::max(pairBytes, (size_t)P.geoms[1 + k].Nr * P.geoms[1 + k].Nc * sizeof(cpx<T>));
+    pairBytesR = std::max(pairBytesR, (size_t)P.geoms[1 + k].Nr * P.geoms[1 + k].Nc * sizeof(cpx<T>));
+  pairBytes = std::max(pairBytes, pairBytesR);
   const int nPairsAll = P.single ? 1 : (nVec + 1) / 2;
   P.chunkPairs = nPairsAll;
   if (pairBytes > 0)
@@ -625,6 +643,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   P.nChunks = (nVec + P.chunkVecs - 1) / P.chunkVecs;
   const long nData = m->nData;
   HIPCHK(P.spec_d.ensure(std::max<size_t>(1, pairBytes * P.chunkPairs)), "alloc planes");
+  if (nR) HIPCHK(P.specR_d.ensure(std::max<size_t>(1, pairBytesR * P.chunkPairs)), "alloc region planes");
   const int nFree = nVec;
   const int nNormal = nFree * (nFree + 1) / 2 + nFree;
   HIPCHK(P.out_d.ensure(sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0)), "alloc results");
@@ -783,16 +802,22 @@ int Enqueue(ifg_model* m, Plan& P) {
     const int4* pairsC = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
     const int2* slotsC = P.slots_d.as<int2>() + (size_t)c * P.chunkVecs;
 
-    // 1. sub-pixel tails of every grid
-    for (int gi = 0; gi < P.nGrids; gi++)
-      if (P.strideMax[gi] > 0)
-        TIMED(gi == 0 ? "k_tail" : "k_tail_region", launch_tail<T>(s, evalArgs(gi, v0), P.geoms[gi], nv, subsampling),
-              "k_tail");
-
-    // 2. oversampled regions: evaluate, convolve with their PSF, block-average
+    // The oversampled regions (tail, evaluate, convolve with their PSF, block-average)
+    // run on the second stream while the main grid is evaluated and convolved; the
+    // main statistic pass waits for them.
+    hipStream_t s2 = m->stream2;
+    if (nR) {
+      HIPCHK(hipEventRecord(m->evFork, s), "fork");
+      HIPCHK(hipStreamWaitEvent(s2, m->evFork, 0), "fork wait");
+    }
+    if (P.strideMax[0] > 0)
+      TIMED("k_tail", launch_tail<T>(s, evalArgs(0, v0), P.geoms[0], nv, subsampling), "k_tail");
     StatArgs<T> sa;
     std::memset(&sa, 0, sizeof(sa));
     for (int k = 0; k < nR; k++) {
+      if (P.strideMax[1 + k] > 0)
+        TIMED_ON(s2, "k_tail_region", launch_tail<T>(s2, evalArgs(1 + k, v0), P.geoms[1 + k], nv, subsampling),
+                 "k_tail");
       Region& R = *m->regions[k];
       ConvGrid& cg = R.conv[which];
       const GridGeom& g = cg.g;
@@ -802,12 +827,13 @@ int Enqueue(ifg_model* m, Plan& P) {
       const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
       const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
       const int4* pairsR = pairsC + (size_t)(1 + k) * P.npc;
-      TIMED("k_eval_rowfft_region",
-            launch_eval_rowfft<T>(s, evalArgs(1 + k, v0), g, pairsR, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan,
-                                  tw_c, cg.rowsK1),
-            "region k_eval_rowfft");
-      TIMED("k_colconv_region", launch_colconv<T>(s, g, P.spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
-            "region k_colconv");
+      TIMED_ON(s2, "k_eval_rowfft_region",
+               launch_eval_rowfft<T>(s2, evalArgs(1 + k, v0), g, pairsR, np, P.specR_d.as<cpx<T>>(), pstride,
+                                     cg.fc->plan, tw_c, cg.rowsK1),
+               "region k_eval_rowfft");
+      TIMED_ON(s2, "k_colconv_region",
+               launch_colconv<T>(s2, g, P.specR_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
+               "region k_colconv");
       StatArgs<T> so;
       std::memset(&so, 0, sizeof(so));
       so.mode = S_NONE;
@@ -815,14 +841,14 @@ int Enqueue(ifg_model* m, Plan& P) {
       so.H = g.H;
       so.nData = (long)g.W * g.H;
       so.modelOut = P.osub_d.as<double>();
-      TIMED("k_rowifft_region",
-            launch_rowifft_stat<T>(s, g, pairsR, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, so, cg.rowsK3,
-                                   agiant),
-            "region k_rowifft");
-      TIMED("k_downsample",
-            launch_downsample(s, P.osub_d.as<double>(), g.W, (long)g.W * g.H, P.ovals_d.as<double>(), P.ovStride,
-                              R.ovOff, R.nbc, R.rowsReplaced, R.scale, nv),
-            "k_downsample");
+      TIMED_ON(s2, "k_rowifft_region",
+               launch_rowifft_stat<T>(s2, g, pairsR, np, P.specR_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, so,
+                                      cg.rowsK3, agiant),
+               "region k_rowifft");
+      TIMED_ON(s2, "k_downsample",
+               launch_downsample(s2, P.osub_d.as<double>(), g.W, (long)g.W * g.H, P.ovals_d.as<double>(),
+                                 P.ovStride, R.ovOff, R.nbc, R.rowsReplaced, R.scale, nv),
+               "k_downsample");
       RegionOut ro;
       std::memset(&ro, 0, sizeof(ro));
       ro.c0 = R.x1 - 1;
@@ -872,11 +898,19 @@ int Enqueue(ifg_model* m, Plan& P) {
             "k_eval_rowfft");
       TIMED("k_colconv", launch_colconv<T>(s, g, P.spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
             "k_colconv");
+      if (nR) {
+        HIPCHK(hipEventRecord(m->evJoin, s2), "join");
+        HIPCHK(hipStreamWaitEvent(s, m->evJoin, 0), "join wait");
+      }
       TIMED("k_rowifft_stat",
             launch_rowifft_stat<T>(s, g, pairsC, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, sa, cg.rowsK3,
                                    agiant),
             "k_rowifft_stat");
     } else {
+      if (nR) {
+        HIPCHK(hipEventRecord(m->evJoin, s2), "join");
+        HIPCHK(hipStreamWaitEvent(s, m->evJoin, 0), "join wait");
+      }
       TIMED("k_eval_stat", launch_eval_stat<T>(s, evalArgs(0, v0), P.geoms[0], sa, nv, P.pixPerBlock, agiant),
             "k_eval_stat");
     }
@@ -1036,9 +1070,11 @@ ifg_model* ifg_create(int device) {
   m->curEvents = &m->pendingEvents;
   const char* ng = getenv("IFG_NO_GRAPH");
   if (ng && ng[0] == '1') m->useGraphs = false;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess ||
+      hipStreamCreate(&m->stream2) != hipSuccess ||
+      hipEventCreateWithFlags(&m->evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->evJoin, hipEventDisableTiming) != hipSuccess) {
     m->err = "no HIP device / stream";
-    m->stream = nullptr;
   }
   return m;
 }

@@ -91,17 +91,21 @@ __device__ __forceinline__ void dft5(cpx<T>* a, int dir) {
 // (sequence s at src + s*ld): butterfly j of sub-transform size Ns reads
 // x[j + r*N/R], twiddles by w^(k*r) with k = j mod Ns, and writes the R outputs
 // to ((j-k)*R + k) + m*Ns.  After all stages the output is in natural order.
+// tw is the length-N twiddle table staged in LDS.  Quotients use a float
+// reciprocal: for b < 2^16 and divisors <= 4096, (b + 0.5) / d sits at least
+// 0.5/d from an integer, far beyond the float rounding of the product.
 template <typename T, int R>
-__device__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ dst, int N, int Ns, int nseq,
-                          int ld, const cpx<T>* __restrict__ tw, int dir) {
+__device__ __forceinline__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ dst, int N, int Ns,
+                                          int nseq, int ld, const cpx<T>* __restrict__ tw, int dir) {
   const int nb = N / R;
   const int total = nb * nseq;
   const int twstep = N / (Ns * R);
+  const float inv_nb = 1.0f / (float)nb, inv_ns = 1.0f / (float)Ns;
   for (int b = threadIdx.x; b < total; b += blockDim.x) {
-    const int seq = b / nb;
+    const int seq = (int)(((float)b + 0.5f) * inv_nb);
     const int j = b - seq * nb;
-    const int k = j % Ns;
-    const cpx<T>* s = src + (long)seq * ld;
+    const int k = j - Ns * (int)(((float)j + 0.5f) * inv_ns);
+    const cpx<T>* s = src + seq * ld;
     cpx<T> a[R];
 #pragma unroll
     for (int r = 0; r < R; r++) a[r] = s[j + r * nb];
@@ -117,7 +121,7 @@ __device__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ d
     if (R == 3) dft3(a, dir);
     if (R == 4) dft4(a, dir);
     if (R == 5) dft5(a, dir);
-    cpx<T>* d = dst + (long)seq * ld + (j - k) * R + k;
+    cpx<T>* d = dst + seq * ld + (j - k) * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++) d[r * Ns] = a[r];
   }
@@ -142,6 +146,12 @@ __device__ cpx<T>* fft_lds(cpx<T>* A, cpx<T>* B, const FFTPlan& pl, int nseq, in
     Ns *= R;
   }
   return A;
+}
+
+// Copy the length-N twiddle table into LDS (caller synchronises).
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __restrict__ tw, int N) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = tw[i];
 }
 
 // ------------------------------------------------------------------ image functions
@@ -455,6 +465,8 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(EvalArgs ea, GridGeom g, co
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + rowsPerBlock * g.Nc;
+  cpx<T>* TW = B + rowsPerBlock * g.Nc;
+  stage_twiddles(TW, tw, g.Nc);
   const int4 pr = pairs[blockIdx.y];
   const int ir0 = blockIdx.x * rowsPerBlock;
   const int total = rowsPerBlock * g.Nc;
@@ -469,7 +481,7 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(EvalArgs ea, GridGeom g, co
     A[e] = cpx<T>{va, vb};
   }
   __syncthreads();
-  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, tw, +1);
+  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, TW, +1);
   cpx<T>* S = spec + (long)blockIdx.y * pairStride;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int rr = e / g.Nc, ir = ir0 + rr;
@@ -484,6 +496,8 @@ __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, 
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + rowsPerBlock * Nc;
+  cpx<T>* TW = B + rowsPerBlock * Nc;
+  stage_twiddles(TW, tw, Nc);
   const int ir0 = blockIdx.x * rowsPerBlock;
   const int total = rowsPerBlock * Nc;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -491,7 +505,7 @@ __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, 
     A[e] = (ir < nRows) ? buf[(long)ir0 * Nc + e] : cpx<T>{0, 0};
   }
   __syncthreads();
-  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, Nc, tw, dir);
+  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, Nc, TW, dir);
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int ir = ir0 + e / Nc;
     if (ir < nRows) buf[(long)ir0 * Nc + e] = out[e];
@@ -509,6 +523,8 @@ __global__ __launch_bounds__(256) void k_colconv(GridGeom g, cpx<T>* __restrict_
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + C * ld;
+  cpx<T>* TW = B + C * ld;
+  stage_twiddles(TW, tw, g.Nr);
   const int c0 = blockIdx.x * C;
   cpx<T>* S = spec + (long)blockIdx.y * pairStride;
   const int Nr = g.Nr, Nc = g.Nc;
@@ -520,7 +536,7 @@ __global__ __launch_bounds__(256) void k_colconv(GridGeom g, cpx<T>* __restrict_
     A[cc * ld + i] = v;
   }
   __syncthreads();
-  cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, tw, +1);
+  cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, TW, +1);
   cpx<T>* Y = (X == A) ? B : A;
   if (mode == 0) {
     for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
@@ -528,7 +544,7 @@ __global__ __launch_bounds__(256) void k_colconv(GridGeom g, cpx<T>* __restrict_
       if (col < Nc) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
     }
     __syncthreads();
-    X = fft_lds<T>(X, Y, pl, C, ld, tw, -1);
+    X = fft_lds<T>(X, Y, pl, C, ld, TW, -1);
     for (int e = threadIdx.x; e < g.Hh * C; e += blockDim.x) {
       const int i = g.hb + e / C, cc = e % C, col = c0 + cc;
       if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
@@ -710,6 +726,8 @@ __global__ __launch_bounds__(256) void k_rowifft_stat(GridGeom g, const int4* __
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + rowsPerBlock * g.Nc;
+  cpx<T>* TW = B + rowsPerBlock * g.Nc;
+  stage_twiddles(TW, tw, g.Nc);
   const int4 pr = pairs[blockIdx.y];
   const int io0 = blockIdx.x * rowsPerBlock;
   const int total = rowsPerBlock * g.Nc;
@@ -719,7 +737,7 @@ __global__ __launch_bounds__(256) void k_rowifft_stat(GridGeom g, const int4* __
     A[e] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + (e - rr * g.Nc)] : cpx<T>{0, 0};
   }
   __syncthreads();
-  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, tw, -1);
+  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, TW, -1);
   ENorm ea, eb;
   en_init(ea);
   en_init(eb);
@@ -882,7 +900,8 @@ __global__ __launch_bounds__(256) void k_normal_final(const double* __restrict__
 // ====================================================================== launchers
 namespace ifg {
 
-static size_t smem_rows(int rows, int N, size_t elem) { return 2 * (size_t)rows * N * elem; }
+// two ping-pong row buffers + the twiddle table
+static size_t smem_rows(int rows, int N, size_t elem) { return (2 * (size_t)rows + 1) * N * elem; }
 
 template <typename T>
 hipError_t launch_tail(hipStream_t s, const EvalArgs& ea, const GridGeom& g, int nVec, int sub) {
@@ -915,7 +934,8 @@ hipError_t launch_colconv(hipStream_t s, const GridGeom& g, cpx<T>* spec, long p
                           const cpx<T>* khat, const FFTPlan& pl, const cpx<T>* tw, int C, int mode) {
   const int ld = g.Nr + 1;
   dim3 grid((g.Nc + C - 1) / C, nPairs);
-  hipLaunchKernelGGL(k_colconv<T>, grid, dim3(256), 2 * (size_t)C * ld * sizeof(cpx<T>), s, g, spec, pairStride,
+  hipLaunchKernelGGL(k_colconv<T>, grid, dim3(256), (2 * (size_t)C * ld + g.Nr) * sizeof(cpx<T>), s, g, spec,
+                     pairStride,
                      khat, pl, tw, C, ld, mode);
   return hipGetLastError();
 }

@@ -77,6 +77,46 @@ class Workload:
         rng = np.random.default_rng(seed)
         return self.truth * rng.uniform(1 - frac, 1 + frac, size=self.truth.size)
 
+    def lm_start(self, seed=1):
+        """LM starting point: every non-positional parameter scaled by U(0.9, 1.1), the
+        centre shifted by U(-5, 5) pixels (seed 1), inside lm_limits()."""
+        rng = np.random.default_rng(seed)
+        x = self.truth * rng.uniform(0.9, 1.1, size=self.truth.size)
+        off = 0
+        for name, new in self.funcs:
+            if new:
+                x[off:off + 2] = self.truth[off:off + 2] + rng.uniform(-5, 5, size=2)
+                off += 2
+            off += {"Sersic": 5, "Exponential": 4, "FlatSky": 1, "Gaussian": 4, "Moffat": 5,
+                    "ExponentialDisk3D": 6}[name]
+        return x
+
+    def lm_limits(self):
+        """Parameter limits for the LM fits: centres within 20 px of the truth, PA in
+        [-360, 360], ellipticities in [0, 0.95], Sersic n in [0.2, 10], the rest in
+        [truth/10, truth*10]."""
+        lim = []
+        off = 0
+        for name, new in self.funcs:
+            if new:
+                lim += [(self.truth[off] - 20, self.truth[off] + 20), (self.truth[off + 1] - 20,
+                                                                        self.truth[off + 1] + 20)]
+                off += 2
+            n = {"Sersic": 5, "Exponential": 4, "FlatSky": 1, "Gaussian": 4, "Moffat": 5,
+                 "ExponentialDisk3D": 6}[name]
+            for k in range(n):
+                t = self.truth[off + k]
+                if name != "FlatSky" and k == 0:
+                    lim.append((-360.0, 360.0))
+                elif name != "FlatSky" and k == 1:
+                    lim.append((0.0, 0.95))
+                elif name == "Sersic" and k == 2:
+                    lim.append((0.2, 10.0))
+                else:
+                    lim.append((t / 10.0, t * 10.0))
+            off += n
+        return lim
+
 
 def moffat_psf(size, fwhm, beta):
     m = ModelObjectGPU(0, PRECISION_FP64)

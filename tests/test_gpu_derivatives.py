@@ -22,12 +22,15 @@ def announce(msg):
     print("[gpu-test]", msg, flush=True)
 
 
-def fd_jacobian(case, params, ifree, rel=1e-6):
+def fd_jacobian(case, params, ifree, rel=1e-8):
     """Central differences of the oracle's deviates, with steps h and h/2.  Pixels
     where the two disagree straddle a jump of the sub-pixel count within the step;
-    they are flagged (mask False) instead of being compared."""
+    they are flagged (mask False) instead of being compared.  The step is small
+    (1e-8 relative) because a PSF -- the 105^2 oversampled one in particular --
+    spreads a single sub-pixel's jump over every pixel it reaches; FP64 rounding of
+    the deviates still leaves the difference quotients accurate to ~1e-7."""
     o = case.build(oracle.OracleModel(threads=8))
-    cols, masks = [], []
+    ests = []
     for j in ifree:
         est = []
         for h in (rel * max(abs(params[j]), 1.0), 0.5 * rel * max(abs(params[j]), 1.0)):
@@ -35,10 +38,10 @@ def fd_jacobian(case, params, ifree, rel=1e-6):
             pp[j] += h
             pm[j] -= h
             est.append((o.ComputeDeviates(pp) - o.ComputeDeviates(pm)) / (2 * h))
-        scale = np.abs(est[0]).max() + 1e-300
-        masks.append(np.abs(est[0] - est[1]) <= 1e-6 * scale + 1e-4 * np.abs(est[0]))
-        cols.append(est[1])
-    return np.array(cols).T, np.array(masks).T
+        ests.append(est)
+    scale = max(np.abs(e[0]).max() for e in ests) + 1e-300   # one scale for all columns
+    masks = [np.abs(e[0] - e[1]) <= 1e-5 * scale + 1e-3 * np.abs(e[0]) for e in ests]
+    return np.array([e[1] for e in ests]).T, np.array(masks).T
 
 
 def osamp_fit_case():
@@ -72,12 +75,13 @@ def test_jacobian_vs_central_differences(which):
     o = case.build(oracle.OracleModel(threads=8))
     assert np.allclose(fvec, o.ComputeDeviates(p), rtol=1e-12, atol=1e-12 * np.abs(fvec).max())
     Jfd, ok = fd_jacobian(case, p, ifree)
+    floor = 1e-9 * np.linalg.norm(Jfd)   # columns that vanish (e.g. PA of a round profile)
     for j in ifree:
         sel = ok[:, j]
         assert sel.mean() >= 0.999, (j, sel.mean())
         num = np.linalg.norm(J[sel, j] - Jfd[sel, j])
         den = np.linalg.norm(Jfd[sel, j])
-        assert num <= 2e-5 * den + 1e-12, (j, num, den, int((~sel).sum()))
+        assert num <= 2e-5 * den + floor, (j, num, den, int((~sel).sum()))
 
 
 def test_normal_equations_match_jacobian():
