@@ -61,37 +61,51 @@ def fft_flops(n):
     return 5.0 * n * math.log2(n)
 
 
+def smooth(n):
+    """Smallest n' >= n whose only prime factors are 2, 3 and 5 (the FFT lengths)."""
+    while True:
+        k = n
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return n
+        n += 1
+
+
+def grid_geometry(W, H, P, split_multiple=1):
+    """Convolution plane of one single-image grid: the evaluated extent is the output
+    plus the PSF halo (W+P-1 columns), the two halves of the image (split row rounded
+    up to a multiple of `split_multiple`) share one complex plane of Nr x Nc."""
+    Hh = -(-((H + 1) // 2) // split_multiple) * split_multiple
+    Lc, Lr = W + P - 1, Hh + P - 1
+    return Lc, Lr, smooth(Lc), smooth(Lr), Hh
+
+
 def kernel_model(w, prec_bytes):
-    """Algorithmic work of the main-grid kernels of one evaluation.
+    """Algorithmic HBM bytes and flops per launch of the merged kernels of one
+    evaluation (main grid + every oversampled region grid), FP32 or FP64 pixels.
 
-    Work counts follow the data-layout of the path: the main grid is evaluated on
-    the data region plus a PSF halo (W+P-1 columns, H+P-1 rows); two real half-images
-    share one complex plane of Nr x Nc, padded to 2^a 3^b 5^c.
-    """
-    W, H, P = w.width, w.height, w.psf.shape[0]
-    Lc, Hh = W + P - 1, (H + 1) // 2
-    Lr = Hh + P - 1
-
-    def smooth(n):
-        while True:
-            k = n
-            for p in (2, 3, 5):
-                while k % p == 0:
-                    k //= p
-            if k == 1:
-                return n
-            n += 1
-    Nc, Nr = smooth(Lc), smooth(Lr)
-    cb = 2 * prec_bytes   # complex element
-    return {
-        # spectrum rows written by the fused evaluate + row-FFT kernel
-        "k_eval_rowfft": {"bytes": Lr * Nc * cb, "flops": Lr * fft_flops(Nc)},
-        # column FFT in, multiply by the PSF spectrum, column FFT back
-        "k_colconv": {"bytes": (Lr * Nc + Nr * Nc + Hh * Nc) * cb,
-                      "flops": Nc * 2 * fft_flops(Nr) + 6 * Nr * Nc},
-        # spectrum rows + data + weights in
-        "k_rowifft_stat": {"bytes": Hh * Nc * cb + W * H * 2 * prec_bytes, "flops": Hh * fft_flops(Nc)},
-    }
+    k_eval_rowfft writes the Lr evaluated rows of each plane (Nc complex each);
+    k_colconv reads them, reads the PSF spectrum (Nr x Nc) and writes the Hh output
+    rows; k_inverse_stat reads those rows plus the data and weight images.  The
+    sub-pixel tail kernel is compute work (tens of kB of output)."""
+    cb = 2 * prec_bytes
+    grids = [(w.width, w.height, w.psf.shape[0], 1)]
+    for (psf, scale, x1, x2, y1, y2) in w.osamp:
+        grids.append(((x2 - x1 + 1) * scale, (y2 - y1 + 1) * scale, psf.shape[0], scale))
+    out = {"k_eval_rowfft": {"bytes": 0.0, "flops": 0.0}, "k_colconv": {"bytes": 0.0, "flops": 0.0},
+           "k_inverse_stat": {"bytes": 0.0, "flops": 0.0}}
+    for (W, H, P, s) in grids:
+        Lc, Lr, Nc, Nr, Hh = grid_geometry(W, H, P, s)
+        out["k_eval_rowfft"]["bytes"] += Lr * Nc * cb
+        out["k_eval_rowfft"]["flops"] += Lr * fft_flops(Nc)
+        out["k_colconv"]["bytes"] += (Lr + Nr + Hh) * Nc * cb
+        out["k_colconv"]["flops"] += Nc * 2 * fft_flops(Nr) + 6 * Nr * Nc
+        out["k_inverse_stat"]["bytes"] += Hh * Nc * cb
+        out["k_inverse_stat"]["flops"] += Hh * fft_flops(Nc)
+    out["k_inverse_stat"]["bytes"] += w.width * w.height * 2 * prec_bytes   # data + weights
+    return out
 
 
 def cpu_baseline(w, seconds):

@@ -185,7 +185,7 @@ struct ConvGrid {
 };
 
 struct Region {
-  int x1, y1, nbc, nbr, scale, pc, pr, rowsReplaced, ovOff;
+  int x1, y1, nbc, nbr, scale, pc, pr, rowsReplaced;
   std::vector<double> psf;   // normalised
   ConvGrid conv[2];          // [0] single (halves), [1] batch (pairs)
 };
@@ -201,9 +201,10 @@ struct Plan {
   size_t recBytes = 0, boxBytes = 0, stageBytes = 0;
   void* pinnedStage = nullptr;
   void* pinnedOut = nullptr;
-  DevBuf stage_d, tail_d, pairs_d, slots_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, osub_d, ovals_d;
-  int nSlots = 1, pixPerBlock = 1024, ovStride = 0;
-  long osubStride = 0;
+  DevBuf stage_d, tail_d, pairs_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, tasks_d;
+  int nSlots = 1, nTasks = 1, nB1 = 0, nB2 = 0, nB3 = 0;
+  size_t lds1 = 0, lds2 = 0, lds3 = 0;
+  long tailWaves = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   Plan() = default;
@@ -237,8 +238,6 @@ std::vector<double> NormalizedPSF(const double* psf, long n, bool normalize) {
 struct ifg_model {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;   // oversampled regions run here, overlapped with the main grid
-  hipEvent_t evFork = nullptr, evJoin = nullptr;
   std::string err;
   int precision = IFG_PRECISION_FP64;
   // model definition
@@ -284,9 +283,6 @@ struct ifg_model {
       (void)hipEventDestroy(pe.second.second);
     }
     plans.clear();
-    if (evFork) (void)hipEventDestroy(evFork);
-    if (evJoin) (void)hipEventDestroy(evJoin);
-    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int fail(int code, const std::string& msg) {
@@ -402,8 +398,11 @@ int ColsPerBlock(int Nr, size_t elem) {
 
 // Build one convolution geometry and its PSF spectrum (scaled by 1/(Nr*Nc)),
 // computed in FP64 on the device with the same FFT kernels, FP32 copy derived.
+// splitMultiple: the row where a single image is split into its two half planes is
+// rounded up to a multiple of this (the region's oversampling scale, so that every
+// s x s block of sub-pixels lies within one half).
 int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double>& psfn, int Pc, int Pr, bool halves,
-              double xa, double ya, double frac) {
+              double xa, double ya, double frac, int splitMultiple) {
   GridGeom& g = cg.g;
   std::memset(&g, 0, sizeof(g));
   const int cx = Pc / 2, cy = Pr / 2;
@@ -411,7 +410,7 @@ int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double
   g.H = H;
   g.hl = Pc - 1 - cx;
   g.hb = Pr - 1 - cy;
-  g.Hh = halves ? (H + 1) / 2 : H;
+  g.Hh = halves ? ((((H + 1) / 2) + splitMultiple - 1) / splitMultiple) * splitMultiple : H;
   g.Lc = W + Pc - 1;
   g.Lr = g.Hh + Pr - 1;
   g.Nc = NextSmooth(g.Lc);
@@ -447,16 +446,21 @@ int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double
   HIPCHK(launch_rowfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fc->plan,
                                cg.fc->tw64.as<cpx<double>>(), +1, rows),
          "PSF row FFT");
-  HIPCHK(launch_colconv<double>(m->stream, g, cg.khat64.as<cpx<double>>(), 0, 1, nullptr, cg.fr->plan,
-                                cg.fr->tw64.as<cpx<double>>(), ColsPerBlock(g.Nr, sizeof(cpx<double>)), 1),
+  HIPCHK(launch_colfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fr->plan,
+                               cg.fr->tw64.as<cpx<double>>(), ColsPerBlock(g.Nr, sizeof(cpx<double>))),
          "PSF column FFT");
   HIPCHK(hipStreamSynchronize(m->stream), "PSF spectrum");
   HIPCHK(hipMemcpy(k.data(), cg.khat64.p, NN * sizeof(cpx<double>), hipMemcpyDeviceToHost), "PSF spectrum D2H");
   std::vector<cpx<float>> k32(NN);
   for (long z = 0; z < NN; z++) k32[z] = {(float)k[z].x, (float)k[z].y};
   HIPCHK(hipMemcpy(cg.khat32.p, k32.data(), NN * sizeof(cpx<float>), hipMemcpyHostToDevice), "PSF spectrum FP32");
-  cg.rowsK1 = RowsPerBlock(g.Nc, m->precision == IFG_PRECISION_FP64 ? sizeof(cpx<double>) : sizeof(cpx<float>));
-  cg.rowsK3 = cg.rowsK1;
+  const size_t elem = m->precision == IFG_PRECISION_FP64 ? sizeof(cpx<double>) : sizeof(cpx<float>);
+  cg.rowsK1 = RowsPerBlock(g.Nc, elem);
+  // inverse-pass rows per block: whole groups of splitMultiple rows (regions average
+  // s x s sub-pixel blocks inside one workgroup)
+  cg.rowsK3 = std::max(1, cg.rowsK1 / splitMultiple) * splitMultiple;
+  if ((2 * (size_t)cg.rowsK3 + 1) * g.Nc * elem > (size_t)maxLds)
+    return m->fail(IFG_ERR_UNSUPPORTED, "oversampled region too wide for one workgroup's LDS");
   cg.C64 = ColsPerBlock(g.Nr, sizeof(cpx<double>));
   cg.C32 = ColsPerBlock(g.Nr, sizeof(cpx<float>));
   cg.ready = true;
@@ -551,11 +555,6 @@ int PrepareDevice(ifg_model* m) {
     int st = (m->precision == IFG_PRECISION_FP64) ? UploadImageArrays<double>(m) : UploadImageArrays<float>(m);
     if (st < 0) return st;
   }
-  int ovOff = 0;
-  for (auto& R : m->regions) {
-    R->ovOff = ovOff;
-    ovOff += R->nbc * R->rowsReplaced;
-  }
   m->devReady = true;
   return IFG_OK;
 }
@@ -563,7 +562,7 @@ int PrepareDevice(ifg_model* m) {
 int EnsureMainConv(ifg_model* m, int which) {
   ConvGrid& cg = m->mainConv[which];
   if (cg.ready) return IFG_OK;
-  return BuildConv(m, cg, m->W, m->H, m->psf, m->Pc, m->Pr, which == 0, 1.0, 1.0, 1.0);
+  return BuildConv(m, cg, m->W, m->H, m->psf, m->Pc, m->Pr, which == 0, 1.0, 1.0, 1.0, 1);
 }
 
 int EnsureRegionConv(ifg_model* m, Region& R, int which) {
@@ -572,7 +571,7 @@ int EnsureRegionConv(ifg_model* m, Region& R, int which) {
   const double frac = 1.0 / R.scale;
   const double off = 0.5 * frac - 0.5;   // core/oversampled_region.cpp:223-226
   return BuildConv(m, cg, R.nbc * R.scale, R.nbr * R.scale, R.psf, R.pc, R.pr, which == 0, (double)R.x1 + off,
-                   (double)R.y1 + off, frac);
+                   (double)R.y1 + off, frac, R.scale);
 }
 
 // ---------------------------------------------------------------- evaluation plans
@@ -609,7 +608,11 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     if ((st = EnsureRegionConv(m, *R, P.which)) < 0) return st;
   P.nGrids = 1 + nR;
   P.geoms.assign(P.nGrids, GridGeom());
-  P.geoms[0] = m->doConv ? m->mainConv[P.which].g : PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
+  if (m->doConv) P.geoms[0] = m->mainConv[P.which].g;
+  else {
+    P.geoms[0] = PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
+    P.geoms[0].Hh = P.single ? (m->H + 1) / 2 : m->H;
+  }
   for (int k = 0; k < nR; k++) P.geoms[1 + k] = m->regions[k]->conv[P.which].g;
   P.strideMax.assign(P.nGrids, 0);
   P.tailOff.assign(P.nGrids + 1, 0);
@@ -625,20 +628,23 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   P.boxBytes = (size_t)P.nGrids * nRecVec * IFG_MAXBOX * sizeof(Box);
   P.stageBytes = P.recBytes + P.boxBytes + 64 * sizeof(int);
   HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
-  HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * nVec, hipHostMallocDefault), "pinned results");
   HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
   HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
 
   // chunks of the batch, so that the complex planes stay within ~1 GiB
   size_t pairBytes = 0, pairBytesR = 0;
+  std::vector<size_t> regOff(nR + 1, 0);
   if (m->doConv) pairBytes = (size_t)P.geoms[0].Nr * P.geoms[0].Nc * sizeof(cpx<T>);
-  for (int k = 0; k < nR; k++)
-    pairBytesR = std::max(pairBytesR, (size_t)P.geoms[1 + k].Nr * P.geoms[1 + k].Nc * sizeof(cpx<T>));
-  pairBytes = std::max(pairBytes, pairBytesR);
+  for (int k = 0; k < nR; k++) {
+    const size_t b = (size_t)P.geoms[1 + k].Nr * P.geoms[1 + k].Nc * sizeof(cpx<T>);
+    regOff[k + 1] = regOff[k] + b;
+  }
+  pairBytesR = regOff[nR];
   const int nPairsAll = P.single ? 1 : (nVec + 1) / 2;
   P.chunkPairs = nPairsAll;
-  if (pairBytes > 0)
-    P.chunkPairs = (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / pairBytes));
+  if (pairBytes + pairBytesR > 0)
+    P.chunkPairs =
+        (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / (pairBytes + pairBytesR)));
   P.chunkVecs = P.single ? 1 : 2 * P.chunkPairs;
   P.nChunks = (nVec + P.chunkVecs - 1) / P.chunkVecs;
   const long nData = m->nData;
@@ -647,60 +653,136 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   const int nFree = nVec;
   const int nNormal = nFree * (nFree + 1) / 2 + nFree;
   HIPCHK(P.out_d.ensure(sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0)), "alloc results");
+  HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0),
+                       hipHostMallocDefault),
+         "pinned results");
+  if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
+  if (flags & PF_MODEL) HIPCHK(P.model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
   if (flags & PF_JAC) {
     HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nFree), "alloc Jacobian columns");
     HIPCHK(P.normPart_d.ensure(sizeof(double) * normal_partials_size(nData, nFree)), "alloc normal partials");
-    if (P.pinnedOut) (void)hipHostFree(P.pinnedOut);
-    P.pinnedOut = nullptr;
-    HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * nNormal, hipHostMallocDefault), "pinned results");
   }
   if (flags & (PF_DEVKEEP | PF_DFAC1 | PF_DFAC2)) {
     HIPCHK(m->devkeep_d.ensure(sizeof(double) * nData), "alloc kept deviates");
     HIPCHK(m->dfac_d.ensure(sizeof(double) * nData), "alloc dfac");
   }
-  if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
-  if (flags & PF_MODEL) HIPCHK(P.model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
-  P.osubStride = 0;
-  P.ovStride = 0;
-  for (auto& R : m->regions) {
-    P.osubStride = std::max(P.osubStride, (long)R->nbc * R->scale * R->nbr * R->scale);
-    P.ovStride += R->nbc * R->rowsReplaced;
-  }
-  if (nR) {
-    HIPCHK(P.osub_d.ensure(sizeof(double) * P.osubStride * P.chunkVecs), "alloc region sub-image");
-    HIPCHK(P.ovals_d.ensure(sizeof(double) * P.ovStride * P.chunkVecs), "alloc region values");
-  }
-  P.pixPerBlock = 1024;
-  const GridGeom& gm = P.geoms[0];
-  if (m->doConv) P.nSlots = (gm.Hh + m->mainConv[P.which].rowsK3 - 1) / m->mainConv[P.which].rowsK3;
-  else P.nSlots = (int)(((long)m->W * m->H + P.pixPerBlock - 1) / P.pixPerBlock);
-  const int nPartPairs = m->doConv ? (P.single ? 1 : P.chunkPairs) : P.chunkVecs;
-  HIPCHK(P.partials_d.ensure(sizeof(double) * 5 * 2 * (size_t)P.nSlots * nPartPairs), "alloc partials");
 
-  // static pair lists (per chunk, per grid) and vector -> partial-slot maps
+  // static pair lists (per chunk, per grid)
   const int npc = P.single ? 1 : P.chunkPairs;
+  P.npc = npc;
   std::vector<int4> pairs((size_t)P.nChunks * P.nGrids * npc, make_int4(0, 0, -1, 0));
-  std::vector<int2> slots((size_t)P.nChunks * P.chunkVecs, make_int2(0, 0));
   for (int c = 0; c < P.nChunks; c++) {
     const int v0 = c * P.chunkVecs, nv = std::min(P.chunkVecs, nVec - v0);
     for (int gi = 0; gi < P.nGrids; gi++) {
       int4* pg = pairs.data() + ((size_t)c * P.nGrids + gi) * npc;
-      if (P.single) pg[0] = make_int4(0, 0, 0, (P.geoms[gi].H + 1) / 2);
+      if (P.single) pg[0] = make_int4(0, 0, 0, P.geoms[gi].Hh);
       else
         for (int p = 0; p < (nv + 1) / 2; p++) pg[p] = make_int4(2 * p, 0, (2 * p + 1 < nv) ? 2 * p + 1 : -1, 0);
     }
-    int2* sl = slots.data() + (size_t)c * P.chunkVecs;
-    for (int v = 0; v < nv; v++) {
-      if (!m->doConv) sl[v] = make_int2(v, 1);
-      else if (P.single) sl[v] = make_int2(0, 3);
-      else sl[v] = make_int2(v / 2, 1 << (v & 1));
-    }
   }
   HIPCHK(P.pairs_d.ensure(sizeof(int4) * pairs.size()), "alloc pairs");
-  HIPCHK(P.slots_d.ensure(sizeof(int2) * slots.size()), "alloc slots");
   HIPCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
-  HIPCHK(hipMemcpy(P.slots_d.p, slots.data(), sizeof(int2) * slots.size(), hipMemcpyHostToDevice), "slots");
-  P.npc = npc;
+
+  // task tables: main grid + every region, per chunk; block ranges of the merged launches
+  const CompRec* recs_d = reinterpret_cast<const CompRec*>(P.stage_d.p);
+  const Box* boxes_d = reinterpret_cast<const Box*>(P.stage_d.as<char>() + P.recBytes);
+  const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.recBytes + P.boxBytes);
+  const bool deriv = IsDeriv(flags);
+  const size_t elem = sizeof(cpx<T>);
+  P.nTasks = P.nGrids;
+  std::vector<Task> tasks((size_t)P.nChunks * P.nTasks);
+  for (int c = 0; c < P.nChunks; c++) {
+    const int v0 = c * P.chunkVecs, nv = std::min(P.chunkVecs, nVec - v0);
+    int b1 = 0, b2 = 0, b3 = 0;
+    for (int gi = 0; gi < P.nGrids; gi++) {
+      Task& Tk = tasks[(size_t)c * P.nTasks + gi];
+      std::memset(&Tk, 0, sizeof(Tk));
+      const bool region = gi > 0;
+      ConvGrid* cg = region ? &m->regions[gi - 1]->conv[P.which] : (m->doConv ? &m->mainConv[P.which] : nullptr);
+      Tk.kind = region ? TK_REGION : (m->doConv ? TK_MAIN_CONV : TK_MAIN_PLAIN);
+      Tk.nPairs = P.single ? 1 : (nv + 1) / 2;
+      Tk.g = P.geoms[gi];
+      Tk.pairs = P.pairs_d.as<int4>() + ((size_t)c * P.nGrids + gi) * npc;
+      EvalArgs& ea = Tk.ea;
+      ea.nComp = nC;
+      for (int n = 0; n < nC; n++) ea.types[n] = m->types[n];
+      ea.tail = P.tail_d.as<double>() + P.tailOff[gi] + (long)v0 * P.strideMax[gi];
+      ea.tailStride = P.strideMax[gi];
+      ea.tailUsed = used_d + gi;
+      if (deriv) {
+        ea.recs = recs_d;
+        ea.recStride = 0;
+        ea.boxes = boxes_d + (long)gi * IFG_MAXBOX;
+        ea.boxStride = 0;
+        ea.deriv = 1;
+        ea.nFree = nv;
+        for (int k = 0; k < nv; k++) ea.freeIdx[k] = P.freeIdx[v0 + k];
+        int off = 0, xy = 0;
+        for (int n = 0; n < nC; n++) {
+          if (m->fsetStart[n]) {
+            xy = off;
+            off += 2;
+          }
+          ea.compXY[n] = xy;
+          ea.compFirst[n] = off;
+          ea.compNP[n] = NParamsOf(m->types[n]);
+          off += ea.compNP[n];
+        }
+      } else {
+        ea.recs = recs_d + (long)v0 * nC;
+        ea.recStride = nC;
+        ea.boxes = boxes_d + ((long)gi * nVec + v0) * IFG_MAXBOX;
+        ea.boxStride = IFG_MAXBOX;
+      }
+      if (Tk.kind == TK_MAIN_PLAIN) {
+        Tk.pixPerBlock = 1024;
+        Tk.b1 = b1; Tk.n1 = 0;
+        Tk.b2 = b2; Tk.n2 = 0;
+        Tk.b3 = b3;
+        Tk.n3 = (int)(((long)Tk.g.W * Tk.g.Hh + Tk.pixPerBlock - 1) / Tk.pixPerBlock);
+      } else {
+        Tk.pc = cg->fc->plan;
+        Tk.pr = cg->fr->plan;
+        Tk.twc = (sizeof(T) == 8) ? cg->fc->tw64.p : cg->fc->tw32.p;
+        Tk.twr = (sizeof(T) == 8) ? cg->fr->tw64.p : cg->fr->tw32.p;
+        Tk.khat = (sizeof(T) == 8) ? cg->khat64.p : cg->khat32.p;
+        Tk.pairStride = (long)Tk.g.Nr * Tk.g.Nc;
+        Tk.spec = region ? (void*)(P.specR_d.as<char>() + regOff[gi - 1] * P.chunkPairs) : P.spec_d.p;
+        Tk.rows1 = cg->rowsK1;
+        Tk.rows3 = cg->rowsK3;
+        Tk.C = (sizeof(T) == 8) ? cg->C64 : cg->C32;
+        Tk.ld = Tk.g.Nr + 1;
+        Tk.b1 = b1; Tk.n1 = (Tk.g.Lr + Tk.rows1 - 1) / Tk.rows1;
+        Tk.b2 = b2; Tk.n2 = (Tk.g.Nc + Tk.C - 1) / Tk.C;
+        Tk.b3 = b3; Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
+        P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
+        P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
+        P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
+      }
+      if (region) {
+        const Region& R = *m->regions[gi - 1];
+        Tk.scale = R.scale;
+        Tk.nbc = R.nbc;
+        Tk.rowsRep = R.rowsReplaced;
+        Tk.c0 = R.x1 - 1;
+        Tk.r0 = R.y1 - 1;
+      }
+      Tk.slot0 = b3;
+      b1 += Tk.n1;
+      b2 += Tk.n2;
+      b3 += Tk.n3;
+    }
+    P.nB1 = b1;
+    P.nB2 = b2;
+    P.nB3 = b3;
+  }
+  P.nSlots = P.nB3;
+  P.tailWaves = 0;
+  for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
+  HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
+  HIPCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
+  HIPCHK(P.partials_d.ensure(sizeof(double) * 5 * 2 * (size_t)P.nSlots * npc), "alloc partials");
+  HIPCHK(hipDeviceSynchronize(), "plan setup");
   return IFG_OK;
 }
 
@@ -744,125 +826,38 @@ int StageCall(ifg_model* m, Plan& P, const double* params, std::vector<int>& nee
   return grow ? 1 : 0;
 }
 
-// Enqueue the whole evaluation on the handle's stream (captured into the graph).
+// Enqueue the whole evaluation on the handle's stream (captured into the graph):
+// the staging copy, then per chunk four merged launches over all grids.
 template <typename T>
 int Enqueue(ifg_model* m, Plan& P) {
-  const int nC = m->nComp(), nR = (int)m->regions.size(), nVec = P.nVec;
+  const int nVec = P.nVec, nR = (int)m->regions.size();
   const long nData = m->nData;
   hipStream_t s = m->stream;
   HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
-  const CompRec* recs_d = reinterpret_cast<const CompRec*>(P.stage_d.p);
-  const Box* boxes_d = reinterpret_cast<const Box*>(P.stage_d.as<char>() + P.recBytes);
-  const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.recBytes + P.boxBytes);
   const bool deriv = IsDeriv(P.flags);
-  auto evalArgs = [&](int gi, int v0) {
-    EvalArgs ea;
-    std::memset(&ea, 0, sizeof(ea));
-    ea.nComp = nC;
-    for (int n = 0; n < nC; n++) ea.types[n] = m->types[n];
-    ea.tail = P.tail_d.as<double>() + P.tailOff[gi] + (long)v0 * P.strideMax[gi];
-    ea.tailStride = P.strideMax[gi];
-    ea.tailUsed = used_d + gi;
-    if (deriv) {
-      ea.recs = recs_d;
-      ea.recStride = 0;
-      ea.boxes = boxes_d + (long)gi * IFG_MAXBOX;
-      ea.boxStride = 0;
-      ea.deriv = 1;
-      ea.nFree = std::min(P.chunkVecs, nVec - v0);
-      for (int k = 0; k < ea.nFree; k++) ea.freeIdx[k] = P.freeIdx[v0 + k];
-      int off = 0, xy = 0;
-      for (int n = 0; n < nC; n++) {
-        if (m->fsetStart[n]) {
-          xy = off;
-          off += 2;
-        }
-        ea.compXY[n] = xy;
-        ea.compFirst[n] = off;
-        ea.compNP[n] = NParamsOf(m->types[n]);
-        off += ea.compNP[n];
-      }
-    } else {
-      ea.recs = recs_d + (long)v0 * nC;
-      ea.recStride = nC;
-      ea.boxes = boxes_d + ((long)gi * nVec + v0) * IFG_MAXBOX;
-      ea.boxStride = IFG_MAXBOX;
-    }
-    return ea;
-  };
   const int mode = (P.flags & PF_JAC) ? S_JAC : (P.flags & PF_GRAD) ? S_GRAD : m->statMode();
-  const double agiant = 1.304e19 / (double)std::max<long>(nData, 1);
   const int subsampling = m->doSubsampling ? 1 : 0;
-  const int which = P.which;
-
   for (int c = 0; c < P.nChunks; c++) {
     const int v0 = c * P.chunkVecs;
     const int nv = std::min(P.chunkVecs, nVec - v0);
     const int np = P.single ? 1 : (nv + 1) / 2;
-    const int4* pairsC = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
-    const int2* slotsC = P.slots_d.as<int2>() + (size_t)c * P.chunkVecs;
-
-    // The oversampled regions (tail, evaluate, convolve with their PSF, block-average)
-    // run on the second stream while the main grid is evaluated and convolved; the
-    // main statistic pass waits for them.
-    hipStream_t s2 = m->stream2;
-    if (nR) {
-      HIPCHK(hipEventRecord(m->evFork, s), "fork");
-      HIPCHK(hipStreamWaitEvent(s2, m->evFork, 0), "fork wait");
-    }
-    if (P.strideMax[0] > 0)
-      TIMED("k_tail", launch_tail<T>(s, evalArgs(0, v0), P.geoms[0], nv, subsampling), "k_tail");
+    const Task* tk = P.tasks_d.as<Task>() + (size_t)c * P.nTasks;
+    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, P.tailWaves, nv, subsampling), "k_tail");
+    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, P.nTasks, P.nB1, np, P.lds1), "k_eval_rowfft");
+    TIMED("k_colconv", launch_colconv<T>(s, tk, P.nTasks, P.nB2, np, P.lds2), "k_colconv");
     StatArgs<T> sa;
     std::memset(&sa, 0, sizeof(sa));
     for (int k = 0; k < nR; k++) {
-      if (P.strideMax[1 + k] > 0)
-        TIMED_ON(s2, "k_tail_region", launch_tail<T>(s2, evalArgs(1 + k, v0), P.geoms[1 + k], nv, subsampling),
-                 "k_tail");
-      Region& R = *m->regions[k];
-      ConvGrid& cg = R.conv[which];
-      const GridGeom& g = cg.g;
-      const long pstride = (long)g.Nr * g.Nc;
-      const cpx<T>* tw_c = (sizeof(T) == 8) ? (const cpx<T>*)cg.fc->tw64.p : (const cpx<T>*)cg.fc->tw32.p;
-      const cpx<T>* tw_r = (sizeof(T) == 8) ? (const cpx<T>*)cg.fr->tw64.p : (const cpx<T>*)cg.fr->tw32.p;
-      const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
-      const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
-      const int4* pairsR = pairsC + (size_t)(1 + k) * P.npc;
-      TIMED_ON(s2, "k_eval_rowfft_region",
-               launch_eval_rowfft<T>(s2, evalArgs(1 + k, v0), g, pairsR, np, P.specR_d.as<cpx<T>>(), pstride,
-                                     cg.fc->plan, tw_c, cg.rowsK1),
-               "region k_eval_rowfft");
-      TIMED_ON(s2, "k_colconv_region",
-               launch_colconv<T>(s2, g, P.specR_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
-               "region k_colconv");
-      StatArgs<T> so;
-      std::memset(&so, 0, sizeof(so));
-      so.mode = S_NONE;
-      so.W = g.W;
-      so.H = g.H;
-      so.nData = (long)g.W * g.H;
-      so.modelOut = P.osub_d.as<double>();
-      TIMED_ON(s2, "k_rowifft_region",
-               launch_rowifft_stat<T>(s2, g, pairsR, np, P.specR_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, so,
-                                      cg.rowsK3, agiant),
-               "region k_rowifft");
-      TIMED_ON(s2, "k_downsample",
-               launch_downsample(s2, P.osub_d.as<double>(), g.W, (long)g.W * g.H, P.ovals_d.as<double>(),
-                                 P.ovStride, R.ovOff, R.nbc, R.rowsReplaced, R.scale, nv),
-               "k_downsample");
+      const Region& R = *m->regions[k];
       RegionOut ro;
       std::memset(&ro, 0, sizeof(ro));
       ro.c0 = R.x1 - 1;
       ro.r0 = R.y1 - 1;
       ro.w = R.nbc;
       ro.h = R.rowsReplaced;
-      ro.off = R.ovOff;
       sa.regs[k] = ro;
     }
     sa.nReg = nR;
-    sa.ovals = P.ovals_d.as<double>();
-    sa.ovStride = P.ovStride;
-
-    // 3. main grid + statistic
     sa.mode = mode;
     sa.modelErrors = m->modelErrors ? 1 : 0;
     sa.W = m->W;
@@ -874,6 +869,7 @@ int Enqueue(ifg_model* m, Plan& P) {
     sa.effGain = m->effGain;
     sa.sky = m->originalSky;
     sa.rnTerm = m->nCombined * m->rnAdu2;
+    sa.agiant = 1.304e19 / (double)std::max<long>(nData, 1);
     sa.devOut = (P.flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
     if (P.flags & PF_DEVKEEP) sa.devOut = m->devkeep_d.as<double>();
     sa.modelOut = (P.flags & PF_MODEL) ? P.model_d.as<double>() + (long)v0 * nData : nullptr;
@@ -882,42 +878,17 @@ int Enqueue(ifg_model* m, Plan& P) {
       sa.dfacKind = (P.flags & PF_DFAC2) ? 2 : 1;
     }
     if (deriv) sa.dfac = m->dfac_d.as<double>();
-    sa.partials = P.partials_d.as<double>();
-    sa.nSlots = P.nSlots;
-    if (m->doConv) {
-      ConvGrid& cg = m->mainConv[which];
-      const GridGeom& g = cg.g;
-      const long pstride = (long)g.Nr * g.Nc;
-      const cpx<T>* tw_c = (sizeof(T) == 8) ? (const cpx<T>*)cg.fc->tw64.p : (const cpx<T>*)cg.fc->tw32.p;
-      const cpx<T>* tw_r = (sizeof(T) == 8) ? (const cpx<T>*)cg.fr->tw64.p : (const cpx<T>*)cg.fr->tw32.p;
-      const cpx<T>* kh = (sizeof(T) == 8) ? (const cpx<T>*)cg.khat64.p : (const cpx<T>*)cg.khat32.p;
-      const int C = (sizeof(T) == 8) ? cg.C64 : cg.C32;
-      TIMED("k_eval_rowfft",
-            launch_eval_rowfft<T>(s, evalArgs(0, v0), g, pairsC, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c,
-                                  cg.rowsK1),
-            "k_eval_rowfft");
-      TIMED("k_colconv", launch_colconv<T>(s, g, P.spec_d.as<cpx<T>>(), pstride, np, kh, cg.fr->plan, tw_r, C, 0),
-            "k_colconv");
-      if (nR) {
-        HIPCHK(hipEventRecord(m->evJoin, s2), "join");
-        HIPCHK(hipStreamWaitEvent(s, m->evJoin, 0), "join wait");
-      }
-      TIMED("k_rowifft_stat",
-            launch_rowifft_stat<T>(s, g, pairsC, np, P.spec_d.as<cpx<T>>(), pstride, cg.fc->plan, tw_c, sa, cg.rowsK3,
-                                   agiant),
-            "k_rowifft_stat");
-    } else {
-      if (nR) {
-        HIPCHK(hipEventRecord(m->evJoin, s2), "join");
-        HIPCHK(hipStreamWaitEvent(s, m->evJoin, 0), "join wait");
-      }
-      TIMED("k_eval_stat", launch_eval_stat<T>(s, evalArgs(0, v0), P.geoms[0], sa, nv, P.pixPerBlock, agiant),
-            "k_eval_stat");
+    const bool reduce = (mode != S_NONE && mode != S_JAC && (P.flags & PF_STAT));
+    if (reduce) {
+      sa.partials = P.partials_d.as<double>();
+      sa.nSlots = P.nSlots;
     }
-    if (mode != S_NONE && mode != S_JAC && (P.flags & PF_STAT))
-      TIMED("k_final",
-            launch_final(s, P.partials_d.as<double>(), P.nSlots, slotsC, nv, mode, P.out_d.as<double>() + v0),
+    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, P.nTasks, P.nB3, np, P.lds3, sa), "k_inverse_stat");
+    if (reduce) {
+      const int4* pairsMain = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
+      TIMED("k_final", launch_final(s, pairsMain, P.partials_d.as<double>(), P.nSlots, np, mode, P.out_d.as<double>() + v0),
             "k_final");
+    }
   }
   if (P.flags & PF_JAC) {
     const int nEnt = nVec * (nVec + 1) / 2 + nVec;
@@ -1070,10 +1041,7 @@ ifg_model* ifg_create(int device) {
   m->curEvents = &m->pendingEvents;
   const char* ng = getenv("IFG_NO_GRAPH");
   if (ng && ng[0] == '1') m->useGraphs = false;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess ||
-      hipStreamCreate(&m->stream2) != hipSuccess ||
-      hipEventCreateWithFlags(&m->evFork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&m->evJoin, hipEventDisableTiming) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
     m->err = "no HIP device / stream";
   }
   return m;
@@ -1232,7 +1200,6 @@ int ifg_add_oversampled_psf(ifg_model* m, const double* psf, int n_columns, int 
     return m->fail(IFG_ERR_UNSUPPORTED,
                    "oversampled region wider than tall with a square-ish PSF: the reference's row count would read "
                    "outside the region");
-  R->ovOff = 0;
   m->regions.push_back(std::move(R));
   return IFG_OK;
 }

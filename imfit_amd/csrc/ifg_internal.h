@@ -86,6 +86,28 @@ struct RegionOut {
   int c0, r0, w, h, off, pad0, pad1, pad2;
 };
 
+// One grid of a merged launch: the main image (with or without PSF) or an
+// oversampled region.  Each merged kernel launch covers every task of a chunk;
+// block x of the launch belongs to the task whose block range contains it.
+enum TaskKind { TK_MAIN_CONV = 0, TK_MAIN_PLAIN = 1, TK_REGION = 2 };
+struct Task {
+  int kind, nPairs;
+  GridGeom g;
+  FFTPlan pc, pr;              // row (length Nc) and column (length Nr) transforms
+  const void* twc;             // twiddles (cpx<T>) of length Nc and Nr
+  const void* twr;
+  const void* khat;            // PSF spectrum (cpx<T>, Nr x Nc, scaled by 1/(Nr Nc))
+  void* spec;                  // complex planes (cpx<T>), pairStride elements per pair
+  long pairStride;
+  const int4* pairs;           // (vecA, rowoffA, vecB or -1, rowoffB) per pair
+  int b1, n1, b2, n2, b3, n3;  // block ranges in the evaluate / column / inverse launches
+  int rows1, rows3, C, ld;     // rows per block (row passes), columns per block and stride (column pass)
+  int slot0;                   // first partial slot of this task's inverse-pass blocks
+  int scale, nbc, rowsRep, c0, r0;   // region: data pixels [c0, c0+nbc) x [r0, r0+rowsRep) it replaces
+  int pixPerBlock;             // TK_MAIN_PLAIN: pixels per block
+  EvalArgs ea;
+};
+
 template <typename T>
 struct StatArgs {
   int mode;           // StatMode
@@ -102,11 +124,10 @@ struct StatArgs {
   const double* dfac; // S_JAC / S_GRAD input factor per data pixel
   double* modelOut;   // [vec * nData] or null
   double* partials;   // [pair][plane][slot] x 5 doubles
-  int nSlots;         // slots per plane
+  int nSlots;         // slots per plane (= inverse-pass blocks per pair)
   int nReg;
-  RegionOut regs[IFG_MAXREG];
-  const double* ovals;   // oversampled-region replacements
-  int ovStride;
+  RegionOut regs[IFG_MAXREG];   // main-grid pixels owned by oversampled regions (skipped by the main tasks)
+  double agiant;
 };
 
 }  // namespace ifg

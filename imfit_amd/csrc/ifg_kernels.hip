@@ -1,17 +1,18 @@
 // ifg_kernels.hip -- gfx950 kernels of the model-image path.
 //
-// Pipeline for one PSF-convolved evaluation (main grid or one oversampled region):
+// One evaluation is four launches, each covering the main grid and every
+// oversampled region of the chunk together ("tasks"):
 //   k_tail          one wavefront per pixel near a component centre: sub-pixel
 //                   integration spread over the 64 lanes (func_sersic.cpp:150-213 etc.)
 //   k_eval_rowfft   evaluates the model (Kahan over components, core/model_object.cpp:1155-1174)
 //                   for two real planes packed as one complex plane, straight into LDS,
 //                   and runs the forward row FFT there
 //   k_colconv       column FFT, multiply by the PSF spectrum, inverse column FFT, in LDS
-//   k_rowifft_stat  inverse row FFT, then the deviates / fit-statistic terms of the
-//                   data pixels (core/model_object.cpp:1413-1496,1633-1775) with
-//                   per-block partial sums
-//   k_final         per-vector reduction of the partials: mp_enorm^2 or 2*Cash sum
-// Without a PSF, k_eval_stat does evaluation + statistic in one pass.
+//   k_inverse_stat  inverse row FFT; regions block-average into the pixels they
+//                   replace; deviates / fit-statistic terms of the data pixels
+//                   (core/model_object.cpp:1413-1496,1633-1775) with per-block partials
+//   k_final         per-pair merge of the partials into each vector's statistic
+// Without a PSF the main grid is evaluated directly inside k_inverse_stat.
 //
 // The real->complex packing: since the PSF is real, conv(a + i b) = conv(a) + i conv(b),
 // so one complex FFT convolution carries two real images (two parameter vectors, or
@@ -362,134 +363,210 @@ __device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, 
   return newValSum;
 }
 
+// Task of block x in a merged launch (block ranges [b, b + n) per task).
+__device__ __forceinline__ int find_task(const Task* tasks, int nTasks, int which, int bx) {
+  for (int t = 0; t < nTasks; t++) {
+    const int b = (which == 1) ? tasks[t].b1 : (which == 2) ? tasks[t].b2 : tasks[t].b3;
+    const int n = (which == 1) ? tasks[t].n1 : (which == 2) ? tasks[t].n2 : tasks[t].n3;
+    if (bx >= b && bx < b + n) return t;
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------------ sub-pixel tails
 // One wavefront per box pixel: the full Kahan sum over components, with each
 // component's nSub x nSub sub-pixel average spread across the 64 lanes.  Sample
 // coordinates follow func_sersic.cpp:165-181 exactly (FP64); only the order of the
 // final sum differs (lane partials + xor butterfly, identical in every lane).
+// In derivative mode the wave forms the sub-pixel mean of d(intensity)/dp instead.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tail(EvalArgs ea, GridGeom g, int nVec, int subsampling) {
-  const int lane = threadIdx.x & 63;
-  const int used = *ea.tailUsed;
-  const long total = (long)nVec * used;
-  const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
-  // grid-stride over (vector, box pixel): the launch shape is fixed, the work count
-  // comes from device memory, so the launch can be replayed from a graph
-  for (long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < total; wave += nWaves) {
-    const int vec = (int)(wave / used);
-    const int loc = (int)(wave - (long)vec * used);
-    const Box* bx = ea.boxes + vec * ea.boxStride;
-    int c = 0, r = 0;
-    bool found = false;
-    for (int b = 0; b < IFG_MAXBOX; b++) {
-      const Box B = bx[b];
-      if (B.w > 0 && loc >= B.off && loc < B.off + B.w * B.h) {
-        c = B.c0 + (loc - B.off) % B.w;
-        r = B.r0 + (loc - B.off) / B.w;
-        found = true;
-        break;
-      }
+__device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int loc, int lane, int subsampling) {
+  const Box* bx = ea.boxes + vec * ea.boxStride;
+  int c = 0, r = 0;
+  bool found = false;
+  for (int b = 0; b < IFG_MAXBOX; b++) {
+    const Box B = bx[b];
+    if (B.w > 0 && loc >= B.off && loc < B.off + B.w * B.h) {
+      c = B.c0 + (loc - B.off) % B.w;
+      r = B.r0 + (loc - B.off) / B.w;
+      found = true;
+      break;
     }
-    if (!found) continue;
-    const double x = g.xa + (double)c * g.frac;
-    const double y = g.ya + (double)r * g.frac;
-    const CompRec* rec = ea.recs + (long)vec * ea.recStride;
-    if (ea.deriv) {
-      // derivative of the sub-pixel mean = mean of the derivative over the same samples
-      const int gidx = ea.freeIdx[vec];
-      double dsum = 0.0;
-      for (int n = 0; n < ea.nComp; n++) {
-        const int slot = grad_slot(ea, n, gidx);
-        if (slot < 0) continue;
-        const int type = ea.types[n];
-        const double* v = rec[n].v;
-        const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, ell_radius(v, x, y), subsampling);
-        if (nSub > 1) {
-          const double deltaSubpix = 1.0 / nSub;
-          const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
-          const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
-          const int n2 = nSub * nSub;
-          double part = 0.0;
-          for (int s = lane; s < n2; s += 64) {
-            const int ii = s / nSub, jj = s - ii * nSub;
-            part += comp_grad(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, slot);
-          }
-          dsum += wave_sum(part) / (nSub * nSub);
-        } else {
-          dsum += comp_grad(type, v, x, y, slot);
-        }
-      }
-      if (lane == 0) const_cast<double*>(ea.tail)[(long)vec * ea.tailStride + loc] = dsum;
-      continue;
-    }
-    double newValSum = 0.0, storedError = 0.0;
+  }
+  if (!found) return;
+  const double x = g.xa + (double)c * g.frac;
+  const double y = g.ya + (double)r * g.frac;
+  const CompRec* rec = ea.recs + (long)vec * ea.recStride;
+  double* out = const_cast<double*>(ea.tail) + (long)vec * ea.tailStride + loc;
+  if (ea.deriv) {
+    const int gidx = ea.freeIdx[vec];
+    double dsum = 0.0;
     for (int n = 0; n < ea.nComp; n++) {
+      const int slot = grad_slot(ea, n, gidx);
+      if (slot < 0) continue;
       const int type = ea.types[n];
       const double* v = rec[n].v;
-      double val;
-      if (type == F_FLATSKY) {
-        val = v[5];
-      } else {
-        const double rad = ell_radius(v, x, y);
-        const int nSub = n_subsamples(type, v, rad, subsampling);
-        if (nSub > 1) {
-          const double deltaSubpix = 1.0 / nSub;
-          const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
-          const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
-          const int n2 = nSub * nSub;
-          double part = 0.0;
-          for (int s = lane; s < n2; s += 64) {
-            const int ii = s / nSub, jj = s - ii * nSub;
-            const double x_ii = x_sub_start + ii * deltaSubpix;
-            const double y_ii = y_sub_start + jj * deltaSubpix;
-            part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
-          }
-          val = wave_sum(part) / (nSub * nSub);
-        } else {
-          val = (double)intensity_r<T>(type, v, rad);
+      const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, ell_radius(v, x, y), subsampling);
+      if (nSub > 1) {
+        const double deltaSubpix = 1.0 / nSub;
+        const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+        const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+        const int n2 = nSub * nSub;
+        double part = 0.0;
+        for (int s = lane; s < n2; s += 64) {
+          const int ii = s / nSub, jj = s - ii * nSub;
+          part += comp_grad(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, slot);
         }
+        dsum += wave_sum(part) / (nSub * nSub);
+      } else {
+        dsum += comp_grad(type, v, x, y, slot);
       }
-      double adjVal = val - storedError;
-      double tempSum = newValSum + adjVal;
-      storedError = (tempSum - newValSum) - adjVal;
-      newValSum = tempSum;
     }
-    if (lane == 0) const_cast<double*>(ea.tail)[(long)vec * ea.tailStride + loc] = newValSum;
+    if (lane == 0) *out = dsum;
+    return;
+  }
+  double newValSum = 0.0, storedError = 0.0;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int type = ea.types[n];
+    const double* v = rec[n].v;
+    double val;
+    if (type == F_FLATSKY) {
+      val = v[5];
+    } else {
+      const double rad = ell_radius(v, x, y);
+      const int nSub = n_subsamples(type, v, rad, subsampling);
+      if (nSub > 1) {
+        const double deltaSubpix = 1.0 / nSub;
+        const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+        const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+        const int n2 = nSub * nSub;
+        double part = 0.0;
+        for (int s = lane; s < n2; s += 64) {
+          const int ii = s / nSub, jj = s - ii * nSub;
+          const double x_ii = x_sub_start + ii * deltaSubpix;
+          const double y_ii = y_sub_start + jj * deltaSubpix;
+          part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
+        }
+        val = wave_sum(part) / (nSub * nSub);
+      } else {
+        val = (double)intensity_r<T>(type, v, rad);
+      }
+    }
+    double adjVal = val - storedError;
+    double tempSum = newValSum + adjVal;
+    storedError = (tempSum - newValSum) - adjVal;
+    newValSum = tempSum;
+  }
+  if (lane == 0) *out = newValSum;
+}
+
+// Grid-stride over (task, vector, box pixel) for every task of the launch: the
+// launch shape is fixed and the work count comes from device memory, so the launch
+// replays from a graph.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, int nTasks, int nVec, int subsampling) {
+  const int lane = threadIdx.x & 63;
+  long total = 0;
+  long tot[IFG_MAXREG + 1];
+  for (int t = 0; t < nTasks; t++) {
+    tot[t] = (tasks[t].ea.tailStride > 0) ? (long)nVec * (*tasks[t].ea.tailUsed) : 0;
+    total += tot[t];
+  }
+  const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
+  for (long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < total; wave += nWaves) {
+    long w = wave;
+    int t = 0;
+    while (w >= tot[t]) {
+      w -= tot[t];
+      t++;
+    }
+    const int used = *tasks[t].ea.tailUsed;
+    const int vec = (int)(w / used);
+    const int loc = (int)(w - (long)vec * used);
+    tail_pixel<T>(tasks[t].ea, tasks[t].g, vec, loc, lane, subsampling);
   }
 }
 
 // ------------------------------------------------------------------ evaluation + row FFT
+// Evaluates two real planes (one complex row: plane A real, plane B imaginary)
+// straight into LDS and runs the forward row FFT there.
 template <typename T>
-__global__ __launch_bounds__(256) void k_eval_rowfft(EvalArgs ea, GridGeom g, const int4* __restrict__ pairs,
-                                                     cpx<T>* __restrict__ spec, long pairStride, FFTPlan pl,
-                                                     const cpx<T>* __restrict__ tw, int rowsPerBlock) {
+__global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, int nTasks) {
   extern __shared__ char smem_raw[];
+  const int t = find_task(tasks, nTasks, 1, blockIdx.x);
+  if (t < 0) return;
+  const Task& Tk = tasks[t];
+  const GridGeom g = Tk.g;
+  const int rows = Tk.rows1;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rowsPerBlock * g.Nc;
-  cpx<T>* TW = B + rowsPerBlock * g.Nc;
-  stage_twiddles(TW, tw, g.Nc);
-  const int4 pr = pairs[blockIdx.y];
-  const int ir0 = blockIdx.x * rowsPerBlock;
-  const int total = rowsPerBlock * g.Nc;
+  cpx<T>* B = A + rows * g.Nc;
+  cpx<T>* TW = B + rows * g.Nc;
+  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  const int4 pr = Tk.pairs[blockIdx.y];
+  const int ir0 = (blockIdx.x - Tk.b1) * rows;
+  const int total = rows * g.Nc;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int rr = e / g.Nc, j = e - rr * g.Nc, ir = ir0 + rr;
     T va = 0, vb = 0;
     if (ir < g.Lr && j < g.Lc) {
       const int c = j - g.hl;
-      va = model_value<T>(ea, g, pr.x, pr.y + ir - g.hb, c);
-      if (pr.z >= 0) vb = model_value<T>(ea, g, pr.z, pr.w + ir - g.hb, c);
+      va = model_value<T>(Tk.ea, g, pr.x, pr.y + ir - g.hb, c);
+      if (pr.z >= 0) vb = model_value<T>(Tk.ea, g, pr.z, pr.w + ir - g.hb, c);
     }
     A[e] = cpx<T>{va, vb};
   }
   __syncthreads();
-  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, TW, +1);
-  cpx<T>* S = spec + (long)blockIdx.y * pairStride;
+  const FFTPlan pl = Tk.pc;
+  cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, +1);
+  cpx<T>* S = (cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int rr = e / g.Nc, ir = ir0 + rr;
     if (ir < g.Lr) S[(long)ir * g.Nc + (e - rr * g.Nc)] = out[e];
   }
 }
 
-// Generic in-place row FFT (PSF spectrum setup).
+// ------------------------------------------------------------------ column pass
+// Forward column FFT of rows [0, Lr) (rest zero), multiply by the PSF spectrum,
+// inverse column FFT, store rows [hb, hb + Hh) -- for C columns per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, int nTasks) {
+  extern __shared__ char smem_raw[];
+  const int t = find_task(tasks, nTasks, 2, blockIdx.x);
+  if (t < 0) return;
+  const Task& Tk = tasks[t];
+  const GridGeom g = Tk.g;
+  const int C = Tk.C, ld = Tk.ld;
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + C * ld;
+  cpx<T>* TW = B + C * ld;
+  stage_twiddles(TW, (const cpx<T>*)Tk.twr, g.Nr);
+  const int c0 = (blockIdx.x - Tk.b2) * C;
+  cpx<T>* S = (cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
+  const cpx<T>* khat = (const cpx<T>*)Tk.khat;
+  const int Nr = g.Nr, Nc = g.Nc;
+  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+    const int i = e / C, cc = e - i * C, col = c0 + cc;
+    cpx<T> v = {0, 0};
+    if (i < g.Lr && col < Nc) v = S[(long)i * Nc + col];
+    A[cc * ld + i] = v;
+  }
+  __syncthreads();
+  const FFTPlan pl = Tk.pr;
+  cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, TW, +1);
+  cpx<T>* Y = (X == A) ? B : A;
+  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+    const int cc = e / Nr, i = e - cc * Nr, col = c0 + cc;
+    if (col < Nc) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
+  }
+  __syncthreads();
+  X = fft_lds<T>(X, Y, pl, C, ld, TW, -1);
+  for (int e = threadIdx.x; e < g.Hh * C; e += blockDim.x) {
+    const int i = g.hb + e / C, cc = e % C, col = c0 + cc;
+    if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
+  }
+}
+
+// Setup-time transforms of the PSF spectrum: in-place row FFTs and column FFTs.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, FFTPlan pl, const cpx<T>* tw,
                                                 int dir, int rowsPerBlock) {
@@ -512,48 +589,24 @@ __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, 
   }
 }
 
-// ------------------------------------------------------------------ column pass
-// mode 0: forward column FFT of rows [0, Lr) (rest zero), multiply by the PSF
-//         spectrum (already scaled by 1/(Nr*Nc)), inverse column FFT, store rows
-//         [hb, hb + Hh);  mode 1: forward column FFT of all rows, store all rows.
 template <typename T>
-__global__ __launch_bounds__(256) void k_colconv(GridGeom g, cpx<T>* __restrict__ spec, long pairStride,
-                                                 const cpx<T>* __restrict__ khat, FFTPlan pl,
-                                                 const cpx<T>* __restrict__ tw, int C, int ld, int mode) {
+__global__ __launch_bounds__(256) void k_colfft(cpx<T>* buf, int Nr, int Nc, FFTPlan pl, const cpx<T>* tw, int C,
+                                                int ld) {
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + C * ld;
   cpx<T>* TW = B + C * ld;
-  stage_twiddles(TW, tw, g.Nr);
+  stage_twiddles(TW, tw, Nr);
   const int c0 = blockIdx.x * C;
-  cpx<T>* S = spec + (long)blockIdx.y * pairStride;
-  const int Nr = g.Nr, Nc = g.Nc;
-  const int rowsIn = (mode == 0) ? g.Lr : Nr;
   for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
     const int i = e / C, cc = e - i * C, col = c0 + cc;
-    cpx<T> v = {0, 0};
-    if (i < rowsIn && col < Nc) v = S[(long)i * Nc + col];
-    A[cc * ld + i] = v;
+    A[cc * ld + i] = (col < Nc) ? buf[(long)i * Nc + col] : cpx<T>{0, 0};
   }
   __syncthreads();
   cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, TW, +1);
-  cpx<T>* Y = (X == A) ? B : A;
-  if (mode == 0) {
-    for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
-      const int cc = e / Nr, i = e - cc * Nr, col = c0 + cc;
-      if (col < Nc) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
-    }
-    __syncthreads();
-    X = fft_lds<T>(X, Y, pl, C, ld, TW, -1);
-    for (int e = threadIdx.x; e < g.Hh * C; e += blockDim.x) {
-      const int i = g.hb + e / C, cc = e % C, col = c0 + cc;
-      if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
-    }
-  } else {
-    for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
-      const int i = e / C, cc = e - i * C, col = c0 + cc;
-      if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
-    }
+  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+    const int i = e / C, cc = e - i * C, col = c0 + cc;
+    if (col < Nc) buf[(long)i * Nc + col] = X[cc * ld + i];
   }
 }
 
@@ -613,15 +666,16 @@ __device__ __forceinline__ void en_merge(ENorm& a, const ENorm& b) {
 }
 
 // Model value m at data pixel (r, c) of vector vec -> deviate / statistic term.
+// Main-grid tasks skip pixels that an oversampled region replaces (the region task
+// produces those, core/downsample.cpp:96-104).
 template <typename T>
 __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r, int c, double m, ENorm& acc,
-                                           double agiant) {
-  for (int k = 0; k < sa.nReg; k++) {   // oversampled regions replace main pixels (core/downsample.cpp:104)
-    const RegionOut& R = sa.regs[k];
-    const int dc = c - R.c0, dr = r - R.r0;
-    if ((unsigned)dc < (unsigned)R.w && (unsigned)dr < (unsigned)R.h)
-      m = sa.ovals[(long)vec * sa.ovStride + R.off + dr * R.w + dc];
-  }
+                                           bool skipRegions) {
+  if (skipRegions)
+    for (int k = 0; k < sa.nReg; k++) {
+      const RegionOut& R = sa.regs[k];
+      if ((unsigned)(c - R.c0) < (unsigned)R.w && (unsigned)(r - R.r0) < (unsigned)R.h) return;
+    }
   const long z = (long)r * sa.W + c;
   if (sa.modelOut) sa.modelOut[(long)vec * sa.nData + z] = m;
   if (sa.mode == S_NONE) return;
@@ -642,7 +696,7 @@ __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r
   }
   if (sa.mode == S_CHI2) {
     const double dev = w * (d - m);
-    en_add(acc, dev, agiant);
+    en_add(acc, dev, sa.agiant);
     if (sa.devOut) sa.devOut[(long)vec * sa.nData + z] = dev;
     if (sa.dfacOut) {
       // d(dev)/dm = -w, plus the model-error weight's own dependence on m
@@ -717,126 +771,160 @@ __device__ void block_reduce_store(ENorm e, double* dst) {
   }
 }
 
-// ------------------------------------------------------------------ inverse row FFT + statistic
-template <typename T>
-__global__ __launch_bounds__(256) void k_rowifft_stat(GridGeom g, const int4* __restrict__ pairs,
-                                                      const cpx<T>* __restrict__ spec, long pairStride, FFTPlan pl,
-                                                      const cpx<T>* __restrict__ tw, StatArgs<T> sa,
-                                                      int rowsPerBlock, double agiant) {
-  extern __shared__ char smem_raw[];
-  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rowsPerBlock * g.Nc;
-  cpx<T>* TW = B + rowsPerBlock * g.Nc;
-  stage_twiddles(TW, tw, g.Nc);
-  const int4 pr = pairs[blockIdx.y];
-  const int io0 = blockIdx.x * rowsPerBlock;
-  const int total = rowsPerBlock * g.Nc;
-  const cpx<T>* S = spec + (long)blockIdx.y * pairStride;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int rr = e / g.Nc, io = io0 + rr;
-    A[e] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + (e - rr * g.Nc)] : cpx<T>{0, 0};
+// Statistic of one vector from its enorm state (final step of mp_enorm squared,
+// core/mp_enorm.cpp:104-121, for chi^2; 2 * sum for Cash / Poisson-MLR; the plain
+// sum for gradients).
+__device__ double final_stat(const ENorm& t, int mode) {
+  if (mode == S_CHI2) {
+    double ans;
+    if (t.s1 != 0.0) {
+      ans = t.x1max * sqrt(t.s1 + (t.s2 / t.x1max) / t.x1max);
+    } else if (t.s2 != 0.0) {
+      double temp;
+      if (t.s2 >= t.x3max) temp = t.s2 * (1.0 + (t.x3max / t.s2) * (t.x3max * t.s3));
+      else temp = t.x3max * ((t.s2 / t.x3max) + (t.x3max * t.s3));
+      ans = sqrt(temp);
+    } else {
+      ans = t.x3max * sqrt(t.s3);
+    }
+    return ans * ans;
   }
-  __syncthreads();
-  cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, g.Nc, TW, -1);
+  if (mode == S_GRAD) return t.s2;
+  return 2.0 * t.s2;
+}
+
+// Merge the partial slots of one plane of one pair (block-wide, result in thread 0).
+__device__ ENorm merge_slots(const double* base, int nSlots) {
+  ENorm e;
+  en_init(e);
+  for (int s = threadIdx.x; s < nSlots; s += blockDim.x) {
+    ENorm o = {base[s * 5], base[s * 5 + 1], base[s * 5 + 2], base[s * 5 + 3], base[s * 5 + 4]};
+    en_merge(e, o);
+  }
+  return block_reduce(e);
+}
+
+// Store this block's partials of plane A and plane B.
+template <typename T>
+__device__ void finish_block(const StatArgs<T>& sa, int slot, const ENorm& ea, const ENorm& eb) {
+  double* base = sa.partials + ((long)blockIdx.y * 2 * sa.nSlots) * 5;
+  block_reduce_store(ea, base + (long)slot * 5);
+  block_reduce_store(eb, base + ((long)sa.nSlots + slot) * 5);
+}
+
+// One block per pair: merge the pair's partial slots into the statistic of its
+// vector(s) (both planes of a single image split into halves are one vector).
+__global__ __launch_bounds__(256) void k_final(const int4* __restrict__ pairs, const double* __restrict__ partials,
+                                               int nSlots, int mode, double* __restrict__ out) {
+  const int pair = blockIdx.x;
+  const int4 pr = pairs[pair];
+  const double* base = partials + ((long)pair * 2 * nSlots) * 5;
+  if (pr.z == pr.x) {
+    ENorm a = merge_slots(base, nSlots);
+    ENorm b = merge_slots(base + (long)nSlots * 5, nSlots);
+    if (threadIdx.x == 0) {
+      en_merge(a, b);
+      out[pr.x] = final_stat(a, mode);
+    }
+  } else {
+    ENorm a = merge_slots(base, nSlots);
+    if (threadIdx.x == 0) out[pr.x] = final_stat(a, mode);
+    if (pr.z >= 0) {
+      ENorm b = merge_slots(base + (long)nSlots * 5, nSlots);
+      if (threadIdx.x == 0) out[pr.z] = final_stat(b, mode);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ inverse pass + statistic
+// One launch over every task of the chunk:
+//   TK_MAIN_CONV: inverse row FFT, then deviates / statistic terms of the data pixels;
+//   TK_MAIN_PLAIN: evaluate the model directly (no PSF) and form the terms;
+//   TK_REGION: inverse row FFT of the oversampled plane, s x s block mean of the
+//     sub-pixels (rows outer, columns inner, core/downsample.cpp:84-106) into the
+//     data pixels the region replaces, and their terms.
+template <typename T>
+__global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, int nTasks, StatArgs<T> sa) {
+  extern __shared__ char smem_raw[];
+  const int t = find_task(tasks, nTasks, 3, blockIdx.x);
+  if (t < 0) return;
+  const Task& Tk = tasks[t];
+  const GridGeom g = Tk.g;
+  const int4 pr = Tk.pairs[blockIdx.y];
+  const int lb = blockIdx.x - Tk.b3;
   ENorm ea, eb;
   en_init(ea);
   en_init(eb);
-  const int nOut = rowsPerBlock * g.W;
-  for (int e = threadIdx.x; e < nOut; e += blockDim.x) {
-    const int rr = e / g.W, c = e - rr * g.W, io = io0 + rr;
-    if (io >= g.Hh) continue;
-    const cpx<T> v = out[rr * g.Nc + c + g.hl];
-    const int ra = pr.y + io;
-    if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)v.x, ea, agiant);
-    if (pr.z >= 0) {
-      const int rb = pr.w + io;
-      if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)v.y, eb, agiant);
-    }
-  }
-  if (sa.partials) {
-    double* base = sa.partials + ((long)blockIdx.y * 2 * sa.nSlots) * 5;
-    block_reduce_store(ea, base + (long)blockIdx.x * 5);
-    block_reduce_store(eb, base + ((long)sa.nSlots + blockIdx.x) * 5);
-  }
-}
-
-// ------------------------------------------------------------------ no-PSF: evaluation + statistic
-template <typename T>
-__global__ __launch_bounds__(256) void k_eval_stat(EvalArgs ea, GridGeom g, StatArgs<T> sa, int pixPerBlock,
-                                                   double agiant) {
-  const int vec = blockIdx.y;
-  const long p0 = (long)blockIdx.x * pixPerBlock;
-  const long npix = (long)g.W * g.H;
-  ENorm acc;
-  en_init(acc);
-  for (int e = threadIdx.x; e < pixPerBlock; e += blockDim.x) {
-    const long p = p0 + e;
-    if (p >= npix) break;
-    const int r = (int)(p / g.W), c = (int)(p - (long)r * g.W);
-    const double m = (double)model_value<T>(ea, g, vec, r, c);
-    stat_pixel<T>(sa, vec, r, c, m, acc, agiant);
-  }
-  if (sa.partials) {
-    // pair = vec, plane 0
-    double* base = sa.partials + ((long)vec * 2 * sa.nSlots) * 5;
-    block_reduce_store(acc, base + (long)blockIdx.x * 5);
-  }
-}
-
-// ------------------------------------------------------------------ oversampled-region block average
-// DownsampleAndReplace's s x s box mean (core/downsample.cpp:84-106), rows outer,
-// columns inner, one output pixel per thread.
-__global__ void k_downsample(const double* __restrict__ osub, int Wos, long osubStride, double* __restrict__ ovals,
-                             int ovStride, int off, int nColsSub, int nRowsSub, int scale, int nVec) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int vec = blockIdx.y;
-  if (t >= nColsSub * nRowsSub || vec >= nVec) return;
-  const int bi = t / nColsSub, bj = t - bi * nColsSub;
-  const double* src = osub + (long)vec * osubStride;
-  double binned = 0.0;
-  for (int ii = bi * scale; ii < bi * scale + scale; ii++)
-    for (int jj = bj * scale; jj < bj * scale + scale; jj++) binned += src[(long)ii * Wos + jj];
-  const double area = scale * scale;
-  ovals[(long)vec * ovStride + off + t] = binned / area;
-}
-
-// ------------------------------------------------------------------ final per-vector reduction
-// vecSlots[v] = (pair, planeMask): the partial slots that belong to vector v.
-__global__ __launch_bounds__(256) void k_final(const double* __restrict__ partials, int nSlots,
-                                               const int2* __restrict__ vecSlots, int mode, double* out) {
-  const int v = blockIdx.x;
-  const int2 vs = vecSlots[v];
-  ENorm e;
-  en_init(e);
-  for (int plane = 0; plane < 2; plane++) {
-    if (!(vs.y & (1 << plane))) continue;
-    const double* base = partials + ((long)vs.x * 2 + plane) * nSlots * 5;
-    for (int s = threadIdx.x; s < nSlots; s += blockDim.x) {
-      ENorm o = {base[s * 5], base[s * 5 + 1], base[s * 5 + 2], base[s * 5 + 3], base[s * 5 + 4]};
-      en_merge(e, o);
-    }
-  }
-  ENorm t = block_reduce(e);
-  if (threadIdx.x == 0) {
-    double ans;
-    if (mode == S_CHI2) {   // final step of mp_enorm (core/mp_enorm.cpp:104-121), squared
-      if (t.s1 != 0.0) {
-        ans = t.x1max * sqrt(t.s1 + (t.s2 / t.x1max) / t.x1max);
-      } else if (t.s2 != 0.0) {
-        double temp;
-        if (t.s2 >= t.x3max) temp = t.s2 * (1.0 + (t.x3max / t.s2) * (t.x3max * t.s3));
-        else temp = t.x3max * ((t.s2 / t.x3max) + (t.x3max * t.s3));
-        ans = sqrt(temp);
-      } else {
-        ans = t.x3max * sqrt(t.s3);
+  if (Tk.kind == TK_MAIN_PLAIN) {   // rows [0, Hh) of plane A and of plane B, no PSF
+    const long npix = (long)g.W * g.Hh;
+    const long p0 = (long)lb * Tk.pixPerBlock;
+    for (int e = threadIdx.x; e < Tk.pixPerBlock; e += blockDim.x) {
+      const long p = p0 + e;
+      if (p >= npix) break;
+      const int r = (int)(p / g.W), c = (int)(p - (long)r * g.W);
+      const int ra = pr.y + r;
+      if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)model_value<T>(Tk.ea, g, pr.x, ra, c), ea, true);
+      if (pr.z >= 0) {
+        const int rb = pr.w + r;
+        if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)model_value<T>(Tk.ea, g, pr.z, rb, c), eb, true);
       }
-      out[v] = ans * ans;
-    } else if (mode == S_GRAD) {
-      out[v] = t.s2;
-    } else {
-      out[v] = 2.0 * t.s2;
+    }
+  } else {
+    const int rows = Tk.rows3;
+    cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* B = A + rows * g.Nc;
+    cpx<T>* TW = B + rows * g.Nc;
+    stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+    const int io0 = lb * rows;
+    const int total = rows * g.Nc;
+    const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+      const int rr = e / g.Nc, io = io0 + rr;
+      A[e] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + (e - rr * g.Nc)] : cpx<T>{0, 0};
+    }
+    __syncthreads();
+    const FFTPlan pl = Tk.pc;
+    cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, -1);
+    if (Tk.kind == TK_MAIN_CONV) {
+      const int nOut = rows * g.W;
+      for (int e = threadIdx.x; e < nOut; e += blockDim.x) {
+        const int rr = e / g.W, c = e - rr * g.W, io = io0 + rr;
+        if (io >= g.Hh) continue;
+        const cpx<T> v = out[rr * g.Nc + c + g.hl];
+        const int ra = pr.y + io;
+        if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)v.x, ea, true);
+        if (pr.z >= 0) {
+          const int rb = pr.w + io;
+          if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)v.y, eb, true);
+        }
+      }
+    } else {   // TK_REGION: rows of this block are whole groups of `scale` sub-rows
+      const int s = Tk.scale;
+      const int nbr = rows / s;
+      const double area = s * s;
+      for (int e = threadIdx.x; e < 2 * nbr * Tk.nbc; e += blockDim.x) {
+        const int plane = e / (nbr * Tk.nbc);
+        const int q = e - plane * nbr * Tk.nbc;
+        const int k = q / Tk.nbc, bj = q - k * Tk.nbc;
+        if (plane == 1 && pr.z < 0) continue;
+        const int io = io0 + k * s;   // first sub-row of this group in the plane
+        if (io >= g.Hh) continue;
+        const int subRow = (plane == 0 ? pr.y : pr.w) + io;
+        const int bi = subRow / s;
+        if (subRow + s > g.H || bi >= Tk.rowsRep) continue;
+        double binned = 0.0;
+        for (int ii = 0; ii < s; ii++)
+          for (int jj = 0; jj < s; jj++) {
+            const cpx<T> v = out[(k * s + ii) * g.Nc + bj * s + jj + g.hl];
+            binned += (double)(plane == 0 ? v.x : v.y);
+          }
+        const double m = binned / area;
+        if (plane == 0) stat_pixel<T>(sa, pr.x, Tk.r0 + bi, Tk.c0 + bj, m, ea, false);
+        else stat_pixel<T>(sa, pr.z, Tk.r0 + bi, Tk.c0 + bj, m, eb, false);
+      }
     }
   }
+  if (sa.partials) finish_block<T>(sa, Tk.slot0 + lb, ea, eb);
 }
 
 // ------------------------------------------------------------------ normal equations
@@ -904,20 +992,32 @@ namespace ifg {
 static size_t smem_rows(int rows, int N, size_t elem) { return (2 * (size_t)rows + 1) * N * elem; }
 
 template <typename T>
-hipError_t launch_tail(hipStream_t s, const EvalArgs& ea, const GridGeom& g, int nVec, int sub) {
-  const long waves = (long)nVec * ea.tailStride;   // upper bound of the work count
-  if (waves <= 0) return hipSuccess;
-  const long blocks = std::min<long>((waves + 3) / 4, 2048);
-  hipLaunchKernelGGL(k_tail<T>, dim3((unsigned)blocks), dim3(256), 0, s, ea, g, nVec, sub);
+hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub) {
+  if (maxWaves <= 0) return hipSuccess;
+  const long blocks = std::min<long>((maxWaves + 3) / 4, 2048);
+  hipLaunchKernelGGL(k_tail<T>, dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_eval_rowfft(hipStream_t s, const EvalArgs& ea, const GridGeom& g, const int4* pairs, int nPairs,
-                              cpx<T>* spec, long pairStride, const FFTPlan& pl, const cpx<T>* tw, int rows) {
-  dim3 grid((g.Lr + rows - 1) / rows, nPairs);
-  hipLaunchKernelGGL(k_eval_rowfft<T>, grid, dim3(256), smem_rows(rows, g.Nc, sizeof(cpx<T>)), s, ea, g, pairs,
-                     spec, pairStride, pl, tw, rows);
+hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds) {
+  if (nBlocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval_rowfft<T>, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_colconv(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds) {
+  if (nBlocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_colconv<T>, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
+                               const StatArgs<T>& sa) {
+  if (nBlocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inverse_stat<T>, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
   return hipGetLastError();
 }
 
@@ -930,40 +1030,16 @@ hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FF
 }
 
 template <typename T>
-hipError_t launch_colconv(hipStream_t s, const GridGeom& g, cpx<T>* spec, long pairStride, int nPairs,
-                          const cpx<T>* khat, const FFTPlan& pl, const cpx<T>* tw, int C, int mode) {
-  const int ld = g.Nr + 1;
-  dim3 grid((g.Nc + C - 1) / C, nPairs);
-  hipLaunchKernelGGL(k_colconv<T>, grid, dim3(256), (2 * (size_t)C * ld + g.Nr) * sizeof(cpx<T>), s, g, spec,
-                     pairStride,
-                     khat, pl, tw, C, ld, mode);
+hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C) {
+  const int ld = Nr + 1;
+  hipLaunchKernelGGL(k_colfft<T>, dim3((Nc + C - 1) / C), dim3(256), (2 * (size_t)C * ld + Nr) * sizeof(cpx<T>), s,
+                     buf, Nr, Nc, pl, tw, C, ld);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_rowifft_stat(hipStream_t s, const GridGeom& g, const int4* pairs, int nPairs, const cpx<T>* spec,
-                               long pairStride, const FFTPlan& pl, const cpx<T>* tw, const StatArgs<T>& sa, int rows,
-                               double agiant) {
-  dim3 grid((g.Hh + rows - 1) / rows, nPairs);
-  hipLaunchKernelGGL(k_rowifft_stat<T>, grid, dim3(256), smem_rows(rows, g.Nc, sizeof(cpx<T>)), s, g, pairs, spec,
-                     pairStride, pl, tw, sa, rows, agiant);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_eval_stat(hipStream_t s, const EvalArgs& ea, const GridGeom& g, const StatArgs<T>& sa, int nVec,
-                            int pixPerBlock, double agiant) {
-  const long npix = (long)g.W * g.H;
-  dim3 grid((unsigned)((npix + pixPerBlock - 1) / pixPerBlock), nVec);
-  hipLaunchKernelGGL(k_eval_stat<T>, grid, dim3(256), 0, s, ea, g, sa, pixPerBlock, agiant);
-  return hipGetLastError();
-}
-
-hipError_t launch_downsample(hipStream_t s, const double* osub, int Wos, long osubStride, double* ovals,
-                             int ovStride, int off, int nColsSub, int nRowsSub, int scale, int nVec) {
-  const int n = nColsSub * nRowsSub;
-  hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256, nVec), dim3(256), 0, s, osub, Wos, osubStride, ovals,
-                     ovStride, off, nColsSub, nRowsSub, scale, nVec);
+hipError_t launch_final(hipStream_t s, const int4* pairs, const double* partials, int nSlots, int nPairs, int mode,
+                        double* out) {
+  hipLaunchKernelGGL(k_final, dim3(nPairs), dim3(256), 0, s, pairs, partials, nSlots, mode, out);
   return hipGetLastError();
 }
 
@@ -985,23 +1061,13 @@ long normal_partials_size(long nData, int nFree) {
   return ((nData + TP - 1) / TP) * (long)(nFree * (nFree + 1) / 2 + nFree);
 }
 
-hipError_t launch_final(hipStream_t s, const double* partials, int nSlots, const int2* vecSlots, int nVec, int mode,
-                        double* out) {
-  hipLaunchKernelGGL(k_final, dim3(nVec), dim3(256), 0, s, partials, nSlots, vecSlots, mode, out);
-  return hipGetLastError();
-}
-
 #define IFG_INST(T)                                                                                                 \
-  template hipError_t launch_tail<T>(hipStream_t, const EvalArgs&, const GridGeom&, int, int);                     \
-  template hipError_t launch_eval_rowfft<T>(hipStream_t, const EvalArgs&, const GridGeom&, const int4*, int,        \
-                                            cpx<T>*, long, const FFTPlan&, const cpx<T>*, int);                      \
+  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, long, int, int);                               \
+  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, int, int, int, size_t);                      \
+  template hipError_t launch_colconv<T>(hipStream_t, const Task*, int, int, int, size_t);                          \
+  template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, int, int, int, size_t, const StatArgs<T>&); \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
-  template hipError_t launch_colconv<T>(hipStream_t, const GridGeom&, cpx<T>*, long, int, const cpx<T>*,          \
-                                        const FFTPlan&, const cpx<T>*, int, int);                                   \
-  template hipError_t launch_rowifft_stat<T>(hipStream_t, const GridGeom&, const int4*, int, const cpx<T>*, long,   \
-                                             const FFTPlan&, const cpx<T>*, const StatArgs<T>&, int, double);       \
-  template hipError_t launch_eval_stat<T>(hipStream_t, const EvalArgs&, const GridGeom&, const StatArgs<T>&, int,  \
-                                          int, double);
+  template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);
 IFG_INST(double)
 IFG_INST(float)
 
