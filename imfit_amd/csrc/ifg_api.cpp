@@ -393,7 +393,8 @@ int RowsPerBlock(int N, size_t elem) {
 int ColsPerBlock(int Nr, size_t elem) {
   const size_t budget = 56 * 1024;
   int C = (int)((budget - (size_t)Nr * elem) / (2 * (size_t)(Nr + 1) * elem));
-  return std::max(1, std::min(4, C));
+  C = std::max(1, std::min(4, C));
+  return C == 3 ? 2 : C;   // a power of two (the kernel splits threads by C)
 }
 
 // Build one convolution geometry and its PSF spectrum (scaled by 1/(Nr*Nc)),
@@ -753,7 +754,11 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         Tk.C = (sizeof(T) == 8) ? cg->C64 : cg->C32;
         Tk.ld = Tk.g.Nr + 1;
         Tk.b1 = b1; Tk.n1 = (Tk.g.Lr + Tk.rows1 - 1) / Tk.rows1;
-        Tk.b2 = b2; Tk.n2 = (Tk.g.Nc + Tk.C - 1) / Tk.C;
+        Tk.nStrips = (Tk.g.Nc + Tk.C - 1) / Tk.C;
+        Tk.stripsPerLine = std::max(1, 128 / (Tk.C * (int)elem));
+        // a multiple of 8 * stripsPerLine blocks keeps every task's range XCD-aligned
+        const int unit = 8 * Tk.stripsPerLine;
+        Tk.b2 = b2; Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
         Tk.b3 = b3; Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
         P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
         P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);

@@ -504,30 +504,33 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
   const int4 pr = Tk.pairs[blockIdx.y];
   const int ir0 = (blockIdx.x - Tk.b1) * rows;
-  const int total = rows * g.Nc;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int rr = e / g.Nc, j = e - rr * g.Nc, ir = ir0 + rr;
-    T va = 0, vb = 0;
-    if (ir < g.Lr && j < g.Lc) {
-      const int c = j - g.hl;
-      va = model_value<T>(Tk.ea, g, pr.x, pr.y + ir - g.hb, c);
-      if (pr.z >= 0) vb = model_value<T>(Tk.ea, g, pr.z, pr.w + ir - g.hb, c);
+  for (int rr = 0; rr < rows; rr++) {
+    const int ir = ir0 + rr;
+    for (int j = threadIdx.x; j < g.Nc; j += blockDim.x) {
+      T va = 0, vb = 0;
+      if (ir < g.Lr && j < g.Lc) {
+        const int c = j - g.hl;
+        va = model_value<T>(Tk.ea, g, pr.x, pr.y + ir - g.hb, c);
+        if (pr.z >= 0) vb = model_value<T>(Tk.ea, g, pr.z, pr.w + ir - g.hb, c);
+      }
+      A[rr * g.Nc + j] = cpx<T>{va, vb};
     }
-    A[e] = cpx<T>{va, vb};
   }
   __syncthreads();
-  const FFTPlan pl = Tk.pc;
+  const FFTPlan& pl = Tk.pc;
   cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, +1);
   cpx<T>* S = (cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int rr = e / g.Nc, ir = ir0 + rr;
-    if (ir < g.Lr) S[(long)ir * g.Nc + (e - rr * g.Nc)] = out[e];
-  }
+  for (int rr = 0; rr < rows && ir0 + rr < g.Lr; rr++)
+    for (int j = threadIdx.x; j < g.Nc; j += blockDim.x) S[(long)(ir0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
 }
 
 // ------------------------------------------------------------------ column pass
 // Forward column FFT of rows [0, Lr) (rest zero), multiply by the PSF spectrum,
-// inverse column FFT, store rows [hb, hb + Hh) -- for C columns per block.
+// inverse column FFT, store rows [hb, hb + Hh) -- for a strip of C columns per
+// block (C a power of two).  The G strips that share one 128-byte line go to
+// blocks on the same XCD (workgroups are spread over the 8 XCDs round-robin by
+// their index; every task's block range starts at a multiple of 8), so each line
+// is fetched into one L2 only.
 template <typename T>
 __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, int nTasks) {
   extern __shared__ char smem_raw[];
@@ -535,35 +538,42 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const GridGeom g = Tk.g;
-  const int C = Tk.C, ld = Tk.ld;
+  const int C = Tk.C, ld = Tk.ld, G = Tk.stripsPerLine;
+  const int lb = blockIdx.x - Tk.b2;
+  const int q = lb / (8 * G), rem = lb - q * 8 * G;
+  const int strip = (q * 8 + (rem & 7)) * G + (rem >> 3);
+  if (strip >= Tk.nStrips) return;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + C * ld;
   cpx<T>* TW = B + C * ld;
   stage_twiddles(TW, (const cpx<T>*)Tk.twr, g.Nr);
-  const int c0 = (blockIdx.x - Tk.b2) * C;
+  const int c0 = strip * C;
   cpx<T>* S = (cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
   const cpx<T>* khat = (const cpx<T>*)Tk.khat;
   const int Nr = g.Nr, Nc = g.Nc;
-  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
-    const int i = e / C, cc = e - i * C, col = c0 + cc;
-    cpx<T> v = {0, 0};
-    if (i < g.Lr && col < Nc) v = S[(long)i * Nc + col];
-    A[cc * ld + i] = v;
+  // thread -> (column cc, first row i0); rows advance by the block's row stride
+  const int cc = threadIdx.x & (C - 1), i0 = threadIdx.x / C, rstep = blockDim.x / C;
+  const int col = c0 + cc;
+  const bool colOk = col < Nc;
+  for (int i = i0; i < Nr; i += rstep) {
+    T vx = 0, vy = 0;
+    if (i < g.Lr && colOk) {
+      const cpx<T> v = S[(long)i * Nc + col];
+      vx = v.x;
+      vy = v.y;
+    }
+    A[cc * ld + i] = cpx<T>{vx, vy};
   }
   __syncthreads();
-  const FFTPlan pl = Tk.pr;
+  const FFTPlan& pl = Tk.pr;
   cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, TW, +1);
   cpx<T>* Y = (X == A) ? B : A;
-  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
-    const int cc = e / Nr, i = e - cc * Nr, col = c0 + cc;
-    if (col < Nc) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
-  }
+  if (colOk)
+    for (int i = i0; i < Nr; i += rstep) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
   __syncthreads();
   X = fft_lds<T>(X, Y, pl, C, ld, TW, -1);
-  for (int e = threadIdx.x; e < g.Hh * C; e += blockDim.x) {
-    const int i = g.hb + e / C, cc = e % C, col = c0 + cc;
-    if (col < Nc) S[(long)i * Nc + col] = X[cc * ld + i];
-  }
+  if (colOk)
+    for (int i = i0; i < g.Hh; i += rstep) S[(long)(g.hb + i) * Nc + col] = X[cc * ld + g.hb + i];
 }
 
 // Setup-time transforms of the PSF spectrum: in-place row FFTs and column FFTs.
@@ -876,26 +886,22 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     cpx<T>* TW = B + rows * g.Nc;
     stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
     const int io0 = lb * rows;
-    const int total = rows * g.Nc;
     const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int rr = e / g.Nc, io = io0 + rr;
-      A[e] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + (e - rr * g.Nc)] : cpx<T>{0, 0};
+    for (int rr = 0; rr < rows; rr++) {
+      const int io = io0 + rr;
+      for (int j = threadIdx.x; j < g.Nc; j += blockDim.x)
+        A[rr * g.Nc + j] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + j] : cpx<T>{0, 0};
     }
     __syncthreads();
-    const FFTPlan pl = Tk.pc;
+    const FFTPlan& pl = Tk.pc;
     cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, -1);
     if (Tk.kind == TK_MAIN_CONV) {
-      const int nOut = rows * g.W;
-      for (int e = threadIdx.x; e < nOut; e += blockDim.x) {
-        const int rr = e / g.W, c = e - rr * g.W, io = io0 + rr;
-        if (io >= g.Hh) continue;
-        const cpx<T> v = out[rr * g.Nc + c + g.hl];
-        const int ra = pr.y + io;
-        if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)v.x, ea, true);
-        if (pr.z >= 0) {
-          const int rb = pr.w + io;
-          if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)v.y, eb, true);
+      for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++) {
+        const int io = io0 + rr, ra = pr.y + io, rb = pr.w + io;
+        for (int c = threadIdx.x; c < g.W; c += blockDim.x) {
+          const cpx<T> v = out[rr * g.Nc + c + g.hl];
+          if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)v.x, ea, true);
+          if (pr.z >= 0 && rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)v.y, eb, true);
         }
       }
     } else {   // TK_REGION: rows of this block are whole groups of `scale` sub-rows
