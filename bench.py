@@ -108,6 +108,19 @@ def kernel_model(w, prec_bytes):
     return out
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from profiles/hbm_traffic_cfg2_1024.md (rocprofv3
+    FETCH_SIZE / WRITE_SIZE of this workload, scaled by a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2_1024.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(w, seconds):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -117,14 +130,14 @@ def cpu_baseline(w, seconds):
     o.GetFitStatistic(p)   # warm-up (first call touches every buffer)
     first = time.perf_counter() - t0
     n, t0 = 0, time.perf_counter()
-    while n < 20 and (time.perf_counter() - t0) < seconds:
+    while (time.perf_counter() - t0) < seconds:
         o.GetFitStatistic(p)
         n += 1
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "%d FP64 GetFitStatistic calls of the same 1024^2 workload after 1 warm-up (%.1f s); CPU "
+            "sample": "%d FP64 GetFitStatistic calls (~%.0f s) of the same 1024^2 workload after 1 warm-up; CPU "
                       "oracle restatement of the reference path with its own mixed-radix FFT (no FFTW on this "
-                      "host), OpenMP schedule(static,10) pixel loops" % (n, first)}
+                      "host), OpenMP schedule(static,10) pixel loops" % (n, dt)}
 
 
 def lm_benchmark(w, repeats):
@@ -158,7 +171,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--lm-repeats", type=int, default=3)
@@ -217,20 +230,23 @@ def main():
         m64.GetFitStatistic(p)
     device_sync()
     fp64_rate = n64 / (time.perf_counter() - t1)
+    stat64 = m64.GetFitStatistic(p)
+    stat_cur = m.GetFitStatistic(p)
+    prec_err = abs(stat_cur - stat64) / abs(stat64)
+    log("FP64 parity mode: %.1f evals/s; statistic %s vs FP64: rel diff %.3e" % (fp64_rate, args.precision, prec_err))
 
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel: algorithmic bytes per launch / its average
+    # event-timed duration; traffic = HBM bytes per launch from the committed
+    # FETCH_SIZE / WRITE_SIZE profile of the same workload
     prec_bytes = 4 if args.precision == "fp32" else 8
     km = kernel_model(w, prec_bytes)
     dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
-    roof = None
-    if dom in km:
-        ach = km[dom]["bytes"] / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-                "traffic": None, "kernel": dom, "avg_us": avg_s * 1e6}
-    else:
-        roof = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
-                "traffic": None, "kernel": dom, "avg_us": avg_s * 1e6}
+    traffic = profiled_traffic("ifg::%s<%s>" % (dom, "float" if args.precision == "fp32" else "double"))
+    ach = km[dom]["bytes"] / avg_s / 1e9 if dom in km else None
+    roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS if ach is not None else None, "traffic": traffic,
+            "kernel": dom, "avg_us": avg_s * 1e6, "alg_bytes": km.get(dom, {}).get("bytes")}
 
     # LM fits/s on the same workload: mpfit (MINPACK LMDIF) with a finite-difference
     # Jacobian -- one GPU ComputeDeviates per free parameter, QR on the host, as the
@@ -253,7 +269,7 @@ def main():
            "config": {"workload": w.name, "image": "1024x1024", "psf": "51x51 Moffat",
                       "oversampled_region": "40x40 at 5x, 255x255 PSF",
                       "precision": "%s pixels and FFT, FP64 deviate reduction" % args.precision,
-                      "fp64_evals_per_s": fp64_rate, "ms_per_step_with_kernel_events": ms_timed,
+                      "fp64_evals_per_s": fp64_rate, "fp32_vs_fp64_stat_rel": prec_err, "ms_per_step_with_kernel_events": ms_timed,
                       "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}},
            "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm}
     print(json.dumps(out))
