@@ -165,6 +165,39 @@ def lm_benchmark(w, repeats):
             "niter_normal": r_ne.niter, "start": "truth x U(0.9,1.1), centre +-5 px (seed 1)"}
 
 
+def aux_benchmarks(steps):
+    """Short measurements of the other single-GPU configurations, reported beside the
+    headline: the 3-D disk model (cfg4) and a 4096-vector population (cfg3)."""
+    import imfit_amd
+    from imfit_amd import workloads
+    out = {}
+    w4 = workloads.cfg4()
+    p4 = w4.start_params(seed=1, frac=0.05)
+    for prec, key in ((imfit_amd.PRECISION_FP32, "fp32"), (imfit_amd.PRECISION_FP64, "fp64")):
+        m = w4.fit_model(prec)
+        m.GetFitStatistic(p4)
+        device_sync()
+        n, t0 = max(3, steps // 20), time.perf_counter()
+        for _ in range(n):
+            m.GetFitStatistic(p4)
+        device_sync()
+        out.setdefault("cfg4_expdisk3d_512", {})["evals_per_s_" + key] = n / (time.perf_counter() - t0)
+    log("cfg4 (3-D disk + bulge, 512^2, PSF 25): %s" % out["cfg4_expdisk3d_512"])
+    w3, pop = workloads.cfg3_population()
+    m3 = w3.fit_model(imfit_amd.PRECISION_FP32)
+    m3.GetFitStatisticBatch(pop[:64])
+    device_sync()
+    t0 = time.perf_counter()
+    chi = m3.GetFitStatisticBatch(pop)
+    device_sync()
+    dt = time.perf_counter() - t0
+    out["cfg3_population_512"] = {"vectors": int(pop.shape[0]), "s_per_batch": dt,
+                                  "chi2_per_s": pop.shape[0] / dt, "finite": bool(np.isfinite(chi).all())}
+    log("cfg3 (4096-vector population, 512^2, PSF 35, FP32): %.3f s per batch, %.0f chi^2/s" %
+        (dt, pop.shape[0] / dt))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +209,7 @@ def main():
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--lm-repeats", type=int, default=3)
     ap.add_argument("--copy-calibration-mb", type=int, default=0)
+    ap.add_argument("--no-aux", action="store_true", help="skip the cfg3 / cfg4 side measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU sharding of the path is design-only "
@@ -257,6 +291,8 @@ def main():
     if not args.no_lm:
         lm = lm_benchmark(w, args.lm_repeats)
 
+    aux = None if args.no_aux else aux_benchmarks(args.steps)
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle baseline")
@@ -271,7 +307,7 @@ def main():
                       "precision": "%s pixels and FFT, FP64 deviate reduction" % args.precision,
                       "fp64_evals_per_s": fp64_rate, "fp32_vs_fp64_stat_rel": prec_err, "ms_per_step_with_kernel_events": ms_timed,
                       "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}},
-           "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm}
+           "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm, "other_configs": aux}
     print(json.dumps(out))
     if args.copy_calibration_mb > 0:
         copy_calibration(args.copy_calibration_mb)

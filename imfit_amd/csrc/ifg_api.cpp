@@ -75,7 +75,8 @@ int TypeOf(const char* name) {
 bool Subsampling(int type) { return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT; }
 
 // FunctionObject::Setup for one component (func_sersic.cpp:109-127, func_exp.cpp:104-118,
-// func_gaussian.cpp:104-121, func_moffat.cpp:111-132, func_flatsky.cpp:85-90).
+// func_gaussian.cpp:104-121, func_moffat.cpp:111-132, func_flatsky.cpp:85-90,
+// func_expdisk3d.cpp:149-172).
 void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
   std::memset(&R, 0, sizeof(R));
   double* v = R.v;
@@ -83,6 +84,17 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
   v[1] = y0;
   if (type == F_FLATSKY) {
     v[5] = p[0];
+    return;
+  }
+  if (type == F_EXPDISK3D) {   // ExponentialDisk3D::Setup (func_expdisk3d.cpp:149-172)
+    const double PA = p[0], inc = p[1], J_0 = p[2], h = p[3], n = p[4], z_0 = p[5];
+    const double PA_rad = (PA + 90.0) * DEG2RAD, inc_rad = inc * DEG2RAD;
+    const double alpha = 2.0 / n;
+    v[2] = cos(PA_rad); v[3] = sin(PA_rad);
+    v[4] = cos(inc_rad); v[7] = sin(inc_rad);
+    v[5] = J_0; v[6] = h; v[8] = alpha; v[9] = z_0; v[10] = alpha * z_0; v[11] = pow(2.0, alpha);
+    v[12] = 20.0 * h;   // INTEGRATION_MULTIPLIER * h (func_expdisk3d.cpp:95,215)
+    v[13] = n;
     return;
   }
   const double PA = p[0], ell = p[1];
@@ -549,8 +561,6 @@ int PrepareDevice(ifg_model* m) {
   if (m->devReady) return IFG_OK;
   if (!m->setupDone) return m->fail(IFG_ERR_STATE, "no image: call ifg_add_image_data or ifg_setup_model_image");
   if (m->types.empty()) return m->fail(IFG_ERR_STATE, "no functions added");
-  for (int t : m->types)
-    if (t == F_EXPDISK3D) return m->fail(IFG_ERR_UNSUPPORTED, "ExponentialDisk3D is not implemented on the GPU yet");
   if (m->dataSet) {
     if (!m->finalDone) return m->fail(IFG_ERR_STATE, "call ifg_final_setup before evaluating against data");
     int st = (m->precision == IFG_PRECISION_FP64) ? UploadImageArrays<double>(m) : UploadImageArrays<float>(m);
@@ -842,13 +852,14 @@ int Enqueue(ifg_model* m, Plan& P) {
   const bool deriv = IsDeriv(P.flags);
   const int mode = (P.flags & PF_JAC) ? S_JAC : (P.flags & PF_GRAD) ? S_GRAD : m->statMode();
   const int subsampling = m->doSubsampling ? 1 : 0;
+  const bool disk = std::find(m->types.begin(), m->types.end(), (int)F_EXPDISK3D) != m->types.end();
   for (int c = 0; c < P.nChunks; c++) {
     const int v0 = c * P.chunkVecs;
     const int nv = std::min(P.chunkVecs, nVec - v0);
     const int np = P.single ? 1 : (nv + 1) / 2;
     const Task* tk = P.tasks_d.as<Task>() + (size_t)c * P.nTasks;
-    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, P.tailWaves, nv, subsampling), "k_tail");
-    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, P.nTasks, P.nB1, np, P.lds1), "k_eval_rowfft");
+    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, P.tailWaves, nv, subsampling, disk), "k_tail");
+    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, P.nTasks, P.nB1, np, P.lds1, disk), "k_eval_rowfft");
     TIMED("k_colconv", launch_colconv<T>(s, tk, P.nTasks, P.nB2, np, P.lds2), "k_colconv");
     StatArgs<T> sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -888,7 +899,7 @@ int Enqueue(ifg_model* m, Plan& P) {
       sa.partials = P.partials_d.as<double>();
       sa.nSlots = P.nSlots;
     }
-    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, P.nTasks, P.nB3, np, P.lds3, sa), "k_inverse_stat");
+    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, P.nTasks, P.nB3, np, P.lds3, sa, disk), "k_inverse_stat");
     if (reduce) {
       const int4* pairsMain = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
       TIMED("k_final", launch_final(s, pairsMain, P.partials_d.as<double>(), P.nSlots, np, mode, P.out_d.as<double>() + v0),
@@ -1017,8 +1028,6 @@ int CheckFree(ifg_model* m, int n_free, const int* ifree, std::vector<int>& out)
     out[j] = ifree ? ifree[j] : j;
     if (out[j] < 0 || out[j] >= m->nParamsTot) return m->fail(IFG_ERR_ARG, "free-parameter index out of range");
   }
-  for (int t : m->types)
-    if (t == F_EXPDISK3D) return m->fail(IFG_ERR_UNSUPPORTED, "analytic derivatives of ExponentialDisk3D");
   return IFG_OK;
 }
 

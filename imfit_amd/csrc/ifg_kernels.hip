@@ -210,22 +210,240 @@ __device__ __forceinline__ double ell_radius(const double* v, double x, double y
   return sqrt(xp * xp + yp_scaled * yp_scaled);
 }
 
-// Point value without sub-pixel integration (bulk pixels).
+// ------------------------------------------------------------------ ExponentialDisk3D
+// Line-of-sight integral of a 3-D exponential disk with a sech^(2/n) vertical
+// profile (func_expdisk3d.cpp:177-268), over s in [-20h, 20h]
+// (INTEGRATION_MULTIPLIER, func_expdisk3d.cpp:95,215).  The reference integrates
+// adaptively (GSL qags, integrator.cpp:42-55); here a fixed-order rule per pixel:
+//   * the integrand has two sharp features along the line of sight -- the
+//     minimum of the cylindrical radius (y_d = 0, a kink when x_d = 0) and the
+//     mid-plane crossing (z_d = 0, a kink for large n) -- taken as breakpoints;
+//   * each of the four half-segments either side of them is mapped as
+//     s = p +- eps (e^u - 1), u in [0, log(1 + D/eps)], eps = the feature width
+//     at that breakpoint (|x_d| / sin i, 2 z_0 / (n cos i), at least 1e-3 px),
+//     so that both the feature and the exponential tail are smooth in u;
+//   * each half is integrated with `panels` 21-point Gauss-Kronrod panels in u.
+// Offsets are formed relative to the breakpoint (y_d, z_d there in FP64), so FP32
+// nodes keep their accuracy.  CompRec: v0 x0  v1 y0  v2 cosPA  v3 sinPA  v4 cos i
+// v5 J_0  v6 h  v7 sin i  v8 alpha = 2/n  v9 z_0  v10 alpha z_0  v11 2^alpha
+// v12 20 h  v13 n.
+// QUADPACK 21-point Kronrod abscissae and weights (Piessens et al. 1983, qk21).
+__constant__ double kXK21[11] = {
+    0.995657163025808080735527280689003, 0.973906528517171720077964012084452, 0.930157491355708226001207180059508,
+    0.865063366688984510732096688423493, 0.780817726586416897063717578345042, 0.679409568299024406234327365114874,
+    0.562757134668604683339000099272694, 0.433395394129247190799265943165784, 0.294392862701460198131126603103866,
+    0.148874338981631210884826001129720, 0.0};
+__constant__ double kWK21[11] = {
+    0.011694638867371874278064396062192, 0.032558162307964727478818972459390, 0.054755896574351996031381300244580,
+    0.075039674810919952767043140916190, 0.093125454583697605535065465083366, 0.109387158802297641899210590325805,
+    0.123491976262065851077600525478480, 0.134709217311473325928054001771707, 0.142775938577060080797094273138717,
+    0.147739104901338491374841515972068, 0.149445554002916905664936468389821};
+
+struct DiskLOS {
+  double xd0, yd0, zd0;        // line-of-sight start in the disk frame
+  double b1, e1, b2, e2, lim;  // breakpoints b1 <= b2 with their feature widths; limit 20 h
+  // half-segment q: 0 = [-lim, b1] from b1 down, 1 = [b1, mid] from b1 up,
+  // 2 = [mid, b2] from b2 down, 3 = [b2, lim] from b2 up
+  __device__ double p(int q) const { return q < 2 ? b1 : b2; }
+  __device__ double eps(int q) const { return q < 2 ? e1 : e2; }
+  __device__ double sgn(int q) const { return (q & 1) ? 1.0 : -1.0; }
+  __device__ double D(int q) const {
+    const double mid = 0.5 * (b1 + b2);
+    return q == 0 ? b1 + lim : q == 1 ? mid - b1 : q == 2 ? b2 - mid : lim - b2;
+  }
+};
+
+__device__ __forceinline__ void disk_setup(const double* v, double x, double y, DiskLOS& L) {
+  const double dx = x - v[0], dy = y - v[1];
+  const double xp = dx * v[2] + dy * v[3];
+  const double yp = -dx * v[3] + dy * v[2];
+  const double ci = v[4], si = v[7], lim = v[12];
+  L.xd0 = xp;
+  L.yd0 = yp * ci;
+  L.zd0 = yp * si;
+  const double aci = fabs(ci), asi = fabs(si);
+  double sR = 0.0, sZ = 0.0, eR = 1.0, eZ = 1.0;
+  if (asi > 1e-12) { sR = -L.yd0 / si; eR = fmax(fabs(L.xd0) / asi, 1e-3); }
+  if (aci > 1e-12) { sZ = L.zd0 / ci; eZ = fmax(v[10] / aci, 1e-3); }
+  if (!(asi > 1e-12)) { sR = sZ; eR = eZ; }
+  if (!(aci > 1e-12)) { sZ = sR; eZ = eR; }
+  sR = fmin(fmax(sR, -lim), lim);
+  sZ = fmin(fmax(sZ, -lim), lim);
+  L.b1 = sR; L.e1 = eR; L.b2 = sZ; L.e2 = eZ;
+  if (sZ < sR) { L.b1 = sZ; L.e1 = eZ; L.b2 = sR; L.e2 = eR; }
+  L.lim = lim;
+}
+
+// Vertical profile (func_expdisk3d.cpp:258-265; COSH_LIMIT = 100).
+__device__ __forceinline__ double disk_vertical(const double* v, double z) {
+  const double t = z / v[10];
+  if (t > 100.0) return v[11] * exp(-z / v[9]);
+  return pow(1.0 / cosh(t), v[8]);
+}
+// FP32 form: sech^a(t) = 2^a e^(-a t) (1 + e^(-2t))^(-a), no overflow of cosh.
+__device__ __forceinline__ float disk_vertical_f32(float z, float inv_z0, float inv_sz0, float a, float two_a) {
+  const float t = z * inv_sz0;
+  float v = two_a * __expf(-z * inv_z0);
+  if (t <= 100.0f) v *= __expf(-a * log1pf(__expf(-2.0f * t)));
+  return v;
+}
+
+// Integral of the density over the line of sight through (x, y), in T arithmetic
+// for the nodes (breakpoints and the mapping set up in FP64).
 template <typename T>
-__device__ __forceinline__ T comp_point(int type, const double* v, double x, double y);
+__device__ T disk_value(const double* v, double x, double y, int panels);
 template <>
-__device__ __forceinline__ double comp_point<double>(int type, const double* v, double x, double y) {
-  if (type == F_FLATSKY) return v[5];
+__device__ double disk_value<double>(const double* v, double x, double y, int panels) {
+  DiskLOS L;
+  disk_setup(v, x, y, L);
+  const double ci = v[4], si = v[7], J0 = v[5], h = v[6];
+  double sum = 0.0;
+  for (int q = 0; q < 4; q++) {
+    const double Dq = L.D(q), pq = L.p(q), eq = L.eps(q), se = L.sgn(q) * eq;
+    if (!(Dq > 0.0)) continue;
+    const double yb = L.yd0 + pq * si, zb = L.zd0 - pq * ci;
+    const double U = log1p(Dq / eq), du = U / panels;
+    for (int k = 0; k < panels; k++) {
+      const double uc = (k + 0.5) * du, hw = 0.5 * du;
+#pragma unroll 1
+      for (int j = 0; j < 21; j++) {
+        const double xk = (j < 11) ? -kXK21[j] : kXK21[20 - j];
+        const double wk = (j < 11) ? kWK21[j] : kWK21[20 - j];
+        const double u = uc + hw * xk;
+        const double e = expm1(u);
+        const double ds = se * e;
+        const double yd = yb + ds * si, zd = zb - ds * ci;
+        const double R = sqrt(L.xd0 * L.xd0 + yd * yd);
+        const double f = J0 * exp(-R / h) * disk_vertical(v, fabs(zd));
+        sum += wk * hw * eq * (e + 1.0) * f;
+      }
+    }
+  }
+  return sum;
+}
+template <>
+__device__ float disk_value<float>(const double* v, double x, double y, int panels) {
+  DiskLOS L;
+  disk_setup(v, x, y, L);
+  const float ci = (float)v[4], si = (float)v[7], J0 = (float)v[5];
+  const float inv_h = (float)(1.0 / v[6]), inv_z0 = (float)(1.0 / v[9]), inv_sz0 = (float)(1.0 / v[10]);
+  const float a = (float)v[8], two_a = (float)v[11];
+  const float xd2 = (float)(L.xd0 * L.xd0);
+  float sum = 0.0f;
+  for (int q = 0; q < 4; q++) {
+    const double Dq = L.D(q), pq = L.p(q), eq = L.eps(q);
+    if (!(Dq > 0.0)) continue;
+    const float yb = (float)(L.yd0 + pq * v[7]), zb = (float)(L.zd0 - pq * v[4]);
+    const float se = (float)(L.sgn(q) * eq), eps = (float)eq;
+    const float U = (float)log1p(Dq / eq), du = U / panels;
+    float part = 0.0f;
+    for (int k = 0; k < panels; k++) {
+      const float uc = (k + 0.5f) * du, hw = 0.5f * du;
+#pragma unroll 1
+      for (int j = 0; j < 21; j++) {
+        const float xk = (float)((j < 11) ? -kXK21[j] : kXK21[20 - j]);
+        const float wk = (float)((j < 11) ? kWK21[j] : kWK21[20 - j]);
+        const float e = expm1f(uc + hw * xk);
+        const float ds = se * e;
+        const float yd = yb + ds * si, zd = zb - ds * ci;
+        const float R = __builtin_sqrtf(xd2 + yd * yd);
+        const float f = __expf(-R * inv_h) * disk_vertical_f32(fabsf(zd), inv_z0, inv_sz0, a, two_a);
+        part += wk * hw * eps * (e + 1.0f) * f;
+      }
+    }
+    sum += part;
+  }
+  return J0 * sum;
+}
+
+// d/dp of the line-of-sight integral (FP64): the integrand's analytic derivative on
+// the same nodes, plus, for h, the moving limits (+-20 h; Leibniz rule).  Slots as
+// comp_grad: 0 X0, 1 Y0, 2 PA, 3 inc, 4 J_0, 5 h, 6 n, 7 z_0.  The breakpoints
+// move with the geometry, but the integrand is continuous in s, so the derivative
+// of the integral is the integral of the derivative.
+__device__ double disk_grad(const double* v, double x, double y, int slot, int panels) {
+  const double DEG2RAD = 0.017453292519943295;
+  DiskLOS L;
+  disk_setup(v, x, y, L);
+  const double cpa = v[2], spa = v[3], ci = v[4], si = v[7], J0 = v[5], h = v[6];
+  const double a = v[8], z0 = v[9], sz0 = v[10], n = v[13];
+  const double dx = x - v[0], dy = y - v[1];
+  const double xp = dx * cpa + dy * spa, yp = -dx * spa + dy * cpa;
+  // d(x_d0, y_d0, z_d0)/dp and d(direction)/dp for the geometric slots
+  double dxd = 0.0, dyd0 = 0.0, dzd0 = 0.0, dsi = 0.0, dci = 0.0;
+  if (slot == 0) { dxd = -cpa; dyd0 = spa * ci; dzd0 = spa * si; }
+  else if (slot == 1) { dxd = -spa; dyd0 = -cpa * ci; dzd0 = -cpa * si; }
+  else if (slot == 2) { dxd = yp * DEG2RAD; dyd0 = -xp * DEG2RAD * ci; dzd0 = -xp * DEG2RAD * si; }
+  else if (slot == 3) { dyd0 = -yp * si * DEG2RAD; dzd0 = yp * ci * DEG2RAD; dsi = ci * DEG2RAD; dci = -si * DEG2RAD; }
+  double sum = 0.0;
+  for (int q = 0; q < 4; q++) {
+    const double Dq = L.D(q), pq = L.p(q), eq = L.eps(q), se = L.sgn(q) * eq;
+    if (!(Dq > 0.0)) continue;
+    const double U = log1p(Dq / eq), du = U / panels;
+    for (int k = 0; k < panels; k++) {
+      const double uc = (k + 0.5) * du, hw = 0.5 * du;
+#pragma unroll 1
+      for (int j = 0; j < 21; j++) {
+        const double xk = (j < 11) ? -kXK21[j] : kXK21[20 - j];
+        const double wk = (j < 11) ? kWK21[j] : kWK21[20 - j];
+        const double e = expm1(uc + hw * xk);
+        const double s = pq + se * e;
+        const double yd = L.yd0 + s * si, zd = L.zd0 - s * ci;
+        const double R = sqrt(L.xd0 * L.xd0 + yd * yd), z = fabs(zd);
+        const double t = z / sz0;
+        const bool expBranch = t > 100.0;
+        const double V = expBranch ? v[11] * exp(-z / z0) : pow(1.0 / cosh(t), a);
+        const double f = J0 * exp(-R / h) * V;
+        const double th = expBranch ? 1.0 : tanh(t);
+        double g;
+        if (slot <= 3) {
+          const double dyd = dyd0 + s * dsi, dzd = dzd0 - s * dci;
+          const double dR = (R > 0.0) ? (L.xd0 * dxd + yd * dyd) / R : 0.0;
+          const double dz = (zd >= 0.0 ? 1.0 : -1.0) * dzd;
+          g = f * (-dR / h - th / z0 * dz);
+        } else if (slot == 4) g = f / J0;
+        else if (slot == 5) g = f * R / (h * h);
+        else if (slot == 6) g = f * (expBranch ? 0.69314718055994530942 : (log(1.0 / cosh(t)) + t * th)) * (-0.5 * a * a);
+        else g = f * (expBranch ? z / (z0 * z0) : a * t * th / z0);
+        sum += wk * hw * eq * (e + 1.0) * g;
+      }
+    }
+  }
+  (void)n;
+  if (slot == 5) {   // limits +-20 h
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      const double s = sgn * L.lim;
+      const double yd = L.yd0 + s * si, zd = L.zd0 - s * ci;
+      const double R = sqrt(L.xd0 * L.xd0 + yd * yd);
+      sum += 20.0 * J0 * exp(-R / h) * disk_vertical(v, fabs(zd));
+    }
+  }
+  return sum;
+}
+
+// Point value of a 2-D (elliptical) component without sub-pixel integration.
+template <typename T>
+__device__ __forceinline__ T comp_point_2d(int type, const double* v, double x, double y);
+template <>
+__device__ __forceinline__ double comp_point_2d<double>(int type, const double* v, double x, double y) {
   return intensity_f64(type, v, ell_radius(v, x, y));
 }
 template <>
-__device__ __forceinline__ float comp_point<float>(int type, const double* v, double x, double y) {
-  if (type == F_FLATSKY) return (float)v[5];
+__device__ __forceinline__ float comp_point_2d<float>(int type, const double* v, double x, double y) {
   float xd = (float)(x - v[0]), yd = (float)(y - v[1]);
   float c = (float)v[2], s = (float)v[3];
   float xp = xd * c + yd * s;
   float yp = (yd * c - xd * s) * (float)v[15];
   return intensity_f32_r2(type, v, xp * xp + yp * yp);
+}
+
+// Point value without sub-pixel integration (bulk pixels).  DISK: the model has an
+// ExponentialDisk3D component (kernels without one carry none of the quadrature).
+template <typename T, bool DISK>
+__device__ __forceinline__ T comp_point(int type, const double* v, double x, double y) {
+  if (type == F_FLATSKY) return (T)v[5];
+  if (DISK && type == F_EXPDISK3D) return disk_value<T>(v, x, y, sizeof(T) == 8 ? 3 : 2);
+  return comp_point_2d<T>(type, v, x, y);
 }
 
 // GALFIT sub-pixel count (func_sersic.cpp:197-213; func_exp.cpp:181-197;
@@ -251,10 +469,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 // d(point intensity)/d(parameter) of one component at (x, y).  slot 0 = X0,
 // slot 1 = Y0, slot 2+k = the component's own parameter k in the reference order
 // (Sersic: PA ell n I_e r_e; Exponential: PA ell I_0 h; Gaussian: PA ell I_0 sigma;
-// Moffat: PA ell I_0 fwhm beta; FlatSky: I_sky).  Geometry enters through the
+// Moffat: PA ell I_0 fwhm beta; FlatSky: I_sky; ExponentialDisk3D: PA inc J_0 h n
+// z_0, see disk_grad).  Geometry enters through the
 // elliptical radius r(x0, y0, PA, ell); at r = 0 the radial terms are taken as 0.
+template <bool DISK>
 __device__ double comp_grad(int type, const double* v, double x, double y, int slot) {
   if (type == F_FLATSKY) return slot == 2 ? 1.0 : 0.0;
+  if (DISK && type == F_EXPDISK3D) return disk_grad(v, x, y, slot, 3);
   const double DEG2RAD = 0.017453292519943295;
   const double c = v[2], s = v[3], q = v[4];
   const double x_diff = x - v[0], y_diff = y - v[1];
@@ -328,18 +549,19 @@ __device__ __forceinline__ int grad_slot(const EvalArgs& ea, int n, int gidx) {
 }
 
 // d(model)/d(parameter freeIdx[j]) at a grid pixel, without sub-pixel integration.
+template <bool DISK>
 __device__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
   const int gidx = ea.freeIdx[j];
   double sum = 0.0;
   for (int n = 0; n < ea.nComp; n++) {
     const int slot = grad_slot(ea, n, gidx);
-    if (slot >= 0) sum += comp_grad(ea.types[n], ea.recs[n].v, x, y, slot);
+    if (slot >= 0) sum += comp_grad<DISK>(ea.types[n], ea.recs[n].v, x, y, slot);
   }
   return sum;
 }
 
 // ------------------------------------------------------------------ model value per grid pixel
-template <typename T>
+template <typename T, bool DISK>
 __device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, int vec, int r, int c) {
   const Box* bx = ea.boxes + vec * ea.boxStride;
 #pragma unroll
@@ -351,11 +573,11 @@ __device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, 
   }
   const double x = g.xa + (double)c * g.frac;
   const double y = g.ya + (double)r * g.frac;
-  if (ea.deriv) return (T)deriv_point(ea, vec, x, y);
+  if (ea.deriv) return (T)deriv_point<DISK>(ea, vec, x, y);
   const CompRec* rec = ea.recs + (long)vec * ea.recStride;
   T newValSum = 0, storedError = 0;
   for (int n = 0; n < ea.nComp; n++) {   // Kahan summation (core/model_object.cpp:1164-1172)
-    T adjVal = comp_point<T>(ea.types[n], rec[n].v, x, y) - storedError;
+    T adjVal = comp_point<T, DISK>(ea.types[n], rec[n].v, x, y) - storedError;
     T tempSum = newValSum + adjVal;
     storedError = (tempSum - newValSum) - adjVal;
     newValSum = tempSum;
@@ -379,7 +601,7 @@ __device__ __forceinline__ int find_task(const Task* tasks, int nTasks, int whic
 // coordinates follow func_sersic.cpp:165-181 exactly (FP64); only the order of the
 // final sum differs (lane partials + xor butterfly, identical in every lane).
 // In derivative mode the wave forms the sub-pixel mean of d(intensity)/dp instead.
-template <typename T>
+template <typename T, bool DISK>
 __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int loc, int lane, int subsampling) {
   const Box* bx = ea.boxes + vec * ea.boxStride;
   int c = 0, r = 0;
@@ -415,11 +637,11 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
         double part = 0.0;
         for (int s = lane; s < n2; s += 64) {
           const int ii = s / nSub, jj = s - ii * nSub;
-          part += comp_grad(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, slot);
+          part += comp_grad<DISK>(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, slot);
         }
         dsum += wave_sum(part) / (nSub * nSub);
       } else {
-        dsum += comp_grad(type, v, x, y, slot);
+        dsum += comp_grad<DISK>(type, v, x, y, slot);
       }
     }
     if (lane == 0) *out = dsum;
@@ -448,6 +670,8 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
           part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
         }
         val = wave_sum(part) / (nSub * nSub);
+      } else if (DISK && type == F_EXPDISK3D) {
+        val = (double)comp_point<T, DISK>(type, v, x, y);
       } else {
         val = (double)intensity_r<T>(type, v, rad);
       }
@@ -463,7 +687,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
 // Grid-stride over (task, vector, box pixel) for every task of the launch: the
 // launch shape is fixed and the work count comes from device memory, so the launch
 // replays from a graph.
-template <typename T>
+template <typename T, bool DISK>
 __global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, int nTasks, int nVec, int subsampling) {
   const int lane = threadIdx.x & 63;
   long total = 0;
@@ -483,14 +707,14 @@ __global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, in
     const int used = *tasks[t].ea.tailUsed;
     const int vec = (int)(w / used);
     const int loc = (int)(w - (long)vec * used);
-    tail_pixel<T>(tasks[t].ea, tasks[t].g, vec, loc, lane, subsampling);
+    tail_pixel<T, DISK>(tasks[t].ea, tasks[t].g, vec, loc, lane, subsampling);
   }
 }
 
 // ------------------------------------------------------------------ evaluation + row FFT
 // Evaluates two real planes (one complex row: plane A real, plane B imaginary)
 // straight into LDS and runs the forward row FFT there.
-template <typename T>
+template <typename T, bool DISK>
 __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, int nTasks) {
   extern __shared__ char smem_raw[];
   const int t = find_task(tasks, nTasks, 1, blockIdx.x);
@@ -510,8 +734,8 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
       T va = 0, vb = 0;
       if (ir < g.Lr && j < g.Lc) {
         const int c = j - g.hl;
-        va = model_value<T>(Tk.ea, g, pr.x, pr.y + ir - g.hb, c);
-        if (pr.z >= 0) vb = model_value<T>(Tk.ea, g, pr.z, pr.w + ir - g.hb, c);
+        va = model_value<T, DISK>(Tk.ea, g, pr.x, pr.y + ir - g.hb, c);
+        if (pr.z >= 0) vb = model_value<T, DISK>(Tk.ea, g, pr.z, pr.w + ir - g.hb, c);
       }
       A[rr * g.Nc + j] = cpx<T>{va, vb};
     }
@@ -853,7 +1077,7 @@ __global__ __launch_bounds__(256) void k_final(const int4* __restrict__ pairs, c
 //   TK_REGION: inverse row FFT of the oversampled plane, s x s block mean of the
 //     sub-pixels (rows outer, columns inner, core/downsample.cpp:84-106) into the
 //     data pixels the region replaces, and their terms.
-template <typename T>
+template <typename T, bool DISK>
 __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, int nTasks, StatArgs<T> sa) {
   extern __shared__ char smem_raw[];
   const int t = find_task(tasks, nTasks, 3, blockIdx.x);
@@ -873,10 +1097,10 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
       if (p >= npix) break;
       const int r = (int)(p / g.W), c = (int)(p - (long)r * g.W);
       const int ra = pr.y + r;
-      if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)model_value<T>(Tk.ea, g, pr.x, ra, c), ea, true);
+      if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)model_value<T, DISK>(Tk.ea, g, pr.x, ra, c), ea, true);
       if (pr.z >= 0) {
         const int rb = pr.w + r;
-        if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)model_value<T>(Tk.ea, g, pr.z, rb, c), eb, true);
+        if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)model_value<T, DISK>(Tk.ea, g, pr.z, rb, c), eb, true);
       }
     }
   } else {
@@ -998,17 +1222,20 @@ namespace ifg {
 static size_t smem_rows(int rows, int N, size_t elem) { return (2 * (size_t)rows + 1) * N * elem; }
 
 template <typename T>
-hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub) {
+hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk) {
   if (maxWaves <= 0) return hipSuccess;
   const long blocks = std::min<long>((maxWaves + 3) / 4, 2048);
-  hipLaunchKernelGGL(k_tail<T>, dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
+  if (disk) hipLaunchKernelGGL((k_tail<T, true>), dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
+  else hipLaunchKernelGGL((k_tail<T, false>), dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds) {
+hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
+                              bool disk) {
   if (nBlocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_rowfft<T>, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  if (disk) hipLaunchKernelGGL((k_eval_rowfft<T, true>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  else hipLaunchKernelGGL((k_eval_rowfft<T, false>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
   return hipGetLastError();
 }
 
@@ -1021,9 +1248,10 @@ hipError_t launch_colconv(hipStream_t s, const Task* tasks, int nTasks, int nBlo
 
 template <typename T>
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                               const StatArgs<T>& sa) {
+                               const StatArgs<T>& sa, bool disk) {
   if (nBlocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_inverse_stat<T>, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
+  if (disk) hipLaunchKernelGGL((k_inverse_stat<T, true>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
+  else hipLaunchKernelGGL((k_inverse_stat<T, false>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
   return hipGetLastError();
 }
 
@@ -1068,10 +1296,11 @@ long normal_partials_size(long nData, int nFree) {
 }
 
 #define IFG_INST(T)                                                                                                 \
-  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, long, int, int);                               \
-  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, int, int, int, size_t);                      \
+  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, long, int, int, bool);                         \
+  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, int, int, int, size_t, bool);                \
   template hipError_t launch_colconv<T>(hipStream_t, const Task*, int, int, int, size_t);                          \
-  template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, int, int, int, size_t, const StatArgs<T>&); \
+  template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, int, int, int, size_t, const StatArgs<T>&,   \
+                                             bool);                                                                \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
   template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);
 IFG_INST(double)

@@ -108,6 +108,10 @@ class Workload:
                 t = self.truth[off + k]
                 if name != "FlatSky" and k == 0:
                     lim.append((-360.0, 360.0))
+                elif name == "ExponentialDisk3D" and k == 1:
+                    lim.append((0.0, 90.0))
+                elif name == "ExponentialDisk3D" and k == 4:
+                    lim.append((1.0, 200.0))
                 elif name != "FlatSky" and k == 1:
                     lim.append((0.0, 0.95))
                 elif name == "Sersic" and k == 2:
@@ -157,3 +161,24 @@ def cfg3_population(n=4096, seed=3):
     rng = np.random.default_rng(seed)
     pop = base.truth * rng.uniform(0.7, 1.3, size=(n, base.truth.size))
     return w, pop
+
+
+def gaussian_psf(size, sigma):
+    """Gaussian PSF (func_gaussian.cpp) at the pixel centres of an odd size x size grid
+    centred on pixel (size+1)/2, sub-pixel integration on, normalised to sum 1."""
+    m = ModelObjectGPU(0, PRECISION_FP64)
+    m.AddFunction("Gaussian", True)
+    m.SetupModelImage(size, size)
+    c = (size + 1) / 2.0
+    m.CreateModelImage(np.array([c, c, 0.0, 0.0, 1.0, sigma]))
+    p = m.GetModelImageVector()
+    return p / p.sum()
+
+
+def cfg4():
+    """512^2 ExponentialDisk3D (PA 0, inclination 85, J_0 1, h 50, n 100, z_0 5) +
+    Sersic bulge (PA 0, ell 0.1, n 1.5, I_e 20, r_e 15) sharing one centre, 25^2
+    Gaussian PSF (sigma 1.5), gain 4, read noise 1 e-, noise seed 4."""
+    truth = [256.5, 256.5, 0.0, 85.0, 1.0, 50.0, 100.0, 5.0, 0.0, 0.1, 1.5, 20.0, 15.0]
+    return Workload("cfg4_512_expdisk3d_sersic_psf25", [("ExponentialDisk3D", True), ("Sersic", False)], truth,
+                    512, 512, psf=gaussian_psf(25, 1.5), gain=4.0, read_noise=1.0, noise_seed=4)

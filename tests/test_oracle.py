@@ -208,3 +208,57 @@ def test_planted_mistake_is_caught():
     if np.array_equal(d["psf"], d["psf"].T):
         pytest.skip("PSF is symmetric")
     assert cases.float32_ulps(out, d["out"]) > 0
+
+
+# ---------------------------------------------------------------- QAGS / ExponentialDisk3D
+# No reference test evaluates a 3-D function, so the oracle's QAGS restatement
+# (GSL 2.5 gsl_integration_qags, function_objects/integrator.cpp:42-55) is pinned to
+# QUADPACK's dqagse through scipy.integrate.quad (fixtures written by
+# tests/golden/make_qags_fixtures.py), and the line-of-sight integrand
+# (function_objects/func_expdisk3d.cpp:177-268) to closed forms.
+
+def test_qags_matches_quadpack_on_test_integrands():
+    d = cases.golden("qags")
+    for (w, a, b), (v, e, n) in zip(d["test_cases"], d["test_results"]):
+        ov, oe, on = oracle.qags_test(int(w), a, b)
+        assert on == int(n), (w, a, b, on, n)          # same subdivision sequence
+        assert abs(ov - v) <= 1e-14 * abs(v), (w, ov, v)
+        assert abs(oe - e) <= 1e-12 * abs(e) + 1e-300, (w, oe, e)
+
+
+def test_qags_matches_quadpack_on_disk_integrals():
+    d = cases.golden("qags")
+    for c, (v, e, n) in zip(d["disk_cases"], d["disk_results"]):
+        ov, oe, on = oracle.expdisk3d_qags(c[:6], c[6], c[7], c[8], c[9])
+        assert on == int(n), (c, on, n)
+        assert abs(ov - v) <= 1e-13 * abs(v), (c, ov, v)
+
+
+def test_expdisk3d_face_on_closed_form():
+    """inc = 0: the line of sight is the z axis, R is fixed and
+    I = J_0 exp(-R/h) * (2 z_0/n) B(1/n, 1/2)  (the sech^(2/n) integral)."""
+    from scipy.special import beta
+    for (h, n, z0) in ((20.0, 2.0, 2.0), (40.0, 1.0, 3.0), (50.0, 100.0, 5.0), (30.0, 4.0, 1.5)):
+        J0 = 1.7
+        p = [25.0, 0.0, J0, h, n, z0]
+        for (x, y) in ((10.0, 10.0), (13.0, 8.0), (30.0, 25.0)):
+            v, _, _ = oracle.expdisk3d_qags(p, 10.0, 10.0, x, y)
+            R = math.hypot(x - 10.0, y - 10.0)
+            a = 2.0 / n
+            ref = J0 * math.exp(-R / h) * a * z0 * beta(a / 2.0, 0.5)
+            assert abs(v - ref) <= 2e-6 * ref, (h, n, z0, x, y, v, ref)
+
+
+def test_expdisk3d_edge_on_closed_form():
+    """inc = 90 deg: the height |z| = |y'| is fixed along the line of sight and
+    I = J_0 sech^(2/n)(n|y'|/(2 z_0)) * 2|x'| K_1(|x'|/h)  (x' = in-plane offset)."""
+    from scipy.special import k1
+    J0, h, n, z0 = 0.8, 25.0, 2.0, 3.0
+    p = [0.0, 90.0, J0, h, n, z0]
+    # PA 0 -> PA_rad = 90 deg: x' = y - y0, y' = -(x - x0)
+    for (x, y) in ((50.0, 57.0), (52.0, 70.0), (48.5, 40.0), (60.0, 90.0)):
+        v, _, _ = oracle.expdisk3d_qags(p, 50.0, 50.0, x, y)
+        xp, yp = y - 50.0, -(x - 50.0)
+        a = 2.0 / n
+        ref = J0 * (1.0 / math.cosh(abs(yp) / (a * z0))) ** a * 2.0 * abs(xp) * k1(abs(xp) / h)
+        assert abs(v - ref) <= 2e-6 * ref, (x, y, v, ref)
