@@ -746,11 +746,11 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         ea.boxStride = IFG_MAXBOX;
       }
       if (Tk.kind == TK_MAIN_PLAIN) {
-        Tk.pixPerBlock = 1024;
+        Tk.rows3 = std::max(1, 1024 / Tk.g.W);   // whole rows, about 1024 pixels per block
         Tk.b1 = b1; Tk.n1 = 0;
         Tk.b2 = b2; Tk.n2 = 0;
         Tk.b3 = b3;
-        Tk.n3 = (int)(((long)Tk.g.W * Tk.g.Hh + Tk.pixPerBlock - 1) / Tk.pixPerBlock);
+        Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
       } else {
         Tk.pc = cg->fc->plan;
         Tk.pr = cg->fr->plan;
@@ -858,8 +858,8 @@ int Enqueue(ifg_model* m, Plan& P) {
     const int nv = std::min(P.chunkVecs, nVec - v0);
     const int np = P.single ? 1 : (nv + 1) / 2;
     const Task* tk = P.tasks_d.as<Task>() + (size_t)c * P.nTasks;
-    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, P.tailWaves, nv, subsampling, disk), "k_tail");
-    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, P.nTasks, P.nB1, np, P.lds1, disk), "k_eval_rowfft");
+    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, P.tailWaves, nv, subsampling, disk, deriv), "k_tail");
+    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, P.nTasks, P.nB1, np, P.lds1, disk, deriv), "k_eval_rowfft");
     TIMED("k_colconv", launch_colconv<T>(s, tk, P.nTasks, P.nB2, np, P.lds2), "k_colconv");
     StatArgs<T> sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -899,7 +899,7 @@ int Enqueue(ifg_model* m, Plan& P) {
       sa.partials = P.partials_d.as<double>();
       sa.nSlots = P.nSlots;
     }
-    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, P.nTasks, P.nB3, np, P.lds3, sa, disk), "k_inverse_stat");
+    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, P.nTasks, P.nB3, np, P.lds3, sa, disk, deriv), "k_inverse_stat");
     if (reduce) {
       const int4* pairsMain = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
       TIMED("k_final", launch_final(s, pairsMain, P.partials_d.as<double>(), P.nSlots, np, mode, P.out_d.as<double>() + v0),

@@ -101,11 +101,11 @@ struct Task {
   long pairStride;
   const int4* pairs;           // (vecA, rowoffA, vecB or -1, rowoffB) per pair
   int b1, n1, b2, n2, b3, n3;  // block ranges in the evaluate / column / inverse launches
-  int rows1, rows3, C, ld;     // rows per block (row passes), columns per block and stride (column pass)
+  int rows1, rows3, C, ld;     // rows per block (row passes; TK_MAIN_PLAIN: rows3 only), columns per block
+                               // and stride (column pass)
   int nStrips, stripsPerLine;  // column pass: strips of C columns, strips sharing a 128-byte line
   int slot0;                   // first partial slot of this task's inverse-pass blocks
   int scale, nbc, rowsRep, c0, r0;   // region: data pixels [c0, c0+nbc) x [r0, r0+rowsRep) it replaces
-  int pixPerBlock;             // TK_MAIN_PLAIN: pixels per block
   EvalArgs ea;
 };
 

@@ -560,29 +560,57 @@ __device__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
   return sum;
 }
 
-// ------------------------------------------------------------------ model value per grid pixel
-template <typename T, bool DISK>
-__device__ __forceinline__ T model_value(const EvalArgs& ea, const GridGeom& g, int vec, int r, int c) {
-  const Box* bx = ea.boxes + vec * ea.boxStride;
+// ------------------------------------------------------------------ model values of grid rows
+// K pixels of one grid row r of vector vec (columns cols[k], valid where ok[k]),
+// grid pixel (c, r) at x = xa + c*frac, y = ya + r*frac: the sum over components
+// with Kahan compensation in component order (core/model_object.cpp:1164-1172),
+// the component loop outside the pixel loop so that each component's constants
+// (rec, in LDS) are read once per K pixels; pixels in a sub-pixel box (bx) take
+// their tail value.  DERIV: d(model)/d(parameter freeIdx[vec]) instead.
+template <typename T, bool DISK, bool DERIV, int K>
+__device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g, int vec, const CompRec* rec,
+                                           const Box* bx, int r, const int (&cols)[K], const bool (&ok)[K],
+                                           T (&out)[K]) {
+  const double y = g.ya + (double)r * g.frac;
+  if (DERIV) {
+#pragma unroll 1
+    for (int k = 0; k < K; k++)
+      out[k] = ok[k] ? (T)deriv_point<DISK>(ea, vec, g.xa + (double)cols[k] * g.frac, y) : (T)0;
+  } else {
+    T sum[K], err[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      sum[k] = 0;
+      err[k] = 0;
+    }
+    for (int n = 0; n < ea.nComp; n++) {
+      const int type = ea.types[n];
+      const double* v = rec[n].v;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (!ok[k]) continue;
+        const T adjVal = comp_point<T, DISK>(type, v, g.xa + (double)cols[k] * g.frac, y) - err[k];
+        const T tempSum = sum[k] + adjVal;
+        err[k] = (tempSum - sum[k]) - adjVal;
+        sum[k] = tempSum;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = sum[k];
+  }
 #pragma unroll
   for (int b = 0; b < IFG_MAXBOX; b++) {
     const Box B = bx[b];
-    const int dc = c - B.c0, dr = r - B.r0;
-    if (B.w > 0 && (unsigned)dc < (unsigned)B.w && (unsigned)dr < (unsigned)B.h)
-      return (T)ea.tail[(long)vec * ea.tailStride + B.off + dr * B.w + dc];
+    const int dr = r - B.r0;
+    if (B.w > 0 && (unsigned)dr < (unsigned)B.h) {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int dc = cols[k] - B.c0;
+        if (ok[k] && (unsigned)dc < (unsigned)B.w)
+          out[k] = (T)ea.tail[(long)vec * ea.tailStride + B.off + dr * B.w + dc];
+      }
+    }
   }
-  const double x = g.xa + (double)c * g.frac;
-  const double y = g.ya + (double)r * g.frac;
-  if (ea.deriv) return (T)deriv_point<DISK>(ea, vec, x, y);
-  const CompRec* rec = ea.recs + (long)vec * ea.recStride;
-  T newValSum = 0, storedError = 0;
-  for (int n = 0; n < ea.nComp; n++) {   // Kahan summation (core/model_object.cpp:1164-1172)
-    T adjVal = comp_point<T, DISK>(ea.types[n], rec[n].v, x, y) - storedError;
-    T tempSum = newValSum + adjVal;
-    storedError = (tempSum - newValSum) - adjVal;
-    newValSum = tempSum;
-  }
-  return newValSum;
 }
 
 // Task of block x in a merged launch (block ranges [b, b + n) per task).
@@ -601,7 +629,7 @@ __device__ __forceinline__ int find_task(const Task* tasks, int nTasks, int whic
 // coordinates follow func_sersic.cpp:165-181 exactly (FP64); only the order of the
 // final sum differs (lane partials + xor butterfly, identical in every lane).
 // In derivative mode the wave forms the sub-pixel mean of d(intensity)/dp instead.
-template <typename T, bool DISK>
+template <typename T, bool DISK, bool DERIV>
 __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int loc, int lane, int subsampling) {
   const Box* bx = ea.boxes + vec * ea.boxStride;
   int c = 0, r = 0;
@@ -620,14 +648,15 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
   const double y = g.ya + (double)r * g.frac;
   const CompRec* rec = ea.recs + (long)vec * ea.recStride;
   double* out = const_cast<double*>(ea.tail) + (long)vec * ea.tailStride + loc;
-  if (ea.deriv) {
+  if (DERIV) {
     const int gidx = ea.freeIdx[vec];
     double dsum = 0.0;
     for (int n = 0; n < ea.nComp; n++) {
       const int slot = grad_slot(ea, n, gidx);
       if (slot < 0) continue;
       const int type = ea.types[n];
-      const double* v = rec[n].v;
+      const CompRec rv = rec[n];   // into registers: read once, not per sample
+      const double* v = rv.v;
       const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, ell_radius(v, x, y), subsampling);
       if (nSub > 1) {
         const double deltaSubpix = 1.0 / nSub;
@@ -650,7 +679,8 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
   double newValSum = 0.0, storedError = 0.0;
   for (int n = 0; n < ea.nComp; n++) {
     const int type = ea.types[n];
-    const double* v = rec[n].v;
+    const CompRec rv = rec[n];   // into registers: read once, not per sample
+    const double* v = rv.v;
     double val;
     if (type == F_FLATSKY) {
       val = v[5];
@@ -663,11 +693,23 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
         const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
         const int n2 = nSub * nSub;
         double part = 0.0;
-        for (int s = lane; s < n2; s += 64) {
-          const int ii = s / nSub, jj = s - ii * nSub;
-          const double x_ii = x_sub_start + ii * deltaSubpix;
-          const double y_ii = y_sub_start + jj * deltaSubpix;
-          part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
+        if (sizeof(T) == 8) {   // FP64: sample coordinates exactly as the reference forms them
+          for (int s = lane; s < n2; s += 64) {
+            const int ii = s / nSub, jj = s - ii * nSub;
+            const double x_ii = x_sub_start + ii * deltaSubpix;
+            const double y_ii = y_sub_start + jj * deltaSubpix;
+            part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
+          }
+        } else {   // FP32 pixels: sample offsets in FP32 from the FP64 start of the pixel's grid
+          const float bx = (float)(x_sub_start - v[0]), by = (float)(y_sub_start - v[1]);
+          const float dl = (float)deltaSubpix, c = (float)v[2], sn = (float)v[3], iq = (float)v[15];
+          const float rn = 1.0f / (float)nSub;
+          for (int s = lane; s < n2; s += 64) {
+            const int ii = (int)(((float)s + 0.5f) * rn), jj = s - ii * nSub;
+            const float xd = bx + (float)ii * dl, yd = by + (float)jj * dl;
+            const float xp = xd * c + yd * sn, yp = (yd * c - xd * sn) * iq;
+            part += (double)intensity_f32_r2(type, v, xp * xp + yp * yp);
+          }
         }
         val = wave_sum(part) / (nSub * nSub);
       } else if (DISK && type == F_EXPDISK3D) {
@@ -687,7 +729,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
 // Grid-stride over (task, vector, box pixel) for every task of the launch: the
 // launch shape is fixed and the work count comes from device memory, so the launch
 // replays from a graph.
-template <typename T, bool DISK>
+template <typename T, bool DISK, bool DERIV>
 __global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, int nTasks, int nVec, int subsampling) {
   const int lane = threadIdx.x & 63;
   long total = 0;
@@ -707,14 +749,35 @@ __global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, in
     const int used = *tasks[t].ea.tailUsed;
     const int vec = (int)(w / used);
     const int loc = (int)(w - (long)vec * used);
-    tail_pixel<T, DISK>(tasks[t].ea, tasks[t].g, vec, loc, lane, subsampling);
+    tail_pixel<T, DISK, DERIV>(tasks[t].ea, tasks[t].g, vec, loc, lane, subsampling);
   }
+}
+
+// The component constants and sub-pixel boxes of a pair's two vectors, staged in
+// LDS so that the pixel loops read them as uniform broadcasts (the task table's
+// pointers are generic, which would otherwise make every read a per-lane load).
+// Derivative planes read their one shared record directly.  Ends with a barrier.
+template <bool DERIV>
+__device__ __forceinline__ void stage_pair(const Task& Tk, int4 pr, CompRec (*sRec)[IFG_MAXCOMP],
+                                           Box (*sBox)[IFG_MAXBOX]) {
+  const int nd = Tk.ea.nComp * 16;
+  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) {
+    const int pl = i >= nd, q = i - pl * nd;
+    const int vec = pl ? pr.z : pr.x;
+    if (!DERIV && vec >= 0) (&sRec[pl][0].v[0])[q] = (&Tk.ea.recs[(long)vec * Tk.ea.recStride].v[0])[q];
+  }
+  if (threadIdx.x < 2 * IFG_MAXBOX) {
+    const int pl = threadIdx.x / IFG_MAXBOX, b = threadIdx.x % IFG_MAXBOX;
+    const int vec = pl ? pr.z : pr.x;
+    if (vec >= 0) sBox[pl][b] = Tk.ea.boxes[vec * Tk.ea.boxStride + b];
+  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ evaluation + row FFT
 // Evaluates two real planes (one complex row: plane A real, plane B imaginary)
 // straight into LDS and runs the forward row FFT there.
-template <typename T, bool DISK>
+template <typename T, bool DISK, bool DERIV>
 __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, int nTasks) {
   extern __shared__ char smem_raw[];
   const int t = find_task(tasks, nTasks, 1, blockIdx.x);
@@ -727,17 +790,31 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   cpx<T>* TW = B + rows * g.Nc;
   stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
   const int4 pr = Tk.pairs[blockIdx.y];
+  __shared__ CompRec sRec[2][IFG_MAXCOMP];
+  __shared__ Box sBox[2][IFG_MAXBOX];
+  stage_pair<DERIV>(Tk, pr, sRec, sBox);
   const int ir0 = (blockIdx.x - Tk.b1) * rows;
+  constexpr int K = 4;   // pixels per thread per pass
   for (int rr = 0; rr < rows; rr++) {
     const int ir = ir0 + rr;
-    for (int j = threadIdx.x; j < g.Nc; j += blockDim.x) {
-      T va = 0, vb = 0;
-      if (ir < g.Lr && j < g.Lc) {
-        const int c = j - g.hl;
-        va = model_value<T, DISK>(Tk.ea, g, pr.x, pr.y + ir - g.hb, c);
-        if (pr.z >= 0) vb = model_value<T, DISK>(Tk.ea, g, pr.z, pr.w + ir - g.hb, c);
+    for (int j0 = threadIdx.x; j0 < g.Nc; j0 += K * blockDim.x) {
+      int cols[K];
+      bool ok[K];
+      T va[K], vb[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int j = j0 + k * blockDim.x;
+        cols[k] = j - g.hl;
+        ok[k] = ir < g.Lr && j < g.Lc;
+        vb[k] = 0;
       }
-      A[rr * g.Nc + j] = cpx<T>{va, vb};
+      row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], pr.y + ir - g.hb, cols, ok, va);
+      if (pr.z >= 0) row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], pr.w + ir - g.hb, cols, ok, vb);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int j = j0 + k * blockDim.x;
+        if (j < g.Nc) A[rr * g.Nc + j] = ok[k] ? cpx<T>{va[k], vb[k]} : cpx<T>{0, 0};
+      }
     }
   }
   __syncthreads();
@@ -903,8 +980,20 @@ __device__ __forceinline__ void en_merge(ENorm& a, const ENorm& b) {
 // Main-grid tasks skip pixels that an oversampled region replaces (the region task
 // produces those, core/downsample.cpp:96-104).
 template <typename T>
+__device__ __forceinline__ void stat_pixel_dw(const StatArgs<T>& sa, int vec, int r, int c, double m, double d,
+                                              double w0, ENorm& acc, bool skipRegions);
+template <typename T>
 __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r, int c, double m, ENorm& acc,
                                            bool skipRegions) {
+  const long z = (long)r * sa.W + c;
+  const bool needDW = sa.mode == S_CHI2 || sa.mode == S_CASH || sa.mode == S_MLR;
+  stat_pixel_dw<T>(sa, vec, r, c, m, needDW ? (double)sa.data[z] : 0.0, needDW ? (double)sa.weight[z] : 0.0, acc,
+                   skipRegions);
+}
+// The same, with the data value d and weight w0 of the pixel already loaded.
+template <typename T>
+__device__ __forceinline__ void stat_pixel_dw(const StatArgs<T>& sa, int vec, int r, int c, double m, double d,
+                                              double w0, ENorm& acc, bool skipRegions) {
   if (skipRegions)
     for (int k = 0; k < sa.nReg; k++) {
       const RegionOut& R = sa.regs[k];
@@ -921,8 +1010,7 @@ __device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r
     acc.s2 += sa.dfac[z] * m;
     return;
   }
-  const double d = (double)sa.data[z];
-  double w = (double)sa.weight[z];
+  double w = w0;
   if (sa.modelErrors && w != 0.0) {   // UpdateWeightVector (core/model_object.cpp:1345-1381)
     double totalFlux = m + sa.sky;
     double noise_squared = totalFlux / sa.effGain + sa.rnTerm;
@@ -1077,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_final(const int4* __restrict__ pairs, c
 //   TK_REGION: inverse row FFT of the oversampled plane, s x s block mean of the
 //     sub-pixels (rows outer, columns inner, core/downsample.cpp:84-106) into the
 //     data pixels the region replaces, and their terms.
-template <typename T, bool DISK>
+template <typename T, bool DISK, bool DERIV>
 __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, int nTasks, StatArgs<T> sa) {
   extern __shared__ char smem_raw[];
   const int t = find_task(tasks, nTasks, 3, blockIdx.x);
@@ -1089,18 +1177,36 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
   ENorm ea, eb;
   en_init(ea);
   en_init(eb);
-  if (Tk.kind == TK_MAIN_PLAIN) {   // rows [0, Hh) of plane A and of plane B, no PSF
-    const long npix = (long)g.W * g.Hh;
-    const long p0 = (long)lb * Tk.pixPerBlock;
-    for (int e = threadIdx.x; e < Tk.pixPerBlock; e += blockDim.x) {
-      const long p = p0 + e;
-      if (p >= npix) break;
-      const int r = (int)(p / g.W), c = (int)(p - (long)r * g.W);
-      const int ra = pr.y + r;
-      if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)model_value<T, DISK>(Tk.ea, g, pr.x, ra, c), ea, true);
-      if (pr.z >= 0) {
-        const int rb = pr.w + r;
-        if (rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)model_value<T, DISK>(Tk.ea, g, pr.z, rb, c), eb, true);
+  if (Tk.kind == TK_MAIN_PLAIN) {   // rows [0, Hh) of plane A and of plane B, no PSF: rows3 rows per block
+    __shared__ CompRec sRec[2][IFG_MAXCOMP];
+    __shared__ Box sBox[2][IFG_MAXBOX];
+    stage_pair<DERIV>(Tk, pr, sRec, sBox);
+    constexpr int K = 4;
+    for (int rr = 0; rr < Tk.rows3; rr++) {
+      const int r = lb * Tk.rows3 + rr;
+      if (r >= g.Hh) break;
+      const int ra = pr.y + r, rb = pr.w + r;
+      for (int j0 = threadIdx.x; j0 < g.W; j0 += K * blockDim.x) {
+        int cols[K];
+        bool ok[K];
+        T va[K], vb[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          cols[k] = j0 + k * blockDim.x;
+          ok[k] = cols[k] < g.W;
+        }
+        if (ra < sa.H) {
+          row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], ra, cols, ok, va);
+#pragma unroll
+          for (int k = 0; k < K; k++)
+            if (ok[k]) stat_pixel<T>(sa, pr.x, ra, cols[k], (double)va[k], ea, true);
+        }
+        if (pr.z >= 0 && rb < sa.H) {
+          row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rb, cols, ok, vb);
+#pragma unroll
+          for (int k = 0; k < K; k++)
+            if (ok[k]) stat_pixel<T>(sa, pr.z, rb, cols[k], (double)vb[k], eb, true);
+        }
       }
     }
   } else {
@@ -1111,6 +1217,28 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
     const int io0 = lb * rows;
     const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
+    // single-row main blocks: fetch the data and weights of this block's pixels
+    // before the inverse FFT so that their latency overlaps it
+    constexpr int K = 4;
+    const bool pre = Tk.kind == TK_MAIN_CONV && rows == 1 && g.W <= K * (int)blockDim.x && io0 < g.Hh &&
+                     (sa.mode == S_CHI2 || sa.mode == S_CASH || sa.mode == S_MLR);
+    T dA[K], wA[K], dB[K], wB[K];
+    if (pre) {
+      const int ra = pr.y + io0, rb = pr.w + io0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int c = threadIdx.x + k * blockDim.x;
+        dA[k] = wA[k] = dB[k] = wB[k] = 0;
+        if (c < g.W && ra < sa.H) {
+          dA[k] = sa.data[(long)ra * sa.W + c];
+          wA[k] = sa.weight[(long)ra * sa.W + c];
+        }
+        if (c < g.W && pr.z >= 0 && rb < sa.H) {
+          dB[k] = sa.data[(long)rb * sa.W + c];
+          wB[k] = sa.weight[(long)rb * sa.W + c];
+        }
+      }
+    }
     for (int rr = 0; rr < rows; rr++) {
       const int io = io0 + rr;
       for (int j = threadIdx.x; j < g.Nc; j += blockDim.x)
@@ -1119,7 +1247,17 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     __syncthreads();
     const FFTPlan& pl = Tk.pc;
     cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, -1);
-    if (Tk.kind == TK_MAIN_CONV) {
+    if (pre) {
+      const int ra = pr.y + io0, rb = pr.w + io0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int c = threadIdx.x + k * blockDim.x;
+        if (c >= g.W) continue;
+        const cpx<T> v = out[c + g.hl];
+        if (ra < sa.H) stat_pixel_dw<T>(sa, pr.x, ra, c, (double)v.x, (double)dA[k], (double)wA[k], ea, true);
+        if (pr.z >= 0 && rb < sa.H) stat_pixel_dw<T>(sa, pr.z, rb, c, (double)v.y, (double)dB[k], (double)wB[k], eb, true);
+      }
+    } else if (Tk.kind == TK_MAIN_CONV) {
       for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++) {
         const int io = io0 + rr, ra = pr.y + io, rb = pr.w + io;
         for (int c = threadIdx.x; c < g.W; c += blockDim.x) {
@@ -1221,21 +1359,30 @@ namespace ifg {
 // two ping-pong row buffers + the twiddle table
 static size_t smem_rows(int rows, int N, size_t elem) { return (2 * (size_t)rows + 1) * N * elem; }
 
+// Kernel variant for (disk, deriv): DISK carries the 3-D disk quadrature, DERIV the
+// analytic derivative planes; each is compiled only where a launch needs it.
+#define IFG_VARIANT(KERNEL, T, disk, deriv, ...)                                           \
+  do {                                                                                     \
+    if (disk && deriv) hipLaunchKernelGGL((KERNEL<T, true, true>), __VA_ARGS__);         \
+    else if (disk) hipLaunchKernelGGL((KERNEL<T, true, false>), __VA_ARGS__);            \
+    else if (deriv) hipLaunchKernelGGL((KERNEL<T, false, true>), __VA_ARGS__);           \
+    else hipLaunchKernelGGL((KERNEL<T, false, false>), __VA_ARGS__);                     \
+  } while (0)
+
 template <typename T>
-hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk) {
+hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk,
+                       bool deriv) {
   if (maxWaves <= 0) return hipSuccess;
   const long blocks = std::min<long>((maxWaves + 3) / 4, 2048);
-  if (disk) hipLaunchKernelGGL((k_tail<T, true>), dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
-  else hipLaunchKernelGGL((k_tail<T, false>), dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
+  IFG_VARIANT(k_tail, T, disk, deriv, dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                              bool disk) {
+                              bool disk, bool deriv) {
   if (nBlocks <= 0) return hipSuccess;
-  if (disk) hipLaunchKernelGGL((k_eval_rowfft<T, true>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
-  else hipLaunchKernelGGL((k_eval_rowfft<T, false>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  IFG_VARIANT(k_eval_rowfft, T, disk, deriv, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
   return hipGetLastError();
 }
 
@@ -1248,10 +1395,9 @@ hipError_t launch_colconv(hipStream_t s, const Task* tasks, int nTasks, int nBlo
 
 template <typename T>
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                               const StatArgs<T>& sa, bool disk) {
+                               const StatArgs<T>& sa, bool disk, bool deriv) {
   if (nBlocks <= 0) return hipSuccess;
-  if (disk) hipLaunchKernelGGL((k_inverse_stat<T, true>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
-  else hipLaunchKernelGGL((k_inverse_stat<T, false>), dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
+  IFG_VARIANT(k_inverse_stat, T, disk, deriv, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
   return hipGetLastError();
 }
 
@@ -1296,11 +1442,11 @@ long normal_partials_size(long nData, int nFree) {
 }
 
 #define IFG_INST(T)                                                                                                 \
-  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, long, int, int, bool);                         \
-  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, int, int, int, size_t, bool);                \
+  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, long, int, int, bool, bool);                   \
+  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, int, int, int, size_t, bool, bool);          \
   template hipError_t launch_colconv<T>(hipStream_t, const Task*, int, int, int, size_t);                          \
   template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, int, int, int, size_t, const StatArgs<T>&,   \
-                                             bool);                                                                \
+                                             bool, bool);                                                          \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
   template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);
 IFG_INST(double)

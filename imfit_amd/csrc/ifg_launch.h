@@ -4,17 +4,18 @@
 
 namespace ifg {
 // disk: the model contains an ExponentialDisk3D (selects the kernel variants that
-// carry its line-of-sight quadrature)
+// carry its line-of-sight quadrature); deriv: the planes are derivative planes
 template <typename T>
-hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk);
+hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk,
+                       bool deriv);
 template <typename T>
 hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                              bool disk);
+                              bool disk, bool deriv);
 template <typename T>
 hipError_t launch_colconv(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds);
 template <typename T>
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                               const StatArgs<T>& sa, bool disk);
+                               const StatArgs<T>& sa, bool disk, bool deriv);
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
                          int dir, int rows);
