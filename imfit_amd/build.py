@@ -16,6 +16,22 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
 
+def build_cpp_driver(verbose=False):
+    """The C++ caller tests/cpp/lm_fit_driver (ModelObject-shaped host class over the
+    C ABI), linked against the in-tree library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "lm_fit_driver.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(root, "tests", "cpp", "lm_fit_driver")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
+           "-L", HERE, "-limfit_gpu", "-Wl,-rpath,$ORIGIN/../../imfit_amd", "-o", out]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    return out
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs = []
@@ -37,3 +53,4 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_driver(verbose="-v" in sys.argv))

@@ -130,3 +130,35 @@ def test_fp32_jacobian_close_to_fp64():
     b = case.build(gpu_model(1)).NormalEquations(p)
     assert np.allclose(b[0], a[0], rtol=1e-4, atol=1e-5 * np.abs(a[0]).max())
     assert np.allclose(b[1], a[1], rtol=1e-3, atol=1e-4 * np.abs(a[1]).max())
+
+
+def test_jacobian_vs_mpfit_forward_differences():
+    """The analytic Jacobian against the reference's own numerical Jacobian rule:
+    mp_fdjac2's forward differences with h = sqrt(MP_MACHEP0) |x| (h = sqrt(eps) when
+    x = 0; solvers/mpfit.cpp:1241-1343), applied to the oracle's deviates.  Forward
+    differences carry an O(h) truncation error and eps |f| / h rounding noise, so
+    columns agree to ~1e-6 of their norm; pixels straddling a sub-pixel-count jump
+    within h are left out as in the central-difference test."""
+    announce("jacobian vs mpfit forward differences (textout3)")
+    case, _ = cases.textout3()
+    p = case.params.astype(float)
+    g = case.build(gpu_model())
+    ifree = np.arange(p.size)
+    fvec, J = g.Jacobian(p, ifree)
+    o = case.build(oracle.OracleModel(threads=8))
+    f0 = o.ComputeDeviates(p)
+    eps = np.sqrt(2.2204460e-16)
+    for j in ifree:
+        h = eps * abs(p[j]) if p[j] != 0 else eps
+        cols = []
+        for hh in (h, 0.5 * h):
+            pp = p.copy()
+            pp[j] += hh
+            cols.append((o.ComputeDeviates(pp) - f0) / hh)
+        scale = np.abs(cols[0]).max() + 1e-300
+        ok = np.abs(cols[0] - cols[1]) <= 1e-4 * scale + 1e-3 * np.abs(cols[0])
+        assert ok.mean() >= 0.999, (j, ok.mean())
+        noise = 100 * 2.2e-16 * np.linalg.norm(f0) / (0.5 * h)
+        num = np.linalg.norm(J[ok, j] - cols[0][ok])
+        den = np.linalg.norm(cols[0][ok])
+        assert num <= 1e-5 * den + noise, (j, num, den, noise)

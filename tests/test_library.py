@@ -28,3 +28,15 @@ def test_product_has_no_oracle_dependency():
                 with open(os.path.join(dirpath, fn)) as f:
                     txt = f.read()
                 assert "import oracle" not in txt and "liboracle" not in txt and "orc_" not in txt, fn
+
+
+def test_cpp_host_class_compiles(tmp_path):
+    """The ModelObject-shaped C++ host class compiles against the C header alone."""
+    import subprocess
+    root = ROOT
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "imfit_amd/host/model_object_gpu.hpp"\n'
+                   'int main() { imfit_gpu::ModelObjectGPU* p = nullptr; return p ? p->GetNParams() : 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", root,
+                        "-I", os.path.join(root, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
