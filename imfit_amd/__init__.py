@@ -81,6 +81,16 @@ _SIGS["ifg_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.c
                        C.c_int)
 _SIGS["ifg_fit_lm_normal"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
 
+_SIGS["ifg_batch_create"] = ([C.c_int], _vp)
+_SIGS["ifg_batch_destroy"] = ([_vp], None)
+_SIGS["ifg_batch_last_error"] = ([_vp], C.c_char_p)
+_SIGS["ifg_batch_add"] = ([_vp, _vp], C.c_int)
+_SIGS["ifg_batch_size"] = ([_vp], C.c_int)
+_SIGS["ifg_batch_fit_statistic"] = ([_vp, _dp, _dp], C.c_int)
+_SIGS["ifg_batch_normal_equations"] = ([_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp], C.c_int)
+_SIGS["ifg_batch_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
+_SIGS["ifg_batch_set_graph_mode"] = ([_vp, C.c_int], C.c_int)
+
 _LIB = C.CDLL(LIB_PATH)
 for _name, (_args, _res) in _SIGS.items():
     _f = getattr(_LIB, _name)
@@ -312,3 +322,98 @@ class ModelObjectGPU:
             out[name.value.decode()] = (float(ms[0]), int(n.value))
             i += 1
         return out
+
+
+class BatchGPU:
+    """Many set-up ModelObjectGPU models (e.g. cutouts of different sizes) evaluated
+    together in merged launches (the ifg_batch_* entry points).  Parameters are given
+    per model (a list of vectors); the models must share precision, statistic and
+    sub-pixel switch, and are kept alive by the batch."""
+
+    def __init__(self, models, device=0):
+        self.h = _LIB.ifg_batch_create(device)
+        if not self.h:
+            raise IfgError("ifg_batch_create failed")
+        self.models = list(models)
+        for m in self.models:
+            self._chk(_LIB.ifg_batch_add(self.h, m.h), "ifg_batch_add")
+        self.nparams = [m.GetNParams() for m in self.models]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _LIB.ifg_batch_destroy(self.h)
+            self.h = None
+
+    def _chk(self, st, what):
+        if st < 0:
+            raise IfgError("%s failed (%d): %s" % (what, st, _LIB.ifg_batch_last_error(self.h).decode()))
+        return st
+
+    def _concat(self, params):
+        if len(params) != len(self.models):
+            raise ValueError("one parameter vector per model")
+        for p, n in zip(params, self.nparams):
+            if len(p) != n:
+                raise ValueError("parameter vector of the wrong length")
+        return np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64) for p in params]))
+
+    def _split(self, x):
+        out, o = [], 0
+        for n in self.nparams:
+            out.append(x[o:o + n].copy())
+            o += n
+        return out
+
+    def GetFitStatistic(self, params):
+        x = self._concat(params)
+        out = np.zeros(len(self.models))
+        self._chk(_LIB.ifg_batch_fit_statistic(self.h, _ptr(x), _ptr(out)), "GetFitStatistic")
+        return out
+
+    def NormalEquations(self, params, ifree=None):
+        """Per model (J^T J as a full matrix, J^T r); and the statistics."""
+        x = self._concat(params)
+        if ifree is None:
+            ifree = [np.arange(n) for n in self.nparams]
+        nf = np.array([len(f) for f in ifree], dtype=np.int32)
+        fr = np.ascontiguousarray(np.concatenate([np.asarray(f) for f in ifree]).astype(np.int32))
+        total = int(sum(k * (k + 1) // 2 + k for k in nf))
+        normal = np.zeros(total)
+        stat = np.zeros(len(self.models))
+        self._chk(_LIB.ifg_batch_normal_equations(self.h, _ptr(x), nf.ctypes.data_as(C.POINTER(C.c_int)),
+                                                  fr.ctypes.data_as(C.POINTER(C.c_int)), _ptr(normal), _ptr(stat)),
+                  "NormalEquations")
+        out, o = [], 0
+        for k in nf:
+            jtj = np.zeros((k, k))
+            iu = np.triu_indices(k)
+            jtj[iu] = normal[o:o + k * (k + 1) // 2]
+            jtj = jtj + np.triu(jtj, 1).T
+            o += k * (k + 1) // 2
+            out.append((jtj, normal[o:o + k].copy()))
+            o += k
+        return out, stat
+
+    def FitLM(self, params, limits=None, fixed=None, ftol=1e-8, xtol=0.0, maxiter=1000):
+        """The normal-equation LM of every model at once.  limits / fixed: per model
+        (lists as for ModelObjectGPU.FitLM) or None.  Returns (params per model,
+        LMResult per model)."""
+        x = self._concat(params)
+        pinfo = None
+        if limits is not None or fixed is not None:
+            pinfo = (ParamInfo * x.size)()
+            o = 0
+            for k, n in enumerate(self.nparams):
+                part = ModelObjectGPU._pinfo(None if limits is None else limits[k],
+                                             None if fixed is None else fixed[k], n)
+                if part is not None:
+                    for i in range(n):
+                        pinfo[o + i] = part[i]
+                o += n
+        cfg = LMConfig(ftol, xtol, 0.0, maxiter, 0)
+        res = (LMResult * len(self.models))()
+        self._chk(_LIB.ifg_batch_fit_lm(self.h, _ptr(x), pinfo, C.byref(cfg), res), "FitLM")
+        return self._split(x), list(res)
+
+    def SetGraphMode(self, on):
+        return self._chk(_LIB.ifg_batch_set_graph_mode(self.h, int(on)), "SetGraphMode")

@@ -214,7 +214,13 @@ struct Plan {
   void* pinnedStage = nullptr;
   void* pinnedOut = nullptr;
   DevBuf stage_d, tail_d, pairs_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, tasks_d;
-  int nSlots = 1, nTasks = 1, nB1 = 0, nB2 = 0, nB3 = 0;
+  DevBuf maps_d, jobs_d, njobs_d, tileMap_d;
+  size_t usedOff = 0, prefixOff = 0;   // staging offsets of the per-grid box pixel counts / tail prefixes
+  int nSlots = 1, nTasks = 1;
+  std::vector<int> nB1, nB2, nB3, nJobs;   // per chunk: blocks of the three passes, final jobs
+  std::vector<long> mapOff, jobOff;        // per chunk: offsets into maps_d (ints) and jobs_d (jobs)
+  std::vector<BlockRanges> br1, br2, br3;  // per chunk: block -> task lookup of the three passes
+  int nTiles = 0;                          // normal equations (PF_JAC): tiles of the one job
   size_t lds1 = 0, lds2 = 0, lds3 = 0;
   long tailWaves = 0;
   hipGraph_t graph = nullptr;
@@ -606,6 +612,55 @@ int BoxAreaMax(const GridGeom& g) {
   return side * side;
 }
 
+// Statistic inputs of model m for a task (outputs set by the caller).
+template <typename T>
+void FillTaskStat(const ifg_model* m, TaskStat& st) {
+  std::memset(&st, 0, sizeof(st));
+  const int nR = (int)m->regions.size();
+  for (int k = 0; k < nR; k++) {
+    const Region& R = *m->regions[k];
+    RegionOut& ro = st.regs[k];
+    ro.c0 = R.x1 - 1;
+    ro.r0 = R.y1 - 1;
+    ro.w = R.nbc;
+    ro.h = R.rowsReplaced;
+  }
+  st.nReg = nR;
+  st.modelErrors = m->modelErrors ? 1 : 0;
+  st.W = m->W;
+  st.H = m->H;
+  st.nData = m->nData;
+  st.data = m->dataSet ? m->data_d.p : nullptr;
+  st.weight = m->dataSet ? m->weight_d.p : nullptr;
+  st.extra = (m->dataSet && m->poissonMLR) ? m->extra_d.p : nullptr;
+  st.effGain = m->effGain;
+  st.sky = m->originalSky;
+  st.rnTerm = m->nCombined * m->rnAdu2;
+  st.agiant = 1.304e19 / (double)std::max<long>(m->nData, 1);   // RGIANT / n (core/mp_enorm.cpp:66)
+}
+
+// Block layout of one chunk's tasks: task t owns nPairs * n blocks of each pass,
+// padded so that every task's column-pass range starts XCD-aligned; the maps give
+// each block its task.  Returns the block counts.
+void LayoutBlocks(std::vector<Task>& tk, int first, int count, std::vector<int>& m1, std::vector<int>& m2,
+                  std::vector<int>& m3, int& slot) {
+  int b1 = 0, b2 = 0, b3 = 0;
+  for (int t = first; t < first + count; t++) {
+    Task& Tk = tk[t];
+    Tk.b1 = b1;
+    Tk.b2 = b2;
+    Tk.b3 = b3;
+    Tk.slot0 = slot;
+    b1 += Tk.nPairs * Tk.n1;
+    b2 += Tk.nPairs * Tk.n2;
+    b3 += Tk.nPairs * Tk.n3;
+    slot += Tk.nPairs * Tk.n3;
+    m1.insert(m1.end(), (size_t)Tk.nPairs * Tk.n1, t - first);
+    m2.insert(m2.end(), (size_t)Tk.nPairs * Tk.n2, t - first);
+    m3.insert(m3.end(), (size_t)Tk.nPairs * Tk.n3, t - first);
+  }
+}
+
 template <typename T>
 int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>& minStride) {
   const int nC = m->nComp(), nR = (int)m->regions.size();
@@ -633,14 +688,6 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     P.strideMax[gi] = sm;
     P.tailOff[gi + 1] = P.tailOff[gi] + (long)sm * nVec;
   }
-  // staging layout: recs | boxes | tail pixel counts (derivative plans: one vector)
-  const int nRecVec = IsDeriv(flags) ? 1 : nVec;
-  P.recBytes = (size_t)nRecVec * nC * sizeof(CompRec);
-  P.boxBytes = (size_t)P.nGrids * nRecVec * IFG_MAXBOX * sizeof(Box);
-  P.stageBytes = P.recBytes + P.boxBytes + 64 * sizeof(int);
-  HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
-  HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
-  HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
 
   // chunks of the batch, so that the complex planes stay within ~1 GiB
   size_t pairBytes = 0, pairBytesR = 0;
@@ -658,6 +705,20 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / (pairBytes + pairBytesR)));
   P.chunkVecs = P.single ? 1 : 2 * P.chunkPairs;
   P.nChunks = (nVec + P.chunkVecs - 1) / P.chunkVecs;
+  P.nTasks = P.nGrids;
+
+  // staging layout: recs | boxes | box pixel counts per grid | tail prefixes per chunk
+  // (derivative plans: one parameter vector)
+  const int nRecVec = IsDeriv(flags) ? 1 : nVec;
+  P.recBytes = (size_t)nRecVec * nC * sizeof(CompRec);
+  P.boxBytes = (size_t)P.nGrids * nRecVec * IFG_MAXBOX * sizeof(Box);
+  P.usedOff = P.recBytes + P.boxBytes;
+  P.prefixOff = P.usedOff + ((P.nGrids * sizeof(int) + 15) / 16) * 16;
+  P.stageBytes = P.prefixOff + sizeof(long) * (size_t)P.nChunks * (P.nTasks + 1);
+  HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
+  HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
+  HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
+
   const long nData = m->nData;
   HIPCHK(P.spec_d.ensure(std::max<size_t>(1, pairBytes * P.chunkPairs)), "alloc planes");
   if (nR) HIPCHK(P.specR_d.ensure(std::max<size_t>(1, pairBytesR * P.chunkPairs)), "alloc region planes");
@@ -668,11 +729,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
                        hipHostMallocDefault),
          "pinned results");
   if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
+  if (flags & PF_JAC) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nFree), "alloc Jacobian columns");
   if (flags & PF_MODEL) HIPCHK(P.model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
-  if (flags & PF_JAC) {
-    HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nFree), "alloc Jacobian columns");
-    HIPCHK(P.normPart_d.ensure(sizeof(double) * normal_partials_size(nData, nFree)), "alloc normal partials");
-  }
   if (flags & (PF_DEVKEEP | PF_DFAC1 | PF_DFAC2)) {
     HIPCHK(m->devkeep_d.ensure(sizeof(double) * nData), "alloc kept deviates");
     HIPCHK(m->dfac_d.ensure(sizeof(double) * nData), "alloc dfac");
@@ -694,17 +752,24 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   HIPCHK(P.pairs_d.ensure(sizeof(int4) * pairs.size()), "alloc pairs");
   HIPCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
 
-  // task tables: main grid + every region, per chunk; block ranges of the merged launches
+  // task tables: main grid + every region, per chunk, with their block layouts
   const CompRec* recs_d = reinterpret_cast<const CompRec*>(P.stage_d.p);
   const Box* boxes_d = reinterpret_cast<const Box*>(P.stage_d.as<char>() + P.recBytes);
-  const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.recBytes + P.boxBytes);
+  const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.usedOff);
   const bool deriv = IsDeriv(flags);
   const size_t elem = sizeof(cpx<T>);
-  P.nTasks = P.nGrids;
   std::vector<Task> tasks((size_t)P.nChunks * P.nTasks);
+  std::vector<int> maps;
+  std::vector<FinalJob> jobs;
+  P.nB1.assign(P.nChunks, 0);
+  P.nB2.assign(P.nChunks, 0);
+  P.nB3.assign(P.nChunks, 0);
+  P.nJobs.assign(P.nChunks, 0);
+  P.mapOff.assign(P.nChunks, 0);
+  P.jobOff.assign(P.nChunks, 0);
+  int maxSlots = 0;
   for (int c = 0; c < P.nChunks; c++) {
     const int v0 = c * P.chunkVecs, nv = std::min(P.chunkVecs, nVec - v0);
-    int b1 = 0, b2 = 0, b3 = 0;
     for (int gi = 0; gi < P.nGrids; gi++) {
       Task& Tk = tasks[(size_t)c * P.nTasks + gi];
       std::memset(&Tk, 0, sizeof(Tk));
@@ -712,6 +777,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
       ConvGrid* cg = region ? &m->regions[gi - 1]->conv[P.which] : (m->doConv ? &m->mainConv[P.which] : nullptr);
       Tk.kind = region ? TK_REGION : (m->doConv ? TK_MAIN_CONV : TK_MAIN_PLAIN);
       Tk.nPairs = P.single ? 1 : (nv + 1) / 2;
+      Tk.nVec = nv;
       Tk.g = P.geoms[gi];
       Tk.pairs = P.pairs_d.as<int4>() + ((size_t)c * P.nGrids + gi) * npc;
       EvalArgs& ea = Tk.ea;
@@ -747,9 +813,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
       }
       if (Tk.kind == TK_MAIN_PLAIN) {
         Tk.rows3 = std::max(1, 1024 / Tk.g.W);   // whole rows, about 1024 pixels per block
-        Tk.b1 = b1; Tk.n1 = 0;
-        Tk.b2 = b2; Tk.n2 = 0;
-        Tk.b3 = b3;
+        Tk.n1 = Tk.n2 = 0;
         Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
       } else {
         Tk.pc = cg->fc->plan;
@@ -763,13 +827,13 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         Tk.rows3 = cg->rowsK3;
         Tk.C = (sizeof(T) == 8) ? cg->C64 : cg->C32;
         Tk.ld = Tk.g.Nr + 1;
-        Tk.b1 = b1; Tk.n1 = (Tk.g.Lr + Tk.rows1 - 1) / Tk.rows1;
+        Tk.n1 = (Tk.g.Lr + Tk.rows1 - 1) / Tk.rows1;
         Tk.nStrips = (Tk.g.Nc + Tk.C - 1) / Tk.C;
         Tk.stripsPerLine = std::max(1, 128 / (Tk.C * (int)elem));
         // a multiple of 8 * stripsPerLine blocks keeps every task's range XCD-aligned
         const int unit = 8 * Tk.stripsPerLine;
-        Tk.b2 = b2; Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
-        Tk.b3 = b3; Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
+        Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
+        Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
         P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
         P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
         P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
@@ -782,21 +846,89 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         Tk.c0 = R.x1 - 1;
         Tk.r0 = R.y1 - 1;
       }
-      Tk.slot0 = b3;
-      b1 += Tk.n1;
-      b2 += Tk.n2;
-      b3 += Tk.n3;
+      FillTaskStat<T>(m, Tk.st);
+      Tk.st.devOut = (flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
+      if (flags & PF_DEVKEEP) Tk.st.devOut = m->devkeep_d.as<double>();
+      Tk.st.modelOut = (flags & PF_MODEL) ? P.model_d.as<double>() + (long)v0 * nData : nullptr;
+      if (flags & (PF_DFAC1 | PF_DFAC2)) Tk.st.dfacOut = m->dfac_d.as<double>();
+      if (deriv) Tk.st.dfac = m->dfac_d.as<double>();
     }
-    P.nB1 = b1;
-    P.nB2 = b2;
-    P.nB3 = b3;
+    std::vector<int> m1, m2, m3;
+    int slot = 0;
+    LayoutBlocks(tasks, c * P.nTasks, P.nTasks, m1, m2, m3, slot);
+    maxSlots = std::max(maxSlots, slot);
+    P.nB1[c] = (int)m1.size();
+    P.nB2[c] = (int)m2.size();
+    P.nB3[c] = (int)m3.size();
+    P.mapOff[c] = (long)maps.size();
+    maps.insert(maps.end(), m1.begin(), m1.end());
+    maps.insert(maps.end(), m2.begin(), m2.end());
+    maps.insert(maps.end(), m3.begin(), m3.end());
+    BlockRanges r1, r2, r3;   // this plan's few tasks: block starts in the kernel arguments
+    std::memset(&r1, 0, sizeof(r1));
+    std::memset(&r2, 0, sizeof(r2));
+    std::memset(&r3, 0, sizeof(r3));
+    r1.n = r2.n = r3.n = P.nTasks;
+    for (int t = 0; t < P.nTasks; t++) {
+      const Task& Tk = tasks[(size_t)c * P.nTasks + t];
+      r1.start[t] = Tk.b1;
+      r2.start[t] = Tk.b2;
+      r3.start[t] = Tk.b3;
+    }
+    r1.start[P.nTasks] = P.nB1[c];
+    r2.start[P.nTasks] = P.nB2[c];
+    r3.start[P.nTasks] = P.nB3[c];
+    P.br1.push_back(r1);
+    P.br2.push_back(r2);
+    P.br3.push_back(r3);
+    // final jobs: one per pair, over the pair's slots in every grid
+    P.jobOff[c] = (long)jobs.size();
+    const int np = P.single ? 1 : (nv + 1) / 2;
+    for (int p = 0; p < np; p++) {
+      FinalJob J;
+      std::memset(&J, 0, sizeof(J));
+      const int4 pr = pairs[((size_t)c * P.nGrids) * npc + p];
+      J.halves = P.single ? 1 : 0;
+      J.outA = v0 + pr.x;
+      J.outB = (!P.single && pr.z >= 0) ? v0 + pr.z : -1;
+      J.nRange = P.nTasks;
+      for (int t = 0; t < P.nTasks; t++) {
+        const Task& Tk = tasks[(size_t)c * P.nTasks + t];
+        J.slot[t] = Tk.slot0 + p * Tk.n3;
+        J.count[t] = Tk.n3;
+      }
+      jobs.push_back(J);
+    }
+    P.nJobs[c] = np;
   }
-  P.nSlots = P.nB3;
+  P.nSlots = maxSlots;
   P.tailWaves = 0;
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
   HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
   HIPCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
-  HIPCHK(P.partials_d.ensure(sizeof(double) * 5 * 2 * (size_t)P.nSlots * npc), "alloc partials");
+  HIPCHK(P.maps_d.ensure(sizeof(int) * std::max<size_t>(1, maps.size())), "alloc block maps");
+  HIPCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice), "block maps");
+  HIPCHK(P.jobs_d.ensure(sizeof(FinalJob) * std::max<size_t>(1, jobs.size())), "alloc final jobs");
+  HIPCHK(hipMemcpy(P.jobs_d.p, jobs.data(), sizeof(FinalJob) * jobs.size(), hipMemcpyHostToDevice), "final jobs");
+  HIPCHK(P.partials_d.ensure(sizeof(double) * 10 * (size_t)std::max(1, P.nSlots)), "alloc partials");
+  if (flags & PF_JAC) {   // one normal-equations job over the Jacobian columns
+    NormalJob J;
+    std::memset(&J, 0, sizeof(J));
+    J.jcol = P.dev_d.as<double>();
+    J.r = m->devkeep_d.as<double>();
+    J.out = P.out_d.as<double>();
+    J.nData = nData;
+    J.nFree = nFree;
+    J.TP = normal_tile_pixels(nFree);
+    J.nTiles = (int)((nData + J.TP - 1) / J.TP);
+    P.nTiles = J.nTiles;
+    HIPCHK(P.normPart_d.ensure(sizeof(double) * (size_t)J.nTiles * nNormal), "alloc normal partials");
+    HIPCHK(P.njobs_d.ensure(sizeof(NormalJob)), "alloc normal job");
+    HIPCHK(hipMemcpy(P.njobs_d.p, &J, sizeof(J), hipMemcpyHostToDevice), "normal job");
+    std::vector<int> tmap(J.nTiles, 0);
+    HIPCHK(P.tileMap_d.ensure(sizeof(int) * tmap.size()), "alloc tile map");
+    HIPCHK(hipMemcpy(P.tileMap_d.p, tmap.data(), sizeof(int) * tmap.size(), hipMemcpyHostToDevice), "tile map");
+  }
   HIPCHK(hipDeviceSynchronize(), "plan setup");
   return IFG_OK;
 }
@@ -807,7 +939,8 @@ int StageCall(ifg_model* m, Plan& P, const double* params, std::vector<int>& nee
   const int nC = m->nComp(), nP = m->nParamsTot, nVec = IsDeriv(P.flags) ? 1 : P.nVec;
   CompRec* recs = reinterpret_cast<CompRec*>(P.pinnedStage);
   Box* boxes = reinterpret_cast<Box*>(reinterpret_cast<char*>(P.pinnedStage) + P.recBytes);
-  int* used = reinterpret_cast<int*>(reinterpret_cast<char*>(P.pinnedStage) + P.recBytes + P.boxBytes);
+  int* used = reinterpret_cast<int*>(reinterpret_cast<char*>(P.pinnedStage) + P.usedOff);
+  long* prefix = reinterpret_cast<long*>(reinterpret_cast<char*>(P.pinnedStage) + P.prefixOff);
   need.assign(P.nGrids, 0);
   for (int v = 0; v < nVec; v++) {
     const double* p = params + (long)v * nP;
@@ -838,79 +971,53 @@ int StageCall(ifg_model* m, Plan& P, const double* params, std::vector<int>& nee
     used[gi] = need[gi];
     grow = grow || need[gi] > P.strideMax[gi];
   }
+  // tail work per chunk and task: its vectors (planes) times the grid's box pixels
+  for (int c = 0; c < P.nChunks; c++) {
+    const int nv = std::min(P.chunkVecs, P.nVec - c * P.chunkVecs);
+    long* pf = prefix + (size_t)c * (P.nTasks + 1);
+    pf[0] = 0;
+    for (int t = 0; t < P.nTasks; t++) pf[t + 1] = pf[t] + (P.strideMax[t] > 0 ? (long)nv * used[t] : 0);
+  }
   return grow ? 1 : 0;
 }
 
 // Enqueue the whole evaluation on the handle's stream (captured into the graph):
-// the staging copy, then per chunk four merged launches over all grids.
+// the staging copy, then per chunk five merged launches over all grids.
 template <typename T>
 int Enqueue(ifg_model* m, Plan& P) {
-  const int nVec = P.nVec, nR = (int)m->regions.size();
-  const long nData = m->nData;
+  const int nVec = P.nVec;
   hipStream_t s = m->stream;
   HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
   const bool deriv = IsDeriv(P.flags);
   const int mode = (P.flags & PF_JAC) ? S_JAC : (P.flags & PF_GRAD) ? S_GRAD : m->statMode();
   const int subsampling = m->doSubsampling ? 1 : 0;
   const bool disk = std::find(m->types.begin(), m->types.end(), (int)F_EXPDISK3D) != m->types.end();
+  const long* prefix_d = reinterpret_cast<const long*>(P.stage_d.as<char>() + P.prefixOff);
+  const bool reduce = (mode != S_NONE && mode != S_JAC && (P.flags & PF_STAT));
+  StatLaunch sl;
+  sl.mode = mode;
+  sl.dfacKind = (P.flags & PF_DFAC2) ? 2 : 1;
+  sl.partials = reduce ? P.partials_d.as<double>() : nullptr;
   for (int c = 0; c < P.nChunks; c++) {
-    const int v0 = c * P.chunkVecs;
-    const int nv = std::min(P.chunkVecs, nVec - v0);
-    const int np = P.single ? 1 : (nv + 1) / 2;
     const Task* tk = P.tasks_d.as<Task>() + (size_t)c * P.nTasks;
-    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, P.tailWaves, nv, subsampling, disk, deriv), "k_tail");
-    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, P.nTasks, P.nB1, np, P.lds1, disk, deriv), "k_eval_rowfft");
-    TIMED("k_colconv", launch_colconv<T>(s, tk, P.nTasks, P.nB2, np, P.lds2), "k_colconv");
-    StatArgs<T> sa;
-    std::memset(&sa, 0, sizeof(sa));
-    for (int k = 0; k < nR; k++) {
-      const Region& R = *m->regions[k];
-      RegionOut ro;
-      std::memset(&ro, 0, sizeof(ro));
-      ro.c0 = R.x1 - 1;
-      ro.r0 = R.y1 - 1;
-      ro.w = R.nbc;
-      ro.h = R.rowsReplaced;
-      sa.regs[k] = ro;
-    }
-    sa.nReg = nR;
-    sa.mode = mode;
-    sa.modelErrors = m->modelErrors ? 1 : 0;
-    sa.W = m->W;
-    sa.H = m->H;
-    sa.nData = nData;
-    sa.data = m->dataSet ? m->data_d.as<T>() : nullptr;
-    sa.weight = m->dataSet ? m->weight_d.as<T>() : nullptr;
-    sa.extra = (m->dataSet && m->poissonMLR) ? m->extra_d.as<T>() : nullptr;
-    sa.effGain = m->effGain;
-    sa.sky = m->originalSky;
-    sa.rnTerm = m->nCombined * m->rnAdu2;
-    sa.agiant = 1.304e19 / (double)std::max<long>(nData, 1);
-    sa.devOut = (P.flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
-    if (P.flags & PF_DEVKEEP) sa.devOut = m->devkeep_d.as<double>();
-    sa.modelOut = (P.flags & PF_MODEL) ? P.model_d.as<double>() + (long)v0 * nData : nullptr;
-    if (P.flags & (PF_DFAC1 | PF_DFAC2)) {
-      sa.dfacOut = m->dfac_d.as<double>();
-      sa.dfacKind = (P.flags & PF_DFAC2) ? 2 : 1;
-    }
-    if (deriv) sa.dfac = m->dfac_d.as<double>();
-    const bool reduce = (mode != S_NONE && mode != S_JAC && (P.flags & PF_STAT));
-    if (reduce) {
-      sa.partials = P.partials_d.as<double>();
-      sa.nSlots = P.nSlots;
-    }
-    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, P.nTasks, P.nB3, np, P.lds3, sa, disk, deriv), "k_inverse_stat");
-    if (reduce) {
-      const int4* pairsMain = P.pairs_d.as<int4>() + (size_t)c * P.nGrids * P.npc;
-      TIMED("k_final", launch_final(s, pairsMain, P.partials_d.as<double>(), P.nSlots, np, mode, P.out_d.as<double>() + v0),
+    const BlockRanges &map1 = P.br1[c], &map2 = P.br2[c], &map3 = P.br3[c];
+    TIMED("k_tail", launch_tail<T>(s, tk, P.nTasks, prefix_d + (size_t)c * (P.nTasks + 1), P.tailWaves, subsampling,
+                                   disk, deriv),
+          "k_tail");
+    TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, map1, P.nB1[c], P.lds1, disk, deriv), "k_eval_rowfft");
+    TIMED("k_colconv", launch_colconv<T>(s, tk, map2, P.nB2[c], P.lds2), "k_colconv");
+    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv), "k_inverse_stat");
+    if (reduce)
+      TIMED("k_final",
+            launch_final(s, P.jobs_d.as<FinalJob>() + P.jobOff[c], P.nJobs[c], P.partials_d.as<double>(), mode,
+                         P.out_d.as<double>()),
             "k_final");
-    }
   }
   if (P.flags & PF_JAC) {
     const int nEnt = nVec * (nVec + 1) / 2 + nVec;
     TIMED("k_normal",
-          launch_normal(s, P.dev_d.as<double>(), m->devkeep_d.as<double>(), nData, nVec, P.normPart_d.as<double>(),
-                        P.out_d.as<double>()),
+          launch_normal(s, P.njobs_d.as<NormalJob>(), 1, P.tileMap_d.as<int>(), P.nTiles, nVec,
+                        P.normPart_d.as<double>()),
           "k_normal");
     HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nEnt, hipMemcpyDeviceToHost, s), "normal D2H");
   } else if (mode != S_NONE && (P.flags & PF_STAT)) {
@@ -1415,6 +1522,608 @@ int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_le
   if (total_ms) *total_ms = it->second.first;
   if (launches) *launches = it->second.second;
   return (int)m->kernelTimes.size();
+}
+
+}  // extern "C"
+
+// ======================================================================= batches
+// Many independent models (cutouts of different sizes, each set up as its own
+// ifg_model) evaluated together: every merged launch covers the grids of all the
+// models, through block -> task maps and per-task statistic records.  Per-call
+// activity flags (one per model, one per task) let the batched LM drop converged
+// models without changing the launch shapes, so each kind of call replays from
+// one captured HIP graph.
+enum BatchKind { BK_STAT = 0, BK_KEEP = 1, BK_JAC = 2 };
+
+struct BatchPlan {
+  int kind = BK_STAT;
+  std::vector<std::vector<int>> freeIdx;    // BK_JAC: per model
+  std::vector<std::vector<int>> minStride;  // per model and grid: tail strides asked for by earlier calls
+  int nModels = 0, nTasks = 0, nB1 = 0, nB2 = 0, nB3 = 0, nJobs = 0, nTiles = 0, maxFree = 0;
+  std::vector<int> firstTask, nGridM, nVecM;   // per model
+  std::vector<GridGeom> geoms;                 // per task
+  std::vector<int> strideMax;                  // per task
+  std::vector<size_t> recOff, boxOff;          // per model: staging offsets (bytes)
+  std::vector<long> outOff;                    // per model: first output (BK_JAC: packed normal equations)
+  size_t activeOff = 0, activeTaskOff = 0, usedOff = 0, prefixOff = 0, stageBytes = 0;
+  long outCount = 0;
+  void* pinnedStage = nullptr;
+  void* pinnedOut = nullptr;
+  DevBuf stage_d, tail_d, pairs_d, spec_d, partials_d, out_d, jac_d, tasks_d, maps_d, jobs_d, njobs_d, tileMap_d,
+      normPart_d;
+  BlockRanges br1, br2, br3;
+  size_t lds1 = 0, lds2 = 0, lds3 = 0;
+  long tailWaves = 0;
+  bool disk = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BatchPlan() = default;
+  BatchPlan(const BatchPlan&) = delete;
+  BatchPlan& operator=(const BatchPlan&) = delete;
+  ~BatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (pinnedStage) (void)hipHostFree(pinnedStage);
+    if (pinnedOut) (void)hipHostFree(pinnedOut);
+  }
+};
+
+struct ifg_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<ifg_model*> models;
+  std::vector<std::unique_ptr<BatchPlan>> plans;
+  bool useGraphs = true;
+  ~ifg_batch() {
+    plans.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+  int hipfail(hipError_t e, const char* where) {
+    err = std::string(where) + ": " + hipGetErrorString(e);
+    return IFG_ERR_HIP;
+  }
+};
+
+#define BCHK(expr, where)                               \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return b->hipfail(_e, where); \
+  } while (0)
+
+namespace {
+
+// The model's error message, prefixed with its index in the batch.
+int BatchModelFail(ifg_batch* b, int k, int st) {
+  b->err = "model " + std::to_string(k) + ": " + b->models[k]->err;
+  return st;
+}
+
+size_t Align16(size_t n) { return (n + 15) / 16 * 16; }
+
+template <typename T>
+int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
+  const int nM = (int)b->models.size();
+  const bool jac = P.kind == BK_JAC;
+  const size_t elem = sizeof(cpx<T>);
+  const int which = jac ? 1 : 0;   // derivative planes pair up; a single vector splits into halves
+  P.nModels = nM;
+  P.firstTask.assign(nM, 0);
+  P.nGridM.assign(nM, 0);
+  P.nVecM.assign(nM, 1);
+  P.recOff.assign(nM, 0);
+  P.boxOff.assign(nM, 0);
+  P.outOff.assign(nM, 0);
+  if ((int)P.minStride.size() != nM) P.minStride.assign(nM, std::vector<int>());
+  // pass 1: geometry, conv grids, sizes
+  size_t stage = 0, specBytes = 0, tailCount = 0, jacCount = 0;
+  long outCount = 0;
+  int st;
+  for (int k = 0; k < nM; k++) {
+    ifg_model* m = b->models[k];
+    if ((st = PrepareDevice(m)) < 0) return BatchModelFail(b, k, st);
+    if (m->doConv && (st = EnsureMainConv(m, which)) < 0) return BatchModelFail(b, k, st);
+    for (auto& R : m->regions)
+      if ((st = EnsureRegionConv(m, *R, which)) < 0) return BatchModelFail(b, k, st);
+    const int nG = 1 + (int)m->regions.size();
+    P.nGridM[k] = nG;
+    P.firstTask[k] = P.nTasks;
+    P.nTasks += nG;
+    const int nv = jac ? (int)P.freeIdx[k].size() : 1;
+    const int nPairs = jac ? (nv + 1) / 2 : 1;
+    P.nVecM[k] = nv;
+    P.maxFree = std::max(P.maxFree, nv);
+    for (int gi = 0; gi < nG; gi++) {
+      GridGeom g;
+      if (gi > 0) g = m->regions[gi - 1]->conv[which].g;
+      else if (m->doConv) g = m->mainConv[which].g;
+      else {
+        g = PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
+        g.Hh = jac ? m->H : (m->H + 1) / 2;
+      }
+      P.geoms.push_back(g);
+      int sm = m->doSubsampling ? IFG_MAXBOX * BoxAreaMax(g) : 0;
+      if (gi < (int)P.minStride[k].size()) sm = std::max(sm, P.minStride[k][gi]);
+      P.strideMax.push_back(sm);
+      tailCount += (size_t)sm * nv;
+      if (g.Nc > 0) specBytes += (size_t)g.Nr * g.Nc * elem * nPairs;
+    }
+    P.disk = P.disk || std::find(m->types.begin(), m->types.end(), (int)F_EXPDISK3D) != m->types.end();
+    P.recOff[k] = stage;
+    stage += (size_t)m->nComp() * sizeof(CompRec);
+    P.boxOff[k] = stage;
+    stage += (size_t)nG * IFG_MAXBOX * sizeof(Box);
+    P.outOff[k] = outCount;
+    outCount += jac ? nv * (nv + 1) / 2 + nv : 1;
+    if (jac) jacCount += (size_t)m->nData * nv;
+  }
+  P.outCount = outCount;
+  P.activeOff = stage;
+  stage += Align16(nM * sizeof(int));
+  P.activeTaskOff = stage;
+  stage += Align16(P.nTasks * sizeof(int));
+  P.usedOff = stage;
+  stage += Align16(P.nTasks * sizeof(int));
+  P.prefixOff = stage;
+  stage += sizeof(long) * (P.nTasks + 1);
+  P.stageBytes = stage;
+  BCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
+  std::memset(P.pinnedStage, 0, P.stageBytes);
+  BCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
+  BCHK(P.tail_d.ensure(std::max<size_t>(1, tailCount) * sizeof(double)), "alloc tail");
+  BCHK(P.spec_d.ensure(std::max<size_t>(1, specBytes)), "alloc planes");
+  BCHK(P.out_d.ensure(sizeof(double) * std::max<long>(1, outCount)), "alloc results");
+  BCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * std::max<long>(1, outCount), hipHostMallocDefault),
+       "pinned results");
+  if (jac) BCHK(P.jac_d.ensure(sizeof(double) * std::max<size_t>(1, jacCount)), "alloc Jacobian columns");
+  if (P.kind != BK_STAT)
+    for (ifg_model* m : b->models) {
+      BCHK(m->devkeep_d.ensure(sizeof(double) * m->nData), "alloc kept deviates");
+      BCHK(m->dfac_d.ensure(sizeof(double) * m->nData), "alloc dfac");
+    }
+
+  // pass 2: pairs, tasks, jobs
+  const char* stage_d = P.stage_d.as<char>();
+  const int* active_d = reinterpret_cast<const int*>(stage_d + P.activeOff);
+  const int* used_d = reinterpret_cast<const int*>(stage_d + P.usedOff);
+  std::vector<int4> pairs;
+  std::vector<size_t> pairOff(P.nTasks, 0);
+  std::vector<Task> tasks(P.nTasks);
+  size_t specOff = 0, tailOff = 0, jacOff = 0;
+  for (int k = 0; k < nM; k++) {
+    ifg_model* m = b->models[k];
+    const int nC = m->nComp(), nv = P.nVecM[k], nPairs = jac ? (nv + 1) / 2 : 1;
+    for (int gi = 0; gi < P.nGridM[k]; gi++) {
+      const int t = P.firstTask[k] + gi;
+      Task& Tk = tasks[t];
+      std::memset(&Tk, 0, sizeof(Tk));
+      const bool region = gi > 0;
+      ConvGrid* cg = region ? &m->regions[gi - 1]->conv[which] : (m->doConv ? &m->mainConv[which] : nullptr);
+      Tk.kind = region ? TK_REGION : (m->doConv ? TK_MAIN_CONV : TK_MAIN_PLAIN);
+      Tk.nPairs = nPairs;
+      Tk.nVec = nv;
+      Tk.g = P.geoms[t];
+      pairOff[t] = pairs.size();
+      if (!jac) pairs.push_back(make_int4(0, 0, 0, Tk.g.Hh));
+      else
+        for (int p = 0; p < nPairs; p++) pairs.push_back(make_int4(2 * p, 0, (2 * p + 1 < nv) ? 2 * p + 1 : -1, 0));
+      EvalArgs& ea = Tk.ea;
+      ea.nComp = nC;
+      for (int n = 0; n < nC; n++) ea.types[n] = m->types[n];
+      ea.tail = P.tail_d.as<double>() + tailOff;
+      ea.tailStride = P.strideMax[t];
+      tailOff += (size_t)P.strideMax[t] * nv;
+      ea.tailUsed = used_d + t;
+      ea.recs = reinterpret_cast<const CompRec*>(stage_d + P.recOff[k]);
+      ea.recStride = jac ? 0 : nC;
+      ea.boxes = reinterpret_cast<const Box*>(stage_d + P.boxOff[k]) + (long)gi * IFG_MAXBOX;
+      ea.boxStride = jac ? 0 : IFG_MAXBOX;
+      if (jac) {
+        ea.deriv = 1;
+        ea.nFree = nv;
+        for (int j = 0; j < nv; j++) ea.freeIdx[j] = P.freeIdx[k][j];
+        int off = 0, xy = 0;
+        for (int n = 0; n < nC; n++) {
+          if (m->fsetStart[n]) {
+            xy = off;
+            off += 2;
+          }
+          ea.compXY[n] = xy;
+          ea.compFirst[n] = off;
+          ea.compNP[n] = NParamsOf(m->types[n]);
+          off += ea.compNP[n];
+        }
+      }
+      if (Tk.kind == TK_MAIN_PLAIN) {
+        Tk.rows3 = std::max(1, 1024 / Tk.g.W);
+        Tk.n1 = Tk.n2 = 0;
+        Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
+      } else {
+        Tk.pc = cg->fc->plan;
+        Tk.pr = cg->fr->plan;
+        Tk.twc = (sizeof(T) == 8) ? cg->fc->tw64.p : cg->fc->tw32.p;
+        Tk.twr = (sizeof(T) == 8) ? cg->fr->tw64.p : cg->fr->tw32.p;
+        Tk.khat = (sizeof(T) == 8) ? cg->khat64.p : cg->khat32.p;
+        Tk.pairStride = (long)Tk.g.Nr * Tk.g.Nc;
+        Tk.spec = P.spec_d.as<char>() + specOff;
+        specOff += (size_t)Tk.pairStride * elem * nPairs;
+        Tk.rows1 = cg->rowsK1;
+        Tk.rows3 = cg->rowsK3;
+        Tk.C = (sizeof(T) == 8) ? cg->C64 : cg->C32;
+        Tk.ld = Tk.g.Nr + 1;
+        Tk.n1 = (Tk.g.Lr + Tk.rows1 - 1) / Tk.rows1;
+        Tk.nStrips = (Tk.g.Nc + Tk.C - 1) / Tk.C;
+        Tk.stripsPerLine = std::max(1, 128 / (Tk.C * (int)elem));
+        const int unit = 8 * Tk.stripsPerLine;
+        Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
+        Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
+        P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
+        P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
+        P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
+      }
+      if (region) {
+        const Region& R = *m->regions[gi - 1];
+        Tk.scale = R.scale;
+        Tk.nbc = R.nbc;
+        Tk.rowsRep = R.rowsReplaced;
+        Tk.c0 = R.x1 - 1;
+        Tk.r0 = R.y1 - 1;
+      }
+      FillTaskStat<T>(m, Tk.st);
+      if (P.kind == BK_KEEP) {
+        Tk.st.devOut = m->devkeep_d.as<double>();
+        Tk.st.dfacOut = m->dfac_d.as<double>();
+      }
+      if (jac) {
+        Tk.st.devOut = P.jac_d.as<double>() + jacOff;
+        Tk.st.dfac = m->dfac_d.as<double>();
+      }
+    }
+    if (jac) jacOff += (size_t)m->nData * nv;
+  }
+  BCHK(P.pairs_d.ensure(sizeof(int4) * std::max<size_t>(1, pairs.size())), "alloc pairs");
+  BCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
+  for (int t = 0; t < P.nTasks; t++) tasks[t].pairs = P.pairs_d.as<int4>() + pairOff[t];
+  std::vector<int> m1, m2, m3;
+  int slot = 0;
+  LayoutBlocks(tasks, 0, P.nTasks, m1, m2, m3, slot);
+  P.nB1 = (int)m1.size();
+  P.nB2 = (int)m2.size();
+  P.nB3 = (int)m3.size();
+  std::vector<int> maps(m1);
+  maps.insert(maps.end(), m2.begin(), m2.end());
+  maps.insert(maps.end(), m3.begin(), m3.end());
+  BCHK(P.maps_d.ensure(sizeof(int) * std::max<size_t>(1, maps.size())), "alloc block maps");
+  BCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice), "block maps");
+  std::memset(&P.br1, 0, sizeof(P.br1));
+  P.br3 = P.br2 = P.br1;
+  P.br1.map = P.maps_d.as<int>();
+  P.br2.map = P.br1.map + P.nB1;
+  P.br3.map = P.br2.map + P.nB2;
+  P.br1.active = P.br2.active = P.br3.active = reinterpret_cast<const int*>(stage_d + P.activeTaskOff);
+  BCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
+  BCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
+  BCHK(P.partials_d.ensure(sizeof(double) * 10 * (size_t)std::max(1, slot)), "alloc partials");
+  if (!jac) {   // one statistic per model, over its grids' slots
+    std::vector<FinalJob> jobs(nM);
+    for (int k = 0; k < nM; k++) {
+      FinalJob& J = jobs[k];
+      std::memset(&J, 0, sizeof(J));
+      J.halves = 1;
+      J.outA = (int)P.outOff[k];
+      J.outB = -1;
+      J.nRange = P.nGridM[k];
+      for (int gi = 0; gi < P.nGridM[k]; gi++) {
+        const Task& Tk = tasks[P.firstTask[k] + gi];
+        J.slot[gi] = Tk.slot0;
+        J.count[gi] = Tk.n3;
+      }
+      J.active = active_d + k;
+    }
+    P.nJobs = nM;
+    BCHK(P.jobs_d.ensure(sizeof(FinalJob) * std::max(1, nM)), "alloc final jobs");
+    BCHK(hipMemcpy(P.jobs_d.p, jobs.data(), sizeof(FinalJob) * nM, hipMemcpyHostToDevice), "final jobs");
+  } else {   // normal equations per model
+    std::vector<NormalJob> jobs(nM);
+    std::vector<int> tmap;
+    long part = 0;
+    size_t jo = 0;
+    for (int k = 0; k < nM; k++) {
+      ifg_model* m = b->models[k];
+      NormalJob& J = jobs[k];
+      std::memset(&J, 0, sizeof(J));
+      const int nv = P.nVecM[k];
+      J.jcol = P.jac_d.as<double>() + jo;
+      jo += (size_t)m->nData * nv;
+      J.r = m->devkeep_d.as<double>();
+      J.out = P.out_d.as<double>() + P.outOff[k];
+      J.nData = m->nData;
+      J.nFree = nv;
+      J.TP = normal_tile_pixels(nv);
+      J.nTiles = (int)((m->nData + J.TP - 1) / J.TP);
+      J.tile0 = (int)tmap.size();
+      J.part0 = part;
+      J.active = active_d + k;
+      part += (long)J.nTiles * (nv * (nv + 1) / 2 + nv);
+      tmap.insert(tmap.end(), J.nTiles, k);
+    }
+    P.nJobs = nM;
+    P.nTiles = (int)tmap.size();
+    BCHK(P.njobs_d.ensure(sizeof(NormalJob) * std::max(1, nM)), "alloc normal jobs");
+    BCHK(hipMemcpy(P.njobs_d.p, jobs.data(), sizeof(NormalJob) * nM, hipMemcpyHostToDevice), "normal jobs");
+    BCHK(P.tileMap_d.ensure(sizeof(int) * std::max<size_t>(1, tmap.size())), "alloc tile map");
+    BCHK(hipMemcpy(P.tileMap_d.p, tmap.data(), sizeof(int) * tmap.size(), hipMemcpyHostToDevice), "tile map");
+    BCHK(P.normPart_d.ensure(sizeof(double) * std::max<long>(1, part)), "alloc normal partials");
+  }
+  P.tailWaves = (long)tailOff;
+  BCHK(hipDeviceSynchronize(), "batch plan setup");
+  return IFG_OK;
+}
+
+// Per call: the activity flags, and for each active model its Setup constants,
+// boxes and box pixel counts; then the tail prefixes.  Returns 1 if a model's boxes
+// need a larger tail stride (the plan is then rebuilt with room for them).
+int StageBatch(ifg_batch* b, BatchPlan& P, const double* params, const int* active) {
+  char* stg = reinterpret_cast<char*>(P.pinnedStage);
+  int* act = reinterpret_cast<int*>(stg + P.activeOff);
+  int* actT = reinterpret_cast<int*>(stg + P.activeTaskOff);
+  int* used = reinterpret_cast<int*>(stg + P.usedOff);
+  long* prefix = reinterpret_cast<long*>(stg + P.prefixOff);
+  bool grow = false;
+  long poff = 0, acc = 0;
+  for (int k = 0; k < P.nModels; k++) {
+    ifg_model* m = b->models[k];
+    const int on = active ? (active[k] != 0) : 1;
+    const double* p = params + poff;
+    poff += m->nParamsTot;
+    act[k] = on;
+    CompRec* recs = reinterpret_cast<CompRec*>(stg + P.recOff[k]);
+    Box* boxes = reinterpret_cast<Box*>(stg + P.boxOff[k]);
+    if (on) {
+      int offset = 0;
+      double x0 = 0, y0 = 0;
+      for (int n = 0; n < m->nComp(); n++) {   // core/model_object.cpp:1126-1135
+        if (m->fsetStart[n]) {
+          x0 = p[offset];
+          y0 = p[offset + 1];
+          offset += 2;
+        }
+        SetupRec(m->types[n], p + offset, x0, y0, recs[n]);
+        offset += NParamsOf(m->types[n]);
+      }
+    }
+    for (int gi = 0; gi < P.nGridM[k]; gi++) {
+      const int t = P.firstTask[k] + gi;
+      actT[t] = on;
+      used[t] = 0;
+      if (on) {
+        const GridGeom& g = P.geoms[t];
+        const bool conv = g.Lc > 0;
+        const int cLo = -g.hl, cHi = conv ? g.Lc - g.hl - 1 : g.W - 1;
+        const int rLo = -g.hb, rHi = conv ? g.H + (g.Lr - g.Hh) : g.H - 1;
+        used[t] = MakeBoxes(m, recs, g, cLo, cHi, rLo, rHi, boxes + (long)gi * IFG_MAXBOX);
+        if (used[t] > P.strideMax[t]) {
+          grow = true;
+          if ((int)P.minStride[k].size() < P.nGridM[k]) P.minStride[k].resize(P.nGridM[k], 0);
+          P.minStride[k][gi] = std::max(P.minStride[k][gi], used[t]);
+        }
+      }
+      prefix[t] = acc;   // tail work: the task's vectors times its box pixels
+      if (on && P.strideMax[t] > 0) acc += (long)P.nVecM[k] * used[t];
+    }
+  }
+  prefix[P.nTasks] = acc;
+  return grow ? 1 : 0;
+}
+
+template <typename T>
+int EnqueueBatch(ifg_batch* b, BatchPlan& P) {
+  hipStream_t s = b->stream;
+  BCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
+  ifg_model* m0 = b->models[0];
+  const bool deriv = P.kind == BK_JAC;
+  const int mode = deriv ? S_JAC : m0->statMode();
+  const int sub = m0->doSubsampling ? 1 : 0;
+  const long* prefix_d = reinterpret_cast<const long*>(P.stage_d.as<char>() + P.prefixOff);
+  const Task* tk = P.tasks_d.as<Task>();
+  StatLaunch sl;
+  sl.mode = mode;
+  sl.dfacKind = 1;
+  sl.partials = deriv ? nullptr : P.partials_d.as<double>();
+  hipError_t e;
+  if ((e = launch_tail<T>(s, tk, P.nTasks, prefix_d, P.tailWaves, sub, P.disk, deriv)) != hipSuccess)
+    return b->hipfail(e, "k_tail");
+  if ((e = launch_eval_rowfft<T>(s, tk, P.br1, P.nB1, P.lds1, P.disk, deriv)) != hipSuccess)
+    return b->hipfail(e, "k_eval_rowfft");
+  if ((e = launch_colconv<T>(s, tk, P.br2, P.nB2, P.lds2)) != hipSuccess) return b->hipfail(e, "k_colconv");
+  if ((e = launch_inverse_stat<T>(s, tk, P.br3, P.nB3, P.lds3, sl, P.disk, deriv)) != hipSuccess)
+    return b->hipfail(e, "k_inverse_stat");
+  if (!deriv)
+    e = launch_final(s, P.jobs_d.as<FinalJob>(), P.nJobs, P.partials_d.as<double>(), mode, P.out_d.as<double>());
+  else
+    e = launch_normal(s, P.njobs_d.as<NormalJob>(), P.nJobs, P.tileMap_d.as<int>(), P.nTiles, P.maxFree,
+                      P.normPart_d.as<double>());
+  if (e != hipSuccess) return b->hipfail(e, deriv ? "k_normal" : "k_final");
+  BCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * P.outCount, hipMemcpyDeviceToHost, s), "results D2H");
+  return IFG_OK;
+}
+
+int BatchCheck(ifg_batch* b) {
+  if (b->models.empty()) return b->fail(IFG_ERR_STATE, "empty batch");
+  const ifg_model* m0 = b->models[0];
+  for (size_t k = 0; k < b->models.size(); k++) {
+    const ifg_model* m = b->models[k];
+    const std::string id = "model " + std::to_string(k) + ": ";
+    if (!m->dataSet || !m->finalDone) return b->fail(IFG_ERR_STATE, id + "no data or no ifg_final_setup");
+    if (m->precision != m0->precision) return b->fail(IFG_ERR_ARG, id + "models of a batch share the precision");
+    if (m->statMode() != m0->statMode()) return b->fail(IFG_ERR_ARG, id + "models of a batch share the statistic");
+    if (m->doSubsampling != m0->doSubsampling) return b->fail(IFG_ERR_ARG, id + "models of a batch share subsampling");
+    if (m->device != b->device) return b->fail(IFG_ERR_ARG, id + "models of a batch share the device");
+  }
+  return IFG_OK;
+}
+
+// One batched evaluation of `kind` for the active models (active: per-model flags,
+// null = all); results stay in the plan's pinned output (per model at outOff).
+template <typename T>
+int RunBatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeIdx, const double* params,
+             const int* active, BatchPlan** out) {
+  BatchPlan* P = nullptr;
+  for (auto& p : b->plans)
+    if (p->kind == kind && (kind != BK_JAC || p->freeIdx == *freeIdx)) P = p.get();
+  std::vector<std::vector<int>> minStride;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!P) {
+      if (b->plans.size() >= 6) b->plans.erase(b->plans.begin());   // bounded plan cache
+      b->plans.emplace_back(new BatchPlan());
+      P = b->plans.back().get();
+      P->kind = kind;
+      if (kind == BK_JAC) P->freeIdx = *freeIdx;
+      P->minStride = minStride;
+      const int st = BuildBatchPlan<T>(b, *P);
+      if (st < 0) {
+        b->plans.pop_back();
+        return st;
+      }
+    }
+    if (StageBatch(b, *P, params, active) == 0) break;
+    minStride = P->minStride;   // rebuild with room for the larger boxes
+    for (size_t k = 0; k < b->plans.size(); k++)
+      if (b->plans[k].get() == P) {
+        b->plans.erase(b->plans.begin() + k);
+        break;
+      }
+    P = nullptr;
+  }
+  if (!P) return b->fail(IFG_ERR_UNSUPPORTED, "sub-pixel box layout does not fit");
+  hipStream_t s = b->stream;
+  if (b->useGraphs) {
+    if (!P->exec) {
+      BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+      const int st = EnqueueBatch<T>(b, *P);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (st < 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      BCHK(e, "end capture");
+      P->graph = graph;
+      BCHK(hipGraphInstantiate(&P->exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    }
+    BCHK(hipGraphLaunch(P->exec, s), "graph launch");
+  } else {
+    const int st = EnqueueBatch<T>(b, *P);
+    if (st < 0) return st;
+  }
+  BCHK(hipStreamSynchronize(s), "batch evaluation");
+  *out = P;
+  return IFG_OK;
+}
+
+int BatchDispatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeIdx, const double* params,
+                  const int* active, BatchPlan** P) {
+  if (!params) return b->fail(IFG_ERR_ARG, "null parameters");
+  int st = BatchCheck(b);
+  if (st < 0) return st;
+  BCHK(hipSetDevice(b->device), "hipSetDevice");
+  (void)hipGetLastError();
+  if (b->models[0]->precision == IFG_PRECISION_FP64) return RunBatch<double>(b, kind, freeIdx, params, active, P);
+  return RunBatch<float>(b, kind, freeIdx, params, active, P);
+}
+
+}  // namespace
+
+namespace ifg {
+// Used by the batched LM driver (ifg_lm.cpp): statistics of the active models
+// (stat[k]), and -- with freeIdx -- their normal equations (packed J^T J upper
+// triangle row-major, then J^T r, per model at normalOff[k]) at the same parameters.
+int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double* stat,
+                  const std::vector<std::vector<int>>* freeIdx, std::vector<double>* normal,
+                  std::vector<long>* normalOff) {
+  BatchPlan* P = nullptr;
+  int st = BatchDispatch(b, freeIdx ? BK_KEEP : BK_STAT, nullptr, params, active, &P);
+  if (st < 0) return st;
+  const int nM = (int)b->models.size();
+  const double* o = reinterpret_cast<const double*>(P->pinnedOut);
+  for (int k = 0; k < nM; k++)
+    if (!active || active[k]) stat[k] = o[P->outOff[k]];
+  if (!freeIdx) return IFG_OK;
+  if ((st = BatchDispatch(b, BK_JAC, freeIdx, params, active, &P)) < 0) return st;
+  normal->assign((const double*)P->pinnedOut, (const double*)P->pinnedOut + P->outCount);
+  normalOff->assign(P->outOff.begin(), P->outOff.end());
+  return IFG_OK;
+}
+int BatchSize(const ifg_batch* b) { return (int)b->models.size(); }
+ifg_model* BatchModel(ifg_batch* b, int k) { return b->models[k]; }
+}  // namespace ifg
+
+extern "C" {
+
+ifg_batch* ifg_batch_create(int device) {
+  ifg_batch* b = new ifg_batch();
+  b->device = device;
+  const char* ng = getenv("IFG_NO_GRAPH");
+  if (ng && ng[0] == '1') b->useGraphs = false;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) b->err = "no HIP device / stream";
+  return b;
+}
+
+void ifg_batch_destroy(ifg_batch* b) { delete b; }
+
+const char* ifg_batch_last_error(const ifg_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int ifg_batch_add(ifg_batch* b, ifg_model* m) {
+  if (!b || !m) return IFG_ERR_ARG;
+  if (!b->stream) return b->fail(IFG_ERR_HIP, "batch has no stream");
+  b->plans.clear();
+  b->models.push_back(m);
+  return (int)b->models.size() - 1;
+}
+
+int ifg_batch_size(const ifg_batch* b) { return b ? (int)b->models.size() : 0; }
+
+int ifg_batch_fit_statistic(ifg_batch* b, const double* params, double* out) {
+  if (!b || !out) return IFG_ERR_ARG;
+  std::vector<double> stat(b->models.size());
+  const int st = ifg::BatchEvaluate(b, params, nullptr, stat.data(), nullptr, nullptr, nullptr);
+  if (st < 0) return st;
+  std::memcpy(out, stat.data(), sizeof(double) * stat.size());
+  return IFG_OK;
+}
+
+int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_free, const int* ifree,
+                               double* normal, double* stat) {
+  if (!b || !normal || !stat) return IFG_ERR_ARG;
+  std::vector<std::vector<int>> fr(b->models.size());
+  long off = 0;
+  for (size_t k = 0; k < b->models.size(); k++) {
+    const int np = b->models[k]->nParamsTot;
+    const int nf = n_free ? n_free[k] : np;
+    if (nf < 1 || nf > IFG_MAXFREE) return b->fail(IFG_ERR_ARG, "n_free out of range (1..48)");
+    for (int j = 0; j < nf; j++) {
+      const int idx = ifree ? ifree[off + j] : j;
+      if (idx < 0 || idx >= np) return b->fail(IFG_ERR_ARG, "free-parameter index out of range");
+      fr[k].push_back(idx);
+    }
+    off += nf;
+  }
+  std::vector<double> nrm;
+  std::vector<long> noff;
+  const int st = ifg::BatchEvaluate(b, params, nullptr, stat, &fr, &nrm, &noff);
+  if (st < 0) return st;
+  std::memcpy(normal, nrm.data(), sizeof(double) * nrm.size());
+  return IFG_OK;
+}
+
+int ifg_batch_set_graph_mode(ifg_batch* b, int on) {
+  if (!b) return IFG_ERR_ARG;
+  b->useGraphs = on != 0;
+  return IFG_OK;
 }
 
 }  // extern "C"

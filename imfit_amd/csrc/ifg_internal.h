@@ -86,9 +86,31 @@ struct RegionOut {
   int c0, r0, w, h, off, pad0, pad1, pad2;
 };
 
+// Statistic inputs and outputs of the model a task belongs to.  data / weight /
+// extra hold T (the plan's precision); output bases are indexed by the task's
+// vector (plane) index times nData.
+struct TaskStat {
+  const void* data;    // [nData]
+  const void* weight;  // [nData], mask folded in (1/sigma, or 1/0 for Cash / model errors)
+  const void* extra;   // [nData] extra Cash terms (MLR), may be null
+  double* devOut;      // [vec * nData] or null (S_JAC: Jacobian columns)
+  double* dfacOut;     // [nData] d(deviate)/d(model) or d(statistic)/d(model), or null
+  const double* dfac;  // S_JAC / S_GRAD input factor per data pixel
+  double* modelOut;    // [vec * nData] or null
+  long nData;
+  int W, H;            // data extent
+  double effGain, sky, rnTerm;   // effective gain, original sky, nCombined * readNoise_adu^2
+  double agiant;       // enorm's giant threshold for nData terms
+  int modelErrors;     // weights from the model (UpdateWeightVector)
+  int nReg;
+  RegionOut regs[IFG_MAXREG];   // data pixels owned by oversampled regions (skipped by main tasks)
+};
+
 // One grid of a merged launch: the main image (with or without PSF) or an
-// oversampled region.  Each merged kernel launch covers every task of a chunk;
-// block x of the launch belongs to the task whose block range contains it.
+// oversampled region, of one model.  A merged launch covers every task of a
+// chunk (of one or many models); its blocks are laid out per task and per pair:
+// task t owns blocks [b, b + nPairs * n) of a launch (n blocks per pair), and a
+// block -> task map gives each block its task.
 enum TaskKind { TK_MAIN_CONV = 0, TK_MAIN_PLAIN = 1, TK_REGION = 2 };
 struct Task {
   int kind, nPairs;
@@ -100,35 +122,55 @@ struct Task {
   void* spec;                  // complex planes (cpx<T>), pairStride elements per pair
   long pairStride;
   const int4* pairs;           // (vecA, rowoffA, vecB or -1, rowoffB) per pair
-  int b1, n1, b2, n2, b3, n3;  // block ranges in the evaluate / column / inverse launches
+  int b1, n1, b2, n2, b3, n3;  // first block and blocks per pair in the evaluate / column / inverse launches
   int rows1, rows3, C, ld;     // rows per block (row passes; TK_MAIN_PLAIN: rows3 only), columns per block
                                // and stride (column pass)
   int nStrips, stripsPerLine;  // column pass: strips of C columns, strips sharing a 128-byte line
-  int slot0;                   // first partial slot of this task's inverse-pass blocks
+  int slot0;                   // partial slot of inverse block (pair p, block lb): slot0 + p * n3 + lb
   int scale, nbc, rowsRep, c0, r0;   // region: data pixels [c0, c0+nbc) x [r0, r0+rowsRep) it replaces
+  int nVec;                    // vectors (planes) of this task in the launch
   EvalArgs ea;
+  TaskStat st;
 };
 
-template <typename T>
-struct StatArgs {
-  int mode;           // StatMode
-  int modelErrors;    // weights from the model (UpdateWeightVector)
-  int W, H;           // data extent
-  long nData;
-  const T* data;      // [nData]
-  const T* weight;    // [nData], mask folded in (1/sigma, or 1/0 for Cash / model errors)
-  const T* extra;     // [nData] extra Cash terms (MLR), may be null
-  double effGain, sky, rnTerm;   // effective gain, original sky, nCombined * readNoise_adu^2
-  double* devOut;     // [vec * nData] or null (S_JAC: Jacobian columns)
-  double* dfacOut;    // [nData] d(deviate)/d(model) or d(statistic)/d(model), or null
-  int dfacKind;       // 1: d(deviate)/dm   2: d(statistic)/dm
-  const double* dfac; // S_JAC / S_GRAD input factor per data pixel
-  double* modelOut;   // [vec * nData] or null
-  double* partials;   // [pair][plane][slot] x 5 doubles
-  int nSlots;         // slots per plane (= inverse-pass blocks per pair)
-  int nReg;
-  RegionOut regs[IFG_MAXREG];   // main-grid pixels owned by oversampled regions (skipped by the main tasks)
-  double agiant;
+// Block -> task lookup of one merged launch: with few tasks (one model's grids),
+// the first block of each task travels in the kernel arguments (start[n] = total);
+// with many (a batch of models), map[block] gives the task.
+#define IFG_ARGTASKS 8
+struct BlockRanges {
+  int n;
+  int start[IFG_ARGTASKS + 1];
+  const int* map;
+  const int* active;   // with map: per-task flag of this call (0: the task's model sits out); may be null
 };
+
+// Launch-wide statistic settings of the inverse pass.
+struct StatLaunch {
+  int mode;           // StatMode
+  int dfacKind;       // 1: d(deviate)/dm   2: d(statistic)/dm
+  double* partials;   // per slot: plane A then plane B, 5 doubles each (null: no reduction)
+};
+
+// One output of the final reduction: the partial slots of one pair (main grid and
+// regions of one model) merged into the statistic of its vector(s).
+struct FinalJob {
+  int outA, outB;     // output indices of plane A's and plane B's vector (outB -1: none)
+  int halves;         // both planes are halves of one image: one output, outA
+  int nRange;
+  int slot[IFG_MAXREG + 1], count[IFG_MAXREG + 1];
+  const int* active;  // flag of the job's model in this call (0: skip); null: always
+};
+
+// Normal equations of one model from its Jacobian columns and deviates.
+struct NormalJob {
+  const double* jcol;   // [j * nData + i]
+  const double* r;      // [nData]
+  double* out;          // packed upper triangle of J^T J, then J^T r
+  long nData;
+  int nFree, TP, tile0, nTiles;   // tiles of TP pixels; this job's first tile in the launch
+  long part0;           // first partial of this job
+  const int* active;    // flag of the job's model in this call (0: skip); null: always
+};
+
 
 }  // namespace ifg

@@ -613,14 +613,19 @@ __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g
   }
 }
 
-// Task of block x in a merged launch (block ranges [b, b + n) per task).
-__device__ __forceinline__ int find_task(const Task* tasks, int nTasks, int which, int bx) {
-  for (int t = 0; t < nTasks; t++) {
-    const int b = (which == 1) ? tasks[t].b1 : (which == 2) ? tasks[t].b2 : tasks[t].b3;
-    const int n = (which == 1) ? tasks[t].n1 : (which == 2) ? tasks[t].n2 : tasks[t].n3;
-    if (bx >= b && bx < b + n) return t;
+// Task of block x of a merged launch (-1 for padding blocks, or when the task's
+// model sits out this call).
+__device__ __forceinline__ int block_task(const BlockRanges& br, int bx) {
+  int t = -1;
+  if (br.map) {
+    t = br.map[bx];
+    if (t >= 0 && br.active && !br.active[t]) t = -1;
+  } else {
+#pragma unroll
+    for (int k = 0; k < IFG_ARGTASKS; k++)
+      if (k < br.n && bx >= br.start[k] && bx < br.start[k + 1]) t = k;
   }
-  return -1;
+  return t;
 }
 
 // ------------------------------------------------------------------ sub-pixel tails
@@ -726,30 +731,29 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
   if (lane == 0) *out = newValSum;
 }
 
-// Grid-stride over (task, vector, box pixel) for every task of the launch: the
-// launch shape is fixed and the work count comes from device memory, so the launch
-// replays from a graph.
+// Grid-stride over (task, vector, box pixel) for every task of the launch.  The
+// per-call work of task t is waves [prefix[t], prefix[t+1]) (its vectors times its
+// box pixels, zero for models that sit out the call), written by the host with
+// the call's parameters; the launch shape is fixed, so it replays from a graph.
 template <typename T, bool DISK, bool DERIV>
-__global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, int nTasks, int nVec, int subsampling) {
+__global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, int nTasks,
+                                              const long* __restrict__ prefix, int subsampling) {
   const int lane = threadIdx.x & 63;
-  long total = 0;
-  long tot[IFG_MAXREG + 1];
-  for (int t = 0; t < nTasks; t++) {
-    tot[t] = (tasks[t].ea.tailStride > 0) ? (long)nVec * (*tasks[t].ea.tailUsed) : 0;
-    total += tot[t];
-  }
+  const long total = prefix[nTasks];
   const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
   for (long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < total; wave += nWaves) {
-    long w = wave;
-    int t = 0;
-    while (w >= tot[t]) {
-      w -= tot[t];
-      t++;
+    int lo = 0, hi = nTasks - 1;   // last t with prefix[t] <= wave
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= wave) lo = mid;
+      else hi = mid - 1;
     }
-    const int used = *tasks[t].ea.tailUsed;
+    const Task& Tk = tasks[lo];
+    const long w = wave - prefix[lo];
+    const int used = *Tk.ea.tailUsed;
     const int vec = (int)(w / used);
     const int loc = (int)(w - (long)vec * used);
-    tail_pixel<T, DISK, DERIV>(tasks[t].ea, tasks[t].g, vec, loc, lane, subsampling);
+    tail_pixel<T, DISK, DERIV>(Tk.ea, Tk.g, vec, loc, lane, subsampling);
   }
 }
 
@@ -778,22 +782,23 @@ __device__ __forceinline__ void stage_pair(const Task& Tk, int4 pr, CompRec (*sR
 // Evaluates two real planes (one complex row: plane A real, plane B imaginary)
 // straight into LDS and runs the forward row FFT there.
 template <typename T, bool DISK, bool DERIV>
-__global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, int nTasks) {
+__global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
   extern __shared__ char smem_raw[];
-  const int t = find_task(tasks, nTasks, 1, blockIdx.x);
+  const int t = block_task(br, blockIdx.x);
   if (t < 0) return;
   const Task& Tk = tasks[t];
+  const int local = blockIdx.x - Tk.b1, pi = local / Tk.n1, lb = local - pi * Tk.n1;
   const GridGeom g = Tk.g;
   const int rows = Tk.rows1;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* B = A + rows * g.Nc;
   cpx<T>* TW = B + rows * g.Nc;
   stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
-  const int4 pr = Tk.pairs[blockIdx.y];
+  const int4 pr = Tk.pairs[pi];
   __shared__ CompRec sRec[2][IFG_MAXCOMP];
   __shared__ Box sBox[2][IFG_MAXBOX];
   stage_pair<DERIV>(Tk, pr, sRec, sBox);
-  const int ir0 = (blockIdx.x - Tk.b1) * rows;
+  const int ir0 = lb * rows;
   constexpr int K = 4;   // pixels per thread per pass
   for (int rr = 0; rr < rows; rr++) {
     const int ir = ir0 + rr;
@@ -820,7 +825,7 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   __syncthreads();
   const FFTPlan& pl = Tk.pc;
   cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, +1);
-  cpx<T>* S = (cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
+  cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
   for (int rr = 0; rr < rows && ir0 + rr < g.Lr; rr++)
     for (int j = threadIdx.x; j < g.Nc; j += blockDim.x) S[(long)(ir0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
 }
@@ -833,14 +838,14 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
 // their index; every task's block range starts at a multiple of 8), so each line
 // is fetched into one L2 only.
 template <typename T>
-__global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, int nTasks) {
+__global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, BlockRanges br) {
   extern __shared__ char smem_raw[];
-  const int t = find_task(tasks, nTasks, 2, blockIdx.x);
+  const int t = block_task(br, blockIdx.x);
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const GridGeom g = Tk.g;
   const int C = Tk.C, ld = Tk.ld, G = Tk.stripsPerLine;
-  const int lb = blockIdx.x - Tk.b2;
+  const int local = blockIdx.x - Tk.b2, pi = local / Tk.n2, lb = local - pi * Tk.n2;
   const int q = lb / (8 * G), rem = lb - q * 8 * G;
   const int strip = (q * 8 + (rem & 7)) * G + (rem >> 3);
   if (strip >= Tk.nStrips) return;
@@ -849,7 +854,7 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
   cpx<T>* TW = B + C * ld;
   stage_twiddles(TW, (const cpx<T>*)Tk.twr, g.Nr);
   const int c0 = strip * C;
-  cpx<T>* S = (cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
+  cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
   const cpx<T>* khat = (const cpx<T>*)Tk.khat;
   const int Nr = g.Nr, Nc = g.Nc;
   // thread -> (column cc, first row i0); rows advance by the block's row stride
@@ -979,70 +984,72 @@ __device__ __forceinline__ void en_merge(ENorm& a, const ENorm& b) {
 // Model value m at data pixel (r, c) of vector vec -> deviate / statistic term.
 // Main-grid tasks skip pixels that an oversampled region replaces (the region task
 // produces those, core/downsample.cpp:96-104).
+__device__ __forceinline__ bool mode_reads_data(int mode) { return mode == S_CHI2 || mode == S_CASH || mode == S_MLR; }
+
 template <typename T>
-__device__ __forceinline__ void stat_pixel_dw(const StatArgs<T>& sa, int vec, int r, int c, double m, double d,
-                                              double w0, ENorm& acc, bool skipRegions);
+__device__ __forceinline__ void stat_pixel_dw(const TaskStat& st, const StatLaunch& sl, int vec, int r, int c,
+                                              double m, double d, double w0, ENorm& acc, bool skipRegions);
 template <typename T>
-__device__ __forceinline__ void stat_pixel(const StatArgs<T>& sa, int vec, int r, int c, double m, ENorm& acc,
-                                           bool skipRegions) {
-  const long z = (long)r * sa.W + c;
-  const bool needDW = sa.mode == S_CHI2 || sa.mode == S_CASH || sa.mode == S_MLR;
-  stat_pixel_dw<T>(sa, vec, r, c, m, needDW ? (double)sa.data[z] : 0.0, needDW ? (double)sa.weight[z] : 0.0, acc,
-                   skipRegions);
+__device__ __forceinline__ void stat_pixel(const TaskStat& st, const StatLaunch& sl, int vec, int r, int c, double m,
+                                           ENorm& acc, bool skipRegions) {
+  const long z = (long)r * st.W + c;
+  const bool needDW = mode_reads_data(sl.mode);
+  stat_pixel_dw<T>(st, sl, vec, r, c, m, needDW ? (double)((const T*)st.data)[z] : 0.0,
+                   needDW ? (double)((const T*)st.weight)[z] : 0.0, acc, skipRegions);
 }
 // The same, with the data value d and weight w0 of the pixel already loaded.
 template <typename T>
-__device__ __forceinline__ void stat_pixel_dw(const StatArgs<T>& sa, int vec, int r, int c, double m, double d,
-                                              double w0, ENorm& acc, bool skipRegions) {
+__device__ __forceinline__ void stat_pixel_dw(const TaskStat& st, const StatLaunch& sl, int vec, int r, int c,
+                                              double m, double d, double w0, ENorm& acc, bool skipRegions) {
   if (skipRegions)
-    for (int k = 0; k < sa.nReg; k++) {
-      const RegionOut& R = sa.regs[k];
+    for (int k = 0; k < st.nReg; k++) {
+      const RegionOut& R = st.regs[k];
       if ((unsigned)(c - R.c0) < (unsigned)R.w && (unsigned)(r - R.r0) < (unsigned)R.h) return;
     }
-  const long z = (long)r * sa.W + c;
-  if (sa.modelOut) sa.modelOut[(long)vec * sa.nData + z] = m;
-  if (sa.mode == S_NONE) return;
-  if (sa.mode == S_JAC) {   // m is d(model)/d(p_vec) here
-    sa.devOut[(long)vec * sa.nData + z] = sa.dfac[z] * m;
+  const long z = (long)r * st.W + c;
+  if (st.modelOut) st.modelOut[(long)vec * st.nData + z] = m;
+  if (sl.mode == S_NONE) return;
+  if (sl.mode == S_JAC) {   // m is d(model)/d(p_vec) here
+    st.devOut[(long)vec * st.nData + z] = st.dfac[z] * m;
     return;
   }
-  if (sa.mode == S_GRAD) {
-    acc.s2 += sa.dfac[z] * m;
+  if (sl.mode == S_GRAD) {
+    acc.s2 += st.dfac[z] * m;
     return;
   }
   double w = w0;
-  if (sa.modelErrors && w != 0.0) {   // UpdateWeightVector (core/model_object.cpp:1345-1381)
-    double totalFlux = m + sa.sky;
-    double noise_squared = totalFlux / sa.effGain + sa.rnTerm;
+  if (st.modelErrors && w != 0.0) {   // UpdateWeightVector (core/model_object.cpp:1345-1381)
+    double totalFlux = m + st.sky;
+    double noise_squared = totalFlux / st.effGain + st.rnTerm;
     w = 1.0 / sqrt(noise_squared);
   }
-  if (sa.mode == S_CHI2) {
+  if (sl.mode == S_CHI2) {
     const double dev = w * (d - m);
-    en_add(acc, dev, sa.agiant);
-    if (sa.devOut) sa.devOut[(long)vec * sa.nData + z] = dev;
-    if (sa.dfacOut) {
+    en_add(acc, dev, st.agiant);
+    if (st.devOut) st.devOut[(long)vec * st.nData + z] = dev;
+    if (st.dfacOut) {
       // d(dev)/dm = -w, plus the model-error weight's own dependence on m
       double f = -w;
-      if (sa.modelErrors && w != 0.0) f += -0.5 * w * w * w / sa.effGain * (d - m);
-      sa.dfacOut[z] = (sa.dfacKind == 2) ? 2.0 * dev * f : f;
+      if (st.modelErrors && w != 0.0) f += -0.5 * w * w * w / st.effGain * (d - m);
+      st.dfacOut[z] = (sl.dfacKind == 2) ? 2.0 * dev * f : f;
     }
   } else {   // Cash / Poisson-MLR (core/model_object.cpp:1386-1401,1703-1775)
-    const double modVal = sa.effGain * (m + sa.sky);
-    const double dataVal = sa.effGain * (d + sa.sky);
+    const double modVal = st.effGain * (m + st.sky);
+    const double dataVal = st.effGain * (d + st.sky);
     const double logModel = (modVal <= 0) ? 1.0e-25 : log(modVal);
-    const double extraTerms = sa.extra ? (double)sa.extra[z] : 0.0;
+    const double extraTerms = st.extra ? (double)((const T*)st.extra)[z] : 0.0;
     const double term = modVal - dataVal * logModel + extraTerms;
     acc.s2 += w * term;
-    const double dev = (sa.mode == S_MLR) ? sqrt(2.0 * w * fabs(term)) : w * (d - m);
-    if (sa.devOut) sa.devOut[(long)vec * sa.nData + z] = dev;
-    if (sa.dfacOut) {
+    const double dev = (sl.mode == S_MLR) ? sqrt(2.0 * w * fabs(term)) : w * (d - m);
+    if (st.devOut) st.devOut[(long)vec * st.nData + z] = dev;
+    if (st.dfacOut) {
       // d(term)/dm = g (1 - D/M) (the log term is constant where M <= 0)
-      const double dterm = (modVal <= 0) ? sa.effGain : sa.effGain * (1.0 - dataVal / modVal);
+      const double dterm = (modVal <= 0) ? st.effGain : st.effGain * (1.0 - dataVal / modVal);
       double f;
-      if (sa.dfacKind == 2) f = 2.0 * w * dterm;   // d(2 sum w term)/dm
-      else if (sa.mode == S_MLR) f = (dev > 0.0) ? w * (term >= 0.0 ? 1.0 : -1.0) * dterm / dev : 0.0;
+      if (sl.dfacKind == 2) f = 2.0 * w * dterm;   // d(2 sum w term)/dm
+      else if (sl.mode == S_MLR) f = (dev > 0.0) ? w * (term >= 0.0 ? 1.0 : -1.0) * dterm / dev : 0.0;
       else f = -w;
-      sa.dfacOut[z] = f;
+      st.dfacOut[z] = f;
     }
   }
 }
@@ -1116,44 +1123,44 @@ __device__ double final_stat(const ENorm& t, int mode) {
 }
 
 // Merge the partial slots of one plane of one pair (block-wide, result in thread 0).
-__device__ ENorm merge_slots(const double* base, int nSlots) {
+
+// Store this block's partials of plane A and plane B in its slot.
+__device__ void finish_block(double* partials, int slot, const ENorm& ea, const ENorm& eb) {
+  block_reduce_store(ea, partials + (long)slot * 10);
+  block_reduce_store(eb, partials + (long)slot * 10 + 5);
+}
+
+// Merge the partials of plane `plane` over a job's slot ranges (fixed order).
+__device__ ENorm merge_job(const FinalJob& J, const double* partials, int plane) {
   ENorm e;
   en_init(e);
-  for (int s = threadIdx.x; s < nSlots; s += blockDim.x) {
-    ENorm o = {base[s * 5], base[s * 5 + 1], base[s * 5 + 2], base[s * 5 + 3], base[s * 5 + 4]};
-    en_merge(e, o);
-  }
+  for (int k = 0; k < J.nRange; k++)
+    for (int s = threadIdx.x; s < J.count[k]; s += blockDim.x) {
+      const double* b = partials + (long)(J.slot[k] + s) * 10 + plane * 5;
+      ENorm o = {b[0], b[1], b[2], b[3], b[4]};
+      en_merge(e, o);
+    }
   return block_reduce(e);
 }
 
-// Store this block's partials of plane A and plane B.
-template <typename T>
-__device__ void finish_block(const StatArgs<T>& sa, int slot, const ENorm& ea, const ENorm& eb) {
-  double* base = sa.partials + ((long)blockIdx.y * 2 * sa.nSlots) * 5;
-  block_reduce_store(ea, base + (long)slot * 5);
-  block_reduce_store(eb, base + ((long)sa.nSlots + slot) * 5);
-}
-
-// One block per pair: merge the pair's partial slots into the statistic of its
-// vector(s) (both planes of a single image split into halves are one vector).
-__global__ __launch_bounds__(256) void k_final(const int4* __restrict__ pairs, const double* __restrict__ partials,
-                                               int nSlots, int mode, double* __restrict__ out) {
-  const int pair = blockIdx.x;
-  const int4 pr = pairs[pair];
-  const double* base = partials + ((long)pair * 2 * nSlots) * 5;
-  if (pr.z == pr.x) {
-    ENorm a = merge_slots(base, nSlots);
-    ENorm b = merge_slots(base + (long)nSlots * 5, nSlots);
+// One block per job: the statistic of the job's vector(s) from the partial slots
+// of its pair (both planes of a single image split into halves are one vector).
+__global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const double* __restrict__ partials,
+                                               int mode, double* __restrict__ out) {
+  const FinalJob& J = jobs[blockIdx.x];
+  if (J.active && !*J.active) return;
+  ENorm a = merge_job(J, partials, 0);
+  if (J.halves) {
+    ENorm b = merge_job(J, partials, 1);
     if (threadIdx.x == 0) {
       en_merge(a, b);
-      out[pr.x] = final_stat(a, mode);
+      out[J.outA] = final_stat(a, mode);
     }
   } else {
-    ENorm a = merge_slots(base, nSlots);
-    if (threadIdx.x == 0) out[pr.x] = final_stat(a, mode);
-    if (pr.z >= 0) {
-      ENorm b = merge_slots(base + (long)nSlots * 5, nSlots);
-      if (threadIdx.x == 0) out[pr.z] = final_stat(b, mode);
+    if (threadIdx.x == 0) out[J.outA] = final_stat(a, mode);
+    if (J.outB >= 0) {
+      ENorm b = merge_job(J, partials, 1);
+      if (threadIdx.x == 0) out[J.outB] = final_stat(b, mode);
     }
   }
 }
@@ -1166,14 +1173,16 @@ __global__ __launch_bounds__(256) void k_final(const int4* __restrict__ pairs, c
 //     sub-pixels (rows outer, columns inner, core/downsample.cpp:84-106) into the
 //     data pixels the region replaces, and their terms.
 template <typename T, bool DISK, bool DERIV>
-__global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, int nTasks, StatArgs<T> sa) {
+__global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
+                                                      StatLaunch sl) {
   extern __shared__ char smem_raw[];
-  const int t = find_task(tasks, nTasks, 3, blockIdx.x);
+  const int t = block_task(br, blockIdx.x);
   if (t < 0) return;
   const Task& Tk = tasks[t];
+  const TaskStat& st = Tk.st;
   const GridGeom g = Tk.g;
-  const int4 pr = Tk.pairs[blockIdx.y];
-  const int lb = blockIdx.x - Tk.b3;
+  const int local = blockIdx.x - Tk.b3, pi = local / Tk.n3, lb = local - pi * Tk.n3;
+  const int4 pr = Tk.pairs[pi];
   ENorm ea, eb;
   en_init(ea);
   en_init(eb);
@@ -1195,17 +1204,17 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
           cols[k] = j0 + k * blockDim.x;
           ok[k] = cols[k] < g.W;
         }
-        if (ra < sa.H) {
+        if (ra < st.H) {
           row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], ra, cols, ok, va);
 #pragma unroll
           for (int k = 0; k < K; k++)
-            if (ok[k]) stat_pixel<T>(sa, pr.x, ra, cols[k], (double)va[k], ea, true);
+            if (ok[k]) stat_pixel<T>(st, sl, pr.x, ra, cols[k], (double)va[k], ea, true);
         }
-        if (pr.z >= 0 && rb < sa.H) {
+        if (pr.z >= 0 && rb < st.H) {
           row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rb, cols, ok, vb);
 #pragma unroll
           for (int k = 0; k < K; k++)
-            if (ok[k]) stat_pixel<T>(sa, pr.z, rb, cols[k], (double)vb[k], eb, true);
+            if (ok[k]) stat_pixel<T>(st, sl, pr.z, rb, cols[k], (double)vb[k], eb, true);
         }
       }
     }
@@ -1216,12 +1225,12 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     cpx<T>* TW = B + rows * g.Nc;
     stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
     const int io0 = lb * rows;
-    const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)blockIdx.y * Tk.pairStride;
+    const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
     // single-row main blocks: fetch the data and weights of this block's pixels
     // before the inverse FFT so that their latency overlaps it
     constexpr int K = 4;
     const bool pre = Tk.kind == TK_MAIN_CONV && rows == 1 && g.W <= K * (int)blockDim.x && io0 < g.Hh &&
-                     (sa.mode == S_CHI2 || sa.mode == S_CASH || sa.mode == S_MLR);
+                     mode_reads_data(sl.mode);
     T dA[K], wA[K], dB[K], wB[K];
     if (pre) {
       const int ra = pr.y + io0, rb = pr.w + io0;
@@ -1229,13 +1238,13 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
       for (int k = 0; k < K; k++) {
         const int c = threadIdx.x + k * blockDim.x;
         dA[k] = wA[k] = dB[k] = wB[k] = 0;
-        if (c < g.W && ra < sa.H) {
-          dA[k] = sa.data[(long)ra * sa.W + c];
-          wA[k] = sa.weight[(long)ra * sa.W + c];
+        if (c < g.W && ra < st.H) {
+          dA[k] = ((const T*)st.data)[(long)ra * st.W + c];
+          wA[k] = ((const T*)st.weight)[(long)ra * st.W + c];
         }
-        if (c < g.W && pr.z >= 0 && rb < sa.H) {
-          dB[k] = sa.data[(long)rb * sa.W + c];
-          wB[k] = sa.weight[(long)rb * sa.W + c];
+        if (c < g.W && pr.z >= 0 && rb < st.H) {
+          dB[k] = ((const T*)st.data)[(long)rb * st.W + c];
+          wB[k] = ((const T*)st.weight)[(long)rb * st.W + c];
         }
       }
     }
@@ -1254,16 +1263,16 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
         const int c = threadIdx.x + k * blockDim.x;
         if (c >= g.W) continue;
         const cpx<T> v = out[c + g.hl];
-        if (ra < sa.H) stat_pixel_dw<T>(sa, pr.x, ra, c, (double)v.x, (double)dA[k], (double)wA[k], ea, true);
-        if (pr.z >= 0 && rb < sa.H) stat_pixel_dw<T>(sa, pr.z, rb, c, (double)v.y, (double)dB[k], (double)wB[k], eb, true);
+        if (ra < st.H) stat_pixel_dw<T>(st, sl, pr.x, ra, c, (double)v.x, (double)dA[k], (double)wA[k], ea, true);
+        if (pr.z >= 0 && rb < st.H) stat_pixel_dw<T>(st, sl, pr.z, rb, c, (double)v.y, (double)dB[k], (double)wB[k], eb, true);
       }
     } else if (Tk.kind == TK_MAIN_CONV) {
       for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++) {
         const int io = io0 + rr, ra = pr.y + io, rb = pr.w + io;
         for (int c = threadIdx.x; c < g.W; c += blockDim.x) {
           const cpx<T> v = out[rr * g.Nc + c + g.hl];
-          if (ra < sa.H) stat_pixel<T>(sa, pr.x, ra, c, (double)v.x, ea, true);
-          if (pr.z >= 0 && rb < sa.H) stat_pixel<T>(sa, pr.z, rb, c, (double)v.y, eb, true);
+          if (ra < st.H) stat_pixel<T>(st, sl, pr.x, ra, c, (double)v.x, ea, true);
+          if (pr.z >= 0 && rb < st.H) stat_pixel<T>(st, sl, pr.z, rb, c, (double)v.y, eb, true);
         }
       }
     } else {   // TK_REGION: rows of this block are whole groups of `scale` sub-rows
@@ -1287,30 +1296,37 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
             binned += (double)(plane == 0 ? v.x : v.y);
           }
         const double m = binned / area;
-        if (plane == 0) stat_pixel<T>(sa, pr.x, Tk.r0 + bi, Tk.c0 + bj, m, ea, false);
-        else stat_pixel<T>(sa, pr.z, Tk.r0 + bi, Tk.c0 + bj, m, eb, false);
+        if (plane == 0) stat_pixel<T>(st, sl, pr.x, Tk.r0 + bi, Tk.c0 + bj, m, ea, false);
+        else stat_pixel<T>(st, sl, pr.z, Tk.r0 + bi, Tk.c0 + bj, m, eb, false);
       }
     }
   }
-  if (sa.partials) finish_block<T>(sa, Tk.slot0 + lb, ea, eb);
+  if (sl.partials) finish_block(sl.partials, Tk.slot0 + local, ea, eb);
 }
 
 // ------------------------------------------------------------------ normal equations
-// J^T J (upper triangle, row-major packed) and J^T r from Jacobian columns
-// jcol[j * nData + i] and deviates r[i]: each block stages a tile of TP pixels of
-// every column in LDS, each thread forms whole dot products over the tile, and
-// k_normal_final sums the per-block partials in a fixed order.
-__global__ __launch_bounds__(256) void k_normal_tile(const double* __restrict__ jcol, const double* __restrict__ r,
-                                                     long nData, int nFree, int TP, double* __restrict__ partials) {
+// J^T J (upper triangle, row-major packed) and J^T r of every job (model) from its
+// Jacobian columns jcol[j * nData + i] and deviates r[i]: each block stages a tile of
+// TP pixels of every column of its job in LDS, each thread forms whole dot products
+// over the tile, and k_normal_final sums a job's tile partials in a fixed order.
+__global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict__ jobs, const int* __restrict__ map,
+                                                     double* __restrict__ partials) {
   extern __shared__ char smem_raw[];
+  const int jb = map[blockIdx.x];
+  if (jb < 0) return;
+  const NormalJob& Jb = jobs[jb];
+  if (Jb.active && !*Jb.active) return;
+  const int TP = Jb.TP, nFree = Jb.nFree;
+  const long nData = Jb.nData;
   double* J = reinterpret_cast<double*>(smem_raw);   // [nFree + 1][TP]
-  const long p0 = (long)blockIdx.x * TP;
+  const int tile = blockIdx.x - Jb.tile0;
+  const long p0 = (long)tile * TP;
   const int nT = nFree * (nFree + 1) / 2;
   for (int e = threadIdx.x; e < (nFree + 1) * TP; e += blockDim.x) {
     const int j = e / TP, p = e - j * TP;
     const long i = p0 + p;
     double v = 0.0;
-    if (i < nData) v = (j < nFree) ? jcol[(long)j * nData + i] : r[i];
+    if (i < nData) v = (j < nFree) ? Jb.jcol[(long)j * nData + i] : Jb.r[i];
     J[e] = v;
   }
   __syncthreads();
@@ -1332,23 +1348,27 @@ __global__ __launch_bounds__(256) void k_normal_tile(const double* __restrict__ 
     const double* y = J + (long)b * TP;
     double acc = 0.0;
     for (int p = 0; p < TP; p++) acc += x[p] * y[p];
-    partials[(long)blockIdx.x * (nT + nFree) + e] = acc;
+    partials[Jb.part0 + (long)tile * (nT + nFree) + e] = acc;
   }
 }
 
-__global__ __launch_bounds__(256) void k_normal_final(const double* __restrict__ partials, int nBlocks, int nEnt,
-                                                      double* __restrict__ out) {
+// Block (e, job): entry e of the job's normal equations, summed over its tiles.
+__global__ __launch_bounds__(256) void k_normal_final(const NormalJob* __restrict__ jobs,
+                                                      const double* __restrict__ partials) {
   __shared__ double red[256];
+  const NormalJob& Jb = jobs[blockIdx.y];
+  const int nEnt = Jb.nFree * (Jb.nFree + 1) / 2 + Jb.nFree;
   const int e = blockIdx.x;
+  if (e >= nEnt || (Jb.active && !*Jb.active)) return;
   double acc = 0.0;
-  for (int b = threadIdx.x; b < nBlocks; b += blockDim.x) acc += partials[(long)b * nEnt + e];
+  for (int b = threadIdx.x; b < Jb.nTiles; b += blockDim.x) acc += partials[Jb.part0 + (long)b * nEnt + e];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[e] = red[0];
+  if (threadIdx.x == 0) Jb.out[e] = red[0];
 }
 
 }  // namespace ifg
@@ -1370,34 +1390,34 @@ static size_t smem_rows(int rows, int N, size_t elem) { return (2 * (size_t)rows
   } while (0)
 
 template <typename T>
-hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk,
-                       bool deriv) {
+hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, const long* prefix, long maxWaves, int sub,
+                       bool disk, bool deriv) {
   if (maxWaves <= 0) return hipSuccess;
   const long blocks = std::min<long>((maxWaves + 3) / 4, 2048);
-  IFG_VARIANT(k_tail, T, disk, deriv, dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, nVec, sub);
+  IFG_VARIANT(k_tail, T, disk, deriv, dim3((unsigned)blocks), dim3(256), 0, s, tasks, nTasks, prefix, sub);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
+hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
                               bool disk, bool deriv) {
   if (nBlocks <= 0) return hipSuccess;
-  IFG_VARIANT(k_eval_rowfft, T, disk, deriv, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  IFG_VARIANT(k_eval_rowfft, T, disk, deriv, dim3(nBlocks), dim3(256), lds, s, tasks, br);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_colconv(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds) {
+hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds) {
   if (nBlocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_colconv<T>, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks);
+  hipLaunchKernelGGL(k_colconv<T>, dim3(nBlocks), dim3(256), lds, s, tasks, br);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                               const StatArgs<T>& sa, bool disk, bool deriv) {
+hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
+                               const StatLaunch& sl, bool disk, bool deriv) {
   if (nBlocks <= 0) return hipSuccess;
-  IFG_VARIANT(k_inverse_stat, T, disk, deriv, dim3(nBlocks, nPairs), dim3(256), lds, s, tasks, nTasks, sa);
+  IFG_VARIANT(k_inverse_stat, T, disk, deriv, dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
   return hipGetLastError();
 }
 
@@ -1417,35 +1437,34 @@ hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPl
   return hipGetLastError();
 }
 
-hipError_t launch_final(hipStream_t s, const int4* pairs, const double* partials, int nSlots, int nPairs, int mode,
+hipError_t launch_final(hipStream_t s, const FinalJob* jobs, int nJobs, const double* partials, int mode,
                         double* out) {
-  hipLaunchKernelGGL(k_final, dim3(nPairs), dim3(256), 0, s, pairs, partials, nSlots, mode, out);
+  if (nJobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_final, dim3(nJobs), dim3(256), 0, s, jobs, partials, mode, out);
   return hipGetLastError();
 }
 
-hipError_t launch_normal(hipStream_t s, const double* jcol, const double* r, long nData, int nFree, double* partials,
-                         double* out) {
-  const int TP = (nFree <= 16) ? 256 : 128;
-  const int nBlocks = (int)((nData + TP - 1) / TP);
-  const int nEnt = nFree * (nFree + 1) / 2 + nFree;
-  hipLaunchKernelGGL(k_normal_tile, dim3(nBlocks), dim3(256), (size_t)(nFree + 1) * TP * sizeof(double), s, jcol, r,
-                     nData, nFree, TP, partials);
+int normal_tile_pixels(int nFree) { return (nFree <= 16) ? 256 : 128; }
+
+hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, int nJobs, const int* tileMap, int nTiles, int maxFree,
+                         double* partials) {
+  if (nJobs <= 0) return hipSuccess;
+  size_t lds = 0;   // the largest job's tile: (nFree + 1) * TP doubles
+  for (int f = 1; f <= maxFree; f++) lds = std::max(lds, (size_t)(f + 1) * normal_tile_pixels(f) * sizeof(double));
+  hipLaunchKernelGGL(k_normal_tile, dim3(nTiles), dim3(256), lds, s, jobs, tileMap, partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_normal_final, dim3(nEnt), dim3(256), 0, s, partials, nBlocks, nEnt, out);
+  const int maxEnt = maxFree * (maxFree + 1) / 2 + maxFree;
+  hipLaunchKernelGGL(k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, partials);
   return hipGetLastError();
-}
-
-long normal_partials_size(long nData, int nFree) {
-  const int TP = (nFree <= 16) ? 256 : 128;
-  return ((nData + TP - 1) / TP) * (long)(nFree * (nFree + 1) / 2 + nFree);
 }
 
 #define IFG_INST(T)                                                                                                 \
-  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, long, int, int, bool, bool);                   \
-  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, int, int, int, size_t, bool, bool);          \
-  template hipError_t launch_colconv<T>(hipStream_t, const Task*, int, int, int, size_t);                          \
-  template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, int, int, int, size_t, const StatArgs<T>&,   \
+  template hipError_t launch_tail<T>(hipStream_t, const Task*, int, const long*, long, int, bool, bool);           \
+  template hipError_t launch_eval_rowfft<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t, bool, bool); \
+  template hipError_t launch_colconv<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t);                 \
+  template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t,             \
+                                             const StatLaunch&,                                                    \
                                              bool, bool);                                                          \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
   template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);

@@ -4,26 +4,28 @@
 
 namespace ifg {
 // disk: the model contains an ExponentialDisk3D (selects the kernel variants that
-// carry its line-of-sight quadrature); deriv: the planes are derivative planes
+// carry its line-of-sight quadrature); deriv: the planes are derivative planes.
+// br: the launch's block -> task lookup.
 template <typename T>
-hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, long maxWaves, int nVec, int sub, bool disk,
-                       bool deriv);
+hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, const long* prefix, long maxWaves, int sub,
+                       bool disk, bool deriv);
 template <typename T>
-hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
+hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
                               bool disk, bool deriv);
 template <typename T>
-hipError_t launch_colconv(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds);
+hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds);
 template <typename T>
-hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, int nTasks, int nBlocks, int nPairs, size_t lds,
-                               const StatArgs<T>& sa, bool disk, bool deriv);
+hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
+                               const StatLaunch& sl, bool disk, bool deriv);
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
                          int dir, int rows);
 template <typename T>
 hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C);
-hipError_t launch_final(hipStream_t s, const int4* pairs, const double* partials, int nSlots, int nPairs, int mode,
+hipError_t launch_final(hipStream_t s, const FinalJob* jobs, int nJobs, const double* partials, int mode,
                         double* out);
-hipError_t launch_normal(hipStream_t s, const double* jcol, const double* r, long nData, int nFree, double* partials,
-                         double* out);
-long normal_partials_size(long nData, int nFree);
+// pixels per tile of a normal-equations job with nFree columns
+int normal_tile_pixels(int nFree);
+hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, int nJobs, const int* tileMap, int nTiles, int maxFree,
+                         double* partials);
 }  // namespace ifg

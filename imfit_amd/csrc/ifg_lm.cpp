@@ -21,6 +21,15 @@
 
 #include "../../include/imfit_gpu.h"
 
+// batch access (ifg_api.cpp)
+namespace ifg {
+int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double* stat,
+                  const std::vector<std::vector<int>>* freeIdx, std::vector<double>* normal,
+                  std::vector<long>* normalOff);
+int BatchSize(const ifg_batch* b);
+ifg_model* BatchModel(ifg_batch* b, int k);
+}  // namespace ifg
+
 namespace {
 
 const double MACHEP0 = 2.2204460e-16;   // solvers/mpfit.h:132
@@ -337,6 +346,164 @@ void Finish(ifg_lm_result* res, int info, int iter, int nfev, int nfree, double 
   (void)x;
 }
 
+
+// Levenberg-Marquardt on the Gauss-Newton normal equations for one model, as a
+// resumable state machine: the driver feeds it the normal equations at Point() when
+// needJ, then asks for a trial point (Trial) and feeds back the statistic there
+// (OnTrial); one driver can so advance many models per batched evaluation.  The
+// damped system (J^T J + lambda diag(J^T J)) p = -J^T r is solved by Cholesky;
+// pegged parameters whose descent direction leaves the box are pinned, and steps are
+// shortened to stay inside the limits as mpfit does (mpfit.cpp:763-806).
+struct NormalLM {
+  int npar = 0, n = 0;
+  Limits L;
+  double ftol = 1e-10, xtol = 1e-10;
+  int maxiter = 200;
+  std::vector<double> jtj, jtr, A, p, x, wa2, xfull, xtrial;
+  double chi2 = 0.0, orig = 0.0, lambda = 1e-3, pn = 0.0, xn = 0.0;
+  int nfev = 0, iter = 0, info = 0;
+  bool needJ = true;
+
+  int Init(int npar_, const double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg) {
+    npar = npar_;
+    if (cfg) {
+      if (cfg->ftol > 0) ftol = cfg->ftol;
+      if (cfg->xtol > 0) xtol = cfg->xtol;
+      if (cfg->maxiter > 0) maxiter = cfg->maxiter;
+    }
+    const int st = SetupLimits(npar, xall, pinfo, L);
+    if (st) return st;
+    n = (int)L.ifree.size();
+    jtj.assign((size_t)n * n, 0.0);
+    jtr.assign(n, 0.0);
+    A.assign((size_t)n * n, 0.0);
+    p.assign(n, 0.0);
+    x.assign(n, 0.0);
+    wa2.assign(n, 0.0);
+    xfull.assign(xall, xall + npar);
+    xtrial = xfull;
+    for (int j = 0; j < n; j++) x[j] = xall[L.ifree[j]];
+    return 0;
+  }
+  bool Done() const { return info != 0; }
+  const double* Point() {
+    for (int j = 0; j < n; j++) xfull[L.ifree[j]] = x[j];
+    return xfull.data();
+  }
+  void OnNormal(const double* jtjFull, const double* jtr_, double chi2_) {
+    std::memcpy(jtj.data(), jtjFull, sizeof(double) * n * n);
+    std::memcpy(jtr.data(), jtr_, sizeof(double) * n);
+    nfev++;
+    if (iter == 0) orig = chi2_;
+    chi2 = chi2_;
+    needJ = false;
+    iter++;
+    if (iter > maxiter) info = IFG_LM_MAXITER;
+  }
+  // packed upper triangle (row-major) of J^T J, then J^T r
+  void OnNormalPacked(const double* packed, double chi2_) {
+    std::vector<double> full((size_t)n * n);
+    int e = 0;
+    for (int a = 0; a < n; a++)
+      for (int c = a; c < n; c++, e++) full[(size_t)a * n + c] = full[(size_t)c * n + a] = packed[e];
+    OnNormal(full.data(), packed + e, chi2_);
+  }
+  // Solve for the next trial point (xtrial); false when the fit ends instead.
+  bool Trial() {
+    for (;;) {
+      std::vector<char> pinned(n, 0);
+      if (L.any)
+        for (int j = 0; j < n; j++) {
+          const bool lp = L.qllim[j] && x[j] <= L.llim[j], up = L.qulim[j] && x[j] >= L.ulim[j];
+          if ((lp && jtr[j] > 0) || (up && jtr[j] < 0)) pinned[j] = 1;   // -J^T r points outwards
+        }
+      for (int a = 0; a < n; a++)
+        for (int c = 0; c < n; c++) {
+          double v = jtj[(size_t)a * n + c];
+          if (a == c) v += lambda * std::max(jtj[(size_t)a * n + a], 1e-300);
+          if (pinned[a] || pinned[c]) v = (a == c) ? 1.0 : 0.0;
+          A[(size_t)a * n + c] = v;
+        }
+      for (int a = 0; a < n; a++) p[a] = pinned[a] ? 0.0 : -jtr[a];
+      bool ok = true;   // Cholesky A = L L^T in place (lower triangle)
+      for (int j = 0; j < n && ok; j++) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        if (!(d > 0.0)) {
+          ok = false;
+          break;
+        }
+        d = sqrt(d);
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; i++) {
+          double s = A[(size_t)i * n + j];
+          for (int k = 0; k < j; k++) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+          A[(size_t)i * n + j] = s / d;
+        }
+      }
+      if (!ok) {
+        lambda *= 10.0;
+        if (lambda > 1e16) {
+          info = IFG_LM_GTOL;
+          return false;
+        }
+        continue;
+      }
+      for (int i = 0; i < n; i++) {   // forward / back substitution
+        double s = p[i];
+        for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * p[k];
+        p[i] = s / A[(size_t)i * n + i];
+      }
+      for (int i = n - 1; i >= 0; i--) {
+        double s = p[i];
+        for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * p[k];
+        p[i] = s / A[(size_t)i * n + i];
+      }
+      LimitStep(L, x.data(), p.data(), wa2.data(), n);
+      xtrial = xfull;
+      for (int j = 0; j < n; j++) xtrial[L.ifree[j]] = wa2[j];
+      pn = xn = 0.0;
+      for (int j = 0; j < n; j++) {
+        pn += p[j] * p[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
+        xn += x[j] * x[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
+      }
+      return true;
+    }
+  }
+  void OnTrial(double chi2new) {
+    nfev++;
+    if (std::isfinite(chi2new) && chi2new < chi2) {
+      const double rel = (chi2 - chi2new) / chi2;
+      for (int j = 0; j < n; j++) x[j] = wa2[j];
+      chi2 = chi2new;
+      lambda = std::max(lambda * 0.1, 1e-12);
+      needJ = true;
+      if (rel <= ftol) info = IFG_LM_OK_CHI;
+      else if (sqrt(pn) <= xtol * sqrt(xn)) info = IFG_LM_OK_PAR;
+    } else {
+      lambda *= 10.0;
+      if (sqrt(pn) <= MACHEP0 * sqrt(xn) || lambda > 1e16) info = IFG_LM_OK_PAR;
+    }
+  }
+  void Result(double* xall, const ifg_param_info* pinfo, ifg_lm_result* res) const {
+    for (int j = 0; j < n; j++) xall[L.ifree[j]] = x[j];
+    if (!res) return;
+    res->status = info;
+    res->niter = iter;
+    res->nfev = nfev;
+    res->nfree = n;
+    res->bestnorm = chi2;
+    res->orignorm = orig;
+    int npegged = 0;
+    if (pinfo)
+      for (int i = 0; i < npar; i++)
+        if ((pinfo[i].limited[0] && pinfo[i].limits[0] == xall[i]) ||
+            (pinfo[i].limited[1] && pinfo[i].limits[1] == xall[i]))
+          npegged++;
+    res->npegged = npegged;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -502,126 +669,88 @@ done:
 // decrease below ftol on an accepted step, or a relative step below xtol.
 int ifg_fit_lm_normal(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                       ifg_lm_result* res) {
-  const int npar = ifg_get_nparams(model);
-  double ftol = 1e-10, xtol = 1e-10;
-  int maxiter = 200;
-  if (cfg) {
-    if (cfg->ftol > 0) ftol = cfg->ftol;
-    if (cfg->xtol > 0) xtol = cfg->xtol;
-    if (cfg->maxiter > 0) maxiter = cfg->maxiter;
-  }
-  Limits L;
-  int st = SetupLimits(npar, xall, pinfo, L);
+  NormalLM S;
+  int st = S.Init(ifg_get_nparams(model), xall, pinfo, cfg);
   if (st) return st;
-  const int n = (int)L.ifree.size();
-  std::vector<double> jtj((size_t)n * n), jtr(n), A((size_t)n * n), p(n), x(n), wa2(n), xnew(xall, xall + npar);
-  for (int j = 0; j < n; j++) x[j] = xall[L.ifree[j]];
-  double chi2 = 0.0, orig = 0.0, lambda = 1e-3;
-  int nfev = 0, iter = 0, info = 0;
-  bool needJ = true;
-  while (info == 0) {
-    if (needJ) {
-      for (int j = 0; j < n; j++) xnew[L.ifree[j]] = x[j];
-      if ((st = ifg_normal_equations(model, xnew.data(), n, L.ifree.data(), jtj.data(), jtr.data(), &chi2)) < 0)
+  std::vector<double> jtj((size_t)S.n * S.n), jtr(S.n);
+  while (!S.Done()) {
+    if (S.needJ) {
+      double chi2 = 0.0;
+      if ((st = ifg_normal_equations(model, S.Point(), S.n, S.L.ifree.data(), jtj.data(), jtr.data(), &chi2)) < 0)
         return st;
-      nfev++;
-      if (iter == 0) orig = chi2;
-      needJ = false;
-      iter++;
-      if (iter > maxiter) {
-        info = IFG_LM_MAXITER;
-        break;
-      }
+      S.OnNormal(jtj.data(), jtr.data(), chi2);
+      if (S.Done()) break;
     }
-    // pin pegged parameters whose descent direction leaves the box
-    std::vector<char> pinned(n, 0);
-    if (L.any)
-      for (int j = 0; j < n; j++) {
-        const bool lp = L.qllim[j] && x[j] <= L.llim[j], up = L.qulim[j] && x[j] >= L.ulim[j];
-        if ((lp && jtr[j] > 0) || (up && jtr[j] < 0)) pinned[j] = 1;   // -J^T r points outwards
-      }
-    for (int a = 0; a < n; a++)
-      for (int b = 0; b < n; b++) {
-        double v = jtj[(size_t)a * n + b];
-        if (a == b) v += lambda * std::max(jtj[(size_t)a * n + a], 1e-300);
-        if (pinned[a] || pinned[b]) v = (a == b) ? 1.0 : 0.0;
-        A[(size_t)a * n + b] = v;
-      }
-    for (int a = 0; a < n; a++) p[a] = pinned[a] ? 0.0 : -jtr[a];
-    bool ok = true;   // Cholesky A = L L^T in place (lower triangle)
-    for (int j = 0; j < n && ok; j++) {
-      double d = A[(size_t)j * n + j];
-      for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-      if (!(d > 0.0)) {
-        ok = false;
-        break;
-      }
-      d = sqrt(d);
-      A[(size_t)j * n + j] = d;
-      for (int i = j + 1; i < n; i++) {
-        double s = A[(size_t)i * n + j];
-        for (int k = 0; k < j; k++) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-        A[(size_t)i * n + j] = s / d;
-      }
-    }
-    if (!ok) {
-      lambda *= 10.0;
-      if (lambda > 1e16) {
-        info = IFG_LM_GTOL;
-        break;
-      }
-      continue;
-    }
-    for (int i = 0; i < n; i++) {   // forward / back substitution
-      double s = p[i];
-      for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * p[k];
-      p[i] = s / A[(size_t)i * n + i];
-    }
-    for (int i = n - 1; i >= 0; i--) {
-      double s = p[i];
-      for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * p[k];
-      p[i] = s / A[(size_t)i * n + i];
-    }
-    LimitStep(L, x.data(), p.data(), wa2.data(), n);
-    for (int j = 0; j < n; j++) xnew[L.ifree[j]] = wa2[j];
-    double chi2new = 0.0;
-    if ((st = ifg_fit_statistic(model, xnew.data(), &chi2new)) < 0) return st;
-    nfev++;
-    double pn = 0.0, xn = 0.0;
-    for (int j = 0; j < n; j++) {
-      pn += p[j] * p[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
-      xn += x[j] * x[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
-    }
-    if (std::isfinite(chi2new) && chi2new < chi2) {
-      const double rel = (chi2 - chi2new) / chi2;
-      for (int j = 0; j < n; j++) x[j] = wa2[j];
-      chi2 = chi2new;
-      lambda = std::max(lambda * 0.1, 1e-12);
-      needJ = true;
-      if (rel <= ftol) info = IFG_LM_OK_CHI;
-      else if (sqrt(pn) <= xtol * sqrt(xn)) info = IFG_LM_OK_PAR;
+    if (!S.Trial()) break;
+    double c = 0.0;
+    if ((st = ifg_fit_statistic(model, S.xtrial.data(), &c)) < 0) return st;
+    S.OnTrial(c);
+  }
+  S.Result(xall, pinfo, res);
+  return S.info;
+}
+
+int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+                     ifg_lm_result* res) {
+  if (!b || !params) return IFG_ERR_ARG;
+  const int nM = ifg::BatchSize(b);
+  if (nM < 1) return IFG_ERR_STATE;
+  std::vector<NormalLM> S(nM);
+  std::vector<long> off(nM + 1, 0);
+  std::vector<int> status(nM, 0);
+  for (int k = 0; k < nM; k++) off[k + 1] = off[k] + ifg_get_nparams(ifg::BatchModel(b, k));
+  std::vector<std::vector<int>> freeIdx(nM);
+  for (int k = 0; k < nM; k++) {
+    const int st = S[k].Init((int)(off[k + 1] - off[k]), params + off[k], pinfo ? pinfo + off[k] : nullptr, cfg);
+    if (st) {   // no fit for this model (e.g. bounds): report its status
+      status[k] = st;
+      S[k].info = st;
+      freeIdx[k].push_back(0);
     } else {
-      lambda *= 10.0;
-      if (sqrt(pn) <= MACHEP0 * sqrt(xn) || lambda > 1e16) info = IFG_LM_OK_PAR;
+      freeIdx[k] = S[k].L.ifree;
     }
   }
-  for (int j = 0; j < n; j++) xall[L.ifree[j]] = x[j];
-  if (res) {
-    res->status = info;
-    res->niter = iter;
-    res->nfev = nfev;
-    res->nfree = n;
-    res->bestnorm = chi2;
-    res->orignorm = orig;
-    int npegged = 0;
-    if (pinfo)
-      for (int i = 0; i < npar; i++)
-        if ((pinfo[i].limited[0] && pinfo[i].limits[0] == xall[i]) ||
-            (pinfo[i].limited[1] && pinfo[i].limits[1] == xall[i]))
-          npegged++;
-    res->npegged = npegged;
+  std::vector<double> X(params, params + off[nM]), stat(nM, 0.0), normal;
+  std::vector<long> noff;
+  std::vector<int> mask(nM, 0);
+  for (;;) {
+    // normal equations of every model that wants them, at its current point
+    int any = 0;
+    for (int k = 0; k < nM; k++) {
+      mask[k] = (!S[k].Done() && S[k].needJ) ? 1 : 0;
+      any += mask[k];
+      if (mask[k]) std::memcpy(&X[off[k]], S[k].Point(), sizeof(double) * (off[k + 1] - off[k]));
+    }
+    if (any) {
+      const int st = ifg::BatchEvaluate(b, X.data(), mask.data(), stat.data(), &freeIdx, &normal, &noff);
+      if (st < 0) return st;
+      for (int k = 0; k < nM; k++)
+        if (mask[k]) S[k].OnNormalPacked(&normal[noff[k]], stat[k]);
+    }
+    // trial steps of every model still iterating
+    any = 0;
+    for (int k = 0; k < nM; k++) {
+      mask[k] = (!S[k].Done() && S[k].Trial()) ? 1 : 0;
+      any += mask[k];
+      if (mask[k]) std::memcpy(&X[off[k]], S[k].xtrial.data(), sizeof(double) * (off[k + 1] - off[k]));
+    }
+    if (!any) break;
+    const int st = ifg::BatchEvaluate(b, X.data(), mask.data(), stat.data(), nullptr, nullptr, nullptr);
+    if (st < 0) return st;
+    for (int k = 0; k < nM; k++)
+      if (mask[k]) S[k].OnTrial(stat[k]);
   }
-  return info;
+  int notConverged = 0;
+  for (int k = 0; k < nM; k++) {
+    if (status[k] == 0) S[k].Result(params + off[k], pinfo ? pinfo + off[k] : nullptr, res ? res + k : nullptr);
+    else if (res) {
+      std::memset(&res[k], 0, sizeof(res[k]));
+      res[k].status = status[k];
+    }
+    const int sk = status[k] ? status[k] : S[k].info;
+    if (sk <= 0 || sk == IFG_LM_MAXITER) notConverged++;
+  }
+  return notConverged;
 }
 
 }  // extern "C"

@@ -182,3 +182,40 @@ def cfg4():
     truth = [256.5, 256.5, 0.0, 85.0, 1.0, 50.0, 100.0, 5.0, 0.0, 0.1, 1.5, 20.0, 15.0]
     return Workload("cfg4_512_expdisk3d_sersic_psf25", [("ExponentialDisk3D", True), ("Sersic", False)], truth,
                     512, 512, psf=gaussian_psf(25, 1.5), gain=4.0, read_noise=1.0, noise_seed=4)
+
+
+def cutouts(n=2000, seed=4, size_range=(64, 512), psf=None, precision=None):
+    """Many-cutout workload: n independent images, width and height each
+    log-uniform in size_range (seed), with 1-3 components drawn from {Sersic
+    (n in [0.5, 6]), Exponential, Gaussian} sharing a centre near the image centre,
+    position angles, ellipticities and sizes scaled to the cutout, a 25^2 Moffat PSF
+    (FWHM 3, beta 3), sky 20 as a FlatSky component, Poisson noise with gain 4 and
+    read noise 2 e-.  Returns a list of Workload objects (truth = generating
+    parameters)."""
+    rng = np.random.default_rng(seed)
+    if psf is None:
+        psf = moffat_psf(25, 3.0, 3.0)
+    lo, hi = np.log(size_range[0]), np.log(size_range[1])
+    out = []
+    for k in range(n):
+        W, H = (int(round(np.exp(rng.uniform(lo, hi)))) for _ in range(2))
+        s = min(W, H)
+        ncomp = int(rng.integers(1, 4))
+        names = list(rng.choice(["Sersic", "Exponential", "Gaussian"], size=ncomp, replace=True))
+        x0 = W / 2.0 + 0.5 + rng.uniform(-0.05, 0.05) * W
+        y0 = H / 2.0 + 0.5 + rng.uniform(-0.05, 0.05) * H
+        funcs, truth = [], [x0, y0]
+        for i, name in enumerate(names):
+            funcs.append((str(name), i == 0))
+            pa, ell = rng.uniform(0, 180), rng.uniform(0.0, 0.6)
+            if name == "Sersic":
+                truth += [pa, ell, rng.uniform(0.5, 6.0), rng.uniform(5.0, 50.0), rng.uniform(0.03, 0.15) * s]
+            elif name == "Exponential":
+                truth += [pa, ell, rng.uniform(20.0, 200.0), rng.uniform(0.04, 0.2) * s]
+            else:
+                truth += [pa, ell, rng.uniform(20.0, 200.0), rng.uniform(0.02, 0.08) * s]
+        funcs.append(("FlatSky", False))
+        truth.append(20.0)
+        out.append(Workload("cutout%04d_%dx%d" % (k, W, H), funcs, truth, W, H, psf=psf, gain=4.0, read_noise=2.0,
+                            noise_seed=int(rng.integers(0, 2**31))))
+    return out

@@ -183,6 +183,37 @@ int ifg_fit_lm(ifg_model* m, double* params, const ifg_param_info* pinfo, const 
 int ifg_fit_lm_normal(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                       ifg_lm_result* res);
 
+/* ---- batches of independent models (no counterpart in the reference, which fits
+ * one image per run) ------------------------------------------------------------
+ * Many ifg_models -- e.g. cutouts of different sizes, each set up through the calls
+ * above up to ifg_final_setup -- evaluated together: every merged launch covers the
+ * grids of all of them.  Parameters of all models are concatenated in the order the
+ * models were added.  The models share the device, the pixel precision, the fit
+ * statistic and the sub-pixel switch; the batch uses them but does not own them
+ * (destroy the batch before its models). */
+typedef struct ifg_batch ifg_batch;
+ifg_batch* ifg_batch_create(int device);
+void ifg_batch_destroy(ifg_batch* b);
+const char* ifg_batch_last_error(const ifg_batch* b);
+/* append a set-up model; returns its index in the batch */
+int ifg_batch_add(ifg_batch* b, ifg_model* m);
+int ifg_batch_size(const ifg_batch* b);
+/* GetFitStatistic of every model: out[nModels] */
+int ifg_batch_fit_statistic(ifg_batch* b, const double* params, double* out);
+/* Normal equations of every model w.r.t. its n_free[k] parameters listed in ifree
+ * (lists concatenated; NULL: every parameter of every model), per model the packed
+ * upper triangle of J^T J (row-major) then J^T r, concatenated; stat[nModels]. */
+int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_free, const int* ifree,
+                               double* normal, double* stat);
+/* ifg_fit_lm_normal on every model at once, each step of all models that are still
+ * iterating in one batched evaluation: pinfo concatenated per model (NULL: all free
+ * and unbounded), params updated in place, res[nModels].  Returns the number of
+ * models that ended without a convergence status (0: all converged), or an error. */
+int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+                     ifg_lm_result* res);
+/* as ifg_set_graph_mode, for the batch's own launches */
+int ifg_batch_set_graph_mode(ifg_batch* b, int on);
+
 /* ---- instrumentation (no counterpart in the reference, whose only timing is
  * wall-clock, core/makeimage_main.cpp:351-363) -------------------------------- */
 /* Record HIP events around every kernel launch of this handle (on its stream);
