@@ -639,25 +639,30 @@ void FillTaskStat(const ifg_model* m, TaskStat& st) {
   st.agiant = 1.304e19 / (double)std::max<long>(m->nData, 1);   // RGIANT / n (core/mp_enorm.cpp:66)
 }
 
-// Block layout of one chunk's tasks: task t owns nPairs * n blocks of each pass,
-// padded so that every task's column-pass range starts XCD-aligned; the maps give
-// each block its task.  Returns the block counts.
-void LayoutBlocks(std::vector<Task>& tk, int first, int count, std::vector<int>& m1, std::vector<int>& m2,
-                  std::vector<int>& m3, int& slot) {
-  int b1 = 0, b2 = 0, b3 = 0;
+// Append the blocks of task t (index `id` in the launch) to a block map: n blocks per
+// pair, entries (task, block within the task).  align > 1: start the task's range at
+// a multiple of align (padding entries (-1, 0)), as the column pass's XCD-aware strip
+// mapping needs.
+void AppendTaskBlocks(std::vector<int2>& map, int id, int nBlocks, int align) {
+  if (align > 1)
+    while (map.size() % align) map.push_back(make_int2(-1, 0));
+  for (int b = 0; b < nBlocks; b++) map.push_back(make_int2(id, b));
+}
+
+// Block layout of one launch's tasks: task t owns nPairs * n blocks of each pass, and
+// partial slots [slot0, slot0 + nPairs * n3); the maps give each block its task.
+void LayoutBlocks(std::vector<Task>& tk, int first, int count, std::vector<int2>& m1, std::vector<int2>& m2,
+                  std::vector<int2>& m3, int& slot) {
   for (int t = first; t < first + count; t++) {
     Task& Tk = tk[t];
-    Tk.b1 = b1;
-    Tk.b2 = b2;
-    Tk.b3 = b3;
+    Tk.b1 = (int)m1.size();
+    AppendTaskBlocks(m1, t - first, Tk.nPairs * Tk.n1, 1);
+    AppendTaskBlocks(m2, t - first, Tk.nPairs * Tk.n2, 8);
+    Tk.b2 = (int)m2.size() - Tk.nPairs * Tk.n2;
+    Tk.b3 = (int)m3.size();
+    AppendTaskBlocks(m3, t - first, Tk.nPairs * Tk.n3, 1);
     Tk.slot0 = slot;
-    b1 += Tk.nPairs * Tk.n1;
-    b2 += Tk.nPairs * Tk.n2;
-    b3 += Tk.nPairs * Tk.n3;
     slot += Tk.nPairs * Tk.n3;
-    m1.insert(m1.end(), (size_t)Tk.nPairs * Tk.n1, t - first);
-    m2.insert(m2.end(), (size_t)Tk.nPairs * Tk.n2, t - first);
-    m3.insert(m3.end(), (size_t)Tk.nPairs * Tk.n3, t - first);
   }
 }
 
@@ -759,7 +764,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   const bool deriv = IsDeriv(flags);
   const size_t elem = sizeof(cpx<T>);
   std::vector<Task> tasks((size_t)P.nChunks * P.nTasks);
-  std::vector<int> maps;
+  std::vector<int2> maps;
   std::vector<FinalJob> jobs;
   P.nB1.assign(P.nChunks, 0);
   P.nB2.assign(P.nChunks, 0);
@@ -853,7 +858,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
       if (flags & (PF_DFAC1 | PF_DFAC2)) Tk.st.dfacOut = m->dfac_d.as<double>();
       if (deriv) Tk.st.dfac = m->dfac_d.as<double>();
     }
-    std::vector<int> m1, m2, m3;
+    std::vector<int2> m1, m2, m3;
     int slot = 0;
     LayoutBlocks(tasks, c * P.nTasks, P.nTasks, m1, m2, m3, slot);
     maxSlots = std::max(maxSlots, slot);
@@ -906,8 +911,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
   HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
   HIPCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
-  HIPCHK(P.maps_d.ensure(sizeof(int) * std::max<size_t>(1, maps.size())), "alloc block maps");
-  HIPCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice), "block maps");
+  HIPCHK(P.maps_d.ensure(sizeof(int2) * std::max<size_t>(1, maps.size())), "alloc block maps");
+  HIPCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int2) * maps.size(), hipMemcpyHostToDevice), "block maps");
   HIPCHK(P.jobs_d.ensure(sizeof(FinalJob) * std::max<size_t>(1, jobs.size())), "alloc final jobs");
   HIPCHK(hipMemcpy(P.jobs_d.p, jobs.data(), sizeof(FinalJob) * jobs.size(), hipMemcpyHostToDevice), "final jobs");
   HIPCHK(P.partials_d.ensure(sizeof(double) * 10 * (size_t)std::max(1, P.nSlots)), "alloc partials");
@@ -925,9 +930,10 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     HIPCHK(P.normPart_d.ensure(sizeof(double) * (size_t)J.nTiles * nNormal), "alloc normal partials");
     HIPCHK(P.njobs_d.ensure(sizeof(NormalJob)), "alloc normal job");
     HIPCHK(hipMemcpy(P.njobs_d.p, &J, sizeof(J), hipMemcpyHostToDevice), "normal job");
-    std::vector<int> tmap(J.nTiles, 0);
-    HIPCHK(P.tileMap_d.ensure(sizeof(int) * tmap.size()), "alloc tile map");
-    HIPCHK(hipMemcpy(P.tileMap_d.p, tmap.data(), sizeof(int) * tmap.size(), hipMemcpyHostToDevice), "tile map");
+    std::vector<int2> tmap(J.nTiles);
+    for (int q = 0; q < J.nTiles; q++) tmap[q] = make_int2(0, q);
+    HIPCHK(P.tileMap_d.ensure(sizeof(int2) * tmap.size()), "alloc tile map");
+    HIPCHK(hipMemcpy(P.tileMap_d.p, tmap.data(), sizeof(int2) * tmap.size(), hipMemcpyHostToDevice), "tile map");
   }
   HIPCHK(hipDeviceSynchronize(), "plan setup");
   return IFG_OK;
@@ -1009,14 +1015,14 @@ int Enqueue(ifg_model* m, Plan& P) {
     TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv), "k_inverse_stat");
     if (reduce)
       TIMED("k_final",
-            launch_final(s, P.jobs_d.as<FinalJob>() + P.jobOff[c], P.nJobs[c], P.partials_d.as<double>(), mode,
-                         P.out_d.as<double>()),
+            launch_final(s, P.jobs_d.as<FinalJob>() + P.jobOff[c], nullptr, P.nJobs[c], P.partials_d.as<double>(),
+                         mode, P.out_d.as<double>()),
             "k_final");
   }
   if (P.flags & PF_JAC) {
     const int nEnt = nVec * (nVec + 1) / 2 + nVec;
     TIMED("k_normal",
-          launch_normal(s, P.njobs_d.as<NormalJob>(), 1, P.tileMap_d.as<int>(), P.nTiles, nVec,
+          launch_normal(s, P.njobs_d.as<NormalJob>(), nullptr, 1, P.tileMap_d.as<int2>(), P.nTiles, nVec,
                         P.normPart_d.as<double>()),
           "k_normal");
     HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nEnt, hipMemcpyDeviceToHost, s), "normal D2H");
@@ -1550,8 +1556,10 @@ struct BatchPlan {
   void* pinnedStage = nullptr;
   void* pinnedOut = nullptr;
   DevBuf stage_d, tail_d, pairs_d, spec_d, partials_d, out_d, jac_d, tasks_d, maps_d, jobs_d, njobs_d, tileMap_d,
-      normPart_d;
+      normPart_d, cmaps_d, ctileMap_d, cjobs_d;
   BlockRanges br1, br2, br3;
+  std::vector<Task> tasksHost;     // block layout (nPairs, n1..n3) for compacted launches
+  std::vector<int> jobTiles;       // BK_JAC: normal-equations tiles of each model's job
   size_t lds1 = 0, lds2 = 0, lds3 = 0;
   long tailWaves = 0;
   bool disk = false;
@@ -1788,26 +1796,30 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
   BCHK(P.pairs_d.ensure(sizeof(int4) * std::max<size_t>(1, pairs.size())), "alloc pairs");
   BCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
   for (int t = 0; t < P.nTasks; t++) tasks[t].pairs = P.pairs_d.as<int4>() + pairOff[t];
-  std::vector<int> m1, m2, m3;
+  std::vector<int2> m1, m2, m3;
   int slot = 0;
   LayoutBlocks(tasks, 0, P.nTasks, m1, m2, m3, slot);
   P.nB1 = (int)m1.size();
   P.nB2 = (int)m2.size();
   P.nB3 = (int)m3.size();
-  std::vector<int> maps(m1);
+  std::vector<int2> maps(m1);
   maps.insert(maps.end(), m2.begin(), m2.end());
   maps.insert(maps.end(), m3.begin(), m3.end());
-  BCHK(P.maps_d.ensure(sizeof(int) * std::max<size_t>(1, maps.size())), "alloc block maps");
-  BCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice), "block maps");
+  BCHK(P.maps_d.ensure(sizeof(int2) * std::max<size_t>(1, maps.size())), "alloc block maps");
+  BCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int2) * maps.size(), hipMemcpyHostToDevice), "block maps");
+  // room for the per-call compacted maps of a partially active batch (same bound)
+  BCHK(P.cmaps_d.ensure(sizeof(int2) * std::max<size_t>(1, maps.size() + 3 * 8)), "alloc compacted maps");
+  P.tasksHost = tasks;
   std::memset(&P.br1, 0, sizeof(P.br1));
   P.br3 = P.br2 = P.br1;
-  P.br1.map = P.maps_d.as<int>();
+  P.br1.map = P.maps_d.as<int2>();
   P.br2.map = P.br1.map + P.nB1;
   P.br3.map = P.br2.map + P.nB2;
   P.br1.active = P.br2.active = P.br3.active = reinterpret_cast<const int*>(stage_d + P.activeTaskOff);
   BCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
   BCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
   BCHK(P.partials_d.ensure(sizeof(double) * 10 * (size_t)std::max(1, slot)), "alloc partials");
+  BCHK(P.cjobs_d.ensure(sizeof(int) * std::max(1, nM)), "alloc compacted job list");
   if (!jac) {   // one statistic per model, over its grids' slots
     std::vector<FinalJob> jobs(nM);
     for (int k = 0; k < nM; k++) {
@@ -1829,7 +1841,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
     BCHK(hipMemcpy(P.jobs_d.p, jobs.data(), sizeof(FinalJob) * nM, hipMemcpyHostToDevice), "final jobs");
   } else {   // normal equations per model
     std::vector<NormalJob> jobs(nM);
-    std::vector<int> tmap;
+    std::vector<int2> tmap;
     long part = 0;
     size_t jo = 0;
     for (int k = 0; k < nM; k++) {
@@ -1849,14 +1861,16 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
       J.part0 = part;
       J.active = active_d + k;
       part += (long)J.nTiles * (nv * (nv + 1) / 2 + nv);
-      tmap.insert(tmap.end(), J.nTiles, k);
+      for (int q = 0; q < J.nTiles; q++) tmap.push_back(make_int2(k, q));
+      P.jobTiles.push_back(J.nTiles);
     }
     P.nJobs = nM;
     P.nTiles = (int)tmap.size();
     BCHK(P.njobs_d.ensure(sizeof(NormalJob) * std::max(1, nM)), "alloc normal jobs");
     BCHK(hipMemcpy(P.njobs_d.p, jobs.data(), sizeof(NormalJob) * nM, hipMemcpyHostToDevice), "normal jobs");
-    BCHK(P.tileMap_d.ensure(sizeof(int) * std::max<size_t>(1, tmap.size())), "alloc tile map");
-    BCHK(hipMemcpy(P.tileMap_d.p, tmap.data(), sizeof(int) * tmap.size(), hipMemcpyHostToDevice), "tile map");
+    BCHK(P.tileMap_d.ensure(sizeof(int2) * std::max<size_t>(1, tmap.size())), "alloc tile map");
+    BCHK(hipMemcpy(P.tileMap_d.p, tmap.data(), sizeof(int2) * tmap.size(), hipMemcpyHostToDevice), "tile map");
+    BCHK(P.ctileMap_d.ensure(sizeof(int2) * std::max<size_t>(1, tmap.size())), "alloc compacted tile map");
     BCHK(P.normPart_d.ensure(sizeof(double) * std::max<long>(1, part)), "alloc normal partials");
   }
   P.tailWaves = (long)tailOff;
@@ -1920,8 +1934,84 @@ int StageBatch(ifg_batch* b, BatchPlan& P, const double* params, const int* acti
   return grow ? 1 : 0;
 }
 
+// The blocks of one batched launch: every model's (the plan's static maps), or the
+// active models' only (maps compacted per call).
+struct BatchLaunch {
+  BlockRanges br1, br2, br3;
+  int nB1 = 0, nB2 = 0, nB3 = 0, nTiles = 0, nJobs = 0;
+  const int2* tiles = nullptr;
+  const int* jobList = nullptr;   // null: every model's job
+};
+
+BatchLaunch FullLaunch(BatchPlan& P) {
+  BatchLaunch L;
+  L.br1 = P.br1;
+  L.br2 = P.br2;
+  L.br3 = P.br3;
+  L.nB1 = P.nB1;
+  L.nB2 = P.nB2;
+  L.nB3 = P.nB3;
+  L.nTiles = P.nTiles;
+  L.tiles = P.tileMap_d.as<int2>();
+  L.nJobs = P.nJobs;
+  return L;
+}
+
+// Maps over the active models' tasks (and normal-equations tiles) only, uploaded on
+// the batch's stream; the host vectors must live until the stream has passed them.
+int CompactLaunch(ifg_batch* b, BatchPlan& P, const int* active, BatchLaunch& L, std::vector<int2>& maps,
+                  std::vector<int2>& tiles, std::vector<int>& jobs) {
+  std::vector<int2> m1, m2, m3;
+  for (int k = 0; k < P.nModels; k++) {
+    if (!active[k]) continue;
+    for (int gi = 0; gi < P.nGridM[k]; gi++) {
+      const int t = P.firstTask[k] + gi;
+      const Task& Tk = P.tasksHost[t];
+      AppendTaskBlocks(m1, t, Tk.nPairs * Tk.n1, 1);
+      AppendTaskBlocks(m2, t, Tk.nPairs * Tk.n2, 8);
+      AppendTaskBlocks(m3, t, Tk.nPairs * Tk.n3, 1);
+    }
+  }
+  maps = m1;
+  maps.insert(maps.end(), m2.begin(), m2.end());
+  maps.insert(maps.end(), m3.begin(), m3.end());
+  L = FullLaunch(P);
+  L.nB1 = (int)m1.size();
+  L.nB2 = (int)m2.size();
+  L.nB3 = (int)m3.size();
+  if (!maps.empty())
+    BCHK(hipMemcpyAsync(P.cmaps_d.p, maps.data(), sizeof(int2) * maps.size(), hipMemcpyHostToDevice, b->stream),
+         "compacted maps");
+  L.br1.map = P.cmaps_d.as<int2>();
+  L.br2.map = L.br1.map + L.nB1;
+  L.br3.map = L.br2.map + L.nB2;
+  // the active models' jobs, and (normal equations) their tiles from the per-job ranges
+  jobs.clear();
+  tiles.clear();
+  for (int k = 0; k < P.nModels; k++) {
+    if (!active[k]) continue;
+    jobs.push_back(k);
+    if (P.kind == BK_JAC)
+      for (int q = 0; q < P.jobTiles[k]; q++) tiles.push_back(make_int2(k, q));
+  }
+  L.nJobs = (int)jobs.size();
+  if (!jobs.empty())
+    BCHK(hipMemcpyAsync(P.cjobs_d.p, jobs.data(), sizeof(int) * jobs.size(), hipMemcpyHostToDevice, b->stream),
+         "compacted job list");
+  L.jobList = P.cjobs_d.as<int>();
+  if (P.kind == BK_JAC) {
+    L.nTiles = (int)tiles.size();
+    if (!tiles.empty())
+      BCHK(hipMemcpyAsync(P.ctileMap_d.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,
+                          b->stream),
+           "compacted tile map");
+    L.tiles = P.ctileMap_d.as<int2>();
+  }
+  return IFG_OK;
+}
+
 template <typename T>
-int EnqueueBatch(ifg_batch* b, BatchPlan& P) {
+int EnqueueBatch(ifg_batch* b, BatchPlan& P, const BatchLaunch& L) {
   hipStream_t s = b->stream;
   BCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
   ifg_model* m0 = b->models[0];
@@ -1937,15 +2027,16 @@ int EnqueueBatch(ifg_batch* b, BatchPlan& P) {
   hipError_t e;
   if ((e = launch_tail<T>(s, tk, P.nTasks, prefix_d, P.tailWaves, sub, P.disk, deriv)) != hipSuccess)
     return b->hipfail(e, "k_tail");
-  if ((e = launch_eval_rowfft<T>(s, tk, P.br1, P.nB1, P.lds1, P.disk, deriv)) != hipSuccess)
+  if ((e = launch_eval_rowfft<T>(s, tk, L.br1, L.nB1, P.lds1, P.disk, deriv)) != hipSuccess)
     return b->hipfail(e, "k_eval_rowfft");
-  if ((e = launch_colconv<T>(s, tk, P.br2, P.nB2, P.lds2)) != hipSuccess) return b->hipfail(e, "k_colconv");
-  if ((e = launch_inverse_stat<T>(s, tk, P.br3, P.nB3, P.lds3, sl, P.disk, deriv)) != hipSuccess)
+  if ((e = launch_colconv<T>(s, tk, L.br2, L.nB2, P.lds2)) != hipSuccess) return b->hipfail(e, "k_colconv");
+  if ((e = launch_inverse_stat<T>(s, tk, L.br3, L.nB3, P.lds3, sl, P.disk, deriv)) != hipSuccess)
     return b->hipfail(e, "k_inverse_stat");
   if (!deriv)
-    e = launch_final(s, P.jobs_d.as<FinalJob>(), P.nJobs, P.partials_d.as<double>(), mode, P.out_d.as<double>());
+    e = launch_final(s, P.jobs_d.as<FinalJob>(), L.jobList, L.nJobs, P.partials_d.as<double>(), mode,
+                     P.out_d.as<double>());
   else
-    e = launch_normal(s, P.njobs_d.as<NormalJob>(), P.nJobs, P.tileMap_d.as<int>(), P.nTiles, P.maxFree,
+    e = launch_normal(s, P.njobs_d.as<NormalJob>(), L.jobList, L.nJobs, L.tiles, L.nTiles, P.maxFree,
                       P.normPart_d.as<double>());
   if (e != hipSuccess) return b->hipfail(e, deriv ? "k_normal" : "k_final");
   BCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * P.outCount, hipMemcpyDeviceToHost, s), "results D2H");
@@ -1971,14 +2062,17 @@ int BatchCheck(ifg_batch* b) {
 // null = all); results stay in the plan's pinned output (per model at outOff).
 template <typename T>
 int RunBatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeIdx, const double* params,
-             const int* active, BatchPlan** out) {
+             const int* active, BatchPlan** out, bool sync) {
   BatchPlan* P = nullptr;
   for (auto& p : b->plans)
     if (p->kind == kind && (kind != BK_JAC || p->freeIdx == *freeIdx)) P = p.get();
   std::vector<std::vector<int>> minStride;
   for (int attempt = 0; attempt < 2; attempt++) {
     if (!P) {
-      if (b->plans.size() >= 6) b->plans.erase(b->plans.begin());   // bounded plan cache
+      if (b->plans.size() >= 6) {   // bounded plan cache (a plan may still be in flight: drain first)
+        BCHK(hipStreamSynchronize(b->stream), "drain before plan eviction");
+        b->plans.erase(b->plans.begin());
+      }
       b->plans.emplace_back(new BatchPlan());
       P = b->plans.back().get();
       P->kind = kind;
@@ -1992,6 +2086,7 @@ int RunBatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeId
     }
     if (StageBatch(b, *P, params, active) == 0) break;
     minStride = P->minStride;   // rebuild with room for the larger boxes
+    BCHK(hipStreamSynchronize(b->stream), "drain before plan rebuild");
     for (size_t k = 0; k < b->plans.size(); k++)
       if (b->plans[k].get() == P) {
         b->plans.erase(b->plans.begin() + k);
@@ -2001,10 +2096,26 @@ int RunBatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeId
   }
   if (!P) return b->fail(IFG_ERR_UNSUPPORTED, "sub-pixel box layout does not fit");
   hipStream_t s = b->stream;
+  int nAct = P->nModels;
+  if (active) {
+    nAct = 0;
+    for (int k = 0; k < P->nModels; k++) nAct += active[k] != 0;
+  }
+  if (nAct < P->nModels) {   // some models sit out: launch over the active ones only (eagerly)
+    BatchLaunch L;
+    std::vector<int2> maps, tiles;
+    std::vector<int> jobs;
+    int st = CompactLaunch(b, *P, active, L, maps, tiles, jobs);
+    if (st < 0) return st;
+    if ((st = EnqueueBatch<T>(b, *P, L)) < 0) return st;
+    if (sync) BCHK(hipStreamSynchronize(s), "batch evaluation");
+    *out = P;
+    return IFG_OK;
+  }
   if (b->useGraphs) {
     if (!P->exec) {
       BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-      const int st = EnqueueBatch<T>(b, *P);
+      const int st = EnqueueBatch<T>(b, *P, FullLaunch(*P));
       hipGraph_t graph = nullptr;
       hipError_t e = hipStreamEndCapture(s, &graph);
       if (st < 0) {
@@ -2017,23 +2128,23 @@ int RunBatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeId
     }
     BCHK(hipGraphLaunch(P->exec, s), "graph launch");
   } else {
-    const int st = EnqueueBatch<T>(b, *P);
+    const int st = EnqueueBatch<T>(b, *P, FullLaunch(*P));
     if (st < 0) return st;
   }
-  BCHK(hipStreamSynchronize(s), "batch evaluation");
+  if (sync) BCHK(hipStreamSynchronize(s), "batch evaluation");
   *out = P;
   return IFG_OK;
 }
 
 int BatchDispatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeIdx, const double* params,
-                  const int* active, BatchPlan** P) {
+                  const int* active, BatchPlan** P, bool sync = true) {
   if (!params) return b->fail(IFG_ERR_ARG, "null parameters");
   int st = BatchCheck(b);
   if (st < 0) return st;
   BCHK(hipSetDevice(b->device), "hipSetDevice");
   (void)hipGetLastError();
-  if (b->models[0]->precision == IFG_PRECISION_FP64) return RunBatch<double>(b, kind, freeIdx, params, active, P);
-  return RunBatch<float>(b, kind, freeIdx, params, active, P);
+  if (b->models[0]->precision == IFG_PRECISION_FP64) return RunBatch<double>(b, kind, freeIdx, params, active, P, sync);
+  return RunBatch<float>(b, kind, freeIdx, params, active, P, sync);
 }
 
 }  // namespace
@@ -2041,21 +2152,23 @@ int BatchDispatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* f
 namespace ifg {
 // Used by the batched LM driver (ifg_lm.cpp): statistics of the active models
 // (stat[k]), and -- with freeIdx -- their normal equations (packed J^T J upper
-// triangle row-major, then J^T r, per model at normalOff[k]) at the same parameters.
+// triangle row-major, then J^T r, per model at normal + normalOff[k], in the plan's
+// output buffer: valid until the next batched call) at the same parameters.
 int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double* stat,
-                  const std::vector<std::vector<int>>* freeIdx, std::vector<double>* normal,
-                  std::vector<long>* normalOff) {
-  BatchPlan* P = nullptr;
-  int st = BatchDispatch(b, freeIdx ? BK_KEEP : BK_STAT, nullptr, params, active, &P);
+                  const std::vector<std::vector<int>>* freeIdx, const double** normal, const long** normalOff) {
+  // the forward pass (kept deviates and d/dm factors) and the derivative pass run
+  // back to back on the batch's stream, with one synchronisation
+  BatchPlan *P = nullptr, *PJ = nullptr;
+  int st = BatchDispatch(b, freeIdx ? BK_KEEP : BK_STAT, nullptr, params, active, &P, freeIdx == nullptr);
   if (st < 0) return st;
+  if (freeIdx && (st = BatchDispatch(b, BK_JAC, freeIdx, params, active, &PJ)) < 0) return st;
   const int nM = (int)b->models.size();
   const double* o = reinterpret_cast<const double*>(P->pinnedOut);
   for (int k = 0; k < nM; k++)
     if (!active || active[k]) stat[k] = o[P->outOff[k]];
   if (!freeIdx) return IFG_OK;
-  if ((st = BatchDispatch(b, BK_JAC, freeIdx, params, active, &P)) < 0) return st;
-  normal->assign((const double*)P->pinnedOut, (const double*)P->pinnedOut + P->outCount);
-  normalOff->assign(P->outOff.begin(), P->outOff.end());
+  *normal = reinterpret_cast<const double*>(PJ->pinnedOut);   // valid until the next batched call
+  *normalOff = PJ->outOff.data();
   return IFG_OK;
 }
 int BatchSize(const ifg_batch* b) { return (int)b->models.size(); }
@@ -2112,11 +2225,13 @@ int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_
     }
     off += nf;
   }
-  std::vector<double> nrm;
-  std::vector<long> noff;
+  const double* nrm = nullptr;
+  const long* noff = nullptr;
   const int st = ifg::BatchEvaluate(b, params, nullptr, stat, &fr, &nrm, &noff);
   if (st < 0) return st;
-  std::memcpy(normal, nrm.data(), sizeof(double) * nrm.size());
+  long total = 0;
+  for (const auto& f : fr) total += (long)f.size() * (f.size() + 1) / 2 + (long)f.size();
+  std::memcpy(normal, nrm, sizeof(double) * total);
   return IFG_OK;
 }
 

@@ -140,7 +140,7 @@ struct Task {
 struct BlockRanges {
   int n;
   int start[IFG_ARGTASKS + 1];
-  const int* map;
+  const int2* map;     // (task, block within the task), task -1 for padding blocks
   const int* active;   // with map: per-task flag of this call (0: the task's model sits out); may be null
 };
 

@@ -613,17 +613,24 @@ __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g
   }
 }
 
-// Task of block x of a merged launch (-1 for padding blocks, or when the task's
-// model sits out this call).
-__device__ __forceinline__ int block_task(const BlockRanges& br, int bx) {
+// Task of block x of a merged launch and the block's index within the task (-1 for
+// padding blocks, or when the task's model sits out this call).  The index covers
+// the task's pairs: pair = local / n, block of the pair = local % n.
+__device__ __forceinline__ int block_task(const BlockRanges& br, int bx, int& local) {
   int t = -1;
+  local = 0;
   if (br.map) {
-    t = br.map[bx];
+    const int2 e = br.map[bx];
+    t = e.x;
+    local = e.y;
     if (t >= 0 && br.active && !br.active[t]) t = -1;
   } else {
 #pragma unroll
     for (int k = 0; k < IFG_ARGTASKS; k++)
-      if (k < br.n && bx >= br.start[k] && bx < br.start[k + 1]) t = k;
+      if (k < br.n && bx >= br.start[k] && bx < br.start[k + 1]) {
+        t = k;
+        local = bx - br.start[k];
+      }
   }
   return t;
 }
@@ -784,10 +791,11 @@ __device__ __forceinline__ void stage_pair(const Task& Tk, int4 pr, CompRec (*sR
 template <typename T, bool DISK, bool DERIV>
 __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
   extern __shared__ char smem_raw[];
-  const int t = block_task(br, blockIdx.x);
+  int local;
+  const int t = block_task(br, blockIdx.x, local);
   if (t < 0) return;
   const Task& Tk = tasks[t];
-  const int local = blockIdx.x - Tk.b1, pi = local / Tk.n1, lb = local - pi * Tk.n1;
+  const int pi = local / Tk.n1, lb = local - pi * Tk.n1;
   const GridGeom g = Tk.g;
   const int rows = Tk.rows1;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -840,12 +848,13 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
 template <typename T>
 __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, BlockRanges br) {
   extern __shared__ char smem_raw[];
-  const int t = block_task(br, blockIdx.x);
+  int local;
+  const int t = block_task(br, blockIdx.x, local);
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const GridGeom g = Tk.g;
   const int C = Tk.C, ld = Tk.ld, G = Tk.stripsPerLine;
-  const int local = blockIdx.x - Tk.b2, pi = local / Tk.n2, lb = local - pi * Tk.n2;
+  const int pi = local / Tk.n2, lb = local - pi * Tk.n2;
   const int q = lb / (8 * G), rem = lb - q * 8 * G;
   const int strip = (q * 8 + (rem & 7)) * G + (rem >> 3);
   if (strip >= Tk.nStrips) return;
@@ -1145,9 +1154,10 @@ __device__ ENorm merge_job(const FinalJob& J, const double* partials, int plane)
 
 // One block per job: the statistic of the job's vector(s) from the partial slots
 // of its pair (both planes of a single image split into halves are one vector).
-__global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const double* __restrict__ partials,
-                                               int mode, double* __restrict__ out) {
-  const FinalJob& J = jobs[blockIdx.x];
+__global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const int* __restrict__ jobList,
+                                               const double* __restrict__ partials, int mode,
+                                               double* __restrict__ out) {
+  const FinalJob& J = jobs[jobList ? jobList[blockIdx.x] : blockIdx.x];
   if (J.active && !*J.active) return;
   ENorm a = merge_job(J, partials, 0);
   if (J.halves) {
@@ -1176,12 +1186,13 @@ template <typename T, bool DISK, bool DERIV>
 __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
                                                       StatLaunch sl) {
   extern __shared__ char smem_raw[];
-  const int t = block_task(br, blockIdx.x);
+  int local;
+  const int t = block_task(br, blockIdx.x, local);
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const TaskStat& st = Tk.st;
   const GridGeom g = Tk.g;
-  const int local = blockIdx.x - Tk.b3, pi = local / Tk.n3, lb = local - pi * Tk.n3;
+  const int pi = local / Tk.n3, lb = local - pi * Tk.n3;
   const int4 pr = Tk.pairs[pi];
   ENorm ea, eb;
   en_init(ea);
@@ -1309,17 +1320,17 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
 // Jacobian columns jcol[j * nData + i] and deviates r[i]: each block stages a tile of
 // TP pixels of every column of its job in LDS, each thread forms whole dot products
 // over the tile, and k_normal_final sums a job's tile partials in a fixed order.
-__global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict__ jobs, const int* __restrict__ map,
+__global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict__ jobs, const int2* __restrict__ map,
                                                      double* __restrict__ partials) {
   extern __shared__ char smem_raw[];
-  const int jb = map[blockIdx.x];
-  if (jb < 0) return;
-  const NormalJob& Jb = jobs[jb];
+  const int2 e = map[blockIdx.x];   // (job, tile of the job)
+  if (e.x < 0) return;
+  const NormalJob& Jb = jobs[e.x];
   if (Jb.active && !*Jb.active) return;
   const int TP = Jb.TP, nFree = Jb.nFree;
   const long nData = Jb.nData;
   double* J = reinterpret_cast<double*>(smem_raw);   // [nFree + 1][TP]
-  const int tile = blockIdx.x - Jb.tile0;
+  const int tile = e.y;
   const long p0 = (long)tile * TP;
   const int nT = nFree * (nFree + 1) / 2;
   for (int e = threadIdx.x; e < (nFree + 1) * TP; e += blockDim.x) {
@@ -1354,9 +1365,10 @@ __global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict
 
 // Block (e, job): entry e of the job's normal equations, summed over its tiles.
 __global__ __launch_bounds__(256) void k_normal_final(const NormalJob* __restrict__ jobs,
+                                                      const int* __restrict__ jobList,
                                                       const double* __restrict__ partials) {
   __shared__ double red[256];
-  const NormalJob& Jb = jobs[blockIdx.y];
+  const NormalJob& Jb = jobs[jobList ? jobList[blockIdx.y] : blockIdx.y];
   const int nEnt = Jb.nFree * (Jb.nFree + 1) / 2 + Jb.nFree;
   const int e = blockIdx.x;
   if (e >= nEnt || (Jb.active && !*Jb.active)) return;
@@ -1437,25 +1449,25 @@ hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPl
   return hipGetLastError();
 }
 
-hipError_t launch_final(hipStream_t s, const FinalJob* jobs, int nJobs, const double* partials, int mode,
-                        double* out) {
+hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
+                        int mode, double* out) {
   if (nJobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_final, dim3(nJobs), dim3(256), 0, s, jobs, partials, mode, out);
+  hipLaunchKernelGGL(k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out);
   return hipGetLastError();
 }
 
 int normal_tile_pixels(int nFree) { return (nFree <= 16) ? 256 : 128; }
 
-hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, int nJobs, const int* tileMap, int nTiles, int maxFree,
-                         double* partials) {
-  if (nJobs <= 0) return hipSuccess;
+hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobList, int nJobs, const int2* tileMap,
+                         int nTiles, int maxFree, double* partials) {
+  if (nJobs <= 0 || nTiles <= 0) return hipSuccess;
   size_t lds = 0;   // the largest job's tile: (nFree + 1) * TP doubles
   for (int f = 1; f <= maxFree; f++) lds = std::max(lds, (size_t)(f + 1) * normal_tile_pixels(f) * sizeof(double));
   hipLaunchKernelGGL(k_normal_tile, dim3(nTiles), dim3(256), lds, s, jobs, tileMap, partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int maxEnt = maxFree * (maxFree + 1) / 2 + maxFree;
-  hipLaunchKernelGGL(k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, partials);
+  hipLaunchKernelGGL(k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, jobList, partials);
   return hipGetLastError();
 }
 

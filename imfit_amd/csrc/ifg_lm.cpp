@@ -24,8 +24,7 @@
 // batch access (ifg_api.cpp)
 namespace ifg {
 int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double* stat,
-                  const std::vector<std::vector<int>>* freeIdx, std::vector<double>* normal,
-                  std::vector<long>* normalOff);
+                  const std::vector<std::vector<int>>* freeIdx, const double** normal, const long** normalOff);
 int BatchSize(const ifg_batch* b);
 ifg_model* BatchModel(ifg_batch* b, int k);
 }  // namespace ifg
@@ -348,38 +347,44 @@ void Finish(ifg_lm_result* res, int info, int iter, int nfev, int nfree, double 
 
 
 // Levenberg-Marquardt on the Gauss-Newton normal equations for one model, as a
-// resumable state machine: the driver feeds it the normal equations at Point() when
-// needJ, then asks for a trial point (Trial) and feeds back the statistic there
-// (OnTrial); one driver can so advance many models per batched evaluation.  The
-// damped system (J^T J + lambda diag(J^T J)) p = -J^T r is solved by Cholesky;
-// pegged parameters whose descent direction leaves the box are pinned, and steps are
-// shortened to stay inside the limits as mpfit does (mpfit.cpp:763-806).
+// resumable state machine: the driver feeds it the normal equations (J^T J, J^T r
+// and the statistic) at Point() when needJ, then asks for a trial point (Trial) and
+// feeds back the statistic there (OnTrial); one driver can so advance many models
+// per batched evaluation.  The step control is mpfit's (solvers/mpfit.cpp:352-1097):
+// a trust region on ||D p|| (D = running maximum of the Jacobian's column norms,
+// initial bound stepfactor * ||D x||), the Levenberg-Marquardt parameter from
+// More's iteration (mp_lmpar, 1783-2070) -- here with Cholesky factors of
+// J^T J + par D^2 instead of a QR factorisation of J --, pegged limits (columns
+// zeroed, 574-598), steps shortened at the limits (763-806), the ratio of actual
+// to predicted reduction driving the bound (880-895), and mpfit's convergence tests
+// (941-987).  chi^2 = ||f||^2, so fnorm = sqrt(statistic).
 struct NormalLM {
   int npar = 0, n = 0;
   Limits L;
-  double ftol = 1e-10, xtol = 1e-10;
-  int maxiter = 200;
-  std::vector<double> jtj, jtr, A, p, x, wa2, xfull, xtrial;
-  double chi2 = 0.0, orig = 0.0, lambda = 1e-3, pn = 0.0, xn = 0.0;
-  int nfev = 0, iter = 0, info = 0;
-  bool needJ = true;
+  double ftol = 1e-10, xtol = 1e-10, gtol = 1e-10, stepfactor = 100.0;
+  int maxiter = 200, maxfev = 0;
+  std::vector<double> jtj, g, A, Lc, p, x, wa2, diag, cnorm, sc, xfull, xtrial;
+  double fnorm = 0.0, fnorm1 = -1.0, xnorm = 0.0, delta = 0.0, par = 0.0, orig = 0.0, gnorm = 0.0;
+  double alpha = 1.0, pnorm = 0.0;
+  int iter = 1, nfev = 0, info = 0;
+  bool needJ = true, firstStep = true;   // firstStep: the first Jacobian is yet to come
 
   int Init(int npar_, const double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg) {
     npar = npar_;
     if (cfg) {
       if (cfg->ftol > 0) ftol = cfg->ftol;
       if (cfg->xtol > 0) xtol = cfg->xtol;
+      if (cfg->gtol > 0) gtol = cfg->gtol;
       if (cfg->maxiter > 0) maxiter = cfg->maxiter;
+      maxfev = cfg->maxfev;
     }
     const int st = SetupLimits(npar, xall, pinfo, L);
     if (st) return st;
     n = (int)L.ifree.size();
     jtj.assign((size_t)n * n, 0.0);
-    jtr.assign(n, 0.0);
-    A.assign((size_t)n * n, 0.0);
-    p.assign(n, 0.0);
-    x.assign(n, 0.0);
-    wa2.assign(n, 0.0);
+    A = Lc = jtj;
+    g.assign(n, 0.0);
+    p = x = wa2 = diag = cnorm = sc = g;
     xfull.assign(xall, xall + npar);
     xtrial = xfull;
     for (int j = 0; j < n; j++) x[j] = xall[L.ifree[j]];
@@ -390,16 +395,6 @@ struct NormalLM {
     for (int j = 0; j < n; j++) xfull[L.ifree[j]] = x[j];
     return xfull.data();
   }
-  void OnNormal(const double* jtjFull, const double* jtr_, double chi2_) {
-    std::memcpy(jtj.data(), jtjFull, sizeof(double) * n * n);
-    std::memcpy(jtr.data(), jtr_, sizeof(double) * n);
-    nfev++;
-    if (iter == 0) orig = chi2_;
-    chi2 = chi2_;
-    needJ = false;
-    iter++;
-    if (iter > maxiter) info = IFG_LM_MAXITER;
-  }
   // packed upper triangle (row-major) of J^T J, then J^T r
   void OnNormalPacked(const double* packed, double chi2_) {
     std::vector<double> full((size_t)n * n);
@@ -408,82 +403,217 @@ struct NormalLM {
       for (int c = a; c < n; c++, e++) full[(size_t)a * n + c] = full[(size_t)c * n + a] = packed[e];
     OnNormal(full.data(), packed + e, chi2_);
   }
-  // Solve for the next trial point (xtrial); false when the fit ends instead.
-  bool Trial() {
-    for (;;) {
-      std::vector<char> pinned(n, 0);
-      if (L.any)
-        for (int j = 0; j < n; j++) {
-          const bool lp = L.qllim[j] && x[j] <= L.llim[j], up = L.qulim[j] && x[j] >= L.ulim[j];
-          if ((lp && jtr[j] > 0) || (up && jtr[j] < 0)) pinned[j] = 1;   // -J^T r points outwards
-        }
-      for (int a = 0; a < n; a++)
-        for (int c = 0; c < n; c++) {
-          double v = jtj[(size_t)a * n + c];
-          if (a == c) v += lambda * std::max(jtj[(size_t)a * n + a], 1e-300);
-          if (pinned[a] || pinned[c]) v = (a == c) ? 1.0 : 0.0;
-          A[(size_t)a * n + c] = v;
-        }
-      for (int a = 0; a < n; a++) p[a] = pinned[a] ? 0.0 : -jtr[a];
-      bool ok = true;   // Cholesky A = L L^T in place (lower triangle)
-      for (int j = 0; j < n && ok; j++) {
-        double d = A[(size_t)j * n + j];
-        for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-        if (!(d > 0.0)) {
-          ok = false;
-          break;
-        }
-        d = sqrt(d);
-        A[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; i++) {
-          double s = A[(size_t)i * n + j];
-          for (int k = 0; k < j; k++) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-          A[(size_t)i * n + j] = s / d;
+  void OnNormal(const double* jtjFull, const double* jtr, double chi2) {
+    std::memcpy(jtj.data(), jtjFull, sizeof(double) * n * n);
+    std::memcpy(g.data(), jtr, sizeof(double) * n);
+    nfev++;
+    needJ = false;
+    if (firstStep) {
+      fnorm = sqrt(std::max(chi2, 0.0));
+      orig = chi2;
+    }
+    if (!std::isfinite(chi2)) {
+      info = IFG_LM_ERR_NAN;
+      return;
+    }
+    // pegged parameters whose gradient points out of the box: zero column (mpfit.cpp:574-598)
+    if (L.any)
+      for (int j = 0; j < n; j++) {
+        const bool lp = L.qllim[j] && x[j] == L.llim[j], up = L.qulim[j] && x[j] == L.ulim[j];
+        if ((lp && g[j] > 0) || (up && g[j] < 0)) {
+          for (int k = 0; k < n; k++) jtj[(size_t)j * n + k] = jtj[(size_t)k * n + j] = 0.0;
+          g[j] = 0.0;
         }
       }
-      if (!ok) {
-        lambda *= 10.0;
-        if (lambda > 1e16) {
-          info = IFG_LM_GTOL;
-          return false;
+    for (int j = 0; j < n; j++) {
+      cnorm[j] = sqrt(std::max(jtj[(size_t)j * n + j], 0.0));
+      if (!std::isfinite(cnorm[j]) || !std::isfinite(g[j])) {
+        info = IFG_LM_ERR_NAN;
+        return;
+      }
+    }
+    if (firstStep) {   // scale by the initial column norms; initial step bound
+      firstStep = false;
+      for (int j = 0; j < n; j++) diag[j] = (cnorm[j] == 0.0) ? 1.0 : cnorm[j];
+      double s = 0.0;
+      for (int j = 0; j < n; j++) s += (diag[j] * x[j]) * (diag[j] * x[j]);
+      xnorm = sqrt(s);
+      delta = stepfactor * xnorm;
+      if (delta == 0.0) delta = stepfactor;
+    }
+    gnorm = 0.0;   // scaled gradient norm: max |(J^T f)_j| / (||J_j|| ||f||)
+    if (fnorm != 0.0)
+      for (int j = 0; j < n; j++)
+        if (cnorm[j] != 0.0) gnorm = std::max(gnorm, fabs(g[j]) / (cnorm[j] * fnorm));
+    if (gnorm <= gtol) {
+      info = IFG_LM_OK_DIR;
+      return;
+    }
+    for (int j = 0; j < n; j++) diag[j] = std::max(diag[j], cnorm[j]);
+  }
+  // Cholesky of M = J^T J + pr D^2 (pinned / empty columns: unit diagonal), after
+  // Jacobi equilibration M = S M' S with S = sqrt(diag M) (Lc holds the factor of
+  // M'); false if not positive definite.  out = -M^-1 J^T f.
+  bool Solve(double pr, std::vector<double>& out) {
+    for (int a = 0; a < n; a++) {
+      const double v = jtj[(size_t)a * n + a] + pr * diag[a] * diag[a];
+      sc[a] = (cnorm[a] == 0.0 || !(v > 0.0)) ? 1.0 : sqrt(v);
+    }
+    for (int a = 0; a < n; a++)
+      for (int c = 0; c < n; c++) {
+        double v = jtj[(size_t)a * n + c];
+        if (a == c) v += pr * diag[a] * diag[a];
+        if (cnorm[a] == 0.0 || cnorm[c] == 0.0) v = (a == c) ? 1.0 : 0.0;
+        Lc[(size_t)a * n + c] = v / (sc[a] * sc[c]);
+      }
+    for (int j = 0; j < n; j++) {
+      double d = Lc[(size_t)j * n + j];
+      for (int k = 0; k < j; k++) d -= Lc[(size_t)j * n + k] * Lc[(size_t)j * n + k];
+      if (!(d > 0.0) || !std::isfinite(d)) return false;
+      d = sqrt(d);
+      Lc[(size_t)j * n + j] = d;
+      for (int i = j + 1; i < n; i++) {
+        double t = Lc[(size_t)i * n + j];
+        for (int k = 0; k < j; k++) t -= Lc[(size_t)i * n + k] * Lc[(size_t)j * n + k];
+        Lc[(size_t)i * n + j] = t / d;
+      }
+    }
+    for (int i = 0; i < n; i++) out[i] = (cnorm[i] == 0.0) ? 0.0 : -g[i] / sc[i];
+    for (int i = 0; i < n; i++) {
+      double t = out[i];
+      for (int k = 0; k < i; k++) t -= Lc[(size_t)i * n + k] * out[k];
+      out[i] = t / Lc[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double t = out[i];
+      for (int k = i + 1; k < n; k++) t -= Lc[(size_t)k * n + i] * out[k];
+      out[i] = t / Lc[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) out[i] /= sc[i];
+    return true;
+  }
+  double DNorm(const std::vector<double>& v) const {
+    double s = 0.0;
+    for (int j = 0; j < n; j++) s += (diag[j] * v[j]) * (diag[j] * v[j]);
+    return sqrt(s);
+  }
+  // ||L^-1 w|| with w = D (D v) / ||D v|| and M = L L^T (L = S Lc), for More's
+  // Newton correction
+  double NewtonDen(const std::vector<double>& v, double dn) const {
+    std::vector<double> y(n);
+    for (int i = 0; i < n; i++) {
+      double t = diag[i] * (diag[i] * v[i]) / dn / sc[i];
+      for (int k = 0; k < i; k++) t -= Lc[(size_t)i * n + k] * y[k];
+      y[i] = t / Lc[(size_t)i * n + i];
+    }
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += y[i] * y[i];
+    return sqrt(s);
+  }
+  // More's iteration for the Levenberg-Marquardt parameter (mp_lmpar): par such that
+  // ||D p|| is within 10% of delta, or 0 when the Gauss-Newton step is inside.
+  void LMPar() {
+    const double p1 = 0.1;
+    bool fullRank = Solve(0.0, p);
+    double dxnorm = fullRank ? DNorm(p) : HUGE_VAL;
+    double fp = dxnorm - delta;
+    if (fullRank && fp <= p1 * delta) {
+      par = 0.0;
+      return;
+    }
+    double parl = 0.0;
+    if (fullRank) {
+      const double t = NewtonDen(p, dxnorm);
+      parl = ((fp / delta) / t) / t;
+    }
+    double gn = 0.0;   // ||D^-1 J^T f||
+    for (int j = 0; j < n; j++)
+      if (cnorm[j] != 0.0) gn += (g[j] / diag[j]) * (g[j] / diag[j]);
+    gn = sqrt(gn);
+    double paru = gn / delta;
+    if (paru == 0.0) paru = DWARF / std::min(delta, p1);
+    par = std::max(par, parl);
+    par = std::min(par, paru);
+    if (par == 0.0) par = gn / (fullRank ? dxnorm : delta);
+    for (int it = 1;; it++) {
+      if (par == 0.0) par = std::max(DWARF, 0.001 * paru);
+      if (!Solve(par, p)) {   // numerically indefinite: move towards the upper bound
+        parl = par;
+        par = std::max(10.0 * par, 0.5 * (par + paru));
+        if (it >= 10) {
+          Solve(paru, p);
+          par = paru;
+          return;
         }
         continue;
       }
-      for (int i = 0; i < n; i++) {   // forward / back substitution
-        double s = p[i];
-        for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * p[k];
-        p[i] = s / A[(size_t)i * n + i];
-      }
-      for (int i = n - 1; i >= 0; i--) {
-        double s = p[i];
-        for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * p[k];
-        p[i] = s / A[(size_t)i * n + i];
-      }
-      LimitStep(L, x.data(), p.data(), wa2.data(), n);
-      xtrial = xfull;
-      for (int j = 0; j < n; j++) xtrial[L.ifree[j]] = wa2[j];
-      pn = xn = 0.0;
-      for (int j = 0; j < n; j++) {
-        pn += p[j] * p[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
-        xn += x[j] * x[j] * std::max(jtj[(size_t)j * n + j], 1e-300);
-      }
-      return true;
+      dxnorm = DNorm(p);
+      const double temp = fp;
+      fp = dxnorm - delta;
+      if (fabs(fp) <= p1 * delta || (parl == 0.0 && fp <= temp && temp < 0.0) || it == 10) return;
+      const double t = NewtonDen(p, dxnorm);
+      const double parc = ((fp / delta) / t) / t;
+      if (fp > 0.0) parl = std::max(parl, par);
+      if (fp < 0.0) paru = std::min(paru, par);
+      par = std::max(parl, par + parc);
     }
   }
-  void OnTrial(double chi2new) {
+  // The next trial point (xtrial); false when the fit ends instead.
+  bool Trial() {
+    if (info != 0) return false;
+    LMPar();
+    alpha = LimitStep(L, x.data(), p.data(), wa2.data(), n);   // p scaled by alpha, wa2 = x + p
+    pnorm = DNorm(p);
+    if (iter == 1) delta = std::min(delta, pnorm);   // first iteration: adjust the initial bound
+    Point();
+    xtrial = xfull;
+    for (int j = 0; j < n; j++) xtrial[L.ifree[j]] = wa2[j];
+    return true;
+  }
+  void OnTrial(double chi2) {
     nfev++;
-    if (std::isfinite(chi2new) && chi2new < chi2) {
-      const double rel = (chi2 - chi2new) / chi2;
-      for (int j = 0; j < n; j++) x[j] = wa2[j];
-      chi2 = chi2new;
-      lambda = std::max(lambda * 0.1, 1e-12);
-      needJ = true;
-      if (rel <= ftol) info = IFG_LM_OK_CHI;
-      else if (sqrt(pn) <= xtol * sqrt(xn)) info = IFG_LM_OK_PAR;
-    } else {
-      lambda *= 10.0;
-      if (sqrt(pn) <= MACHEP0 * sqrt(xn) || lambda > 1e16) info = IFG_LM_OK_PAR;
+    fnorm1 = sqrt(chi2);   // NaN propagates: the step is then rejected
+    double actred = -1.0;
+    if (0.1 * fnorm1 < fnorm) {
+      const double t = fnorm1 / fnorm;
+      actred = 1.0 - t * t;
     }
+    double jp2 = 0.0;   // ||J p||^2 = p^T J^T J p (pegged columns zero)
+    for (int a = 0; a < n; a++) {
+      double t = 0.0;
+      for (int c = 0; c < n; c++) t += jtj[(size_t)a * n + c] * p[c];
+      jp2 += p[a] * t;
+    }
+    const double temp1 = sqrt(std::max(jp2, 0.0)) * alpha / fnorm;
+    const double temp2 = (sqrt(alpha * par) * pnorm) / fnorm;
+    const double prered = temp1 * temp1 + (temp2 * temp2) / 0.5;
+    const double dirder = -(temp1 * temp1 + temp2 * temp2);
+    const double ratio = (prered != 0.0) ? actred / prered : 0.0;
+    if (ratio <= 0.25) {   // step-bound update (mpfit.cpp:880-895)
+      double temp = (actred >= 0.0) ? 0.5 : 0.5 * dirder / (dirder + 0.5 * actred);
+      if (0.1 * fnorm1 >= fnorm || temp < 0.1) temp = 0.1;
+      delta = temp * std::min(delta, pnorm / 0.1);
+      par = par / temp;
+    } else if (par == 0.0 || ratio >= 0.75) {
+      delta = pnorm / 0.5;
+      par = 0.5 * par;
+    }
+    if (ratio >= 0.0001) {   // successful iteration
+      for (int j = 0; j < n; j++) x[j] = wa2[j];
+      xnorm = DNorm(x);
+      fnorm = fnorm1;
+      iter++;
+      needJ = true;
+    }
+    if (fabs(actred) <= ftol && prered <= ftol && 0.5 * ratio <= 1.0) info = IFG_LM_OK_CHI;
+    if (delta <= xtol * xnorm) info = IFG_LM_OK_PAR;
+    if (fabs(actred) <= ftol && prered <= ftol && 0.5 * ratio <= 1.0 && info == IFG_LM_OK_PAR) info = IFG_LM_OK_BOTH;
+    if (info != 0) return;
+    if (maxfev > 0 && nfev >= maxfev) info = IFG_LM_MAXITER;
+    if (iter >= maxiter) info = IFG_LM_MAXITER;
+    if (fabs(actred) <= MACHEP0 && prered <= MACHEP0 && 0.5 * ratio <= 1.0) info = IFG_LM_FTOL;
+    if (delta <= MACHEP0 * xnorm) info = IFG_LM_XTOL;
+    if (gnorm <= MACHEP0) info = IFG_LM_GTOL;
   }
   void Result(double* xall, const ifg_param_info* pinfo, ifg_lm_result* res) const {
     for (int j = 0; j < n; j++) xall[L.ifree[j]] = x[j];
@@ -492,7 +622,7 @@ struct NormalLM {
     res->niter = iter;
     res->nfev = nfev;
     res->nfree = n;
-    res->bestnorm = chi2;
+    res->bestnorm = fnorm * fnorm;
     res->orignorm = orig;
     int npegged = 0;
     if (pinfo)
@@ -661,12 +791,8 @@ done:
   return info;
 }
 
-// Levenberg-Marquardt on the normal equations, with J^T J and J^T r from the GPU.
-// The damped system (J^T J + lambda diag(J^T J)) p = -J^T r is solved by Cholesky;
-// lambda falls by 10 after an accepted step and rises by 10 after a rejected one.
-// Steps are shortened at the parameter limits as in mpfit, and limits whose
-// gradient points outwards pin their parameter.  Convergence: a relative chi^2
-// decrease below ftol on an accepted step, or a relative step below xtol.
+// Levenberg-Marquardt on the normal equations, with J^T J and J^T r reduced on the
+// GPU (NormalLM: mpfit's trust-region step control on the normal equations).
 int ifg_fit_lm_normal(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                       ifg_lm_result* res) {
   NormalLM S;
@@ -710,8 +836,9 @@ int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, 
       freeIdx[k] = S[k].L.ifree;
     }
   }
-  std::vector<double> X(params, params + off[nM]), stat(nM, 0.0), normal;
-  std::vector<long> noff;
+  std::vector<double> X(params, params + off[nM]), stat(nM, 0.0);
+  const double* normal = nullptr;
+  const long* noff = nullptr;
   std::vector<int> mask(nM, 0);
   for (;;) {
     // normal equations of every model that wants them, at its current point
