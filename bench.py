@@ -108,15 +108,18 @@ def kernel_model(w, prec_bytes):
     return out
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from profiles/hbm_traffic_cfg2_1024.md (rocprofv3
-    FETCH_SIZE / WRITE_SIZE of this workload, scaled by a copy kernel), or None."""
+def profiled_traffic(kernel, prec):
+    """HBM bytes per dispatch of `kernel` ("k_colconv", ...) in precision `prec` ("float" or
+    "double") from profiles/hbm_traffic_cfg2_1024.md (rocprofv3 FETCH_SIZE / WRITE_SIZE of
+    this workload, scaled by a copy kernel), or None.  Kernels with compile-time variants
+    are listed as <T,DISK,DERIV>; this workload runs the plain one."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2_1024.md")
     if not os.path.exists(path):
         return None
+    names = ("ifg::%s<%s>" % (kernel, prec), "ifg::%s<%s,false,false>" % (kernel, prec))
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] in names:
             return float(f[2])
     return None
 
@@ -321,7 +324,7 @@ def main():
     km = kernel_model(w, prec_bytes)
     dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
-    traffic = profiled_traffic("ifg::%s<%s>" % (dom, "float" if args.precision == "fp32" else "double"))
+    traffic = profiled_traffic(dom, "float" if args.precision == "fp32" else "double")
     ach = km[dom]["bytes"] / avg_s / 1e9 if dom in km else None
     roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS if ach is not None else None, "traffic": traffic,
