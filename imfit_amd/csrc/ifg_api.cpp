@@ -15,8 +15,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -146,15 +148,92 @@ int NextSmooth(int n) {   // smallest 2^a 3^b 5^c >= n
   }
 }
 
+// LDS bank depth of one Stockham stage's data writes for a wavefront of 64 lanes
+// on one sequence (lane j writes element (j - k) R + k + r Ns, k = j mod Ns, for
+// each r): LDS has 64 banks of 4 bytes, a complex element spans `dw` dwords, and an
+// instruction takes as many passes as the most dwords any bank serves (the lanes'
+// elements are distinct).  The stage's reads (j + r N/R) and twiddle reads are
+// lane-contiguous.
+int StageWriteDepth(int N, int Ns, int R, int dw) {
+  const int lanes = std::min(64, N / R);
+  int depth = 0;
+  for (int r = 0; r < R; r++) {
+    int count[64] = {0};
+    int worst = 0;
+    for (int j = 0; j < lanes; j++) {
+      const int k = j % Ns;
+      const long e = (long)(j - k) * R + k + (long)r * Ns;
+      for (int d = 0; d < dw; d++) worst = std::max(worst, ++count[(e * dw + d) % 64]);
+    }
+    depth += worst;
+  }
+  return depth;
+}
+
+// Stage radices of a length-N transform, from the radices the kernels implement,
+// minimising the LDS passes of the stage writes (FP32 elements) plus a fixed cost per
+// stage (its barrier and loop; about 24 passes), then the butterfly-loop trips of a
+// 256-thread workgroup.  Dynamic programming over the sub-transform size Ns (a
+// stage's cost depends only on Ns and R).  Plans are cached per length.
 FFTPlan MakePlan(int N) {
+  static std::mutex mu;
+  static std::map<int, FFTPlan> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto hit = cache.find(N);
+  if (hit != cache.end()) return hit->second;
+  static const int kRadices[] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+  const long kStageCost = 24;
+  struct Cost {
+    long passes, trips;
+    int next;   // radix of the stage taken at this Ns (0: done)
+  };
+  std::map<int, Cost> memo;   // keyed by Ns
+  std::function<Cost(int)> solve = [&](int Ns) -> Cost {
+    if (Ns == N) return Cost{0, 0, 0};
+    auto it = memo.find(Ns);
+    if (it != memo.end()) return it->second;
+    Cost best{1L << 40, 1L << 40, 0};
+    for (int R : kRadices) {
+      if ((N / Ns) % R) continue;
+      const Cost sub = solve(Ns * R);
+      const long passes = sub.passes + StageWriteDepth(N, Ns, R, 2) + kStageCost;
+      const long trips = sub.trips + (N / R + 255) / 256;
+      if (passes < best.passes || (passes == best.passes && trips < best.trips)) best = Cost{passes, trips, R};
+    }
+    memo[Ns] = best;
+    return best;
+  };
   FFTPlan pl;
   std::memset(&pl, 0, sizeof(pl));
   pl.N = N;
-  int k = N;
-  while (k % 4 == 0 && pl.nst < IFG_MAXSTAGE) { pl.rad[pl.nst++] = 4; k /= 4; }
-  for (int p : {2, 3, 5})
-    while (k % p == 0 && pl.nst < IFG_MAXSTAGE) { pl.rad[pl.nst++] = p; k /= p; }
+  for (int Ns = 1; Ns < N && pl.nst < IFG_MAXSTAGE;) {
+    const int R = solve(Ns).next;
+    if (R == 0) break;
+    pl.rad[pl.nst++] = R;
+    Ns *= R;
+  }
+  cache[N] = pl;
   return pl;
+}
+
+// Twiddle table of a plan, in the layout fft_stage reads: for each stage (sub-
+// transform size Ns, radix R), w^(k*r) = exp(-2 pi i k r / (Ns R)) at
+// (Ns - 1) + (r - 1) * Ns + k.  The stages' tables add up to N - 1 entries; the
+// table is padded to N.
+std::vector<cpx<double>> PlanTwiddles(const FFTPlan& pl) {
+  std::vector<cpx<double>> t(pl.N, cpx<double>{0.0, 0.0});
+  int Ns = 1;
+  for (int s = 0; s < pl.nst; s++) {
+    const int R = pl.rad[s];
+    for (int r = 1; r < R; r++)
+      for (int k = 0; k < Ns; k++) {
+        const long double ang =
+            -2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) / (long double)(Ns * R);
+        t[(Ns - 1) + (r - 1) * Ns + k] = {(double)cosl(ang), (double)sinl(ang)};
+      }
+    Ns *= R;
+  }
+  return t;
 }
 
 struct DevBuf {
@@ -377,13 +456,9 @@ FFTLen* GetFFT(ifg_model* m, int N, int* status) {
   if (it != m->ffts.end()) return it->second.get();
   std::unique_ptr<FFTLen> f(new FFTLen());
   f->plan = MakePlan(N);
-  std::vector<cpx<double>> t64(N);
+  const std::vector<cpx<double>> t64 = PlanTwiddles(f->plan);
   std::vector<cpx<float>> t32(N);
-  for (int j = 0; j < N; j++) {
-    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)N;
-    t64[j] = {(double)cosl(ang), (double)sinl(ang)};
-    t32[j] = {(float)t64[j].x, (float)t64[j].y};
-  }
+  for (int j = 0; j < N; j++) t32[j] = {(float)t64[j].x, (float)t64[j].y};
   hipError_t e = f->tw64.ensure(N * sizeof(cpx<double>));
   if (e == hipSuccess) e = f->tw32.ensure(N * sizeof(cpx<float>));
   if (e == hipSuccess) e = hipMemcpy(f->tw64.p, t64.data(), N * sizeof(cpx<double>), hipMemcpyHostToDevice);

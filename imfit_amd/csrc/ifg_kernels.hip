@@ -28,6 +28,38 @@
 
 namespace ifg {
 
+// ------------------------------------------------------------------ global loads
+// Every kernel is launched with kThreads threads per workgroup (the launchers at the
+// end of this file); using the constant instead of blockDim.x saves a load of the
+// dispatch's group size inside every loop.
+constexpr int kThreads = 256;
+
+template <typename V>
+__device__ __forceinline__ V gload(const V* __restrict__ p) {
+  return *p;
+}
+
+// dst[i] = i < nValid ? src[i] : 0 for i in [0, n), nValid <= n.  Each thread issues
+// U loads before its first LDS write, so they are in flight together rather than one
+// memory round trip per element.  Indexes are clamped into [0, nValid) so that the
+// loads need no branch.  The caller synchronises.
+template <int U, typename V>
+__device__ __forceinline__ void lds_fill(V* dst, const V* src, int n, int nValid) {
+  const int step = kThreads;
+  for (int base = threadIdx.x; base < n; base += U * step) {
+    V v[U];
+    if (nValid > 0) {
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = gload(src + min(base + u * step, nValid - 1));
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = base + u * step;
+      if (i < n) dst[i] = i < nValid ? v[u] : V{};
+    }
+  }
+}
+
 // ------------------------------------------------------------------ complex helpers
 template <typename T>
 __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
@@ -88,21 +120,150 @@ __device__ __forceinline__ void dft5(cpx<T>* a, int dir) {
   a[3] = csub(p2, q2);
 }
 
+// Composite radices, computed in registers from the small DFTs above.  Factors
+// with no common divisor use the Good-Thomas index maps (no twiddles between the
+// factors); otherwise Cooley-Tukey with the roots of unity below.
+template <int R>
+struct Roots;   // cos(2 pi m / R), sin(2 pi m / R)
+template <>
+struct Roots<8> {
+  static constexpr double c[8] = {1.0, 0.7071067811865475244008, 0.0, -0.7071067811865475244008,
+                                  -1.0, -0.7071067811865475244008, 0.0, 0.7071067811865475244008};
+  static constexpr double s[8] = {0.0, 0.7071067811865475244008, 1.0, 0.7071067811865475244008,
+                                  0.0, -0.7071067811865475244008, -1.0, -0.7071067811865475244008};
+};
+template <>
+struct Roots<9> {
+  static constexpr double c[9] = {1.0, 0.7660444431189780352024, 0.1736481776669303488517, -0.5,
+                                  -0.9396926207859083840541, -0.9396926207859083840541, -0.5,
+                                  0.1736481776669303488517, 0.7660444431189780352024};
+  static constexpr double s[9] = {0.0, 0.6427876096865393263226, 0.9848077530122080593667,
+                                  0.8660254037844386467637, 0.3420201433256687330441, -0.3420201433256687330441,
+                                  -0.8660254037844386467637, -0.9848077530122080593667,
+                                  -0.6427876096865393263226};
+};
+template <>
+struct Roots<16> {
+  static constexpr double c[16] = {1.0, 0.9238795325112867561282, 0.7071067811865475244008,
+                                   0.3826834323650897717285, 0.0, -0.3826834323650897717285,
+                                   -0.7071067811865475244008, -0.9238795325112867561282, -1.0,
+                                   -0.9238795325112867561282, -0.7071067811865475244008,
+                                   -0.3826834323650897717285, 0.0, 0.3826834323650897717285,
+                                   0.7071067811865475244008, 0.9238795325112867561282};
+  static constexpr double s[16] = {0.0, 0.3826834323650897717285, 0.7071067811865475244008,
+                                   0.9238795325112867561282, 1.0, 0.9238795325112867561282,
+                                   0.7071067811865475244008, 0.3826834323650897717285, 0.0,
+                                   -0.3826834323650897717285, -0.7071067811865475244008,
+                                   -0.9238795325112867561282, -1.0, -0.9238795325112867561282,
+                                   -0.7071067811865475244008, -0.3826834323650897717285};
+};
+
+constexpr int cgcd(int a, int b) { return b ? cgcd(b, a % b) : a; }
+constexpr int cinv(int a, int m) {   // a^-1 mod m for coprime a, m
+  for (int x = 1; x < m; x++)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+
+// x * exp(-2 pi i dir m / R); m is a constant once the callers' loops are unrolled,
+// so the branches and table reads fold away.
+template <typename T, int R>
+__device__ __forceinline__ cpx<T> rot(cpx<T> x, int m, int dir) {
+  m %= R;
+  if (m == 0) return x;
+  if (2 * m == R) return cpx<T>{-x.x, -x.y};
+  if (4 * m == R) return mul_mi(x, dir);
+  if (4 * m == 3 * R) return mul_mi(x, -dir);
+  const T c = (T)Roots<R>::c[m], sn = (T)Roots<R>::s[m];
+  return cmul(x, cpx<T>{c, dir > 0 ? -sn : sn});
+}
+
+template <typename T, int R>
+struct DFT;
+template <typename T>
+struct DFT<T, 2> {
+  static __device__ __forceinline__ void run(cpx<T>* a, int) { dft2(a); }
+};
+template <typename T>
+struct DFT<T, 3> {
+  static __device__ __forceinline__ void run(cpx<T>* a, int dir) { dft3(a, dir); }
+};
+template <typename T>
+struct DFT<T, 4> {
+  static __device__ __forceinline__ void run(cpx<T>* a, int dir) { dft4(a, dir); }
+};
+template <typename T>
+struct DFT<T, 5> {
+  static __device__ __forceinline__ void run(cpx<T>* a, int dir) { dft5(a, dir); }
+};
+
+// R = P * Q point DFT of a[0..R) in place.
+//   Good-Thomas (gcd(P, Q) = 1): n = (Q n1 + P n2) mod R, k = (Q qi k1 + P pi k2) mod R
+//     with Q qi = 1 (mod P), P pi = 1 (mod Q), so that w_R^(nk) = w_P^(n1 k1) w_Q^(n2 k2).
+//   Cooley-Tukey: n = Q n1 + n2, k = k1 + P k2, w_R^(nk) = w_P^(n1 k1) w_R^(n2 k1) w_Q^(n2 k2).
+template <typename T, int P, int Q>
+struct DFTComposite {
+  static constexpr int R = P * Q;
+  static __device__ __forceinline__ void run(cpx<T>* a, int dir) {
+    constexpr bool pfa = cgcd(P, Q) == 1;
+    constexpr int qi = pfa ? cinv(Q % P, P) : 0, pi = pfa ? cinv(P % Q, Q) : 0;
+    cpx<T> y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; n2++) {
+      cpx<T> t[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; n1++) t[n1] = a[pfa ? (Q * n1 + P * n2) % R : Q * n1 + n2];
+      DFT<T, P>::run(t, dir);
+#pragma unroll
+      for (int k1 = 0; k1 < P; k1++) {
+        if constexpr (pfa) y[n2 * P + k1] = t[k1];
+        else y[n2 * P + k1] = rot<T, R>(t[k1], n2 * k1, dir);
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) {
+      cpx<T> t[Q];
+#pragma unroll
+      for (int n2 = 0; n2 < Q; n2++) t[n2] = y[n2 * P + k1];
+      DFT<T, Q>::run(t, dir);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; k2++) a[pfa ? (Q * qi * k1 + P * pi * k2) % R : k1 + P * k2] = t[k2];
+    }
+  }
+};
+template <typename T>
+struct DFT<T, 6> : DFTComposite<T, 2, 3> {};
+template <typename T>
+struct DFT<T, 8> : DFTComposite<T, 2, 4> {};
+template <typename T>
+struct DFT<T, 9> : DFTComposite<T, 3, 3> {};
+template <typename T>
+struct DFT<T, 10> : DFTComposite<T, 2, 5> {};
+template <typename T>
+struct DFT<T, 12> : DFTComposite<T, 4, 3> {};
+template <typename T>
+struct DFT<T, 15> : DFTComposite<T, 3, 5> {};
+template <typename T>
+struct DFT<T, 16> : DFTComposite<T, 4, 4> {};
+
 // One Stockham autosort stage of radix R over `nseq` sequences of length N
 // (sequence s at src + s*ld): butterfly j of sub-transform size Ns reads
 // x[j + r*N/R], twiddles by w^(k*r) with k = j mod Ns, and writes the R outputs
 // to ((j-k)*R + k) + m*Ns.  After all stages the output is in natural order.
-// tw is the length-N twiddle table staged in LDS.  Quotients use a float
-// reciprocal: for b < 2^16 and divisors <= 4096, (b + 0.5) / d sits at least
-// 0.5/d from an integer, far beyond the float rounding of the product.
+// tw is the plan's twiddle table staged in LDS: stage by stage, the stage with
+// sub-transform size Ns holds w^(k*r) = exp(-2 pi i k r / (Ns R)) at
+// (Ns - 1) + (r - 1) * Ns + k, so that lanes with consecutive k read consecutive
+// entries (no LDS bank conflicts).  Quotients use a float reciprocal: for
+// b < 2^16 and divisors <= 4096, (b + 0.5) / d sits at least 0.5/d from an
+// integer, far beyond the float rounding of the product.
 template <typename T, int R>
 __device__ __forceinline__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ dst, int N, int Ns,
                                           int nseq, int ld, const cpx<T>* __restrict__ tw, int dir) {
   const int nb = N / R;
   const int total = nb * nseq;
-  const int twstep = N / (Ns * R);
+  const cpx<T>* __restrict__ tws = tw + (Ns - 1);
   const float inv_nb = 1.0f / (float)nb, inv_ns = 1.0f / (float)Ns;
-  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+  for (int b = threadIdx.x; b < total; b += kThreads) {
     const int seq = (int)(((float)b + 0.5f) * inv_nb);
     const int j = b - seq * nb;
     const int k = j - Ns * (int)(((float)j + 0.5f) * inv_ns);
@@ -113,15 +274,12 @@ __device__ __forceinline__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        cpx<T> w = tw[k * r * twstep];
+        cpx<T> w = tws[(r - 1) * Ns + k];
         if (dir < 0) w.y = -w.y;
         a[r] = cmul(a[r], w);
       }
     }
-    if (R == 2) dft2(a);
-    if (R == 3) dft3(a, dir);
-    if (R == 4) dft4(a, dir);
-    if (R == 5) dft5(a, dir);
+    DFT<T, R>::run(a, dir);
     cpx<T>* d = dst + seq * ld + (j - k) * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++) d[r * Ns] = a[r];
@@ -131,15 +289,24 @@ __device__ __forceinline__ void fft_stage(const cpx<T>* __restrict__ src, cpx<T>
 // Full FFT of nseq sequences held in LDS buffer A (B is scratch); returns the
 // buffer that holds the result.  Ends with a barrier.
 template <typename T>
-__device__ cpx<T>* fft_lds(cpx<T>* A, cpx<T>* B, const FFTPlan& pl, int nseq, int ld, const cpx<T>* tw,
+__device__ __forceinline__ cpx<T>* fft_lds(cpx<T>* A, cpx<T>* B, const FFTPlan& pl, int nseq, int ld, const cpx<T>* tw,
                            int dir) {
   int Ns = 1;
   for (int s = 0; s < pl.nst; s++) {
     const int R = pl.rad[s];
-    if (R == 4) fft_stage<T, 4>(A, B, pl.N, Ns, nseq, ld, tw, dir);
-    else if (R == 2) fft_stage<T, 2>(A, B, pl.N, Ns, nseq, ld, tw, dir);
-    else if (R == 3) fft_stage<T, 3>(A, B, pl.N, Ns, nseq, ld, tw, dir);
-    else fft_stage<T, 5>(A, B, pl.N, Ns, nseq, ld, tw, dir);
+    switch (R) {
+      case 2: fft_stage<T, 2>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 3: fft_stage<T, 3>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 4: fft_stage<T, 4>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 5: fft_stage<T, 5>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 6: fft_stage<T, 6>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 8: fft_stage<T, 8>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 9: fft_stage<T, 9>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 10: fft_stage<T, 10>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 12: fft_stage<T, 12>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      case 15: fft_stage<T, 15>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+      default: fft_stage<T, 16>(A, B, pl.N, Ns, nseq, ld, tw, dir); break;
+    }
     __syncthreads();
     cpx<T>* t = A;
     A = B;
@@ -152,7 +319,7 @@ __device__ cpx<T>* fft_lds(cpx<T>* A, cpx<T>* B, const FFTPlan& pl, int nseq, in
 // Copy the length-N twiddle table into LDS (caller synchronises).
 template <typename T>
 __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __restrict__ tw, int N) {
-  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = tw[i];
+  lds_fill<8>(dst, tw, N, N);
 }
 
 // ------------------------------------------------------------------ image functions
@@ -747,8 +914,8 @@ __global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, in
                                               const long* __restrict__ prefix, int subsampling) {
   const int lane = threadIdx.x & 63;
   const long total = prefix[nTasks];
-  const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
-  for (long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < total; wave += nWaves) {
+  const long nWaves = (long)gridDim.x * (kThreads >> 6);
+  for (long wave = ((long)blockIdx.x * kThreads + threadIdx.x) >> 6; wave < total; wave += nWaves) {
     int lo = 0, hi = nTasks - 1;   // last t with prefix[t] <= wave
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -772,7 +939,7 @@ template <bool DERIV>
 __device__ __forceinline__ void stage_pair(const Task& Tk, int4 pr, CompRec (*sRec)[IFG_MAXCOMP],
                                            Box (*sBox)[IFG_MAXBOX]) {
   const int nd = Tk.ea.nComp * 16;
-  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) {
+  for (int i = threadIdx.x; i < 2 * nd; i += kThreads) {
     const int pl = i >= nd, q = i - pl * nd;
     const int vec = pl ? pr.z : pr.x;
     if (!DERIV && vec >= 0) (&sRec[pl][0].v[0])[q] = (&Tk.ea.recs[(long)vec * Tk.ea.recStride].v[0])[q];
@@ -810,13 +977,13 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   constexpr int K = 4;   // pixels per thread per pass
   for (int rr = 0; rr < rows; rr++) {
     const int ir = ir0 + rr;
-    for (int j0 = threadIdx.x; j0 < g.Nc; j0 += K * blockDim.x) {
+    for (int j0 = threadIdx.x; j0 < g.Nc; j0 += K * kThreads) {
       int cols[K];
       bool ok[K];
       T va[K], vb[K];
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int j = j0 + k * blockDim.x;
+        const int j = j0 + k * kThreads;
         cols[k] = j - g.hl;
         ok[k] = ir < g.Lr && j < g.Lc;
         vb[k] = 0;
@@ -825,7 +992,7 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
       if (pr.z >= 0) row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], pr.w + ir - g.hb, cols, ok, vb);
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int j = j0 + k * blockDim.x;
+        const int j = j0 + k * kThreads;
         if (j < g.Nc) A[rr * g.Nc + j] = ok[k] ? cpx<T>{va[k], vb[k]} : cpx<T>{0, 0};
       }
     }
@@ -835,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, +1);
   cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
   for (int rr = 0; rr < rows && ir0 + rr < g.Lr; rr++)
-    for (int j = threadIdx.x; j < g.Nc; j += blockDim.x) S[(long)(ir0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
+    for (int j = threadIdx.x; j < g.Nc; j += kThreads) S[(long)(ir0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
 }
 
 // ------------------------------------------------------------------ column pass
@@ -866,25 +1033,38 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
   cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
   const cpx<T>* khat = (const cpx<T>*)Tk.khat;
   const int Nr = g.Nr, Nc = g.Nc;
-  // thread -> (column cc, first row i0); rows advance by the block's row stride
-  const int cc = threadIdx.x & (C - 1), i0 = threadIdx.x / C, rstep = blockDim.x / C;
+  // thread -> (column cc, first row i0); rows advance by the block's row stride.
+  // Loads go in batches of U per thread (indexes clamped in range, values zeroed
+  // outside it) so that they overlap.
+  const int cc = threadIdx.x & (C - 1), i0 = threadIdx.x / C, rstep = kThreads / C;
   const int col = c0 + cc;
   const bool colOk = col < Nc;
-  for (int i = i0; i < Nr; i += rstep) {
-    T vx = 0, vy = 0;
-    if (i < g.Lr && colOk) {
-      const cpx<T> v = S[(long)i * Nc + col];
-      vx = v.x;
-      vy = v.y;
+  const int colc = colOk ? col : Nc - 1;
+  constexpr int U = 10;
+  for (int ib = i0; ib < Nr; ib += U * rstep) {
+    cpx<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = gload(S + (long)min(ib + u * rstep, g.Lr - 1) * Nc + colc);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = ib + u * rstep;
+      if (i < Nr) A[cc * ld + i] = (i < g.Lr && colOk) ? v[u] : cpx<T>{0, 0};
     }
-    A[cc * ld + i] = cpx<T>{vx, vy};
   }
   __syncthreads();
   const FFTPlan& pl = Tk.pr;
   cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, TW, +1);
   cpx<T>* Y = (X == A) ? B : A;
-  if (colOk)
-    for (int i = i0; i < Nr; i += rstep) X[cc * ld + i] = cmul(X[cc * ld + i], khat[(long)i * Nc + col]);
+  for (int ib = i0; ib < Nr; ib += U * rstep) {
+    cpx<T> k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) k[u] = gload(khat + (long)min(ib + u * rstep, Nr - 1) * Nc + colc);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = ib + u * rstep;
+      if (i < Nr && colOk) X[cc * ld + i] = cmul(X[cc * ld + i], k[u]);
+    }
+  }
   __syncthreads();
   X = fft_lds<T>(X, Y, pl, C, ld, TW, -1);
   if (colOk)
@@ -902,13 +1082,13 @@ __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, 
   stage_twiddles(TW, tw, Nc);
   const int ir0 = blockIdx.x * rowsPerBlock;
   const int total = rowsPerBlock * Nc;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+  for (int e = threadIdx.x; e < total; e += kThreads) {
     const int ir = ir0 + e / Nc;
     A[e] = (ir < nRows) ? buf[(long)ir0 * Nc + e] : cpx<T>{0, 0};
   }
   __syncthreads();
   cpx<T>* out = fft_lds<T>(A, B, pl, rowsPerBlock, Nc, TW, dir);
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+  for (int e = threadIdx.x; e < total; e += kThreads) {
     const int ir = ir0 + e / Nc;
     if (ir < nRows) buf[(long)ir0 * Nc + e] = out[e];
   }
@@ -923,13 +1103,13 @@ __global__ __launch_bounds__(256) void k_colfft(cpx<T>* buf, int Nr, int Nc, FFT
   cpx<T>* TW = B + C * ld;
   stage_twiddles(TW, tw, Nr);
   const int c0 = blockIdx.x * C;
-  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+  for (int e = threadIdx.x; e < Nr * C; e += kThreads) {
     const int i = e / C, cc = e - i * C, col = c0 + cc;
     A[cc * ld + i] = (col < Nc) ? buf[(long)i * Nc + col] : cpx<T>{0, 0};
   }
   __syncthreads();
   cpx<T>* X = fft_lds<T>(A, B, pl, C, ld, TW, +1);
-  for (int e = threadIdx.x; e < Nr * C; e += blockDim.x) {
+  for (int e = threadIdx.x; e < Nr * C; e += kThreads) {
     const int i = e / C, cc = e - i * C, col = c0 + cc;
     if (col < Nc) buf[(long)i * Nc + col] = X[cc * ld + i];
   }
@@ -1063,52 +1243,6 @@ __device__ __forceinline__ void stat_pixel_dw(const TaskStat& st, const StatLaun
   }
 }
 
-// Block-wide merge of the enorm state; the result is valid in thread 0.
-__device__ ENorm block_reduce(ENorm e) {
-  __shared__ double red[5][16];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // deterministic: xor butterfly inside the wave (both partners merge lower-with-
-  // upper, so they agree bitwise), then the waves in order
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    ENorm o;
-    o.s1 = __shfl_xor(e.s1, m, 64);
-    o.x1max = __shfl_xor(e.x1max, m, 64);
-    o.s2 = __shfl_xor(e.s2, m, 64);
-    o.s3 = __shfl_xor(e.s3, m, 64);
-    o.x3max = __shfl_xor(e.x3max, m, 64);
-    if (lane & m) {
-      ENorm t = o;
-      en_merge(t, e);
-      e = t;
-    } else {
-      en_merge(e, o);
-    }
-  }
-  __syncthreads();
-  if (lane == 0) {
-    red[0][wv] = e.s1; red[1][wv] = e.x1max; red[2][wv] = e.s2; red[3][wv] = e.s3; red[4][wv] = e.x3max;
-  }
-  __syncthreads();
-  ENorm t = e;
-  if (threadIdx.x == 0) {
-    t = ENorm{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0]};
-    const int nw = blockDim.x >> 6;
-    for (int k = 1; k < nw; k++) {
-      ENorm o = {red[0][k], red[1][k], red[2][k], red[3][k], red[4][k]};
-      en_merge(t, o);
-    }
-  }
-  return t;
-}
-
-__device__ void block_reduce_store(ENorm e, double* dst) {
-  ENorm t = block_reduce(e);
-  if (threadIdx.x == 0) {
-    dst[0] = t.s1; dst[1] = t.x1max; dst[2] = t.s2; dst[3] = t.s3; dst[4] = t.x3max;
-  }
-}
-
 // Statistic of one vector from its enorm state (final step of mp_enorm squared,
 // core/mp_enorm.cpp:104-121, for chi^2; 2 * sum for Cash / Poisson-MLR; the plain
 // sum for gradients).
@@ -1131,25 +1265,94 @@ __device__ double final_stat(const ENorm& t, int mode) {
   return 2.0 * t.s2;
 }
 
-// Merge the partial slots of one plane of one pair (block-wide, result in thread 0).
-
-// Store this block's partials of plane A and plane B in its slot.
-__device__ void finish_block(double* partials, int slot, const ENorm& ea, const ENorm& eb) {
-  block_reduce_store(ea, partials + (long)slot * 10);
-  block_reduce_store(eb, partials + (long)slot * 10 + 5);
+// Block-wide merge of two enorm states at once, deterministic: an xor butterfly
+// inside each wave (both partners merge lower-with-upper, so they agree bitwise),
+// then the waves in order.  The results are valid in thread 0.
+__device__ void block_reduce2(ENorm& a, ENorm& b) {
+  __shared__ double red[10][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    ENorm oa, ob;
+    oa.s1 = __shfl_xor(a.s1, m, 64);
+    oa.x1max = __shfl_xor(a.x1max, m, 64);
+    oa.s2 = __shfl_xor(a.s2, m, 64);
+    oa.s3 = __shfl_xor(a.s3, m, 64);
+    oa.x3max = __shfl_xor(a.x3max, m, 64);
+    ob.s1 = __shfl_xor(b.s1, m, 64);
+    ob.x1max = __shfl_xor(b.x1max, m, 64);
+    ob.s2 = __shfl_xor(b.s2, m, 64);
+    ob.s3 = __shfl_xor(b.s3, m, 64);
+    ob.x3max = __shfl_xor(b.x3max, m, 64);
+    if (lane & m) {
+      en_merge(oa, a);
+      a = oa;
+      en_merge(ob, b);
+      b = ob;
+    } else {
+      en_merge(a, oa);
+      en_merge(b, ob);
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    red[0][wv] = a.s1; red[1][wv] = a.x1max; red[2][wv] = a.s2; red[3][wv] = a.s3; red[4][wv] = a.x3max;
+    red[5][wv] = b.s1; red[6][wv] = b.x1max; red[7][wv] = b.s2; red[8][wv] = b.s3; red[9][wv] = b.x3max;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = ENorm{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0]};
+    b = ENorm{red[5][0], red[6][0], red[7][0], red[8][0], red[9][0]};
+    const int nw = kThreads >> 6;
+    for (int k = 1; k < nw; k++) {
+      ENorm oa = {red[0][k], red[1][k], red[2][k], red[3][k], red[4][k]};
+      ENorm ob = {red[5][k], red[6][k], red[7][k], red[8][k], red[9][k]};
+      en_merge(a, oa);
+      en_merge(b, ob);
+    }
+  }
 }
 
-// Merge the partials of plane `plane` over a job's slot ranges (fixed order).
-__device__ ENorm merge_job(const FinalJob& J, const double* partials, int plane) {
-  ENorm e;
-  en_init(e);
-  for (int k = 0; k < J.nRange; k++)
-    for (int s = threadIdx.x; s < J.count[k]; s += blockDim.x) {
-      const double* b = partials + (long)(J.slot[k] + s) * 10 + plane * 5;
-      ENorm o = {b[0], b[1], b[2], b[3], b[4]};
-      en_merge(e, o);
+// Store this block's partials of plane A and plane B in its slot.
+__device__ void finish_block(double* partials, int slot, ENorm ea, ENorm eb) {
+  block_reduce2(ea, eb);
+  if (threadIdx.x == 0) {
+    double* dst = partials + (long)slot * 10;
+    dst[0] = ea.s1; dst[1] = ea.x1max; dst[2] = ea.s2; dst[3] = ea.s3; dst[4] = ea.x3max;
+    dst[5] = eb.s1; dst[6] = eb.x1max; dst[7] = eb.s2; dst[8] = eb.s3; dst[9] = eb.x3max;
+  }
+}
+
+// Merge the partials of both planes over a job's slot ranges.  Each thread merges
+// its slots s, s + kThreads, s + 2 kThreads, ... in that order (two slots' loads in
+// flight at a time), then the block merges the threads in a fixed order.
+__device__ void merge_job(const FinalJob& J, const double* partials, ENorm& a, ENorm& b) {
+  en_init(a);
+  en_init(b);
+  for (int k = 0; k < J.nRange; k++) {
+    const int cnt = J.count[k];
+    const double* base = partials + (long)J.slot[k] * 10;
+    for (int s = threadIdx.x; s < cnt; s += 2 * kThreads) {
+      const int s2 = min(s + kThreads, cnt - 1);
+      double v[20];
+#pragma unroll
+      for (int q = 0; q < 10; q++) {
+        v[q] = base[(long)s * 10 + q];
+        v[10 + q] = base[(long)s2 * 10 + q];
+      }
+      ENorm o = {v[0], v[1], v[2], v[3], v[4]};
+      en_merge(a, o);
+      o = ENorm{v[5], v[6], v[7], v[8], v[9]};
+      en_merge(b, o);
+      if (s + kThreads < cnt) {
+        o = ENorm{v[10], v[11], v[12], v[13], v[14]};
+        en_merge(a, o);
+        o = ENorm{v[15], v[16], v[17], v[18], v[19]};
+        en_merge(b, o);
+      }
     }
-  return block_reduce(e);
+  }
+  block_reduce2(a, b);
 }
 
 // One block per job: the statistic of the job's vector(s) from the partial slots
@@ -1159,19 +1362,15 @@ __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs
                                                double* __restrict__ out) {
   const FinalJob& J = jobs[jobList ? jobList[blockIdx.x] : blockIdx.x];
   if (J.active && !*J.active) return;
-  ENorm a = merge_job(J, partials, 0);
+  ENorm a, b;
+  merge_job(J, partials, a, b);
+  if (threadIdx.x != 0) return;
   if (J.halves) {
-    ENorm b = merge_job(J, partials, 1);
-    if (threadIdx.x == 0) {
-      en_merge(a, b);
-      out[J.outA] = final_stat(a, mode);
-    }
+    en_merge(a, b);
+    out[J.outA] = final_stat(a, mode);
   } else {
-    if (threadIdx.x == 0) out[J.outA] = final_stat(a, mode);
-    if (J.outB >= 0) {
-      ENorm b = merge_job(J, partials, 1);
-      if (threadIdx.x == 0) out[J.outB] = final_stat(b, mode);
-    }
+    out[J.outA] = final_stat(a, mode);
+    if (J.outB >= 0) out[J.outB] = final_stat(b, mode);
   }
 }
 
@@ -1206,13 +1405,13 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
       const int r = lb * Tk.rows3 + rr;
       if (r >= g.Hh) break;
       const int ra = pr.y + r, rb = pr.w + r;
-      for (int j0 = threadIdx.x; j0 < g.W; j0 += K * blockDim.x) {
+      for (int j0 = threadIdx.x; j0 < g.W; j0 += K * kThreads) {
         int cols[K];
         bool ok[K];
         T va[K], vb[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-          cols[k] = j0 + k * blockDim.x;
+          cols[k] = j0 + k * kThreads;
           ok[k] = cols[k] < g.W;
         }
         if (ra < st.H) {
@@ -1240,14 +1439,14 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     // single-row main blocks: fetch the data and weights of this block's pixels
     // before the inverse FFT so that their latency overlaps it
     constexpr int K = 4;
-    const bool pre = Tk.kind == TK_MAIN_CONV && rows == 1 && g.W <= K * (int)blockDim.x && io0 < g.Hh &&
+    const bool pre = Tk.kind == TK_MAIN_CONV && rows == 1 && g.W <= K * kThreads && io0 < g.Hh &&
                      mode_reads_data(sl.mode);
     T dA[K], wA[K], dB[K], wB[K];
     if (pre) {
       const int ra = pr.y + io0, rb = pr.w + io0;
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int c = threadIdx.x + k * blockDim.x;
+        const int c = threadIdx.x + k * kThreads;
         dA[k] = wA[k] = dB[k] = wB[k] = 0;
         if (c < g.W && ra < st.H) {
           dA[k] = ((const T*)st.data)[(long)ra * st.W + c];
@@ -1259,11 +1458,9 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
         }
       }
     }
-    for (int rr = 0; rr < rows; rr++) {
-      const int io = io0 + rr;
-      for (int j = threadIdx.x; j < g.Nc; j += blockDim.x)
-        A[rr * g.Nc + j] = (io < g.Hh) ? S[(long)(g.hb + io) * g.Nc + j] : cpx<T>{0, 0};
-    }
+    // rows io0 .. io0 + rows - 1 of the output band are contiguous in the plane
+    const int validRows = max(0, min(rows, g.Hh - io0));
+    lds_fill<8>(A, S + (long)(g.hb + io0) * g.Nc, rows * g.Nc, validRows * g.Nc);
     __syncthreads();
     const FFTPlan& pl = Tk.pc;
     cpx<T>* out = fft_lds<T>(A, B, pl, rows, g.Nc, TW, -1);
@@ -1271,7 +1468,7 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
       const int ra = pr.y + io0, rb = pr.w + io0;
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int c = threadIdx.x + k * blockDim.x;
+        const int c = threadIdx.x + k * kThreads;
         if (c >= g.W) continue;
         const cpx<T> v = out[c + g.hl];
         if (ra < st.H) stat_pixel_dw<T>(st, sl, pr.x, ra, c, (double)v.x, (double)dA[k], (double)wA[k], ea, true);
@@ -1280,7 +1477,7 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     } else if (Tk.kind == TK_MAIN_CONV) {
       for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++) {
         const int io = io0 + rr, ra = pr.y + io, rb = pr.w + io;
-        for (int c = threadIdx.x; c < g.W; c += blockDim.x) {
+        for (int c = threadIdx.x; c < g.W; c += kThreads) {
           const cpx<T> v = out[rr * g.Nc + c + g.hl];
           if (ra < st.H) stat_pixel<T>(st, sl, pr.x, ra, c, (double)v.x, ea, true);
           if (pr.z >= 0 && rb < st.H) stat_pixel<T>(st, sl, pr.z, rb, c, (double)v.y, eb, true);
@@ -1290,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
       const int s = Tk.scale;
       const int nbr = rows / s;
       const double area = s * s;
-      for (int e = threadIdx.x; e < 2 * nbr * Tk.nbc; e += blockDim.x) {
+      for (int e = threadIdx.x; e < 2 * nbr * Tk.nbc; e += kThreads) {
         const int plane = e / (nbr * Tk.nbc);
         const int q = e - plane * nbr * Tk.nbc;
         const int k = q / Tk.nbc, bj = q - k * Tk.nbc;
@@ -1333,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict
   const int tile = e.y;
   const long p0 = (long)tile * TP;
   const int nT = nFree * (nFree + 1) / 2;
-  for (int e = threadIdx.x; e < (nFree + 1) * TP; e += blockDim.x) {
+  for (int e = threadIdx.x; e < (nFree + 1) * TP; e += kThreads) {
     const int j = e / TP, p = e - j * TP;
     const long i = p0 + p;
     double v = 0.0;
@@ -1341,7 +1538,7 @@ __global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict
     J[e] = v;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < nT + nFree; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nT + nFree; e += kThreads) {
     int a, b;
     if (e < nT) {   // e -> (a, b), a <= b, row-major packed upper triangle
       a = 0;
@@ -1373,10 +1570,10 @@ __global__ __launch_bounds__(256) void k_normal_final(const NormalJob* __restric
   const int e = blockIdx.x;
   if (e >= nEnt || (Jb.active && !*Jb.active)) return;
   double acc = 0.0;
-  for (int b = threadIdx.x; b < Jb.nTiles; b += blockDim.x) acc += partials[Jb.part0 + (long)b * nEnt + e];
+  for (int b = threadIdx.x; b < Jb.nTiles; b += kThreads) acc += partials[Jb.part0 + (long)b * nEnt + e];
   red[threadIdx.x] = acc;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
