@@ -1267,34 +1267,50 @@ __device__ double final_stat(const ENorm& t, int mode) {
 
 // Block-wide merge of two enorm states at once, deterministic: an xor butterfly
 // inside each wave (both partners merge lower-with-upper, so they agree bitwise),
-// then the waves in order.  The results are valid in thread 0.
+// then the waves in order.  A wave whose lanes hold no small or large terms (only
+// the s2 sums, the common case) shuffles just the two s2 values: merging the zero
+// states would add nothing, so the result is the same bit for bit.  The results are
+// valid in thread 0.  Called once per block (its LDS array is not reused).
 __device__ void block_reduce2(ENorm& a, ENorm& b) {
-  __shared__ double red[10][16];
+  __shared__ double red[10][kThreads / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (!__any(a.x1max > 0.0 || a.x3max > 0.0 || b.x1max > 0.0 || b.x3max > 0.0)) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    ENorm oa, ob;
-    oa.s1 = __shfl_xor(a.s1, m, 64);
-    oa.x1max = __shfl_xor(a.x1max, m, 64);
-    oa.s2 = __shfl_xor(a.s2, m, 64);
-    oa.s3 = __shfl_xor(a.s3, m, 64);
-    oa.x3max = __shfl_xor(a.x3max, m, 64);
-    ob.s1 = __shfl_xor(b.s1, m, 64);
-    ob.x1max = __shfl_xor(b.x1max, m, 64);
-    ob.s2 = __shfl_xor(b.s2, m, 64);
-    ob.s3 = __shfl_xor(b.s3, m, 64);
-    ob.x3max = __shfl_xor(b.x3max, m, 64);
-    if (lane & m) {
-      en_merge(oa, a);
-      a = oa;
-      en_merge(ob, b);
-      b = ob;
-    } else {
-      en_merge(a, oa);
-      en_merge(b, ob);
+    for (int m = 32; m >= 1; m >>= 1) {
+      const double oa = __shfl_xor(a.s2, m, 64), ob = __shfl_xor(b.s2, m, 64);
+      if (lane & m) {
+        a.s2 = oa + a.s2;
+        b.s2 = ob + b.s2;
+      } else {
+        a.s2 += oa;
+        b.s2 += ob;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      ENorm oa, ob;
+      oa.s1 = __shfl_xor(a.s1, m, 64);
+      oa.x1max = __shfl_xor(a.x1max, m, 64);
+      oa.s2 = __shfl_xor(a.s2, m, 64);
+      oa.s3 = __shfl_xor(a.s3, m, 64);
+      oa.x3max = __shfl_xor(a.x3max, m, 64);
+      ob.s1 = __shfl_xor(b.s1, m, 64);
+      ob.x1max = __shfl_xor(b.x1max, m, 64);
+      ob.s2 = __shfl_xor(b.s2, m, 64);
+      ob.s3 = __shfl_xor(b.s3, m, 64);
+      ob.x3max = __shfl_xor(b.x3max, m, 64);
+      if (lane & m) {
+        en_merge(oa, a);
+        a = oa;
+        en_merge(ob, b);
+        b = ob;
+      } else {
+        en_merge(a, oa);
+        en_merge(b, ob);
+      }
     }
   }
-  __syncthreads();
   if (lane == 0) {
     red[0][wv] = a.s1; red[1][wv] = a.x1max; red[2][wv] = a.s2; red[3][wv] = a.s3; red[4][wv] = a.x3max;
     red[5][wv] = b.s1; red[6][wv] = b.x1max; red[7][wv] = b.s2; red[8][wv] = b.s3; red[9][wv] = b.x3max;
@@ -1303,8 +1319,7 @@ __device__ void block_reduce2(ENorm& a, ENorm& b) {
   if (threadIdx.x == 0) {
     a = ENorm{red[0][0], red[1][0], red[2][0], red[3][0], red[4][0]};
     b = ENorm{red[5][0], red[6][0], red[7][0], red[8][0], red[9][0]};
-    const int nw = kThreads >> 6;
-    for (int k = 1; k < nw; k++) {
+    for (int k = 1; k < kThreads / 64; k++) {
       ENorm oa = {red[0][k], red[1][k], red[2][k], red[3][k], red[4][k]};
       ENorm ob = {red[5][k], red[6][k], red[7][k], red[8][k], red[9][k]};
       en_merge(a, oa);
