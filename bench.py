@@ -236,12 +236,19 @@ def cutout_benchmark(n=2000, n_plain=10):
                 c_plain, c_batch = models[k].GetFitStatistic(x), models[k].GetFitStatistic(xb[k])
                 rel.append(abs(c_batch - c_plain) / c_plain)
             dtp = time.perf_counter() - t0
+            matched = [r for r in rel if r <= 1e-9]
             res["plain_fd_mpfit_fits_per_s"] = n_plain / dtp
             res["plain_sample"] = n_plain
+            # A cutout whose fit has two minima can end in either, with either method: its
+            # end point then depends on rounding (from starts perturbed by 1e-9, mpfit
+            # itself lands in both), so matched and unmatched cases are reported apart.
+            res["sample_matched_chi2_1e-9"] = len(matched)
+            res["chi2_rel_diff_max_matched"] = float(max(matched)) if matched else None
             res["chi2_rel_diff_max_on_sample"] = float(max(rel))
             res["speedup_fp64"] = (n / dt) / (n_plain / dtp)
-            log("cfg5 plain FD mpfit (fp64, one cutout at a time): %.2f fits/s; chi2 rel diff max %.2e" %
-                (n_plain / dtp, max(rel)))
+            log("cfg5 plain FD mpfit (fp64, one cutout at a time): %.2f fits/s; same minimum (chi2 within 1e-9) "
+                "on %d of %d sampled cutouts (max rel diff %.2e there), max rel diff overall %.2e" %
+                (n_plain / dtp, len(matched), n_plain, max(matched) if matched else float("nan"), max(rel)))
         del b, models
     return res
 
@@ -260,8 +267,8 @@ def main():
     ap.add_argument("--no-aux", action="store_true", help="skip the cfg3 / cfg4 side measurements")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU sharding of the path is design-only "
-              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: this build measures one GPU; sharding the path over several GPUs is a design "
+              "only, not built. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import imfit_amd

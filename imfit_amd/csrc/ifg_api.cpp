@@ -292,6 +292,8 @@ struct Plan {
   size_t recBytes = 0, boxBytes = 0, stageBytes = 0;
   void* pinnedStage = nullptr;
   void* pinnedOut = nullptr;
+  double* outDev = nullptr;   // pinnedOut as the device addresses it
+  bool directOut = false;     // k_final writes the statistics into pinnedOut itself (few jobs)
   DevBuf stage_d, tail_d, pairs_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, tasks_d;
   DevBuf maps_d, jobs_d, njobs_d, tileMap_d;
   size_t usedOff = 0, prefixOff = 0;   // staging offsets of the per-grid box pixel counts / tail prefixes
@@ -808,6 +810,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0),
                        hipHostMallocDefault),
          "pinned results");
+  HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
   if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
   if (flags & PF_JAC) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nFree), "alloc Jacobian columns");
   if (flags & PF_MODEL) HIPCHK(P.model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
@@ -982,6 +985,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     P.nJobs[c] = np;
   }
   P.nSlots = maxSlots;
+  P.directOut = jobs.size() <= 8;
   P.tailWaves = 0;
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
   HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
@@ -1088,10 +1092,10 @@ int Enqueue(ifg_model* m, Plan& P) {
     TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, map1, P.nB1[c], P.lds1, disk, deriv), "k_eval_rowfft");
     TIMED("k_colconv", launch_colconv<T>(s, tk, map2, P.nB2[c], P.lds2), "k_colconv");
     TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv), "k_inverse_stat");
-    if (reduce)
+    if (reduce)   // a few results go straight into the pinned results (mapped): no copy back
       TIMED("k_final",
             launch_final(s, P.jobs_d.as<FinalJob>() + P.jobOff[c], nullptr, P.nJobs[c], P.partials_d.as<double>(),
-                         mode, P.out_d.as<double>()),
+                         mode, P.directOut ? P.outDev : P.out_d.as<double>(), P.directOut ? 1 : 0),
             "k_final");
   }
   if (P.flags & PF_JAC) {
@@ -1101,7 +1105,7 @@ int Enqueue(ifg_model* m, Plan& P) {
                         P.normPart_d.as<double>()),
           "k_normal");
     HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nEnt, hipMemcpyDeviceToHost, s), "normal D2H");
-  } else if (mode != S_NONE && (P.flags & PF_STAT)) {
+  } else if (reduce && !P.directOut) {
     HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nVec, hipMemcpyDeviceToHost, s), "results D2H");
   }
   return IFG_OK;
@@ -2109,7 +2113,7 @@ int EnqueueBatch(ifg_batch* b, BatchPlan& P, const BatchLaunch& L) {
     return b->hipfail(e, "k_inverse_stat");
   if (!deriv)
     e = launch_final(s, P.jobs_d.as<FinalJob>(), L.jobList, L.nJobs, P.partials_d.as<double>(), mode,
-                     P.out_d.as<double>());
+                     P.out_d.as<double>(), 0);
   else
     e = launch_normal(s, P.njobs_d.as<NormalJob>(), L.jobList, L.nJobs, L.tiles, L.nTiles, P.maxFree,
                       P.normPart_d.as<double>());

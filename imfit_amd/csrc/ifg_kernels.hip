@@ -736,13 +736,15 @@ __device__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
 // their tail value.  DERIV: d(model)/d(parameter freeIdx[vec]) instead.
 template <typename T, bool DISK, bool DERIV, int K>
 __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g, int vec, const CompRec* rec,
-                                           const Box* bx, int r, const int (&cols)[K], const bool (&ok)[K],
-                                           T (&out)[K]) {
-  const double y = g.ya + (double)r * g.frac;
+                                           const Box* bx, const int (&rws)[K], const int (&cols)[K],
+                                           const bool (&ok)[K], T (&out)[K]) {
+  double y[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) y[k] = g.ya + (double)rws[k] * g.frac;
   if (DERIV) {
 #pragma unroll 1
     for (int k = 0; k < K; k++)
-      out[k] = ok[k] ? (T)deriv_point<DISK>(ea, vec, g.xa + (double)cols[k] * g.frac, y) : (T)0;
+      out[k] = ok[k] ? (T)deriv_point<DISK>(ea, vec, g.xa + (double)cols[k] * g.frac, y[k]) : (T)0;
   } else {
     T sum[K], err[K];
 #pragma unroll
@@ -756,7 +758,7 @@ __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g
 #pragma unroll
       for (int k = 0; k < K; k++) {
         if (!ok[k]) continue;
-        const T adjVal = comp_point<T, DISK>(type, v, g.xa + (double)cols[k] * g.frac, y) - err[k];
+        const T adjVal = comp_point<T, DISK>(type, v, g.xa + (double)cols[k] * g.frac, y[k]) - err[k];
         const T tempSum = sum[k] + adjVal;
         err[k] = (tempSum - sum[k]) - adjVal;
         sum[k] = tempSum;
@@ -768,14 +770,12 @@ __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g
 #pragma unroll
   for (int b = 0; b < IFG_MAXBOX; b++) {
     const Box B = bx[b];
-    const int dr = r - B.r0;
-    if (B.w > 0 && (unsigned)dr < (unsigned)B.h) {
+    if (B.w <= 0) continue;
 #pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int dc = cols[k] - B.c0;
-        if (ok[k] && (unsigned)dc < (unsigned)B.w)
-          out[k] = (T)ea.tail[(long)vec * ea.tailStride + B.off + dr * B.w + dc];
-      }
+    for (int k = 0; k < K; k++) {
+      const int dr = rws[k] - B.r0, dc = cols[k] - B.c0;
+      if (ok[k] && (unsigned)dr < (unsigned)B.h && (unsigned)dc < (unsigned)B.w)
+        out[k] = (T)ea.tail[(long)vec * ea.tailStride + B.off + dr * B.w + dc];
     }
   }
 }
@@ -974,27 +974,29 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   __shared__ Box sBox[2][IFG_MAXBOX];
   stage_pair<DERIV>(Tk, pr, sRec, sBox);
   const int ir0 = lb * rows;
-  constexpr int K = 4;   // pixels per thread per pass
-  for (int rr = 0; rr < rows; rr++) {
-    const int ir = ir0 + rr;
-    for (int j0 = threadIdx.x; j0 < g.Nc; j0 += K * kThreads) {
-      int cols[K];
-      bool ok[K];
-      T va[K], vb[K];
+  // the block's rows x Nc plane entries as one index space, K per thread per pass
+  // (one pass for rows of up to K * kThreads entries)
+  constexpr int K = 5;
+  const int total = rows * g.Nc;
+  for (int i0 = threadIdx.x; i0 < total; i0 += K * kThreads) {
+    int cols[K], ra[K], rb[K];
+    bool ok[K];
+    T va[K], vb[K];
 #pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int j = j0 + k * kThreads;
-        cols[k] = j - g.hl;
-        ok[k] = ir < g.Lr && j < g.Lc;
-        vb[k] = 0;
-      }
-      row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], pr.y + ir - g.hb, cols, ok, va);
-      if (pr.z >= 0) row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], pr.w + ir - g.hb, cols, ok, vb);
+    for (int k = 0; k < K; k++) {
+      const int i = i0 + k * kThreads, rr = i / g.Nc, j = i - rr * g.Nc, ir = ir0 + rr;
+      cols[k] = j - g.hl;
+      ra[k] = pr.y + ir - g.hb;
+      rb[k] = pr.w + ir - g.hb;
+      ok[k] = i < total && ir < g.Lr && j < g.Lc;
+      vb[k] = 0;
+    }
+    row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], ra, cols, ok, va);
+    if (pr.z >= 0) row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rb, cols, ok, vb);
 #pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int j = j0 + k * kThreads;
-        if (j < g.Nc) A[rr * g.Nc + j] = ok[k] ? cpx<T>{va[k], vb[k]} : cpx<T>{0, 0};
-      }
+    for (int k = 0; k < K; k++) {
+      const int i = i0 + k * kThreads;
+      if (i < total) A[i] = ok[k] ? cpx<T>{va[k], vb[k]} : cpx<T>{0, 0};
     }
   }
   __syncthreads();
@@ -1271,7 +1273,7 @@ __device__ double final_stat(const ENorm& t, int mode) {
 // the s2 sums, the common case) shuffles just the two s2 values: merging the zero
 // states would add nothing, so the result is the same bit for bit.  The results are
 // valid in thread 0.  Called once per block (its LDS array is not reused).
-__device__ void block_reduce2(ENorm& a, ENorm& b) {
+__device__ __forceinline__ void block_reduce2(ENorm& a, ENorm& b) {
   __shared__ double red[10][kThreads / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (!__any(a.x1max > 0.0 || a.x3max > 0.0 || b.x1max > 0.0 || b.x3max > 0.0)) {
@@ -1329,7 +1331,7 @@ __device__ void block_reduce2(ENorm& a, ENorm& b) {
 }
 
 // Store this block's partials of plane A and plane B in its slot.
-__device__ void finish_block(double* partials, int slot, ENorm ea, ENorm eb) {
+__device__ __forceinline__ void finish_block(double* partials, int slot, ENorm ea, ENorm eb) {
   block_reduce2(ea, eb);
   if (threadIdx.x == 0) {
     double* dst = partials + (long)slot * 10;
@@ -1341,7 +1343,7 @@ __device__ void finish_block(double* partials, int slot, ENorm ea, ENorm eb) {
 // Merge the partials of both planes over a job's slot ranges.  Each thread merges
 // its slots s, s + kThreads, s + 2 kThreads, ... in that order (two slots' loads in
 // flight at a time), then the block merges the threads in a fixed order.
-__device__ void merge_job(const FinalJob& J, const double* partials, ENorm& a, ENorm& b) {
+__device__ __forceinline__ void merge_job(const FinalJob& J, const double* partials, ENorm& a, ENorm& b) {
   en_init(a);
   en_init(b);
   for (int k = 0; k < J.nRange; k++) {
@@ -1370,22 +1372,29 @@ __device__ void merge_job(const FinalJob& J, const double* partials, ENorm& a, E
   block_reduce2(a, b);
 }
 
-// One block per job: the statistic of the job's vector(s) from the partial slots
-// of its pair (both planes of a single image split into halves are one vector).
-__global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const int* __restrict__ jobList,
-                                               const double* __restrict__ partials, int mode,
-                                               double* __restrict__ out) {
-  const FinalJob& J = jobs[jobList ? jobList[blockIdx.x] : blockIdx.x];
-  if (J.active && !*J.active) return;
-  ENorm a, b;
-  merge_job(J, partials, a, b);
-  if (threadIdx.x != 0) return;
+// The statistic(s) of a job from its merged planes (thread 0).
+__device__ __forceinline__ void write_job(const FinalJob& J, ENorm a, const ENorm& b, int mode, double* out) {
   if (J.halves) {
     en_merge(a, b);
     out[J.outA] = final_stat(a, mode);
   } else {
     out[J.outA] = final_stat(a, mode);
     if (J.outB >= 0) out[J.outB] = final_stat(b, mode);
+  }
+}
+
+// One block per job: the statistic of the job's vector(s) from the partial slots
+// of its pair (both planes of a single image split into halves are one vector).
+__global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const int* __restrict__ jobList,
+                                               const double* __restrict__ partials, int mode,
+                                               double* __restrict__ out, int hostOut) {
+  const FinalJob& J = jobs[jobList ? jobList[blockIdx.x] : blockIdx.x];
+  if (J.active && !*J.active) return;
+  ENorm a, b;
+  merge_job(J, partials, a, b);
+  if (threadIdx.x == 0) {
+    write_job(J, a, b, mode, out);
+    if (hostOut) __threadfence_system();   // the host reads it once the stream is done
   }
 }
 
@@ -1429,14 +1438,20 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
           cols[k] = j0 + k * kThreads;
           ok[k] = cols[k] < g.W;
         }
+        int rwa[K], rwb[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          rwa[k] = ra;
+          rwb[k] = rb;
+        }
         if (ra < st.H) {
-          row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], ra, cols, ok, va);
+          row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], rwa, cols, ok, va);
 #pragma unroll
           for (int k = 0; k < K; k++)
             if (ok[k]) stat_pixel<T>(st, sl, pr.x, ra, cols[k], (double)va[k], ea, true);
         }
         if (pr.z >= 0 && rb < st.H) {
-          row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rb, cols, ok, vb);
+          row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rwb, cols, ok, vb);
 #pragma unroll
           for (int k = 0; k < K; k++)
             if (ok[k]) stat_pixel<T>(st, sl, pr.z, rb, cols[k], (double)vb[k], eb, true);
@@ -1662,9 +1677,9 @@ hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPl
 }
 
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
-                        int mode, double* out) {
+                        int mode, double* out, int hostOut) {
   if (nJobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out);
+  hipLaunchKernelGGL(k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut);
   return hipGetLastError();
 }
 

@@ -22,9 +22,10 @@ hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FF
                          int dir, int rows);
 template <typename T>
 hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C);
-// jobList: the jobs of this launch (null: jobs 0 .. nJobs-1)
+// jobList: the jobs of this launch (null: jobs 0 .. nJobs-1); hostOut: out is mapped
+// host memory (each job block ends with a system-scope fence)
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
-                        int mode, double* out);
+                        int mode, double* out, int hostOut);
 // pixels per tile of a normal-equations job with nFree columns
 int normal_tile_pixels(int nFree);
 hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobList, int nJobs, const int2* tileMap,
