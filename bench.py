@@ -178,6 +178,7 @@ def aux_benchmarks(steps):
     p4 = w4.start_params(seed=1, frac=0.05)
     for prec, key in ((imfit_amd.PRECISION_FP32, "fp32"), (imfit_amd.PRECISION_FP64, "fp64")):
         m = w4.fit_model(prec)
+        m.Autotune(p4)
         m.GetFitStatistic(p4)
         device_sync()
         n, t0 = max(3, steps // 20), time.perf_counter()
@@ -188,6 +189,7 @@ def aux_benchmarks(steps):
     log("cfg4 (3-D disk + bulge, 512^2, PSF 25): %s" % out["cfg4_expdisk3d_512"])
     w3, pop = workloads.cfg3_population()
     m3 = w3.fit_model(imfit_amd.PRECISION_FP32)
+    m3.Autotune(pop[:1024])   # the population grids' shapes (kept for the full population)
     m3.GetFitStatisticBatch(pop[:64])
     device_sync()
     t0 = time.perf_counter()
@@ -219,6 +221,7 @@ def cutout_benchmark(n=2000, n_plain=10):
     for prec, key in ((imfit_amd.PRECISION_FP64, "fp64"), (imfit_amd.PRECISION_FP32, "fp32")):
         models = [w.fit_model(prec) for w in ws]
         b = imfit_amd.BatchGPU(models)
+        b.Autotune(starts)
         b.GetFitStatistic(starts)   # plans and graphs
         device_sync()
         t0 = time.perf_counter()
@@ -279,6 +282,10 @@ def main():
     prec = imfit_amd.PRECISION_FP32 if args.precision == "fp32" else imfit_amd.PRECISION_FP64
     m = w.fit_model(prec)
     p = w.start_params()
+    trials = m.Autotune(p)   # launch shapes for this geometry (untimed, once per geometry)
+    shapes = {"main_cols_rows": list(m.GetLaunchShape(0)), "region_cols_rows": list(m.GetLaunchShape(1)),
+              "autotune_trials": trials}
+    log("autotuned launch shapes (columns per workgroup, rows per workgroup): %s" % shapes)
     for _ in range(args.warmup):
         m.GetFitStatistic(p)
     device_sync()
@@ -310,6 +317,7 @@ def main():
 
     # FP64 parity-mode rate, reported alongside
     m64 = w.fit_model(imfit_amd.PRECISION_FP64)
+    m64.Autotune(p)
     for _ in range(3):
         m64.GetFitStatistic(p)
     device_sync()
@@ -361,7 +369,8 @@ def main():
                       "oversampled_region": "40x40 at 5x, 255x255 PSF",
                       "precision": "%s pixels and FFT, FP64 deviate reduction" % args.precision,
                       "fp64_evals_per_s": fp64_rate, "fp32_vs_fp64_stat_rel": prec_err, "ms_per_step_with_kernel_events": ms_timed,
-                      "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}},
+                      "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()},
+                      "launch_shapes": shapes},
            "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm, "other_configs": aux}
     print(json.dumps(out))
     if args.copy_calibration_mb > 0:

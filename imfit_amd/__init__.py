@@ -90,6 +90,9 @@ _SIGS["ifg_batch_fit_statistic"] = ([_vp, _dp, _dp], C.c_int)
 _SIGS["ifg_batch_normal_equations"] = ([_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp], C.c_int)
 _SIGS["ifg_batch_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
 _SIGS["ifg_batch_set_graph_mode"] = ([_vp, C.c_int], C.c_int)
+_SIGS["ifg_autotune"] = ([_vp, _dp, C.c_int], C.c_int)
+_SIGS["ifg_get_launch_shape"] = ([_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int)
+_SIGS["ifg_batch_autotune"] = ([_vp, _dp, C.POINTER(C.c_int), _dp], C.c_int)
 
 _LIB = C.CDLL(LIB_PATH)
 for _name, (_args, _res) in _SIGS.items():
@@ -308,6 +311,23 @@ class ModelObjectGPU:
     def SetGraphMode(self, on):
         return self._chk(_LIB.ifg_set_graph_mode(self.h, int(on)), "SetGraphMode")
 
+    # ---- launch-shape autotuning
+    def Autotune(self, params):
+        """Time GetFitStatistic (one vector) or GetFitStatisticBatch (rows of a 2-D array)
+        at params under nearby launch shapes and keep the fastest; returns the number of
+        timed trials (0: the shapes came from this process's record for the geometry)."""
+        p = np.ascontiguousarray(np.atleast_2d(np.asarray(params, dtype=np.float64)))
+        if p.shape[1] != self.GetNParams():
+            raise ValueError("params must have GetNParams() columns")
+        return self._chk(_LIB.ifg_autotune(self.h, _ptr(p), p.shape[0]), "Autotune")
+
+    def GetLaunchShape(self, grid=0, population=False):
+        """(columns per workgroup of the column pass, rows per workgroup of the row passes)
+        of grid 0 (main image) or 1.. (oversampled regions)."""
+        c, r = C.c_int(0), C.c_int(0)
+        self._chk(_LIB.ifg_get_launch_shape(self.h, grid, int(population), C.byref(c), C.byref(r)), "GetLaunchShape")
+        return c.value, r.value
+
     def GetKernelTiming(self):
         """{kernel name: (total ms, launches)} recorded with HIP events on the handle's stream."""
         out = {}
@@ -417,3 +437,11 @@ class BatchGPU:
 
     def SetGraphMode(self, on):
         return self._chk(_LIB.ifg_batch_set_graph_mode(self.h, int(on)), "SetGraphMode")
+
+    def Autotune(self, params):
+        """One column-strip width and one rows-per-workgroup multiplier for every model,
+        timed on GetFitStatistic at params; returns (trials, width, multiplier)."""
+        x = self._concat(params)
+        c, f = C.c_int(0), np.zeros(1)
+        n = self._chk(_LIB.ifg_batch_autotune(self.h, _ptr(x), C.byref(c), _ptr(f)), "Autotune")
+        return n, c.value, float(f[0])

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -59,6 +60,7 @@ int NParamsOf(int type) {
     case F_GAUSS: return 4;
     case F_MOFFAT: return 5;
     case F_EXPDISK3D: return 6;
+    case F_GENSERSIC: return 6;
   }
   return 0;
 }
@@ -71,14 +73,17 @@ int TypeOf(const char* name) {
   if (s == "Gaussian") return F_GAUSS;
   if (s == "Moffat") return F_MOFFAT;
   if (s == "ExponentialDisk3D") return F_EXPDISK3D;
+  if (s == "Sersic_GenEllipse") return F_GENSERSIC;
   return 0;
 }
 
-bool Subsampling(int type) { return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT; }
+bool Subsampling(int type) {
+  return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT || type == F_GENSERSIC;
+}
 
 // FunctionObject::Setup for one component (func_sersic.cpp:109-127, func_exp.cpp:104-118,
 // func_gaussian.cpp:104-121, func_moffat.cpp:111-132, func_flatsky.cpp:85-90,
-// func_expdisk3d.cpp:149-172).
+// func_expdisk3d.cpp:149-172, func_gen-sersic.cpp:112-133).
 void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
   std::memset(&R, 0, sizeof(R));
   double* v = R.v;
@@ -107,11 +112,14 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
   v[4] = q;
   v[15] = 1.0 / q;
   switch (type) {
-    case F_SERSIC: {
-      const double n = p[2], I_e = p[3], r_e = p[4];
+    case F_SERSIC:
+    case F_GENSERSIC: {   // Sersic_GenEllipse: PA, ell, c0, n, I_e, r_e
+      const int o = (type == F_GENSERSIC) ? 3 : 2;
+      const double n = p[o], I_e = p[o + 1], r_e = p[o + 2];
       const double bn = Calculate_bn(n);
       v[5] = I_e; v[6] = r_e; v[7] = bn; v[8] = 1.0 / n;
       v[9] = n; v[10] = Calculate_dbn_dn(n);
+      if (type == F_GENSERSIC) v[11] = p[2] + 2.0;   // ellExp = c0 + 2.0 (func_gen-sersic.cpp:130)
       v[12] = 1.0 / (r_e * r_e); v[13] = 0.5 / n; v[14] = -bn * LOG2E;
       break;
     }
@@ -579,7 +587,10 @@ GridGeom PlainGeom(int W, int H, double xa, double ya, double frac) {
 // Sub-pixel boxes of one parameter vector on one grid: every component that may
 // subsample contributes the grid pixels within 10 pixels of its centre (r < 10
 // implies |x - x0| < 10 and |y - y0| < 10 since q <= 1), plus one pixel of margin.
-// Identical boxes are merged; beyond IFG_MAXBOX the boxes collapse to their union.
+// A generalized ellipse with exponent e > 2 reaches 10 * 2^(1/2 - 1/e) from the
+// centre (the corners of |x'| < 10, |y'| < 10q); with e <= 2 it lies inside the
+// ellipse.  Identical boxes are merged; beyond IFG_MAXBOX the boxes collapse to
+// their union.
 int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cLo, int cHi, int rLo, int rHi,
               Box* out) {
   struct B { int c0, c1, r0, r1; };
@@ -589,8 +600,14 @@ int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cL
       if (!Subsampling(m->types[k])) continue;
       const double x0 = recs[k].v[0], y0 = recs[k].v[1];
       if (!std::isfinite(x0) || !std::isfinite(y0)) continue;
-      double fc0 = std::floor((x0 - 10.0 - g.xa) / g.frac) - 1, fc1 = std::ceil((x0 + 10.0 - g.xa) / g.frac) + 1;
-      double fr0 = std::floor((y0 - 10.0 - g.ya) / g.frac) - 1, fr1 = std::ceil((y0 + 10.0 - g.ya) / g.frac) + 1;
+      double rad = 10.0;
+      if (m->types[k] == F_GENSERSIC) {
+        const double e = recs[k].v[11];
+        if (!(e > 0.0)) continue;
+        if (e > 2.0) rad = 10.0 * std::pow(2.0, 0.5 - 1.0 / e);
+      }
+      double fc0 = std::floor((x0 - rad - g.xa) / g.frac) - 1, fc1 = std::ceil((x0 + rad - g.xa) / g.frac) + 1;
+      double fr0 = std::floor((y0 - rad - g.ya) / g.frac) - 1, fr1 = std::ceil((y0 + rad - g.ya) / g.frac) + 1;
       fc0 = std::max(fc0, (double)cLo); fc1 = std::min(fc1, (double)cHi);
       fr0 = std::max(fr0, (double)rLo); fr1 = std::min(fr1, (double)rHi);
       if (fc0 > fc1 || fr0 > fr1) continue;
@@ -2318,6 +2335,265 @@ int ifg_batch_set_graph_mode(ifg_batch* b, int on) {
   if (!b) return IFG_ERR_ARG;
   b->useGraphs = on != 0;
   return IFG_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== launch-shape autotuning
+// The launch shapes of a convolution grid: the column pass's strip width (columns per
+// workgroup, a power of two) and the row passes' rows per workgroup.  The defaults
+// (ColsPerBlock, RowsPerBlock) suit large images; the tuner times the caller's own
+// evaluation under nearby shapes, one grid and one parameter at a time, keeps the
+// fastest, and records the choice per geometry for the process, so that later models
+// with the same grids take it without timing.  Shapes change only how work is split
+// between workgroups: every pixel and transform is computed the same way; the
+// statistic's reduction groups its terms by workgroup, so it can move in the last
+// bits.
+namespace {
+
+struct GridShape {
+  int C = 1, rows1 = 1;
+};
+
+struct TuneGrid {
+  ConvGrid* cg;
+  int split;   // region scale (rows of the inverse pass come in whole groups of it); 1: main grid
+};
+
+std::mutex gTuneMu;
+std::map<std::string, std::vector<GridShape>> gTuned;
+
+size_t ElemBytes(const ifg_model* m) {
+  return m->precision == IFG_PRECISION_FP64 ? sizeof(cpx<double>) : sizeof(cpx<float>);
+}
+
+int MaxLds(const ifg_model* m) {
+  int maxLds = 65536;
+  (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device);
+  return maxLds;
+}
+
+int Rows3For(int rows1, int split) { return split == 1 ? rows1 : std::max(1, rows1 / split) * split; }
+
+bool ShapeFits(const ConvGrid& cg, const GridShape& s, size_t elem, int maxLds, int split) {
+  const GridGeom& g = cg.g;
+  if (s.C < 1 || s.C > 8 || (s.C & (s.C - 1)) || s.rows1 < 1) return false;
+  if ((2 * (size_t)s.C * (g.Nr + 1) + g.Nr) * elem > (size_t)maxLds) return false;
+  if ((2 * (size_t)s.rows1 + 1) * g.Nc * elem > (size_t)maxLds) return false;
+  return (2 * (size_t)Rows3For(s.rows1, split) + 1) * g.Nc * elem <= (size_t)maxLds;
+}
+
+GridShape GetShape(const ifg_model* m, const ConvGrid& cg) {
+  GridShape s;
+  s.C = m->precision == IFG_PRECISION_FP64 ? cg.C64 : cg.C32;
+  s.rows1 = cg.rowsK1;
+  return s;
+}
+
+void SetShape(const ifg_model* m, ConvGrid& cg, const GridShape& s, int split) {
+  if (m->precision == IFG_PRECISION_FP64) cg.C64 = s.C;
+  else cg.C32 = s.C;
+  cg.rowsK1 = s.rows1;
+  cg.rowsK3 = Rows3For(s.rows1, split);
+}
+
+// The grids an evaluation of nVec vectors uses (built on the way by a first evaluation).
+std::vector<TuneGrid> TuneGrids(ifg_model* m, int which) {
+  std::vector<TuneGrid> v;
+  if (m->doConv && m->mainConv[which].ready) v.push_back({&m->mainConv[which], 1});
+  for (auto& R : m->regions)
+    if (R->conv[which].ready) v.push_back({&R->conv[which], R->scale});
+  return v;
+}
+
+std::string GeometryKey(const ifg_model* m, const std::vector<TuneGrid>& grids, int which, int nVec) {
+  std::string k = "p" + std::to_string(m->precision) + "w" + std::to_string(which) + "v" + std::to_string(nVec) +
+                  "c" + std::to_string(m->nComp());
+  for (int t : m->types) k += "," + std::to_string(t);
+  for (const TuneGrid& tg : grids) {
+    const GridGeom& g = tg.cg->g;
+    k += "|" + std::to_string(g.W) + "x" + std::to_string(g.H) + ":" + std::to_string(g.Nc) + "x" + std::to_string(g.Nr) +
+         ":" + std::to_string(g.Lr) + ":" + std::to_string(g.Hh) + ":" + std::to_string(tg.split);
+  }
+  return k;
+}
+
+// Median seconds per evaluation after dropping the plans (the first call rebuilds them).
+int TimeModel(ifg_model* m, const double* params, int nVec, double* seconds) {
+  (void)hipStreamSynchronize(m->stream);
+  m->plans.clear();
+  std::vector<double> out(nVec);
+  int st = Run(m, params, nVec, out.data(), nullptr, nullptr);
+  if (st < 0) return st;
+  std::vector<double> t;
+  for (int r = 0; r < 5; r++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((st = Run(m, params, nVec, out.data(), nullptr, nullptr)) < 0) return st;
+    t.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    if (r == 1 && t[0] + t[1] > 0.05) break;   // long evaluations: two runs are enough
+  }
+  std::sort(t.begin(), t.end());
+  *seconds = t[t.size() / 2];
+  return IFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ifg_autotune(ifg_model* m, const double* params, int n_vectors) {
+  if (!m || !params || n_vectors < 1) return IFG_ERR_ARG;
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  std::vector<double> out(n_vectors);
+  int st = Run(m, params, n_vectors, out.data(), nullptr, nullptr);   // builds the grids
+  if (st < 0) return st;
+  const int which = n_vectors == 1 ? 0 : 1;
+  std::vector<TuneGrid> grids = TuneGrids(m, which);
+  if (grids.empty()) return 0;
+  const std::string key = GeometryKey(m, grids, which, n_vectors);
+  {
+    std::lock_guard<std::mutex> lock(gTuneMu);
+    auto it = gTuned.find(key);
+    if (it != gTuned.end()) {
+      for (size_t i = 0; i < grids.size(); i++) SetShape(m, *grids[i].cg, it->second[i], grids[i].split);
+      (void)hipStreamSynchronize(m->stream);
+      m->plans.clear();
+      return 0;
+    }
+  }
+  const size_t elem = ElemBytes(m);
+  const int maxLds = MaxLds(m);
+  int trials = 0;
+  double best = 0.0;
+  if ((st = TimeModel(m, params, n_vectors, &best)) < 0) return st;
+  trials++;
+  for (TuneGrid& tg : grids) {
+    GridShape cur = GetShape(m, *tg.cg);
+    for (int knob = 0; knob < 2; knob++) {
+      std::vector<GridShape> cand;
+      if (knob == 0)
+        for (int C : {1, 2, 4, 8}) cand.push_back({C, cur.rows1});
+      else
+        for (int r : {cur.rows1 / 2, cur.rows1 * 2}) cand.push_back({cur.C, r});
+      for (const GridShape& s : cand) {
+        if ((s.C == cur.C && s.rows1 == cur.rows1) || !ShapeFits(*tg.cg, s, elem, maxLds, tg.split)) continue;
+        SetShape(m, *tg.cg, s, tg.split);
+        double t = 0.0;
+        if ((st = TimeModel(m, params, n_vectors, &t)) < 0) return st;
+        trials++;
+        if (t < best) {
+          best = t;
+          cur = s;
+        }
+      }
+      SetShape(m, *tg.cg, cur, tg.split);
+    }
+  }
+  (void)hipStreamSynchronize(m->stream);
+  m->plans.clear();
+  std::vector<GridShape> rec;
+  for (const TuneGrid& tg : grids) rec.push_back(GetShape(m, *tg.cg));
+  std::lock_guard<std::mutex> lock(gTuneMu);
+  gTuned[key] = rec;
+  return trials;
+}
+
+int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* cols_per_block, int* rows_per_block) {
+  if (!m || grid < 0) return IFG_ERR_ARG;
+  const int which = batch_grids ? 1 : 0;
+  const ConvGrid* cg = nullptr;
+  if (grid == 0) cg = m->doConv ? &m->mainConv[which] : nullptr;
+  else if (grid - 1 < (int)m->regions.size()) cg = &m->regions[grid - 1]->conv[which];
+  if (!cg || !cg->ready) return IFG_ERR_STATE;
+  const GridShape s = GetShape(m, *cg);
+  if (cols_per_block) *cols_per_block = s.C;
+  if (rows_per_block) *rows_per_block = s.rows1;
+  return IFG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// A shape for every grid of the batch's models from one strip width and one
+// multiplier of each grid's default rows per workgroup (grids where it does not fit
+// keep their defaults).
+void SetBatchShapes(ifg_batch* b, int C, double rowsFactor) {
+  for (ifg_model* m : b->models) {
+    const size_t elem = ElemBytes(m);
+    const int maxLds = MaxLds(m);
+    for (TuneGrid& tg : TuneGrids(m, 0)) {
+      GridShape def;
+      def.C = ColsPerBlock(tg.cg->g.Nr, elem);
+      def.rows1 = RowsPerBlock(tg.cg->g.Nc, elem);
+      GridShape s;
+      s.C = C;
+      s.rows1 = std::max(1, (int)std::lround(def.rows1 * rowsFactor));
+      if (!ShapeFits(*tg.cg, s, elem, maxLds, tg.split)) s = def;
+      SetShape(m, *tg.cg, s, tg.split);
+    }
+    (void)hipStreamSynchronize(m->stream);
+    m->plans.clear();
+  }
+}
+
+int TimeBatch(ifg_batch* b, const double* params, double* seconds) {
+  (void)hipStreamSynchronize(b->stream);
+  b->plans.clear();
+  std::vector<double> out(b->models.size());
+  int st = ifg::BatchEvaluate(b, params, nullptr, out.data(), nullptr, nullptr, nullptr);
+  if (st < 0) return st;
+  std::vector<double> t;
+  for (int r = 0; r < 3; r++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((st = ifg::BatchEvaluate(b, params, nullptr, out.data(), nullptr, nullptr, nullptr)) < 0) return st;
+    t.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  }
+  std::sort(t.begin(), t.end());
+  *seconds = t[1];
+  return IFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ifg_batch_autotune(ifg_batch* b, const double* params, int* cols_per_block, double* rows_factor) {
+  if (!b || !params || b->models.empty()) return IFG_ERR_ARG;
+  std::vector<double> out(b->models.size());
+  int st = ifg::BatchEvaluate(b, params, nullptr, out.data(), nullptr, nullptr, nullptr);   // builds the grids
+  if (st < 0) return st;
+  int bestC = 4, trials = 0;
+  double bestF = 1.0, best = 0.0;
+  SetBatchShapes(b, bestC, bestF);
+  if ((st = TimeBatch(b, params, &best)) < 0) return st;
+  trials++;
+  for (int C : {1, 2, 8}) {
+    SetBatchShapes(b, C, bestF);
+    double t = 0.0;
+    if ((st = TimeBatch(b, params, &t)) < 0) return st;
+    trials++;
+    if (t < best) {
+      best = t;
+      bestC = C;
+    }
+  }
+  for (double f : {0.5, 2.0}) {
+    SetBatchShapes(b, bestC, f);
+    double t = 0.0;
+    if ((st = TimeBatch(b, params, &t)) < 0) return st;
+    trials++;
+    if (t < best) {
+      best = t;
+      bestF = f;
+    }
+  }
+  SetBatchShapes(b, bestC, bestF);
+  (void)hipStreamSynchronize(b->stream);
+  b->plans.clear();
+  if (cols_per_block) *cols_per_block = bestC;
+  if (rows_factor) *rows_factor = bestF;
+  return trials;
 }
 
 }  // extern "C"

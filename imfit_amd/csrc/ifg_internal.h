@@ -11,7 +11,7 @@
 
 namespace ifg {
 
-enum FuncType { F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6 };
+enum FuncType { F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6, F_GENSERSIC = 7 };
 
 // S_JAC: write Jacobian columns dfac * dm/dp_j;  S_GRAD: accumulate sum(gfac * dm/dp_j)
 enum StatMode { S_NONE = 0, S_CHI2 = 1, S_CASH = 2, S_MLR = 3, S_JAC = 4, S_GRAD = 5 };
@@ -25,6 +25,7 @@ struct cpx {
 // C library's cos/sin/pow so that the device sees the reference's exact values.
 //   v[0] x0  v[1] y0  v[2] cosPA  v[3] sinPA  v[4] q
 //   Sersic:      v[5] I_e  v[6] r_e  v[7] bn   v[8] 1/n
+//   Sersic_GenEllipse: as Sersic, and v[11] c0 + 2 (the generalized-ellipse exponent)
 //   Exponential: v[5] I_0  v[6] h
 //   Gaussian:    v[5] I_0  v[6] sigma  v[7] 2 sigma^2
 //   Moffat:      v[5] I_0  v[6] alpha  v[7] beta

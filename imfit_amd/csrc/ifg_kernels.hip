@@ -327,7 +327,8 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 // func_exp.cpp:127-130, func_gaussian.cpp:123-128, func_moffat.cpp:140-147).
 __device__ __forceinline__ double intensity_f64(int type, const double* v, double r) {
   switch (type) {
-    case F_SERSIC: return v[5] * exp(-v[7] * (pow((r / v[6]), v[8]) - 1.0));
+    case F_SERSIC:
+    case F_GENSERSIC: return v[5] * exp(-v[7] * (pow((r / v[6]), v[8]) - 1.0));
     case F_EXP: return v[5] * exp(-r / v[6]);
     case F_GAUSS: {
       double r_squared = r * r;
@@ -345,7 +346,8 @@ __device__ __forceinline__ double intensity_f64(int type, const double* v, doubl
 // FP32 intensity from the squared radius, with hardware exp2/log2 (v_exp_f32, v_log_f32).
 __device__ __forceinline__ float intensity_f32_r2(int type, const double* v, float r2) {
   switch (type) {
-    case F_SERSIC: {
+    case F_SERSIC:
+    case F_GENSERSIC: {
       float p = __builtin_amdgcn_exp2f((float)v[13] * __builtin_amdgcn_logf(r2 * (float)v[12]));
       return (float)v[5] * __builtin_amdgcn_exp2f((float)v[14] * (p - 1.0f));
     }
@@ -375,6 +377,33 @@ __device__ __forceinline__ double ell_radius(const double* v, double x, double y
   double xp = x_diff * v[2] + y_diff * v[3];
   double yp_scaled = (-x_diff * v[3] + y_diff * v[2]) / v[4];
   return sqrt(xp * xp + yp_scaled * yp_scaled);
+}
+
+// Generalized-ellipse radius (|x'|^e + |y'/q|^e)^(1/e), e = c0 + 2
+// (func_gen-sersic.cpp:153-165; 1/e formed as the reference's Setup forms it).
+__device__ __forceinline__ double gen_radius(const double* v, double x, double y) {
+  double x_diff = x - v[0];
+  double y_diff = y - v[1];
+  double xp = fabs(x_diff * v[2] + y_diff * v[3]);
+  double yp_scaled = fabs((-x_diff * v[3] + y_diff * v[2]) / v[4]);
+  double powerSum = pow(xp, v[11]) + pow(yp_scaled, v[11]);
+  return pow(powerSum, 1.0 / v[11]);
+}
+
+// Radius of a component that depends on one (elliptical or generalized) radius.
+__device__ __forceinline__ double comp_radius(int type, const double* v, double x, double y) {
+  return type == F_GENSERSIC ? gen_radius(v, x, y) : ell_radius(v, x, y);
+}
+
+// Squared radius from FP32 frame coordinates (xp, y'/q): the ellipse's xp^2 + yp^2,
+// or the generalized ellipse's radius squared with hardware exp2/log2.
+__device__ __forceinline__ float radius2_f32(int type, const double* v, float xp, float yp) {
+  if (type != F_GENSERSIC) return xp * xp + yp * yp;
+  const float e = (float)v[11], inve = 1.0f / e;
+  const float a = __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(fabsf(xp)));
+  const float b = __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(fabsf(yp)));
+  const float r = __builtin_amdgcn_exp2f(inve * __builtin_amdgcn_logf(a + b));
+  return r * r;
 }
 
 // ------------------------------------------------------------------ ExponentialDisk3D
@@ -593,7 +622,7 @@ template <typename T>
 __device__ __forceinline__ T comp_point_2d(int type, const double* v, double x, double y);
 template <>
 __device__ __forceinline__ double comp_point_2d<double>(int type, const double* v, double x, double y) {
-  return intensity_f64(type, v, ell_radius(v, x, y));
+  return intensity_f64(type, v, comp_radius(type, v, x, y));
 }
 template <>
 __device__ __forceinline__ float comp_point_2d<float>(int type, const double* v, double x, double y) {
@@ -601,7 +630,7 @@ __device__ __forceinline__ float comp_point_2d<float>(int type, const double* v,
   float c = (float)v[2], s = (float)v[3];
   float xp = xd * c + yd * s;
   float yp = (yd * c - xd * s) * (float)v[15];
-  return intensity_f32_r2(type, v, xp * xp + yp * yp);
+  return intensity_f32_r2(type, v, radius2_f32(type, v, xp, yp));
 }
 
 // Point value without sub-pixel integration (bulk pixels).  DISK: the model has an
@@ -619,8 +648,8 @@ __device__ __forceinline__ int n_subsamples(int type, const double* v, double r,
   if (!on || !(r < 10.0)) return 1;
   double size = v[6];
   double small = (type == F_GAUSS || type == F_MOFFAT) ? 40.0 : 20.0;
-  double rlim = (type == F_SERSIC) ? 4.0 : 3.0;
-  if (type != F_SERSIC && type != F_EXP && type != F_GAUSS && type != F_MOFFAT) return 1;
+  double rlim = (type == F_SERSIC || type == F_GENSERSIC) ? 4.0 : 3.0;
+  if (type != F_SERSIC && type != F_EXP && type != F_GAUSS && type != F_MOFFAT && type != F_GENSERSIC) return 1;
   if ((size <= 1.0) && (r <= 1.0)) return min(100, (int)(small / size));
   if (r <= rlim) return 20;
   return min(100, (int)(20 / r));
@@ -639,9 +668,50 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Moffat: PA ell I_0 fwhm beta; FlatSky: I_sky; ExponentialDisk3D: PA inc J_0 h n
 // z_0, see disk_grad).  Geometry enters through the
 // elliptical radius r(x0, y0, PA, ell); at r = 0 the radial terms are taken as 0.
+// Analytic derivative of the generalized-ellipse Sersic (func_gen-sersic.cpp:112-165)
+// at (x, y): I = I_e exp(-bn (u - 1)), u = (R/r_e)^(1/n), R = S^(1/e),
+// S = |X|^e + |Y|^e, e = c0 + 2, X = dx cos + dy sin, Y = (-dx sin + dy cos)/q, so
+// dR/dX = (R/S) |X|^(e-1) sgn X (and for Y), dR/de = R ((|X|^e ln|X| + |Y|^e ln|Y|)/(S e)
+// - ln S / e^2).  Slots: 0 x0, 1 y0, 2 PA, 3 ell, 4 c0, 5 n, 6 I_e, 7 r_e.
+__device__ double gensersic_grad(const double* v, double x, double y, int slot) {
+  const double DEG2RAD = 0.017453292519943295;
+  const double c = v[2], s = v[3], q = v[4], e = v[11];
+  const double dx = x - v[0], dy = y - v[1];
+  const double X = dx * c + dy * s, Y = (-dx * s + dy * c) / q;
+  const double ax = fabs(X), ay = fabs(Y);
+  const double A = pow(ax, e), B = pow(ay, e), S = A + B;
+  const double R = pow(S, 1.0 / e);
+  const double Ie = v[5], re = v[6], bn = v[7], invn = v[8];
+  const double u = pow(R / re, invn);
+  const double ex = exp(-bn * (u - 1.0));
+  const double I = Ie * ex;
+  if (slot == 5) {   // n: bn(n) and u(n)
+    const double dudn = (R > 0.0) ? -u * log(R / re) * invn * invn : 0.0;
+    return I * (-v[10] * (u - 1.0) - bn * dudn);
+  }
+  if (slot == 6) return ex;
+  if (slot == 7) return I * bn * u * invn / re;
+  if (!(R > 0.0)) return 0.0;
+  const double dIdR = -I * bn * u * invn / R;
+  if (slot == 4) {
+    const double lx = ax > 0.0 ? A * log(ax) : 0.0, ly = ay > 0.0 ? B * log(ay) : 0.0;
+    return dIdR * R * ((lx + ly) / (S * e) - log(S) / (e * e));
+  }
+  const double RS = R / S;
+  const double gX = ax > 0.0 ? RS * pow(ax, e - 1.0) * (X > 0.0 ? 1.0 : -1.0) : 0.0;
+  const double gY = ay > 0.0 ? RS * pow(ay, e - 1.0) * (Y > 0.0 ? 1.0 : -1.0) : 0.0;
+  double drdp;
+  if (slot == 0) drdp = -gX * c + gY * s / q;
+  else if (slot == 1) drdp = -gX * s - gY * c / q;
+  else if (slot == 2) drdp = DEG2RAD * (gX * q * Y - gY * X / q);
+  else drdp = gY * Y / q;
+  return dIdR * drdp;
+}
+
 template <bool DISK>
 __device__ double comp_grad(int type, const double* v, double x, double y, int slot) {
   if (type == F_FLATSKY) return slot == 2 ? 1.0 : 0.0;
+  if (type == F_GENSERSIC) return gensersic_grad(v, x, y, slot);
   if (DISK && type == F_EXPDISK3D) return disk_grad(v, x, y, slot, 3);
   const double DEG2RAD = 0.017453292519943295;
   const double c = v[2], s = v[3], q = v[4];
@@ -836,7 +906,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
       const int type = ea.types[n];
       const CompRec rv = rec[n];   // into registers: read once, not per sample
       const double* v = rv.v;
-      const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, ell_radius(v, x, y), subsampling);
+      const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
       if (nSub > 1) {
         const double deltaSubpix = 1.0 / nSub;
         const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
@@ -864,7 +934,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
     if (type == F_FLATSKY) {
       val = v[5];
     } else {
-      const double rad = ell_radius(v, x, y);
+      const double rad = comp_radius(type, v, x, y);
       const int nSub = n_subsamples(type, v, rad, subsampling);
       if (nSub > 1) {
         const double deltaSubpix = 1.0 / nSub;
@@ -877,7 +947,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
             const int ii = s / nSub, jj = s - ii * nSub;
             const double x_ii = x_sub_start + ii * deltaSubpix;
             const double y_ii = y_sub_start + jj * deltaSubpix;
-            part += (double)intensity_r<T>(type, v, ell_radius(v, x_ii, y_ii));
+            part += (double)intensity_r<T>(type, v, comp_radius(type, v, x_ii, y_ii));
           }
         } else {   // FP32 pixels: sample offsets in FP32 from the FP64 start of the pixel's grid
           const float bx = (float)(x_sub_start - v[0]), by = (float)(y_sub_start - v[1]);
@@ -887,7 +957,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
             const int ii = (int)(((float)s + 0.5f) * rn), jj = s - ii * nSub;
             const float xd = bx + (float)ii * dl, yd = by + (float)jj * dl;
             const float xp = xd * c + yd * sn, yp = (yd * c - xd * sn) * iq;
-            part += (double)intensity_f32_r2(type, v, xp * xp + yp * yp);
+            part += (double)intensity_f32_r2(type, v, radius2_f32(type, v, xp, yp));
           }
         }
         val = wave_sum(part) / (nSub * nSub);

@@ -65,7 +65,7 @@ int ifg_set_precision(ifg_model* m, int precision);
 /* ---- model definition ----------------------------------------------------- */
 /* AddFunction + DefineFunctionSets (core/model_object.cpp:277-340; name -> class
  * as in core/add_functions.cpp:163-385).  Supported names: "Sersic",
- * "Exponential", "FlatSky", "Gaussian", "Moffat", "ExponentialDisk3D". */
+ * "Exponential", "FlatSky", "Gaussian", "Moffat", "ExponentialDisk3D", "Sersic_GenEllipse". */
 int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set);
 /* FunctionObject::SetSubsampling for every component (imfit --no-subsampling) */
 int ifg_set_subsampling(ifg_model* m, int do_subsampling);
@@ -227,6 +227,26 @@ int ifg_set_graph_mode(ifg_model* m, int on);
  * `index` and returns the number of entries (0 when index is out of range). */
 int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_len, double* total_ms,
                           long* launches);
+
+/* ---- launch-shape autotuning (no counterpart in the reference; the FFT convolver
+ * and the per-pixel loops of core/model_object.cpp have no launch shapes).
+ * Each convolution grid (main image, oversampled regions) has two launch shapes: the
+ * column pass's strip width (columns per workgroup: 1, 2, 4 or 8) and the row passes'
+ * rows per workgroup.  ifg_autotune times this model's GetFitStatistic of n_vectors
+ * vectors at params under nearby shapes (one grid and one shape at a time), keeps the
+ * fastest, and records the choice for the geometry in this process: a later call for
+ * a model with the same grids applies the record without timing.  Returns the number
+ * of timed trials (0: taken from the record) or an error.  Results depend on the
+ * shapes only through the grouping of the statistic's reduction (last bits). */
+int ifg_autotune(ifg_model* m, const double* params, int n_vectors);
+/* The current shapes of grid `grid` (0: main image, 1..: oversampled regions) on the
+ * single-evaluation grids (batch_grids 0) or the population grids (1). */
+int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* cols_per_block,
+                         int* rows_per_block);
+/* As ifg_autotune for a batch's ifg_batch_fit_statistic: one strip width and one
+ * multiplier of each grid's default rows per workgroup for every model (grids where
+ * the choice does not fit keep their defaults).  Returns the number of trials. */
+int ifg_batch_autotune(ifg_batch* b, const double* params, int* cols_per_block, double* rows_factor);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,9 @@
 //   * GetValue of Sersic/Exponential/Gaussian/Moffat/FlatSky: unit_tests/unittest_funcs.t.h
 //     known answers; Calculate_bn against the reference's own helper_funcs.cpp built
 //     into oracle/_ref.
-//   * model images: tests/sersic+exp_orig.fits, tests/biggertest_orig.fits (float32).
+//   * model images: tests/sersic+exp_orig.fits, tests/biggertest_orig.fits (float32);
+//     Sersic_GenEllipse: tests/gensersictest_orig.fits and, convolved with
+//     tests/psf_moffat_35_n4699z.fits, tests/gensersictest612_conv_cutout512.fits.
 //   * chi^2 / Cash: imfit_textout1, imfit_textout3, imfit_textout4c, imfit_textout4d.
 //   * convolution: tests/convolve_out_n5419.fits, exact direct convolution, numpy.fft.
 //   * oversampled regions: tests/oversampled_orig.fits, tests/oversampled2both_orig.fits;
@@ -132,6 +134,53 @@ struct Sersic : Func {   // function_objects/func_sersic.cpp:109-213
   double CalculateIntensity(double r) const { return I_e * exp(-bn * (pow((r / r_e), invn) - 1.0)); }
   int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, r_e, 2 * SUBSAMPLE_R, 4.0); }
   double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+// Generalized-ellipse Sersic (function_objects/func_gen-sersic.cpp:112-246): the
+// Sersic profile of the "generalized ellipse" radius
+// r = (|x'|^(c0+2) + |y'/q|^(c0+2))^(1/(c0+2)) (:153-165), parameters
+// PA, ell, c0, n, I_e, r_e (:121-127), Sersic's sub-pixel rule (:234-245).
+struct GenSersic : Func {
+  double PA, ell, c0, n, I_e, r_e, q, PA_rad, cosPA, sinPA, ellExp, invEllExp, bn, invn;
+  int NParams() const override { return 6; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; ell = p[o + 1]; c0 = p[o + 2]; n = p[o + 3]; I_e = p[o + 4]; r_e = p[o + 5];
+    q = 1.0 - ell;
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    ellExp = c0 + 2.0;
+    invEllExp = 1.0 / ellExp;
+    bn = Calculate_bn(n);
+    invn = 1.0 / n;
+  }
+  double Radius(double dx, double dy) const {
+    double xp = fabs(dx * cosPA + dy * sinPA);
+    double yp_scaled = fabs((-dx * sinPA + dy * cosPA) / q);
+    double powerSum = pow(xp, ellExp) + pow(yp_scaled, ellExp);
+    return pow(powerSum, invEllExp);
+  }
+  double CalculateIntensity(double r) const { return I_e * exp(-bn * (pow((r / r_e), invn) - 1.0)); }
+  int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, r_e, 2 * SUBSAMPLE_R, 4.0); }
+  double GetValue(double x, double y) override {
+    double r = Radius(x - x0, y - y0);
+    int nSub = CalculateSubsamples(r);
+    if (nSub > 1) {
+      double deltaSubpix = 1.0 / nSub;
+      double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+      double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+      double theSum = 0.0;
+      for (int ii = 0; ii < nSub; ii++) {
+        double x_ii = x_sub_start + ii * deltaSubpix;
+        for (int jj = 0; jj < nSub; jj++) {
+          double y_ii = y_sub_start + jj * deltaSubpix;
+          theSum += CalculateIntensity(Radius(x_ii - x0, y_ii - y0));
+        }
+      }
+      return theSum / (nSub * nSub);
+    }
+    return CalculateIntensity(r);
+  }
 };
 
 struct Exponential : Func {   // function_objects/func_exp.cpp:104-197
@@ -602,6 +651,7 @@ struct ExpDisk3D : Func {
 Func* MakeFunc(const char* name) {   // core/add_functions.cpp:163-385 (name -> class)
   std::string s(name);
   if (s == "Sersic") return new Sersic();
+  if (s == "Sersic_GenEllipse") return new GenSersic();
   if (s == "Exponential") return new Exponential();
   if (s == "Gaussian") return new Gaussian();
   if (s == "Moffat") return new Moffat();

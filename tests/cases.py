@@ -101,6 +101,16 @@ def biggertest64():
     return Case(funcs, p, ncols=64, nrows=64), golden("biggertest64")["image"]
 
 
+def gensersic512(conv=False):
+    """do_makeimage_tests:49,55 (tests/makeimage_reference/config_makeimage_gensersic512.dat):
+    one Sersic_GenEllipse at (256, 256), PA 30, ell 0.5, c0 -1, n 2.5, I_e 100, r_e 50 on
+    a 512^2 image; convolved with tests/psf_moffat_35_n4699z.fits for the second golden."""
+    d = golden("gensersic512")
+    case = Case([("Sersic_GenEllipse", True)], [256, 256, 30, 0.5, -1.0, 2.5, 100, 50], ncols=512, nrows=512,
+                psf=d["psf"] if conv else None)
+    return case, (d["conv"] if conv else d["image"])
+
+
 def oversampled(two=False):
     d = golden("oversampled")
     funcs = [("Gaussian", True)]

@@ -14,6 +14,10 @@ Each fixture records which reference test produced it:
   n3073          do_imfit_tests:96-99 -> imfit_textout4, imfit_textout4c, imfit_textout4d
   sersic_exp512  do_makeimage_tests:51,155 -> tests/sersic+exp_orig.fits
   biggertest64   do_makeimage_tests:48 -> tests/biggertest_orig.fits
+  gensersic512   do_makeimage_tests:49,55,129-178 -> tests/gensersictest_orig.fits,
+                 tests/gensersictest612_conv_cutout512.fits (config
+                 tests/makeimage_reference/config_makeimage_gensersic512.dat; PSF
+                 tests/psf_moffat_35_n4699z.fits)
   oversampled    do_makeimage_tests:68,70,199,215 -> tests/oversampled_orig.fits,
                  tests/oversampled2both_orig.fits
   conv_n5419     do_psfconvolve_tests:11,60 -> tests/convolve_out_n5419.fits
@@ -121,6 +125,12 @@ def main():
                         image=read_fits(t("sersic+exp_orig.fits")))
     np.savez_compressed(os.path.join(OUT, "biggertest64.npz"),
                         image=read_fits(t("biggertest_orig.fits")))
+    np.savez_compressed(   # float32 as stored: the files are BITPIX -32
+        os.path.join(OUT, "gensersic512.npz"),
+        image=read_fits(t("gensersictest_orig.fits")).astype(np.float32),
+        conv=read_fits(t("gensersictest612_conv_cutout512.fits")).astype(np.float32),
+        psf=read_fits(t("psf_moffat_35_n4699z.fits")),
+    )
     np.savez_compressed(
         os.path.join(OUT, "oversampled.npz"),
         image=read_fits(t("oversampled_orig.fits")),

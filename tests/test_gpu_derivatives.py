@@ -62,13 +62,33 @@ def osamp_fit_case():
     return c
 
 
-@pytest.mark.parametrize("which", ["n3073", "textout3", "osamp"])
+def gensersic_fit_case(c0):
+    """A Sersic_GenEllipse fit on 80x72 pixels with a 13^2 Gaussian PSF and a sky:
+    data = the oracle model at shifted parameters (boxy c0 > 0, disky c0 < 0)."""
+    funcs = [("Sersic_GenEllipse", True), ("FlatSky", False)]
+    p = np.array([40.3, 35.6, 25.0, 0.35, c0, 2.2, 30.0, 9.0, 5.0])
+    psf = cases.gaussian_psf(13, 1.5)
+    base = cases.Case(funcs, p, ncols=80, nrows=72, psf=psf)
+    o = base.build(oracle.OracleModel(threads=8))
+    truth = p.copy()
+    truth[0] += 0.4
+    truth[7] *= 1.1
+    o.CreateModelImage(truth)
+    data = o.GetModelImageVector()
+    return cases.Case(funcs, p, data=data, psf=psf, chars=(4.0, 1.0, 1.0, 1, 0.0))
+
+
+@pytest.mark.parametrize("which", ["n3073", "textout3", "osamp", "gensersic_boxy", "gensersic_disky"])
 def test_jacobian_vs_central_differences(which):
     announce("jacobian %s" % which)
     if which == "n3073":
         case, _ = cases.n3073()
     elif which == "textout3":
         case, _ = cases.textout3()
+    elif which == "gensersic_boxy":
+        case = gensersic_fit_case(0.6)
+    elif which == "gensersic_disky":
+        case = gensersic_fit_case(-0.5)
     else:
         case = osamp_fit_case()
     p = case.params.astype(float)

@@ -79,6 +79,26 @@ def test_model_image_no_psf(mk):
     assert cases.float32_ulps(img, golden) <= 1
 
 
+@pytest.mark.parametrize("conv", [False, True])
+def test_gensersic_goldens(conv):
+    """Sersic_GenEllipse on the GPU: the oracle within 1e-12 and the reference's
+    gensersictest goldens within one float32 ulp; FP32 mode within 5e-6 of the
+    image maximum."""
+    announce("gensersic conv=%s" % conv)
+    case, golden = cases.gensersic512(conv)
+    g = case.build(gpu_model())
+    g.CreateModelImage(case.params)
+    img = g.GetModelImageVector()
+    o = case.build(oracle.OracleModel(threads=8))
+    o.CreateModelImage(case.params)
+    ref = o.GetModelImageVector()
+    assert_images_close(img, ref, conv=conv)
+    assert cases.float32_ulps(img, golden) <= 1
+    g32 = case.build(gpu_model(1))
+    g32.CreateModelImage(case.params)
+    assert np.abs(g32.GetModelImageVector() - ref).max() <= 5e-6 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("two", [False, True])
 def test_oversampled_regions(two):
     announce("oversampled two=%s" % two)

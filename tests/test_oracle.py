@@ -152,6 +152,34 @@ def test_model_image_goldens(mk):
     assert cases.float32_ulps(m.GetModelImageVector(), ref) == 0
 
 
+@pytest.mark.parametrize("conv", [False, True])
+def test_gensersic_goldens(conv):
+    """Sersic_GenEllipse (boxy, c0 = -1): tests/gensersictest_orig.fits exactly, and the
+    PSF-convolved tests/gensersictest612_conv_cutout512.fits within one float32 ulp (the
+    reference convolves with FFTW, the oracle with its own FFT)."""
+    case, ref = cases.gensersic512(conv)
+    m = case.build(oracle.OracleModel(threads=8))
+    m.CreateModelImage(case.params)
+    img = m.GetModelImageVector()
+    assert cases.float32_ulps(img, ref) <= (1 if conv else 0)
+
+
+def test_gensersic_reduces_to_sersic():
+    """c0 = 0 is the plain ellipse: the generalized radius (|x'|^2 + |y'/q|^2)^(1/2) and
+    the Sersic profile with the same sub-pixel rule give Sersic's image (to rounding of
+    pow versus sqrt)."""
+    a = oracle.OracleModel(threads=8)
+    a.AddFunction("Sersic_GenEllipse", True)
+    a.SetupModelImage(64, 48)
+    b = oracle.OracleModel(threads=8)
+    b.AddFunction("Sersic", True)
+    b.SetupModelImage(64, 48)
+    a.CreateModelImage(np.array([30.3, 22.7, 25.0, 0.3, 0.0, 1.7, 10.0, 6.0]))
+    b.CreateModelImage(np.array([30.3, 22.7, 25.0, 0.3, 1.7, 10.0, 6.0]))
+    ia, ib = a.GetModelImageVector(), b.GetModelImageVector()
+    assert np.allclose(ia, ib, rtol=1e-13, atol=0)
+
+
 @pytest.mark.parametrize("two", [False, True])
 def test_oversampled_goldens(two):
     # do_makeimage_tests:190-199 compares only pixels > 1e-7 (FFT noise floor)
