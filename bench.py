@@ -124,6 +124,20 @@ def profiled_traffic(kernel, prec):
     return None
 
 
+def profiled_valu(kernel, prec):
+    """VALU wave instructions per dispatch of `kernel` from profiles/valu_cfg2_1024.md
+    (rocprofv3 SQ_INSTS_VALU of this workload), or None."""
+    path = os.path.join(ROOT, "profiles", "valu_cfg2_1024.md")
+    if not os.path.exists(path):
+        return None
+    names = ("ifg::%s<%s>" % (kernel, prec), "ifg::%s<%s,false,false>" % (kernel, prec))
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "valu_insts" and f[1] in names:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(w, seconds):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -268,6 +282,8 @@ def main():
     ap.add_argument("--lm-repeats", type=int, default=3)
     ap.add_argument("--copy-calibration-mb", type=int, default=0)
     ap.add_argument("--no-aux", action="store_true", help="skip the cfg3 / cfg4 side measurements")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (profiling runs: no tuning trials in the trace)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; sharding the path over several GPUs is a design "
@@ -282,13 +298,13 @@ def main():
     prec = imfit_amd.PRECISION_FP32 if args.precision == "fp32" else imfit_amd.PRECISION_FP64
     m = w.fit_model(prec)
     p = w.start_params()
-    trials = m.Autotune(p)   # launch shapes for this geometry (untimed, once per geometry)
-    shapes = {"main_cols_rows": list(m.GetLaunchShape(0)), "region_cols_rows": list(m.GetLaunchShape(1)),
-              "autotune_trials": trials}
-    log("autotuned launch shapes (columns per workgroup, rows per workgroup): %s" % shapes)
+    trials = 0 if args.no_autotune else m.Autotune(p)   # launch shapes for this geometry (untimed, once)
     for _ in range(args.warmup):
         m.GetFitStatistic(p)
     device_sync()
+    shapes = {"main_cols_rows": list(m.GetLaunchShape(0)), "region_cols_rows": list(m.GetLaunchShape(1)),
+              "autotune_trials": trials}
+    log("launch shapes (columns per workgroup, rows per workgroup): %s" % shapes)
     # timed region: K evaluations, each replaying the captured graph of the call
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -317,7 +333,8 @@ def main():
 
     # FP64 parity-mode rate, reported alongside
     m64 = w.fit_model(imfit_amd.PRECISION_FP64)
-    m64.Autotune(p)
+    if not args.no_autotune:
+        m64.Autotune(p)
     for _ in range(3):
         m64.GetFitStatistic(p)
     device_sync()
@@ -341,9 +358,15 @@ def main():
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
     traffic = profiled_traffic(dom, "float" if args.precision == "fp32" else "double")
     ach = km[dom]["bytes"] / avg_s / 1e9 if dom in km else None
+    # VALU issue time of the same kernel (4 cycles per wave64 instruction on one of the
+    # 1024 SIMDs at 2.4 GHz) as a fraction of its duration: what share of the kernel the
+    # vector ALUs are busy issuing
+    valu = profiled_valu(dom, "float" if args.precision == "fp32" else "double")
+    valu_frac = 4.0 * valu / 1024 / 2.4e9 / avg_s if valu is not None else None
     roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS if ach is not None else None, "traffic": traffic,
-            "kernel": dom, "avg_us": avg_s * 1e6, "alg_bytes": km.get(dom, {}).get("bytes")}
+            "kernel": dom, "avg_us": avg_s * 1e6, "alg_bytes": km.get(dom, {}).get("bytes"),
+            "valu_issue_frac": valu_frac}
 
     # LM fits/s on the same workload: mpfit (MINPACK LMDIF) with a finite-difference
     # Jacobian -- one GPU ComputeDeviates per free parameter, QR on the host, as the
