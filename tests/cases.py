@@ -88,6 +88,47 @@ def n3073(stat="chi2"):
     return c, float(d["cash"] if stat == "cash" else d["chi2"])
 
 
+def ic3478_c(errors=None):
+    """tests/imfit_reference/imfit_config_ic3478_64x64c.dat (n fixed at 2.4; gain 4.725,
+    read noise 4.3, original sky 130.1) on tests/ic3478rss_64x64.fits, with errors from
+    the data (imfit_textout3d2), a sigma image (3e) or a variance image (3e2)."""
+    d, e = golden("ic3478_64"), golden("ic3478_64_errors")
+    kw = {}
+    if errors == "sigma":
+        kw = dict(errors=e["sigma"], err_type=100)
+    elif errors == "variance":
+        kw = dict(errors=e["variance"], err_type=110)
+    return Case([("Sersic", True)], [32, 32, 18, 0.2, 2.4, 15, 25], data=d["data"],
+                chars=(4.725, 4.3, 1.0, 1, 130.1), **kw)
+
+
+def n3073_cps():
+    """imfit_textout4b: the n3073 fit on the counts-per-second image with exposure
+    time 53.907456 s and original sky 2.8628692847238053 (do_imfit_tests:61)."""
+    d, c = golden("n3073"), golden("n3073_cps")
+    return Case([("Sersic", True), ("Exponential", False)], [101, 97, 110, 0.2, 1.5, 150, 15, 55, 0.2, 70, 45],
+                data=c["data"], psf=d["psf"], mask=d["mask"], chars=(4.725, 4.3, 53.907456, 1, 2.8628692847238053))
+
+
+def poisson_mlr():
+    """imfit_textout5c: tests/imfit_reference/config_imfit_poisson.dat (Exponential +
+    fixed FlatSky 20) on tests/testimage_poisson_lowsn20.fits, Poisson MLR statistic."""
+    d = golden("poisson_lowsn20")
+    return Case([("Exponential", True), ("FlatSky", False)], [74, 74, 5, 0.4, 90, 15, 20], data=d["data"],
+                stat="mlr")
+
+
+def twogauss_osamp():
+    """imfit_textout4e3: tests/imfit_reference/config_imfit_2gauss_small.dat (gain 1000)
+    on tests/twogaussian_psf+2osamp_noisy.fits, PSF tests/psf_moffat_35.fits, regions
+    35:45,35:45 and 10:20,5:15 oversampled 3x with tests/psf_moffat_35_oversamp3.fits."""
+    d = golden("twogauss_osamp")
+    osamp = [(d["psf_osamp"], 3, 35, 45, 35, 45), (d["psf_osamp"], 3, 10, 20, 5, 15)]
+    return Case([("Gaussian", True), ("Gaussian", True), ("FlatSky", False)],
+                [40, 40, 0, 0, 105, 1.5, 15, 10, 0, 0, 95, 1.5, 100], data=d["data"], psf=d["psf"], osamp=osamp,
+                chars=(1000.0, 0.0, 1.0, 1, 0.0))
+
+
 def sersic_exp512():
     return Case([("Sersic", True), ("Exponential", False)], [256, 256, 30, 0.5, 2.5, 100, 50, 30, 0.75, 10, 150],
                 ncols=512, nrows=512), golden("sersic_exp512")["image"]
@@ -140,6 +181,18 @@ LIMITS = {
     # tests/imfit_reference/imfit_config_n3073.dat
     "n3073": [(90, 110), (90, 110), (0, 180), (0, 1), (0, 5), (0, 500), (0, 100), (0, 90), (0, 1), (0, 600),
               (0, 1000)],
+    # tests/imfit_reference/imfit_config_ic3478_64x64c.dat (n fixed)
+    "ic3478_c": [(1, 64), (1, 64), (0, 90), (0, 1), None, (0, 500), (0, 100)],
+    # tests/imfit_reference/config_imfit_poisson.dat (sky fixed)
+    "poisson": [(72, 78), (72, 78), (0, 180), (0, 0.9), (1, 200), (1, 50), None],
+    # tests/imfit_reference/config_imfit_2gauss_small.dat (PA, ell, sky fixed)
+    "twogauss": [(35, 45), (35, 45), None, None, (20, 150), (0.5, 2.5), (10, 20), (5, 15), None, None, (20, 150),
+                 (0.5, 2.5), None],
+}
+FIXED = {
+    "ic3478_c": [0, 0, 0, 0, 1, 0, 0],
+    "poisson": [0, 0, 0, 0, 0, 0, 1],
+    "twogauss": [0, 0, 1, 1, 0, 0, 0, 0, 1, 1, 0, 0, 1],
 }
 
 

@@ -20,6 +20,10 @@ Each fixture records which reference test produced it:
                  tests/psf_moffat_35_n4699z.fits)
   oversampled    do_makeimage_tests:68,70,199,215 -> tests/oversampled_orig.fits,
                  tests/oversampled2both_orig.fits
+  ic3478_64_errors  do_imfit_tests:56-58 -> imfit_textout3d2, imfit_textout3e, imfit_textout3e2
+  n3073_cps      do_imfit_tests:61 -> imfit_textout4b
+  poisson_lowsn20  do_imfit_tests:112 -> tests/imfit_reference/imfit_textout5c_tail
+  twogauss_osamp do_imfit_tests:107 -> imfit_textout4e3
   conv_n5419     do_psfconvolve_tests:11,60 -> tests/convolve_out_n5419.fits
   conv_5x5       do_psfconvolve_tests:8-9 -> tests/psfconvolve_out{1,2}.txt (padded PSF)
 """
@@ -84,6 +88,26 @@ def t(p):
     return os.path.join(REF, "tests", p)
 
 
+def fit_record(path, stats=True):
+    """The archived outcome of an imfit LM fit printout: initial and final statistic,
+    NITER, NFEV (when printed) and the best-fit parameter values in parameter order
+    (the "NAME  value # +/- err" lines)."""
+    with open(path) as f:
+        txt = f.read()
+    best = [float(v) for v in re.findall(r"^\w+\s+(-?[0-9.]+(?:e[-+]?\d+)?) # \+/-", txt, re.M)]
+    rec = {"best": np.array(best)}
+    if stats:
+        rec["initial"] = grab(path, r"INITIAL CHI\^2 = ([0-9.]+)")
+        rec["final"] = grab(path, r"CHI-SQUARE = ([0-9.]+)")
+        rec["niter"] = grab(path, r"NITER = (\d+)", int)
+        rec["nfev"] = grab(path, r"NFEV = (\d+)", int)
+    return rec
+
+
+def flat(prefix, rec):
+    return {prefix + "_" + k: v for k, v in rec.items()}
+
+
 def main():
     lin = "imfit_reference/linux_x86_64/"
     # --- textout1: 32x32 Exponential, user noise image, no subsampling
@@ -119,6 +143,32 @@ def main():
         lm_nfev=grab(t(lin + "imfit_textout4"), r"NFEV = (\d+)", int),
         best=np.array([100.7073, 97.0877, 111.919, 0.184262, 3.2805, 78.2972, 26.7976,
                        51.724, 0.35226, 39.7753, 53.6058]),
+    )
+    # --- further LM fits of do_imfit_tests (archived printouts: outcome and counts)
+    np.savez_compressed(   # do_imfit_tests:56-58 (config c: n fixed; data, sigma, variance errors)
+        os.path.join(OUT, "ic3478_64_errors.npz"),
+        sigma=read_fits(t("ic3478rss_64x64_sigma.fits")),
+        variance=read_fits(t("ic3478rss_64x64_variance.fits")),
+        **flat("t3d2", fit_record(t(lin + "imfit_textout3d2"))),
+        **flat("t3e", fit_record(t(lin + "imfit_textout3e"))),
+        **flat("t3e2", fit_record(t(lin + "imfit_textout3e2"))),
+    )
+    np.savez_compressed(   # do_imfit_tests:61 (counts per second, exposure time and sky)
+        os.path.join(OUT, "n3073_cps.npz"),
+        data=read_fits(t("n3073rss_small_cps.fits")),
+        **flat("t4b", fit_record(t(lin + "imfit_textout4b"))),
+    )
+    np.savez_compressed(   # do_imfit_tests:112 (Poisson MLR with L-M; tail printout only)
+        os.path.join(OUT, "poisson_lowsn20.npz"),
+        data=read_fits(t("testimage_poisson_lowsn20.fits")),
+        **flat("t5c", fit_record(t("imfit_reference/imfit_textout5c_tail"), stats=False)),
+    )
+    np.savez_compressed(   # do_imfit_tests:107 (two Gaussians, two oversampled regions, L-M)
+        os.path.join(OUT, "twogauss_osamp.npz"),
+        data=read_fits(t("twogaussian_psf+2osamp_noisy.fits")),
+        psf=read_fits(t("psf_moffat_35.fits")),
+        psf_osamp=read_fits(t("psf_moffat_35_oversamp3.fits")),
+        **flat("t4e3", fit_record(t(lin + "imfit_textout4e3"))),
     )
     # --- makeimage goldens (float32 images written by SaveVectorAsImage)
     np.savez_compressed(os.path.join(OUT, "sersic_exp512.npz"),

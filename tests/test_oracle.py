@@ -152,6 +152,23 @@ def test_model_image_goldens(mk):
     assert cases.float32_ulps(m.GetModelImageVector(), ref) == 0
 
 
+@pytest.mark.parametrize("which", ["t3d2", "t3e", "t3e2", "t4b"])
+def test_initial_statistics_of_archived_fits(which):
+    """INITIAL CHI^2 of imfit_textout3d2 / 3e / 3e2 (errors from the data, a sigma image,
+    a variance image) and 4b (counts per second, exposure time and sky), printed with
+    6 decimals.  (4e3's printed initial value, two oversampled regions, is 3.8e-7
+    relative above the oracle's; its fit still reproduces the archived counts and
+    final chi^2 -- tests/test_gpu_lm.py.)"""
+    if which.startswith("t3"):
+        case = cases.ic3478_c({"t3d2": None, "t3e": "sigma", "t3e2": "variance"}[which])
+        d = cases.golden("ic3478_64_errors")
+    else:
+        case, d = cases.n3073_cps(), cases.golden("n3073_cps")
+    m = case.build(oracle.OracleModel(threads=8))
+    ref = float(d[which + "_initial"])
+    assert abs(m.GetFitStatistic(case.params) - ref) <= 5e-7 + 1e-12 * ref
+
+
 @pytest.mark.parametrize("conv", [False, True])
 def test_gensersic_goldens(conv):
     """Sersic_GenEllipse (boxy, c0 = -1): tests/gensersictest_orig.fits exactly, and the
