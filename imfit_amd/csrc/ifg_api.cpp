@@ -1073,9 +1073,10 @@ int StageCall(ifg_model* m, Plan& P, const double* params, std::vector<int>& nee
     used[gi] = need[gi];
     grow = grow || need[gi] > P.strideMax[gi];
   }
-  // tail work per chunk and task: its vectors (planes) times the grid's box pixels
+  // tail work per chunk and task: its vectors times the grid's box pixels (one wave
+  // each); derivative planes share one wave per box pixel (every plane of the chunk)
   for (int c = 0; c < P.nChunks; c++) {
-    const int nv = std::min(P.chunkVecs, P.nVec - c * P.chunkVecs);
+    const int nv = IsDeriv(P.flags) ? 1 : std::min(P.chunkVecs, P.nVec - c * P.chunkVecs);
     long* pf = prefix + (size_t)c * (P.nTasks + 1);
     pf[0] = 0;
     for (int t = 0; t < P.nTasks; t++) pf[t + 1] = pf[t] + (P.strideMax[t] > 0 ? (long)nv * used[t] : 0);
@@ -2022,8 +2023,8 @@ int StageBatch(ifg_batch* b, BatchPlan& P, const double* params, const int* acti
           P.minStride[k][gi] = std::max(P.minStride[k][gi], used[t]);
         }
       }
-      prefix[t] = acc;   // tail work: the task's vectors times its box pixels
-      if (on && P.strideMax[t] > 0) acc += (long)P.nVecM[k] * used[t];
+      prefix[t] = acc;   // tail work: the task's vectors times its box pixels (derivative planes: one wave per pixel)
+      if (on && P.strideMax[t] > 0) acc += (long)(P.kind == BK_JAC ? 1 : P.nVecM[k]) * used[t];
     }
   }
   prefix[P.nTasks] = acc;

@@ -668,12 +668,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Moffat: PA ell I_0 fwhm beta; FlatSky: I_sky; ExponentialDisk3D: PA inc J_0 h n
 // z_0, see disk_grad).  Geometry enters through the
 // elliptical radius r(x0, y0, PA, ell); at r = 0 the radial terms are taken as 0.
-// Analytic derivative of the generalized-ellipse Sersic (func_gen-sersic.cpp:112-165)
-// at (x, y): I = I_e exp(-bn (u - 1)), u = (R/r_e)^(1/n), R = S^(1/e),
-// S = |X|^e + |Y|^e, e = c0 + 2, X = dx cos + dy sin, Y = (-dx sin + dy cos)/q, so
-// dR/dX = (R/S) |X|^(e-1) sgn X (and for Y), dR/de = R ((|X|^e ln|X| + |Y|^e ln|Y|)/(S e)
-// - ln S / e^2).  Slots: 0 x0, 1 y0, 2 PA, 3 ell, 4 c0, 5 n, 6 I_e, 7 r_e.
-__device__ double gensersic_grad(const double* v, double x, double y, int slot) {
+// Slot derivatives of one component: g[s] = d(intensity)/d(parameter slot s) at
+// (x, y), for the slots whose bit is set in `need` (the others are 0).  Slots: 0 x0,
+// 1 y0, then the component's own parameters in order (2 PA, 3 ell, ...).  The
+// radius, intensity and transcendentals are shared by all slots; each slot's value
+// is the same expression whether one slot or several are asked for.
+//
+// Generalized-ellipse Sersic (func_gen-sersic.cpp:112-165): I = I_e exp(-bn (u - 1)),
+// u = (R/r_e)^(1/n), R = S^(1/e), S = |X|^e + |Y|^e, e = c0 + 2, X = dx cos + dy sin,
+// Y = (-dx sin + dy cos)/q, so dR/dX = (R/S) |X|^(e-1) sgn X (and for Y), dR/de =
+// R ((|X|^e ln|X| + |Y|^e ln|Y|)/(S e) - ln S / e^2).  Slots: 4 c0, 5 n, 6 I_e, 7 r_e.
+__device__ __forceinline__ void gensersic_grad_all(const double* v, double x, double y, unsigned need,
+                                                   double (&g)[8]) {
   const double DEG2RAD = 0.017453292519943295;
   const double c = v[2], s = v[3], q = v[4], e = v[11];
   const double dx = x - v[0], dy = y - v[1];
@@ -685,41 +691,54 @@ __device__ double gensersic_grad(const double* v, double x, double y, int slot) 
   const double u = pow(R / re, invn);
   const double ex = exp(-bn * (u - 1.0));
   const double I = Ie * ex;
-  if (slot == 5) {   // n: bn(n) and u(n)
+  if (need & (1u << 5)) {   // n: bn(n) and u(n)
     const double dudn = (R > 0.0) ? -u * log(R / re) * invn * invn : 0.0;
-    return I * (-v[10] * (u - 1.0) - bn * dudn);
+    g[5] = I * (-v[10] * (u - 1.0) - bn * dudn);
   }
-  if (slot == 6) return ex;
-  if (slot == 7) return I * bn * u * invn / re;
-  if (!(R > 0.0)) return 0.0;
+  if (need & (1u << 6)) g[6] = ex;
+  if (need & (1u << 7)) g[7] = I * bn * u * invn / re;
+  if (!(R > 0.0)) return;
   const double dIdR = -I * bn * u * invn / R;
-  if (slot == 4) {
+  if (need & (1u << 4)) {
     const double lx = ax > 0.0 ? A * log(ax) : 0.0, ly = ay > 0.0 ? B * log(ay) : 0.0;
-    return dIdR * R * ((lx + ly) / (S * e) - log(S) / (e * e));
+    g[4] = dIdR * R * ((lx + ly) / (S * e) - log(S) / (e * e));
   }
+  if (!(need & 15u)) return;
   const double RS = R / S;
   const double gX = ax > 0.0 ? RS * pow(ax, e - 1.0) * (X > 0.0 ? 1.0 : -1.0) : 0.0;
   const double gY = ay > 0.0 ? RS * pow(ay, e - 1.0) * (Y > 0.0 ? 1.0 : -1.0) : 0.0;
-  double drdp;
-  if (slot == 0) drdp = -gX * c + gY * s / q;
-  else if (slot == 1) drdp = -gX * s - gY * c / q;
-  else if (slot == 2) drdp = DEG2RAD * (gX * q * Y - gY * X / q);
-  else drdp = gY * Y / q;
-  return dIdR * drdp;
+  if (need & 1u) g[0] = dIdR * (-gX * c + gY * s / q);
+  if (need & 2u) g[1] = dIdR * (-gX * s - gY * c / q);
+  if (need & 4u) g[2] = dIdR * (DEG2RAD * (gX * q * Y - gY * X / q));
+  if (need & 8u) g[3] = dIdR * (gY * Y / q);
 }
 
 template <bool DISK>
-__device__ double comp_grad(int type, const double* v, double x, double y, int slot) {
-  if (type == F_FLATSKY) return slot == 2 ? 1.0 : 0.0;
-  if (type == F_GENSERSIC) return gensersic_grad(v, x, y, slot);
-  if (DISK && type == F_EXPDISK3D) return disk_grad(v, x, y, slot, 3);
+__device__ __forceinline__ void comp_grad_all(int type, const double* v, double x, double y, unsigned need,
+                                              double (&g)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) g[k] = 0.0;
+  if (type == F_FLATSKY) {
+    g[2] = 1.0;
+    return;
+  }
+  if (DISK && type == F_EXPDISK3D) {
+#pragma unroll 1
+    for (int k = 0; k < 8; k++)
+      if (need & (1u << k)) g[k] = disk_grad(v, x, y, k, 3);
+    return;
+  }
+  if (type == F_GENSERSIC) {
+    gensersic_grad_all(v, x, y, need, g);
+    return;
+  }
   const double DEG2RAD = 0.017453292519943295;
   const double c = v[2], s = v[3], q = v[4];
   const double x_diff = x - v[0], y_diff = y - v[1];
   const double xp = x_diff * c + y_diff * s;
   const double yp = (-x_diff * s + y_diff * c) / q;
   const double r = sqrt(xp * xp + yp * yp);
-  double I, dIdr, dloc = 0.0;
+  double I, dIdr;
   switch (type) {
     case F_SERSIC: {   // I = I_e exp(-bn (u - 1)), u = (r/r_e)^(1/n)
       const double Ie = v[5], re = v[6], bn = v[7], invn = v[8];
@@ -727,12 +746,13 @@ __device__ double comp_grad(int type, const double* v, double x, double y, int s
       const double e = exp(-bn * (u - 1.0));
       I = Ie * e;
       dIdr = (r > 0.0) ? -I * bn * u * invn / r : 0.0;
-      if (slot == 4) {   // n: bn(n) and u(n)
+      if (need & (1u << 4)) {   // n: bn(n) and u(n)
         const double dbn = v[10];
         const double dudn = (r > 0.0) ? -u * log(r / re) * invn * invn : 0.0;
-        dloc = I * (-dbn * (u - 1.0) - bn * dudn);
-      } else if (slot == 5) dloc = e;
-      else if (slot == 6) dloc = I * bn * u * invn / re;
+        g[4] = I * (-dbn * (u - 1.0) - bn * dudn);
+      }
+      if (need & (1u << 5)) g[5] = e;
+      if (need & (1u << 6)) g[6] = I * bn * u * invn / re;
       break;
     }
     case F_EXP: {
@@ -740,8 +760,8 @@ __device__ double comp_grad(int type, const double* v, double x, double y, int s
       const double e = exp(-r / h);
       I = I0 * e;
       dIdr = -I / h;
-      if (slot == 4) dloc = e;
-      else if (slot == 5) dloc = I * r / (h * h);
+      if (need & (1u << 4)) g[4] = e;
+      if (need & (1u << 5)) g[5] = I * r / (h * h);
       break;
     }
     case F_GAUSS: {
@@ -749,8 +769,8 @@ __device__ double comp_grad(int type, const double* v, double x, double y, int s
       const double e = exp(-(r * r) / v[7]);
       I = I0 * e;
       dIdr = -I * r / (sig * sig);
-      if (slot == 4) dloc = e;
-      else if (slot == 5) dloc = I * r * r / (sig * sig * sig);
+      if (need & (1u << 4)) g[4] = e;
+      if (need & (1u << 5)) g[5] = I * r * r / (sig * sig * sig);
       break;
     }
     case F_MOFFAT: {   // I = I_0 t^-beta, t = 1 + (r/alpha)^2
@@ -760,21 +780,34 @@ __device__ double comp_grad(int type, const double* v, double x, double y, int s
       I = I0 * e;
       dIdr = -2.0 * beta * I * r / (alpha * alpha * t);
       const double dIdalpha = 2.0 * beta * I * r * r / (alpha * alpha * alpha * t);
-      if (slot == 4) dloc = e;
-      else if (slot == 5) dloc = dIdalpha * alpha / fwhm;
-      else if (slot == 6) dloc = -I * log(t) + dIdalpha * dalpha_dbeta;
+      if (need & (1u << 4)) g[4] = e;
+      if (need & (1u << 5)) g[5] = dIdalpha * alpha / fwhm;
+      if (need & (1u << 6)) g[6] = -I * log(t) + dIdalpha * dalpha_dbeta;
       break;
     }
-    default: return 0.0;
+    default: return;
   }
-  if (slot >= 4) return dloc;
-  if (!(r > 0.0)) return 0.0;
-  double drdp;
-  if (slot == 0) drdp = (-xp * c + yp * s / q) / r;
-  else if (slot == 1) drdp = (-xp * s - yp * c / q) / r;
-  else if (slot == 2) drdp = DEG2RAD * xp * yp * (q - 1.0 / q) / r;
-  else drdp = yp * yp / (q * r);
-  return dIdr * drdp;
+  if (!(r > 0.0)) return;
+  if (need & 1u) g[0] = dIdr * ((-xp * c + yp * s / q) / r);
+  if (need & 2u) g[1] = dIdr * ((-xp * s - yp * c / q) / r);
+  if (need & 4u) g[2] = dIdr * (DEG2RAD * xp * yp * (q - 1.0 / q) / r);
+  if (need & 8u) g[3] = dIdr * (yp * yp / (q * r));
+}
+
+// g[s] for a slot s known only at run time (selects, no indexed register access).
+__device__ __forceinline__ double pick8(const double (&g)[8], int s) {
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k == s) r = g[k];
+  return r;
+}
+
+template <bool DISK>
+__device__ double comp_grad(int type, const double* v, double x, double y, int slot) {
+  double g[8];
+  comp_grad_all<DISK>(type, v, x, y, 1u << slot, g);
+  return pick8(g, slot);
 }
 
 // Which slot of component n the global parameter gidx is, or -1.
@@ -787,7 +820,7 @@ __device__ __forceinline__ int grad_slot(const EvalArgs& ea, int n, int gidx) {
 
 // d(model)/d(parameter freeIdx[j]) at a grid pixel, without sub-pixel integration.
 template <bool DISK>
-__device__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
+__device__ __forceinline__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
   const int gidx = ea.freeIdx[j];
   double sum = 0.0;
   for (int n = 0; n < ea.nComp; n++) {
@@ -795,6 +828,69 @@ __device__ double deriv_point(const EvalArgs& ea, int j, double x, double y) {
     if (slot >= 0) sum += comp_grad<DISK>(ea.types[n], ea.recs[n].v, x, y, slot);
   }
   return sum;
+}
+
+// Planes jA and jB at one point: each component's slots come from one shared
+// evaluation, summed over components in order as deriv_point does for one plane.
+template <bool DISK>
+__device__ __forceinline__ void deriv_point2(const EvalArgs& ea, int jA, int jB, double x, double y, double& dA,
+                                             double& dB) {
+  const int gA = ea.freeIdx[jA], gB = ea.freeIdx[jB];
+  double sA = 0.0, sB = 0.0;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int slA = grad_slot(ea, n, gA), slB = grad_slot(ea, n, gB);
+    if (slA < 0 && slB < 0) continue;
+    double gs[8];
+    comp_grad_all<DISK>(ea.types[n], ea.recs[n].v, x, y, (slA >= 0 ? 1u << slA : 0u) | (slB >= 0 ? 1u << slB : 0u),
+                        gs);
+    if (slA >= 0) sA += pick8(gs, slA);
+    if (slB >= 0) sB += pick8(gs, slB);
+  }
+  dA = sA;
+  dB = sB;
+}
+
+// Derivative planes jA and jB (jB -1: none) on K pixels of their rows ra / rb
+// (equal for derivative pairs), then the sub-pixel tail values inside the boxes.
+template <typename T, bool DISK, int K>
+__device__ __forceinline__ void row_derivs2(const EvalArgs& ea, const GridGeom& g, int jA, int jB, const Box* bx,
+                                            const int (&ra)[K], const int (&rb)[K], const int (&cols)[K],
+                                            const bool (&ok)[K], T (&outA)[K], T (&outB)[K]) {
+#pragma unroll 1
+  for (int k = 0; k < K; k++) {
+    outA[k] = 0;
+    outB[k] = 0;
+    if (!ok[k]) continue;
+    const double x = g.xa + (double)cols[k] * g.frac;
+    const double yA = g.ya + (double)ra[k] * g.frac;
+    if (jB >= 0 && rb[k] == ra[k]) {
+      double dA, dB;
+      deriv_point2<DISK>(ea, jA, jB, x, yA, dA, dB);
+      outA[k] = (T)dA;
+      outB[k] = (T)dB;
+    } else {   // one plane at a time (kept inline: same sums as deriv_point)
+      double d, unused;
+      deriv_point2<DISK>(ea, jA, jA, x, yA, d, unused);
+      outA[k] = (T)d;
+      if (jB >= 0) {
+        deriv_point2<DISK>(ea, jB, jB, x, g.ya + (double)rb[k] * g.frac, d, unused);
+        outB[k] = (T)d;
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < IFG_MAXBOX; b++) {
+    const Box B = bx[b];
+    if (B.w <= 0) continue;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int dc = cols[k] - B.c0, drA = ra[k] - B.r0, drB = rb[k] - B.r0;
+      if (!ok[k] || (unsigned)dc >= (unsigned)B.w) continue;
+      if ((unsigned)drA < (unsigned)B.h) outA[k] = (T)ea.tail[(long)jA * ea.tailStride + B.off + drA * B.w + dc];
+      if (jB >= 0 && (unsigned)drB < (unsigned)B.h)
+        outB[k] = (T)ea.tail[(long)jB * ea.tailStride + B.off + drB * B.w + dc];
+    }
+  }
 }
 
 // ------------------------------------------------------------------ model values of grid rows
@@ -897,32 +993,48 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
   const double y = g.ya + (double)r * g.frac;
   const CompRec* rec = ea.recs + (long)vec * ea.recStride;
   double* out = const_cast<double*>(ea.tail) + (long)vec * ea.tailStride + loc;
-  if (DERIV) {
-    const int gidx = ea.freeIdx[vec];
-    double dsum = 0.0;
+  if (DERIV) {   // every derivative plane at this box pixel (the launch counts box pixels, not planes)
+    double* outs = const_cast<double*>(ea.tail) + loc;   // plane j at outs[j * tailStride]
+    if (lane == 0)
+      for (int j = 0; j < ea.nFree; j++) outs[(long)j * ea.tailStride] = 0.0;
     for (int n = 0; n < ea.nComp; n++) {
-      const int slot = grad_slot(ea, n, gidx);
-      if (slot < 0) continue;
+      unsigned need = 0;   // this component's slots that some plane differentiates by
+      for (int j = 0; j < ea.nFree; j++) {
+        const int slot = grad_slot(ea, n, ea.freeIdx[j]);
+        if (slot >= 0) need |= 1u << slot;
+      }
+      if (!need) continue;
       const int type = ea.types[n];
       const CompRec rv = rec[n];   // into registers: read once, not per sample
       const double* v = rv.v;
       const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
+      double mean[8];
       if (nSub > 1) {
         const double deltaSubpix = 1.0 / nSub;
         const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
         const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
         const int n2 = nSub * nSub;
-        double part = 0.0;
+        double part[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) part[k] = 0.0;
         for (int s = lane; s < n2; s += 64) {
           const int ii = s / nSub, jj = s - ii * nSub;
-          part += comp_grad<DISK>(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, slot);
+          double gs[8];
+          comp_grad_all<DISK>(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, need, gs);
+#pragma unroll
+          for (int k = 0; k < 8; k++) part[k] += gs[k];
         }
-        dsum += wave_sum(part) / (nSub * nSub);
+#pragma unroll
+        for (int k = 0; k < 8; k++) mean[k] = (need & (1u << k)) ? wave_sum(part[k]) / (nSub * nSub) : 0.0;
       } else {
-        dsum += comp_grad<DISK>(type, v, x, y, slot);
+        comp_grad_all<DISK>(type, v, x, y, need, mean);
       }
+      if (lane == 0)   // components in order, as the sum over components of each plane
+        for (int j = 0; j < ea.nFree; j++) {
+          const int slot = grad_slot(ea, n, ea.freeIdx[j]);
+          if (slot >= 0) outs[(long)j * ea.tailStride] += pick8(mean, slot);
+        }
     }
-    if (lane == 0) *out = dsum;
     return;
   }
   double newValSum = 0.0, storedError = 0.0;
@@ -1061,8 +1173,12 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
       ok[k] = i < total && ir < g.Lr && j < g.Lc;
       vb[k] = 0;
     }
-    row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], ra, cols, ok, va);
-    if (pr.z >= 0) row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rb, cols, ok, vb);
+    if (DERIV) {
+      row_derivs2<T, DISK, K>(Tk.ea, g, pr.x, pr.z, sBox[0], ra, rb, cols, ok, va, vb);
+    } else {
+      row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], ra, cols, ok, va);
+      if (pr.z >= 0) row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rb, cols, ok, vb);
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int i = i0 + k * kThreads;
