@@ -225,9 +225,60 @@ def main():
         padded_psf_delta=printed_block(t("psfconvolve_out1.txt"),
                                        "The whole padded, normalized PSF image"),
     )
+    copy_inputs()
     for fn in sorted(os.listdir(OUT)):
         if fn.endswith(".npz"):
             print("%-22s %8d bytes" % (fn, os.path.getsize(os.path.join(OUT, fn))))
+
+
+# Input files of the reference's regression and unit tests, copied byte for byte so
+# that the configuration-file and FITS readers run on the very files imfit read.
+FITS_INPUTS = [
+    # do_imfit_tests:63-99,107 and do_makeimage_tests:68-98 inputs
+    "testimage_expdisk32.fits", "uniform_image32.fits",
+    "ic3478rss_64x64.fits", "ic3478rss_64x64_sigma.fits", "ic3478rss_64x64_variance.fits",
+    "n3073rss_small.fits", "n3073rss_small_mask.fits", "n3073rss_small_cps.fits", "psf_moffat_35_n4699z.fits",
+    "twogaussian_psf+2osamp_noisy.fits", "psf_moffat_35.fits", "psf_moffat_35_oversamp3.fits",
+    "oversamp_test4.fits", "psf_standard.fits", "psf_oversamp.fits",
+    "testimage_3x3_nan.fits", "testimage_3x3_allzeros.fits",
+    # unit_tests/unittest_image_io.t.h:17-25
+    "test_emptyhdu.fits", "test_table.fits", "test_multiextension_hdu0empty.fits",
+    "test_multiextension_2images.fits", "multiextension_ic3478rss_64x64_twice.fits",
+]
+CONFIG_INPUTS = [
+    # do_imfit_tests:63-107
+    "imfit_reference/config_imfit_expdisk32.dat", "imfit_reference/imfit_config_ic3478_64x64.dat",
+    "imfit_reference/imfit_config_ic3478_64x64b.dat", "imfit_reference/imfit_config_ic3478_64x64c.dat",
+    "imfit_reference/imfit_config_n3073.dat", "imfit_reference/config_imfit_2gauss_small.dat",
+    "imfit_reference/config_imfit_gauss-oversample-test2.dat", "imfit_reference/config_imfit_poisson.dat",
+    "imfit_reference/config_imfit_badparamline.dat",
+    # do_makeimage_tests:48-98
+    "makeimage_reference/config_biggertest_4c.dat", "makeimage_reference/config_makeimage_gensersic512.dat",
+    "makeimage_reference/config_makeimage_sersic+exp512.dat",
+    "makeimage_reference/config_makeimage_sersic+exp512_nosize.dat",
+    "makeimage_reference/config_makeimage_gauss-oversample.dat",
+    "makeimage_reference/config_makeimage_gauss-oversample2.dat",
+    "makeimage_reference/config_makeimage_sersictest512_bad1.dat",
+    "makeimage_reference/config_makeimage_sersictest512_bad2.dat",
+    "makeimage_reference/config_makeimage_sersictest512_bad3.dat",
+    "makeimage_reference/config_makeimage_sersictest512_bad4.dat",
+    "makeimage_reference/config_makeimage_sersictest512_bad5.dat",
+    # unit_tests/unittest_config_parser.t.h:35-52
+    "config_makeimage_sersictest512.dat", "config_makeimage_sersictest512_with_labels.dat",
+    "config_imfit_sersictest512_badlimits4.dat",
+    "config_makeimage-optional_params0.dat", "config_makeimage-optional_params1.dat",
+    "config_makeimage-optional_params2.dat", "config_makeimage-optional_params3.dat",
+    "config_makeimage-optional_params_bad1.dat",
+]
+
+
+def copy_inputs():
+    import shutil
+    for sub, names in (("fits", FITS_INPUTS), ("configs", CONFIG_INPUTS)):
+        dst = os.path.join(OUT, sub)
+        os.makedirs(dst, exist_ok=True)
+        for name in names:
+            shutil.copyfile(t(name), os.path.join(dst, os.path.basename(name)))
 
 
 if __name__ == "__main__":
