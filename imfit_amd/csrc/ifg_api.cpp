@@ -61,6 +61,7 @@ int NParamsOf(int type) {
     case F_MOFFAT: return 5;
     case F_EXPDISK3D: return 6;
     case F_GENSERSIC: return 6;
+    case F_BROKENEXP: return 7;
   }
   return 0;
 }
@@ -74,14 +75,17 @@ int TypeOf(const char* name) {
   if (s == "Moffat") return F_MOFFAT;
   if (s == "ExponentialDisk3D") return F_EXPDISK3D;
   if (s == "Sersic_GenEllipse") return F_GENSERSIC;
+  if (s == "BrokenExponential") return F_BROKENEXP;
   return 0;
 }
 
 bool Subsampling(int type) {
-  return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT || type == F_GENSERSIC;
+  return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT || type == F_GENSERSIC ||
+         type == F_BROKENEXP;
 }
 
 // FunctionObject::Setup for one component (func_sersic.cpp:109-127, func_exp.cpp:104-118,
+// func_broken-exp.cpp:88-111,
 // func_gaussian.cpp:104-121, func_moffat.cpp:111-132, func_flatsky.cpp:85-90,
 // func_expdisk3d.cpp:149-172, func_gen-sersic.cpp:112-133).
 void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
@@ -127,6 +131,15 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
       const double I_0 = p[2], h = p[3];
       v[5] = I_0; v[6] = h;
       v[12] = -LOG2E / h;
+      break;
+    }
+    case F_BROKENEXP: {   // PA, ell, I_0, h1, h2, r_break, alpha
+      const double I_0 = p[2], h1 = p[3], h2 = p[4], r_b = p[5], alpha = p[6];
+      const double exponent = (1.0 / alpha) * (1.0 / h1 - 1.0 / h2);
+      const double S = pow((1.0 + exp(-alpha * r_b)), (-exponent));
+      v[5] = I_0 * S; v[6] = h1; v[7] = h2; v[8] = r_b; v[9] = alpha; v[10] = exponent;
+      v[11] = r_b / h2 - r_b / h1;
+      v[12] = -LOG2E / h1; v[13] = S; v[14] = log1p(exp(-alpha * r_b));
       break;
     }
     case F_GAUSS: {

@@ -11,7 +11,9 @@
 
 namespace ifg {
 
-enum FuncType { F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6, F_GENSERSIC = 7 };
+enum FuncType {
+  F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6, F_GENSERSIC = 7, F_BROKENEXP = 8
+};
 
 // S_JAC: write Jacobian columns dfac * dm/dp_j;  S_GRAD: accumulate sum(gfac * dm/dp_j)
 enum StatMode { S_NONE = 0, S_CHI2 = 1, S_CASH = 2, S_MLR = 3, S_JAC = 4, S_GRAD = 5 };
@@ -27,6 +29,8 @@ struct cpx {
 //   Sersic:      v[5] I_e  v[6] r_e  v[7] bn   v[8] 1/n
 //   Sersic_GenEllipse: as Sersic, and v[11] c0 + 2 (the generalized-ellipse exponent)
 //   Exponential: v[5] I_0  v[6] h
+//   BrokenExponential: v[5] I_0 S  v[6] h1  v[7] h2  v[8] r_b  v[9] alpha  v[10] exponent
+//                      v[11] r_b/h2 - r_b/h1  v[12] -log2(e)/h1  v[13] S  v[14] ln(1 + e^(-alpha r_b))
 //   Gaussian:    v[5] I_0  v[6] sigma  v[7] 2 sigma^2
 //   Moffat:      v[5] I_0  v[6] alpha  v[7] beta
 //   FlatSky:     v[5] I_sky

@@ -330,6 +330,9 @@ __device__ __forceinline__ double intensity_f64(int type, const double* v, doubl
     case F_SERSIC:
     case F_GENSERSIC: return v[5] * exp(-v[7] * (pow((r / v[6]), v[8]) - 1.0));
     case F_EXP: return v[5] * exp(-r / v[6]);
+    case F_BROKENEXP:   // func_broken-exp.cpp:118-131, outer form past alpha (r - r_b) = 100
+      if (v[9] * (r - v[8]) > 100.0) return v[5] * exp(v[11] - r / v[7]);
+      return v[5] * exp(-r / v[6]) * pow(1.0 + exp(v[9] * (r - v[8])), v[10]);
     case F_GAUSS: {
       double r_squared = r * r;
       return v[5] * exp(-r_squared / v[7]);
@@ -352,6 +355,12 @@ __device__ __forceinline__ float intensity_f32_r2(int type, const double* v, flo
       return (float)v[5] * __builtin_amdgcn_exp2f((float)v[14] * (p - 1.0f));
     }
     case F_EXP: return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * __builtin_sqrtf(r2));
+    case F_BROKENEXP: {   // log2 I = log2(I_0 S) - r log2e/h1 + E log2(1 + e^t), t = alpha (r - r_b)
+      const float r = __builtin_sqrtf(r2);
+      const float t2 = (float)v[9] * 1.44269504088896341f * (r - (float)v[8]);   // t log2(e)
+      const float sp = fmaxf(t2, 0.0f) + __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-fabsf(t2)));
+      return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * r + (float)v[10] * sp);
+    }
     case F_GAUSS: return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * r2);
     case F_MOFFAT:
       return (float)v[5] * __builtin_amdgcn_exp2f((float)v[13] * __builtin_amdgcn_logf(1.0f + r2 * (float)v[12]));
@@ -643,13 +652,16 @@ __device__ __forceinline__ T comp_point(int type, const double* v, double x, dou
 }
 
 // GALFIT sub-pixel count (func_sersic.cpp:197-213; func_exp.cpp:181-197;
-// func_gaussian.cpp:187-203; func_moffat.cpp:200-216).  v[6] is r_e / h / sigma / alpha.
+// func_broken-exp.cpp:225-243; func_gaussian.cpp:187-203; func_moffat.cpp:200-216).
+// v[6] is r_e / h / h1 / sigma / alpha.
 __device__ __forceinline__ int n_subsamples(int type, const double* v, double r, int on) {
   if (!on || !(r < 10.0)) return 1;
   double size = v[6];
   double small = (type == F_GAUSS || type == F_MOFFAT) ? 40.0 : 20.0;
   double rlim = (type == F_SERSIC || type == F_GENSERSIC) ? 4.0 : 3.0;
-  if (type != F_SERSIC && type != F_EXP && type != F_GAUSS && type != F_MOFFAT && type != F_GENSERSIC) return 1;
+  if (type != F_SERSIC && type != F_EXP && type != F_GAUSS && type != F_MOFFAT && type != F_GENSERSIC &&
+      type != F_BROKENEXP)
+    return 1;
   if ((size <= 1.0) && (r <= 1.0)) return min(100, (int)(small / size));
   if (r <= rlim) return 20;
   return min(100, (int)(20 / r));
@@ -662,6 +674,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ------------------------------------------------------------------ analytic derivatives
+// Parameter slots of one component: x0, y0 and up to seven own parameters.
+constexpr int kSlots = 9;
+
 // d(point intensity)/d(parameter) of one component at (x, y).  slot 0 = X0,
 // slot 1 = Y0, slot 2+k = the component's own parameter k in the reference order
 // (Sersic: PA ell n I_e r_e; Exponential: PA ell I_0 h; Gaussian: PA ell I_0 sigma;
@@ -679,7 +694,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Y = (-dx sin + dy cos)/q, so dR/dX = (R/S) |X|^(e-1) sgn X (and for Y), dR/de =
 // R ((|X|^e ln|X| + |Y|^e ln|Y|)/(S e) - ln S / e^2).  Slots: 4 c0, 5 n, 6 I_e, 7 r_e.
 __device__ __forceinline__ void gensersic_grad_all(const double* v, double x, double y, unsigned need,
-                                                   double (&g)[8]) {
+                                                   double (&g)[kSlots]) {
   const double DEG2RAD = 0.017453292519943295;
   const double c = v[2], s = v[3], q = v[4], e = v[11];
   const double dx = x - v[0], dy = y - v[1];
@@ -715,16 +730,16 @@ __device__ __forceinline__ void gensersic_grad_all(const double* v, double x, do
 
 template <bool DISK>
 __device__ __forceinline__ void comp_grad_all(int type, const double* v, double x, double y, unsigned need,
-                                              double (&g)[8]) {
+                                              double (&g)[kSlots]) {
 #pragma unroll
-  for (int k = 0; k < 8; k++) g[k] = 0.0;
+  for (int k = 0; k < kSlots; k++) g[k] = 0.0;
   if (type == F_FLATSKY) {
     g[2] = 1.0;
     return;
   }
   if (DISK && type == F_EXPDISK3D) {
 #pragma unroll 1
-    for (int k = 0; k < 8; k++)
+    for (int k = 0; k < 8; k++)   // the disk's slots: x0 y0 PA inc J_0 h n z_0
       if (need & (1u << k)) g[k] = disk_grad(v, x, y, k, 3);
     return;
   }
@@ -764,6 +779,39 @@ __device__ __forceinline__ void comp_grad_all(int type, const double* v, double 
       if (need & (1u << 5)) g[5] = I * r / (h * h);
       break;
     }
+    case F_BROKENEXP: {
+      // ln I = ln I_0 - r/h1 + E A, A = ln(1 + e^t) - ln(1 + e^(-alpha r_b)), t = alpha (r - r_b),
+      // E = (1/h1 - 1/h2)/alpha; past t = 100 the outer form ln I = ln I_0 - E ln(1 + e^(-alpha r_b))
+      // + r_b/h2 - r_b/h1 - r/h2 (the branch the value takes).  Slots: 4 I_0, 5 h1, 6 h2, 7 r_b, 8 alpha.
+      const double h1 = v[6], h2 = v[7], rb = v[8], al = v[9], E = v[10], S = v[13], L0 = v[14];
+      const double t = al * (r - rb);
+      const double sb = 1.0 / (1.0 + exp(al * rb));   // sigma(-alpha r_b)
+      double f;                                         // I / I_0
+      if (t > 100.0) {
+        f = S * exp(v[11] - r / h2);
+        I = v[5] * exp(v[11] - r / h2);
+        dIdr = -I / h2;
+        if (need & (1u << 5)) g[5] = I * (L0 / (al * h1 * h1) + rb / (h1 * h1));
+        if (need & (1u << 6)) g[6] = I * ((r - rb) / (h2 * h2) - L0 / (al * h2 * h2));
+        if (need & (1u << 7)) g[7] = I * (E * al * sb + 1.0 / h2 - 1.0 / h1);
+        if (need & (1u << 8)) g[8] = I * (E / al * L0 + E * rb * sb);
+      } else {
+        const double et = exp(t);
+        const double e1 = exp(-r / h1);
+        const double P = pow(1.0 + et, E);
+        const double st = et / (1.0 + et);   // sigma(t)
+        const double A = log1p(et) - L0;
+        f = S * e1 * P;
+        I = v[5] * e1 * P;
+        dIdr = I * (-1.0 / h1 + E * al * st);
+        if (need & (1u << 5)) g[5] = I * (r / (h1 * h1) - A / (al * h1 * h1));
+        if (need & (1u << 6)) g[6] = I * (A / (al * h2 * h2));
+        if (need & (1u << 7)) g[7] = I * E * al * (sb - st);
+        if (need & (1u << 8)) g[8] = I * (E * ((r - rb) * st + rb * sb) - E / al * A);
+      }
+      if (need & (1u << 4)) g[4] = f;
+      break;
+    }
     case F_GAUSS: {
       const double I0 = v[5], sig = v[6];
       const double e = exp(-(r * r) / v[7]);
@@ -795,19 +843,19 @@ __device__ __forceinline__ void comp_grad_all(int type, const double* v, double 
 }
 
 // g[s] for a slot s known only at run time (selects, no indexed register access).
-__device__ __forceinline__ double pick8(const double (&g)[8], int s) {
+__device__ __forceinline__ double pick_slot(const double (&g)[kSlots], int s) {
   double r = 0.0;
 #pragma unroll
-  for (int k = 0; k < 8; k++)
+  for (int k = 0; k < kSlots; k++)
     if (k == s) r = g[k];
   return r;
 }
 
 template <bool DISK>
 __device__ double comp_grad(int type, const double* v, double x, double y, int slot) {
-  double g[8];
+  double g[kSlots];
   comp_grad_all<DISK>(type, v, x, y, 1u << slot, g);
-  return pick8(g, slot);
+  return pick_slot(g, slot);
 }
 
 // Which slot of component n the global parameter gidx is, or -1.
@@ -840,11 +888,11 @@ __device__ __forceinline__ void deriv_point2(const EvalArgs& ea, int jA, int jB,
   for (int n = 0; n < ea.nComp; n++) {
     const int slA = grad_slot(ea, n, gA), slB = grad_slot(ea, n, gB);
     if (slA < 0 && slB < 0) continue;
-    double gs[8];
+    double gs[kSlots];
     comp_grad_all<DISK>(ea.types[n], ea.recs[n].v, x, y, (slA >= 0 ? 1u << slA : 0u) | (slB >= 0 ? 1u << slB : 0u),
                         gs);
-    if (slA >= 0) sA += pick8(gs, slA);
-    if (slB >= 0) sB += pick8(gs, slB);
+    if (slA >= 0) sA += pick_slot(gs, slA);
+    if (slB >= 0) sB += pick_slot(gs, slB);
   }
   dA = sA;
   dB = sB;
@@ -1008,31 +1056,31 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
       const CompRec rv = rec[n];   // into registers: read once, not per sample
       const double* v = rv.v;
       const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
-      double mean[8];
+      double mean[kSlots];
       if (nSub > 1) {
         const double deltaSubpix = 1.0 / nSub;
         const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
         const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
         const int n2 = nSub * nSub;
-        double part[8];
+        double part[kSlots];
 #pragma unroll
-        for (int k = 0; k < 8; k++) part[k] = 0.0;
+        for (int k = 0; k < kSlots; k++) part[k] = 0.0;
         for (int s = lane; s < n2; s += 64) {
           const int ii = s / nSub, jj = s - ii * nSub;
-          double gs[8];
+          double gs[kSlots];
           comp_grad_all<DISK>(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, need, gs);
 #pragma unroll
-          for (int k = 0; k < 8; k++) part[k] += gs[k];
+          for (int k = 0; k < kSlots; k++) part[k] += gs[k];
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) mean[k] = (need & (1u << k)) ? wave_sum(part[k]) / (nSub * nSub) : 0.0;
+        for (int k = 0; k < kSlots; k++) mean[k] = (need & (1u << k)) ? wave_sum(part[k]) / (nSub * nSub) : 0.0;
       } else {
         comp_grad_all<DISK>(type, v, x, y, need, mean);
       }
       if (lane == 0)   // components in order, as the sum over components of each plane
         for (int j = 0; j < ea.nFree; j++) {
           const int slot = grad_slot(ea, n, ea.freeIdx[j]);
-          if (slot >= 0) outs[(long)j * ea.tailStride] += pick8(mean, slot);
+          if (slot >= 0) outs[(long)j * ea.tailStride] += pick_slot(mean, slot);
         }
     }
     return;

@@ -11,7 +11,7 @@
 // FMA contraction without -march).
 //
 // Pinning (what checks each part; see tests/test_oracle.py):
-//   * GetValue of Sersic/Exponential/Gaussian/Moffat/FlatSky: unit_tests/unittest_funcs.t.h
+//   * GetValue of Sersic/Exponential/BrokenExponential/Gaussian/Moffat/FlatSky: unit_tests/unittest_funcs.t.h
 //     known answers; Calculate_bn against the reference's own helper_funcs.cpp built
 //     into oracle/_ref.
 //   * model images: tests/sersic+exp_orig.fits, tests/biggertest_orig.fits (float32);
@@ -195,6 +195,33 @@ struct Exponential : Func {   // function_objects/func_exp.cpp:104-197
   }
   double CalculateIntensity(double r) const { return I_0 * exp(-r / h); }
   int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, h, 2 * SUBSAMPLE_R, 3.0); }
+  double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+// Broken exponential (function_objects/func_broken-exp.cpp:88-245): inner scale
+// length h1, outer h2, break radius r_b, sharpness alpha; I = I_0 S e^(-r/h1)
+// (1 + e^(alpha (r - r_b)))^E with E = (1/h1 - 1/h2)/alpha and
+// S = (1 + e^(-alpha r_b))^(-E), and the outer-exponential form once
+// alpha (r - r_b) > 100 (:118-131).  Sub-pixel rule of the exponential, on h1 (:225-243).
+struct BrokenExp : Func {
+  double PA, ell, I_0, h1, h2, r_b, alpha, q, PA_rad, cosPA, sinPA, exponent, I_0_times_S, delta_Rb_scaled;
+  int NParams() const override { return 7; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; ell = p[o + 1]; I_0 = p[o + 2]; h1 = p[o + 3]; h2 = p[o + 4]; r_b = p[o + 5]; alpha = p[o + 6];
+    q = 1.0 - ell;
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    exponent = (1.0 / alpha) * (1.0 / h1 - 1.0 / h2);
+    double S = pow((1.0 + exp(-alpha * r_b)), (-exponent));
+    I_0_times_S = I_0 * S;
+    delta_Rb_scaled = r_b / h2 - r_b / h1;
+  }
+  double CalculateIntensity(double r) const {
+    if (alpha * (r - r_b) > 100.0) return I_0_times_S * exp(delta_Rb_scaled - r / h2);
+    return I_0_times_S * exp(-r / h1) * pow(1.0 + exp(alpha * (r - r_b)), exponent);
+  }
+  int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, h1, 2 * SUBSAMPLE_R, 3.0); }
   double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
 };
 
@@ -653,6 +680,7 @@ Func* MakeFunc(const char* name) {   // core/add_functions.cpp:163-385 (name -> 
   if (s == "Sersic") return new Sersic();
   if (s == "Sersic_GenEllipse") return new GenSersic();
   if (s == "Exponential") return new Exponential();
+  if (s == "BrokenExponential") return new BrokenExp();
   if (s == "Gaussian") return new Gaussian();
   if (s == "Moffat") return new Moffat();
   if (s == "FlatSky") return new FlatSky();

@@ -129,6 +129,15 @@ def twogauss_osamp():
                 chars=(1000.0, 0.0, 1.0, 1, 0.0))
 
 
+def brokenexp(alpha=3.0, psf=True):
+    """BrokenExponential + FlatSky on 96x80 pixels (13^2 Gaussian PSF): with alpha 3
+    every pixel past r = r_break + 100/alpha ~ 51 px takes the outer-exponential form
+    (func_broken-exp.cpp:118-131); with alpha 0.3 none does."""
+    return Case([("BrokenExponential", True), ("FlatSky", False)],
+                [48.3, 40.6, 25.0, 0.3, 50.0, 12.0, 5.0, 18.0, alpha, 5.0], ncols=96, nrows=80,
+                psf=gaussian_psf(13, 1.5) if psf else None)
+
+
 def sersic_exp512():
     return Case([("Sersic", True), ("Exponential", False)], [256, 256, 30, 0.5, 2.5, 100, 50, 30, 0.75, 10, 150],
                 ncols=512, nrows=512), golden("sersic_exp512")["image"]

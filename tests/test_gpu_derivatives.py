@@ -78,10 +78,26 @@ def gensersic_fit_case(c0):
     return cases.Case(funcs, p, data=data, psf=psf, chars=(4.0, 1.0, 1.0, 1, 0.0))
 
 
-@pytest.mark.parametrize("which", ["n3073", "textout3", "osamp", "gensersic_boxy", "gensersic_disky"])
+def brokenexp_fit_case(alpha):
+    """BrokenExponential + sky fit data: the oracle model at shifted centre and outer
+    scale length (alpha 3: both the full and the outer-exponential forms occur)."""
+    base = cases.brokenexp(alpha, psf=True)
+    o = base.build(oracle.OracleModel(threads=8))
+    truth = base.params.copy()
+    truth[0] += 0.4
+    truth[6] *= 1.1
+    o.CreateModelImage(truth)
+    return cases.Case(base.funcs, base.params, data=o.GetModelImageVector(), psf=base.psf,
+                      chars=(4.0, 1.0, 1.0, 1, 0.0))
+
+
+@pytest.mark.parametrize("which", ["n3073", "textout3", "osamp", "gensersic_boxy", "gensersic_disky",
+                                   "brokenexp_sharp", "brokenexp_soft"])
 def test_jacobian_vs_central_differences(which):
     announce("jacobian %s" % which)
-    if which == "n3073":
+    if which.startswith("brokenexp"):
+        case = brokenexp_fit_case(3.0 if which.endswith("sharp") else 0.3)
+    elif which == "n3073":
         case, _ = cases.n3073()
     elif which == "textout3":
         case, _ = cases.textout3()

@@ -99,6 +99,23 @@ def test_gensersic_goldens(conv):
     assert np.abs(g32.GetModelImageVector() - ref).max() <= 5e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("alpha,psf", [(3.0, False), (3.0, True), (0.3, True)])
+def test_broken_exponential_images(alpha, psf):
+    """BrokenExponential on the GPU against the oracle (pinned by the reference's unit
+    test values): FP64 within 1e-12, FP32 within 5e-6 of the image maximum."""
+    announce("broken exponential alpha=%g psf=%s" % (alpha, psf))
+    case = cases.brokenexp(alpha, psf)
+    g = case.build(gpu_model())
+    g.CreateModelImage(case.params)
+    o = case.build(oracle.OracleModel(threads=8))
+    o.CreateModelImage(case.params)
+    ref = o.GetModelImageVector()
+    assert_images_close(g.GetModelImageVector(), ref, conv=psf)
+    g32 = case.build(gpu_model(1))
+    g32.CreateModelImage(case.params)
+    assert np.abs(g32.GetModelImageVector() - ref).max() <= 5e-6 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("two", [False, True])
 def test_oversampled_regions(two):
     announce("oversampled two=%s" % two)

@@ -54,6 +54,35 @@ def test_moffat_known_answers():   # unit_tests/unittest_funcs.t.h:975-1033
         assert abs(oracle.func_value("Moffat", p, 10, 10, x, y) - 0.96153846153846145) < DELTA
 
 
+@pytest.mark.parametrize("ell", [0.0, 0.5])
+def test_broken_exponential_known_answers(ell):   # unit_tests/unittest_funcs.t.h:1667-1740
+    """I_0 100, h1 50, h2 10, r_break 50, alpha 10 at PA 90 centred on (100, 100):
+    values at r = 0, 1, r_break and 2 r_break along both axes (y offsets scaled by
+    q = 1 - ell); r = 2 r_break takes the outer-exponential form (alpha (r - r_b) > 100)."""
+    p = [90.0, ell, 100.0, 50.0, 10.0, 50.0, 10.0]
+    q = 1.0 - ell
+    f = lambda x, y: oracle.func_value("BrokenExponential", p, 100.0, 100.0, x, y)
+    assert abs(f(100.0, 100.0) - 100.0) < DELTA
+    for r, val in [(1.0, 98.019867330675524), (50.0, 36.584512991317212), (100.0, 0.24787521766663584)]:
+        for x, y in [(100.0 + r, 100.0), (100.0 - r, 100.0), (100.0, 100.0 - q * r), (100.0, 100.0 + q * r)]:
+            assert abs(f(x, y) - val) < DELTA, (r, x, y)
+
+
+def test_broken_exponential_with_equal_scale_lengths_is_exponential():
+    """h1 = h2 makes the exponent (1/h1 - 1/h2)/alpha zero, so S = 1 and
+    I = I_0 e^(-r/h1) for any break radius and sharpness; the sub-pixel rules are the
+    same (on h1), so whole images agree with Exponential's (subsampling on)."""
+    a = oracle.OracleModel(threads=8)
+    a.AddFunction("BrokenExponential", True)
+    a.SetupModelImage(64, 48)
+    b = oracle.OracleModel(threads=8)
+    b.AddFunction("Exponential", True)
+    b.SetupModelImage(64, 48)
+    a.CreateModelImage(np.array([30.3, 22.7, 25.0, 0.3, 10.0, 6.0, 6.0, 15.0, 0.7]))
+    b.CreateModelImage(np.array([30.3, 22.7, 25.0, 0.3, 10.0, 6.0]))
+    assert np.array_equal(a.GetModelImageVector(), b.GetModelImageVector())
+
+
 def test_flatsky_known_answers():   # unit_tests/unittest_funcs.t.h:874-905
     for v in (1.0, -1.0):
         for x, y in [(10, 10), (11, 10), (9, 10), (10, 9)]:
