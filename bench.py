@@ -284,6 +284,9 @@ def main():
     ap.add_argument("--no-aux", action="store_true", help="skip the cfg3 / cfg4 side measurements")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (profiling runs: no tuning trials in the trace)")
+    ap.add_argument("--launch-shapes", default=None,
+                    help="main and region shapes 'C,R,C,R' (columns, rows per workgroup) of the FP32 model "
+                         "instead of tuning them (counter runs repeat the shapes a timed run chose)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; sharding the path over several GPUs is a design "
@@ -298,7 +301,13 @@ def main():
     prec = imfit_amd.PRECISION_FP32 if args.precision == "fp32" else imfit_amd.PRECISION_FP64
     m = w.fit_model(prec)
     p = w.start_params()
-    trials = 0 if args.no_autotune else m.Autotune(p)   # launch shapes for this geometry (untimed, once)
+    fixed_shapes = args.no_autotune or args.launch_shapes is not None
+    trials = 0 if fixed_shapes else m.Autotune(p)   # launch shapes for this geometry (untimed, once)
+    if args.launch_shapes is not None:
+        m.GetFitStatistic(p)   # builds the grids
+        v = [int(t) for t in args.launch_shapes.split(",")]
+        for grid in range(len(v) // 2):
+            m.SetLaunchShape(grid, v[2 * grid], v[2 * grid + 1])
     for _ in range(args.warmup):
         m.GetFitStatistic(p)
     device_sync()
@@ -314,8 +323,9 @@ def main():
     ms_per_step = 1e3 * dt / args.steps
     value = args.steps / dt
     log("chi2 = %.10e, %.4f ms/eval, %.1f evals/s" % (stat, ms_per_step, value))
-    # same K evaluations again with HIP event nodes around every kernel (on the
-    # library's stream) for the per-kernel durations
+    # same K evaluations again, eager, with a pair of HIP events carried by every kernel's
+    # dispatch (hipExtLaunchKernel on the library's stream: the kernels' own start and end
+    # timestamps) for the per-kernel durations
     m.SetKernelTiming(True)
     for _ in range(args.warmup):
         m.GetFitStatistic(p)
@@ -333,7 +343,7 @@ def main():
 
     # FP64 parity-mode rate, reported alongside
     m64 = w.fit_model(imfit_amd.PRECISION_FP64)
-    if not args.no_autotune:
+    if not fixed_shapes:
         m64.Autotune(p)
     for _ in range(3):
         m64.GetFitStatistic(p)

@@ -92,6 +92,7 @@ _SIGS["ifg_batch_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig
 _SIGS["ifg_batch_set_graph_mode"] = ([_vp, C.c_int], C.c_int)
 _SIGS["ifg_autotune"] = ([_vp, _dp, C.c_int], C.c_int)
 _SIGS["ifg_get_launch_shape"] = ([_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int)
+_SIGS["ifg_set_launch_shape"] = ([_vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int)
 _SIGS["ifg_batch_autotune"] = ([_vp, _dp, C.POINTER(C.c_int), _dp], C.c_int)
 
 _LIB = C.CDLL(LIB_PATH)
@@ -320,6 +321,10 @@ class ModelObjectGPU:
         if p.shape[1] != self.GetNParams():
             raise ValueError("params must have GetNParams() columns")
         return self._chk(_LIB.ifg_autotune(self.h, _ptr(p), p.shape[0]), "Autotune")
+
+    def SetLaunchShape(self, grid, cols, rows, population=False):
+        """Apply a launch shape (as GetLaunchShape reports it) to grid 0 (main) or a region."""
+        return self._chk(_LIB.ifg_set_launch_shape(self.h, grid, int(population), cols, rows), "SetLaunchShape")
 
     def GetLaunchShape(self, grid=0, population=False):
         """(columns per workgroup of the column pass, rows per workgroup of the row passes)

@@ -455,19 +455,24 @@ void HarvestTimes(ifg_model* m) {
   m->pendingEvents.clear();
 }
 
-// Launch `expr` (a hipError_t-returning launcher call), bracketed by events when
-// the handle's kernel timing is on.
+// Launch `expr` (a hipError_t-returning launcher call); when the handle's kernel
+// timing is on, the launcher's kernels carry a pair of events that take their own
+// dispatch timestamps (a launcher that launches nothing gets an empty interval).
 #define TIMED_ON(strm, name, expr, where)                                     \
   do {                                                                        \
     hipEvent_t _e0 = nullptr, _e1 = nullptr;                                  \
     if (m->timing) {                                                          \
       _e0 = TakeEvent(m);                                                     \
       _e1 = TakeEvent(m);                                                     \
-      (void)hipEventRecord(_e0, strm);                                        \
+      launch_events() = LaunchEvents{_e0, _e1, false};                        \
     }                                                                         \
     hipError_t _err = (expr);                                                 \
     if (m->timing) {                                                          \
-      (void)hipEventRecord(_e1, strm);                                        \
+      if (!launch_events().used) {                                            \
+        (void)hipEventRecord(_e0, strm);                                      \
+        (void)hipEventRecord(_e1, strm);                                      \
+      }                                                                       \
+      launch_events() = LaunchEvents{};                                       \
       m->curEvents->push_back({name, {_e0, _e1}});                            \
     }                                                                         \
     if (_err != hipSuccess) return m->hipfail(_err, where);                   \
@@ -2522,6 +2527,29 @@ int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* col
   const GridShape s = GetShape(m, *cg);
   if (cols_per_block) *cols_per_block = s.C;
   if (rows_per_block) *rows_per_block = s.rows1;
+  return IFG_OK;
+}
+
+int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_block, int rows_per_block) {
+  if (!m || grid < 0) return IFG_ERR_ARG;
+  const int which = batch_grids ? 1 : 0;
+  ConvGrid* cg = nullptr;
+  int split = 1;
+  if (grid == 0) {
+    cg = m->doConv ? &m->mainConv[which] : nullptr;
+  } else if (grid - 1 < (int)m->regions.size()) {
+    cg = &m->regions[grid - 1]->conv[which];
+    split = m->regions[grid - 1]->scale;
+  }
+  if (!cg || !cg->ready) return m->fail(IFG_ERR_STATE, "no such grid yet: evaluate once before setting its shape");
+  GridShape s;
+  s.C = cols_per_block;
+  s.rows1 = rows_per_block;
+  if (!ShapeFits(*cg, s, ElemBytes(m), MaxLds(m), split))
+    return m->fail(IFG_ERR_ARG, "launch shape does not fit this grid (columns 1, 2, 4 or 8; rows within LDS)");
+  (void)hipStreamSynchronize(m->stream);
+  SetShape(m, *cg, s, split);
+  m->plans.clear();   // plans hold the launch shapes
   return IFG_OK;
 }
 

@@ -24,7 +24,10 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <hip/hip_ext.h>
+
 #include "ifg_internal.h"
+#include "ifg_launch.h"
 
 namespace ifg {
 
@@ -1852,14 +1855,33 @@ namespace ifg {
 // two ping-pong row buffers + the twiddle table
 static size_t smem_rows(int rows, int N, size_t elem) { return (2 * (size_t)rows + 1) * N * elem; }
 
+LaunchEvents& launch_events() {
+  static thread_local LaunchEvents ev;
+  return ev;
+}
+
+// One kernel launch; with timing events set, through hipExtLaunchKernel with the
+// start event on the launcher's first kernel and the stop event on its last.
+#define IFG_LAUNCH(first, last, KERNEL, grid, block, lds, s, ...)                                          \
+  do {                                                                                                    \
+    LaunchEvents& ev_ = launch_events();                                                                  \
+    if (ev_.start || ev_.stop) {                                                                          \
+      hipExtLaunchKernelGGL(KERNEL, grid, block, (std::uint32_t)(lds), s, (first) ? ev_.start : nullptr,  \
+                            (last) ? ev_.stop : nullptr, 0u, __VA_ARGS__);                                \
+      ev_.used = true;                                                                                    \
+    } else {                                                                                              \
+      hipLaunchKernelGGL(KERNEL, grid, block, lds, s, __VA_ARGS__);                                       \
+    }                                                                                                     \
+  } while (0)
+
 // Kernel variant for (disk, deriv): DISK carries the 3-D disk quadrature, DERIV the
 // analytic derivative planes; each is compiled only where a launch needs it.
-#define IFG_VARIANT(KERNEL, T, disk, deriv, ...)                                           \
-  do {                                                                                     \
-    if (disk && deriv) hipLaunchKernelGGL((KERNEL<T, true, true>), __VA_ARGS__);         \
-    else if (disk) hipLaunchKernelGGL((KERNEL<T, true, false>), __VA_ARGS__);            \
-    else if (deriv) hipLaunchKernelGGL((KERNEL<T, false, true>), __VA_ARGS__);           \
-    else hipLaunchKernelGGL((KERNEL<T, false, false>), __VA_ARGS__);                     \
+#define IFG_VARIANT(KERNEL, T, disk, deriv, ...)                                    \
+  do {                                                                              \
+    if (disk && deriv) IFG_LAUNCH(true, true, (KERNEL<T, true, true>), __VA_ARGS__);  \
+    else if (disk) IFG_LAUNCH(true, true, (KERNEL<T, true, false>), __VA_ARGS__);     \
+    else if (deriv) IFG_LAUNCH(true, true, (KERNEL<T, false, true>), __VA_ARGS__);    \
+    else IFG_LAUNCH(true, true, (KERNEL<T, false, false>), __VA_ARGS__);              \
   } while (0)
 
 template <typename T>
@@ -1882,7 +1904,7 @@ hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRange
 template <typename T>
 hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds) {
   if (nBlocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_colconv<T>, dim3(nBlocks), dim3(256), lds, s, tasks, br);
+  IFG_LAUNCH(true, true, k_colconv<T>, dim3(nBlocks), dim3(256), lds, s, tasks, br);
   return hipGetLastError();
 }
 
@@ -1897,23 +1919,23 @@ hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRang
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
                          int dir, int rows) {
-  hipLaunchKernelGGL(k_rowfft<T>, dim3((nRows + rows - 1) / rows), dim3(256), smem_rows(rows, Nc, sizeof(cpx<T>)), s,
-                     buf, nRows, Nc, pl, tw, dir, rows);
+  IFG_LAUNCH(true, true, k_rowfft<T>, dim3((nRows + rows - 1) / rows), dim3(256), smem_rows(rows, Nc, sizeof(cpx<T>)),
+             s, buf, nRows, Nc, pl, tw, dir, rows);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C) {
   const int ld = Nr + 1;
-  hipLaunchKernelGGL(k_colfft<T>, dim3((Nc + C - 1) / C), dim3(256), (2 * (size_t)C * ld + Nr) * sizeof(cpx<T>), s,
-                     buf, Nr, Nc, pl, tw, C, ld);
+  IFG_LAUNCH(true, true, k_colfft<T>, dim3((Nc + C - 1) / C), dim3(256), (2 * (size_t)C * ld + Nr) * sizeof(cpx<T>),
+             s, buf, Nr, Nc, pl, tw, C, ld);
   return hipGetLastError();
 }
 
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
                         int mode, double* out, int hostOut) {
   if (nJobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut);
+  IFG_LAUNCH(true, true, k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut);
   return hipGetLastError();
 }
 
@@ -1924,11 +1946,11 @@ hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobLis
   if (nJobs <= 0 || nTiles <= 0) return hipSuccess;
   size_t lds = 0;   // the largest job's tile: (nFree + 1) * TP doubles
   for (int f = 1; f <= maxFree; f++) lds = std::max(lds, (size_t)(f + 1) * normal_tile_pixels(f) * sizeof(double));
-  hipLaunchKernelGGL(k_normal_tile, dim3(nTiles), dim3(256), lds, s, jobs, tileMap, partials);
+  IFG_LAUNCH(true, false, k_normal_tile, dim3(nTiles), dim3(256), lds, s, jobs, tileMap, partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int maxEnt = maxFree * (maxFree + 1) / 2 + maxFree;
-  hipLaunchKernelGGL(k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, jobList, partials);
+  IFG_LAUNCH(false, true, k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, jobList, partials);
   return hipGetLastError();
 }
 

@@ -3,6 +3,16 @@
 #include "ifg_internal.h"
 
 namespace ifg {
+// Kernel timing: when `start` / `stop` are set, the launchers below hand them to the
+// first / last kernel they launch through hipExtLaunchKernel, so the events carry that
+// dispatch's own start and end timestamps; `used` reports whether a kernel was
+// launched at all.  Unset (the default), launches are plain and graph-capturable.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool used = false;
+};
+LaunchEvents& launch_events();   // this host thread's
+
 // disk: the model contains an ExponentialDisk3D (selects the kernel variants that
 // carry its line-of-sight quadrature); deriv: the planes are derivative planes.
 // br: the launch's block -> task lookup.

@@ -244,6 +244,11 @@ int ifg_autotune(ifg_model* m, const double* params, int n_vectors);
  * single-evaluation grids (batch_grids 0) or the population grids (1). */
 int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* cols_per_block,
                          int* rows_per_block);
+/* Set the launch shape of a grid as ifg_get_launch_shape reports it, e.g. a shape
+ * found by ifg_autotune in an earlier process.  The grid must exist (one evaluation
+ * with the same n_vectors class first).  IFG_ERR_ARG if the shape does not fit.
+ * No counterpart in the reference (its CPU path has no launch shapes). */
+int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_block, int rows_per_block);
 /* As ifg_autotune for a batch's ifg_batch_fit_statistic: one strip width and one
  * multiplier of each grid's default rows per workgroup for every model (grids where
  * the choice does not fit keep their defaults).  Returns the number of trials. */

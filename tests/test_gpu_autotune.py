@@ -68,3 +68,25 @@ def test_batch_autotune_keeps_statistics():
     # every model alone still agrees with the batch
     single = np.array([m.GetFitStatistic(p) for m, p in zip(models, starts)])
     assert np.allclose(single, got, rtol=1e-12, atol=0)
+
+
+def test_set_launch_shape():
+    """A shape set by hand is the one reported and used (statistic unchanged to the
+    reduction's rounding); shapes that do not fit are rejected with an error."""
+    import imfit_amd
+    announce("set launch shape: n3073, FP32")
+    case = cases.n3073()[0]
+    m = case.build(imfit_amd.ModelObjectGPU(0, imfit_amd.PRECISION_FP32))
+    p = case.params.copy()
+    s0 = m.GetFitStatistic(p)
+    for cols, rows in [(1, 1), (8, 2), (2, 1)]:
+        m.SetLaunchShape(0, cols, rows)
+        assert m.GetLaunchShape(0) == (cols, rows)
+        s = m.GetFitStatistic(p)
+        assert abs(s - s0) <= 1e-12 * abs(s0), (cols, rows, s, s0)
+    with pytest.raises(imfit_amd.IfgError):
+        m.SetLaunchShape(0, 3, 1)        # not a power of two
+    with pytest.raises(imfit_amd.IfgError):
+        m.SetLaunchShape(0, 1, 100000)   # rows beyond LDS
+    with pytest.raises(imfit_amd.IfgError):
+        m.SetLaunchShape(5, 1, 1)        # no such grid
