@@ -92,6 +92,12 @@ _SIGS["ifg_batch_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig
 _SIGS["ifg_batch_set_graph_mode"] = ([_vp, C.c_int], C.c_int)
 _SIGS["ifg_autotune"] = ([_vp, _dp, C.c_int], C.c_int)
 _SIGS["ifg_get_launch_shape"] = ([_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int)
+_SIGS["ifg_init_genrand"] = ([C.c_ulong], None)
+_SIGS["ifg_use_bootstrap"] = ([_vp], C.c_int)
+_SIGS["ifg_make_bootstrap_sample"] = ([_vp], C.c_int)
+_SIGS["ifg_set_bootstrap_indices"] = ([_vp, C.POINTER(C.c_long), C.c_long], C.c_int)
+_SIGS["ifg_get_bootstrap_indices"] = ([_vp, C.POINTER(C.c_long)], C.c_int)
+_SIGS["ifg_stop_bootstrap"] = ([_vp], C.c_int)
 _SIGS["ifg_set_launch_shape"] = ([_vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int)
 _SIGS["ifg_batch_autotune"] = ([_vp, _dp, C.POINTER(C.c_int), _dp], C.c_int)
 
@@ -119,6 +125,11 @@ def _ptr(a):
 
 class IfgError(RuntimeError):
     pass
+
+
+def init_genrand(seed):
+    """Seed the process-wide MT19937 that bootstrap samples are drawn from (imfit --seed)."""
+    _LIB.ifg_init_genrand(int(seed))
 
 
 class ModelObjectGPU:
@@ -203,6 +214,27 @@ class ModelObjectGPU:
 
     def GetNValidPixels(self):
         return _LIB.ifg_get_nvalid(self.h)
+
+    # ---- bootstrap resampling (core/model_object.cpp:2036-2087)
+    def UseBootstrap(self):
+        return self._chk(_LIB.ifg_use_bootstrap(self.h), "UseBootstrap")
+
+    def MakeBootstrapSample(self):
+        return self._chk(_LIB.ifg_make_bootstrap_sample(self.h), "MakeBootstrapSample")
+
+    def SetBootstrapIndices(self, indices):
+        idx = np.ascontiguousarray(indices, dtype=np.int64)
+        return self._chk(_LIB.ifg_set_bootstrap_indices(self.h, idx.ctypes.data_as(C.POINTER(C.c_long)), idx.size),
+                         "SetBootstrapIndices")
+
+    def GetBootstrapIndices(self):
+        out = np.empty(self.GetNValidPixels(), dtype=np.int64)
+        self._chk(_LIB.ifg_get_bootstrap_indices(self.h, out.ctypes.data_as(C.POINTER(C.c_long))),
+                  "GetBootstrapIndices")
+        return out
+
+    def StopBootstrap(self):
+        return self._chk(_LIB.ifg_stop_bootstrap(self.h), "StopBootstrap")
 
     # ---- hot path
     def CreateModelImage(self, params):

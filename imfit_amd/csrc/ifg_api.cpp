@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -377,6 +378,10 @@ struct ifg_model {
   int nCombined = 1;
   bool maskExists = false, externalErrors = false, weightSet = false;
   bool useCash = false, poissonMLR = false, modelErrors = false, dataErrors = true;
+  // bootstrap resampling: the drawn data-pixel indices and how often each pixel was drawn
+  bool bootstrap = false;
+  std::vector<long> bootIdx;
+  std::vector<double> bootCount;
   std::vector<std::unique_ptr<Region>> regions;
   // device state
   bool devReady = false;
@@ -659,17 +664,37 @@ int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cL
   return off;
 }
 
+// Weight of data pixel z as the device uses it.  A bootstrap sample sums each drawn
+// pixel's term once per draw (core/model_object.cpp:1443-1454,1651-1683,1715-1760);
+// with c draws of pixel z that is c times its term, so the weight carries sqrt(c)
+// (chi^2: the deviate w (d - m) is squared) or c (Cash and Poisson-MLR terms are
+// linear in w).
+double EffectiveWeight(const ifg_model* m, long z) {
+  if (!m->bootstrap) return m->weight[z];
+  const double c = m->bootCount[z];
+  return m->useCash ? m->weight[z] * c : m->weight[z] * std::sqrt(c);
+}
+
+template <typename T>
+int UploadWeights(ifg_model* m) {
+  const long n = m->nData;
+  std::vector<T> tmp(n);
+  HIPCHK(m->weight_d.ensure(n * sizeof(T)), "alloc weights");
+  for (long z = 0; z < n; z++) tmp[z] = (T)EffectiveWeight(m, z);
+  HIPCHK(hipMemcpy(m->weight_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload weights");
+  return IFG_OK;
+}
+
 template <typename T>
 int UploadImageArrays(ifg_model* m) {
   const long n = m->nData;
   std::vector<T> tmp(n);
   HIPCHK(m->data_d.ensure(n * sizeof(T)), "alloc data");
-  HIPCHK(m->weight_d.ensure(n * sizeof(T)), "alloc weights");
   HIPCHK(m->extra_d.ensure(n * sizeof(T)), "alloc extra terms");
   for (long z = 0; z < n; z++) tmp[z] = (T)m->data[z];
   HIPCHK(hipMemcpy(m->data_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload data");
-  for (long z = 0; z < n; z++) tmp[z] = (T)m->weight[z];
-  HIPCHK(hipMemcpy(m->weight_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload weights");
+  int st = UploadWeights<T>(m);
+  if (st < 0) return st;
   for (long z = 0; z < n; z++) tmp[z] = (T)m->extra[z];
   HIPCHK(hipMemcpy(m->extra_d.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload extra terms");
   return IFG_OK;
@@ -1538,6 +1563,87 @@ int ifg_which_fit_statistic(const ifg_model* m) {
 
 long ifg_get_ndata(const ifg_model* m) { return m->nData; }
 long ifg_get_nvalid(const ifg_model* m) { return m->nValid; }
+
+// ---- bootstrap resampling (core/model_object.cpp:2036-2087, core/bootstrap_errors.cpp:150-232)
+// The reference draws samples from one process-wide Mersenne Twister (MT19937,
+// core/mersenne_twister.cpp), seeded by init_genrand; std::mt19937 is that generator
+// (same recurrence, seeding and tempering), and 5489 is its seed when none is given.
+static std::mutex gRngMu;
+static std::mt19937 gRng(5489u);
+
+void ifg_init_genrand(unsigned long seed) {
+  std::lock_guard<std::mutex> lock(gRngMu);
+  gRng.seed((std::mt19937::result_type)(seed & 0xffffffffUL));
+}
+
+static int SetBootstrapSample(ifg_model* m, std::vector<long>&& idx) {
+  m->bootIdx = std::move(idx);
+  m->bootCount.assign(m->nData, 0.0);
+  for (long b : m->bootIdx) m->bootCount[b] += 1.0;
+  m->bootstrap = true;
+  if (!m->devReady) return IFG_OK;   // PrepareDevice uploads the weights
+  HIPCHK(hipStreamSynchronize(m->stream), "bootstrap weights");
+  return m->precision == IFG_PRECISION_FP64 ? UploadWeights<double>(m) : UploadWeights<float>(m);
+}
+
+static int BootstrapReady(ifg_model* m) {
+  if (!m->finalDone) return m->fail(IFG_ERR_STATE, "call ifg_final_setup before bootstrap resampling");
+  if (m->modelErrors)
+    return m->fail(IFG_ERR_UNSUPPORTED, "bootstrap resampling with weights from the model is not supported");
+  return IFG_OK;
+}
+
+int ifg_make_bootstrap_sample(ifg_model* m) {
+  if (!m) return IFG_ERR_ARG;
+  int st = BootstrapReady(m);
+  if (st < 0) return st;
+  std::vector<long> idx(m->nValid);
+  {   // nValid draws of floor(genrand_real2() * nData), skipping pixels of weight 0
+    std::lock_guard<std::mutex> lock(gRngMu);
+    for (long i = 0; i < m->nValid; i++) {
+      long n;
+      do {
+        n = (long)std::floor((double)gRng() * (1.0 / 4294967296.0) * m->nData);
+      } while (!(m->weight[n] > 0.0));
+      idx[i] = n;
+    }
+  }
+  return SetBootstrapSample(m, std::move(idx));
+}
+
+int ifg_use_bootstrap(ifg_model* m) {
+  return ifg_make_bootstrap_sample(m);   // UseBootstrap draws a first sample
+}
+
+int ifg_set_bootstrap_indices(ifg_model* m, const long* indices, long n) {
+  if (!m || (!indices && n > 0)) return IFG_ERR_ARG;
+  int st = BootstrapReady(m);
+  if (st < 0) return st;
+  if (n != m->nValid) return m->fail(IFG_ERR_ARG, "a bootstrap sample has one index per valid data pixel");
+  std::vector<long> idx(indices, indices + n);
+  for (long b : idx)
+    if (b < 0 || b >= m->nData || !(m->weight[b] > 0.0))
+      return m->fail(IFG_ERR_ARG, "bootstrap index outside the image or on a masked pixel");
+  return SetBootstrapSample(m, std::move(idx));
+}
+
+int ifg_get_bootstrap_indices(const ifg_model* m, long* indices) {
+  if (!m || !indices) return IFG_ERR_ARG;
+  if (!m->bootstrap) return IFG_ERR_STATE;
+  std::copy(m->bootIdx.begin(), m->bootIdx.end(), indices);
+  return IFG_OK;
+}
+
+int ifg_stop_bootstrap(ifg_model* m) {
+  if (!m) return IFG_ERR_ARG;
+  if (!m->bootstrap) return IFG_OK;
+  m->bootstrap = false;
+  m->bootIdx.clear();
+  m->bootCount.clear();
+  if (!m->devReady) return IFG_OK;
+  HIPCHK(hipStreamSynchronize(m->stream), "bootstrap weights");
+  return m->precision == IFG_PRECISION_FP64 ? UploadWeights<double>(m) : UploadWeights<float>(m);
+}
 
 int ifg_create_model_image(ifg_model* m, const double* params) {
   m->lastParams.assign(params, params + m->nParamsTot);

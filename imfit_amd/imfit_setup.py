@@ -25,7 +25,7 @@ no data, image size from NCOLS/NROWS or the arguments.
 import numpy as np
 
 from . import (MASK_ZERO_IS_BAD, MASK_ZERO_IS_GOOD, PRECISION_FP64, WEIGHTS_ARE_SIGMAS, WEIGHTS_ARE_VARIANCES,
-               WEIGHTS_ARE_WEIGHTS, ModelObjectGPU)
+               WEIGHTS_ARE_WEIGHTS, BatchGPU, ModelObjectGPU, init_genrand)
 from .config import ConfigError, ReadConfigFile, SplitString
 from .fits import DetermineImageOffset, FitsError, read_image, write_image
 
@@ -127,12 +127,62 @@ class ImfitSetup:
     def fit_statistic(self, params=None):
         return self.model.GetFitStatistic(self.params if params is None else params)
 
+    def to_data_frame(self, params):
+        """Inverse of to_file_frame: X0/Y0 relative to the data image (section)."""
+        p = np.array(params, dtype=np.float64)
+        xs, ys = self._xy()
+        p[xs] -= self.offsets[0]
+        p[ys] -= self.offsets[1]
+        return p
+
     def fit(self, ftol=1e-8, method="mpfit", jacobian="fd", **kw):
         """Levenberg-Marquardt from the configuration's values, limits and fixed flags.
         Returns (best parameters in the file frame, LMResult)."""
         x, res = self.model.FitLM(self.params, limits=self.limits, fixed=self.fixed, ftol=ftol, method=method,
                                   jacobian=jacobian, **kw)
         return self.to_file_frame(x), res
+
+    def clone(self):
+        """A second, independent GPU model set up from the same files and options."""
+        return imfit_setup(**self._kwargs)
+
+    def bootstrap(self, best, n_iterations, seed, ftol=1e-8, batched=False):
+        """Bootstrap resampling of the fit (core/bootstrap_errors.cpp:150-232): seed the
+        generator, UseBootstrap (which draws a first sample), then per iteration a new
+        sample and an L-M fit from `best` (file frame).  Returns the parameters of the
+        successful fits in the file frame, one row per iteration.
+
+        batched=False fits the samples one after another with mpfit and a finite-
+        difference Jacobian, as imfit does.  batched=True draws the same samples into
+        one model each (clones of this setup) and fits them all at once with the
+        normal-equation LM of BatchGPU."""
+        start = self.to_data_frame(best)
+        rows = []
+        if not batched:
+            init_genrand(seed)
+            self.model.UseBootstrap()
+            try:
+                for _ in range(n_iterations):
+                    self.model.MakeBootstrapSample()
+                    x, res = self.model.FitLM(start, limits=self.limits, fixed=self.fixed, ftol=ftol)
+                    if res.status > 0:
+                        rows.append(self.to_file_frame(x))
+            finally:
+                self.model.StopBootstrap()
+            return np.array(rows)
+        clones = [self.clone() for _ in range(n_iterations)]
+        models = [c.model for c in clones]
+        init_genrand(seed)
+        models[0].UseBootstrap()
+        for m in models:
+            m.MakeBootstrapSample()
+        batch = BatchGPU(models)
+        xs, results = batch.FitLM([start] * n_iterations, limits=[self.limits] * n_iterations,
+                                  fixed=[self.fixed] * n_iterations, ftol=ftol)
+        for x, res in zip(xs, results):
+            if res.status > 0:
+                rows.append(self.to_file_frame(x))
+        return np.array(rows)
 
 
 def imfit_setup(config, image, psf=None, noise=None, mask=None, errors_are_variances=False,
@@ -143,6 +193,7 @@ def imfit_setup(config, image, psf=None, noise=None, mask=None, errors_are_varia
     """Configure a ModelObjectGPU from a configuration file and FITS file names as
     imfit would; returns an ImfitSetup.  Raises ConfigError, SetupError or the
     library's IfgError where imfit prints an error and exits."""
+    kwargs = dict(locals())
     cfg = ReadConfigFile(config, mode2D=True, with_limits=True)
     chars = cfg.imfit_options(gain=gain, readnoise=readnoise, exptime=exptime, ncombined=ncombined,
                               original_sky=sky)
@@ -186,7 +237,9 @@ def imfit_setup(config, image, psf=None, noise=None, mask=None, errors_are_varia
             cfg.limits[k].offset = off
             if limits[k] is not None:
                 limits[k] = (limits[k][0] - off, limits[k][1] - off)
-    return ImfitSetup(m, cfg, params, limits, cfg.fixed(), (xoff, yoff), chars)
+    s = ImfitSetup(m, cfg, params, limits, cfg.fixed(), (xoff, yoff), chars)
+    s._kwargs = kwargs
+    return s
 
 
 class MakeimageSetup:

@@ -105,6 +105,31 @@ int ifg_which_fit_statistic(const ifg_model* m);
 long ifg_get_ndata(const ifg_model* m);
 long ifg_get_nvalid(const ifg_model* m);
 
+/* ---- bootstrap resampling ------------------------------------------------- */
+/* init_genrand (core/mersenne_twister.cpp:60-76): seed the process-wide MT19937
+ * that bootstrap samples are drawn from, as core/bootstrap_errors.cpp:174-178 does
+ * with imfit's --seed. */
+void ifg_init_genrand(unsigned long seed);
+/* UseBootstrap (core/model_object.cpp:2042-2053): bootstrap mode on, with a first
+ * sample drawn.  After ifg_final_setup; IFG_ERR_UNSUPPORTED with model errors. */
+int ifg_use_bootstrap(ifg_model* m);
+/* MakeBootstrapSample (core/model_object.cpp:2061-2087): nValid draws of
+ * floor(genrand_real2() * nData), masked pixels redrawn.  The statistic then sums
+ * each drawn pixel's term once per draw (as ChiSquared / CashStatistic and
+ * ComputeDeviates do over the index list, core/model_object.cpp:1443-1454,
+ * 1651-1683, 1715-1760).  ComputeDeviates and the Jacobian stay per data pixel:
+ * each pixel's deviate is multiplied by sqrt(times drawn), so their sums of
+ * squares, J^T J and the fits are those of the resampled data. */
+int ifg_make_bootstrap_sample(ifg_model* m);
+/* The same with a caller's sample: n = nValid data-pixel indices, each unmasked.
+ * No counterpart in the reference (its samples come from its own generator). */
+int ifg_set_bootstrap_indices(ifg_model* m, const long* indices, long n);
+/* The current sample's nValid indices, in draw order.  No counterpart in the reference. */
+int ifg_get_bootstrap_indices(const ifg_model* m, long* indices);
+/* Back to the full data set.  No counterpart in the reference (a ModelObject stays
+ * in bootstrap mode once UseBootstrap is called). */
+int ifg_stop_bootstrap(ifg_model* m);
+
 /* ---- hot path ------------------------------------------------------------- */
 /* CreateModelImage (core/model_object.cpp:1103-1229) */
 int ifg_create_model_image(ifg_model* m, const double* params);

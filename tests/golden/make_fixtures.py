@@ -26,6 +26,8 @@ Each fixture records which reference test produced it:
   twogauss_osamp do_imfit_tests:107 -> imfit_textout4e3
   conv_n5419     do_psfconvolve_tests:11,60 -> tests/convolve_out_n5419.fits
   conv_5x5       do_psfconvolve_tests:8-9 -> tests/psfconvolve_out{1,2}.txt (padded PSF)
+  bootstrap_ic3478  do_imfit_tests:133-134 -> bootstrap_output_seed10_tail.dat,
+                 bootstrap_output_seed10_2_tail.dat (seed 10, whole image and section)
 """
 import os
 import re
@@ -225,6 +227,7 @@ def main():
         padded_psf_delta=printed_block(t("psfconvolve_out1.txt"),
                                        "The whole padded, normalized PSF image"),
     )
+    bootstrap_fixture()
     copy_inputs()
     for fn in sorted(os.listdir(OUT)):
         if fn.endswith(".npz"):
@@ -270,6 +273,20 @@ CONFIG_INPUTS = [
     "config_makeimage-optional_params2.dat", "config_makeimage-optional_params3.dat",
     "config_makeimage-optional_params_bad1.dat",
 ]
+
+
+def bootstrap_fixture():
+    """do_imfit_tests:133-134,311-324: five bootstrap fits (seed 10) of the ic3478 fit with
+    imfit_config_ic3478_64x64b.dat, on the whole image and on the section
+    [10:64,10:64]; the archived tails list the fitted parameters, one fit per row."""
+    def rows(path):
+        with open(path) as f:
+            return np.array([[float(v) for v in line.split()] for line in f
+                             if line.strip() and not line.startswith("#")])
+    lin = "imfit_reference/linux_x86_64/"
+    np.savez_compressed(os.path.join(OUT, "bootstrap_ic3478.npz"),
+                        seed10=rows(t(lin + "bootstrap_output_seed10_tail.dat")),
+                        seed10_section=rows(t(lin + "bootstrap_output_seed10_2_tail.dat")))
 
 
 def copy_inputs():
