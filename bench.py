@@ -215,7 +215,53 @@ def aux_benchmarks(steps):
     log("cfg3 (4096-vector population, 512^2, PSF 35, FP32): %.3f s per batch, %.0f chi^2/s" %
         (dt, pop.shape[0] / dt))
     out["cfg5_cutouts"] = cutout_benchmark()
+    out["bootstrap_cfg1"] = bootstrap_benchmark()
     return out
+
+
+def bootstrap_benchmark(n=256, n_serial=4, seed=10):
+    """Bootstrap error estimation on the 256^2 Sersic+Exp+sky image (cfg1): the main fit,
+    then n resamples (imfit's draw rule, MT19937 seeded with `seed`) fitted from its best
+    values -- all at once by the batched normal-equation LM (one model per sample),
+    versus imfit's loop (one mpfit fit with a finite-difference Jacobian per sample, on
+    the GPU) on the first n_serial samples; chi^2 compared on those."""
+    import imfit_amd
+    from imfit_amd import workloads
+    w = workloads.cfg1()
+    w.make_data()
+    lim = w.lm_limits()
+    m = w.fit_model(imfit_amd.PRECISION_FP64)
+    best, _ = m.FitLM(w.lm_start(seed=1), limits=lim, ftol=1e-10, method="normal")
+    models = [w.fit_model(imfit_amd.PRECISION_FP64) for _ in range(n)]
+    imfit_amd.init_genrand(seed)
+    models[0].UseBootstrap()
+    for mk in models:
+        mk.MakeBootstrapSample()
+    b = imfit_amd.BatchGPU(models)
+    b.GetFitStatistic([best] * n)   # plans and graphs
+    device_sync()
+    t0 = time.perf_counter()
+    xb, rb = b.FitLM([best] * n, limits=[lim] * n, ftol=1e-10)
+    dt = time.perf_counter() - t0
+    imfit_amd.init_genrand(seed)
+    m.UseBootstrap()
+    rel = []
+    t1 = time.perf_counter()
+    for k in range(n_serial):
+        m.MakeBootstrapSample()
+        x, r = m.FitLM(best, limits=lim, ftol=1e-10, jacobian="fd")
+        c_serial, c_batch = m.GetFitStatistic(x), m.GetFitStatistic(xb[k])
+        rel.append(abs(c_batch - c_serial) / c_serial)
+    dts = time.perf_counter() - t1
+    res = {"samples": n, "image": "256x256, no PSF", "batched_fits_per_s": n / dt,
+           "batched_mean_niter": float(np.mean([r.niter for r in rb])),
+           "serial_fd_mpfit_fits_per_s": n_serial / dts, "serial_sample": n_serial,
+           "chi2_rel_diff_max_on_sample": float(max(rel)), "speedup": (n / dt) / (n_serial / dts),
+           "param_spread_rel": [float(v) for v in np.std(np.array(xb), axis=0) / np.abs(best)]}
+    log("bootstrap (cfg1, %d resamples): batched %.1f fits/s, serial mpfit %.2f fits/s (x%.0f); "
+        "chi2 rel diff on the sample <= %.2e" % (n, n / dt, n_serial / dts, res["speedup"], max(rel)))
+    del b, models
+    return res
 
 
 def cutout_benchmark(n=2000, n_plain=10):
