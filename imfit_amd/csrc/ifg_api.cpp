@@ -63,6 +63,7 @@ int NParamsOf(int type) {
     case F_EXPDISK3D: return 6;
     case F_GENSERSIC: return 6;
     case F_BROKENEXP: return 7;
+    case F_EDGEONDISK: return 5;
   }
   return 0;
 }
@@ -77,12 +78,13 @@ int TypeOf(const char* name) {
   if (s == "ExponentialDisk3D") return F_EXPDISK3D;
   if (s == "Sersic_GenEllipse") return F_GENSERSIC;
   if (s == "BrokenExponential") return F_BROKENEXP;
+  if (s == "EdgeOnDisk") return F_EDGEONDISK;
   return 0;
 }
 
 bool Subsampling(int type) {
   return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT || type == F_GENSERSIC ||
-         type == F_BROKENEXP;
+         type == F_BROKENEXP || type == F_EDGEONDISK;
 }
 
 // FunctionObject::Setup for one component (func_sersic.cpp:109-127, func_exp.cpp:104-118,
@@ -96,6 +98,15 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
   v[1] = y0;
   if (type == F_FLATSKY) {
     v[5] = p[0];
+    return;
+  }
+  if (type == F_EDGEONDISK) {   // EdgeOnDisk::Setup (func_edge-on-disk.cpp:95-116): PA, L_0, h, n, z_0
+    const double PA = p[0], L_0 = p[1], h = p[2], n = p[3], z_0 = p[4];
+    const double PA_rad = (PA + 90.0) * DEG2RAD;
+    const double alpha = 2.0 / n;
+    v[2] = cos(PA_rad); v[3] = sin(PA_rad); v[4] = 1.0; v[15] = 1.0;
+    v[5] = 2.0 * h * L_0; v[6] = h; v[7] = n; v[8] = alpha; v[9] = z_0; v[10] = alpha * z_0;
+    v[11] = pow(2.0, alpha); v[13] = L_0;
     return;
   }
   if (type == F_EXPDISK3D) {   // ExponentialDisk3D::Setup (func_expdisk3d.cpp:149-172)
@@ -432,6 +443,15 @@ struct ifg_model {
     if (_e != hipSuccess) return m->hipfail(_e, where); \
   } while (0)
 
+// The kernels' DISK variants carry the functions with heavy per-sample code (the 3-D
+// disk's line-of-sight quadrature, the edge-on disk's Bessel function); models
+// without them run kernels that contain none of it.
+bool HeavyVariant(const ifg_model* m) {
+  for (int t : m->types)
+    if (t == F_EXPDISK3D || t == F_EDGEONDISK) return true;
+  return false;
+}
+
 namespace {
 
 hipEvent_t TakeEvent(ifg_model* m) {
@@ -623,6 +643,38 @@ int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cL
       if (!Subsampling(m->types[k])) continue;
       const double x0 = recs[k].v[0], y0 = recs[k].v[1];
       if (!std::isfinite(x0) || !std::isfinite(y0)) continue;
+      if (m->types[k] == F_EDGEONDISK) {   // pixels within 10 of either axis (func_edge-on-disk.cpp:225)
+        for (int axis = 0; axis < 2; axis++) {
+          // |u x_d + w y_d| < 10 with (u, w) = (cos, sin) (along R) or (-sin, cos) (along z):
+          // per grid row, the columns where it holds; their bounding box, widened by a pixel
+          const double c = recs[k].v[2], sn = recs[k].v[3];
+          const double u = axis == 0 ? c : -sn, wv = axis == 0 ? sn : c;
+          const double L = 10.0 + 2.0 * g.frac;
+          int bc0 = cHi + 1, bc1 = cLo - 1, br0 = rHi + 1, br1 = rLo - 1;
+          for (int r = rLo; r <= rHi; r++) {
+            const double yd = g.ya + r * g.frac - y0;
+            double xlo, xhi;
+            if (std::fabs(u) < 1e-12) {
+              if (std::fabs(wv * yd) >= L) continue;
+              xlo = g.xa + cLo * g.frac; xhi = g.xa + cHi * g.frac;
+            } else {
+              const double a = (-L - wv * yd) / u + x0, b = (L - wv * yd) / u + x0;
+              xlo = std::min(a, b); xhi = std::max(a, b);
+            }
+            const double fc0 = std::max(std::floor((xlo - g.xa) / g.frac) - 1, (double)cLo);
+            const double fc1 = std::min(std::ceil((xhi - g.xa) / g.frac) + 1, (double)cHi);
+            if (fc0 > fc1) continue;
+            bc0 = std::min(bc0, (int)fc0); bc1 = std::max(bc1, (int)fc1);
+            br0 = std::min(br0, r); br1 = std::max(br1, r);
+          }
+          if (bc0 > bc1) continue;
+          B b = {bc0, bc1, std::max(br0 - 1, rLo), std::min(br1 + 1, rHi)};
+          bool dup = false;
+          for (auto& o : bs) dup = dup || (o.c0 == b.c0 && o.c1 == b.c1 && o.r0 == b.r0 && o.r1 == b.r1);
+          if (!dup) bs.push_back(b);
+        }
+        continue;
+      }
       double rad = 10.0;
       if (m->types[k] == F_GENSERSIC) {
         const double e = recs[k].v[11];
@@ -1137,7 +1189,7 @@ int Enqueue(ifg_model* m, Plan& P) {
   const bool deriv = IsDeriv(P.flags);
   const int mode = (P.flags & PF_JAC) ? S_JAC : (P.flags & PF_GRAD) ? S_GRAD : m->statMode();
   const int subsampling = m->doSubsampling ? 1 : 0;
-  const bool disk = std::find(m->types.begin(), m->types.end(), (int)F_EXPDISK3D) != m->types.end();
+  const bool disk = HeavyVariant(m);
   const long* prefix_d = reinterpret_cast<const long*>(P.stage_d.as<char>() + P.prefixOff);
   const bool reduce = (mode != S_NONE && mode != S_JAC && (P.flags & PF_STAT));
   StatLaunch sl;
@@ -1881,7 +1933,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
       tailCount += (size_t)sm * nv;
       if (g.Nc > 0) specBytes += (size_t)g.Nr * g.Nc * elem * nPairs;
     }
-    P.disk = P.disk || std::find(m->types.begin(), m->types.end(), (int)F_EXPDISK3D) != m->types.end();
+    P.disk = P.disk || HeavyVariant(m);
     P.recOff[k] = stage;
     stage += (size_t)m->nComp() * sizeof(CompRec);
     P.boxOff[k] = stage;

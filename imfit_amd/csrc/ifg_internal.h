@@ -12,7 +12,8 @@
 namespace ifg {
 
 enum FuncType {
-  F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6, F_GENSERSIC = 7, F_BROKENEXP = 8
+  F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6, F_GENSERSIC = 7, F_BROKENEXP = 8,
+  F_EDGEONDISK = 9
 };
 
 // S_JAC: write Jacobian columns dfac * dm/dp_j;  S_GRAD: accumulate sum(gfac * dm/dp_j)
@@ -31,6 +32,7 @@ struct cpx {
 //   Exponential: v[5] I_0  v[6] h
 //   BrokenExponential: v[5] I_0 S  v[6] h1  v[7] h2  v[8] r_b  v[9] alpha  v[10] exponent
 //                      v[11] r_b/h2 - r_b/h1  v[12] -log2(e)/h1  v[13] S  v[14] ln(1 + e^(-alpha r_b))
+//   EdgeOnDisk:  v[5] 2 h L_0  v[6] h  v[7] n  v[8] 2/n  v[9] z_0  v[10] (2/n) z_0  v[11] 2^(2/n)  v[13] L_0
 //   Gaussian:    v[5] I_0  v[6] sigma  v[7] 2 sigma^2
 //   Moffat:      v[5] I_0  v[6] alpha  v[7] beta
 //   FlatSky:     v[5] I_sky

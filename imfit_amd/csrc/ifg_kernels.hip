@@ -418,6 +418,97 @@ __device__ __forceinline__ float radius2_f32(int type, const double* v, float xp
   return r * r;
 }
 
+// ------------------------------------------------------------------ EdgeOnDisk
+// Modified Bessel functions (K_0(x), K_1(x)) for x > 0; the reference calls GSL's
+// gsl_sf_bessel_K1 (func_edge-on-disk.cpp:168).  x <= 2: the power series of
+// Abramowitz & Stegun 9.6.13 (K_0) and 9.6.11 (K_1) with I_0, I_1 summed alongside;
+// x > 2: Steed's continued fraction for K_0 and K_1 (Thompson & Barnett 1986, with
+// Temme's normalisation sum).  Both hold ~2e-15 relative accuracy.
+__device__ __forceinline__ double2 bessel_k01(double x) {
+  const double EULER = 0.57721566490153286061;
+  if (x <= 2.0) {
+    const double y = 0.25 * x * x, lx = log(0.5 * x);
+    double t0 = 1.0, t1 = 1.0, H = 0.0, I0 = 1.0, I1s = 1.0, S0 = 0.0, S1 = 1.0 - 2.0 * EULER;
+    for (int k = 1; k < 40; k++) {
+      t0 *= y / (double)(k * k);
+      t1 *= y / (double)(k * (k + 1));
+      H += 1.0 / k;
+      I0 += t0;
+      I1s += t1;
+      S0 += H * t0;
+      S1 += (2.0 * H + 1.0 / (k + 1) - 2.0 * EULER) * t1;   // psi(k+1) + psi(k+2)
+      if (t0 < 1e-17 * I0) break;
+    }
+    const double k0 = -(lx + EULER) * I0 + S0;
+    const double k1 = 1.0 / x + lx * (0.5 * x * I1s) - 0.25 * x * S1;
+    return make_double2(k0, k1);
+  }
+  double b = 2.0 * (1.0 + x), d = 1.0 / b, h = d, delh = d;
+  double q1 = 0.0, q2 = 1.0, q = 0.25, c = 0.25, a = -0.25, sum = 1.0 + q * delh;
+  for (int i = 2; i < 200; i++) {
+    a -= 2.0 * (i - 1);
+    c = -a * c / i;
+    const double qnew = (q1 - b * q2) / a;
+    q1 = q2;
+    q2 = qnew;
+    q += c * qnew;
+    b += 2.0;
+    d = 1.0 / (b + a * d);
+    delh = (b * d - 1.0) * delh;
+    h += delh;
+    const double dels = q * delh;
+    sum += dels;
+    if (fabs(dels) < 1e-16 * fabs(sum)) break;
+  }
+  h *= 0.25;
+  const double k0 = sqrt(3.14159265358979323846 / (2.0 * x)) * exp(-x) / sum;
+  return make_double2(k0, k0 * (x + 0.5 - h) / x);
+}
+
+// EdgeOnDisk (func_edge-on-disk.cpp:118-246): an exponential disk seen edge-on,
+// Sigma_00 (R/h) K_1(R/h) sech^alpha(z / (alpha z_0)) with alpha = 2/n, the vertical
+// profile replaced by 2^alpha e^(-z/z_0) past z / (alpha z_0) = 100.  R and z are the
+// distances along and across the major axis (no axis ratio).
+//   v[2] cosPA  v[3] sinPA  v[5] Sigma_00 = 2 h L_0  v[6] h  v[7] n  v[8] alpha
+//   v[9] z_0  v[10] alpha z_0  v[11] 2^alpha  v[13] L_0
+__device__ __forceinline__ double edge_intensity(const double* v, double R, double z) {
+  double I_radial;
+  if (R == 0.0) {
+    I_radial = v[5];
+  } else {
+    const double scaledR = R / v[6];
+    I_radial = v[5] * scaledR * bessel_k01(scaledR).y;
+  }
+  double verticalScaling;
+  if ((z / v[10]) > 100.0) {
+    verticalScaling = v[11] * exp(-z / v[9]);
+  } else {
+    const double sech = 1.0 / cosh(z / v[10]);
+    verticalScaling = pow(sech, v[8]);
+  }
+  return I_radial * verticalScaling;
+}
+
+__device__ __forceinline__ double edge_value(const double* v, double x, double y) {
+  const double x_diff = x - v[0], y_diff = y - v[1];
+  const double R = fabs(x_diff * v[2] + y_diff * v[3]);
+  const double z = fabs(-x_diff * v[3] + y_diff * v[2]);
+  return edge_intensity(v, R, z);
+}
+
+// Sub-pixel count (func_edge-on-disk.cpp:217-246): within 10 px of either axis.
+__device__ __forceinline__ int edge_subsamples(const double* v, double x, double y, int on) {
+  const double x_diff = x - v[0], y_diff = y - v[1];
+  const double R = fabs(x_diff * v[2] + y_diff * v[3]);
+  const double z = fabs(-x_diff * v[3] + y_diff * v[2]);
+  if (!on || !((R < 10.0) || (z < 10.0))) return 1;
+  const double h = v[6], z_0 = v[9];
+  if (((h <= 1.0) && (R <= 1.0)) || ((z_0 <= 1.0) && (z <= 1.0)))
+    return max(min(100, (int)(20 / h)), min(100, (int)(20 / z_0)));
+  if ((R <= 3.0) || (z <= 3.0)) return 20;
+  return max(min(100, (int)(20 / R)), min(100, (int)(20 / z)));
+}
+
 // ------------------------------------------------------------------ ExponentialDisk3D
 // Line-of-sight integral of a 3-D exponential disk with a sech^(2/n) vertical
 // profile (func_expdisk3d.cpp:177-268), over s in [-20h, 20h]
@@ -646,11 +737,13 @@ __device__ __forceinline__ float comp_point_2d<float>(int type, const double* v,
 }
 
 // Point value without sub-pixel integration (bulk pixels).  DISK: the model has an
-// ExponentialDisk3D component (kernels without one carry none of the quadrature).
+// ExponentialDisk3D or EdgeOnDisk component (kernels without one carry none of the
+// line-of-sight quadrature or Bessel-function code).
 template <typename T, bool DISK>
 __device__ __forceinline__ T comp_point(int type, const double* v, double x, double y) {
   if (type == F_FLATSKY) return (T)v[5];
   if (DISK && type == F_EXPDISK3D) return disk_value<T>(v, x, y, sizeof(T) == 8 ? 3 : 2);
+  if (DISK && type == F_EDGEONDISK) return (T)edge_value(v, x, y);   // the Bessel function in FP64
   return comp_point_2d<T>(type, v, x, y);
 }
 
@@ -731,6 +824,51 @@ __device__ __forceinline__ void gensersic_grad_all(const double* v, double x, do
   if (need & 8u) g[3] = dIdR * (gY * Y / q);
 }
 
+// EdgeOnDisk: I = I_rad(R) V(z), R = |x'|, z = |y'| in the component frame.
+// d(s K_1(s))/ds = -s K_0(s); V = sech(t)^alpha, t = z / (alpha z_0) (or 2^alpha e^(-z/z_0)
+// past t = 100), alpha = 2/n.  Slots: 2 PA, 3 L_0, 4 h, 5 n, 6 z_0.
+__device__ __forceinline__ void edge_grad_all(const double* v, double x, double y, unsigned need,
+                                              double (&g)[kSlots]) {
+  const double DEG2RAD = 0.017453292519943295;
+  const double c = v[2], sn = v[3], S00 = v[5], h = v[6], alpha = v[8], z0 = v[9];
+  const double dx = x - v[0], dy = y - v[1];
+  const double xp = dx * c + dy * sn, yp = -dx * sn + dy * c;
+  const double R = fabs(xp), z = fabs(yp);
+  double Irad = S00, dIrad_dR = 0.0, dIrad_dh = S00 / h;
+  if (R != 0.0) {
+    const double s = R / h;
+    const double2 k = bessel_k01(s);
+    Irad = S00 * s * k.y;
+    dIrad_dR = -S00 * s * k.x / h;
+    dIrad_dh = Irad / h + S00 * s * s * k.x / h;
+  }
+  const double t = z / v[10];
+  double V, dV_dz, dV_dz0, dlnV_dn;
+  if (t > 100.0) {
+    V = v[11] * exp(-z / z0);
+    dV_dz = -V / z0;
+    dV_dz0 = V * z / (z0 * z0);
+    dlnV_dn = -0.5 * alpha * alpha * 0.69314718055994530942;
+  } else {
+    const double th = tanh(t);
+    V = pow(1.0 / cosh(t), alpha);
+    dV_dz = -V * th / z0;
+    dV_dz0 = V * alpha * th * t / z0;
+    dlnV_dn = -0.5 * alpha * alpha * (log(1.0 / cosh(t)) + t * th);
+  }
+  if (need & (1u << 3)) g[3] = (Irad / v[13]) * V;
+  if (need & (1u << 4)) g[4] = dIrad_dh * V;
+  if (need & (1u << 5)) g[5] = Irad * V * dlnV_dn;
+  if (need & (1u << 6)) g[6] = Irad * dV_dz0;
+  if (!(need & 7u)) return;
+  const double dI_dR = dIrad_dR * V, dI_dz = Irad * dV_dz;
+  const double sx = (xp > 0.0) ? 1.0 : ((xp < 0.0) ? -1.0 : 0.0);
+  const double sz = (yp > 0.0) ? 1.0 : ((yp < 0.0) ? -1.0 : 0.0);
+  if (need & 1u) g[0] = dI_dR * (-sx * c) + dI_dz * (sz * sn);
+  if (need & 2u) g[1] = dI_dR * (-sx * sn) + dI_dz * (-sz * c);
+  if (need & 4u) g[2] = DEG2RAD * (dI_dR * sx * yp - dI_dz * sz * xp);
+}
+
 template <bool DISK>
 __device__ __forceinline__ void comp_grad_all(int type, const double* v, double x, double y, unsigned need,
                                               double (&g)[kSlots]) {
@@ -748,6 +886,10 @@ __device__ __forceinline__ void comp_grad_all(int type, const double* v, double 
   }
   if (type == F_GENSERSIC) {
     gensersic_grad_all(v, x, y, need, g);
+    return;
+  }
+  if (DISK && type == F_EDGEONDISK) {
+    edge_grad_all(v, x, y, need, g);
     return;
   }
   const double DEG2RAD = 0.017453292519943295;
@@ -1058,7 +1200,9 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
       const int type = ea.types[n];
       const CompRec rv = rec[n];   // into registers: read once, not per sample
       const double* v = rv.v;
-      const int nSub = (type == F_FLATSKY) ? 1 : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
+      const int nSub = (type == F_FLATSKY)              ? 1
+                       : (DISK && type == F_EDGEONDISK) ? edge_subsamples(v, x, y, subsampling)
+                                                        : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
       double mean[kSlots];
       if (nSub > 1) {
         const double deltaSubpix = 1.0 / nSub;
@@ -1096,6 +1240,22 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
     double val;
     if (type == F_FLATSKY) {
       val = v[5];
+    } else if (DISK && type == F_EDGEONDISK) {
+      const int nSub = edge_subsamples(v, x, y, subsampling);
+      if (nSub > 1) {
+        const double deltaSubpix = 1.0 / nSub;
+        const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+        const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+        const int n2 = nSub * nSub;
+        double part = 0.0;
+        for (int s = lane; s < n2; s += 64) {
+          const int ii = s / nSub, jj = s - ii * nSub;
+          part += edge_value(v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix);
+        }
+        val = wave_sum(part) / (nSub * nSub);
+      } else {
+        val = edge_value(v, x, y);
+      }
     } else {
       const double rad = comp_radius(type, v, x, y);
       const int nSub = n_subsamples(type, v, rad, subsampling);

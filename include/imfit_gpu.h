@@ -65,8 +65,8 @@ int ifg_set_precision(ifg_model* m, int precision);
 /* ---- model definition ----------------------------------------------------- */
 /* AddFunction + DefineFunctionSets (core/model_object.cpp:277-340; name -> class
  * as in core/add_functions.cpp:163-385).  Supported names: "Sersic",
- * "Exponential", "BrokenExponential", "FlatSky", "Gaussian", "Moffat", "ExponentialDisk3D",
- * "Sersic_GenEllipse". */
+ * "Exponential", "BrokenExponential", "EdgeOnDisk", "FlatSky", "Gaussian", "Moffat",
+ * "ExponentialDisk3D", "Sersic_GenEllipse". */
 int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set);
 /* FunctionObject::SetSubsampling for every component (imfit --no-subsampling) */
 int ifg_set_subsampling(ifg_model* m, int do_subsampling);

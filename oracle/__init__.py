@@ -77,6 +77,8 @@ def lib():
         L.orc_func_value.restype = C.c_double
         L.orc_calculate_bn.argtypes = [C.c_double]
         L.orc_calculate_bn.restype = C.c_double
+        L.orc_bessel_k01.argtypes = [C.c_double, _dp, _dp]
+        L.orc_bessel_k01.restype = None
         L.orc_enorm.argtypes = [C.c_long, _dp]
         L.orc_enorm.restype = C.c_double
         L.orc_expdisk3d_qags.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -230,6 +232,12 @@ def enorm(x):
 
 def calculate_bn(n):
     return lib().orc_calculate_bn(n)
+
+
+def bessel_k01(x):
+    k0, k1 = np.zeros(1), np.zeros(1)
+    lib().orc_bessel_k01(float(x), _ptr(k0), _ptr(k1))
+    return float(k0[0]), float(k1[0])
 
 
 def expdisk3d_qags(params6, x0, y0, x, y, epsrel=1e-6):

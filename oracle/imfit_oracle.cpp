@@ -11,7 +11,8 @@
 // FMA contraction without -march).
 //
 // Pinning (what checks each part; see tests/test_oracle.py):
-//   * GetValue of Sersic/Exponential/BrokenExponential/Gaussian/Moffat/FlatSky: unit_tests/unittest_funcs.t.h
+//   * GetValue of Sersic/Exponential/BrokenExponential/EdgeOnDisk/Gaussian/Moffat/FlatSky:
+//     unit_tests/unittest_funcs.t.h known answers; BesselK01 against scipy.special.k0/k1
 //     known answers; Calculate_bn against the reference's own helper_funcs.cpp built
 //     into oracle/_ref.
 //   * model images: tests/sersic+exp_orig.fits, tests/biggertest_orig.fits (float32);
@@ -223,6 +224,126 @@ struct BrokenExp : Func {
   }
   int CalculateSubsamples(double r) const { return GalfitSubsamples(doSubsampling, r, h1, 2 * SUBSAMPLE_R, 3.0); }
   double GetValue(double x, double y) override { return EllipticalGetValue(*this, x, y); }
+};
+
+// Modified Bessel K_0 and K_1 (the reference calls GSL's gsl_sf_bessel_K1,
+// function_objects/func_edge-on-disk.cpp:168): Abramowitz & Stegun 9.6.13 / 9.6.11
+// series for x <= 2, Steed's continued fraction (Thompson & Barnett 1986, Temme's
+// normalisation) above; pinned against scipy.special.k0 / k1.
+void BesselK01(double x, double* k0, double* k1) {
+  const double EULER = 0.57721566490153286061;
+  if (x <= 2.0) {
+    double y = 0.25 * x * x, lx = log(0.5 * x);
+    double t0 = 1.0, t1 = 1.0, H = 0.0, I0 = 1.0, I1s = 1.0, S0 = 0.0, S1 = 1.0 - 2.0 * EULER;
+    for (int k = 1; k < 40; k++) {
+      t0 *= y / (double)(k * k);
+      t1 *= y / (double)(k * (k + 1));
+      H += 1.0 / k;
+      I0 += t0;
+      I1s += t1;
+      S0 += H * t0;
+      S1 += (2.0 * H + 1.0 / (k + 1) - 2.0 * EULER) * t1;
+      if (t0 < 1e-17 * I0) break;
+    }
+    *k0 = -(lx + EULER) * I0 + S0;
+    *k1 = 1.0 / x + lx * (0.5 * x * I1s) - 0.25 * x * S1;
+    return;
+  }
+  double b = 2.0 * (1.0 + x), d = 1.0 / b, h = d, delh = d;
+  double q1 = 0.0, q2 = 1.0, q = 0.25, c = 0.25, a = -0.25, sum = 1.0 + q * delh;
+  for (int i = 2; i < 200; i++) {
+    a -= 2.0 * (i - 1);
+    c = -a * c / i;
+    double qnew = (q1 - b * q2) / a;
+    q1 = q2;
+    q2 = qnew;
+    q += c * qnew;
+    b += 2.0;
+    d = 1.0 / (b + a * d);
+    delh = (b * d - 1.0) * delh;
+    h += delh;
+    double dels = q * delh;
+    sum += dels;
+    if (fabs(dels) < 1e-16 * fabs(sum)) break;
+  }
+  h *= 0.25;
+  *k0 = sqrt(M_PI / (2.0 * x)) * exp(-x) / sum;
+  *k1 = *k0 * (x + 0.5 - h) / x;
+}
+
+// EdgeOnDisk (function_objects/func_edge-on-disk.cpp:95-246): PA, L_0, h, n, z_0;
+// R and z along and across the major axis, sub-pixel rule within 10 px of either axis.
+struct EdgeOnDisk : Func {
+  double PA, L_0, h, n, z_0, PA_rad, cosPA, sinPA, alpha, scaledZ0, Sigma_00, two_to_alpha;
+  int NParams() const override { return 5; }
+  void Setup(const double* p, int o, double xc, double yc) override {
+    x0 = xc; y0 = yc;
+    PA = p[o]; L_0 = p[o + 1]; h = p[o + 2]; n = p[o + 3]; z_0 = p[o + 4];
+    PA_rad = (PA + 90.0) * DEG2RAD;
+    cosPA = cos(PA_rad); sinPA = sin(PA_rad);
+    alpha = 2.0 / n;
+    scaledZ0 = alpha * z_0;
+    Sigma_00 = 2.0 * h * L_0;
+    two_to_alpha = pow(2.0, alpha);
+  }
+  double CalculateIntensity(double r, double z) const {
+    double I_radial, verticalScaling;
+    if (r == 0.0) {
+      I_radial = Sigma_00;
+    } else {
+      double scaledR = r / h, k0, k1;
+      BesselK01(scaledR, &k0, &k1);
+      I_radial = Sigma_00 * scaledR * k1;
+    }
+    if ((z / scaledZ0) > 100.0) {
+      verticalScaling = two_to_alpha * exp(-z / z_0);
+    } else {
+      double sech = 1.0 / cosh(z / scaledZ0);
+      verticalScaling = pow(sech, alpha);
+    }
+    return I_radial * verticalScaling;
+  }
+  int CalculateSubsamples(double r, double z) const {
+    int nSamples = 1;
+    double R_abs = fabs(r), z_abs = fabs(z);
+    if (doSubsampling && ((R_abs < 10.0) || (z_abs < 10.0))) {
+      if (((h <= 1.0) && (R_abs <= 1.0)) || ((z_0 <= 1.0) && (z_abs <= 1.0))) {
+        int nSr = std::min(100, (int)(2 * SUBSAMPLE_R / h)), nSz = std::min(100, (int)(2 * SUBSAMPLE_R / z_0));
+        nSamples = std::max(nSr, nSz);
+      } else if ((R_abs <= 3.0) || (z_abs <= 3.0)) {
+        nSamples = 2 * SUBSAMPLE_R;
+      } else {
+        int nSr = std::min(100, (int)(2 * SUBSAMPLE_R / R_abs)), nSz = std::min(100, (int)(2 * SUBSAMPLE_R / z_abs));
+        nSamples = std::max(nSr, nSz);
+      }
+    }
+    return nSamples;
+  }
+  double GetValue(double x, double y) override {
+    double x_diff = x - x0, y_diff = y - y0;
+    double R = fabs(x_diff * cosPA + y_diff * sinPA);
+    double z = fabs(-x_diff * sinPA + y_diff * cosPA);
+    int nSub = CalculateSubsamples(R, z);
+    if (nSub > 1) {
+      double deltaSubpix = 1.0 / nSub;
+      double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+      double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+      double theSum = 0.0;
+      for (int ii = 0; ii < nSub; ii++) {
+        double x_ii = x_sub_start + ii * deltaSubpix;
+        for (int jj = 0; jj < nSub; jj++) {
+          double y_ii = y_sub_start + jj * deltaSubpix;
+          x_diff = x_ii - x0;
+          y_diff = y_ii - y0;
+          R = fabs(x_diff * cosPA + y_diff * sinPA);
+          z = fabs(-x_diff * sinPA + y_diff * cosPA);
+          theSum += CalculateIntensity(R, z);
+        }
+      }
+      return theSum / (nSub * nSub);
+    }
+    return CalculateIntensity(R, z);
+  }
 };
 
 struct Gaussian : Func {   // function_objects/func_gaussian.cpp:102-203
@@ -681,6 +802,7 @@ Func* MakeFunc(const char* name) {   // core/add_functions.cpp:163-385 (name -> 
   if (s == "Sersic_GenEllipse") return new GenSersic();
   if (s == "Exponential") return new Exponential();
   if (s == "BrokenExponential") return new BrokenExp();
+  if (s == "EdgeOnDisk") return new EdgeOnDisk();
   if (s == "Gaussian") return new Gaussian();
   if (s == "Moffat") return new Moffat();
   if (s == "FlatSky") return new FlatSky();
@@ -1256,6 +1378,7 @@ double orc_func_value(const char* name, const double* params, double x0, double 
   return v;
 }
 double orc_calculate_bn(double n) { return Calculate_bn(n); }
+void orc_bessel_k01(double x, double* k0, double* k1) { BesselK01(x, k0, k1); }
 double orc_enorm(long n, const double* x) { return mp_enorm(n, x); }
 void orc_downsample_and_replace(const double* osImg, int nOsCols, int nOsRows, int nOsPsfCols, int nOsPsfRows,
                                 double* mainImg, int nMainCols, int nMainRows, int nMainPsfCols, int nMainPsfRows,

@@ -138,6 +138,14 @@ def brokenexp(alpha=3.0, psf=True):
                 psf=gaussian_psf(13, 1.5) if psf else None)
 
 
+def edgeondisk(PA=90.0, psf=True):
+    """EdgeOnDisk + FlatSky on 96x80 pixels (13^2 Gaussian PSF): sub-pixel bands within
+    10 px of both axes (func_edge-on-disk.cpp:217-246), axis-aligned at PA 90 and
+    rotated otherwise."""
+    return Case([("EdgeOnDisk", True), ("FlatSky", False)], [48.3, 40.6, PA, 2.0, 12.0, 2.0, 3.0, 5.0],
+                ncols=96, nrows=80, psf=gaussian_psf(13, 1.5) if psf else None)
+
+
 def sersic_exp512():
     return Case([("Sersic", True), ("Exponential", False)], [256, 256, 30, 0.5, 2.5, 100, 50, 30, 0.75, 10, 150],
                 ncols=512, nrows=512), golden("sersic_exp512")["image"]

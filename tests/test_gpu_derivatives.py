@@ -91,11 +91,25 @@ def brokenexp_fit_case(alpha):
                       chars=(4.0, 1.0, 1.0, 1, 0.0))
 
 
+def edgeondisk_fit_case(PA):
+    """EdgeOnDisk + sky fit data: the oracle model at a shifted centre and scale height."""
+    base = cases.edgeondisk(PA, psf=True)
+    o = base.build(oracle.OracleModel(threads=8))
+    truth = base.params.copy()
+    truth[1] += 0.3
+    truth[6] *= 1.1
+    o.CreateModelImage(truth)
+    return cases.Case(base.funcs, base.params, data=o.GetModelImageVector(), psf=base.psf,
+                      chars=(4.0, 1.0, 1.0, 1, 0.0))
+
+
 @pytest.mark.parametrize("which", ["n3073", "textout3", "osamp", "gensersic_boxy", "gensersic_disky",
-                                   "brokenexp_sharp", "brokenexp_soft"])
+                                   "brokenexp_sharp", "brokenexp_soft", "edgeondisk_90", "edgeondisk_30"])
 def test_jacobian_vs_central_differences(which):
     announce("jacobian %s" % which)
-    if which.startswith("brokenexp"):
+    if which.startswith("edgeondisk"):
+        case = edgeondisk_fit_case(float(which.split("_")[1]))
+    elif which.startswith("brokenexp"):
         case = brokenexp_fit_case(3.0 if which.endswith("sharp") else 0.3)
     elif which == "n3073":
         case, _ = cases.n3073()

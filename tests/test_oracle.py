@@ -336,3 +336,40 @@ def test_expdisk3d_edge_on_closed_form():
         a = 2.0 / n
         ref = J0 * (1.0 / math.cosh(abs(yp) / (a * z0))) ** a * 2.0 * abs(xp) * k1(abs(xp) / h)
         assert abs(v - ref) <= 2e-6 * ref, (x, y, v, ref)
+
+
+def test_bessel_k01_against_scipy():
+    """The oracle's K_0 / K_1 (series below x = 2, continued fraction above) against
+    scipy.special.k0 / k1 (Cephes Chebyshev expansions) over 1e-8 <= x <= 700."""
+    from scipy.special import k0, k1
+    xs = np.concatenate([np.logspace(-8, np.log10(2.0), 300), np.linspace(2.0, 5.0, 150)[1:],
+                         np.logspace(np.log10(5.0), np.log10(700.0), 300)])
+    for x in xs:
+        a0, a1 = oracle.bessel_k01(x)
+        assert abs(a0 / k0(x) - 1) < 5e-15 and abs(a1 / k1(x) - 1) < 5e-15, x
+
+
+def test_edge_on_disk_known_answers():   # unit_tests/unittest_funcs.t.h:1990-2015
+    """PA 90, L_0 1, h 10, z_0 1 centred on (100, 100): the centre is Sigma_00 = 2 h L_0;
+    with n = 1000 (nearly exponential vertically) 10 px above the plane the value is
+    Sigma_00 2^(2/n) e^(-10/z_0)."""
+    h, L0, z0 = 10.0, 1.0, 1.0
+    f = lambda p, x, y: oracle.func_value("EdgeOnDisk", p, 100.0, 100.0, x, y)
+    assert abs(f([90.0, L0, h, 1.0, z0], 100.0, 100.0) - 2.0 * h * L0) < DELTA
+    want = 2.0 * h * L0 * 2.0 ** (2.0 / 1000.0) * math.exp(-10.0 / z0)
+    assert abs(f([90.0, L0, h, 1000.0, z0], 100.0, 110.0) - want) < DELTA
+
+
+def test_edge_on_disk_closed_form():
+    """Off the centre the value is Sigma_00 (R/h) K_1(R/h) sech^(2/n)(n z / (2 z_0)) with
+    R, z along and across the major axis (PA 30 here), checked with scipy's K_1."""
+    from scipy.special import k1
+    PA, L0, h, n, z0 = 30.0, 2.0, 7.0, 1.5, 2.5
+    t = math.radians(PA + 90.0)
+    for x, y in [(112.0, 95.0), (80.3, 121.7), (100.5, 99.0), (140.0, 100.0)]:
+        dx, dy = x - 100.0, y - 100.0
+        R = abs(dx * math.cos(t) + dy * math.sin(t))
+        z = abs(-dx * math.sin(t) + dy * math.cos(t))
+        want = 2 * h * L0 * (R / h) * k1(R / h) * (1.0 / math.cosh(n * z / (2 * z0))) ** (2.0 / n)
+        got = oracle.func_value("EdgeOnDisk", [PA, L0, h, n, z0], 100.0, 100.0, x, y)
+        assert abs(got - want) <= 1e-13 * want, (x, y, got, want)
