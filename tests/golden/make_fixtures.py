@@ -26,6 +26,7 @@ Each fixture records which reference test produced it:
   twogauss_osamp do_imfit_tests:107 -> imfit_textout4e3
   conv_n5419     do_psfconvolve_tests:11,60 -> tests/convolve_out_n5419.fits
   conv_5x5       do_psfconvolve_tests:8-9 -> tests/psfconvolve_out{1,2}.txt (padded PSF)
+  multimfit      do_multimfit_tests.sh:59-85 -> multimfit_textout1a, 1d, 1e, 3a, 3b
   bootstrap_ic3478  do_imfit_tests:133-134 -> bootstrap_output_seed10_tail.dat,
                  bootstrap_output_seed10_2_tail.dat (seed 10, whole image and section)
 """
@@ -228,6 +229,7 @@ def main():
                                        "The whole padded, normalized PSF image"),
     )
     bootstrap_fixture()
+    multimfit_fixture()
     copy_inputs()
     for fn in sorted(os.listdir(OUT)):
         if fn.endswith(".npz"):
@@ -244,6 +246,10 @@ FITS_INPUTS = [
     "twogaussian_psf+2osamp_noisy.fits", "psf_moffat_35.fits", "psf_moffat_35_oversamp3.fits",
     "oversamp_test4.fits", "psf_standard.fits", "psf_oversamp.fits",
     "testimage_3x3_nan.fits", "testimage_3x3_allzeros.fits",
+    # do_multimfit_tests.sh:59-85
+    "multimfit_reference/smallDataImage1a.fits", "multimfit_reference/smallDataImage1b.fits",
+    "multimfit_reference/smallgauss_5x5_0.fits", "multimfit_reference/smallgauss_5x5_1.fits",
+    "multimfit_reference/smallgauss_5x5_sigma.fits",
     # unit_tests/unittest_image_io.t.h:17-25
     "test_emptyhdu.fits", "test_table.fits", "test_multiextension_hdu0empty.fits",
     "test_multiextension_2images.fits", "multiextension_ic3478rss_64x64_twice.fits",
@@ -266,6 +272,12 @@ CONFIG_INPUTS = [
     "makeimage_reference/config_makeimage_sersictest512_bad3.dat",
     "makeimage_reference/config_makeimage_sersictest512_bad4.dat",
     "makeimage_reference/config_makeimage_sersictest512_bad5.dat",
+    # do_multimfit_tests.sh:59-85
+    "multimfit_reference/config_imfit_smalldataimages.dat",
+    "multimfit_reference/config_imfit_smalldataimages_with-limits.dat",
+    "multimfit_reference/config_imfit_smallgauss.dat", "multimfit_reference/imageinfo_multimfit0.txt",
+    "multimfit_reference/imageinfo_multimfit0_with-limits.txt", "multimfit_reference/imageinfo_multimfit3.txt",
+    "multimfit_reference/imageinfo_multimfit3b.txt",
     # unit_tests/unittest_config_parser.t.h:35-52
     "config_makeimage_sersictest512.dat", "config_makeimage_sersictest512_with_labels.dat",
     "config_imfit_sersictest512_badlimits4.dat",
@@ -287,6 +299,35 @@ def bootstrap_fixture():
     np.savez_compressed(os.path.join(OUT, "bootstrap_ic3478.npz"),
                         seed10=rows(t(lin + "bootstrap_output_seed10_tail.dat")),
                         seed10_section=rows(t(lin + "bootstrap_output_seed10_2_tail.dat")))
+
+
+def multimfit_fixture():
+    """do_multimfit_tests.sh:59-85: multimfit printouts -- the statistic (1a: fit statistic
+    only; 1d, 1e: L-M fits on two 2x2 images, 1e fitting FLUX_SCALE; 3a, 3b: Nelder-Mead
+    fits of a Gaussian on two 5x5 images, 3b on image sections) and the best-fit main
+    model and image-description parameters, in printed order."""
+    def record(name):
+        with open(t("multimfit_reference/" + name)) as f:
+            txt = f.read()
+        out = {"chi2": grab(t("multimfit_reference/" + name), r"CHI-SQUARE = ([0-9.]+)")}
+        for key, head in (("model", "# Main model parameters"), ("image", "# Image-description parameters")):
+            if head not in txt:
+                continue
+            vals = []
+            for line in txt.split(head, 1)[1].split("\n")[1:]:
+                line = line.strip()
+                if not line or line.startswith("#") or line.startswith("Saving"):
+                    break
+                if line.startswith("FUNCTION"):
+                    continue
+                vals.append(float(line.split()[1]))
+            out[key] = np.array(vals)
+        return out
+    recs = {}
+    for k in ("1a", "1d", "1e", "3a", "3b"):
+        for name, v in record("multimfit_textout" + k).items():
+            recs["t%s_%s" % (k, name)] = v
+    np.savez_compressed(os.path.join(OUT, "multimfit.npz"), **recs)
 
 
 def copy_inputs():

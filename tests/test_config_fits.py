@@ -291,3 +291,16 @@ def test_write_and_read_back(tmp_path):    # unittest_image_io.t.h:232-305
     assert np.array_equal(F.read_image(p), a)
     F.write_image(a, p)
     assert np.array_equal(F.read_image(p), a.astype(np.float32).astype(np.float64))
+
+
+def test_image_info_files():
+    """tests/multimfit_reference/imageinfo_multimfit3b.txt: two images (sections of the
+    5x5 files), sky 100, the second image's scale, angle and flux scale fixed."""
+    from imfit_amd.multimage import read_image_info
+    infos = read_image_info(cfg("imageinfo_multimfit3b.txt"), lambda n: n)
+    assert len(infos) == 2
+    assert infos[0].data.endswith("smallgauss_5x5_0.fits[2:5,2:5]") and infos[0].original_sky == 100.0
+    assert infos[1].params == [1.0, 0.0, 1.0, 2.0, 2.0]
+    assert [l.fixed for l in infos[1].limits] == [1, 1, 1, 0, 0]
+    info0 = read_image_info(cfg("imageinfo_multimfit0_with-limits.txt"), lambda n: n)
+    assert info0[1].limits[2].limited == [1, 1] and info0[1].limits[2].limits == [0.0, 5.0]
