@@ -93,6 +93,11 @@ class ModelObjectGPU {
   long GetNDataValues() const { return ifg_get_ndata(h_); }
   long GetNValidPixels() const { return ifg_get_nvalid(h_); }
 
+  // bootstrap resampling (core/model_object.cpp:2036-2087); the generator is seeded
+  // process-wide with ifg_init_genrand, as imfit seeds init_genrand
+  int UseBootstrap() { return Check(ifg_use_bootstrap(h_), "UseBootstrap"); }
+  int MakeBootstrapSample() { return Check(ifg_make_bootstrap_sample(h_), "MakeBootstrapSample"); }
+
   // ---- hot path (core/model_object.h:132,141,153,155,157,223)
   void CreateModelImage(const double params[]) {
     model_.clear();
