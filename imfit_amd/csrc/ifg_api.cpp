@@ -883,7 +883,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     P.tailOff[gi + 1] = P.tailOff[gi] + (long)sm * nVec;
   }
 
-  // chunks of the batch, so that the complex planes stay within ~1 GiB
+  // chunks of the batch, so that the complex planes stay within ~4 GiB, of equal
+  // size (a short last chunk would pay a full launch's fill and drain)
   size_t pairBytes = 0, pairBytesR = 0;
   std::vector<size_t> regOff(nR + 1, 0);
   if (m->doConv) pairBytes = (size_t)P.geoms[0].Nr * P.geoms[0].Nc * sizeof(cpx<T>);
@@ -894,9 +895,12 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   pairBytesR = regOff[nR];
   const int nPairsAll = P.single ? 1 : (nVec + 1) / 2;
   P.chunkPairs = nPairsAll;
-  if (pairBytes + pairBytesR > 0)
-    P.chunkPairs =
-        (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, (size_t)(1u << 30) / (pairBytes + pairBytesR)));
+  if (pairBytes + pairBytesR > 0) {
+    const int maxPairs =
+        (int)std::max<size_t>(1, std::min<size_t>(nPairsAll, ((size_t)4 << 30) / (pairBytes + pairBytesR)));
+    const int n = (nPairsAll + maxPairs - 1) / maxPairs;
+    P.chunkPairs = (nPairsAll + n - 1) / n;
+  }
   P.chunkVecs = P.single ? 1 : 2 * P.chunkPairs;
   P.nChunks = (nVec + P.chunkVecs - 1) / P.chunkVecs;
   P.nTasks = P.nGrids;

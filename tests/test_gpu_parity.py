@@ -218,3 +218,21 @@ def test_graph_replays_match_eager():
             gb = gg.GetFitStatisticBatch(P[: k + 2])
             assert np.array_equal(eb, gb), k
             assert np.array_equal(ge.ComputeDeviates(P[k]), gg.ComputeDeviates(P[k])), k
+
+
+def test_population_across_chunks():
+    """A population whose complex planes exceed one chunk (4096 vectors of the 512^2
+    Sersic + Exponential + FlatSky workload with a 35^2 Moffat PSF, FP64: 576^2
+    planes, three chunks of 683 pairs): vectors at the chunk edges, first and last,
+    against the oracle within 1e-12."""
+    import imfit_amd
+    from imfit_amd import workloads
+    announce("population across chunks (512^2, 4096 vectors, fp64)")
+    w, pop = workloads.cfg3_population()
+    g = w.fit_model(imfit_amd.PRECISION_FP64)
+    batch = g.GetFitStatisticBatch(pop)
+    assert batch.shape == (4096,) and np.all(np.isfinite(batch))
+    o = w.setup_fit(oracle.OracleModel(threads=8))
+    for k in (0, 1, 1365, 1366, 1367, 2731, 2732, 4094, 4095):
+        ref = o.GetFitStatistic(pop[k])
+        assert abs(batch[k] - ref) <= 1e-12 * abs(ref), (k, batch[k], ref)
