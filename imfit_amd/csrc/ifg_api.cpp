@@ -840,6 +840,19 @@ void AppendTaskBlocks(std::vector<int2>& map, int id, int nBlocks, int align) {
 
 // Block layout of one launch's tasks: task t owns nPairs * n blocks of each pass, and
 // partial slots [slot0, slot0 + nPairs * n3); the maps give each block its task.
+// No-PSF main grid: each inverse-pass block takes a run of px consecutive pixels of
+// the Hh x W band (stored in rows3), a multiple of the 256 threads between 256 and
+// 1024, so that a launch with few pairs (one small image) still spreads over ~1024
+// blocks rather than a few blocks that walk their rows one after another.
+void PlainBlocks(Task& Tk, long pairsInLaunch) {
+  const long band = (long)Tk.g.Hh * Tk.g.W;
+  long px = (band * std::max(1L, pairsInLaunch) / 1024 + 255) / 256 * 256;
+  px = std::max(256L, std::min(1024L, px));
+  Tk.rows3 = (int)px;
+  Tk.n1 = Tk.n2 = 0;
+  Tk.n3 = (int)((band + px - 1) / px);
+}
+
 void LayoutBlocks(std::vector<Task>& tk, int first, int count, std::vector<int2>& m1, std::vector<int2>& m2,
                   std::vector<int2>& m3, int& slot) {
   for (int t = first; t < first + count; t++) {
@@ -1011,9 +1024,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         ea.boxStride = IFG_MAXBOX;
       }
       if (Tk.kind == TK_MAIN_PLAIN) {
-        Tk.rows3 = std::max(1, 1024 / Tk.g.W);   // whole rows, about 1024 pixels per block
-        Tk.n1 = Tk.n2 = 0;
-        Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
+        PlainBlocks(Tk, Tk.nPairs);
       } else {
         Tk.pc = cg->fc->plan;
         Tk.pr = cg->fr->plan;
@@ -2024,9 +2035,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
         }
       }
       if (Tk.kind == TK_MAIN_PLAIN) {
-        Tk.rows3 = std::max(1, 1024 / Tk.g.W);
-        Tk.n1 = Tk.n2 = 0;
-        Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
+        PlainBlocks(Tk, (long)nPairs * nM);
       } else {
         Tk.pc = cg->fc->plan;
         Tk.pr = cg->fr->plan;

@@ -130,8 +130,8 @@ struct Task {
   long pairStride;
   const int4* pairs;           // (vecA, rowoffA, vecB or -1, rowoffB) per pair
   int b1, n1, b2, n2, b3, n3;  // first block and blocks per pair in the evaluate / column / inverse launches
-  int rows1, rows3, C, ld;     // rows per block (row passes; TK_MAIN_PLAIN: rows3 only), columns per block
-                               // and stride (column pass)
+  int rows1, rows3, C, ld;     // rows per block (row passes; TK_MAIN_PLAIN: rows3 = pixels per inverse
+                               // block), columns per block and stride (column pass)
   int nStrips, stripsPerLine;  // column pass: strips of C columns, strips sharing a 128-byte line
   int slot0;                   // partial slot of inverse block (pair p, block lb): slot0 + p * n3 + lb
   int scale, nbc, rowsRep, c0, r0;   // region: data pixels [c0, c0+nbc) x [r0, r0+rowsRep) it replaces
