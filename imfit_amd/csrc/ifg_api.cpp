@@ -842,11 +842,12 @@ void AppendTaskBlocks(std::vector<int2>& map, int id, int nBlocks, int align) {
 // partial slots [slot0, slot0 + nPairs * n3); the maps give each block its task.
 // No-PSF main grid: each inverse-pass block takes a run of px consecutive pixels of
 // the Hh x W band (stored in rows3), a multiple of the 256 threads between 256 and
-// 1024, so that a launch with few pairs (one small image) still spreads over ~1024
-// blocks rather than a few blocks that walk their rows one after another.
-void PlainBlocks(Task& Tk, long pairsInLaunch) {
+// 1024, so that a task with few pairs (one small image) still spreads over ~1024
+// blocks rather than a few blocks that walk their rows one after another.  The
+// layout depends on the task alone (the order of its partial sums with it).
+void PlainBlocks(Task& Tk, long nPairs) {
   const long band = (long)Tk.g.Hh * Tk.g.W;
-  long px = (band * std::max(1L, pairsInLaunch) / 1024 + 255) / 256 * 256;
+  long px = (band * std::max(1L, nPairs) / 1024 + 255) / 256 * 256;
   px = std::max(256L, std::min(1024L, px));
   Tk.rows3 = (int)px;
   Tk.n1 = Tk.n2 = 0;
@@ -2035,7 +2036,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
         }
       }
       if (Tk.kind == TK_MAIN_PLAIN) {
-        PlainBlocks(Tk, (long)nPairs * nM);
+        PlainBlocks(Tk, nPairs);   // as the model's own plan: batched results equal single-model ones
       } else {
         Tk.pc = cg->fc->plan;
         Tk.pr = cg->fr->plan;

@@ -1818,18 +1818,16 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
   en_init(ea);
   en_init(eb);
   if (Tk.kind == TK_MAIN_PLAIN) {   // no PSF: pixels [p0, p0 + rows3) of the Hh x W band of planes A and B,
-                                    // K per thread in flight, data and weights fetched before the model
+                                    // K per thread in flight
     __shared__ CompRec sRec[2][IFG_MAXCOMP];
     __shared__ Box sBox[2][IFG_MAXBOX];
     stage_pair<DERIV>(Tk, pr, sRec, sBox);
     constexpr int K = 4;
     const long band = (long)g.Hh * g.W, p0 = (long)lb * Tk.rows3;
     const int npx = (int)min((long)Tk.rows3, band - p0);
-    const bool needDW = mode_reads_data(sl.mode);
     for (int e0 = threadIdx.x; e0 < npx; e0 += K * kThreads) {
       int rwa[K], rwb[K], cols[K];
       bool okA[K], okB[K];
-      double dA[K], wA[K], dB[K], wB[K];
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int e = e0 + k * kThreads;
@@ -1840,29 +1838,18 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
         rwb[k] = pr.w + r;
         okA[k] = e < npx && rwa[k] < st.H;
         okB[k] = e < npx && pr.z >= 0 && rwb[k] < st.H;
-        dA[k] = wA[k] = dB[k] = wB[k] = 0.0;
-        if (needDW && okA[k]) {
-          const long z = (long)rwa[k] * st.W + cols[k];
-          dA[k] = (double)((const T*)st.data)[z];
-          wA[k] = (double)((const T*)st.weight)[z];
-        }
-        if (needDW && okB[k]) {
-          const long z = (long)rwb[k] * st.W + cols[k];
-          dB[k] = (double)((const T*)st.data)[z];
-          wB[k] = (double)((const T*)st.weight)[z];
-        }
       }
       T va[K];
       row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.x, sRec[0], sBox[0], rwa, cols, okA, va);
 #pragma unroll
       for (int k = 0; k < K; k++)
-        if (okA[k]) stat_pixel_dw<T>(st, sl, pr.x, rwa[k], cols[k], (double)va[k], dA[k], wA[k], ea, true);
+        if (okA[k]) stat_pixel<T>(st, sl, pr.x, rwa[k], cols[k], (double)va[k], ea, true);
       if (pr.z >= 0) {
         T vb[K];
         row_values<T, DISK, DERIV, K>(Tk.ea, g, pr.z, sRec[1], sBox[1], rwb, cols, okB, vb);
 #pragma unroll
         for (int k = 0; k < K; k++)
-          if (okB[k]) stat_pixel_dw<T>(st, sl, pr.z, rwb[k], cols[k], (double)vb[k], dB[k], wB[k], eb, true);
+          if (okB[k]) stat_pixel<T>(st, sl, pr.z, rwb[k], cols[k], (double)vb[k], eb, true);
       }
     }
   } else {
