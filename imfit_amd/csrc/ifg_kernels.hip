@@ -1308,7 +1308,10 @@ __global__ __launch_bounds__(256) void k_tail(const Task* __restrict__ tasks, in
   const int lane = threadIdx.x & 63;
   const long total = prefix[nTasks];
   const long nWaves = (long)gridDim.x * (kThreads >> 6);
-  for (long wave = ((long)blockIdx.x * kThreads + threadIdx.x) >> 6; wave < total; wave += nWaves) {
+  // the wave's index, made visibly uniform: the task search, box and component
+  // reads below become scalar loads
+  const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (long wave = (long)blockIdx.x * (kThreads >> 6) + wib; wave < total; wave += nWaves) {
     int lo = 0, hi = nTasks - 1;   // last t with prefix[t] <= wave
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
