@@ -324,6 +324,7 @@ struct Plan {
   std::vector<long> tailOff;
   size_t recBytes = 0, boxBytes = 0, stageBytes = 0;
   void* pinnedStage = nullptr;
+  void* stageDev = nullptr;   // pinnedStage as the device sees it (mapped)
   void* pinnedOut = nullptr;
   double* outDev = nullptr;   // pinnedOut as the device addresses it
   bool directOut = false;     // k_final writes the statistics into pinnedOut itself (few jobs)
@@ -785,8 +786,9 @@ int EnsureRegionConv(ifg_model* m, Region& R, int which) {
 // parameter values: grids, FFT geometry, pairing of planes, buffer sizes and the
 // launch shapes.  Per call only the Setup constants, the sub-pixel boxes and their
 // pixel counts change; they are staged in pinned memory and reach the device in one
-// copy.  The whole sequence (copy, kernels, result copy) is captured once as a HIP
-// graph and replayed.
+// copy (a small kernel reading the mapped buffer, or a DMA copy for large stagings).
+// The whole sequence (copy, kernels, result copy) is captured once as a HIP graph
+// and replayed.
 // PF_DEVKEEP / PF_DFAC1 / PF_DFAC2: forward pass that leaves deviates and
 // d(deviate)/dm (1) or d(statistic)/dm (2) in the model's device buffers for a
 // following derivative pass.  PF_JAC / PF_GRAD: derivative pass (planes are
@@ -928,6 +930,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   P.prefixOff = P.usedOff + ((P.nGrids * sizeof(int) + 15) / 16) * 16;
   P.stageBytes = P.prefixOff + sizeof(long) * (size_t)P.nChunks * (P.nTasks + 1);
   HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
+  HIPCHK(hipHostGetDevicePointer(&P.stageDev, P.pinnedStage, 0), "map pinned staging");
   HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
   HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
 
@@ -1201,7 +1204,12 @@ template <typename T>
 int Enqueue(ifg_model* m, Plan& P) {
   const int nVec = P.nVec;
   hipStream_t s = m->stream;
-  HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
+  // a few KB (one evaluation, one Jacobian): read by a kernel from the mapped
+  // buffer; larger stagings (populations) by a DMA copy
+  if (P.stageDev && P.stageBytes % 8 == 0 && P.stageBytes <= 65536)
+    HIPCHK(launch_stage_copy(s, P.stage_d.p, P.stageDev, (long)(P.stageBytes / 8)), "upload call data");
+  else
+    HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
   const bool deriv = IsDeriv(P.flags);
   const int mode = (P.flags & PF_JAC) ? S_JAC : (P.flags & PF_GRAD) ? S_GRAD : m->statMode();
   const int subsampling = m->doSubsampling ? 1 : 0;

@@ -1944,6 +1944,15 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
   if (sl.partials) finish_block(sl.partials, Tk.slot0 + local, ea, eb);
 }
 
+// ------------------------------------------------------------------ call staging
+// The call's staging (component records, boxes, prefixes: a few KB for one
+// evaluation) read straight from the mapped pinned host buffer into device memory
+// by the graph's first kernel, in place of a DMA copy node and its hand-off.
+__global__ __launch_bounds__(256) void k_stage_copy(unsigned long long* __restrict__ dst,
+                                                    const unsigned long long* __restrict__ src, long n8) {
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n8; i += (long)gridDim.x * kThreads) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------ normal equations
 // J^T J (upper triangle, row-major packed) and J^T r of every job (model) from its
 // Jacobian columns jcol[j * nData + i] and deviates r[i]: each block stages a tile of
@@ -2094,6 +2103,14 @@ hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPl
   const int ld = Nr + 1;
   IFG_LAUNCH(true, true, k_colfft<T>, dim3((Nc + C - 1) / C), dim3(256), (2 * (size_t)C * ld + Nr) * sizeof(cpx<T>),
              s, buf, Nr, Nc, pl, tw, C, ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, long n8) {
+  if (n8 <= 0) return hipSuccess;
+  const long blocks = std::min<long>((n8 + kThreads - 1) / kThreads, 16);
+  IFG_LAUNCH(false, false, k_stage_copy, dim3((unsigned)blocks), dim3(kThreads), 0, s, (unsigned long long*)dst,
+             (const unsigned long long*)srcMapped, n8);
   return hipGetLastError();
 }
 

@@ -32,6 +32,8 @@ hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FF
                          int dir, int rows);
 template <typename T>
 hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C);
+// n8 8-byte words from mapped pinned host memory into device memory
+hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, long n8);
 // jobList: the jobs of this launch (null: jobs 0 .. nJobs-1); hostOut: out is mapped
 // host memory (each job block ends with a system-scope fence)
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
