@@ -40,3 +40,18 @@ def test_cpp_host_class_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", root,
                         "-I", os.path.join(root, "include"), str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_address_arguments():
+    """The per-evaluation wrapper passes array addresses: parameters that are already
+    C-contiguous float64 go through uncopied, anything else is converted first."""
+    import numpy as np
+    import imfit_amd
+    p = np.arange(6, dtype=np.float64)
+    assert imfit_amd._f64(p) is p
+    assert imfit_amd._addr(p) == p.ctypes.data
+    for other in (p[::2], p.astype(np.float32), list(p), np.asfortranarray(p.reshape(2, 3))):
+        q = imfit_amd._f64(other)
+        assert q.dtype == np.float64 and q.flags.c_contiguous
+        assert np.array_equal(q, np.asarray(other, dtype=np.float64))
+    assert imfit_amd._FAST.ifg_fit_statistic.argtypes is not None
