@@ -160,6 +160,7 @@ class ModelObjectGPU:
             raise IfgError("ifg_create failed")
         self._chk(_LIB.ifg_set_precision(self.h, precision), "SetPrecision")
         self.ncols = self.nrows = 0
+        self._nparams = 0          # GetNParams(), kept current by AddFunction
         self._stat = np.zeros(1)   # GetFitStatistic's result slot
         self._stat_addr = _addr(self._stat)
 
@@ -176,13 +177,23 @@ class ModelObjectGPU:
 
     # ---- setup (same call order as core/setup_model_object.cpp:58-153)
     def AddFunction(self, name, new_set=False):
-        return self._chk(_LIB.ifg_add_function(self.h, name.encode(), int(new_set)), "AddFunction")
+        st = self._chk(_LIB.ifg_add_function(self.h, name.encode(), int(new_set)), "AddFunction")
+        self._nparams = _LIB.ifg_get_nparams(self.h)
+        return st
 
     def SetSubsampling(self, on):
         return self._chk(_LIB.ifg_set_subsampling(self.h, int(on)), "SetSubsampling")
 
     def GetNParams(self):
         return _LIB.ifg_get_nparams(self.h)
+
+    def _vec(self, params):
+        """params as a C-contiguous float64 vector of GetNParams() values: the library
+        reads exactly that many doubles, so a shorter vector is refused here."""
+        p = _f64(params)
+        if p.ndim != 1 or p.shape[0] != self._nparams:
+            raise ValueError("expected %d parameters, got shape %s" % (self._nparams, p.shape))
+        return p
 
     def AddPSFVector(self, psf, normalize=True):
         psf = np.ascontiguousarray(psf, dtype=np.float64)
@@ -259,7 +270,7 @@ class ModelObjectGPU:
 
     # ---- hot path
     def CreateModelImage(self, params):
-        p = np.ascontiguousarray(params, dtype=np.float64)
+        p = self._vec(params)
         self._chk(_LIB.ifg_create_model_image(self.h, _ptr(p)), "CreateModelImage")
 
     def GetModelImageVector(self):
@@ -268,13 +279,15 @@ class ModelObjectGPU:
         return out
 
     def ComputeDeviates(self, params):
-        p = np.ascontiguousarray(params, dtype=np.float64)
+        p = self._vec(params)
         out = np.empty(self.nrows * self.ncols)
         self._chk(_LIB.ifg_compute_deviates(self.h, _ptr(p), _ptr(out)), "ComputeDeviates")
         return out
 
     def GetFitStatistic(self, params):
         p = _f64(params)
+        if p.ndim != 1 or p.shape[0] != self._nparams:
+            self._vec(p)   # raises
         st = _FAST.ifg_fit_statistic(self.h, _addr(p), self._stat_addr)
         if st < 0:
             self._chk(st, "GetFitStatistic")
@@ -282,8 +295,8 @@ class ModelObjectGPU:
 
     def GetFitStatisticBatch(self, params):
         p = np.ascontiguousarray(params, dtype=np.float64)
-        if p.ndim != 2:
-            raise ValueError("params must be (n_vectors, nParams)")
+        if p.ndim != 2 or p.shape[1] != self._nparams:
+            raise ValueError("params must be (n_vectors, %d)" % self._nparams)
         out = np.zeros(p.shape[0])
         self._chk(_LIB.ifg_fit_statistic_batch(self.h, _ptr(p), p.shape[0], _ptr(out)), "GetFitStatisticBatch")
         return out
@@ -296,7 +309,7 @@ class ModelObjectGPU:
 
     def Jacobian(self, params, ifree=None):
         """(deviates, J) with J[:, j] = d(deviate)/d(param ifree[j]) (nData x nFree)."""
-        p = np.ascontiguousarray(params, dtype=np.float64)
+        p = self._vec(params)
         fr = self._ifree(ifree)
         n = self.nrows * self.ncols
         fvec = np.empty(n)
@@ -307,7 +320,7 @@ class ModelObjectGPU:
 
     def NormalEquations(self, params, ifree=None):
         """(J^T J, J^T r, statistic) reduced on the GPU."""
-        p = np.ascontiguousarray(params, dtype=np.float64)
+        p = self._vec(params)
         fr = self._ifree(ifree)
         jtj = np.empty((fr.size, fr.size))
         jtr = np.empty(fr.size)
@@ -318,7 +331,7 @@ class ModelObjectGPU:
 
     def GetFitStatisticGradient(self, params):
         """(gradient of GetFitStatistic w.r.t. every parameter, statistic)."""
-        p = np.ascontiguousarray(params, dtype=np.float64)
+        p = self._vec(params)
         g = np.empty(self.GetNParams())
         stat = np.zeros(1)
         self._chk(_LIB.ifg_fit_statistic_gradient(self.h, _ptr(p), _ptr(g), _ptr(stat)), "GetFitStatisticGradient")
@@ -346,7 +359,7 @@ class ModelObjectGPU:
         """Levenberg-Marquardt fit.  method "mpfit": MINPACK LMDIF as imfit runs it, with
         jacobian "fd" (one GPU evaluation per free parameter) or "analytic"; method
         "normal": LM on GPU-reduced normal equations.  Returns (params, LMResult)."""
-        x = np.array(params, dtype=np.float64)
+        x = np.array(self._vec(params), dtype=np.float64)
         pinfo = self._pinfo(limits, fixed, x.size)
         cfg = LMConfig(ftol, xtol, 0.0, maxiter, 0)
         res = LMResult()

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/imfit_gpu.h"
+#include "ifg_guard.h"
 #include "ifg_internal.h"
 #include "ifg_launch.h"
 
@@ -371,7 +372,7 @@ std::vector<double> NormalizedPSF(const double* psf, long n, bool normalize) {
 struct ifg_model {
   int device = 0;
   hipStream_t stream = nullptr;
-  std::string err;
+  mutable std::string err;
   int precision = IFG_PRECISION_FP64;
   // model definition
   std::vector<int> types, fsetStart;
@@ -1378,8 +1379,8 @@ int RunDerivDispatch(ifg_model* m, const double* params, const std::vector<int>&
 // ======================================================================= C ABI
 extern "C" {
 
-ifg_model* ifg_create(int device) {
-  ifg_model* m = new ifg_model();
+ifg_model* ifg_create(int device) try {
+  std::unique_ptr<ifg_model> m(new ifg_model());
   m->device = device;
   m->curEvents = &m->pendingEvents;
   const char* ng = getenv("IFG_NO_GRAPH");
@@ -1387,26 +1388,33 @@ ifg_model* ifg_create(int device) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
     m->err = "no HIP device / stream";
   }
-  return m;
+  return m.release();
+} catch (...) {
+  return nullptr;
 }
 
-void ifg_destroy(ifg_model* m) {
+void ifg_destroy(ifg_model* m) try {
   if (!m) return;
   (void)hipSetDevice(m->device);
   delete m;
+} catch (...) {
 }
 
 const char* ifg_last_error(const ifg_model* m) { return m ? m->err.c_str() : "null handle"; }
 
-int ifg_set_precision(ifg_model* m, int precision) {
+int ifg_set_precision(ifg_model* m, int precision) try {
+  if (!m) return IFG_ERR_ARG;
   if (precision != IFG_PRECISION_FP64 && precision != IFG_PRECISION_FP32)
     return m->fail(IFG_ERR_ARG, "unknown precision");
   if (m->devReady && precision != m->precision) return m->fail(IFG_ERR_STATE, "precision set after first evaluation");
   m->precision = precision;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set) {
+int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set) try {
+  if (!m) return IFG_ERR_ARG;
   const int t = TypeOf(name);
   if (!t) return m->fail(IFG_ERR_ARG, std::string("unknown function ") + (name ? name : "(null)"));
   if (m->nComp() >= IFG_MAXCOMP) return m->fail(IFG_ERR_UNSUPPORTED, "too many components");
@@ -1416,16 +1424,22 @@ int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set
   m->fsetStart.push_back(newSet);
   m->nParamsTot += NParamsOf(t) + (newSet ? 2 : 0);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_set_subsampling(ifg_model* m, int do_subsampling) {
+int ifg_set_subsampling(ifg_model* m, int do_subsampling) try {
+  if (!m) return IFG_ERR_ARG;
   m->doSubsampling = do_subsampling != 0;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_get_nparams(const ifg_model* m) { return m->nParamsTot; }
+int ifg_get_nparams(const ifg_model* m) { return m ? m->nParamsTot : 0; }
 
-int ifg_add_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int normalize) {
+int ifg_add_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int normalize) try {
+  if (!m) return IFG_ERR_ARG;
   if (!psf || n_columns < 1 || n_rows < 1) return m->fail(IFG_ERR_ARG, "bad PSF dimensions");
   const long n = (long)n_columns * n_rows;
   double sum = 0.0;
@@ -1440,9 +1454,12 @@ int ifg_add_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int 
   m->Pr = n_rows;
   m->doConv = true;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_setup_model_image(ifg_model* m, int n_columns, int n_rows) {
+int ifg_setup_model_image(ifg_model* m, int n_columns, int n_rows) try {
+  if (!m) return IFG_ERR_ARG;
   if (n_columns < 1 || n_rows < 1) return m->fail(IFG_ERR_ARG, "bad image size");
   if (m->setupDone) return m->fail(IFG_ERR_STATE, "image already set up");
   m->W = n_columns;
@@ -1450,9 +1467,12 @@ int ifg_setup_model_image(ifg_model* m, int n_columns, int n_rows) {
   m->nData = (long)n_columns * n_rows;
   m->setupDone = true;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_add_image_data(ifg_model* m, const double* data, int n_columns, int n_rows) {
+int ifg_add_image_data(ifg_model* m, const double* data, int n_columns, int n_rows) try {
+  if (!m) return IFG_ERR_ARG;
   if (!data) return m->fail(IFG_ERR_ARG, "null data");
   int st = ifg_setup_model_image(m, n_columns, n_rows);
   if (st < 0) return st;
@@ -1462,10 +1482,13 @@ int ifg_add_image_data(ifg_model* m, const double* data, int n_columns, int n_ro
   m->weight.assign(m->nData, 1.0);
   m->extra.assign(m->nData, 0.0);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
 int ifg_add_image_characteristics(ifg_model* m, double gain, double read_noise, double exp_time, int n_combined,
-                                  double original_sky) {
+                                  double original_sky) try {
+  if (!m) return IFG_ERR_ARG;
   if (!(gain > 0.0) || !(read_noise >= 0.0) || !(exp_time > 0.0) || n_combined < 1 || !(original_sky >= 0.0))
     return m->fail(IFG_ERR_ARG, "bad image characteristics");   // core/model_object.cpp:535-537 asserts
   m->gain = gain;
@@ -1476,9 +1499,12 @@ int ifg_add_image_characteristics(ifg_model* m, double gain, double read_noise, 
   m->effGain = gain * exp_time * n_combined;
   m->rnAdu2 = read_noise * read_noise / (m->effGain * m->effGain);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_add_error_vector(ifg_model* m, const double* errors, long n_values, int input_type) {
+int ifg_add_error_vector(ifg_model* m, const double* errors, long n_values, int input_type) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m->dataSet || n_values != m->nData || !errors) return m->fail(IFG_ERR_ARG, "error image size mismatch");
   std::vector<double> w(errors, errors + n_values);
   for (long z = 0; z < n_values; z++) {   // core/model_object.cpp:597-617
@@ -1493,9 +1519,12 @@ int ifg_add_error_vector(ifg_model* m, const double* errors, long n_values, int 
   m->externalErrors = true;
   m->weightSet = true;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_add_mask_vector(ifg_model* m, const double* mask, long n_values, int input_type) {
+int ifg_add_mask_vector(ifg_model* m, const double* mask, long n_values, int input_type) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m->dataSet || n_values != m->nData || !mask) return m->fail(IFG_ERR_ARG, "mask size mismatch");
   if (input_type != IFG_MASK_ZERO_IS_GOOD && input_type != IFG_MASK_ZERO_IS_BAD)
     return m->fail(IFG_ERR_ARG, "unknown mask input type");
@@ -1510,10 +1539,13 @@ int ifg_add_mask_vector(ifg_model* m, const double* mask, long n_values, int inp
   }
   m->maskExists = true;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
 int ifg_add_oversampled_psf(ifg_model* m, const double* psf, int n_columns, int n_rows, int scale, int x1, int x2,
-                            int y1, int y2, int normalize) {
+                            int y1, int y2, int normalize) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m->setupDone) return m->fail(IFG_ERR_STATE, "oversampled PSF added before the image");
   if (!psf || n_columns < 1 || n_rows < 1 || scale < 1 || !(x1 < x2) || !(y1 < y2))
     return m->fail(IFG_ERR_ARG, "bad oversampled-PSF arguments");
@@ -1545,30 +1577,42 @@ int ifg_add_oversampled_psf(ifg_model* m, const double* psf, int n_columns, int 
                    "outside the region");
   m->regions.push_back(std::move(R));
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_use_cash_statistic(ifg_model* m) {
+int ifg_use_cash_statistic(ifg_model* m) try {
+  if (!m) return IFG_ERR_ARG;
   m->useCash = true;
   m->weight.assign(m->nData, 1.0);
   m->extra.assign(m->nData, 0.0);
   m->weightSet = true;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_use_poisson_mlr(ifg_model* m) {
+int ifg_use_poisson_mlr(ifg_model* m) try {
+  if (!m) return IFG_ERR_ARG;
   m->poissonMLR = true;
   return ifg_use_cash_statistic(m);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_use_model_errors(ifg_model* m) {
+int ifg_use_model_errors(ifg_model* m) try {
+  if (!m) return IFG_ERR_ARG;
   m->modelErrors = true;
   m->dataErrors = false;
   m->weight.assign(m->nData, 1.0);
   m->weightSet = true;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_final_setup(ifg_model* m) {   // core/model_object.cpp:967-1079
+int ifg_final_setup(ifg_model* m) try {   // core/model_object.cpp:967-1079
+  if (!m) return IFG_ERR_ARG;
   if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
   const long n = m->nData;
   int status = IFG_OK;
@@ -1630,15 +1674,18 @@ int ifg_final_setup(ifg_model* m) {   // core/model_object.cpp:967-1079
   m->devReady = false;
   m->plans.clear();
   return status;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
 int ifg_which_fit_statistic(const ifg_model* m) {
+  if (!m) return IFG_ERR_ARG;
   if (m->useCash) return m->poissonMLR ? IFG_FITSTAT_POISSON_MLR : IFG_FITSTAT_CASH;
   return IFG_FITSTAT_CHISQUARE;
 }
 
-long ifg_get_ndata(const ifg_model* m) { return m->nData; }
-long ifg_get_nvalid(const ifg_model* m) { return m->nValid; }
+long ifg_get_ndata(const ifg_model* m) { return m ? m->nData : 0; }
+long ifg_get_nvalid(const ifg_model* m) { return m ? m->nValid : 0; }
 
 // ---- bootstrap resampling (core/model_object.cpp:2036-2087, core/bootstrap_errors.cpp:150-232)
 // The reference draws samples from one process-wide Mersenne Twister (MT19937,
@@ -1647,9 +1694,10 @@ long ifg_get_nvalid(const ifg_model* m) { return m->nValid; }
 static std::mutex gRngMu;
 static std::mt19937 gRng(5489u);
 
-void ifg_init_genrand(unsigned long seed) {
+void ifg_init_genrand(unsigned long seed) try {
   std::lock_guard<std::mutex> lock(gRngMu);
   gRng.seed((std::mt19937::result_type)(seed & 0xffffffffUL));
+} catch (...) {
 }
 
 static int SetBootstrapSample(ifg_model* m, std::vector<long>&& idx) {
@@ -1669,7 +1717,8 @@ static int BootstrapReady(ifg_model* m) {
   return IFG_OK;
 }
 
-int ifg_make_bootstrap_sample(ifg_model* m) {
+int ifg_make_bootstrap_sample(ifg_model* m) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m) return IFG_ERR_ARG;
   int st = BootstrapReady(m);
   if (st < 0) return st;
@@ -1685,13 +1734,19 @@ int ifg_make_bootstrap_sample(ifg_model* m) {
     }
   }
   return SetBootstrapSample(m, std::move(idx));
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_use_bootstrap(ifg_model* m) {
+int ifg_use_bootstrap(ifg_model* m) try {
+  if (!m) return IFG_ERR_ARG;
   return ifg_make_bootstrap_sample(m);   // UseBootstrap draws a first sample
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_set_bootstrap_indices(ifg_model* m, const long* indices, long n) {
+int ifg_set_bootstrap_indices(ifg_model* m, const long* indices, long n) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m || (!indices && n > 0)) return IFG_ERR_ARG;
   int st = BootstrapReady(m);
   if (st < 0) return st;
@@ -1701,16 +1756,22 @@ int ifg_set_bootstrap_indices(ifg_model* m, const long* indices, long n) {
     if (b < 0 || b >= m->nData || !(m->weight[b] > 0.0))
       return m->fail(IFG_ERR_ARG, "bootstrap index outside the image or on a masked pixel");
   return SetBootstrapSample(m, std::move(idx));
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_get_bootstrap_indices(const ifg_model* m, long* indices) {
+int ifg_get_bootstrap_indices(const ifg_model* m, long* indices) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m || !indices) return IFG_ERR_ARG;
   if (!m->bootstrap) return IFG_ERR_STATE;
   std::copy(m->bootIdx.begin(), m->bootIdx.end(), indices);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_stop_bootstrap(ifg_model* m) {
+int ifg_stop_bootstrap(ifg_model* m) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m) return IFG_ERR_ARG;
   if (!m->bootstrap) return IFG_OK;
   m->bootstrap = false;
@@ -1719,9 +1780,12 @@ int ifg_stop_bootstrap(ifg_model* m) {
   if (!m->devReady) return IFG_OK;
   HIPCHK(hipStreamSynchronize(m->stream), "bootstrap weights");
   return m->precision == IFG_PRECISION_FP64 ? UploadWeights<double>(m) : UploadWeights<float>(m);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_create_model_image(ifg_model* m, const double* params) {
+int ifg_create_model_image(ifg_model* m, const double* params) try {
+  if (!m) return IFG_ERR_ARG;
   m->lastParams.assign(params, params + m->nParamsTot);
   std::vector<double> model(m->nData);
   int st = Run(m, params, 1, nullptr, nullptr, model.data());
@@ -1729,9 +1793,12 @@ int ifg_create_model_image(ifg_model* m, const double* params) {
   m->modelValid = true;
   m->modelHost.swap(model);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_get_model_image(ifg_model* m, double* out) {
+int ifg_get_model_image(ifg_model* m, double* out) try {
+  if (!m) return IFG_ERR_ARG;
   if (!out) return m->fail(IFG_ERR_ARG, "null output");
   if (!m->modelValid) {
     if (m->lastParams.empty()) return m->fail(IFG_ERR_STATE, "no model computed yet");
@@ -1741,41 +1808,56 @@ int ifg_get_model_image(ifg_model* m, double* out) {
   }
   std::memcpy(out, m->modelHost.data(), sizeof(double) * m->nData);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_compute_deviates(ifg_model* m, const double* params, double* y) {
+int ifg_compute_deviates(ifg_model* m, const double* params, double* y) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
   if (!y) return m->fail(IFG_ERR_ARG, "null output");
   m->lastParams.assign(params, params + m->nParamsTot);
   m->modelValid = false;
   return Run(m, params, 1, nullptr, y, nullptr);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_fit_statistic(ifg_model* m, const double* params, double* out) {
+int ifg_fit_statistic(ifg_model* m, const double* params, double* out) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
   if (!out) return m->fail(IFG_ERR_ARG, "null output");
   m->lastParams.assign(params, params + m->nParamsTot);
   m->modelValid = false;
   return Run(m, params, 1, out, nullptr, nullptr);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, double* out) {
+int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, double* out) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
   if (!out) return m->fail(IFG_ERR_ARG, "null output");
   m->modelValid = false;
   return Run(m, params, n_vectors, out, nullptr, nullptr);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_jacobian(ifg_model* m, const double* params, int n_free, const int* ifree, double* fvec, double* fjac) {
+int ifg_jacobian(ifg_model* m, const double* params, int n_free, const int* ifree, double* fvec, double* fjac) try {
+  if (!m) return IFG_ERR_ARG;
   std::vector<int> fr;
   int st = CheckFree(m, n_free, ifree, fr);
   if (st < 0) return st;
   if (!fjac) return m->fail(IFG_ERR_ARG, "null fjac");
   return RunDerivDispatch(m, params, fr, PF_JAC, nullptr, fvec, fjac, nullptr, nullptr);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
 int ifg_normal_equations(ifg_model* m, const double* params, int n_free, const int* ifree, double* jtj, double* jtr,
-                         double* stat) {
+                         double* stat) try {
+  if (!m) return IFG_ERR_ARG;
   std::vector<int> fr;
   int st = CheckFree(m, n_free, ifree, fr);
   if (st < 0) return st;
@@ -1793,28 +1875,40 @@ int ifg_normal_equations(ifg_model* m, const double* params, int n_free, const i
   if (jtr)
     for (int a = 0; a < n_free; a++) jtr[a] = packed[e + a];
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_fit_statistic_gradient(ifg_model* m, const double* params, double* grad, double* stat) {
+int ifg_fit_statistic_gradient(ifg_model* m, const double* params, double* grad, double* stat) try {
+  if (!m) return IFG_ERR_ARG;
   std::vector<int> fr;
   int st = CheckFree(m, m->nParamsTot, nullptr, fr);
   if (st < 0) return st;
   if (!grad) return m->fail(IFG_ERR_ARG, "null gradient");
   return RunDerivDispatch(m, params, fr, PF_GRAD, stat, nullptr, nullptr, nullptr, grad);
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_set_kernel_timing(ifg_model* m, int on) {
+int ifg_set_kernel_timing(ifg_model* m, int on) try {
+  if (!m) return IFG_ERR_ARG;
   m->timing = on != 0;
   m->kernelTimes.clear();
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_set_graph_mode(ifg_model* m, int on) {
+int ifg_set_graph_mode(ifg_model* m, int on) try {
+  if (!m) return IFG_ERR_ARG;
   m->useGraphs = on != 0;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_len, double* total_ms, long* launches) {
+int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_len, double* total_ms, long* launches) try {
+  if (!m) return IFG_ERR_ARG;
   if (index < 0 || index >= (int)m->kernelTimes.size()) return 0;
   auto it = m->kernelTimes.begin();
   std::advance(it, index);
@@ -1825,6 +1919,8 @@ int ifg_get_kernel_timing(const ifg_model* m, int index, char* name, int name_le
   if (total_ms) *total_ms = it->second.first;
   if (launches) *launches = it->second.second;
   return (int)m->kernelTimes.size();
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
 }  // extern "C"
@@ -1876,7 +1972,7 @@ struct BatchPlan {
 struct ifg_batch {
   int device = 0;
   hipStream_t stream = nullptr;
-  std::string err;
+  mutable std::string err;
   std::vector<ifg_model*> models;
   std::vector<std::unique_ptr<BatchPlan>> plans;
   bool useGraphs = true;
@@ -2468,44 +2564,68 @@ int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double*
 }
 int BatchSize(const ifg_batch* b) { return (int)b->models.size(); }
 ifg_model* BatchModel(ifg_batch* b, int k) { return b->models[k]; }
+void SetHandleError(const ifg_model* m, const char* msg) noexcept {
+  try {
+    m->err = msg;
+  } catch (...) {
+  }
+}
+void SetHandleError(const ifg_batch* b, const char* msg) noexcept {
+  try {
+    b->err = msg;
+  } catch (...) {
+  }
+}
 }  // namespace ifg
 
 extern "C" {
 
-ifg_batch* ifg_batch_create(int device) {
-  ifg_batch* b = new ifg_batch();
+ifg_batch* ifg_batch_create(int device) try {
+  std::unique_ptr<ifg_batch> b(new ifg_batch());
   b->device = device;
   const char* ng = getenv("IFG_NO_GRAPH");
   if (ng && ng[0] == '1') b->useGraphs = false;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) b->err = "no HIP device / stream";
-  return b;
+  return b.release();
+} catch (...) {
+  return nullptr;
 }
 
-void ifg_batch_destroy(ifg_batch* b) { delete b; }
+void ifg_batch_destroy(ifg_batch* b) try {
+  delete b;
+} catch (...) {
+}
 
 const char* ifg_batch_last_error(const ifg_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
-int ifg_batch_add(ifg_batch* b, ifg_model* m) {
+int ifg_batch_add(ifg_batch* b, ifg_model* m) try {
+  if (!b) return IFG_ERR_ARG;
   if (!b || !m) return IFG_ERR_ARG;
   if (!b->stream) return b->fail(IFG_ERR_HIP, "batch has no stream");
   b->plans.clear();
   b->models.push_back(m);
   return (int)b->models.size() - 1;
+} catch (...) {
+  return ExceptionStatus(b);
 }
 
 int ifg_batch_size(const ifg_batch* b) { return b ? (int)b->models.size() : 0; }
 
-int ifg_batch_fit_statistic(ifg_batch* b, const double* params, double* out) {
+int ifg_batch_fit_statistic(ifg_batch* b, const double* params, double* out) try {
+  if (!b) return IFG_ERR_ARG;
   if (!b || !out) return IFG_ERR_ARG;
   std::vector<double> stat(b->models.size());
   const int st = ifg::BatchEvaluate(b, params, nullptr, stat.data(), nullptr, nullptr, nullptr);
   if (st < 0) return st;
   std::memcpy(out, stat.data(), sizeof(double) * stat.size());
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(b);
 }
 
 int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_free, const int* ifree,
-                               double* normal, double* stat) {
+                               double* normal, double* stat) try {
+  if (!b) return IFG_ERR_ARG;
   if (!b || !normal || !stat) return IFG_ERR_ARG;
   std::vector<std::vector<int>> fr(b->models.size());
   long off = 0;
@@ -2528,12 +2648,17 @@ int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_
   for (const auto& f : fr) total += (long)f.size() * (f.size() + 1) / 2 + (long)f.size();
   std::memcpy(normal, nrm, sizeof(double) * total);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(b);
 }
 
-int ifg_batch_set_graph_mode(ifg_batch* b, int on) {
+int ifg_batch_set_graph_mode(ifg_batch* b, int on) try {
+  if (!b) return IFG_ERR_ARG;
   if (!b) return IFG_ERR_ARG;
   b->useGraphs = on != 0;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(b);
 }
 
 }  // extern "C"
@@ -2640,7 +2765,8 @@ int TimeModel(ifg_model* m, const double* params, int nVec, double* seconds) {
 
 extern "C" {
 
-int ifg_autotune(ifg_model* m, const double* params, int n_vectors) {
+int ifg_autotune(ifg_model* m, const double* params, int n_vectors) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m || !params || n_vectors < 1) return IFG_ERR_ARG;
   if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
   std::vector<double> out(n_vectors);
@@ -2695,9 +2821,12 @@ int ifg_autotune(ifg_model* m, const double* params, int n_vectors) {
   std::lock_guard<std::mutex> lock(gTuneMu);
   gTuned[key] = rec;
   return trials;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* cols_per_block, int* rows_per_block) {
+int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* cols_per_block, int* rows_per_block) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m || grid < 0) return IFG_ERR_ARG;
   const int which = batch_grids ? 1 : 0;
   const ConvGrid* cg = nullptr;
@@ -2708,9 +2837,12 @@ int ifg_get_launch_shape(const ifg_model* m, int grid, int batch_grids, int* col
   if (cols_per_block) *cols_per_block = s.C;
   if (rows_per_block) *rows_per_block = s.rows1;
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
-int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_block, int rows_per_block) {
+int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_block, int rows_per_block) try {
+  if (!m) return IFG_ERR_ARG;
   if (!m || grid < 0) return IFG_ERR_ARG;
   const int which = batch_grids ? 1 : 0;
   ConvGrid* cg = nullptr;
@@ -2731,6 +2863,8 @@ int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_b
   SetShape(m, *cg, s, split);
   m->plans.clear();   // plans hold the launch shapes
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
 }
 
 }  // extern "C"
@@ -2780,7 +2914,8 @@ int TimeBatch(ifg_batch* b, const double* params, double* seconds) {
 
 extern "C" {
 
-int ifg_batch_autotune(ifg_batch* b, const double* params, int* cols_per_block, double* rows_factor) {
+int ifg_batch_autotune(ifg_batch* b, const double* params, int* cols_per_block, double* rows_factor) try {
+  if (!b) return IFG_ERR_ARG;
   if (!b || !params || b->models.empty()) return IFG_ERR_ARG;
   std::vector<double> out(b->models.size());
   int st = ifg::BatchEvaluate(b, params, nullptr, out.data(), nullptr, nullptr, nullptr);   // builds the grids
@@ -2816,6 +2951,8 @@ int ifg_batch_autotune(ifg_batch* b, const double* params, int* cols_per_block, 
   if (cols_per_block) *cols_per_block = bestC;
   if (rows_factor) *rows_factor = bestF;
   return trials;
+} catch (...) {
+  return ExceptionStatus(b);
 }
 
 }  // extern "C"

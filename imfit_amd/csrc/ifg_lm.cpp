@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/imfit_gpu.h"
+#include "ifg_guard.h"
 
 // batch access (ifg_api.cpp)
 namespace ifg {
@@ -639,7 +640,8 @@ struct NormalLM {
 extern "C" {
 
 int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
-               int jacobian_mode, ifg_lm_result* res) {
+               int jacobian_mode, ifg_lm_result* res) try {
+  if (!model) return IFG_ERR_ARG;
   const int npar = ifg_get_nparams(model);
   const long m = ifg_get_ndata(model);
   double ftol = 1e-10, xtol = 1e-10, gtol = 1e-10, stepfactor = 100.0;   // mpfit.cpp:377-388
@@ -789,12 +791,15 @@ done:
   for (int j = 0; j < nfree; j++) xall[L.ifree[j]] = x[j];
   Finish(res, info, iter, nfev, nfree, fnorm, fnorm1, orignorm, L, x.data(), pinfo, npar, xall);
   return info;
+} catch (...) {
+  return ifg::ExceptionStatus(model);
 }
 
 // Levenberg-Marquardt on the normal equations, with J^T J and J^T r reduced on the
 // GPU (NormalLM: mpfit's trust-region step control on the normal equations).
 int ifg_fit_lm_normal(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
-                      ifg_lm_result* res) {
+                      ifg_lm_result* res) try {
+  if (!model) return IFG_ERR_ARG;
   NormalLM S;
   int st = S.Init(ifg_get_nparams(model), xall, pinfo, cfg);
   if (st) return st;
@@ -814,10 +819,13 @@ int ifg_fit_lm_normal(ifg_model* model, double* xall, const ifg_param_info* pinf
   }
   S.Result(xall, pinfo, res);
   return S.info;
+} catch (...) {
+  return ifg::ExceptionStatus(model);
 }
 
 int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
-                     ifg_lm_result* res) {
+                     ifg_lm_result* res) try {
+  if (!b) return IFG_ERR_ARG;
   if (!b || !params) return IFG_ERR_ARG;
   const int nM = ifg::BatchSize(b);
   if (nM < 1) return IFG_ERR_STATE;
@@ -878,6 +886,8 @@ int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, 
     if (sk <= 0 || sk == IFG_LM_MAXITER) notConverged++;
   }
   return notConverged;
+} catch (...) {
+  return ifg::ExceptionStatus(b);
 }
 
 }  // extern "C"

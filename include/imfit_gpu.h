@@ -8,7 +8,9 @@
  * ComputeDeviates, GetFitStatistic).  Plain pointers and sizes only; every call
  * returns an int status, negative on error (message via ifg_last_error), like
  * the reference's int-status setup methods (e.g. core/model_object.cpp:517-523,
- * 839-856).  Caller buffers are copied, never modified.
+ * 839-856).  No C++ exception leaves any entry point, and a null handle is
+ * IFG_ERR_ARG (the getters of counts return 0 for it).  Caller buffers are
+ * copied, never modified.
  *
  * Pixel layout everywhere: row-major, x (column) fastest, as the reference's
  * pixel vectors.  Parameter vectors use the reference layout: X0,Y0 at the start
@@ -35,6 +37,8 @@ typedef struct ifg_model ifg_model;
 #define IFG_ERR_DATA (-3)        /* FinalSetupForFitting found bad data / weights */
 #define IFG_ERR_HIP (-4)         /* HIP runtime error (message has the HIP error string) */
 #define IFG_ERR_UNSUPPORTED (-5) /* configuration outside what the library implements */
+#define IFG_ERR_NOMEM (-6)       /* host memory exhausted inside the call */
+#define IFG_ERR_INTERNAL (-7)    /* any other failure inside the library (message via ifg_last_error) */
 
 /* input-type codes: same values as core/definitions.h:75-82 */
 #define IFG_WEIGHTS_ARE_SIGMAS 100

@@ -246,6 +246,8 @@ FITS_INPUTS = [
     "twogaussian_psf+2osamp_noisy.fits", "psf_moffat_35.fits", "psf_moffat_35_oversamp3.fits",
     "oversamp_test4.fits", "psf_standard.fits", "psf_oversamp.fits",
     "testimage_3x3_nan.fits", "testimage_3x3_allzeros.fits",
+    # do_imfit_tests:138-139,330 (data-based vs model-based weights)
+    "flatsky_128x128.fits",
     # do_multimfit_tests.sh:59-85
     "multimfit_reference/smallDataImage1a.fits", "multimfit_reference/smallDataImage1b.fits",
     "multimfit_reference/smallgauss_5x5_0.fits", "multimfit_reference/smallgauss_5x5_1.fits",
@@ -260,7 +262,7 @@ CONFIG_INPUTS = [
     "imfit_reference/imfit_config_ic3478_64x64b.dat", "imfit_reference/imfit_config_ic3478_64x64c.dat",
     "imfit_reference/imfit_config_n3073.dat", "imfit_reference/config_imfit_2gauss_small.dat",
     "imfit_reference/config_imfit_gauss-oversample-test2.dat", "imfit_reference/config_imfit_poisson.dat",
-    "imfit_reference/config_imfit_badparamline.dat",
+    "imfit_reference/config_imfit_badparamline.dat", "imfit_reference/config_imfit_flatsky.dat",
     # do_makeimage_tests:48-98
     "makeimage_reference/config_biggertest_4c.dat", "makeimage_reference/config_makeimage_gensersic512.dat",
     "makeimage_reference/config_makeimage_sersic+exp512.dat",
