@@ -100,6 +100,7 @@ _SIGS["ifg_get_bootstrap_indices"] = ([_vp, C.POINTER(C.c_long)], C.c_int)
 _SIGS["ifg_stop_bootstrap"] = ([_vp], C.c_int)
 _SIGS["ifg_set_launch_shape"] = ([_vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int)
 _SIGS["ifg_batch_autotune"] = ([_vp, _dp, C.POINTER(C.c_int), _dp], C.c_int)
+_SIGS["ifg_fit_statistic_noalloc"] = ([_vp, _dp, _dp], C.c_int)
 
 _LIB = C.CDLL(LIB_PATH)
 for _name, (_args, _res) in _SIGS.items():
@@ -292,6 +293,15 @@ class ModelObjectGPU:
         if st < 0:
             self._chk(st, "GetFitStatistic")
         return float(self._stat[0])
+
+    def GetFitStatisticNoAlloc(self, params):
+        """GetFitStatistic through ifg_fit_statistic_noalloc: no device or pinned
+        allocation in the call (per-call memory from the library's fixed per-GPU work
+        area).  The handle needs one ordinary evaluation first."""
+        p = self._vec(params)
+        out = np.zeros(1)
+        self._chk(_LIB.ifg_fit_statistic_noalloc(self.h, _ptr(p), _ptr(out)), "GetFitStatisticNoAlloc")
+        return float(out[0])
 
     def GetFitStatisticBatch(self, params):
         p = np.ascontiguousarray(params, dtype=np.float64)

@@ -273,20 +273,28 @@ std::vector<cpx<double>> PlanTwiddles(const FFTPlan& pl) {
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
+  bool owned = true;   // false: a view into memory the buffer does not own (no hipFree)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
   }
   hipError_t ensure(size_t bytes) {
     if (bytes <= n) return hipSuccess;
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
     hipError_t e = hipMalloc(&p, bytes);
     if (e == hipSuccess) n = bytes;
     return e;
+  }
+  void view(void* q, size_t bytes) {
+    if (p && owned) (void)hipFree(p);
+    p = q;
+    n = bytes;
+    owned = false;
   }
   template <typename T>
   T* as() const { return reinterpret_cast<T*>(p); }
@@ -341,15 +349,35 @@ struct Plan {
   long tailWaves = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // A plan built in a caller-independent work area (WorkArea below) allocates
+  // nothing: its device tables and call staging sit at the front of the area, are
+  // written into tableHost on the host and reach the device in one copy per call;
+  // results come back into outHost.
+  bool inArea = false;
+  std::vector<unsigned char> tableHost;
+  size_t tableBytes = 0;
+  void* tableDev = nullptr;
+  std::vector<double> outHost;
   Plan() = default;
   Plan(const Plan&) = delete;
   Plan& operator=(const Plan&) = delete;
   ~Plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (pinnedStage) (void)hipHostFree(pinnedStage);
-    if (pinnedOut) (void)hipHostFree(pinnedOut);
+    if (!inArea && pinnedStage) (void)hipHostFree(pinnedStage);
+    if (!inArea && pinnedOut) (void)hipHostFree(pinnedOut);
   }
+};
+
+// Device memory of a fixed size that a plan carves its buffers from instead of
+// allocating: the first tableCap bytes take the tables and the call staging (with
+// a host mirror of the same layout), the rest the planes, sub-pixel values,
+// partial sums and results.
+struct WorkArea {
+  char* base = nullptr;
+  size_t size = 0, tableCap = 0;
+  size_t tableUsed = 0, bulkUsed = 0;
+  unsigned char* mirror = nullptr;
 };
 
 // Kahan-normalised PSF (core/convolver.cpp:296-311)
@@ -403,6 +431,8 @@ struct ifg_model {
   DevBuf data_d, weight_d, extra_d;
   DevBuf devkeep_d, dfac_d;   // forward-pass outputs consumed by derivative passes
   std::vector<std::unique_ptr<Plan>> plans;
+  std::unique_ptr<Plan> areaPlan;   // GetFitStatistic plan in the no-allocation work area
+  std::vector<int> areaNeed;
   bool useGraphs = true;
   std::vector<double> modelHost;   // last CreateModelImage result
   std::vector<double> lastParams;
@@ -872,8 +902,38 @@ void LayoutBlocks(std::vector<Task>& tk, int first, int count, std::vector<int2>
   }
 }
 
+// A plan's device buffer: allocated (wa null) or carved from the work area, 256-B
+// aligned; table buffers also have a place in the area's host mirror.
+int PlanBuf(ifg_model* m, WorkArea* wa, DevBuf& b, size_t bytes, bool table, const char* what) {
+  if (!wa) {
+    HIPCHK(b.ensure(bytes), what);
+    return IFG_OK;
+  }
+  size_t& used = table ? wa->tableUsed : wa->bulkUsed;
+  const size_t cap = table ? wa->tableCap : wa->size - wa->tableCap;
+  const size_t off = (used + 255) / 256 * 256;
+  if (off > cap || bytes > cap - off)
+    return m->fail(IFG_ERR_UNSUPPORTED, std::string("evaluation does not fit the no-allocation work area (") +
+                                            std::to_string(wa->size >> 20) + " MiB): " + what);
+  b.view(wa->base + (table ? 0 : wa->tableCap) + off, bytes);
+  used = off + bytes;
+  return IFG_OK;
+}
+
+// Contents of a table buffer: copied now (allocated plans) or written into the
+// work area's host mirror, which each call copies to the device.
+int PlanUpload(ifg_model* m, WorkArea* wa, DevBuf& b, const void* src, size_t bytes, const char* what) {
+  if (!wa) {
+    HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), what);
+    return IFG_OK;
+  }
+  std::memcpy(wa->mirror + (static_cast<char*>(b.p) - wa->base), src, bytes);
+  return IFG_OK;
+}
+
 template <typename T>
-int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>& minStride) {
+int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>& minStride,
+              WorkArea* wa = nullptr) {
   const int nC = m->nComp(), nR = (int)m->regions.size();
   int st;
   P.nVec = nVec;
@@ -930,21 +990,39 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   P.usedOff = P.recBytes + P.boxBytes;
   P.prefixOff = P.usedOff + ((P.nGrids * sizeof(int) + 15) / 16) * 16;
   P.stageBytes = P.prefixOff + sizeof(long) * (size_t)P.nChunks * (P.nTasks + 1);
-  HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
-  HIPCHK(hipHostGetDevicePointer(&P.stageDev, P.pinnedStage, 0), "map pinned staging");
-  HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
-  HIPCHK(P.tail_d.ensure(std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double)), "alloc tail");
+  if (wa) {   // work-area plans: statistics of one vector only, staged through the mirror
+    if (flags != PF_STAT || nVec != 1) return m->fail(IFG_ERR_ARG, "work-area plans evaluate one statistic");
+    P.inArea = true;
+    if ((st = PlanBuf(m, wa, P.stage_d, P.stageBytes, true, "call staging")) < 0) return st;
+    P.pinnedStage = wa->mirror + (P.stage_d.as<char>() - wa->base);
+    P.stageDev = nullptr;
+  } else {
+    HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
+    HIPCHK(hipHostGetDevicePointer(&P.stageDev, P.pinnedStage, 0), "map pinned staging");
+    HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
+  }
+  if ((st = PlanBuf(m, wa, P.tail_d, std::max<long>(1, P.tailOff[P.nGrids]) * sizeof(double), false,
+                    "sub-pixel values")) < 0)
+    return st;
 
   const long nData = m->nData;
-  HIPCHK(P.spec_d.ensure(std::max<size_t>(1, pairBytes * P.chunkPairs)), "alloc planes");
-  if (nR) HIPCHK(P.specR_d.ensure(std::max<size_t>(1, pairBytesR * P.chunkPairs)), "alloc region planes");
+  if ((st = PlanBuf(m, wa, P.spec_d, std::max<size_t>(1, pairBytes * P.chunkPairs), false, "planes")) < 0)
+    return st;
+  if (nR && (st = PlanBuf(m, wa, P.specR_d, std::max<size_t>(1, pairBytesR * P.chunkPairs), false,
+                          "region planes")) < 0)
+    return st;
   const int nFree = nVec;
   const int nNormal = nFree * (nFree + 1) / 2 + nFree;
-  HIPCHK(P.out_d.ensure(sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0)), "alloc results");
-  HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0),
-                       hipHostMallocDefault),
-         "pinned results");
-  HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
+  const size_t outBytes = sizeof(double) * std::max(nVec, (flags & PF_JAC) ? nNormal : 0);
+  if ((st = PlanBuf(m, wa, P.out_d, outBytes, false, "results")) < 0) return st;
+  if (wa) {
+    P.outHost.assign(outBytes / sizeof(double), 0.0);
+    P.pinnedOut = P.outHost.data();
+    P.outDev = nullptr;
+  } else {
+    HIPCHK(hipHostMalloc(&P.pinnedOut, outBytes, hipHostMallocDefault), "pinned results");
+    HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
+  }
   if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
   if (flags & PF_JAC) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nFree), "alloc Jacobian columns");
   if (flags & PF_MODEL) HIPCHK(P.model_d.ensure(sizeof(double) * nData * nVec), "alloc model");
@@ -966,8 +1044,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         for (int p = 0; p < (nv + 1) / 2; p++) pg[p] = make_int4(2 * p, 0, (2 * p + 1 < nv) ? 2 * p + 1 : -1, 0);
     }
   }
-  HIPCHK(P.pairs_d.ensure(sizeof(int4) * pairs.size()), "alloc pairs");
-  HIPCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
+  if ((st = PlanBuf(m, wa, P.pairs_d, sizeof(int4) * pairs.size(), true, "pair lists")) < 0) return st;
+  if ((st = PlanUpload(m, wa, P.pairs_d, pairs.data(), sizeof(int4) * pairs.size(), "pairs")) < 0) return st;
 
   // task tables: main grid + every region, per chunk, with their block layouts
   const CompRec* recs_d = reinterpret_cast<const CompRec*>(P.stage_d.p);
@@ -1117,16 +1195,26 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     P.nJobs[c] = np;
   }
   P.nSlots = maxSlots;
-  P.directOut = jobs.size() <= 8;
+  P.directOut = !wa && jobs.size() <= 8;
   P.tailWaves = 0;
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
-  HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc tasks");
-  HIPCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
-  HIPCHK(P.maps_d.ensure(sizeof(int2) * std::max<size_t>(1, maps.size())), "alloc block maps");
-  HIPCHK(hipMemcpy(P.maps_d.p, maps.data(), sizeof(int2) * maps.size(), hipMemcpyHostToDevice), "block maps");
-  HIPCHK(P.jobs_d.ensure(sizeof(FinalJob) * std::max<size_t>(1, jobs.size())), "alloc final jobs");
-  HIPCHK(hipMemcpy(P.jobs_d.p, jobs.data(), sizeof(FinalJob) * jobs.size(), hipMemcpyHostToDevice), "final jobs");
-  HIPCHK(P.partials_d.ensure(sizeof(double) * 10 * (size_t)std::max(1, P.nSlots)), "alloc partials");
+  if ((st = PlanBuf(m, wa, P.tasks_d, sizeof(Task) * tasks.size(), true, "task table")) < 0 ||
+      (st = PlanUpload(m, wa, P.tasks_d, tasks.data(), sizeof(Task) * tasks.size(), "tasks")) < 0)
+    return st;
+  if ((st = PlanBuf(m, wa, P.maps_d, sizeof(int2) * std::max<size_t>(1, maps.size()), true, "block maps")) < 0 ||
+      (st = PlanUpload(m, wa, P.maps_d, maps.data(), sizeof(int2) * maps.size(), "block maps")) < 0)
+    return st;
+  if ((st = PlanBuf(m, wa, P.jobs_d, sizeof(FinalJob) * std::max<size_t>(1, jobs.size()), true, "final jobs")) < 0 ||
+      (st = PlanUpload(m, wa, P.jobs_d, jobs.data(), sizeof(FinalJob) * jobs.size(), "final jobs")) < 0)
+    return st;
+  if ((st = PlanBuf(m, wa, P.partials_d, sizeof(double) * 10 * (size_t)std::max(1, P.nSlots), false,
+                    "partial sums")) < 0)
+    return st;
+  if (wa) {   // every table byte is in the mirror: one copy per call
+    P.tableBytes = wa->tableUsed;
+    P.tableDev = wa->base;
+    return IFG_OK;
+  }
   if (flags & PF_JAC) {   // one normal-equations job over the Jacobian columns
     NormalJob J;
     std::memset(&J, 0, sizeof(J));
@@ -1207,7 +1295,9 @@ int Enqueue(ifg_model* m, Plan& P) {
   hipStream_t s = m->stream;
   // a few KB (one evaluation, one Jacobian): read by a kernel from the mapped
   // buffer; larger stagings (populations) by a DMA copy
-  if (P.stageDev && P.stageBytes % 8 == 0 && P.stageBytes <= 65536)
+  if (P.inArea)   // tables and staging, from the host mirror (pageable memory: an ordered copy)
+    HIPCHK(hipMemcpyAsync(P.tableDev, P.tableHost.data(), P.tableBytes, hipMemcpyHostToDevice, s), "upload tables");
+  else if (P.stageDev && P.stageBytes % 8 == 0 && P.stageBytes <= 65536)
     HIPCHK(launch_stage_copy(s, P.stage_d.p, P.stageDev, (long)(P.stageBytes / 8)), "upload call data");
   else
     HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
@@ -1374,7 +1464,69 @@ int RunDerivDispatch(ifg_model* m, const double* params, const std::vector<int>&
   return RunDeriv<float>(m, params, freeIdx, kind, stat, fvec, fjac, normal, grad);
 }
 
+// GetFitStatistic with every per-call device buffer carved from a work area of
+// `size` bytes at `base` (device memory the caller keeps), launched eagerly: no
+// hipMalloc, pinned allocation or graph.  The model's own resident state (data,
+// weights, PSF spectra, twiddles) must exist already: one ordinary single-vector
+// evaluation of the handle creates it.
+template <typename T>
+int EvaluateInArea(ifg_model* m, const double* params, double* out, char* base, size_t size) {
+  if (!m->devReady || (m->doConv && !m->mainConv[0].ready))
+    return m->fail(IFG_ERR_STATE, "no resident model state: evaluate the handle once (ifg_fit_statistic) first");
+  for (auto& R : m->regions)
+    if (!R->conv[0].ready)
+      return m->fail(IFG_ERR_STATE, "no resident region state: evaluate the handle once (ifg_fit_statistic) first");
+  constexpr size_t kTableCap = (size_t)1 << 20;
+  if (size <= kTableCap) return m->fail(IFG_ERR_ARG, "work area smaller than its table part");
+  Plan* P = m->areaPlan.get();
+  if (P && P->tableDev != base) P = nullptr;
+  int st;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!P) {
+      m->areaPlan.reset(new Plan());
+      P = m->areaPlan.get();
+      P->tableHost.assign(kTableCap, 0);
+      WorkArea wa;
+      wa.base = base;
+      wa.size = size;
+      wa.tableCap = kTableCap;
+      wa.mirror = P->tableHost.data();
+      if ((st = BuildPlan<T>(m, *P, 1, PF_STAT, m->areaNeed, &wa)) < 0) {
+        m->areaPlan.reset();
+        return st;
+      }
+    }
+    if (StageCall(m, *P, params, m->areaNeed) == 0) break;
+    m->areaPlan.reset();   // a larger sub-pixel box layout: rebuild with room for it
+    P = nullptr;
+  }
+  if (!P) return m->fail(IFG_ERR_UNSUPPORTED, "sub-pixel box layout does not fit");
+  const bool timing = m->timing;   // per-launch events are not taken on this path
+  m->timing = false;
+  st = Enqueue<T>(m, *P);
+  m->timing = timing;
+  if (st < 0) return st;
+  HIPCHK(hipStreamSynchronize(m->stream), "evaluation");
+  *out = P->outHost[0];
+  return IFG_OK;
+}
+
 }  // namespace
+
+namespace ifg {
+int FitStatisticInArea(ifg_model* m, const double* params, double* out, void* base, size_t size) {
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!params || !out) return m->fail(IFG_ERR_ARG, "null parameters or output");
+  HIPCHK(hipSetDevice(m->device), "hipSetDevice");
+  (void)hipGetLastError();
+  m->lastParams.assign(params, params + m->nParamsTot);
+  m->modelValid = false;
+  char* b = static_cast<char*>(base);
+  if (m->precision == IFG_PRECISION_FP64) return EvaluateInArea<double>(m, params, out, b, size);
+  return EvaluateInArea<float>(m, params, out, b, size);
+}
+int ModelDevice(const ifg_model* m) { return m->device; }
+}  // namespace ifg
 
 // ======================================================================= C ABI
 extern "C" {
@@ -1673,6 +1825,7 @@ int ifg_final_setup(ifg_model* m) try {   // core/model_object.cpp:967-1079
   m->finalDone = true;
   m->devReady = false;
   m->plans.clear();
+  m->areaPlan.reset();
   return status;
 } catch (...) {
   return ExceptionStatus(m);
@@ -2746,6 +2899,7 @@ std::string GeometryKey(const ifg_model* m, const std::vector<TuneGrid>& grids, 
 int TimeModel(ifg_model* m, const double* params, int nVec, double* seconds) {
   (void)hipStreamSynchronize(m->stream);
   m->plans.clear();
+  m->areaPlan.reset();
   std::vector<double> out(nVec);
   int st = Run(m, params, nVec, out.data(), nullptr, nullptr);
   if (st < 0) return st;
@@ -2783,6 +2937,7 @@ int ifg_autotune(ifg_model* m, const double* params, int n_vectors) try {
       for (size_t i = 0; i < grids.size(); i++) SetShape(m, *grids[i].cg, it->second[i], grids[i].split);
       (void)hipStreamSynchronize(m->stream);
       m->plans.clear();
+      m->areaPlan.reset();
       return 0;
     }
   }
@@ -2816,6 +2971,7 @@ int ifg_autotune(ifg_model* m, const double* params, int n_vectors) try {
   }
   (void)hipStreamSynchronize(m->stream);
   m->plans.clear();
+  m->areaPlan.reset();
   std::vector<GridShape> rec;
   for (const TuneGrid& tg : grids) rec.push_back(GetShape(m, *tg.cg));
   std::lock_guard<std::mutex> lock(gTuneMu);
@@ -2862,6 +3018,7 @@ int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_b
   (void)hipStreamSynchronize(m->stream);
   SetShape(m, *cg, s, split);
   m->plans.clear();   // plans hold the launch shapes
+  m->areaPlan.reset();
   return IFG_OK;
 } catch (...) {
   return ExceptionStatus(m);
@@ -2890,6 +3047,7 @@ void SetBatchShapes(ifg_batch* b, int C, double rowsFactor) {
     }
     (void)hipStreamSynchronize(m->stream);
     m->plans.clear();
+    m->areaPlan.reset();
   }
 }
 

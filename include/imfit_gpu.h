@@ -149,6 +149,20 @@ int ifg_fit_statistic(ifg_model* m, const double* params, double* out);
  * DREAM chains (cdream/dream.cpp:268-300), LM finite-difference columns
  * (solvers/mpfit.cpp:1241-1343).  params: n_vectors x nParams, row-major. */
 int ifg_fit_statistic_batch(ifg_model* m, const double* params, int n_vectors, double* out);
+/* GetFitStatistic that allocates nothing.  No counterpart in the reference; it is
+ * for callers that cannot allocate device or pinned memory in the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).  The
+ * per-call memory of ifg_fit_statistic (complex planes, sub-pixel values, partial
+ * sums, call tables, results) comes from one fixed array per GPU of
+ * IFG_NOALLOC_SCRATCH_BYTES that the library holds until the process ends; the
+ * kernels are the same and launch eagerly, and the result equals
+ * ifg_fit_statistic's.  The handle's resident state (data, weights, PSF spectra)
+ * must exist: one ordinary ifg_fit_statistic call on the handle creates it
+ * (IFG_ERR_STATE otherwise).  An evaluation whose work memory exceeds the array
+ * returns IFG_ERR_UNSUPPORTED.  All calls on one GPU share the array: calls that
+ * overlap on one GPU race. */
+#define IFG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int ifg_fit_statistic_noalloc(ifg_model* m, const double* params, double* out);
 
 /* ---- derivatives (no counterpart in the reference: mpfit differentiates by
  * finite differences, solvers/mpfit.cpp:1102-1356) ----------------------------
