@@ -9,8 +9,10 @@ sub-pixel integration, convolve it (and the oversampled region) with the PSF,
 form the weighted deviates against the data and reduce chi^2 in FP64.  Data,
 weights and PSF spectra are resident in HBM before timing starts.
 
-Prints one JSON line (see the keys below).  The CPU baseline is the FP64 CPU
-oracle of the same evaluation, timed on this host's cores on a bounded sample.
+The headline `value` is in FP64, the reference's arithmetic (parity mode: pixels,
+FFT and reduction in FP64); the FP32-pixel mode is timed the same way and reported
+beside it.  Prints one JSON line (see the keys below).  The CPU baseline is the FP64
+CPU oracle of the same evaluation, timed on this host's cores on a bounded sample.
 """
 import argparse
 import ctypes
@@ -124,13 +126,13 @@ def profiled_traffic(kernel, prec):
     return None
 
 
-def profiled_valu(kernel, prec):
-    """VALU wave instructions per dispatch of `kernel` from profiles/valu_cfg2_1024.md
-    (rocprofv3 SQ_INSTS_VALU of this workload), or None."""
-    path = os.path.join(ROOT, "profiles", "valu_cfg2_1024.md")
+def profiled_valu(kernel, prec, profile="valu_cfg2_1024.md", variants=("", ",false,false")):
+    """VALU wave instructions per dispatch of `kernel` from profiles/<profile>
+    (rocprofv3 SQ_INSTS_VALU of the workload), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
-    names = ("ifg::%s<%s>" % (kernel, prec), "ifg::%s<%s,false,false>" % (kernel, prec))
+    names = tuple("ifg::%s<%s%s>" % (kernel, prec, v) for v in variants)
     for line in open(path):
         f = line.split()
         if len(f) == 3 and f[0] == "valu_insts" and f[1] in names:
@@ -138,69 +140,122 @@ def profiled_valu(kernel, prec):
     return None
 
 
-def cpu_baseline(w, seconds):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, p, gpu_stats, warmup=5, timed=20):
+    """The FP64 CPU oracle of the same evaluation (the reference's algorithm restated,
+    with a real-input FFT in place of FFTW) on this host: `warmup` untimed calls, then
+    the median of `timed` GetFitStatistic calls, with 1 thread and with all threads
+    (OMP_NUM_THREADS, else os.cpu_count()); the FFT's share of the time; and, as the
+    checker, the GPU statistics' relative error against the oracle's."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    o = w.setup_fit(oracle.OracleModel(threads=threads))
-    p = w.start_params()
-    t0 = time.perf_counter()
-    o.GetFitStatistic(p)   # warm-up (first call touches every buffer)
-    first = time.perf_counter() - t0
-    n, t0 = 0, time.perf_counter()
-    while (time.perf_counter() - t0) < seconds:
-        o.GetFitStatistic(p)
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "%d FP64 GetFitStatistic calls (~%.0f s) of the same 1024^2 workload after 1 warm-up; CPU "
-                      "oracle restatement of the reference path with its own mixed-radix FFT (no FFTW on this "
-                      "host), OpenMP schedule(static,10) pixel loops" % (n, dt)}
+    res = {}
+    stat_ref = None
+    for thr in sorted({1, threads}):
+        o = w.setup_fit(oracle.OracleModel(threads=thr))
+        for _ in range(warmup):
+            stat_ref = o.GetFitStatistic(p)
+        oracle.conv_seconds(reset=True)
+        ts = []
+        for _ in range(timed):
+            t0 = time.perf_counter()
+            o.GetFitStatistic(p)
+            ts.append(time.perf_counter() - t0)
+        res[thr] = (float(np.median(ts)), oracle.conv_seconds(reset=True) / sum(ts))
+        log("cpu oracle, %d thread(s): median %.4f s per evaluation (%.1f evals/s), FFT share %.2f" %
+            (thr, res[thr][0], 1.0 / res[thr][0], res[thr][1]))
+    out = {"value": 1.0 / res[threads][0], "unit": "evals/s", "cores": threads, "kind": "port",
+           "sample": "%d warm-up + median of %d FP64 GetFitStatistic calls of the same 1024^2 workload, with 1 and %d "
+                     "threads; CPU oracle restatement of the reference path (OpenMP schedule(static,10) pixel "
+                     "loops, Kahan sums, serial deviates, mp_enorm) with its own vectorised real-input FFT in "
+                     "place of FFTW (absent on this host)" % (warmup, timed, threads),
+           "value_1_thread": 1.0 / res[1][0], "cpu_model": cpu_model(),
+           "fft_share_1_thread": res[1][1], "fft_share_all_threads": res[threads][1]}
+    for key, s in gpu_stats.items():
+        out["gpu_%s_vs_oracle_stat_rel" % key] = abs(s - stat_ref) / abs(stat_ref)
+    return out
 
 
 def lm_benchmark(w, repeats):
+    """LM fits/s on the 1024^2 workload, FP64, every leg with ftol 1e-10 from the same
+    start: (1) mpfit (MINPACK LMDIF, the reference's caller) with its forward-difference
+    Jacobian -- one GPU ComputeDeviates per free parameter, QR of the m x n Jacobian on
+    the host; (2) LM on the normal equations with the same forward-difference Jacobian,
+    its nFree + 1 evaluations in one batched GPU launch and J^T J, J^T r reduced on the
+    GPU; (3) the same LM with the exact (analytic) Jacobian.  (2) vs (3) isolates the
+    exact Jacobian's gain; (1) vs (2) the gain of the GPU normal equations."""
     import imfit_amd
     m = w.fit_model(imfit_amd.PRECISION_FP64)
     x0, lim = w.lm_start(), w.lm_limits()
-    log("LM: mpfit with finite-difference Jacobian (GPU evaluations, host QR)")
-    t0 = time.perf_counter()
-    x_fd, r_fd = m.FitLM(x0, limits=lim, ftol=1e-10, jacobian="fd")
-    t_fd = time.perf_counter() - t0
-    c_fd = m.GetFitStatistic(x_fd)
-    log("  %.3f s, status %d, niter %d, nfev %d, chi2 %.10f" % (t_fd, r_fd.status, r_fd.niter, r_fd.nfev, c_fd))
-    log("LM: normal equations with the analytic GPU Jacobian")
-    ts = []
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        x_ne, r_ne = m.FitLM(x0, limits=lim, ftol=1e-12, method="normal")
-        ts.append(time.perf_counter() - t0)
-    c_ne = m.GetFitStatistic(x_ne)
-    t_ne = float(np.median(ts))
-    log("  %.4f s (median of %d), status %d, niter %d, chi2 %.10f" % (t_ne, repeats, r_ne.status, r_ne.niter, c_ne))
-    return {"fits_per_s_fd_mpfit": 1.0 / t_fd, "fits_per_s_normal_analytic": 1.0 / t_ne,
-            "speedup": t_fd / t_ne, "chi2_fd": c_fd, "chi2_normal": c_ne,
-            "chi2_rel_diff": (c_ne - c_fd) / c_fd, "niter_fd": r_fd.niter, "nfev_fd": r_fd.nfev,
-            "niter_normal": r_ne.niter, "start": "truth x U(0.9,1.1), centre +-5 px (seed 1)"}
+    legs = {}
+    for key, kw, reps in (("fd_mpfit", dict(method="mpfit", jacobian="fd"), 1),
+                          ("fd_normal", dict(method="normal", jacobian="fd"), repeats),
+                          ("analytic_normal", dict(method="normal", jacobian="analytic"), repeats)):
+        if reps > 1:
+            m.FitLM(x0, limits=lim, ftol=1e-10, **kw)   # plans and graphs (untimed)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            x, r = m.FitLM(x0, limits=lim, ftol=1e-10, **kw)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        c = m.GetFitStatistic(x)
+        legs[key] = {"s_per_fit": t, "fits_per_s": 1.0 / t, "chi2": c, "status": r.status, "niter": r.niter,
+                     "nfev": r.nfev}
+        log("LM %-16s %.4f s per fit (median of %d), status %d, niter %d, nfev %d, chi2 %.10f" %
+            (key, t, reps, r.status, r.niter, r.nfev, c))
+    c0 = legs["fd_mpfit"]["chi2"]
+    for v in legs.values():
+        v["chi2_rel_diff_vs_fd_mpfit"] = (v["chi2"] - c0) / c0
+    return {"legs": legs, "ftol": 1e-10,
+            "speedup_analytic_vs_fd_normal": legs["fd_normal"]["s_per_fit"] / legs["analytic_normal"]["s_per_fit"],
+            "speedup_analytic_vs_fd_mpfit": legs["fd_mpfit"]["s_per_fit"] / legs["analytic_normal"]["s_per_fit"],
+            "start": "truth x U(0.9,1.1), centre +-5 px (seed 1)"}
 
 
-def aux_benchmarks(steps):
+def valu_roofline(kt, prec, profile, variants):
+    """VALU-bound workloads: the kernel with the most event time, its average duration
+    and its VALU issue fraction (4 cycles per wave64 instruction on one of the 1024
+    SIMDs at 2.4 GHz, over the duration) from the committed SQ_INSTS_VALU profile."""
+    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
+    valu = profiled_valu(dom, prec, profile, variants)
+    return {"bound": "valu", "kernel": dom, "avg_us": avg_s * 1e6,
+            "valu_issue_frac": 4.0 * valu / 1024 / 2.4e9 / avg_s if valu is not None else None,
+            "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}}
+
+
+def aux_benchmarks(steps, fits=True):
     """Short measurements of the other single-GPU configurations, reported beside the
-    headline: the 3-D disk model (cfg4) and a 4096-vector population (cfg3)."""
+    headline: the 3-D disk model (cfg4), a 4096-vector population (cfg3), many
+    cutouts (cfg5) and bootstrap resampling on the 256^2 image."""
     import imfit_amd
     from imfit_amd import workloads
     out = {}
     w4 = workloads.cfg4()
     p4 = w4.start_params(seed=1, frac=0.05)
+    res4 = {}
     for prec, key in ((imfit_amd.PRECISION_FP32, "fp32"), (imfit_amd.PRECISION_FP64, "fp64")):
         m = w4.fit_model(prec)
         m.Autotune(p4)
-        m.GetFitStatistic(p4)
-        device_sync()
-        n, t0 = max(3, steps // 20), time.perf_counter()
-        for _ in range(n):
-            m.GetFitStatistic(p4)
-        device_sync()
-        out.setdefault("cfg4_expdisk3d_512", {})["evals_per_s_" + key] = n / (time.perf_counter() - t0)
-    log("cfg4 (3-D disk + bulge, 512^2, PSF 25): %s" % out["cfg4_expdisk3d_512"])
+        n = max(5, steps // 20)
+        _, dt = time_statistic(m, p4, n, 2)
+        res4["evals_per_s_" + key] = n / dt
+        kt, _ = kernel_times(m, p4, n, 1)
+        res4["roofline_" + key] = valu_roofline(kt, "float" if key == "fp32" else "double", "valu_cfg3_cfg4.md",
+                                                (",true,false",))
+    out["cfg4_expdisk3d_512"] = res4
+    log("cfg4 (3-D disk + bulge, 512^2, PSF 25): %.1f evals/s fp32, %.1f evals/s fp64" %
+        (res4["evals_per_s_fp32"], res4["evals_per_s_fp64"]))
     w3, pop = workloads.cfg3_population()
     m3 = w3.fit_model(imfit_amd.PRECISION_FP32)
     m3.Autotune(pop[:1024])   # the population grids' shapes (kept for the full population)
@@ -212,12 +267,16 @@ def aux_benchmarks(steps):
         chi = m3.GetFitStatisticBatch(pop)
     device_sync()
     dt = (time.perf_counter() - t0) / reps
+    kt3, _ = kernel_times(m3, None, 2, 1, call=lambda: m3.GetFitStatisticBatch(pop))
     out["cfg3_population_512"] = {"vectors": int(pop.shape[0]), "s_per_batch": dt,
-                                  "chi2_per_s": pop.shape[0] / dt, "finite": bool(np.isfinite(chi).all())}
+                                  "chi2_per_s": pop.shape[0] / dt, "finite": bool(np.isfinite(chi).all()),
+                                  "roofline_fp32": valu_roofline(kt3, "float", "valu_cfg3_cfg4.md",
+                                                                 ("", ",false,false"))}
     log("cfg3 (4096-vector population, 512^2, PSF 35, FP32): %.3f s per batch, %.0f chi^2/s" %
         (dt, pop.shape[0] / dt))
-    out["cfg5_cutouts"] = cutout_benchmark()
-    out["bootstrap_cfg1"] = bootstrap_benchmark()
+    if fits:
+        out["cfg5_cutouts"] = cutout_benchmark()
+        out["bootstrap_cfg1"] = bootstrap_benchmark()
     return out
 
 
@@ -266,12 +325,13 @@ def bootstrap_benchmark(n=256, n_serial=4, seed=10):
     return res
 
 
-def cutout_benchmark(n=2000, n_plain=10):
+def cutout_benchmark(n=2000, n_plain=100):
     """Many-cutout fits: n cutouts (64-512 px, 1-3 components, 25^2 Moffat PSF) fitted
     all at once by the batched normal-equation LM (analytic Jacobians, J^T J and J^T r
     reduced on the GPU in FP64, every model's step in one batched evaluation), versus
-    plain GPU mpfit with finite-difference Jacobians, one cutout after another, on a
-    sample; chi^2 compared on the sample."""
+    plain GPU fits one cutout after another on a sample of n_plain: mpfit with
+    finite-difference Jacobians (chi^2 compared), and the normal-equation LM with
+    batched finite differences."""
     import imfit_amd
     from imfit_amd import workloads
     ws = workloads.cutouts(n)
@@ -301,6 +361,13 @@ def cutout_benchmark(n=2000, n_plain=10):
                 c_plain, c_batch = models[k].GetFitStatistic(x), models[k].GetFitStatistic(xb[k])
                 rel.append(abs(c_batch - c_plain) / c_plain)
             dtp = time.perf_counter() - t0
+            # the normal-equation LM with batched forward differences, one cutout at a time
+            t1 = time.perf_counter()
+            for k in range(n_plain):
+                models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, method="normal", jacobian="fd")
+            dtn = time.perf_counter() - t1
+            res["plain_fd_normal_fits_per_s"] = n_plain / dtn
+            res["speedup_fp64_vs_fd_normal"] = (n / dt) / (n_plain / dtn)
             matched = [r for r in rel if r <= 1e-9]
             res["plain_fd_mpfit_fits_per_s"] = n_plain / dtp
             res["plain_sample"] = n_plain
@@ -318,22 +385,75 @@ def cutout_benchmark(n=2000, n_plain=10):
     return res
 
 
+def time_statistic(m, p, steps, warmup):
+    """`warmup` untimed GetFitStatistic calls, then `steps` timed ones between device
+    synchronisations; returns (statistic, seconds)."""
+    for _ in range(warmup):
+        m.GetFitStatistic(p)
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stat = m.GetFitStatistic(p)
+    device_sync()
+    return stat, time.perf_counter() - t0
+
+
+def kernel_times(m, p, steps, warmup, call=None):
+    """The same calls again, eager, each kernel carrying a pair of HIP events on the
+    library's stream (hipExtLaunchKernel: the dispatch's own start and end
+    timestamps); returns ({kernel: (total ms, launches)}, ms per call)."""
+    call = call or (lambda: m.GetFitStatistic(p))
+    m.SetKernelTiming(True)
+    for _ in range(warmup):
+        call()
+    m.SetKernelTiming(True)   # clears the warm-up totals
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    device_sync()
+    ms = 1e3 * (time.perf_counter() - t1) / steps
+    kt = m.GetKernelTiming()
+    m.SetKernelTiming(False)
+    return kt, ms
+
+
+def roofline(w, kt, prec):
+    """Roofline of the kernel with the most event time: algorithmic bytes per launch
+    (kernel_model) over its average duration against 8 TB/s; `traffic` = HBM bytes per
+    launch from the committed FETCH_SIZE / WRITE_SIZE profile of the same workload; and
+    the VALU issue fraction from the committed SQ_INSTS_VALU profile."""
+    tname = "float" if prec == "fp32" else "double"
+    km = kernel_model(w, 4 if prec == "fp32" else 8)
+    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
+    ach = km[dom]["bytes"] / avg_s / 1e9 if dom in km else None
+    # VALU issue time (4 cycles per wave64 instruction on one of the 1024 SIMDs at
+    # 2.4 GHz) as a fraction of the kernel's duration
+    valu = profiled_valu(dom, tname)
+    return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS if ach is not None else None, "traffic": profiled_traffic(dom, tname),
+            "kernel": dom, "avg_us": avg_s * 1e6, "alg_bytes": km.get(dom, {}).get("bytes"),
+            "valu_issue_frac": 4.0 * valu / 1024 / 2.4e9 / avg_s if valu is not None else None}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp64",
+                    help="arithmetic of the headline value (fp64: the reference's; fp32 is reported beside it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--lm-repeats", type=int, default=3)
     ap.add_argument("--copy-calibration-mb", type=int, default=0)
-    ap.add_argument("--no-aux", action="store_true", help="skip the cfg3 / cfg4 side measurements")
+    ap.add_argument("--no-aux", action="store_true", help="skip the other workloads' side measurements")
+    ap.add_argument("--aux-only", action="store_true",
+                    help="only the 3-D disk and population measurements (profiling runs of those kernels)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (profiling runs: no tuning trials in the trace)")
     ap.add_argument("--launch-shapes", default=None,
-                    help="main and region shapes 'C,R,C,R' (columns, rows per workgroup) of the FP32 model "
+                    help="main and region shapes 'C,R,C,R' (columns, rows per workgroup) of the headline model "
                          "instead of tuning them (counter runs repeat the shapes a timed run chose)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -344,114 +464,70 @@ def main():
     import imfit_amd
     from imfit_amd import workloads
 
+    if args.aux_only:
+        print(json.dumps(aux_benchmarks(args.steps, fits=False)))
+        return
     log("building the 1024^2 workload (PSFs and data on the GPU)")
     w = workloads.cfg2()
-    prec = imfit_amd.PRECISION_FP32 if args.precision == "fp32" else imfit_amd.PRECISION_FP64
-    m = w.fit_model(prec)
     p = w.start_params()
     fixed_shapes = args.no_autotune or args.launch_shapes is not None
-    trials = 0 if fixed_shapes else m.Autotune(p)   # launch shapes for this geometry (untimed, once)
-    if args.launch_shapes is not None:
-        m.GetFitStatistic(p)   # builds the grids
-        v = [int(t) for t in args.launch_shapes.split(",")]
-        for grid in range(len(v) // 2):
-            m.SetLaunchShape(grid, v[2 * grid], v[2 * grid + 1])
-    for _ in range(args.warmup):
-        m.GetFitStatistic(p)
-    device_sync()
-    shapes = {"main_cols_rows": list(m.GetLaunchShape(0)), "region_cols_rows": list(m.GetLaunchShape(1)),
-              "autotune_trials": trials}
-    log("launch shapes (columns per workgroup, rows per workgroup): %s" % shapes)
-    # timed region: K evaluations, each replaying the captured graph of the call
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stat = m.GetFitStatistic(p)
-    device_sync()
-    dt = time.perf_counter() - t0
-    ms_per_step = 1e3 * dt / args.steps
-    value = args.steps / dt
-    log("chi2 = %.10e, %.4f ms/eval, %.1f evals/s" % (stat, ms_per_step, value))
-    # same K evaluations again, eager, with a pair of HIP events carried by every kernel's
-    # dispatch (hipExtLaunchKernel on the library's stream: the kernels' own start and end
-    # timestamps) for the per-kernel durations
-    m.SetKernelTiming(True)
-    for _ in range(args.warmup):
-        m.GetFitStatistic(p)
-    m.SetKernelTiming(True)   # clears the warm-up totals
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        m.GetFitStatistic(p)
-    device_sync()
-    ms_timed = 1e3 * (time.perf_counter() - t1) / args.steps
-    kt = m.GetKernelTiming()
-    m.SetKernelTiming(False)
-    log("with kernel events: %.4f ms/eval" % ms_timed)
-    for k, (tot, n) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
-        log("  %-22s %9.2f us avg  x%d" % (k, 1e3 * tot / n, n))
+    other = "fp32" if args.precision == "fp64" else "fp64"
+    models, results = {}, {}
+    for key in (args.precision, other):
+        m = w.fit_model(imfit_amd.PRECISION_FP32 if key == "fp32" else imfit_amd.PRECISION_FP64)
+        trials = 0
+        if key == args.precision and args.launch_shapes is not None:
+            m.GetFitStatistic(p)   # builds the grids
+            v = [int(t) for t in args.launch_shapes.split(",")]
+            for grid in range(len(v) // 2):
+                m.SetLaunchShape(grid, v[2 * grid], v[2 * grid + 1])
+        elif not fixed_shapes:
+            trials = m.Autotune(p)   # launch shapes for this geometry (untimed, once)
+        # timed region: K evaluations, each replaying the captured graph of the call
+        stat, dt = time_statistic(m, p, args.steps, args.warmup)
+        kt, ms_timed = kernel_times(m, p, args.steps, args.warmup)
+        shapes = {"main_cols_rows": list(m.GetLaunchShape(0)), "region_cols_rows": list(m.GetLaunchShape(1)),
+                  "autotune_trials": trials}
+        results[key] = {"stat": stat, "s": dt, "evals_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+                        "ms_per_step_with_kernel_events": ms_timed, "kt": kt, "shapes": shapes}
+        models[key] = m
+        log("%s: chi2 = %.10e, %.4f ms/eval, %.1f evals/s (shapes %s)" %
+            (key, stat, 1e3 * dt / args.steps, args.steps / dt, shapes))
+        log("%s with kernel events: %.4f ms/eval" % (key, ms_timed))
+        for k, (tot, n) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+            log("  %-22s %9.2f us avg  x%d" % (k, 1e3 * tot / n, n))
+    head, side = results[args.precision], results[other]
+    rel_fp32_fp64 = abs(results["fp32"]["stat"] - results["fp64"]["stat"]) / abs(results["fp64"]["stat"])
+    log("statistic fp32 vs fp64: rel diff %.3e" % rel_fp32_fp64)
+    roof = roofline(w, head["kt"], args.precision)
+    side_roof = roofline(w, side["kt"], other)
 
-    # FP64 parity-mode rate, reported alongside
-    m64 = w.fit_model(imfit_amd.PRECISION_FP64)
-    if not fixed_shapes:
-        m64.Autotune(p)
-    for _ in range(3):
-        m64.GetFitStatistic(p)
-    device_sync()
-    t1 = time.perf_counter()
-    n64 = max(10, args.steps // 4)
-    for _ in range(n64):
-        m64.GetFitStatistic(p)
-    device_sync()
-    fp64_rate = n64 / (time.perf_counter() - t1)
-    stat64 = m64.GetFitStatistic(p)
-    stat_cur = m.GetFitStatistic(p)
-    prec_err = abs(stat_cur - stat64) / abs(stat64)
-    log("FP64 parity mode: %.1f evals/s; statistic %s vs FP64: rel diff %.3e" % (fp64_rate, args.precision, prec_err))
-
-    # roofline of the dominant kernel: algorithmic bytes per launch / its average
-    # event-timed duration; traffic = HBM bytes per launch from the committed
-    # FETCH_SIZE / WRITE_SIZE profile of the same workload
-    prec_bytes = 4 if args.precision == "fp32" else 8
-    km = kernel_model(w, prec_bytes)
-    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
-    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
-    traffic = profiled_traffic(dom, "float" if args.precision == "fp32" else "double")
-    ach = km[dom]["bytes"] / avg_s / 1e9 if dom in km else None
-    # VALU issue time of the same kernel (4 cycles per wave64 instruction on one of the
-    # 1024 SIMDs at 2.4 GHz) as a fraction of its duration: what share of the kernel the
-    # vector ALUs are busy issuing
-    valu = profiled_valu(dom, "float" if args.precision == "fp32" else "double")
-    valu_frac = 4.0 * valu / 1024 / 2.4e9 / avg_s if valu is not None else None
-    roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": ach / PEAK_HBM_GBS if ach is not None else None, "traffic": traffic,
-            "kernel": dom, "avg_us": avg_s * 1e6, "alg_bytes": km.get(dom, {}).get("bytes"),
-            "valu_issue_frac": valu_frac}
-
-    # LM fits/s on the same workload: mpfit (MINPACK LMDIF) with a finite-difference
-    # Jacobian -- one GPU ComputeDeviates per free parameter, QR on the host, as the
-    # reference's caller does -- versus LM on the normal equations with the GPU's
-    # analytic Jacobian (J^T J, J^T r reduced on the device).  Both in FP64, compared
-    # at the chi^2 they reach.
-    lm = None
-    if not args.no_lm:
-        lm = lm_benchmark(w, args.lm_repeats)
-
+    lm = None if args.no_lm else lm_benchmark(w, args.lm_repeats)
     aux = None if args.no_aux else aux_benchmarks(args.steps)
 
     cpu = None
     if not args.no_cpu:
         log("timing the CPU oracle baseline")
-        cpu = cpu_baseline(w, args.cpu_seconds)
-        log("cpu: %.3f evals/s on %d threads" % (cpu["value"], cpu["cores"]))
+        cpu = cpu_baseline(w, p, {"fp64": results["fp64"]["stat"], "fp32": results["fp32"]["stat"]})
 
-    out = {"metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+    def kernels_us(r):
+        return {k: 1e3 * t / n for k, (t, n) in r["kt"].items()}
+
+    out = {"metric": METRIC, "value": head["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
            "config": {"workload": w.name, "image": "1024x1024", "psf": "51x51 Moffat",
                       "oversampled_region": "40x40 at 5x, 255x255 PSF",
-                      "precision": "%s pixels and FFT, FP64 deviate reduction" % args.precision,
-                      "fp64_evals_per_s": fp64_rate, "fp32_vs_fp64_stat_rel": prec_err, "ms_per_step_with_kernel_events": ms_timed,
-                      "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()},
-                      "launch_shapes": shapes},
+                      "precision": "%s pixels, FFT and reduction" % args.precision if args.precision == "fp64"
+                      else "fp32 pixels and FFT, FP64 deviate reduction",
+                      "ms_per_step_with_kernel_events": head["ms_per_step_with_kernel_events"],
+                      "kernels_us": kernels_us(head), "launch_shapes": head["shapes"],
+                      other: {"evals_per_s": side["evals_per_s"], "ms_per_step": side["ms_per_step"],
+                              "steps": args.steps, "kernels_us": kernels_us(side), "launch_shapes": side["shapes"],
+                              "roofline": side_roof,
+                              "precision": "fp32 pixels and FFT, FP64 deviate reduction" if other == "fp32"
+                              else "fp64 pixels, FFT and reduction"},
+                      "fp32_vs_fp64_stat_rel": rel_fp32_fp64},
            "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm, "other_configs": aux}
     print(json.dumps(out))
     if args.copy_calibration_mb > 0:

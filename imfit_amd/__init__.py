@@ -80,6 +80,8 @@ class LMResult(C.Structure):
 _SIGS["ifg_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.c_int, C.POINTER(LMResult)],
                        C.c_int)
 _SIGS["ifg_fit_lm_normal"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
+_SIGS["ifg_fit_lm_normal_fd"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)],
+                                 C.c_int)
 
 _SIGS["ifg_batch_create"] = ([C.c_int], _vp)
 _SIGS["ifg_batch_destroy"] = ([_vp], None)
@@ -364,16 +366,22 @@ class ModelObjectGPU:
                     arr[i].limited[1], arr[i].limits[1] = 1, hi
         return arr
 
-    def FitLM(self, params, limits=None, fixed=None, ftol=1e-8, xtol=0.0, maxiter=1000, jacobian="fd",
+    def FitLM(self, params, limits=None, fixed=None, ftol=1e-8, xtol=0.0, maxiter=1000, jacobian=None,
               method="mpfit"):
         """Levenberg-Marquardt fit.  method "mpfit": MINPACK LMDIF as imfit runs it, with
-        jacobian "fd" (one GPU evaluation per free parameter) or "analytic"; method
-        "normal": LM on GPU-reduced normal equations.  Returns (params, LMResult)."""
+        jacobian "fd" (default; one GPU evaluation per free parameter) or "analytic";
+        method "normal": LM on GPU-reduced normal equations, with jacobian "analytic"
+        (default) or "fd" (forward differences from one batched launch).  Returns
+        (params, LMResult)."""
+        if jacobian is None:
+            jacobian = "analytic" if method == "normal" else "fd"
         x = np.array(self._vec(params), dtype=np.float64)
         pinfo = self._pinfo(limits, fixed, x.size)
         cfg = LMConfig(ftol, xtol, 0.0, maxiter, 0)
         res = LMResult()
-        if method == "normal":
+        if method == "normal" and jacobian == "fd":
+            st = _LIB.ifg_fit_lm_normal_fd(self.h, _ptr(x), pinfo, C.byref(cfg), C.byref(res))
+        elif method == "normal":
             st = _LIB.ifg_fit_lm_normal(self.h, _ptr(x), pinfo, C.byref(cfg), C.byref(res))
         else:
             mode = 1 if jacobian == "analytic" else 0

@@ -431,6 +431,9 @@ struct ifg_model {
   DevBuf data_d, weight_d, extra_d;
   DevBuf devkeep_d, dfac_d;   // forward-pass outputs consumed by derivative passes
   std::vector<std::unique_ptr<Plan>> plans;
+  DevBuf fdJob_d, fdTiles_d, fdPart_d, fdOut_d;   // normal equations of a finite-difference Jacobian
+  int fdTiles = 0;
+  double fdScale = -1.0;   // ||w d|| over the valid pixels: rounding scale of the chi^2 deviates
   std::unique_ptr<Plan> areaPlan;   // GetFitStatistic plan in the no-allocation work area
   std::vector<int> areaNeed;
   bool useGraphs = true;
@@ -1464,6 +1467,86 @@ int RunDerivDispatch(ifg_model* m, const double* params, const std::vector<int>&
   return RunDeriv<float>(m, params, freeIdx, kind, stat, fvec, fjac, normal, grad);
 }
 
+// Normal equations of a forward-difference Jacobian: the point x and the nFree
+// points x + h_j e_j in one batched evaluation that keeps its deviates on the
+// device, the columns (dev_j - dev_0) / h_j formed there, and J^T J, J^T r reduced by
+// the tile kernels that serve the analytic Jacobian.  stat: the statistic at x.
+template <typename T>
+int NormalFD(ifg_model* m, const double* x, const std::vector<int>& freeIdx, const double* h, double* jtj,
+             double* jtr, double* stat) {
+  const int nFree = (int)freeIdx.size(), nVec = nFree + 1, nP = m->nParamsTot;
+  std::vector<double> pts((size_t)nVec * nP), stats(nVec);
+  for (int v = 0; v < nVec; v++) std::memcpy(&pts[(size_t)v * nP], x, sizeof(double) * nP);
+  for (int j = 0; j < nFree; j++) pts[(size_t)(j + 1) * nP + freeIdx[j]] = x[freeIdx[j]] + h[j];
+  int st = Evaluate<T>(m, pts.data(), nVec, stats.data(), nullptr, nullptr, PF_DEV);
+  if (st < 0) return st;
+  Plan* P = FindPlan(m, nVec, PF_STAT | PF_DEV, std::vector<int>());
+  if (!P) return m->fail(IFG_ERR_INTERNAL, "finite-difference plan missing");
+  FDSteps hs;
+  std::memset(&hs, 0, sizeof(hs));
+  for (int j = 0; j < nFree; j++) hs.h[j] = h[j];
+  const long nData = m->nData;
+  const int nEnt = nFree * (nFree + 1) / 2 + nFree;
+  NormalJob J;
+  std::memset(&J, 0, sizeof(J));
+  J.jcol = P->dev_d.as<double>() + nData;
+  J.r = P->dev_d.as<double>();
+  J.nData = nData;
+  J.nFree = nFree;
+  J.TP = normal_tile_pixels(nFree);
+  J.nTiles = (int)((nData + J.TP - 1) / J.TP);
+  HIPCHK(m->fdJob_d.ensure(sizeof(NormalJob)), "alloc normal job");
+  HIPCHK(m->fdPart_d.ensure(sizeof(double) * (size_t)J.nTiles * nEnt), "alloc normal partials");
+  HIPCHK(m->fdOut_d.ensure(sizeof(double) * nEnt), "alloc normal results");
+  J.out = m->fdOut_d.as<double>();
+  hipStream_t s = m->stream;
+  if (m->fdTiles != J.nTiles) {
+    std::vector<int2> tmap(J.nTiles);
+    for (int q = 0; q < J.nTiles; q++) tmap[q] = make_int2(0, q);
+    HIPCHK(m->fdTiles_d.ensure(sizeof(int2) * tmap.size()), "alloc tile map");
+    HIPCHK(hipMemcpy(m->fdTiles_d.p, tmap.data(), sizeof(int2) * tmap.size(), hipMemcpyHostToDevice), "tile map");
+    m->fdTiles = J.nTiles;
+  }
+  HIPCHK(hipMemcpyAsync(m->fdJob_d.p, &J, sizeof(J), hipMemcpyHostToDevice, s), "normal job");
+  TIMED("k_fd_columns", launch_fd_columns(s, P->dev_d.as<double>(), nData, nFree, hs), "k_fd_columns");
+  TIMED("k_normal",
+        launch_normal(s, m->fdJob_d.as<NormalJob>(), nullptr, 1, m->fdTiles_d.as<int2>(), J.nTiles, nFree,
+                      m->fdPart_d.as<double>()),
+        "k_normal");
+  std::vector<double> packed(nEnt);
+  HIPCHK(hipMemcpyAsync(packed.data(), m->fdOut_d.p, sizeof(double) * nEnt, hipMemcpyDeviceToHost, s), "normal D2H");
+  HIPCHK(hipStreamSynchronize(s), "finite-difference normal equations");
+  if (m->timing) HarvestTimes(m);
+  int e = 0;
+  for (int a = 0; a < nFree; a++)
+    for (int c = a; c < nFree; c++, e++) jtj[(size_t)a * nFree + c] = jtj[(size_t)c * nFree + a] = packed[e];
+  for (int j = 0; j < nFree; j++) jtr[j] = packed[e + j];
+  // A parameter that no longer changes the model (the position angle of a profile at
+  // ellipticity 0) has an exactly zero derivative, but its forward difference is the
+  // rounding noise of the two evaluations, ~eps ||w m|| / |h|: such a column, squared
+  // into J^T J, steers the step at random.  chi^2 columns within 100 times that floor
+  // (||w m|| taken as ||w d||) are set to zero, as the exact derivative would be.
+  if (m->statMode() == S_CHI2) {
+    if (m->fdScale < 0.0) {
+      double s2 = 0.0;
+      for (long z = 0; z < nData; z++) {
+        const double v = EffectiveWeight(m, z) * m->data[z];
+        s2 += v * v;
+      }
+      m->fdScale = std::sqrt(s2);
+    }
+    for (int j = 0; j < nFree; j++) {
+      const double floorNorm = 100.0 * 2.220446049250313e-16 * m->fdScale / std::fabs(h[j]);
+      if (std::sqrt(std::max(jtj[(size_t)j * nFree + j], 0.0)) <= floorNorm) {
+        for (int k = 0; k < nFree; k++) jtj[(size_t)j * nFree + k] = jtj[(size_t)k * nFree + j] = 0.0;
+        jtr[j] = 0.0;
+      }
+    }
+  }
+  *stat = stats[0];
+  return IFG_OK;
+}
+
 // GetFitStatistic with every per-call device buffer carved from a work area of
 // `size` bytes at `base` (device memory the caller keeps), launched eagerly: no
 // hipMalloc, pinned allocation or graph.  The model's own resident state (data,
@@ -1526,6 +1609,20 @@ int FitStatisticInArea(ifg_model* m, const double* params, double* out, void* ba
   return EvaluateInArea<float>(m, params, out, b, size);
 }
 int ModelDevice(const ifg_model* m) { return m->device; }
+int NormalEquationsFD(ifg_model* m, const double* x, int nFree, const int* ifree, const double* h, double* jtj,
+                      double* jtr, double* stat) {
+  if (!m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!x || !h || !jtj || !jtr || !stat) return m->fail(IFG_ERR_ARG, "null argument");
+  std::vector<int> freeIdx;
+  int st = CheckFree(m, nFree, ifree, freeIdx);
+  if (st < 0) return st;
+  HIPCHK(hipSetDevice(m->device), "hipSetDevice");
+  (void)hipGetLastError();
+  m->lastParams.assign(x, x + m->nParamsTot);
+  m->modelValid = false;
+  if (m->precision == IFG_PRECISION_FP64) return NormalFD<double>(m, x, freeIdx, h, jtj, jtr, stat);
+  return NormalFD<float>(m, x, freeIdx, h, jtj, jtr, stat);
+}
 }  // namespace ifg
 
 // ======================================================================= C ABI
@@ -1823,6 +1920,7 @@ int ifg_final_setup(ifg_model* m) try {   // core/model_object.cpp:967-1079
   if (badData && status == IFG_OK) status = m->fail(IFG_ERR_DATA, "non-finite unmasked data values");
   if (m->nValid < 1 && status == IFG_OK) status = m->fail(IFG_ERR_DATA, "no valid data values");
   m->finalDone = true;
+  m->fdScale = -1.0;
   m->devReady = false;
   m->plans.clear();
   m->areaPlan.reset();
@@ -1854,6 +1952,7 @@ void ifg_init_genrand(unsigned long seed) try {
 }
 
 static int SetBootstrapSample(ifg_model* m, std::vector<long>&& idx) {
+  m->fdScale = -1.0;
   m->bootIdx = std::move(idx);
   m->bootCount.assign(m->nData, 0.0);
   for (long b : m->bootIdx) m->bootCount[b] += 1.0;
@@ -1928,6 +2027,7 @@ int ifg_stop_bootstrap(ifg_model* m) try {
   if (!m) return IFG_ERR_ARG;
   if (!m->bootstrap) return IFG_OK;
   m->bootstrap = false;
+  m->fdScale = -1.0;
   m->bootIdx.clear();
   m->bootCount.clear();
   if (!m->devReady) return IFG_OK;

@@ -169,6 +169,11 @@ struct FinalJob {
 };
 
 // Normal equations of one model from its Jacobian columns and deviates.
+// Forward-difference steps of a Jacobian's columns (passed by value)
+struct FDSteps {
+  double h[IFG_MAXFREE];
+};
+
 struct NormalJob {
   const double* jcol;   // [j * nData + i]
   const double* r;      // [nData]

@@ -1953,6 +1953,19 @@ __global__ __launch_bounds__(256) void k_stage_copy(unsigned long long* __restri
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n8; i += (long)gridDim.x * kThreads) dst[i] = src[i];
 }
 
+// ------------------------------------------------------------------ finite differences
+// Forward-difference Jacobian columns from deviate vectors evaluated together:
+// dev[(j + 1) * nData + i] holds the deviates at x + h_j e_j, dev[i] those at x;
+// column j becomes (dev_j - dev_0) / h_j in place, as mp_fdjac2 forms it
+// (solvers/mpfit.cpp:1294-1298).
+__global__ __launch_bounds__(256) void k_fd_columns(double* __restrict__ dev, long nData, FDSteps h) {
+  const int j = blockIdx.y;
+  double* col = dev + (long)(j + 1) * nData;
+  const double hj = h.h[j];
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < nData; i += (long)gridDim.x * kThreads)
+    col[i] = (col[i] - dev[i]) / hj;
+}
+
 // ------------------------------------------------------------------ normal equations
 // J^T J (upper triangle, row-major packed) and J^T r of every job (model) from its
 // Jacobian columns jcol[j * nData + i] and deviates r[i]: each block stages a tile of
@@ -2122,6 +2135,14 @@ hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList,
 }
 
 int normal_tile_pixels(int nFree) { return (nFree <= 16) ? 256 : 128; }
+
+hipError_t launch_fd_columns(hipStream_t s, double* dev, long nData, int nFree, const FDSteps& h) {
+  if (nFree < 1 || nFree > IFG_MAXFREE) return hipErrorInvalidValue;
+  const long blocks = std::min<long>((nData + kThreads - 1) / kThreads, 1024);
+  IFG_LAUNCH(true, true, k_fd_columns, dim3((unsigned)std::max<long>(blocks, 1), nFree), dim3(256), 0, s, dev, nData,
+             h);
+  return hipGetLastError();
+}
 
 hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobList, int nJobs, const int2* tileMap,
                          int nTiles, int maxFree, double* partials) {

@@ -38,6 +38,8 @@ hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, lo
 // host memory (each job block ends with a system-scope fence)
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
                         int mode, double* out, int hostOut);
+// Jacobian columns (dev_j - dev_0) / h_j in place, from nFree + 1 deviate vectors
+hipError_t launch_fd_columns(hipStream_t s, double* dev, long nData, int nFree, const FDSteps& h);
 // pixels per tile of a normal-equations job with nFree columns
 int normal_tile_pixels(int nFree);
 hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobList, int nJobs, const int2* tileMap,

@@ -28,6 +28,9 @@ int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double*
                   const std::vector<std::vector<int>>* freeIdx, const double** normal, const long** normalOff);
 int BatchSize(const ifg_batch* b);
 ifg_model* BatchModel(ifg_batch* b, int k);
+// normal equations of a forward-difference Jacobian with steps h (one batched launch)
+int NormalEquationsFD(ifg_model* m, const double* x, int nFree, const int* ifree, const double* h, double* jtj,
+                      double* jtr, double* stat);
 }  // namespace ifg
 
 namespace {
@@ -810,6 +813,46 @@ int ifg_fit_lm_normal(ifg_model* model, double* xall, const ifg_param_info* pinf
       if ((st = ifg_normal_equations(model, S.Point(), S.n, S.L.ifree.data(), jtj.data(), jtr.data(), &chi2)) < 0)
         return st;
       S.OnNormal(jtj.data(), jtr.data(), chi2);
+      if (S.Done()) break;
+    }
+    if (!S.Trial()) break;
+    double c = 0.0;
+    if ((st = ifg_fit_statistic(model, S.xtrial.data(), &c)) < 0) return st;
+    S.OnTrial(c);
+  }
+  S.Result(xall, pinfo, res);
+  return S.info;
+} catch (...) {
+  return ifg::ExceptionStatus(model);
+}
+
+// The same normal-equation LM with mp_fdjac2's forward-difference Jacobian
+// (solvers/mpfit.cpp:1241-1300: h = sqrt(eps) |x|, sqrt(eps) at 0, negated against an
+// upper limit): its nFree + 1 evaluations run as one batched launch, the columns and
+// J^T J, J^T r are formed on the GPU.
+int ifg_fit_lm_normal_fd(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+                         ifg_lm_result* res) try {
+  if (!model) return IFG_ERR_ARG;
+  NormalLM S;
+  int st = S.Init(ifg_get_nparams(model), xall, pinfo, cfg);
+  if (st) return st;
+  std::vector<double> jtj((size_t)S.n * S.n), jtr(S.n), h(S.n);
+  const double eps = sqrt(MACHEP0);
+  while (!S.Done()) {
+    if (S.needJ) {
+      const double* xp = S.Point();
+      for (int j = 0; j < S.n; j++) {
+        const double temp = xp[S.L.ifree[j]];
+        h[j] = eps * fabs(temp);
+        if (h[j] == 0.0) h[j] = eps;
+        if (S.L.qulim[j] && temp > S.L.ulim[j] - h[j]) h[j] = -h[j];
+      }
+      double chi2 = 0.0;
+      if ((st = ifg::NormalEquationsFD(model, xp, S.n, S.L.ifree.data(), h.data(), jtj.data(), jtr.data(), &chi2)) <
+          0)
+        return st;
+      S.OnNormal(jtj.data(), jtr.data(), chi2);
+      S.nfev += S.n;
       if (S.Done()) break;
     }
     if (!S.Trial()) break;

@@ -135,7 +135,7 @@ class ImfitSetup:
         p[ys] -= self.offsets[1]
         return p
 
-    def fit(self, ftol=1e-8, method="mpfit", jacobian="fd", **kw):
+    def fit(self, ftol=1e-8, method="mpfit", jacobian=None, **kw):
         """Levenberg-Marquardt from the configuration's values, limits and fixed flags.
         Returns (best parameters in the file frame, LMResult)."""
         x, res = self.model.FitLM(self.params, limits=self.limits, fixed=self.fixed, ftol=ftol, method=method,

@@ -226,6 +226,13 @@ int ifg_fit_lm(ifg_model* m, double* params, const ifg_param_info* pinfo, const 
  * on the GPU in FP64 (ifg_normal_equations); no counterpart in the reference. */
 int ifg_fit_lm_normal(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                       ifg_lm_result* res);
+/* ifg_fit_lm_normal with mpfit's forward-difference Jacobian in place of the exact one
+ * (mp_fdjac2's steps, solvers/mpfit.cpp:1241-1300): the nFree + 1 evaluations of each
+ * Jacobian run as one batched launch and J^T J, J^T r are reduced on the GPU.  The
+ * plain-GPU finite-difference LM that the exact Jacobian is measured against; no
+ * counterpart in the reference. */
+int ifg_fit_lm_normal_fd(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+                         ifg_lm_result* res);
 
 /* ---- batches of independent models (no counterpart in the reference, which fits
  * one image per run) ------------------------------------------------------------

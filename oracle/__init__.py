@@ -75,6 +75,8 @@ def lib():
         L.orc_get_nvalid.restype = C.c_long
         L.orc_func_value.argtypes = [C.c_char_p, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_func_value.restype = C.c_double
+        L.orc_conv_seconds.argtypes = [C.c_int]
+        L.orc_conv_seconds.restype = C.c_double
         L.orc_calculate_bn.argtypes = [C.c_double]
         L.orc_calculate_bn.restype = C.c_double
         L.orc_bessel_k01.argtypes = [C.c_double, _dp, _dp]
@@ -223,6 +225,11 @@ def convolve(image, psf, normalize=True, mode=0):
     lib().orc_convolve(_ptr(img), img.shape[1], img.shape[0], _ptr(psf), psf.shape[1], psf.shape[0],
                        int(normalize), mode)
     return img
+
+
+def conv_seconds(reset=False):
+    """Wall time the oracle spent in FFT convolutions since the last reset."""
+    return lib().orc_conv_seconds(int(reset))
 
 
 def enorm(x):

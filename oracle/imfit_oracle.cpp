@@ -811,96 +811,316 @@ Func* MakeFunc(const char* name) {   // core/add_functions.cpp:163-385 (name -> 
 }
 
 // ------------------------------------------------------------------ CPU FFT
-// Mixed-radix (2,3,4,5,7) Stockham complex FFT, used where the reference calls
-// FFTW (core/convolver.cpp:231-282,365,395).  Sizes are padded to the next
-// 2^a 3^b 5^c 7^d length, which leaves the *linear* convolution unchanged.
+// Used where the reference calls FFTW's r2c / c2r transforms
+// (core/convolver.cpp:231-282,365,395).  Sizes are padded to the next 2^a 3^b 5^c
+// length >= nImage + nPSF - 1, which leaves the *linear* convolution unchanged.
+// Complex transforms are Stockham autosort with radix-4/2/3/5 butterflies and
+// per-stage twiddle tables (computed in long double), run on kBatch transforms at
+// once in split real / imaginary arrays (element j of transform b at [j * kBatch + b])
+// so that every butterfly is a vector loop over the batch.  Real images go through
+// half-length spectra: two real rows share one complex row transform and are split
+// by Hermitian symmetry, the column transforms run over the nc/2 + 1 stored
+// columns only, and the inverse packs two half spectra into one complex transform.
 int NextSmooth(int n) {
   for (int m = std::max(n, 1);; m++) {
     int k = m;
-    for (int p : {2, 3, 5, 7})
+    for (int p : {2, 3, 5})
       while (k % p == 0) k /= p;
     if (k == 1) return m;
   }
 }
 
+constexpr int kBatch = 8;
+
 struct FFT1D {
   int N = 0;
-  vector<int> radices;
-  vector<cplx> tw;   // exp(-2 pi i k / N)
+  vector<int> radices, ns;             // per stage: radix, product of the earlier radices
+  vector<vector<double>> twr, twi;     // per stage: exp(-2 pi i k r / (Ns R)), [k][r - 1]
   void plan(int n) {
     N = n;
     radices.clear();
+    ns.clear();
+    twr.clear();
+    twi.clear();
     int k = n;
     while (k % 4 == 0) { radices.push_back(4); k /= 4; }
-    for (int p : {2, 3, 5, 7})
+    for (int p : {2, 3, 5})
       while (k % p == 0) { radices.push_back(p); k /= p; }
     if (k != 1) { fprintf(stderr, "oracle FFT: unsupported length %d\n", n); abort(); }
-    tw.resize(n);
-    for (int j = 0; j < n; j++) {
-      long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
-      tw[j] = cplx((double)cosl(ang), (double)sinl(ang));
-    }
-  }
-  // x: N values at stride `stride`; work: N scratch; sign -1 forward, +1 inverse.
-  void run(cplx* x, int stride, cplx* work, int sign) const {
-    vector<cplx> buf(N);
-    cplx* src = work;
-    cplx* dst = buf.data();
-    for (int j = 0; j < N; j++) src[j] = x[(long)j * stride];
     int Ns = 1;
     for (int R : radices) {
-      const int nb = N / R;
-      for (int j = 0; j < nb; j++) {
-        const int k = j % Ns;
-        cplx v[7];
-        for (int r = 0; r < R; r++) {
-          cplx t = tw[(long)k * r * (N / (Ns * R)) % N];
-          if (sign > 0) t = std::conj(t);
-          v[r] = src[j + r * nb] * t;
+      ns.push_back(Ns);
+      vector<double> tr((size_t)Ns * (R - 1)), ti((size_t)Ns * (R - 1));
+      for (int kk = 0; kk < Ns; kk++)
+        for (int r = 1; r < R; r++) {
+          const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)(kk * r) /
+                                  (long double)(Ns * R);
+          tr[(size_t)kk * (R - 1) + r - 1] = (double)cosl(ang);
+          ti[(size_t)kk * (R - 1) + r - 1] = (double)sinl(ang);
         }
-        cplx o[7];
-        for (int m = 0; m < R; m++) {
-          cplx s = 0;
-          for (int r = 0; r < R; r++) {
-            cplx t = tw[(long)((m * r) % R) * (N / R)];
-            if (sign > 0) t = std::conj(t);
-            s += v[r] * t;
-          }
-          o[m] = s;
-        }
-        const int base = (j / Ns) * Ns * R + k;
-        for (int m = 0; m < R; m++) dst[base + m * Ns] = o[m];
-      }
-      std::swap(src, dst);
+      twr.push_back(std::move(tr));
+      twi.push_back(std::move(ti));
       Ns *= R;
     }
-    for (int j = 0; j < N; j++) x[(long)j * stride] = src[j];
+  }
+  // kBatch transforms in place on (xr, xi) [N * kBatch]; (wr, wi) the same size of
+  // scratch.  sign -1 forward, +1 inverse (unscaled).
+  void run(double* xr, double* xi, double* wr, double* wi, int sign) const {
+    constexpr int B = kBatch;
+    double *sr = xr, *si = xi, *dr = wr, *di = wi;
+    const double s = (double)sign;
+    const double c3 = -0.5, s3 = 0.86602540378443864676 * s;   // exp(sign 2 pi i / 3) = c3 + i s3
+    const double c51 = 0.30901699437494742410, c52 = -0.80901699437494742410;
+    const double s51 = 0.95105651629515357212 * s, s52 = 0.58778525229247312917 * s;
+    for (size_t st = 0; st < radices.size(); st++) {
+      const int R = radices[st], Ns = ns[st], nb = N / R, nq = nb / Ns;
+      const double* Tr = twr[st].data();
+      const double* Ti = twi[st].data();
+      for (int k = 0; k < Ns; k++) {
+        double wre[4], wim[4];
+        for (int r = 1; r < R; r++) {
+          wre[r - 1] = Tr[(size_t)k * (R - 1) + r - 1];
+          wim[r - 1] = s < 0 ? Ti[(size_t)k * (R - 1) + r - 1] : -Ti[(size_t)k * (R - 1) + r - 1];
+        }
+        for (int q = 0; q < nq; q++) {
+          const size_t j = (size_t)(q * Ns + k) * B, o = ((size_t)q * Ns * R + k) * B;
+          const size_t in1 = j + (size_t)nb * B, in2 = j + 2 * (size_t)nb * B, in3 = j + 3 * (size_t)nb * B,
+                       in4 = j + 4 * (size_t)nb * B;
+          const size_t o1 = o + (size_t)Ns * B, o2 = o + 2 * (size_t)Ns * B, o3 = o + 3 * (size_t)Ns * B,
+                       o4 = o + 4 * (size_t)Ns * B;
+          if (R == 4) {
+#pragma omp simd
+            for (int b = 0; b < B; b++) {
+              const double v0r = sr[j + b], v0i = si[j + b];
+              const double v1r = sr[in1 + b] * wre[0] - si[in1 + b] * wim[0], v1i = sr[in1 + b] * wim[0] + si[in1 + b] * wre[0];
+              const double v2r = sr[in2 + b] * wre[1] - si[in2 + b] * wim[1], v2i = sr[in2 + b] * wim[1] + si[in2 + b] * wre[1];
+              const double v3r = sr[in3 + b] * wre[2] - si[in3 + b] * wim[2], v3i = sr[in3 + b] * wim[2] + si[in3 + b] * wre[2];
+              const double ar = v0r + v2r, ai = v0i + v2i, br = v0r - v2r, bi = v0i - v2i;
+              const double cr = v1r + v3r, ci = v1i + v3i, dr_ = v1r - v3r, di_ = v1i - v3i;
+              const double idr = -s * di_, idi = s * dr_;   // sign * i * d
+              dr[o + b] = ar + cr; di[o + b] = ai + ci;
+              dr[o1 + b] = br + idr; di[o1 + b] = bi + idi;
+              dr[o2 + b] = ar - cr; di[o2 + b] = ai - ci;
+              dr[o3 + b] = br - idr; di[o3 + b] = bi - idi;
+            }
+          } else if (R == 2) {
+#pragma omp simd
+            for (int b = 0; b < B; b++) {
+              const double v0r = sr[j + b], v0i = si[j + b];
+              const double v1r = sr[in1 + b] * wre[0] - si[in1 + b] * wim[0], v1i = sr[in1 + b] * wim[0] + si[in1 + b] * wre[0];
+              dr[o + b] = v0r + v1r; di[o + b] = v0i + v1i;
+              dr[o1 + b] = v0r - v1r; di[o1 + b] = v0i - v1i;
+            }
+          } else if (R == 3) {
+#pragma omp simd
+            for (int b = 0; b < B; b++) {
+              const double v0r = sr[j + b], v0i = si[j + b];
+              const double v1r = sr[in1 + b] * wre[0] - si[in1 + b] * wim[0], v1i = sr[in1 + b] * wim[0] + si[in1 + b] * wre[0];
+              const double v2r = sr[in2 + b] * wre[1] - si[in2 + b] * wim[1], v2i = sr[in2 + b] * wim[1] + si[in2 + b] * wre[1];
+              const double t1r = v1r + v2r, t1i = v1i + v2i;
+              const double t2r = v0r + c3 * t1r, t2i = v0i + c3 * t1i;
+              const double t3r = -s3 * (v1i - v2i), t3i = s3 * (v1r - v2r);   // i s3 (v1 - v2)
+              dr[o + b] = v0r + t1r; di[o + b] = v0i + t1i;
+              dr[o1 + b] = t2r + t3r; di[o1 + b] = t2i + t3i;
+              dr[o2 + b] = t2r - t3r; di[o2 + b] = t2i - t3i;
+            }
+          } else {   // R == 5
+#pragma omp simd
+            for (int b = 0; b < B; b++) {
+              const double v0r = sr[j + b], v0i = si[j + b];
+              const double v1r = sr[in1 + b] * wre[0] - si[in1 + b] * wim[0], v1i = sr[in1 + b] * wim[0] + si[in1 + b] * wre[0];
+              const double v2r = sr[in2 + b] * wre[1] - si[in2 + b] * wim[1], v2i = sr[in2 + b] * wim[1] + si[in2 + b] * wre[1];
+              const double v3r = sr[in3 + b] * wre[2] - si[in3 + b] * wim[2], v3i = sr[in3 + b] * wim[2] + si[in3 + b] * wre[2];
+              const double v4r = sr[in4 + b] * wre[3] - si[in4 + b] * wim[3], v4i = sr[in4 + b] * wim[3] + si[in4 + b] * wre[3];
+              const double a1r = v1r + v4r, a1i = v1i + v4i, a2r = v2r + v3r, a2i = v2i + v3i;
+              const double b1r = v1r - v4r, b1i = v1i - v4i, b2r = v2r - v3r, b2i = v2i - v3i;
+              const double p1r = v0r + c51 * a1r + c52 * a2r, p1i = v0i + c51 * a1i + c52 * a2i;
+              const double p2r = v0r + c52 * a1r + c51 * a2r, p2i = v0i + c52 * a1i + c51 * a2i;
+              const double q1r = s51 * b1r + s52 * b2r, q1i = s51 * b1i + s52 * b2i;
+              const double q2r = s52 * b1r - s51 * b2r, q2i = s52 * b1i - s51 * b2i;
+              dr[o + b] = v0r + a1r + a2r; di[o + b] = v0i + a1i + a2i;
+              dr[o1 + b] = p1r - q1i; di[o1 + b] = p1i + q1r;   // p1 + i q1
+              dr[o4 + b] = p1r + q1i; di[o4 + b] = p1i - q1r;
+              dr[o2 + b] = p2r - q2i; di[o2 + b] = p2i + q2r;
+              dr[o3 + b] = p2r + q2i; di[o3 + b] = p2i - q2r;
+            }
+          }
+        }
+      }
+      std::swap(sr, dr);
+      std::swap(si, di);
+    }
+    if (sr != xr) {
+      std::copy(sr, sr + (size_t)N * B, xr);
+      std::copy(si, si + (size_t)N * B, xi);
+    }
   }
 };
 
-void FFT2D(vector<cplx>& a, int nr, int nc, const FFT1D& fr, const FFT1D& fc, int sign, int nthreads) {
-#pragma omp parallel for schedule(static) num_threads(nthreads)
-  for (int i = 0; i < nr; i++) {
-    vector<cplx> work(nc);
-    fc.run(&a[(long)i * nc], 1, work.data(), sign);
+// Work arrays of one thread for batched transforms of length up to n.
+struct FFTWork {
+  vector<double> xr, xi, wr, wi;
+  explicit FFTWork(int n) : xr((size_t)n * kBatch), xi((size_t)n * kBatch), wr((size_t)n * kBatch), wi((size_t)n * kBatch) {}
+};
+
+// Half spectra of real nrImg x ncImg images zero-padded to nr x nc: H[i * nh + k],
+// nh = nc / 2 + 1, for every padded row i.
+struct RealFFT2D {
+  int nr = 0, nc = 0, nh = 0, nthreads = 1;
+  FFT1D fr, fc;
+  void plan(int nRows, int nCols, int nthr) {
+    nr = nRows;
+    nc = nCols;
+    nh = nc / 2 + 1;
+    nthreads = nthr;
+    fr.plan(nr);
+    fc.plan(nc);
   }
-#pragma omp parallel for schedule(static) num_threads(nthreads)
-  for (int j = 0; j < nc; j++) {
-    vector<cplx> work(nr);
-    fr.run(&a[j], nc, work.data(), sign);
+  // rows of img (nrImg x ncImg, row stride ncImg) -> H rows 0..nrImg-1 (the rest zero):
+  // rows 2p and 2p+1 share one complex transform, split by Hermitian symmetry;
+  // kBatch such pairs per batched transform.
+  void rows_forward(const double* img, int nrImg, int ncImg, vector<cplx>& H) const {
+    constexpr int B = kBatch;
+    H.assign((size_t)nr * nh, cplx(0.0, 0.0));
+    const int nPairs = (nrImg + 1) / 2, nGroups = (nPairs + B - 1) / B;
+#pragma omp parallel num_threads(nthreads)
+    {
+      FFTWork W(nc);
+#pragma omp for schedule(static)
+      for (int g = 0; g < nGroups; g++) {
+        std::fill(W.xr.begin(), W.xr.end(), 0.0);
+        std::fill(W.xi.begin(), W.xi.end(), 0.0);
+        for (int b = 0; b < B; b++) {
+          const int i0 = 2 * (g * B + b), i1 = i0 + 1;
+          if (i0 >= nrImg) break;
+          const double* a = img + (size_t)i0 * ncImg;
+          for (int jj = 0; jj < ncImg; jj++) W.xr[(size_t)jj * B + b] = a[jj];
+          if (i1 < nrImg) {
+            const double* c = img + (size_t)i1 * ncImg;
+            for (int jj = 0; jj < ncImg; jj++) W.xi[(size_t)jj * B + b] = c[jj];
+          }
+        }
+        fc.run(W.xr.data(), W.xi.data(), W.wr.data(), W.wi.data(), -1);
+        for (int b = 0; b < B; b++) {
+          const int i0 = 2 * (g * B + b), i1 = i0 + 1;
+          if (i0 >= nrImg) break;
+          cplx* h0 = &H[(size_t)i0 * nh];
+          cplx* h1 = (i1 < nrImg) ? &H[(size_t)i1 * nh] : nullptr;
+          for (int k = 0; k < nh; k++) {
+            const size_t e = (size_t)k * B + b, f = (size_t)((nc - k) % nc) * B + b;
+            const double zr = W.xr[e], zi = W.xi[e], cr = W.xr[f], ci = -W.xi[f];   // Z[k], conj Z[nc-k]
+            h0[k] = cplx(0.5 * (zr + cr), 0.5 * (zi + ci));
+            if (h1) h1[k] = cplx(0.5 * (zi - ci), -0.5 * (zr - cr));   // -i (Z - conj Z') / 2
+          }
+        }
+      }
+    }
   }
-}
+  // every stored column: forward transform over nr rows; then, with spec given,
+  // multiply by spec and transform back (the column half of a convolution);
+  // kBatch adjacent columns per batched transform.
+  void columns(vector<cplx>& H, const vector<cplx>* spec) const {
+    constexpr int B = kBatch;
+    const int nBlocks = (nh + B - 1) / B;
+#pragma omp parallel num_threads(nthreads)
+    {
+      FFTWork W(nr);
+#pragma omp for schedule(static)
+      for (int blk = 0; blk < nBlocks; blk++) {
+        const int k0 = blk * B, nk = std::min(B, nh - k0);
+        if (nk < B) {
+          std::fill(W.xr.begin(), W.xr.end(), 0.0);
+          std::fill(W.xi.begin(), W.xi.end(), 0.0);
+        }
+        for (int i = 0; i < nr; i++) {
+          const cplx* row = &H[(size_t)i * nh + k0];
+          for (int c = 0; c < nk; c++) {
+            W.xr[(size_t)i * B + c] = row[c].real();
+            W.xi[(size_t)i * B + c] = row[c].imag();
+          }
+        }
+        fr.run(W.xr.data(), W.xi.data(), W.wr.data(), W.wi.data(), -1);
+        if (spec) {
+          for (int i = 0; i < nr; i++)   // complex multiply (core/convolver.cpp:372-379)
+            for (int c = 0; c < nk; c++) {
+              const cplx sp = (*spec)[(size_t)i * nh + k0 + c];
+              const size_t e = (size_t)i * B + c;
+              const double ar = W.xr[e], bi = W.xi[e], cr = sp.real(), di = sp.imag();
+              W.xr[e] = ar * cr - bi * di;
+              W.xi[e] = bi * cr + ar * di;
+            }
+          fr.run(W.xr.data(), W.xi.data(), W.wr.data(), W.wi.data(), +1);
+        }
+        for (int i = 0; i < nr; i++) {
+          cplx* row = &H[(size_t)i * nh + k0];
+          for (int c = 0; c < nk; c++) row[c] = cplx(W.xr[(size_t)i * B + c], W.xi[(size_t)i * B + c]);
+        }
+      }
+    }
+  }
+  // H rows 0..nrOut-1 -> real rows (scaled by `scale`) into out (row stride ncOut,
+  // the first ncOut columns): two half spectra X, Y as one transform of X + iY.
+  void rows_inverse(const vector<cplx>& H, int nrOut, int ncOut, double scale, double* out) const {
+    constexpr int B = kBatch;
+    const int nPairs = (nrOut + 1) / 2, nGroups = (nPairs + B - 1) / B;
+#pragma omp parallel num_threads(nthreads)
+    {
+      FFTWork W(nc);
+#pragma omp for schedule(static)
+      for (int g = 0; g < nGroups; g++) {
+        for (int b = 0; b < B; b++) {
+          const int i0 = 2 * (g * B + b), i1 = i0 + 1;
+          if (i0 >= nrOut) {
+            for (int k = 0; k < nc; k++) W.xr[(size_t)k * B + b] = W.xi[(size_t)k * B + b] = 0.0;
+            continue;
+          }
+          const cplx* x = &H[(size_t)i0 * nh];
+          const cplx* y = (i1 < nrOut) ? &H[(size_t)i1 * nh] : nullptr;
+          for (int k = 0; k < nc; k++) {   // X + i Y, Hermitian completion above nh
+            cplx xk, yk(0.0, 0.0);
+            if (k < nh) {
+              xk = x[k];
+              if (y) yk = y[k];
+            } else {
+              xk = std::conj(x[nc - k]);
+              if (y) yk = std::conj(y[nc - k]);
+            }
+            W.xr[(size_t)k * B + b] = xk.real() - yk.imag();
+            W.xi[(size_t)k * B + b] = xk.imag() + yk.real();
+          }
+        }
+        fc.run(W.xr.data(), W.xi.data(), W.wr.data(), W.wi.data(), +1);
+        for (int b = 0; b < B; b++) {
+          const int i0 = 2 * (g * B + b), i1 = i0 + 1;
+          if (i0 >= nrOut) break;
+          double* o0 = out + (size_t)i0 * ncOut;
+          for (int jj = 0; jj < ncOut; jj++) o0[jj] = scale * W.xr[(size_t)jj * B + b];
+          if (i1 < nrOut) {
+            double* o1 = out + (size_t)i1 * ncOut;
+            for (int jj = 0; jj < ncOut; jj++) o1[jj] = scale * W.xi[(size_t)jj * B + b];
+          }
+        }
+      }
+    }
+  }
+};
+
+// Wall time spent in FFT convolutions since the last reset (the CPU baseline
+// reports the FFT's share of an evaluation).
+double gConvSeconds = 0.0;
 
 // ------------------------------------------------------------------ Convolver
 // core/convolver.cpp: SetupPSF (175-184), DoFullSetup (204-331), ConvolveImage
-// (340-414), ShiftAndWrapPSF (421-441).  mode 0 = FFT (padded to a smooth length
-// >= nImage + nPSF - 1), mode 1 = exact direct summation of the same linear
-// convolution (the definition both FFT codes must reproduce).
+// (340-414), ShiftAndWrapPSF (421-441).  mode 0 = FFT (r2c / c2r through half
+// spectra, padded to a smooth length >= nImage + nPSF - 1), mode 1 = exact direct
+// summation of the same linear convolution (the definition both FFT codes must
+// reproduce).
 struct Convolver {
   int ncPsf = 0, nrPsf = 0, ncImg = 0, nrImg = 0, ncPad = 0, nrPad = 0, mode = 0, nthreads = 1;
   vector<double> psf;   // normalised copy
-  vector<cplx> psfFFT;
-  FFT1D fr, fc;
+  vector<cplx> psfFFT;  // half spectrum of the wrapped PSF
+  RealFFT2D fft;
+  vector<cplx> H;       // work half spectrum
   int Setup(const double* psfIn, int nc, int nr, bool normalize, int ncImage, int nrImage, int convMode, int nthr) {
     ncPsf = nc; nrPsf = nr; ncImg = ncImage; nrImg = nrImage; mode = convMode; nthreads = nthr;
     psf.assign(psfIn, psfIn + (long)nc * nr);
@@ -917,16 +1137,16 @@ struct Convolver {
     if (mode == 0) {
       ncPad = NextSmooth(ncImg + ncPsf - 1);
       nrPad = NextSmooth(nrImg + nrPsf - 1);
-      fr.plan(nrPad);
-      fc.plan(ncPad);
-      psfFFT.assign((long)ncPad * nrPad, 0.0);
+      fft.plan(nrPad, ncPad, nthreads);
+      vector<double> wrapped((size_t)ncPad * nrPad, 0.0);
       const int cx = ncPsf / 2, cy = nrPsf / 2;   // ShiftAndWrapPSF
       for (int i = 0; i < nrPsf; i++)
         for (int j = 0; j < ncPsf; j++) {
           int destCol = (ncPad - cx + j) % ncPad, destRow = (nrPad - cy + i) % nrPad;
-          psfFFT[(long)destRow * ncPad + destCol] = psf[(long)i * ncPsf + j];
+          wrapped[(size_t)destRow * ncPad + destCol] = psf[(size_t)i * ncPsf + j];
         }
-      FFT2D(psfFFT, nrPad, ncPad, fr, fc, -1, nthreads);
+      fft.rows_forward(wrapped.data(), nrPad, ncPad, psfFFT);
+      fft.columns(psfFFT, nullptr);
     }
     return 0;
   }
@@ -952,18 +1172,11 @@ struct Convolver {
       std::copy(out.begin(), out.end(), img);
       return;
     }
-    vector<cplx> a((long)ncPad * nrPad, 0.0);
-    for (int i = 0; i < nrImg; i++)
-      for (int j = 0; j < ncImg; j++) a[(long)i * ncPad + j] = img[(long)i * ncImg + j];
-    FFT2D(a, nrPad, ncPad, fr, fc, -1, nthreads);
-    for (size_t z = 0; z < a.size(); z++) {   // complex multiply (core/convolver.cpp:372-379)
-      double ar = a[z].real(), bi = a[z].imag(), c = psfFFT[z].real(), d = psfFFT[z].imag();
-      a[z] = cplx(ar * c - bi * d, bi * c + ar * d);
-    }
-    FFT2D(a, nrPad, ncPad, fr, fc, +1, nthreads);
-    const double rescale = 1.0 / ((double)ncPad * (double)nrPad);
-    for (int i = 0; i < nrImg; i++)
-      for (int j = 0; j < ncImg; j++) img[(long)i * ncImg + j] = rescale * a[(long)i * ncPad + j].real();
+    const double t0 = omp_get_wtime();
+    fft.rows_forward(img, nrImg, ncImg, H);
+    fft.columns(H, &psfFFT);
+    fft.rows_inverse(H, nrImg, ncImg, 1.0 / ((double)ncPad * (double)nrPad), img);
+    gConvSeconds += omp_get_wtime() - t0;
   }
 };
 
@@ -1364,6 +1577,11 @@ int orc_get_expanded_model_image(orc_model* m, double* out) {
 }
 int orc_get_weights(orc_model* m, double* out) { std::copy(m->weight.begin(), m->weight.end(), out); return 0; }
 int orc_compute_deviates(orc_model* m, const double* params, double* y) { m->ComputeDeviates(y, params); return 0; }
+double orc_conv_seconds(int reset) {
+  const double t = gConvSeconds;
+  if (reset) gConvSeconds = 0.0;
+  return t;
+}
 int orc_fit_statistic(orc_model* m, const double* params, double* out) { *out = m->GetFitStatistic(params); return 0; }
 
 // ---- single-function and primitive entry points for unit-level pins

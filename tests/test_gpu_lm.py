@@ -97,9 +97,15 @@ def test_analytic_and_normal_lm_reach_same_minimum(name):
     c_an = g.GetFitStatistic(x_an)
     x_ne, r_ne = g.FitLM(case.params, limits=lim, ftol=1e-12, method="normal")
     c_ne = g.GetFitStatistic(x_ne)
-    print(name, "fd %.9f analytic %.9f normal %.9f (iters %d, %d)" % (c_fd, c_an, c_ne, r_an.niter, r_ne.niter))
+    # the normal-equation LM with mp_fdjac2's forward-difference Jacobian (one batched
+    # launch of nFree + 1 evaluations per Jacobian)
+    x_nf, r_nf = g.FitLM(case.params, limits=lim, ftol=1e-12, method="normal", jacobian="fd")
+    c_nf = g.GetFitStatistic(x_nf)
+    print(name, "fd %.9f analytic %.9f normal %.9f normal-fd %.9f (iters %d, %d, %d)" %
+          (c_fd, c_an, c_ne, c_nf, r_an.niter, r_ne.niter, r_nf.niter))
     assert abs(c_an - c_fd) <= 1e-9 * c_fd
     assert abs(c_ne - c_fd) <= 1e-9 * c_fd
+    assert r_nf.status > 0 and abs(c_nf - c_fd) <= 1e-9 * c_fd
     # converged to machine tolerance, exact-derivative mpfit and the normal-equation
     # LM agree on the parameters to 1e-6 relative
     x_an2, _ = g.FitLM(case.params, limits=lim, ftol=1e-15, xtol=1e-15, jacobian="analytic")
