@@ -337,6 +337,8 @@ struct Plan {
   void* pinnedOut = nullptr;
   double* outDev = nullptr;   // pinnedOut as the device addresses it
   bool directOut = false;     // k_final writes the statistics into pinnedOut itself (few jobs)
+  unsigned int* doneHost = nullptr;   // one-job plans: word after the results that k_final sets to 1
+  unsigned int* doneDev = nullptr;    // (the same, as the device addresses it)
   DevBuf stage_d, tail_d, pairs_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, tasks_d;
   DevBuf maps_d, jobs_d, njobs_d, tileMap_d;
   size_t usedOff = 0, prefixOff = 0;   // staging offsets of the per-grid box pixel counts / tail prefixes
@@ -1022,8 +1024,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     P.outHost.assign(outBytes / sizeof(double), 0.0);
     P.pinnedOut = P.outHost.data();
     P.outDev = nullptr;
-  } else {
-    HIPCHK(hipHostMalloc(&P.pinnedOut, outBytes, hipHostMallocDefault), "pinned results");
+  } else {   // results, then a completion word
+    HIPCHK(hipHostMalloc(&P.pinnedOut, outBytes + 64, hipHostMallocDefault), "pinned results");
     HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
   }
   if (flags & PF_DEV) HIPCHK(P.dev_d.ensure(sizeof(double) * nData * nVec), "alloc deviates");
@@ -1132,7 +1134,6 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
         P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
         P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
-        P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
       }
       if (region) {
         const Region& R = *m->regions[gi - 1];
@@ -1142,6 +1143,10 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         Tk.c0 = R.x1 - 1;
         Tk.r0 = R.y1 - 1;
       }
+      if (region)   // within the main grid's LDS, or 52 KB (three blocks per CU with allocation granules)
+        Tk.subPar = RegionSubPar(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, std::max<size_t>(P.lds3, 53248));
+      if (Tk.kind != TK_MAIN_PLAIN)
+        P.lds3 = std::max(P.lds3, InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
       FillTaskStat<T>(m, Tk.st);
       Tk.st.devOut = (flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
       if (flags & PF_DEVKEEP) Tk.st.devOut = m->devkeep_d.as<double>();
@@ -1199,6 +1204,10 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   }
   P.nSlots = maxSlots;
   P.directOut = !wa && jobs.size() <= 8;
+  if (P.directOut && jobs.size() == 1 && P.nChunks == 1 && flags == PF_STAT) {
+    P.doneHost = reinterpret_cast<unsigned int*>(reinterpret_cast<char*>(P.pinnedOut) + outBytes);
+    P.doneDev = reinterpret_cast<unsigned int*>(reinterpret_cast<char*>(P.outDev) + outBytes);
+  }
   P.tailWaves = 0;
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
   if ((st = PlanBuf(m, wa, P.tasks_d, sizeof(Task) * tasks.size(), true, "task table")) < 0 ||
@@ -1322,11 +1331,12 @@ int Enqueue(ifg_model* m, Plan& P) {
           "k_tail");
     TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, map1, P.nB1[c], P.lds1, disk, deriv), "k_eval_rowfft");
     TIMED("k_colconv", launch_colconv<T>(s, tk, map2, P.nB2[c], P.lds2), "k_colconv");
-    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv), "k_inverse_stat");
+    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv, !m->doConv),
+          "k_inverse_stat");
     if (reduce)   // a few results go straight into the pinned results (mapped): no copy back
       TIMED("k_final",
             launch_final(s, P.jobs_d.as<FinalJob>() + P.jobOff[c], nullptr, P.nJobs[c], P.partials_d.as<double>(),
-                         mode, P.directOut ? P.outDev : P.out_d.as<double>(), P.directOut ? 1 : 0),
+                         mode, P.directOut ? P.outDev : P.out_d.as<double>(), P.directOut ? 1 : 0, P.doneDev),
             "k_final");
   }
   if (P.flags & PF_JAC) {
@@ -1340,6 +1350,17 @@ int Enqueue(ifg_model* m, Plan& P) {
     HIPCHK(hipMemcpyAsync(P.pinnedOut, P.out_d.p, sizeof(double) * nVec, hipMemcpyDeviceToHost, s), "results D2H");
   }
   return IFG_OK;
+}
+
+// Wait for a one-job evaluation by spinning on the completion word its k_final sets
+// in mapped host memory, instead of the runtime's stream wait.  Gives up after
+// 20 ms (false: the caller then waits on the stream, which also reports errors).
+bool WaitDone(const unsigned int* done) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int it = 0;; it++) {
+    if (__atomic_load_n(done, __ATOMIC_ACQUIRE) != 0u) return true;
+    if ((it & 255) == 255 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) return false;
+  }
 }
 
 Plan* FindPlan(ifg_model* m, int nVec, int flags, const std::vector<int>& freeIdx) {
@@ -1380,6 +1401,7 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
   }
   if (!P) return m->fail(IFG_ERR_UNSUPPORTED, "sub-pixel box layout does not fit");
   hipStream_t s = m->stream;
+  if (P->doneHost) __atomic_store_n(P->doneHost, 0u, __ATOMIC_RELEASE);
   // Kernel timing brackets every launch with events, which stream capture does not
   // take, so timed evaluations launch eagerly.
   if (m->useGraphs && !m->timing) {
@@ -1412,7 +1434,7 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       HIPCHK(hipMemcpyAsync(devOut, P->dev_d.p, sizeof(double) * m->nData * nVec, hipMemcpyDeviceToHost, s),
              "Jacobian D2H");
   }
-  HIPCHK(hipStreamSynchronize(s), "evaluation");
+  if (!(P->doneHost && !m->timing && WaitDone(P->doneHost))) HIPCHK(hipStreamSynchronize(s), "evaluation");
   if (flags & PF_JAC) {
     if (normalOut) std::memcpy(normalOut, P->pinnedOut, sizeof(double) * (nVec * (nVec + 1) / 2 + nVec));
   } else if (outStat) {
@@ -2209,6 +2231,7 @@ struct BatchPlan {
   size_t lds1 = 0, lds2 = 0, lds3 = 0;
   long tailWaves = 0;
   bool disk = false;
+  bool plain = false;   // some model has no PSF: its main grid is evaluated in the inverse pass
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   BatchPlan() = default;
@@ -2307,6 +2330,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
       if (g.Nc > 0) specBytes += (size_t)g.Nr * g.Nc * elem * nPairs;
     }
     P.disk = P.disk || HeavyVariant(m);
+    P.plain = P.plain || !m->doConv;
     P.recOff[k] = stage;
     stage += (size_t)m->nComp() * sizeof(CompRec);
     P.boxOff[k] = stage;
@@ -2415,7 +2439,6 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
         Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
         P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
         P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
-        P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
       }
       if (region) {
         const Region& R = *m->regions[gi - 1];
@@ -2425,6 +2448,10 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
         Tk.c0 = R.x1 - 1;
         Tk.r0 = R.y1 - 1;
       }
+      if (region)   // within the main grid's LDS, or 52 KB (three blocks per CU with allocation granules)
+        Tk.subPar = RegionSubPar(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, std::max<size_t>(P.lds3, 53248));
+      if (Tk.kind != TK_MAIN_PLAIN)
+        P.lds3 = std::max(P.lds3, InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
       FillTaskStat<T>(m, Tk.st);
       if (P.kind == BK_KEEP) {
         Tk.st.devOut = m->devkeep_d.as<double>();
@@ -2674,7 +2701,7 @@ int EnqueueBatch(ifg_batch* b, BatchPlan& P, const BatchLaunch& L) {
   if ((e = launch_eval_rowfft<T>(s, tk, L.br1, L.nB1, P.lds1, P.disk, deriv)) != hipSuccess)
     return b->hipfail(e, "k_eval_rowfft");
   if ((e = launch_colconv<T>(s, tk, L.br2, L.nB2, P.lds2)) != hipSuccess) return b->hipfail(e, "k_colconv");
-  if ((e = launch_inverse_stat<T>(s, tk, L.br3, L.nB3, P.lds3, sl, P.disk, deriv)) != hipSuccess)
+  if ((e = launch_inverse_stat<T>(s, tk, L.br3, L.nB3, P.lds3, sl, P.disk, deriv, P.plain)) != hipSuccess)
     return b->hipfail(e, "k_inverse_stat");
   if (!deriv)
     e = launch_final(s, P.jobs_d.as<FinalJob>(), L.jobList, L.nJobs, P.partials_d.as<double>(), mode,

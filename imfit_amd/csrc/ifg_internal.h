@@ -119,6 +119,26 @@ struct TaskStat {
 // task t owns blocks [b, b + nPairs * n) of a launch (n blocks per pair), and a
 // block -> task map gives each block its task.
 enum TaskKind { TK_MAIN_CONV = 0, TK_MAIN_PLAIN = 1, TK_REGION = 2 };
+
+// LDS bytes of an inverse-pass block: two buffers of rows3 rows of Nc values and the
+// twiddle table; an oversampled region transforms its sub-rows subPar at a time
+// (buffers of subPar rows) and keeps the s x s block sums of its rows3 / scale base
+// rows of nbc pixels, for both planes, after the table.
+__host__ __device__ inline size_t InverseLdsBytes(int kind, int rows3, int Nc, int scale, int nbc, size_t elem,
+                                                  int subPar) {
+  if (kind == TK_REGION)
+    return (2 * (size_t)subPar + 1) * Nc * elem + 2 * (size_t)(rows3 / scale) * nbc * sizeof(double);
+  return (2 * (size_t)rows3 + 1) * Nc * elem;
+}
+
+// Sub-rows a region's inverse block transforms together: the most (up to its rows3)
+// whose LDS stays within `budget` bytes, so that region blocks do not lower the
+// occupancy of the main grid's blocks in the same launch; at least one.
+__host__ __device__ inline int RegionSubPar(int rows3, int Nc, int scale, int nbc, size_t elem, size_t budget) {
+  int p = rows3;
+  while (p > 1 && InverseLdsBytes(TK_REGION, rows3, Nc, scale, nbc, elem, p) > budget) p--;
+  return p;
+}
 struct Task {
   int kind, nPairs;
   GridGeom g;
@@ -135,6 +155,7 @@ struct Task {
   int nStrips, stripsPerLine;  // column pass: strips of C columns, strips sharing a 128-byte line
   int slot0;                   // partial slot of inverse block (pair p, block lb): slot0 + p * n3 + lb
   int scale, nbc, rowsRep, c0, r0;   // region: data pixels [c0, c0+nbc) x [r0, r0+rowsRep) it replaces
+  int subPar;                  // region: sub-rows transformed together in the inverse pass
   int nVec;                    // vectors (planes) of this task in the launch
   EvalArgs ea;
   TaskStat st;

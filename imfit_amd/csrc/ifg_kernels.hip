@@ -1376,8 +1376,9 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
   stage_pair<DERIV>(Tk, pr, sRec, sBox);
   const int ir0 = lb * rows;
   // the block's rows x Nc plane entries as one index space, K per thread per pass
-  // (one pass for rows of up to K * kThreads entries)
-  constexpr int K = 5;
+  // (one pass for rows of up to K * kThreads entries); FP64 takes 2 per pass, which
+  // keeps the kernel at 3 waves per SIMD (160 VGPRs; 5 per pass needs 198: 2 waves)
+  constexpr int K = sizeof(T) == 8 ? 2 : 5;
   const int total = rows * g.Nc;
   for (int i0 = threadIdx.x; i0 < total; i0 += K * kThreads) {
     int cols[K], ra[K], rb[K];
@@ -1790,9 +1791,11 @@ __device__ __forceinline__ void write_job(const FinalJob& J, ENorm a, const ENor
 
 // One block per job: the statistic of the job's vector(s) from the partial slots
 // of its pair (both planes of a single image split into halves are one vector).
+// done (launches of one job writing to mapped host memory): set to 1 after the
+// result is visible system-wide, so that the host can stop waiting on it alone.
 __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const int* __restrict__ jobList,
                                                const double* __restrict__ partials, int mode,
-                                               double* __restrict__ out, int hostOut) {
+                                               double* __restrict__ out, int hostOut, unsigned int* done) {
   const FinalJob& J = jobs[jobList ? jobList[blockIdx.x] : blockIdx.x];
   if (J.active && !*J.active) return;
   ENorm a, b;
@@ -1800,6 +1803,10 @@ __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs
   if (threadIdx.x == 0) {
     write_job(J, a, b, mode, out);
     if (hostOut) __threadfence_system();   // the host reads it once the stream is done
+    if (done) {
+      __hip_atomic_store(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __threadfence_system();
+    }
   }
 }
 
@@ -1810,7 +1817,10 @@ __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs
 //   TK_REGION: inverse row FFT of the oversampled plane, s x s block mean of the
 //     sub-pixels (rows outer, columns inner, core/downsample.cpp:84-106) into the
 //     data pixels the region replaces, and their terms.
-template <typename T, bool DISK, bool DERIV>
+// PLAIN: the launch has a no-PSF main grid (TK_MAIN_PLAIN), whose pixels this kernel
+// evaluates itself; without it the kernel carries none of the profile code and needs
+// far fewer registers (more waves per SIMD for the convolved grids).
+template <typename T, bool DISK, bool DERIV, bool PLAIN>
 __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
                                                       StatLaunch sl) {
   extern __shared__ char smem_raw[];
@@ -1825,7 +1835,8 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
   ENorm ea, eb;
   en_init(ea);
   en_init(eb);
-  if (Tk.kind == TK_MAIN_PLAIN) {   // no PSF: pixels [p0, p0 + rows3) of the Hh x W band of planes A and B,
+  if (PLAIN && Tk.kind == TK_MAIN_PLAIN) {   // no PSF: pixels [p0, p0 + rows3) of the Hh x W band of planes A
+                                             // and B,
                                     // K per thread in flight
     __shared__ CompRec sRec[2][IFG_MAXCOMP];
     __shared__ Box sBox[2][IFG_MAXBOX];
@@ -1862,12 +1873,65 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     }
   } else {
     const int rows = Tk.rows3;
+    const bool region = Tk.kind == TK_REGION;
+    // LDS layout (InverseLdsBytes): two buffers of `rows` rows (subPar rows for
+    // regions, which transform their sub-rows subPar at a time), the twiddle table,
+    // then the regions' block sums
+    const int bufRows = region ? Tk.subPar : rows;
     cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-    cpx<T>* B = A + rows * g.Nc;
-    cpx<T>* TW = B + rows * g.Nc;
+    cpx<T>* B = A + bufRows * g.Nc;
+    cpx<T>* TW = B + bufRows * g.Nc;
     stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
     const int io0 = lb * rows;
     const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
+    if (region) {
+      // TK_REGION: the block's rows are whole groups of `scale` sub-rows, inverse
+      // transformed subPar at a time.  Each sub-row's s values per base pixel are added
+      // to that pixel's sum in column order, sub-row after sub-row: the sum runs rows
+      // outer, columns inner, as DownsampleAndReplace's (core/downsample.cpp:84-106).
+      const int s = Tk.scale, par = Tk.subPar;
+      const int nbr = rows / s;
+      const int nAcc = 2 * nbr * Tk.nbc;
+      double* acc = reinterpret_cast<double*>(TW + g.Nc);
+      for (int e = threadIdx.x; e < nAcc; e += kThreads) acc[e] = 0.0;
+      const int nSub = max(0, min(rows, g.Hh - io0));
+      for (int sr0 = 0; sr0 < nSub; sr0 += par) {
+        const int np = min(par, nSub - sr0);
+        lds_fill<8>(A, S + (long)(g.hb + io0 + sr0) * g.Nc, np * g.Nc, np * g.Nc);
+        __syncthreads();
+        const cpx<T>* out = fft_lds<T>(A, B, Tk.pc, np, g.Nc, TW, -1);
+        for (int e = threadIdx.x; e < 2 * Tk.nbc; e += kThreads) {
+          const int plane = e / Tk.nbc, bj = e - plane * Tk.nbc;
+          for (int rr = 0; rr < np; rr++) {
+            double* a = acc + (plane * nbr + (sr0 + rr) / s) * Tk.nbc + bj;
+            double v = *a;
+            for (int jj = 0; jj < s; jj++) {
+              const cpx<T> x = out[rr * g.Nc + bj * s + jj + g.hl];
+              v += (double)(plane == 0 ? x.x : x.y);
+            }
+            *a = v;
+          }
+        }
+        __syncthreads();   // these sub-rows are read before the next ones are loaded
+      }
+      const double area = s * s;
+      for (int e = threadIdx.x; e < nAcc; e += kThreads) {
+        const int plane = e / (nbr * Tk.nbc);
+        const int q = e - plane * nbr * Tk.nbc;
+        const int k = q / Tk.nbc, bj = q - k * Tk.nbc;
+        if (plane == 1 && pr.z < 0) continue;
+        const int io = io0 + k * s;   // first sub-row of this group in the plane
+        if (io >= g.Hh) continue;
+        const int subRow = (plane == 0 ? pr.y : pr.w) + io;
+        const int bi = subRow / s;
+        if (subRow + s > g.H || bi >= Tk.rowsRep) continue;
+        const double m = acc[e] / area;
+        if (plane == 0) stat_pixel<T>(st, sl, pr.x, Tk.r0 + bi, Tk.c0 + bj, m, ea, false);
+        else stat_pixel<T>(st, sl, pr.z, Tk.r0 + bi, Tk.c0 + bj, m, eb, false);
+      }
+      if (sl.partials) finish_block(sl.partials, Tk.slot0 + local, ea, eb);
+      return;
+    }
     // single-row main blocks: fetch the data and weights of this block's pixels
     // before the inverse FFT so that their latency overlaps it
     constexpr int K = 4;
@@ -1914,30 +1978,6 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
           if (ra < st.H) stat_pixel<T>(st, sl, pr.x, ra, c, (double)v.x, ea, true);
           if (pr.z >= 0 && rb < st.H) stat_pixel<T>(st, sl, pr.z, rb, c, (double)v.y, eb, true);
         }
-      }
-    } else {   // TK_REGION: rows of this block are whole groups of `scale` sub-rows
-      const int s = Tk.scale;
-      const int nbr = rows / s;
-      const double area = s * s;
-      for (int e = threadIdx.x; e < 2 * nbr * Tk.nbc; e += kThreads) {
-        const int plane = e / (nbr * Tk.nbc);
-        const int q = e - plane * nbr * Tk.nbc;
-        const int k = q / Tk.nbc, bj = q - k * Tk.nbc;
-        if (plane == 1 && pr.z < 0) continue;
-        const int io = io0 + k * s;   // first sub-row of this group in the plane
-        if (io >= g.Hh) continue;
-        const int subRow = (plane == 0 ? pr.y : pr.w) + io;
-        const int bi = subRow / s;
-        if (subRow + s > g.H || bi >= Tk.rowsRep) continue;
-        double binned = 0.0;
-        for (int ii = 0; ii < s; ii++)
-          for (int jj = 0; jj < s; jj++) {
-            const cpx<T> v = out[(k * s + ii) * g.Nc + bj * s + jj + g.hl];
-            binned += (double)(plane == 0 ? v.x : v.y);
-          }
-        const double m = binned / area;
-        if (plane == 0) stat_pixel<T>(st, sl, pr.x, Tk.r0 + bi, Tk.c0 + bj, m, ea, false);
-        else stat_pixel<T>(st, sl, pr.z, Tk.r0 + bi, Tk.c0 + bj, m, eb, false);
       }
     }
   }
@@ -2097,9 +2137,16 @@ hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& b
 
 template <typename T>
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
-                               const StatLaunch& sl, bool disk, bool deriv) {
+                               const StatLaunch& sl, bool disk, bool deriv, bool plain) {
   if (nBlocks <= 0) return hipSuccess;
-  IFG_VARIANT(k_inverse_stat, T, disk, deriv, dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  if (plain) {
+    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  } else {   // convolved grids only: the profile code (the one user of DISK and DERIV) is compiled out
+    IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  }
   return hipGetLastError();
 }
 
@@ -2128,9 +2175,10 @@ hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, lo
 }
 
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
-                        int mode, double* out, int hostOut) {
+                        int mode, double* out, int hostOut, unsigned int* done) {
   if (nJobs <= 0) return hipSuccess;
-  IFG_LAUNCH(true, true, k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut);
+  if (done && (nJobs != 1 || !hostOut)) return hipErrorInvalidValue;
+  IFG_LAUNCH(true, true, k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut, done);
   return hipGetLastError();
 }
 
@@ -2163,7 +2211,7 @@ hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobLis
   template hipError_t launch_colconv<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t);                 \
   template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t,             \
                                              const StatLaunch&,                                                    \
-                                             bool, bool);                                                          \
+                                             bool, bool, bool);                                                    \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
   template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);
 IFG_INST(double)

@@ -25,8 +25,9 @@ hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRange
 template <typename T>
 hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds);
 template <typename T>
+// plain: some task of the launch is a no-PSF main grid (evaluated in this pass)
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
-                               const StatLaunch& sl, bool disk, bool deriv);
+                               const StatLaunch& sl, bool disk, bool deriv, bool plain);
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
                          int dir, int rows);
@@ -35,9 +36,10 @@ hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPl
 // n8 8-byte words from mapped pinned host memory into device memory
 hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, long n8);
 // jobList: the jobs of this launch (null: jobs 0 .. nJobs-1); hostOut: out is mapped
-// host memory (each job block ends with a system-scope fence)
+// host memory (each job block ends with a system-scope fence); done (one job, hostOut
+// only, else null): mapped host word set to 1 once the result is visible
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
-                        int mode, double* out, int hostOut);
+                        int mode, double* out, int hostOut, unsigned int* done = nullptr);
 // Jacobian columns (dev_j - dev_0) / h_j in place, from nFree + 1 deviate vectors
 hipError_t launch_fd_columns(hipStream_t s, double* dev, long nData, int nFree, const FDSteps& h);
 // pixels per tile of a normal-equations job with nFree columns
