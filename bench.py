@@ -118,7 +118,7 @@ def profiled_traffic(kernel, prec):
     path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2_1024.md")
     if not os.path.exists(path):
         return None
-    names = ("ifg::%s<%s>" % (kernel, prec), "ifg::%s<%s,false,false>" % (kernel, prec))
+    names = tuple("ifg::%s<%s%s>" % (kernel, prec, v) for v in ("", ",false,false", ",false,false,false"))
     for line in open(path):
         f = line.split()
         if len(f) == 3 and f[0] == "traffic_bytes" and f[1] in names:
@@ -126,7 +126,7 @@ def profiled_traffic(kernel, prec):
     return None
 
 
-def profiled_valu(kernel, prec, profile="valu_cfg2_1024.md", variants=("", ",false,false")):
+def profiled_valu(kernel, prec, profile="valu_cfg2_1024.md", variants=("", ",false,false", ",false,false,false")):
     """VALU wave instructions per dispatch of `kernel` from profiles/<profile>
     (rocprofv3 SQ_INSTS_VALU of the workload), or None."""
     path = os.path.join(ROOT, "profiles", profile)
@@ -252,7 +252,7 @@ def aux_benchmarks(steps, fits=True):
         res4["evals_per_s_" + key] = n / dt
         kt, _ = kernel_times(m, p4, n, 1)
         res4["roofline_" + key] = valu_roofline(kt, "float" if key == "fp32" else "double", "valu_cfg3_cfg4.md",
-                                                (",true,false",))
+                                                ("", ",true,false", ",false,false,false"))
     out["cfg4_expdisk3d_512"] = res4
     log("cfg4 (3-D disk + bulge, 512^2, PSF 25): %.1f evals/s fp32, %.1f evals/s fp64" %
         (res4["evals_per_s_fp32"], res4["evals_per_s_fp64"]))
@@ -271,7 +271,7 @@ def aux_benchmarks(steps, fits=True):
     out["cfg3_population_512"] = {"vectors": int(pop.shape[0]), "s_per_batch": dt,
                                   "chi2_per_s": pop.shape[0] / dt, "finite": bool(np.isfinite(chi).all()),
                                   "roofline_fp32": valu_roofline(kt3, "float", "valu_cfg3_cfg4.md",
-                                                                 ("", ",false,false"))}
+                                                                 ("", ",false,false", ",false,false,false"))}
     log("cfg3 (4096-vector population, 512^2, PSF 35, FP32): %.3f s per batch, %.0f chi^2/s" %
         (dt, pop.shape[0] / dt))
     if fits:
