@@ -17,19 +17,22 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
 
 
 def build_cpp_driver(verbose=False):
-    """The C++ caller tests/cpp/lm_fit_driver (ModelObject-shaped host class over the
-    C ABI), linked against the in-tree library."""
+    """The C++ callers tests/cpp/lm_fit_driver and tests/cpp/dropin_driver
+    (ModelObject-shaped host class over the C ABI), linked against the in-tree library."""
     root = os.path.dirname(HERE)
-    src = os.path.join(root, "tests", "cpp", "lm_fit_driver.cpp")
-    if not os.path.exists(src):
-        return None
-    out = os.path.join(root, "tests", "cpp", "lm_fit_driver")
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
-           "-L", HERE, "-limfit_gpu", "-Wl,-rpath,$ORIGIN/../../imfit_amd", "-o", out]
-    if verbose:
-        print(" ".join(cmd))
-    subprocess.run(cmd, check=True)
-    return out
+    outs = []
+    for name in ("lm_fit_driver", "dropin_driver"):
+        src = os.path.join(root, "tests", "cpp", name + ".cpp")
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(root, "tests", "cpp", name)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
+               "-L", HERE, "-limfit_gpu", "-Wl,-rpath,$ORIGIN/../../imfit_amd", "-o", out]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True)
+        outs.append(out)
+    return outs
 
 
 def build(verbose=False):

@@ -437,6 +437,7 @@ struct ifg_model {
   int fdTiles = 0;
   double fdScale = -1.0;   // ||w d|| over the valid pixels: rounding scale of the chi^2 deviates
   std::unique_ptr<Plan> areaPlan;   // GetFitStatistic plan in the no-allocation work area
+  bool inflight = false;            // the last evaluation returned on its completion word (see Quiesce)
   std::vector<int> areaNeed;
   bool useGraphs = true;
   std::vector<double> modelHost;   // last CreateModelImage result
@@ -450,6 +451,7 @@ struct ifg_model {
   std::map<std::string, std::pair<double, long>> kernelTimes;   // name -> (total ms, launches)
 
   ~ifg_model() {
+    if (stream) (void)hipStreamSynchronize(stream);   // a polled evaluation may still be retiring
     for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
     for (auto& pe : pendingEvents) {
       (void)hipEventDestroy(pe.second.first);
@@ -479,6 +481,13 @@ struct ifg_model {
     hipError_t _e = (expr);                          \
     if (_e != hipSuccess) return m->hipfail(_e, where); \
   } while (0)
+
+// An evaluation that returned on its completion word (WaitDone) may still be retiring
+// on the stream: wait for it before anything its plan holds is freed or rebuilt.
+void Quiesce(ifg_model* m) {
+  if (m->inflight && m->stream) (void)hipStreamSynchronize(m->stream);
+  m->inflight = false;
+}
 
 // The kernels' DISK variants carry the functions with heavy per-sample code (the 3-D
 // disk's line-of-sight quadrature, the edge-on disk's Bessel function); models
@@ -1381,7 +1390,10 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
   std::vector<int> need;
   for (int attempt = 0; attempt < 2; attempt++) {
     if (!P) {
-      if (m->plans.size() >= 8) m->plans.erase(m->plans.begin());   // bounded plan cache
+      if (m->plans.size() >= 8) {   // bounded plan cache
+        Quiesce(m);
+        m->plans.erase(m->plans.begin());
+      }
       m->plans.emplace_back(new Plan());
       P = m->plans.back().get();
       P->freeIdx = freeIdx;
@@ -1392,6 +1404,7 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
     }
     if (StageCall(m, *P, params, need) == 0) break;
     // a box layout larger than the plan's tail stride: rebuild with room for it
+    Quiesce(m);
     for (size_t k = 0; k < m->plans.size(); k++)
       if (m->plans[k].get() == P) {
         m->plans.erase(m->plans.begin() + k);
@@ -1434,7 +1447,8 @@ int Evaluate(ifg_model* m, const double* params, int nVec, double* outStat, doub
       HIPCHK(hipMemcpyAsync(devOut, P->dev_d.p, sizeof(double) * m->nData * nVec, hipMemcpyDeviceToHost, s),
              "Jacobian D2H");
   }
-  if (!(P->doneHost && !m->timing && WaitDone(P->doneHost))) HIPCHK(hipStreamSynchronize(s), "evaluation");
+  m->inflight = P->doneHost && !m->timing && WaitDone(P->doneHost);
+  if (!m->inflight) HIPCHK(hipStreamSynchronize(s), "evaluation");
   if (flags & PF_JAC) {
     if (normalOut) std::memcpy(normalOut, P->pinnedOut, sizeof(double) * (nVec * (nVec + 1) / 2 + nVec));
   } else if (outStat) {
@@ -1944,6 +1958,7 @@ int ifg_final_setup(ifg_model* m) try {   // core/model_object.cpp:967-1079
   m->finalDone = true;
   m->fdScale = -1.0;
   m->devReady = false;
+  Quiesce(m);
   m->plans.clear();
   m->areaPlan.reset();
   return status;
@@ -2083,6 +2098,18 @@ int ifg_get_model_image(ifg_model* m, double* out) try {
   }
   std::memcpy(out, m->modelHost.data(), sizeof(double) * m->nData);
   return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
+}
+
+int ifg_get_expanded_model_image(ifg_model* m, double* out) try {
+  if (!m) return IFG_ERR_ARG;
+  if (!out) return m->fail(IFG_ERR_ARG, "null output");
+  if (m->doConv)   // the GPU planes hold the data grid plus the PSF halo, not the padded model grid
+    return m->fail(IFG_ERR_UNSUPPORTED,
+                   "the PSF-padded model grid is not computed on the GPU; use the reference's CPU ModelObject "
+                   "for it (makeimage --save-expanded)");
+  return ifg_get_model_image(m, out);   // without a PSF the model grid is the data grid
 } catch (...) {
   return ExceptionStatus(m);
 }
@@ -3025,6 +3052,7 @@ std::string GeometryKey(const ifg_model* m, const std::vector<TuneGrid>& grids, 
 // Median seconds per evaluation after dropping the plans (the first call rebuilds them).
 int TimeModel(ifg_model* m, const double* params, int nVec, double* seconds) {
   (void)hipStreamSynchronize(m->stream);
+  Quiesce(m);
   m->plans.clear();
   m->areaPlan.reset();
   std::vector<double> out(nVec);
@@ -3063,6 +3091,7 @@ int ifg_autotune(ifg_model* m, const double* params, int n_vectors) try {
     if (it != gTuned.end()) {
       for (size_t i = 0; i < grids.size(); i++) SetShape(m, *grids[i].cg, it->second[i], grids[i].split);
       (void)hipStreamSynchronize(m->stream);
+      Quiesce(m);
       m->plans.clear();
       m->areaPlan.reset();
       return 0;
@@ -3097,6 +3126,7 @@ int ifg_autotune(ifg_model* m, const double* params, int n_vectors) try {
     }
   }
   (void)hipStreamSynchronize(m->stream);
+  Quiesce(m);
   m->plans.clear();
   m->areaPlan.reset();
   std::vector<GridShape> rec;
@@ -3144,6 +3174,7 @@ int ifg_set_launch_shape(ifg_model* m, int grid, int batch_grids, int cols_per_b
     return m->fail(IFG_ERR_ARG, "launch shape does not fit this grid (columns 1, 2, 4 or 8; rows within LDS)");
   (void)hipStreamSynchronize(m->stream);
   SetShape(m, *cg, s, split);
+  Quiesce(m);
   m->plans.clear();   // plans hold the launch shapes
   m->areaPlan.reset();
   return IFG_OK;
@@ -3173,6 +3204,7 @@ void SetBatchShapes(ifg_batch* b, int C, double rowsFactor) {
       SetShape(m, *tg.cg, s, tg.split);
     }
     (void)hipStreamSynchronize(m->stream);
+    Quiesce(m);
     m->plans.clear();
     m->areaPlan.reset();
   }

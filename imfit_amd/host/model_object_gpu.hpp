@@ -121,6 +121,14 @@ class ModelObjectGPU {
     return model_.data();
   }
 
+  // GetExpandedModelImageVector (core/model_object.cpp:2297-2305): the model on the
+  // PSF-padded grid; null (with the library's message) where the GPU does not compute it
+  const double* GetExpandedModelImageVector() {
+    model_.assign((size_t)nColumns_ * nRows_, NaN());
+    if (Check(ifg_get_expanded_model_image(h_, model_.data()), "GetExpandedModelImageVector") < 0) return nullptr;
+    return model_.data();
+  }
+
   // ---- extensions over the reference interface
   // population form of GetFitStatistic: params is nVectors x nParams, row-major
   void GetFitStatisticBatch(const double* params, int nVectors, double* out) {

@@ -140,6 +140,12 @@ int ifg_create_model_image(ifg_model* m, const double* params);
 /* GetModelImageVector (core/model_object.cpp:1310-1335): data-size image of the
  * last CreateModelImage / ComputeDeviates / GetFitStatistic call */
 int ifg_get_model_image(ifg_model* m, double* out);
+/* GetExpandedModelImageVector (core/model_object.cpp:2297-2305): the model on the
+ * reference's PSF-padded grid (nColumns + 2 P_c) x (nRows + 2 P_r).  Without a PSF that
+ * grid is the data grid and this equals ifg_get_model_image.  With a PSF the padded
+ * border is not computed on the GPU (its planes hold the data grid and the PSF halo
+ * only): IFG_ERR_UNSUPPORTED, and the caller keeps the reference's CPU path for it. */
+int ifg_get_expanded_model_image(ifg_model* m, double* out);
 /* ComputeDeviates (core/model_object.cpp:1413-1496): y[nData] */
 int ifg_compute_deviates(ifg_model* m, const double* params, double* y);
 /* GetFitStatistic (core/model_object.cpp:1620-1626): chi^2, Cash or Poisson-MLR */
