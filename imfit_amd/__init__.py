@@ -25,6 +25,7 @@ _vp = C.c_void_p
 WEIGHTS_ARE_SIGMAS, WEIGHTS_ARE_VARIANCES, WEIGHTS_ARE_WEIGHTS = 100, 110, 120
 MASK_ZERO_IS_GOOD, MASK_ZERO_IS_BAD = 10, 20
 PRECISION_FP64, PRECISION_FP32 = 0, 1
+GRADIENT_ADJOINT, GRADIENT_TANGENT = 0, 1
 
 _SIGS = {
     "ifg_create": ([C.c_int], _vp),
@@ -57,6 +58,9 @@ _SIGS = {
     "ifg_jacobian": ([_vp, _dp, C.c_int, C.POINTER(C.c_int), _dp, _dp], C.c_int),
     "ifg_normal_equations": ([_vp, _dp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp], C.c_int),
     "ifg_fit_statistic_gradient": ([_vp, _dp, _dp, _dp], C.c_int),
+    "ifg_set_gradient_method": ([_vp, C.c_int], C.c_int),
+    "ifg_deviates_vjp": ([_vp, _dp, _dp, _dp, _dp], C.c_int),
+    "ifg_model_image_vjp": ([_vp, _dp, _dp, _dp], C.c_int),
     "ifg_set_kernel_timing": ([_vp, C.c_int], C.c_int),
     "ifg_set_graph_mode": ([_vp, C.c_int], C.c_int),
     "ifg_get_kernel_timing": ([_vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_long)], C.c_int),
@@ -348,6 +352,34 @@ class ModelObjectGPU:
         stat = np.zeros(1)
         self._chk(_LIB.ifg_fit_statistic_gradient(self.h, _ptr(p), _ptr(g), _ptr(stat)), "GetFitStatisticGradient")
         return g, float(stat[0])
+
+    def SetGradientMethod(self, method):
+        """GRADIENT_ADJOINT (reverse mode, the default) or GRADIENT_TANGENT (one
+        derivative plane per parameter) for GetFitStatisticGradient."""
+        self._chk(_LIB.ifg_set_gradient_method(self.h, int(method)), "SetGradientMethod")
+
+    def _outputs(self, u):
+        w = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        if w.size != self.nrows * self.ncols:
+            raise ValueError("expected %d output weights, got %d" % (self.nrows * self.ncols, w.size))
+        return w
+
+    def DeviatesVJP(self, params, u):
+        """(J^T u for the deviates' Jacobian J, deviates) by the adjoint pass."""
+        p = self._vec(params)
+        w = self._outputs(u)
+        g = np.empty(self.GetNParams())
+        fvec = np.empty(self.nrows * self.ncols)
+        self._chk(_LIB.ifg_deviates_vjp(self.h, _ptr(p), _ptr(w), _ptr(g), _ptr(fvec)), "DeviatesVJP")
+        return g, fvec
+
+    def ModelImageVJP(self, params, u):
+        """sum_i u_i d(model_i)/d(params) over the nData model pixels (adjoint pass)."""
+        p = self._vec(params)
+        w = self._outputs(u)
+        g = np.empty(self.GetNParams())
+        self._chk(_LIB.ifg_model_image_vjp(self.h, _ptr(p), _ptr(w), _ptr(g)), "ModelImageVJP")
+        return g
 
     # ---- Levenberg-Marquardt fits
     @staticmethod

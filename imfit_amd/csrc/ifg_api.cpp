@@ -341,6 +341,9 @@ struct Plan {
   unsigned int* doneDev = nullptr;    // (the same, as the device addresses it)
   DevBuf stage_d, tail_d, pairs_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, tasks_d;
   DevBuf maps_d, jobs_d, njobs_d, tileMap_d;
+  DevBuf aTail_d, adjPart_d;   // adjoint plans: box pixels' adjoint values, per-slot partial sums
+  int adjWidth = 0;            // nComp * IFG_SLOTS
+  long adjTailBlocks = 0;
   size_t usedOff = 0, prefixOff = 0;   // staging offsets of the per-grid box pixel counts / tail prefixes
   int nSlots = 1, nTasks = 1;
   std::vector<int> nB1, nB2, nB3, nJobs;   // per chunk: blocks of the three passes, final jobs
@@ -432,6 +435,8 @@ struct ifg_model {
   ConvGrid mainConv[2];
   DevBuf data_d, weight_d, extra_d;
   DevBuf devkeep_d, dfac_d;   // forward-pass outputs consumed by derivative passes
+  DevBuf adjU_d;              // caller's output weights of a vector-Jacobian product
+  int gradMethod = IFG_GRADIENT_ADJOINT;
   std::vector<std::unique_ptr<Plan>> plans;
   DevBuf fdJob_d, fdTiles_d, fdPart_d, fdOut_d;   // normal equations of a finite-difference Jacobian
   int fdTiles = 0;
@@ -838,8 +843,12 @@ int EnsureRegionConv(ifg_model* m, Region& R, int which) {
 // d(deviate)/dm (1) or d(statistic)/dm (2) in the model's device buffers for a
 // following derivative pass.  PF_JAC / PF_GRAD: derivative pass (planes are
 // d(model)/d(p_j)) producing Jacobian columns + normal equations, or the gradient.
+// PF_ADJ: adjoint (reverse-mode) pass over one parameter vector, weighted by the
+// forward pass's d/dm factors (PF_ADJ_DFAC) and / or the caller's output weights
+// (PF_ADJ_U).
 enum PlanFlags {
-  PF_STAT = 1, PF_DEV = 2, PF_MODEL = 4, PF_DEVKEEP = 8, PF_DFAC1 = 16, PF_DFAC2 = 32, PF_JAC = 64, PF_GRAD = 128
+  PF_STAT = 1, PF_DEV = 2, PF_MODEL = 4, PF_DEVKEEP = 8, PF_DFAC1 = 16, PF_DFAC2 = 32, PF_JAC = 64, PF_GRAD = 128,
+  PF_ADJ = 256, PF_ADJ_DFAC = 512, PF_ADJ_U = 1024
 };
 inline bool IsDeriv(int flags) { return (flags & (PF_JAC | PF_GRAD)) != 0; }
 
@@ -1501,6 +1510,305 @@ int RunDerivDispatch(ifg_model* m, const double* params, const std::vector<int>&
   if (m->precision == IFG_PRECISION_FP64)
     return RunDeriv<double>(m, params, freeIdx, kind, stat, fvec, fjac, normal, grad);
   return RunDeriv<float>(m, params, freeIdx, kind, stat, fvec, fjac, normal, grad);
+}
+
+// ---------------------------------------------------------------- adjoint (reverse-mode) plans
+// One parameter vector on the single-evaluation grids (both halves of each image in
+// one complex plane): the adjoint row pass over the Hh output rows, the column pass
+// with the conjugate PSF spectrum, the inverse pass over the Lr evaluated rows (rows1
+// rows per block), the box pixels, and the sums per (component, slot).
+template <typename T>
+int BuildAdjPlan(ifg_model* m, Plan& P, int flags, const std::vector<int>& minStride) {
+  const int nC = m->nComp(), nR = (int)m->regions.size();
+  int st;
+  P.nVec = 1;
+  P.flags = flags;
+  P.single = true;
+  P.which = 0;
+  if (m->doConv && (st = EnsureMainConv(m, 0)) < 0) return st;
+  for (auto& R : m->regions)
+    if ((st = EnsureRegionConv(m, *R, 0)) < 0) return st;
+  P.nGrids = 1 + nR;
+  P.geoms.assign(P.nGrids, GridGeom());
+  if (m->doConv) P.geoms[0] = m->mainConv[0].g;
+  else {
+    P.geoms[0] = PlainGeom(m->W, m->H, 1.0, 1.0, 1.0);
+    P.geoms[0].Hh = (m->H + 1) / 2;
+  }
+  for (int k = 0; k < nR; k++) P.geoms[1 + k] = m->regions[k]->conv[0].g;
+  P.strideMax.assign(P.nGrids, 0);
+  P.tailOff.assign(P.nGrids + 1, 0);
+  for (int gi = 0; gi < P.nGrids; gi++) {
+    int sm = m->doSubsampling ? IFG_MAXBOX * BoxAreaMax(P.geoms[gi]) : 0;
+    if (gi < (int)minStride.size()) sm = std::max(sm, minStride[gi]);
+    P.strideMax[gi] = sm;
+    P.tailOff[gi + 1] = P.tailOff[gi] + sm;
+  }
+  P.chunkPairs = P.chunkVecs = P.nChunks = P.npc = 1;
+  P.nTasks = P.nGrids;
+  P.recBytes = (size_t)nC * sizeof(CompRec);
+  P.boxBytes = (size_t)P.nGrids * IFG_MAXBOX * sizeof(Box);
+  P.usedOff = P.recBytes + P.boxBytes;
+  P.prefixOff = P.usedOff + ((P.nGrids * sizeof(int) + 15) / 16) * 16;
+  P.stageBytes = P.prefixOff + sizeof(long) * (size_t)(P.nTasks + 1);
+  HIPCHK(hipHostMalloc(&P.pinnedStage, P.stageBytes, hipHostMallocDefault), "pinned staging");
+  HIPCHK(hipHostGetDevicePointer(&P.stageDev, P.pinnedStage, 0), "map pinned staging");
+  HIPCHK(P.stage_d.ensure(P.stageBytes), "alloc staging");
+  HIPCHK(P.aTail_d.ensure(sizeof(double) * 2 * std::max<long>(1, P.tailOff[P.nGrids])), "alloc adjoint box values");
+  size_t pairBytes = 0;
+  std::vector<size_t> regOff(nR + 1, 0);
+  if (m->doConv) pairBytes = (size_t)P.geoms[0].Nr * P.geoms[0].Nc * sizeof(cpx<T>);
+  for (int k = 0; k < nR; k++) regOff[k + 1] = regOff[k] + (size_t)P.geoms[1 + k].Nr * P.geoms[1 + k].Nc * sizeof(cpx<T>);
+  HIPCHK(P.spec_d.ensure(std::max<size_t>(1, pairBytes)), "alloc planes");
+  if (nR) HIPCHK(P.specR_d.ensure(std::max<size_t>(1, regOff[nR])), "alloc region planes");
+  if (flags & PF_ADJ_U) HIPCHK(m->adjU_d.ensure(sizeof(double) * m->nData), "alloc output weights");
+  std::vector<int4> pairs(P.nGrids);
+  for (int gi = 0; gi < P.nGrids; gi++) pairs[gi] = make_int4(0, 0, 0, P.geoms[gi].Hh);
+  HIPCHK(P.pairs_d.ensure(sizeof(int4) * pairs.size()), "alloc pairs");
+  HIPCHK(hipMemcpy(P.pairs_d.p, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice), "pairs");
+
+  const CompRec* recs_d = reinterpret_cast<const CompRec*>(P.stage_d.p);
+  const Box* boxes_d = reinterpret_cast<const Box*>(P.stage_d.as<char>() + P.recBytes);
+  const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.usedOff);
+  const size_t elem = sizeof(cpx<T>);
+  std::vector<Task> tasks(P.nTasks);
+  for (int gi = 0; gi < P.nGrids; gi++) {
+    Task& Tk = tasks[gi];
+    std::memset(&Tk, 0, sizeof(Tk));
+    const bool region = gi > 0;
+    ConvGrid* cg = region ? &m->regions[gi - 1]->conv[0] : (m->doConv ? &m->mainConv[0] : nullptr);
+    Tk.kind = region ? TK_REGION : (m->doConv ? TK_MAIN_CONV : TK_MAIN_PLAIN);
+    Tk.nPairs = 1;
+    Tk.nVec = 1;
+    Tk.g = P.geoms[gi];
+    Tk.pairs = P.pairs_d.as<int4>() + gi;
+    Tk.aTail = P.aTail_d.as<double>() + 2 * P.tailOff[gi];
+    EvalArgs& ea = Tk.ea;
+    ea.nComp = nC;
+    for (int n = 0; n < nC; n++) ea.types[n] = m->types[n];
+    ea.tailStride = P.strideMax[gi];
+    ea.tailUsed = used_d + gi;
+    ea.recs = recs_d;
+    ea.recStride = nC;
+    ea.boxes = boxes_d + (long)gi * IFG_MAXBOX;
+    ea.boxStride = IFG_MAXBOX;
+    int off = 0, xy = 0;
+    for (int n = 0; n < nC; n++) {
+      if (m->fsetStart[n]) {
+        xy = off;
+        off += 2;
+      }
+      ea.compXY[n] = xy;
+      ea.compFirst[n] = off;
+      ea.compNP[n] = NParamsOf(m->types[n]);
+      off += ea.compNP[n];
+    }
+    if (Tk.kind == TK_MAIN_PLAIN) {
+      PlainBlocks(Tk, 1);
+      P.lds3 = std::max(P.lds3, 2 * (size_t)Tk.rows3 * sizeof(double));
+    } else {
+      Tk.pc = cg->fc->plan;
+      Tk.pr = cg->fr->plan;
+      Tk.twc = (sizeof(T) == 8) ? cg->fc->tw64.p : cg->fc->tw32.p;
+      Tk.twr = (sizeof(T) == 8) ? cg->fr->tw64.p : cg->fr->tw32.p;
+      Tk.khat = (sizeof(T) == 8) ? cg->khat64.p : cg->khat32.p;
+      Tk.pairStride = (long)Tk.g.Nr * Tk.g.Nc;
+      Tk.spec = region ? (void*)(P.specR_d.as<char>() + regOff[gi - 1]) : P.spec_d.p;
+      Tk.rows1 = cg->rowsK1;
+      Tk.rows3 = cg->rowsK1;
+      Tk.C = (sizeof(T) == 8) ? cg->C64 : cg->C32;
+      Tk.ld = Tk.g.Nr + 1;
+      Tk.n1 = (Tk.g.Hh + Tk.rows1 - 1) / Tk.rows1;
+      Tk.nStrips = (Tk.g.Nc + Tk.C - 1) / Tk.C;
+      Tk.stripsPerLine = std::max(1, 128 / (Tk.C * (int)elem));
+      const int unit = 8 * Tk.stripsPerLine;
+      Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
+      Tk.n3 = (Tk.g.Lr + Tk.rows3 - 1) / Tk.rows3;
+      P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
+      P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
+      P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
+    }
+    if (region) {
+      const Region& R = *m->regions[gi - 1];
+      Tk.scale = R.scale;
+      Tk.nbc = R.nbc;
+      Tk.rowsRep = R.rowsReplaced;
+      Tk.c0 = R.x1 - 1;
+      Tk.r0 = R.y1 - 1;
+    }
+    FillTaskStat<T>(m, Tk.st);
+    Tk.st.dfac = (flags & PF_ADJ_DFAC) ? m->dfac_d.as<double>() : nullptr;
+    Tk.st.uvec = (flags & PF_ADJ_U) ? m->adjU_d.as<double>() : nullptr;
+  }
+  std::vector<int2> m1, m2, m3;
+  int slot = 0;
+  LayoutBlocks(tasks, 0, P.nTasks, m1, m2, m3, slot);
+  P.nB1.assign(1, (int)m1.size());
+  P.nB2.assign(1, (int)m2.size());
+  P.nB3.assign(1, (int)m3.size());
+  BlockRanges r1, r2, r3;
+  std::memset(&r1, 0, sizeof(r1));
+  std::memset(&r2, 0, sizeof(r2));
+  std::memset(&r3, 0, sizeof(r3));
+  r1.n = r2.n = r3.n = P.nTasks;
+  for (int t = 0; t < P.nTasks; t++) {
+    r1.start[t] = tasks[t].b1;
+    r2.start[t] = tasks[t].b2;
+    r3.start[t] = tasks[t].b3;
+  }
+  r1.start[P.nTasks] = P.nB1[0];
+  r2.start[P.nTasks] = P.nB2[0];
+  r3.start[P.nTasks] = P.nB3[0];
+  P.br1.assign(1, r1);
+  P.br2.assign(1, r2);
+  P.br3.assign(1, r3);
+  P.nSlots = slot;
+  P.tailWaves = 0;
+  for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += P.strideMax[gi];
+  P.adjTailBlocks = P.tailWaves > 0 ? std::min<long>((P.tailWaves + 3) / 4, 2048) : 0;
+  P.adjWidth = nC * IFG_SLOTS;
+  HIPCHK(P.adjPart_d.ensure(sizeof(double) * (size_t)(P.nSlots + P.adjTailBlocks) * P.adjWidth),
+         "alloc adjoint partials");
+  HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * P.adjWidth + 64, hipHostMallocDefault), "pinned results");
+  HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
+  HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc task table");
+  HIPCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
+  HIPCHK(hipDeviceSynchronize(), "adjoint plan setup");
+  return IFG_OK;
+}
+
+template <typename T>
+int EnqueueAdj(ifg_model* m, Plan& P) {
+  hipStream_t s = m->stream;
+  if (P.stageBytes % 8 == 0 && P.stageBytes <= 65536)
+    HIPCHK(launch_stage_copy(s, P.stage_d.p, P.stageDev, (long)(P.stageBytes / 8)), "upload call data");
+  else
+    HIPCHK(hipMemcpyAsync(P.stage_d.p, P.pinnedStage, P.stageBytes, hipMemcpyHostToDevice, s), "upload call data");
+  const long* prefix_d = reinterpret_cast<const long*>(P.stage_d.as<char>() + P.prefixOff);
+  AdjLaunch al;
+  al.partials = P.adjPart_d.as<double>();
+  al.width = P.adjWidth;
+  al.tailSlot0 = P.nSlots;
+  TIMED("k_adjoint",
+        launch_adjoint<T>(s, P.tasks_d.as<Task>(), P.nTasks, P.br1[0], P.nB1[0], P.lds1, P.br2[0], P.nB2[0], P.lds2,
+                          P.br3[0], P.nB3[0], P.lds3, prefix_d, P.adjTailBlocks, m->doSubsampling ? 1 : 0,
+                          HeavyVariant(m), al, (int)(P.nSlots + P.adjTailBlocks), P.outDev, 1),
+        "k_adjoint");
+  return IFG_OK;
+}
+
+// Adjoint pass at params: out[nComp * IFG_SLOTS] = per (component, slot) the sum over
+// the grid pixels of a * d(intensity)/d(slot).  uHost: the caller's output weights
+// (PF_ADJ_U), copied to the device before the pass.
+template <typename T>
+int EvaluateAdj(ifg_model* m, const double* params, int flags, const double* uHost, double* out) {
+  int st = PrepareDevice(m);
+  if (st < 0) return st;
+  flags |= PF_ADJ;
+  Plan* P = FindPlan(m, 1, flags, std::vector<int>());
+  std::vector<int> need;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (!P) {
+      if (m->plans.size() >= 8) {
+        Quiesce(m);
+        m->plans.erase(m->plans.begin());
+      }
+      m->plans.emplace_back(new Plan());
+      P = m->plans.back().get();
+      if ((st = BuildAdjPlan<T>(m, *P, flags, need)) < 0) {
+        m->plans.pop_back();
+        return st;
+      }
+    }
+    if (StageCall(m, *P, params, need) == 0) break;
+    Quiesce(m);
+    for (size_t k = 0; k < m->plans.size(); k++)
+      if (m->plans[k].get() == P) {
+        m->plans.erase(m->plans.begin() + k);
+        break;
+      }
+    P = nullptr;
+  }
+  if (!P) return m->fail(IFG_ERR_UNSUPPORTED, "sub-pixel box layout does not fit");
+  hipStream_t s = m->stream;
+  if (flags & PF_ADJ_U)
+    HIPCHK(hipMemcpyAsync(m->adjU_d.p, uHost, sizeof(double) * m->nData, hipMemcpyHostToDevice, s),
+           "output weights H2D");
+  if (m->useGraphs && !m->timing) {
+    if (!P->exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+      st = EnqueueAdj<T>(m, *P);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (st < 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      HIPCHK(e, "end capture");
+      P->graph = graph;
+      HIPCHK(hipGraphInstantiate(&P->exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    }
+    HIPCHK(hipGraphLaunch(P->exec, s), "graph launch");
+  } else {
+    if ((st = EnqueueAdj<T>(m, *P)) < 0) return st;
+  }
+  HIPCHK(hipStreamSynchronize(s), "adjoint pass");
+  std::memcpy(out, P->pinnedOut, sizeof(double) * P->adjWidth);
+  if (m->timing) HarvestTimes(m);
+  return IFG_OK;
+}
+
+// Per (component, slot) sums -> gradient w.r.t. the parameter vector (X0 / Y0 of a
+// function set collect the sums of all its components, in component order).
+void SlotsToParams(const ifg_model* m, const std::vector<double>& slots, double* grad) {
+  std::fill(grad, grad + m->nParamsTot, 0.0);
+  int off = 0, xy = 0;
+  for (int n = 0; n < m->nComp(); n++) {
+    if (m->fsetStart[n]) {
+      xy = off;
+      off += 2;
+    }
+    const double* sl = slots.data() + (size_t)n * IFG_SLOTS;
+    grad[xy] += sl[0];
+    grad[xy + 1] += sl[1];
+    const int np = NParamsOf(m->types[n]);
+    for (int k = 0; k < np; k++) grad[off + k] += sl[2 + k];
+    off += np;
+  }
+}
+
+// kind 0: gradient of the fit statistic (stat: the statistic); 1: deviates' VJP with
+// weights u (fvec: the deviates); 2: model image's VJP with weights u.
+template <typename T>
+int RunAdjoint(ifg_model* m, const double* params, int kind, const double* u, double* grad, double* stat,
+               double* fvec) {
+  int st;
+  int flags = 0;
+  if (kind == 0 || kind == 1) {
+    double s0 = 0.0;
+    const int fwd = PF_DEVKEEP | (kind == 0 ? PF_DFAC2 : PF_DFAC1);
+    if ((st = Evaluate<T>(m, params, 1, &s0, fvec, nullptr, fwd)) < 0) return st;
+    if (stat) *stat = s0;
+    flags |= PF_ADJ_DFAC;
+  }
+  if (kind >= 1) flags |= PF_ADJ_U;
+  std::vector<double> slots((size_t)m->nComp() * IFG_SLOTS);
+  if ((st = EvaluateAdj<T>(m, params, flags, u, slots.data())) < 0) return st;
+  SlotsToParams(m, slots, grad);
+  return IFG_OK;
+}
+
+int RunAdjointDispatch(ifg_model* m, const double* params, int kind, const double* u, double* grad, double* stat,
+                       double* fvec) {
+  if (kind != 2 && !m->dataSet) return m->fail(IFG_ERR_STATE, "no data image");
+  if (!params || !grad) return m->fail(IFG_ERR_ARG, "null parameters or gradient");
+  if (kind != 0 && !u) return m->fail(IFG_ERR_ARG, "null output weights");
+  HIPCHK(hipSetDevice(m->device), "hipSetDevice");
+  (void)hipGetLastError();
+  m->lastParams.assign(params, params + m->nParamsTot);
+  m->modelValid = false;
+  if (m->precision == IFG_PRECISION_FP64) return RunAdjoint<double>(m, params, kind, u, grad, stat, fvec);
+  return RunAdjoint<float>(m, params, kind, u, grad, stat, fvec);
 }
 
 // Normal equations of a forward-difference Jacobian: the point x and the nFree
@@ -2183,11 +2491,36 @@ int ifg_normal_equations(ifg_model* m, const double* params, int n_free, const i
 
 int ifg_fit_statistic_gradient(ifg_model* m, const double* params, double* grad, double* stat) try {
   if (!m) return IFG_ERR_ARG;
+  if (!grad) return m->fail(IFG_ERR_ARG, "null gradient");
+  if (m->gradMethod == IFG_GRADIENT_ADJOINT) return RunAdjointDispatch(m, params, 0, nullptr, grad, stat, nullptr);
   std::vector<int> fr;
   int st = CheckFree(m, m->nParamsTot, nullptr, fr);
   if (st < 0) return st;
-  if (!grad) return m->fail(IFG_ERR_ARG, "null gradient");
   return RunDerivDispatch(m, params, fr, PF_GRAD, stat, nullptr, nullptr, nullptr, grad);
+} catch (...) {
+  return ExceptionStatus(m);
+}
+
+int ifg_set_gradient_method(ifg_model* m, int method) try {
+  if (!m) return IFG_ERR_ARG;
+  if (method != IFG_GRADIENT_ADJOINT && method != IFG_GRADIENT_TANGENT)
+    return m->fail(IFG_ERR_ARG, "unknown gradient method");
+  m->gradMethod = method;
+  return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
+}
+
+int ifg_deviates_vjp(ifg_model* m, const double* params, const double* u, double* grad, double* fvec) try {
+  if (!m) return IFG_ERR_ARG;
+  return RunAdjointDispatch(m, params, 1, u, grad, nullptr, fvec);
+} catch (...) {
+  return ExceptionStatus(m);
+}
+
+int ifg_model_image_vjp(ifg_model* m, const double* params, const double* u, double* grad) try {
+  if (!m) return IFG_ERR_ARG;
+  return RunAdjointDispatch(m, params, 2, u, grad, nullptr, nullptr);
 } catch (...) {
   return ExceptionStatus(m);
 }

@@ -8,6 +8,7 @@
 #define IFG_MAXREG 4     // oversampled regions per model
 #define IFG_MAXSTAGE 16  // FFT stages
 #define IFG_MAXFREE 48   // parameters of a derivative pass
+#define IFG_SLOTS 9      // derivative slots per component: x0, y0 and up to seven own parameters
 
 namespace ifg {
 
@@ -102,7 +103,8 @@ struct TaskStat {
   const void* extra;   // [nData] extra Cash terms (MLR), may be null
   double* devOut;      // [vec * nData] or null (S_JAC: Jacobian columns)
   double* dfacOut;     // [nData] d(deviate)/d(model) or d(statistic)/d(model), or null
-  const double* dfac;  // S_JAC / S_GRAD input factor per data pixel
+  const double* dfac;  // S_JAC / S_GRAD input factor per data pixel; adjoint pass: d(output)/dm
+  const double* uvec;  // adjoint pass: caller's weights of the outputs per data pixel (null: 1)
   double* modelOut;    // [vec * nData] or null
   long nData;
   int W, H;            // data extent
@@ -157,8 +159,19 @@ struct Task {
   int scale, nbc, rowsRep, c0, r0;   // region: data pixels [c0, c0+nbc) x [r0, r0+rowsRep) it replaces
   int subPar;                  // region: sub-rows transformed together in the inverse pass
   int nVec;                    // vectors (planes) of this task in the launch
+  double* aTail;               // adjoint pass: the adjoint value of each sub-pixel box pixel, per plane
+                               // (plane p at aTail + p * ea.tailStride, box layout as Box.off)
   EvalArgs ea;
   TaskStat st;
+};
+
+// Adjoint (reverse-mode) pass: per-slot partial sums of sum_pixels a * d(intensity)/d(slot),
+// `width` = nComp * kSlots doubles per partial slot: the inverse-pass blocks use
+// their partial slots (Task.slot0 + block), the tail-pass blocks slots tailSlot0 + block.
+struct AdjLaunch {
+  double* partials;
+  int width;
+  int tailSlot0;
 };
 
 // Block -> task lookup of one merged launch: with few tasks (one model's grids),

@@ -776,7 +776,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ------------------------------------------------------------------ analytic derivatives
 // Parameter slots of one component: x0, y0 and up to seven own parameters.
-constexpr int kSlots = 9;
+constexpr int kSlots = IFG_SLOTS;
 
 // d(point intensity)/d(parameter) of one component at (x, y).  slot 0 = X0,
 // slot 1 = Y0, slot 2+k = the component's own parameter k in the reference order
@@ -1172,6 +1172,38 @@ __device__ __forceinline__ int block_task(const BlockRanges& br, int bx, int& lo
 // coordinates follow func_sersic.cpp:165-181 exactly (FP64); only the order of the
 // final sum differs (lane partials + xor butterfly, identical in every lane).
 // In derivative mode the wave forms the sub-pixel mean of d(intensity)/dp instead.
+// Slot derivatives of one component at a box pixel (x, y), averaged over the same
+// nSub x nSub sub-pixel samples as its value (the sample count is piecewise constant
+// in the parameters, so the mean of the derivatives is the derivative of the mean);
+// the samples spread over the wave's 64 lanes, the result identical in every lane.
+template <bool DISK>
+__device__ __forceinline__ void tail_grad_mean(int type, const double* v, double x, double y, unsigned need,
+                                               int subsampling, int lane, double (&mean)[kSlots]) {
+  const int nSub = (type == F_FLATSKY)              ? 1
+                   : (DISK && type == F_EDGEONDISK) ? edge_subsamples(v, x, y, subsampling)
+                                                    : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
+  if (nSub > 1) {
+    const double deltaSubpix = 1.0 / nSub;
+    const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
+    const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
+    const int n2 = nSub * nSub;
+    double part[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) part[k] = 0.0;
+    for (int s = lane; s < n2; s += 64) {
+      const int ii = s / nSub, jj = s - ii * nSub;
+      double gs[kSlots];
+      comp_grad_all<DISK>(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, need, gs);
+#pragma unroll
+      for (int k = 0; k < kSlots; k++) part[k] += gs[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) mean[k] = (need & (1u << k)) ? wave_sum(part[k]) / (nSub * nSub) : 0.0;
+  } else {
+    comp_grad_all<DISK>(type, v, x, y, need, mean);
+  }
+}
+
 template <typename T, bool DISK, bool DERIV>
 __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int loc, int lane, int subsampling) {
   const Box* bx = ea.boxes + vec * ea.boxStride;
@@ -1202,33 +1234,9 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
         if (slot >= 0) need |= 1u << slot;
       }
       if (!need) continue;
-      const int type = ea.types[n];
       const CompRec rv = rec[n];   // into registers: read once, not per sample
-      const double* v = rv.v;
-      const int nSub = (type == F_FLATSKY)              ? 1
-                       : (DISK && type == F_EDGEONDISK) ? edge_subsamples(v, x, y, subsampling)
-                                                        : n_subsamples(type, v, comp_radius(type, v, x, y), subsampling);
       double mean[kSlots];
-      if (nSub > 1) {
-        const double deltaSubpix = 1.0 / nSub;
-        const double x_sub_start = x - 0.5 + 0.5 * deltaSubpix;
-        const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
-        const int n2 = nSub * nSub;
-        double part[kSlots];
-#pragma unroll
-        for (int k = 0; k < kSlots; k++) part[k] = 0.0;
-        for (int s = lane; s < n2; s += 64) {
-          const int ii = s / nSub, jj = s - ii * nSub;
-          double gs[kSlots];
-          comp_grad_all<DISK>(type, v, x_sub_start + ii * deltaSubpix, y_sub_start + jj * deltaSubpix, need, gs);
-#pragma unroll
-          for (int k = 0; k < kSlots; k++) part[k] += gs[k];
-        }
-#pragma unroll
-        for (int k = 0; k < kSlots; k++) mean[k] = (need & (1u << k)) ? wave_sum(part[k]) / (nSub * nSub) : 0.0;
-      } else {
-        comp_grad_all<DISK>(type, v, x, y, need, mean);
-      }
+      tail_grad_mean<DISK>(ea.types[n], rv.v, x, y, need, subsampling, lane, mean);
       if (lane == 0)   // components in order, as the sum over components of each plane
         for (int j = 0; j < ea.nFree; j++) {
           const int slot = grad_slot(ea, n, ea.freeIdx[j]);
@@ -1420,7 +1428,9 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
 // blocks on the same XCD (workgroups are spread over the 8 XCDs round-robin by
 // their index; every task's block range starts at a multiple of 8), so each line
 // is fetched into one L2 only.
-template <typename T>
+// ADJ: the adjoint of that map (correlation with the PSF): rows [hb, hb + Hh) in,
+// multiplied by the conjugate spectrum, rows [0, Lr) out.
+template <typename T, bool ADJ>
 __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, BlockRanges br) {
   extern __shared__ char smem_raw[];
   int local;
@@ -1449,14 +1459,15 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
   const bool colOk = col < Nc;
   const int colc = colOk ? col : Nc - 1;
   constexpr int U = 10;
+  const int inLo = ADJ ? g.hb : 0, inHi = ADJ ? g.hb + g.Hh : g.Lr;   // rows holding values
   for (int ib = i0; ib < Nr; ib += U * rstep) {
     cpx<T> v[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) v[u] = gload(S + (long)min(ib + u * rstep, g.Lr - 1) * Nc + colc);
+    for (int u = 0; u < U; u++) v[u] = gload(S + (long)min(max(ib + u * rstep, inLo), inHi - 1) * Nc + colc);
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = ib + u * rstep;
-      if (i < Nr) A[cc * ld + i] = (i < g.Lr && colOk) ? v[u] : cpx<T>{0, 0};
+      if (i < Nr) A[cc * ld + i] = (i >= inLo && i < inHi && colOk) ? v[u] : cpx<T>{0, 0};
     }
   }
   __syncthreads();
@@ -1470,13 +1481,17 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = ib + u * rstep;
-      if (i < Nr && colOk) X[cc * ld + i] = cmul(X[cc * ld + i], k[u]);
+      if (i < Nr && colOk) X[cc * ld + i] = cmul(X[cc * ld + i], ADJ ? cpx<T>{k[u].x, -k[u].y} : k[u]);
     }
   }
   __syncthreads();
   X = fft_lds<T>(X, Y, pl, C, ld, TW, -1);
-  if (colOk)
-    for (int i = i0; i < g.Hh; i += rstep) S[(long)(g.hb + i) * Nc + col] = X[cc * ld + g.hb + i];
+  if (colOk) {
+    if (ADJ)
+      for (int i = i0; i < g.Lr; i += rstep) S[(long)i * Nc + col] = X[cc * ld + i];
+    else
+      for (int i = i0; i < g.Hh; i += rstep) S[(long)(g.hb + i) * Nc + col] = X[cc * ld + g.hb + i];
+  }
 }
 
 // Setup-time transforms of the PSF spectrum: in-place row FFTs and column FFTs.
@@ -1984,6 +1999,335 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
   if (sl.partials) finish_block(sl.partials, Tk.slot0 + local, ea, eb);
 }
 
+// ------------------------------------------------------------------ adjoint (reverse-mode) pass
+// The gradient of a scalar sum_z u_z F_z(m_z) of the data-pixel model values m, or a
+// vector-Jacobian product, without one derivative plane per parameter.  With the
+// per-data-pixel factor f_z = dF/dm (the forward pass leaves it in TaskStat.dfac)
+// times the caller's weight u_z, the model's linear stages are run backwards once:
+//   * the s x s block mean of a region (core/downsample.cpp:84-106) spreads f / s^2
+//     over the block's sub-pixels; main-grid pixels that a region replaces get 0
+//     (their convolved value is overwritten, core/downsample.cpp:96-104);
+//   * the convolution's adjoint is the correlation with the PSF: the same FFT
+//     passes with the conjugate spectrum (k_adj_rowfft, k_colconv<T, true>,
+//     k_adj_inverse), giving the adjoint value a of every evaluated grid pixel
+//     (planes of both halves of the image: each is its own linear map, so their
+//     contributions add);
+//   * then sum_pixels a * d(intensity)/d(slot) for every slot of every component,
+//     computed once per pixel from one shared evaluation of the component
+//     (comp_grad_all); sub-pixel box pixels use the mean of the derivatives over
+//     the same samples as their value (k_adj_tail).
+// Regions are recomputed here, not stored from the forward pass.
+
+// Adjoint input of output position (plane, output row io, plane column j) of a task:
+// the weight with which the value there enters the outputs.
+__device__ __forceinline__ double adj_input(const Task& Tk, int4 pr, int plane, int io, int j) {
+  const TaskStat& st = Tk.st;
+  const GridGeom& g = Tk.g;
+  const int oc = j - g.hl;
+  if (oc < 0 || oc >= g.W || io < 0 || io >= g.Hh) return 0.0;
+  if (plane == 1 && pr.z < 0) return 0.0;
+  const int orow = (plane ? pr.w : pr.y) + io;
+  int r, c;
+  double f = 1.0;
+  if (Tk.kind == TK_REGION) {
+    const int s = Tk.scale;
+    const int bi = orow / s, bj = oc / s;
+    if (bi * s + s > g.H || bi >= Tk.rowsRep || bj >= Tk.nbc) return 0.0;
+    r = Tk.r0 + bi;
+    c = Tk.c0 + bj;
+    f = 1.0 / (double)(s * s);
+  } else {
+    if (orow >= st.H) return 0.0;
+    r = orow;
+    c = oc;
+    for (int k = 0; k < st.nReg; k++) {
+      const RegionOut& R = st.regs[k];
+      if ((unsigned)(c - R.c0) < (unsigned)R.w && (unsigned)(r - R.r0) < (unsigned)R.h) return 0.0;
+    }
+  }
+  const long z = (long)r * st.W + c;
+  if (st.dfac) f *= st.dfac[z];
+  if (st.uvec) f *= st.uvec[z];
+  return f;
+}
+
+// Adjoint inputs of the output rows of both planes, placed at plane rows
+// [hb, hb + Hh) and columns [hl, hl + W), then the forward row FFT (rows1 rows per
+// block, Hh / rows1 blocks per pair).
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
+  extern __shared__ char smem_raw[];
+  int local;
+  const int t = block_task(br, blockIdx.x, local);
+  if (t < 0) return;
+  const Task& Tk = tasks[t];
+  const int pi = local / Tk.n1, lb = local - pi * Tk.n1;
+  const GridGeom g = Tk.g;
+  const int rows = Tk.rows1;
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + rows * g.Nc;
+  cpx<T>* TW = B + rows * g.Nc;
+  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  const int4 pr = Tk.pairs[pi];
+  const int io0 = lb * rows;
+  const int total = rows * g.Nc;
+  for (int i = threadIdx.x; i < total; i += kThreads) {
+    const int rr = i / g.Nc, j = i - rr * g.Nc;
+    A[i] = cpx<T>{(T)adj_input(Tk, pr, 0, io0 + rr, j), (T)adj_input(Tk, pr, 1, io0 + rr, j)};
+  }
+  __syncthreads();
+  cpx<T>* out = fft_lds<T>(A, B, Tk.pc, rows, g.Nc, TW, +1);
+  cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
+  for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++)
+    for (int j = threadIdx.x; j < g.Nc; j += kThreads) S[(long)(g.hb + io0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
+}
+
+// Block sum of one component's kSlots accumulators into dst[0 .. kSlots) (thread 0's
+// wave order, deterministic); every thread must call it.
+__device__ __forceinline__ void block_sum_slots(double (&acc)[kSlots], double* dst) {
+  __shared__ double red[kSlots][kThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kSlots; k++) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[k][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kSlots) {
+    double v = red[threadIdx.x][0];
+    for (int w = 1; w < kThreads / 64; w++) v += red[threadIdx.x][w];
+    dst[threadIdx.x] = v;
+  }
+  __syncthreads();   // red is reused by the next call
+}
+
+// Slots of component n that have a parameter: x0, y0 and its own parameters.
+__device__ __forceinline__ unsigned comp_slots(const EvalArgs& ea, int n) {
+  return ea.types[n] == F_FLATSKY ? 4u : ((1u << (2 + ea.compNP[n])) - 1u);
+}
+
+// sum over the block's items of a * d(intensity)/d(slot), per component, into the
+// block's partial slot.  item(e, x, y, a) gives item e's grid position and adjoint value.
+template <bool DISK, typename Item>
+__device__ __forceinline__ void adj_accumulate(const EvalArgs& ea, const CompRec* rec, int nItems, Item item,
+                                               double* dst) {
+  for (int n = 0; n < ea.nComp; n++) {
+    const unsigned need = comp_slots(ea, n);
+    const int type = ea.types[n];
+    double acc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) acc[k] = 0.0;
+    for (int e = threadIdx.x; e < nItems; e += kThreads) {
+      double x, y, a;
+      item(e, x, y, a);
+      if (a == 0.0) continue;
+      double gs[kSlots];
+      comp_grad_all<DISK>(type, rec[n].v, x, y, need, gs);
+#pragma unroll
+      for (int k = 0; k < kSlots; k++) acc[k] += a * gs[k];
+    }
+    block_sum_slots(acc, dst + n * kSlots);
+  }
+}
+
+// Whether grid pixel (c, r) lies in one of the boxes; its offset in the box layout.
+__device__ __forceinline__ bool in_box(const Box* bx, int c, int r, int& off) {
+#pragma unroll
+  for (int b = 0; b < IFG_MAXBOX; b++) {
+    const Box B = bx[b];
+    const int dr = r - B.r0, dc = c - B.c0;
+    if (B.w > 0 && (unsigned)dr < (unsigned)B.h && (unsigned)dc < (unsigned)B.w) {
+      off = B.off + dr * B.w + dc;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Inverse row FFT of plane rows [ir0, ir0 + rows3) of the correlated planes (the
+// adjoint values of the evaluated grid pixels, both planes), or for a grid without
+// PSF the adjoint inputs themselves (pixel runs of the band, as k_inverse_stat's);
+// box pixels hand their value to k_adj_tail (Task.aTail) and take no part here.
+template <typename T, bool DISK>
+__global__ __launch_bounds__(256) void k_adj_inverse(const Task* __restrict__ tasks, BlockRanges br, AdjLaunch al) {
+  extern __shared__ char smem_raw[];
+  int local;
+  const int t = block_task(br, blockIdx.x, local);
+  if (t < 0) return;
+  const Task& Tk = tasks[t];
+  const GridGeom g = Tk.g;
+  const int pi = local / Tk.n3, lb = local - pi * Tk.n3;
+  const int4 pr = Tk.pairs[pi];
+  __shared__ CompRec sRec[2][IFG_MAXCOMP];
+  __shared__ Box sBox[2][IFG_MAXBOX];
+  stage_pair<false>(Tk, pr, sRec, sBox);
+  double* dst = al.partials + (long)(Tk.slot0 + local) * al.width;
+  const long ts = Tk.ea.tailStride;
+  if (Tk.kind == TK_MAIN_PLAIN) {
+    double* aL = reinterpret_cast<double*>(smem_raw);   // [plane][pixel of the run]
+    const int run = Tk.rows3;
+    const long band = (long)g.Hh * g.W, p0 = (long)lb * run;
+    const int npx = (int)min((long)run, band - p0);
+    for (int e = threadIdx.x; e < npx; e += kThreads) {
+      const long q = p0 + e;
+      const int r = (int)(q / g.W), c = (int)(q - (long)r * g.W);
+#pragma unroll
+      for (int pl = 0; pl < 2; pl++) {
+        double a = adj_input(Tk, pr, pl, r, c + g.hl);
+        int off;
+        if (in_box(sBox[pl], c, (pl ? pr.w : pr.y) + r, off)) {
+          Tk.aTail[pl * ts + off] = a;
+          a = 0.0;
+        }
+        aL[pl * run + e] = a;
+      }
+    }
+    __syncthreads();
+    adj_accumulate<DISK>(
+        Tk.ea, sRec[0], 2 * npx,
+        [&](int e, double& x, double& y, double& a) {
+          const int pl = e >= npx, k = e - pl * npx;
+          const long q = p0 + k;
+          const int r = (int)(q / g.W), c = (int)(q - (long)r * g.W);
+          x = g.xa + (double)c * g.frac;
+          y = g.ya + (double)((pl ? pr.w : pr.y) + r) * g.frac;
+          a = aL[pl * run + k];
+        },
+        dst);
+    return;
+  }
+  const int rows = Tk.rows3;
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + rows * g.Nc;
+  cpx<T>* TW = B + rows * g.Nc;
+  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  const int ir0 = lb * rows;
+  const int validRows = max(0, min(rows, g.Lr - ir0));
+  const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
+  lds_fill<8>(A, S + (long)ir0 * g.Nc, rows * g.Nc, validRows * g.Nc);
+  __syncthreads();
+  cpx<T>* out = fft_lds<T>(A, B, Tk.pc, rows, g.Nc, TW, -1);
+  // grid pixel of plane entry (row ir, column j): column j - hl, row rowoff + ir - hb
+  const int nItems = validRows * g.Lc;
+  for (int e = threadIdx.x; e < nItems; e += kThreads) {
+    const int rr = e / g.Lc, j = e - rr * g.Lc, ir = ir0 + rr;
+    cpx<T>& v = out[rr * g.Nc + j];
+    int off;
+    if (in_box(sBox[0], j - g.hl, pr.y + ir - g.hb, off)) {
+      Tk.aTail[off] = (double)v.x;
+      v.x = 0;
+    }
+    if (pr.z >= 0 && in_box(sBox[1], j - g.hl, pr.w + ir - g.hb, off)) {
+      Tk.aTail[ts + off] = (double)v.y;
+      v.y = 0;
+    }
+  }
+  __syncthreads();
+  adj_accumulate<DISK>(
+      Tk.ea, sRec[0], 2 * nItems,
+      [&](int e, double& x, double& y, double& a) {
+        const int pl = e >= nItems, k = e - pl * nItems;
+        const int rr = k / g.Lc, j = k - rr * g.Lc;
+        const cpx<T> v = out[rr * g.Nc + j];
+        x = g.xa + (double)(j - g.hl) * g.frac;
+        y = g.ya + (double)((pl ? pr.w : pr.y) + ir0 + rr - g.hb) * g.frac;
+        a = pl ? (pr.z >= 0 ? (double)v.y : 0.0) : (double)v.x;
+      },
+      dst);
+}
+
+// Box pixels: one wave per pixel (grid-stride over the tasks' box pixels, as
+// k_tail), the adjoint value of the pixel (the sum over the planes that evaluate
+// it) times each component's sub-pixel mean of d(intensity)/d(slot); a pixel that
+// several boxes hold counts in the first one only.  Lane L keeps
+// the sums of slots L, L + 64, L + 128 of the (component, slot) list; the block's
+// waves are added in order into the block's partial slot.
+template <bool DISK>
+__global__ __launch_bounds__(256) void k_adj_tail(const Task* __restrict__ tasks, int nTasks,
+                                                  const long* __restrict__ prefix, int subsampling, AdjLaunch al) {
+  __shared__ double red[kThreads / 64][3 * 64];
+  const int lane = threadIdx.x & 63;
+  const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  double acc[3] = {0.0, 0.0, 0.0};
+  const long total = prefix[nTasks];
+  const long nWaves = (long)gridDim.x * (kThreads >> 6);
+  for (long wave = (long)blockIdx.x * (kThreads >> 6) + wib; wave < total; wave += nWaves) {
+    int lo = 0, hi = nTasks - 1;   // last t with prefix[t] <= wave
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= wave) lo = mid;
+      else hi = mid - 1;
+    }
+    const Task& Tk = tasks[lo];
+    const EvalArgs& ea = Tk.ea;
+    const GridGeom& g = Tk.g;
+    const int loc = (int)(wave - prefix[lo]);
+    const int4 pr = Tk.pairs[0];
+    int c = 0, r = 0;
+    bool found = false;
+    for (int b = 0; b < IFG_MAXBOX; b++) {
+      const Box B = ea.boxes[b];
+      if (B.w > 0 && loc >= B.off && loc < B.off + B.w * B.h) {
+        c = B.c0 + (loc - B.off) % B.w;
+        r = B.r0 + (loc - B.off) / B.w;
+        found = true;
+        break;
+      }
+    }
+    if (!found) continue;
+    // boxes may overlap: a pixel counts once, in the first box that holds it (the
+    // entry k_adj_inverse wrote)
+    int first;
+    if (!in_box(ea.boxes, c, r, first) || first != loc) continue;
+    // the planes that evaluate grid row r: rows [rowoff - lo, rowoff - lo + span)
+    const bool plain = Tk.kind == TK_MAIN_PLAIN;
+    const int rlo = plain ? 0 : g.hb, span = plain ? g.Hh : g.Lr;
+    double a = 0.0;
+    if ((unsigned)(r - pr.y + rlo) < (unsigned)span) a += Tk.aTail[loc];
+    if (pr.z >= 0 && (unsigned)(r - pr.w + rlo) < (unsigned)span) a += Tk.aTail[(long)ea.tailStride + loc];
+    if (a == 0.0) continue;
+    const double x = g.xa + (double)c * g.frac, y = g.ya + (double)r * g.frac;
+    for (int n = 0; n < ea.nComp; n++) {
+      const CompRec rv = ea.recs[n];
+      double mean[kSlots];
+      tail_grad_mean<DISK>(ea.types[n], rv.v, x, y, comp_slots(ea, n), subsampling, lane, mean);
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int idx = lane + 64 * q - n * kSlots;
+        if (idx >= 0 && idx < kSlots) acc[q] += a * pick_slot(mean, idx);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++) red[wib][lane + 64 * q] = acc[q];
+  __syncthreads();
+  double* dst = al.partials + (long)(al.tailSlot0 + blockIdx.x) * al.width;
+  for (int i = threadIdx.x; i < al.width; i += kThreads) {
+    double v = red[0][i];
+    for (int w = 1; w < kThreads / 64; w++) v += red[w][i];
+    dst[i] = v;
+  }
+}
+
+// out[i] = sum over the partial slots [0, nSlots) of entry i (one block per entry,
+// slots strided over the threads, then the block's fixed-order reduction).
+__global__ __launch_bounds__(256) void k_adj_final(const double* __restrict__ partials, int nSlots, int width,
+                                                   double* __restrict__ out, int hostOut) {
+  const int i = blockIdx.x;
+  double v = 0.0;
+  for (int s = threadIdx.x; s < nSlots; s += kThreads) v += partials[(long)s * width + i];
+  __shared__ double red[kThreads / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = red[0];
+    for (int w = 1; w < kThreads / 64; w++) sum += red[w];
+    out[i] = sum;
+    if (hostOut) __threadfence_system();
+  }
+}
+
 // ------------------------------------------------------------------ call staging
 // The call's staging (component records, boxes, prefixes: a few KB for one
 // evaluation) read straight from the mapped pinned host buffer into device memory
@@ -2131,7 +2475,7 @@ hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRange
 template <typename T>
 hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds) {
   if (nBlocks <= 0) return hipSuccess;
-  IFG_LAUNCH(true, true, k_colconv<T>, dim3(nBlocks), dim3(256), lds, s, tasks, br);
+  IFG_LAUNCH(true, true, (k_colconv<T, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br);
   return hipGetLastError();
 }
 
@@ -2147,6 +2491,30 @@ hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRang
   } else {   // convolved grids only: the profile code (the one user of DISK and DERIV) is compiled out
     IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
   }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_adjoint(hipStream_t s, const Task* tasks, int nTasks, const BlockRanges& br1, int nB1, size_t lds1,
+                          const BlockRanges& br2, int nB2, size_t lds2, const BlockRanges& br3, int nB3, size_t lds3,
+                          const long* prefix, long tailBlocks, int sub, bool disk, const AdjLaunch& al, int nSlots,
+                          double* out, int hostOut) {
+  if (nB1 > 0) IFG_LAUNCH(true, false, k_adj_rowfft<T>, dim3(nB1), dim3(256), lds1, s, tasks, br1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (nB2 > 0) IFG_LAUNCH(nB1 <= 0, false, (k_colconv<T, true>), dim3(nB2), dim3(256), lds2, s, tasks, br2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (nB3 > 0) {
+    if (disk) IFG_LAUNCH(nB1 <= 0, false, (k_adj_inverse<T, true>), dim3(nB3), dim3(256), lds3, s, tasks, br3, al);
+    else IFG_LAUNCH(nB1 <= 0, false, (k_adj_inverse<T, false>), dim3(nB3), dim3(256), lds3, s, tasks, br3, al);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (tailBlocks > 0) {
+    if (disk) IFG_LAUNCH(false, false, k_adj_tail<true>, dim3((unsigned)tailBlocks), dim3(256), 0, s, tasks, nTasks, prefix, sub, al);
+    else IFG_LAUNCH(false, false, k_adj_tail<false>, dim3((unsigned)tailBlocks), dim3(256), 0, s, tasks, nTasks, prefix, sub, al);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  IFG_LAUNCH(false, true, k_adj_final, dim3(al.width), dim3(256), 0, s, al.partials, nSlots, al.width, out, hostOut);
   return hipGetLastError();
 }
 
@@ -2213,6 +2581,9 @@ hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobLis
                                              const StatLaunch&,                                                    \
                                              bool, bool, bool);                                                    \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
+  template hipError_t launch_adjoint<T>(hipStream_t, const Task*, int, const BlockRanges&, int, size_t,             \
+                                        const BlockRanges&, int, size_t, const BlockRanges&, int, size_t,           \
+                                        const long*, long, int, bool, const AdjLaunch&, int, double*, int);         \
   template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);
 IFG_INST(double)
 IFG_INST(float)

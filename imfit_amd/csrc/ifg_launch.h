@@ -28,6 +28,16 @@ template <typename T>
 // plain: some task of the launch is a no-PSF main grid (evaluated in this pass)
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
                                const StatLaunch& sl, bool disk, bool deriv, bool plain);
+// Adjoint pass of one single-image plan (see k_adj_rowfft .. k_adj_final): row pass
+// (nB1 blocks over the convolved grids), column pass, inverse pass (nB3 blocks over
+// every grid, partial slot per block), tail pass (tailBlocks blocks, partial slots
+// al.tailSlot0 ..), then out[al.width] = the sums over all nSlots partial slots;
+// hostOut: out is mapped host memory.
+template <typename T>
+hipError_t launch_adjoint(hipStream_t s, const Task* tasks, int nTasks, const BlockRanges& br1, int nB1, size_t lds1,
+                          const BlockRanges& br2, int nB2, size_t lds2, const BlockRanges& br3, int nB3, size_t lds3,
+                          const long* prefix, long tailBlocks, int sub, bool disk, const AdjLaunch& al, int nSlots,
+                          double* out, int hostOut);
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
                          int dir, int rows);

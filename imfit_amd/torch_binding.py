@@ -1,19 +1,25 @@
 """torch.autograd bindings of the GPU model (thin wrappers over the C ABI).
 
-`fit_statistic(model, params)` and `deviates(model, params)` are differentiable
-functions of a 1-D parameter tensor, for use with torch optimisers or inside
-larger autograd graphs:
+`fit_statistic(model, params)`, `deviates(model, params)` and
+`model_image(model, params)` are differentiable functions of a 1-D parameter
+tensor, for use with torch optimisers or inside larger autograd graphs:
 
     * fit_statistic: GetFitStatistic (chi^2, Cash or Poisson-MLR) in the forward
-      pass; the backward pass uses the exact gradient that the library computes in
-      the same call (ifg_fit_statistic_gradient);
+      pass; when the parameters require a gradient, the same call also returns the
+      exact gradient from the library's reverse-mode (adjoint) pass
+      (ifg_fit_statistic_gradient), which the backward pass scales;
     * deviates: ComputeDeviates in the forward pass; the backward pass is the
-      vector-Jacobian product J^T g with the exact Jacobian (ifg_jacobian).
+      vector-Jacobian product J^T g computed on the GPU by the adjoint pass
+      (ifg_deviates_vjp), without forming J;
+    * model_image: CreateModelImage (the nData-pixel image) in the forward pass;
+      the backward pass is the adjoint pass with the incoming gradient as the
+      output weights (ifg_model_image_vjp).
 
 The library evaluates on its own HIP stream from host parameter vectors, so the
-parameters are copied to the host (a synchronising .cpu()); outputs are returned on
-the parameters' device and dtype.  `model` is an imfit_amd.ModelObjectGPU after
-FinalSetupForFitting.
+parameters (and incoming gradients) are copied to the host (a synchronising
+.cpu()); outputs are returned on the parameters' device and dtype.  `model` is an
+imfit_amd.ModelObjectGPU after FinalSetupForFitting (model_image: after
+SetupModelImage or AddImageDataVector).
 """
 import numpy as np
 import torch
@@ -25,11 +31,19 @@ def _host(params):
     return np.ascontiguousarray(params.detach().to("cpu", torch.float64).numpy())
 
 
+def _host_grad(g):
+    return np.ascontiguousarray(g.detach().to("cpu", torch.float64).reshape(-1).numpy())
+
+
 class _FitStatistic(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, model):
-        grad, stat = model.GetFitStatisticGradient(_host(params))
-        ctx.save_for_backward(torch.from_numpy(grad).to(params.device, params.dtype))
+        p = _host(params)
+        if ctx.needs_input_grad[0]:
+            grad, stat = model.GetFitStatisticGradient(p)
+            ctx.save_for_backward(torch.from_numpy(grad).to(params.device, params.dtype))
+        else:
+            stat = model.GetFitStatistic(p)
         return params.new_tensor(stat)
 
     @staticmethod
@@ -41,14 +55,28 @@ class _FitStatistic(torch.autograd.Function):
 class _Deviates(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, model):
-        fvec, J = model.Jacobian(_host(params))
-        ctx.save_for_backward(torch.from_numpy(J).to(params.device, params.dtype))
-        return torch.from_numpy(fvec).to(params.device, params.dtype)
+        p = _host(params)
+        ctx.model, ctx.p = model, p
+        return torch.from_numpy(model.ComputeDeviates(p)).to(params.device, params.dtype)
 
     @staticmethod
     def backward(ctx, g):
-        (J,) = ctx.saved_tensors
-        return J.t() @ g, None
+        v, _ = ctx.model.DeviatesVJP(ctx.p, _host_grad(g))
+        return torch.from_numpy(v).to(g.device, g.dtype), None
+
+
+class _ModelImage(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, model):
+        p = _host(params)
+        ctx.model, ctx.p = model, p
+        model.CreateModelImage(p)
+        return torch.from_numpy(model.GetModelImageVector()).to(params.device, params.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        v = ctx.model.ModelImageVJP(ctx.p, _host_grad(g))
+        return torch.from_numpy(v).to(g.device, g.dtype), None
 
 
 def fit_statistic(model, params):
@@ -59,3 +87,8 @@ def fit_statistic(model, params):
 def deviates(model, params):
     """Differentiable deviates (nData,) of `model` at `params`."""
     return _Deviates.apply(params, model)
+
+
+def model_image(model, params):
+    """Differentiable model image (nRows, nColumns) of `model` at `params`."""
+    return _ModelImage.apply(params, model)

@@ -186,8 +186,29 @@ int ifg_jacobian(ifg_model* m, const double* params, int n_free, const int* ifre
  * fit statistic at params. */
 int ifg_normal_equations(ifg_model* m, const double* params, int n_free, const int* ifree, double* jtj, double* jtr,
                          double* stat);
-/* Gradient of GetFitStatistic w.r.t. all nParams parameters (and the statistic). */
+/* Gradient of GetFitStatistic w.r.t. all nParams parameters (and the statistic).
+ * Default method: reverse mode (adjoint) -- one forward evaluation that keeps
+ * d(statistic)/d(model) per data pixel, then one pass backwards through the model:
+ * the oversampled regions' block means spread over their sub-pixels, the PSF
+ * convolution's adjoint (correlation with the PSF: the FFT passes with the
+ * conjugate PSF spectrum, core/convolver.cpp:372-379 transposed), main-grid pixels
+ * that a region replaces contributing nothing (core/downsample.cpp:96-104), and the
+ * analytic derivatives of every component summed against the result.  Its cost does
+ * not grow with the number of parameters. */
 int ifg_fit_statistic_gradient(ifg_model* m, const double* params, double* grad, double* stat);
+/* Method of ifg_fit_statistic_gradient: IFG_GRADIENT_ADJOINT (default) or
+ * IFG_GRADIENT_TANGENT (one derivative plane per parameter through the whole
+ * forward path, as ifg_jacobian).  Both are exact; they differ in rounding only. */
+#define IFG_GRADIENT_ADJOINT 0
+#define IFG_GRADIENT_TANGENT 1
+int ifg_set_gradient_method(ifg_model* m, int method);
+/* Vector-Jacobian product of ComputeDeviates: grad[k] = sum_i u[i] d(deviate_i)/d(param k)
+ * over the nData deviates, by the adjoint pass; fvec (nData, may be NULL) gets the
+ * deviates.  For torch's backward of the deviates (J^T u without forming J). */
+int ifg_deviates_vjp(ifg_model* m, const double* params, const double* u, double* grad, double* fvec);
+/* Vector-Jacobian product of CreateModelImage (the nData-pixel model image that
+ * ifg_get_model_image returns): grad[k] = sum_i u[i] d(model_i)/d(param k). */
+int ifg_model_image_vjp(ifg_model* m, const double* params, const double* u, double* grad);
 
 /* ---- Levenberg-Marquardt callers ----------------------------------------------
  * Parameter constraints: the fixed / limited / limits fields of mpfit's mp_par
