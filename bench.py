@@ -222,6 +222,42 @@ def lm_benchmark(w, repeats):
             "start": "truth x U(0.9,1.1), centre +-5 px (seed 1)"}
 
 
+def gradient_benchmark(models, p, results, steps):
+    """Exact gradient of the statistic on the 1024^2 workload, per precision: the
+    reverse-mode (adjoint) pass against the tangent method (one derivative plane per
+    parameter through the forward path), each as ms per gradient and in units of one
+    evaluation (the headline's ms per step), with their largest difference relative to
+    the gradient's largest entry."""
+    import imfit_amd
+    out = {}
+    reps = max(5, steps // 10)
+    for key, m in models.items():
+        res, grads = {}, {}
+        for method, name in ((imfit_amd.GRADIENT_ADJOINT, "adjoint"), (imfit_amd.GRADIENT_TANGENT, "tangent")):
+            m.SetGradientMethod(method)
+            m.GetFitStatisticGradient(p)   # plans and graphs (untimed)
+            device_sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g, _ = m.GetFitStatisticGradient(p)
+            device_sync()
+            dt = (time.perf_counter() - t0) / reps
+            grads[name] = g
+            res[name] = {"ms_per_gradient": 1e3 * dt, "evaluations": dt / (1e-3 * results[key]["ms_per_step"])}
+        m.SetGradientMethod(imfit_amd.GRADIENT_ADJOINT)
+        kt, _ = kernel_times(m, p, 3, 1, call=lambda: m.GetFitStatisticGradient(p))
+        res["adjoint"]["kernels_us"] = {k: 1e3 * t / n for k, (t, n) in kt.items()}
+        res["adjoint_vs_tangent_max_rel"] = float(np.abs(grads["adjoint"] - grads["tangent"]).max() /
+                                                  np.abs(grads["tangent"]).max())
+        res["speedup_adjoint_vs_tangent"] = res["tangent"]["ms_per_gradient"] / res["adjoint"]["ms_per_gradient"]
+        res["reps"] = reps
+        out[key] = res
+        log("gradient %s: adjoint %.3f ms (%.2f evaluations), tangent %.3f ms (%.2f evaluations), max rel diff %.2e" %
+            (key, res["adjoint"]["ms_per_gradient"], res["adjoint"]["evaluations"], res["tangent"]["ms_per_gradient"],
+             res["tangent"]["evaluations"], res["adjoint_vs_tangent_max_rel"]))
+    return out
+
+
 def valu_roofline(kt, prec, profile, variants):
     """VALU-bound workloads: the kernel with the most event time, its average duration
     and its VALU issue fraction (4 cycles per wave64 instruction on one of the 1024
@@ -502,6 +538,7 @@ def main():
     roof = roofline(w, head["kt"], args.precision)
     side_roof = roofline(w, side["kt"], other)
 
+    grad = gradient_benchmark(models, p, results, args.steps)
     lm = None if args.no_lm else lm_benchmark(w, args.lm_repeats)
     aux = None if args.no_aux else aux_benchmarks(args.steps)
 
@@ -528,7 +565,7 @@ def main():
                               "precision": "fp32 pixels and FFT, FP64 deviate reduction" if other == "fp32"
                               else "fp64 pixels, FFT and reduction"},
                       "fp32_vs_fp64_stat_rel": rel_fp32_fp64},
-           "roofline": roof, "cpu_baseline": cpu, "lm_fits": lm, "other_configs": aux}
+           "roofline": roof, "cpu_baseline": cpu, "gradient": grad, "lm_fits": lm, "other_configs": aux}
     print(json.dumps(out))
     if args.copy_calibration_mb > 0:
         copy_calibration(args.copy_calibration_mb)
