@@ -88,6 +88,14 @@ def lib():
         L.orc_expdisk3d_qags.restype = C.c_double
         L.orc_qags_test.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.POINTER(C.c_int)]
         L.orc_qags_test.restype = C.c_double
+        L.orc_set_psf_interpolation.argtypes = [vp, C.c_int]
+        L.orc_set_psf_interpolation.restype = C.c_int
+        L.orc_psf_interp_value.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.orc_psf_interp_value.restype = C.c_double
+        L.orc_lanczos.argtypes = [C.c_double, C.c_int]
+        L.orc_lanczos.restype = C.c_double
+        L.orc_spline_knot_derivatives.argtypes = [_dp, C.c_int, _dp]
+        L.orc_spline_knot_derivatives.restype = None
         _LIB = L
     return _LIB
 
@@ -171,6 +179,11 @@ class OracleModel:
         psf = np.ascontiguousarray(psf, dtype=np.float64)
         return self._chk(self._L.orc_add_oversampled_psf(self.h, _ptr(psf), psf.shape[1], psf.shape[0], scale,
                                                          x1, x2, y1, y2, int(normalize)), "AddOversampledPsf")
+
+    def SetupPsfInterpolation(self, method=1):
+        """PointSource interpolation: 1 bicubic, 2 lanczos2, 3 lanczos3 (status as the
+        reference's: -1 no PSF, -2 PSF smaller than 4 x 4)."""
+        return self._L.orc_set_psf_interpolation(self.h, int(method))
 
     def UseCashStatistic(self):
         self._L.orc_use_cash_statistic(self.h)
@@ -260,3 +273,21 @@ def qags_test(which, a, b, epsrel=1e-6):
     nev = C.c_int(0)
     v = lib().orc_qags_test(which, a, b, epsrel, _ptr(err), C.byref(nev))
     return v, float(err[0]), nev.value
+
+
+def psf_interp_value(method, psf, x, y):
+    """PsfInterpolator value at offset (x, y) from the PSF centre (1 bicubic, 2 lanczos2, 3 lanczos3)."""
+    p = np.ascontiguousarray(psf, dtype=np.float64)
+    return lib().orc_psf_interp_value(int(method), _ptr(p), p.shape[1], p.shape[0], float(x), float(y))
+
+
+def lanczos(x, n):
+    return lib().orc_lanczos(float(x), int(n))
+
+
+def spline_knot_derivatives(y):
+    """Knot derivatives of the natural cubic spline through y at unit spacing."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    d = np.empty_like(y)
+    lib().orc_spline_knot_derivatives(_ptr(y), y.size, _ptr(d))
+    return d

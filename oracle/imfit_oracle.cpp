@@ -393,6 +393,163 @@ struct FlatSky : Func {   // function_objects/func_flatsky.cpp:85-98
   double GetValue(double, double) override { return I_sky; }
 };
 
+// ------------------------------------------------------------------ PSF interpolation
+// PsfInterpolator_bicubic / _lanczos2 / _lanczos3 (function_objects/psf_interpolators.cpp)
+// and PointSource / PointSourceRot (function_objects/func_pointsource.cpp,
+// func_pointsource-rot.cpp).  Coordinates are offsets from the PSF centre; the knots
+// of the PSF image sit at xArray[i] = i - (nCols - 1)/2 (psf_interpolators.cpp:50-57).
+const int INTERP_BICUBIC = 1, INTERP_LANCZOS2 = 2, INTERP_LANCZOS3 = 3;   // psf_interpolators.h:8-10
+const double PI_TRUNC = 3.14159265358979;          // psf_interpolators.cpp:30 (as the reference writes it)
+const double PI_SQUARED_TRUNC = 9.86960440108936;  // psf_interpolators.cpp:31
+
+// Lanczos (psf_interpolators.cpp:296-305): 1 for |x| < 1e-6, 0 beyond n.
+double Lanczos(double x, int n) {
+  double x_abs = fabs(x);
+  if (x_abs < 1.0e-6) return 1.0;
+  if (x_abs > n) return 0.0;
+  return (n * sin(PI_TRUNC * x_abs) * sin(PI_TRUNC * x_abs / n)) / (PI_SQUARED_TRUNC * x_abs * x_abs);
+}
+
+// Knot derivatives of the natural cubic spline through n values at unit spacing: the
+// spline GSL 2.5's gsl_interp2d_bicubic builds along each axis (gsl_interp_cspline,
+// zero second derivative at both ends; psf_interpolators.cpp:62-63 is the call).  With
+// c = y''/2, the interior equations c[i-1] + 4 c[i] + c[i+1] = 3 (y[i+1] - 2 y[i] + y[i-1])
+// are solved by the symmetric tridiagonal LDL^T recursion; the derivative at knot i is
+// the spline's derivative on interval i at its start, b_i = dy - (c[i+1] + 2 c[i]) / 3,
+// and at the last knot that of the last interval at its end, b + 2 c + 3 d.
+void SplineKnotDerivatives(const double* y, long ys, int n, double* d, long ds) {
+  vector<double> c(n, 0.0);
+  const int m = n - 2;   // interior unknowns c[1 .. n-2]
+  if (m == 1) {
+    c[1] = 3.0 * ((y[2 * ys] - y[ys]) - (y[ys] - y[0])) / 4.0;
+  } else if (m > 1) {
+    vector<double> alpha(m), gamma(m), z(m);
+    alpha[0] = 4.0;
+    gamma[0] = 1.0 / alpha[0];
+    for (int i = 1; i < m - 1; i++) {
+      alpha[i] = 4.0 - 1.0 * gamma[i - 1];
+      gamma[i] = 1.0 / alpha[i];
+    }
+    alpha[m - 1] = 4.0 - 1.0 * gamma[m - 2];
+    for (int i = 0; i < m; i++) {
+      const double g = 3.0 * ((y[(i + 2) * ys] - y[(i + 1) * ys]) - (y[(i + 1) * ys] - y[i * ys]));
+      z[i] = (i == 0) ? g : g - gamma[i - 1] * z[i - 1];
+    }
+    vector<double> x(m);
+    x[m - 1] = z[m - 1] / alpha[m - 1];
+    for (int i = m - 2; i >= 0; i--) x[i] = z[i] / alpha[i] - gamma[i] * x[i + 1];
+    for (int i = 0; i < m; i++) c[i + 1] = x[i];
+  }
+  for (int i = 0; i < n; i++) {
+    const int k = (i < n - 1) ? i : n - 2;   // interval
+    const double delx = (i < n - 1) ? 0.0 : 1.0;
+    const double dy = y[(k + 1) * ys] - y[k * ys];
+    const double b = dy - (c[k + 1] + 2.0 * c[k]) / 3.0;
+    const double dd = (c[k + 1] - c[k]) / 3.0;
+    d[i * ds] = b + delx * (2.0 * c[k] + 3.0 * dd * delx);
+  }
+}
+
+struct PsfInterp {
+  int method = INTERP_BICUBIC, nc = 0, nr = 0;
+  double xBound = 0, yBound = 0;
+  vector<double> z, zx, zy, zxy;   // [row][col]; bicubic knot derivatives d/dx, d/dy, d2/dxdy
+  void Init(int meth, const double* img, int ncols, int nrows) {
+    method = meth;
+    nc = ncols;
+    nr = nrows;
+    xBound = (nc - 1) / 2.0;
+    yBound = (nr - 1) / 2.0;
+    z.assign(img, img + (long)nc * nr);
+    if (method != INTERP_BICUBIC) return;
+    zx.assign(z.size(), 0.0);
+    zy.assign(z.size(), 0.0);
+    zxy.assign(z.size(), 0.0);
+    for (int r = 0; r < nr; r++) SplineKnotDerivatives(&z[(long)r * nc], 1, nc, &zx[(long)r * nc], 1);
+    for (int c = 0; c < nc; c++) SplineKnotDerivatives(&z[c], nc, nr, &zy[c], nc);
+    for (int r = 0; r < nr; r++) SplineKnotDerivatives(&zy[(long)r * nc], 1, nc, &zxy[(long)r * nc], 1);
+  }
+  // cell index of x on knots k - bound, k = 0 .. n-1: the last k <= n-2 with knot <= x
+  static int Cell(double x, double bound, int n) {
+    int i = (int)floor(x + bound);
+    if (i > n - 2) i = n - 2;
+    if (i < 0) i = 0;
+    while (i > 0 && (i - bound) > x) i--;
+    while (i < n - 2 && (i + 1 - bound) <= x) i++;
+    return i;
+  }
+  double Value(double x, double y) const {
+    if (x < -xBound || x > xBound || y < -yBound || y > yBound) return 0.0;
+    if (method == INTERP_BICUBIC) {   // bicubic Hermite patch of the cell (unit spacing)
+      const int i = Cell(x, xBound, nc), j = Cell(y, yBound, nr);
+      const double t = x - (i - xBound), u = y - (j - yBound);
+      const double H[2] = {(1.0 + 2.0 * t) * (1.0 - t) * (1.0 - t), t * t * (3.0 - 2.0 * t)};
+      const double G[2] = {t * (1.0 - t) * (1.0 - t), t * t * (t - 1.0)};
+      const double Hu[2] = {(1.0 + 2.0 * u) * (1.0 - u) * (1.0 - u), u * u * (3.0 - 2.0 * u)};
+      const double Gu[2] = {u * (1.0 - u) * (1.0 - u), u * u * (u - 1.0)};
+      double v = 0.0;
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++) {
+          const long k = (long)(j + b) * nc + (i + a);
+          v += z[k] * H[a] * Hu[b] + zx[k] * G[a] * Hu[b] + zy[k] * H[a] * Gu[b] + zxy[k] * G[a] * Gu[b];
+        }
+      return v;
+    }
+    // Lanczos2 / Lanczos3 (psf_interpolators.cpp:153-190, 235-272), with the
+    // reference's PSF index ix * nColumns + iy
+    const int n = (method == INTERP_LANCZOS2) ? 2 : 3;
+    const int midx = (int)floor(x + xBound), midy = (int)floor(y + yBound);   // FindIndex (:284-290)
+    double newVal = 0.0;
+    for (int i = -n; i <= n; i++) {
+      const int ix = midx + i;
+      if (ix < 0 || ix >= nc) continue;
+      const double x_dat = ix - xBound;
+      for (int jj = -n; jj <= n; jj++) {
+        const int iy = midy + jj;
+        if (iy < 0 || iy >= nr) continue;
+        const double y_dat = iy - yBound;
+        const long idx = (long)ix * nc + iy;
+        const double pix = (idx < (long)nc * nr) ? z[idx] : 0.0;
+        newVal += Lanczos(x - x_dat, n) * Lanczos(y - y_dat, n) * pix;
+      }
+    }
+    return newVal;
+  }
+};
+
+struct PointSource : Func {   // func_pointsource.cpp:213-235, func_pointsource-rot.cpp:200-245
+  bool rot = false;
+  double I_tot = 0, cosPA = 1, sinPA = 0;
+  int scale = 1;                       // oversampling scale of the grid being evaluated
+  const PsfInterp* interp = nullptr;
+  PointSource(bool r) : rot(r) { doSubsampling = false; }
+  int NParams() const override { return rot ? 2 : 1; }
+  void Setup(const double* p, int off, double xc, double yc) override {
+    x0 = xc;
+    y0 = yc;
+    if (rot) {
+      const double PA_rad = (p[off] + 90.0) * DEG2RAD;
+      cosPA = cos(PA_rad);
+      sinPA = sin(PA_rad);
+      I_tot = p[off + 1];
+    } else {
+      I_tot = p[off];
+    }
+  }
+  double GetValue(double x, double y) override {
+    double x_diff = scale * (x - x0), y_diff = scale * (y - y0);
+    if (rot) {
+      const double xp = x_diff * cosPA + y_diff * sinPA;
+      const double yp = -x_diff * sinPA + y_diff * cosPA;
+      x_diff = xp;
+      y_diff = yp;
+    }
+    return I_tot * interp->Value(x_diff, y_diff);
+  }
+};
+
+inline bool IsPointSource(Func* f) { return dynamic_cast<PointSource*>(f) != nullptr; }
+
 // ------------------------------------------------------------------ QAGS restatement
 // QUADPACK QAGS as reimplemented by GSL 2.5 (gsl_integration_qags, called from
 // function_objects/integrator.cpp:42-55 with epsabs=0, epsrel=1e-6, limit=1000):
@@ -807,6 +964,8 @@ Func* MakeFunc(const char* name) {   // core/add_functions.cpp:163-385 (name -> 
   if (s == "Moffat") return new Moffat();
   if (s == "FlatSky") return new FlatSky();
   if (s == "ExponentialDisk3D") return new ExpDisk3D();
+  if (s == "PointSource") return new PointSource(false);
+  if (s == "PointSourceRot") return new PointSource(true);
   return nullptr;
 }
 
@@ -1231,6 +1390,9 @@ struct OsampRegion {   // core/oversampled_region.cpp:207-430
   double subpixFrac, startOffset;
   Convolver conv;
   vector<double> model;
+  // bicubic interpolator of the oversampled PSF as passed in: the region builds it
+  // before its Convolver normalises the PSF (core/oversampled_region.cpp:176-185)
+  PsfInterp interp;
 };
 
 struct Model {
@@ -1245,6 +1407,35 @@ struct Model {
   vector<double> psfIn;
   bool psfNormalize = true;
   Convolver conv;
+  // point sources: interpolator of the (normalised) PSF, method from SetupPsfInterpolation
+  int interpMethod = INTERP_BICUBIC;
+  PsfInterp interp;
+  bool interpReady = false;
+  bool HasPointSources() const {
+    for (Func* f : funcs)
+      if (IsPointSource(f)) return true;
+    return false;
+  }
+  // ModelObject::SetupPsfInterpolation (core/model_object.cpp:343-380): the PSF pixels
+  // stored by AddPSFVector, Kahan-normalised when requested (:827-876, NormalizePSF :2581-2594)
+  int SetupPsfInterpolation() {
+    if (!doConvolution) return -1;
+    if (nPsfCols < 4 || nPsfRows < 4) return -2;
+    vector<double> p = psfIn;
+    if (psfNormalize) {
+      double psfSum = 0.0, storedError = 0.0;
+      for (size_t k = 0; k < p.size(); k++) {
+        double adjustedVal = p[k] - storedError;
+        double tempSum = psfSum + adjustedVal;
+        storedError = (tempSum - psfSum) - adjustedVal;
+        psfSum = tempSum;
+      }
+      for (size_t k = 0; k < p.size(); k++) p[k] = p[k] / psfSum;
+    }
+    interp.Init(interpMethod, p.data(), nPsfCols, nPsfRows);
+    interpReady = true;
+    return 0;
+  }
   vector<OsampRegion*> regions;
   // data and weights (core/model_object.cpp:462-819)
   vector<double> data, weight, mask, extraCash, model, deviates;
@@ -1289,7 +1480,8 @@ struct Model {
       long j = k % nModelCols, i = k / nModelCols;
       double y = (double)(i - nPsfRows + 1), x = (double)(j - nPsfCols + 1);
       double newValSum = 0.0, storedError = 0.0;
-      for (int n = 0; n < nF; n++) {   // Kahan summation over components
+      for (int n = 0; n < nF; n++) {   // Kahan summation over components (point sources later)
+        if (IsPointSource(funcs[n])) continue;
         double adjVal = funcs[n]->GetValue(x, y) - storedError;
         double tempSum = newValSum + adjVal;
         storedError = (tempSum - newValSum) - adjVal;
@@ -1298,7 +1490,43 @@ struct Model {
       model[i * nModelCols + j] = newValSum;
     }
     if (doConvolution) conv.Convolve(model.data());
+    // point sources after the convolution (core/model_object.cpp:1183-1218), with scale 1
+    // on the main grid at every call (the reference keeps the last region's scale in its
+    // PointSource objects for later calls, oversampled_region.cpp:371-375 against
+    // model_object.cpp:1190-1192; its first evaluation, which the makeimage regression
+    // checks, uses 1)
+    if (HasPointSources()) {
+      if (!interpReady) SetupPsfInterpolation();
+#pragma omp parallel for schedule(static, 10) num_threads(nthreads)
+      for (long k = 0; k < nModelVals; k++) {
+        long j = k % nModelCols, i = k / nModelCols;
+        double y = (double)(i - nPsfRows + 1), x = (double)(j - nPsfCols + 1);
+        model[i * nModelCols + j] += PointSourceSumAt(x, y, &interp, 1);
+      }
+    }
     for (OsampRegion* R : regions) ComputeRegionAndDownsample(*R);
+  }
+
+  // thread-safe form of PointSourceSum: the function objects are shared, so the
+  // per-call interpolator and scale go in as arguments
+  double PointSourceSumAt(double x, double y, const PsfInterp* in, int scale) const {
+    double newValSum = 0.0, storedError = 0.0;
+    for (Func* f : funcs) {
+      const PointSource* ps = dynamic_cast<const PointSource*>(f);
+      if (!ps) continue;
+      double x_diff = scale * (x - ps->x0), y_diff = scale * (y - ps->y0);
+      if (ps->rot) {
+        const double xp = x_diff * ps->cosPA + y_diff * ps->sinPA;
+        const double yp = -x_diff * ps->sinPA + y_diff * ps->cosPA;
+        x_diff = xp;
+        y_diff = yp;
+      }
+      double adjVal = ps->I_tot * in->Value(x_diff, y_diff) - storedError;
+      double tempSum = newValSum + adjVal;
+      storedError = (tempSum - newValSum) - adjVal;
+      newValSum = tempSum;
+    }
+    return newValSum;
   }
 
   // OversampledRegion::ComputeRegionAndDownsample (core/oversampled_region.cpp:284-430)
@@ -1312,6 +1540,7 @@ struct Model {
       double x = R.x1 + R.startOffset + (j - R.ncPsf) * R.subpixFrac;
       double newValSum = 0.0, storedError = 0.0;
       for (int n = 0; n < nF; n++) {
+        if (IsPointSource(funcs[n])) continue;
         double adjVal = funcs[n]->GetValue(x, y) - storedError;
         double tempSum = newValSum + adjVal;
         storedError = (tempSum - newValSum) - adjVal;
@@ -1320,6 +1549,15 @@ struct Model {
       R.model[(long)i * R.nCols + j] = newValSum;
     }
     R.conv.Convolve(R.model.data());
+    if (HasPointSources()) {   // oversampled_region.cpp:362-402: region PSF, region scale
+#pragma omp parallel for schedule(static, 10) num_threads(nthreads)
+      for (long k = 0; k < nVals; k++) {
+        int j = (int)(k % R.nCols), i = (int)(k / R.nCols);
+        double y = R.y1 + R.startOffset + (i - R.nrPsf) * R.subpixFrac;
+        double x = R.x1 + R.startOffset + (j - R.ncPsf) * R.subpixFrac;
+        R.model[(long)i * R.nCols + j] += PointSourceSumAt(x, y, &R.interp, R.scale);
+      }
+    }
     DownsampleAndReplace(R.model.data(), R.nCols, R.nRows, R.ncPsf, R.nrPsf, model.data(), nModelCols, nModelRows,
                          nPsfCols, nPsfRows, R.x1, R.y1, R.scale);
   }
@@ -1543,6 +1781,7 @@ int orc_add_oversampled_psf(orc_model* m, const double* psf, int nc, int nr, int
   R->nCols = R->nBaseCols * scale + 2 * nc;
   R->nRows = R->nBaseRows * scale + 2 * nr;
   R->model.assign((long)R->nCols * R->nRows, 0.0);
+  if (nc >= 4 && nr >= 4) R->interp.Init(INTERP_BICUBIC, psf, nc, nr);
   R->conv.Setup(psf, nc, nr, normalize != 0, R->nCols, R->nRows, m->convMode, m->nthreads);
   m->regions.push_back(R);
   return 0;
@@ -1584,7 +1823,26 @@ double orc_conv_seconds(int reset) {
 }
 int orc_fit_statistic(orc_model* m, const double* params, double* out) { *out = m->GetFitStatistic(params); return 0; }
 
+// SetupPsfInterpolation's method (INTERP_*); the interpolator is built at the first
+// evaluation.  Returns -1 without a PSF, -2 for a PSF smaller than 4 x 4
+// (core/model_object.cpp:349-360), else 0.
+int orc_set_psf_interpolation(orc_model* m, int method) {
+  if (method != INTERP_BICUBIC && method != INTERP_LANCZOS2 && method != INTERP_LANCZOS3) return -3;
+  m->interpMethod = method;
+  m->interpReady = false;
+  if (!m->doConvolution) return -1;
+  if (m->nPsfCols < 4 || m->nPsfRows < 4) return -2;
+  return 0;
+}
+
 // ---- single-function and primitive entry points for unit-level pins
+double orc_psf_interp_value(int method, const double* psf, int nc, int nr, double x, double y) {
+  PsfInterp in;
+  in.Init(method, psf, nc, nr);
+  return in.Value(x, y);
+}
+double orc_lanczos(double x, int n) { return Lanczos(x, n); }
+void orc_spline_knot_derivatives(const double* y, int n, double* d) { SplineKnotDerivatives(y, 1, n, d, 1); }
 double orc_func_value(const char* name, const double* params, double x0, double y0, double x, double y,
                       int subsampling) {
   Func* f = MakeFunc(name);

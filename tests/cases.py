@@ -35,6 +35,8 @@ class Case:
         m.SetSubsampling(self.subsampling)
         if self.psf is not None:
             m.AddPSFVector(self.psf, self.normalize_psf)
+        if getattr(self, "interp", None) is not None:
+            m.SetupPsfInterpolation(self.interp)
         if self.data is not None:
             m.AddImageDataVector(self.data)
             m.AddImageCharacteristics(*self.chars)
@@ -290,3 +292,60 @@ def disk_los_tight(params6, x0, y0, x, y, epsrel=1e-12):
     L = 20.0 * params6[3]
     pts = sorted(p for p in bps if -L < p < L)
     return integrate.quad(f, -L, L, epsabs=0.0, epsrel=epsrel, limit=5000, points=pts or None)[0]
+
+
+# ---------------------------------------------------------------- PointSource
+def fits_image(name):
+    from imfit_amd.fits import read_image
+    return read_image(os.path.join(GOLDEN, "fits", name))
+
+
+def pointsource_osamp(method=None):
+    """tests/makeimage_reference/config_makeimage_pointsource-oversample.dat: a 200x200
+    image of one PointSource (X0 = Y0 = 100, I_tot 1) with --psf tests/psf_standard.fits,
+    --overpsf tests/psf_oversamp.fits, --overpsf_scale 3, --overpsf_region 80:120,80:120
+    (do_makeimage_tests:74); the archived output is tests/oversampled_pointsource.fits,
+    compared on pixels > 1e-7 (do_makeimage_tests:239)."""
+    psf, psf_os = fits_image("psf_standard.fits"), fits_image("psf_oversamp.fits")
+    c = Case([("PointSource", True)], [100.0, 100.0, 1.0], ncols=200, nrows=200, psf=psf,
+             osamp=[(psf_os, 3, 80, 120, 80, 120)])
+    c.interp = method
+    return c, fits_image("oversampled_pointsource.fits")
+
+
+def pointsource_fit(method=1, rot=False, osamp=False, stat="chi2", seed=11):
+    """Two point sources (one rotated with PointSourceRot when rot) on a Sersic galaxy
+    and a sky, 90x84 pixels, with an asymmetric 15x13 Moffat-like PSF (so that the
+    transposed Lanczos index and the rotation matter), optionally a 20x20 region
+    oversampled 3x with a 39x39 PSF; Poisson data from the oracle model at shifted
+    parameters."""
+    import oracle
+    yy, xx = np.mgrid[0:13, 0:15].astype(float)
+    psf = 1.0 / (1.0 + ((xx - 7.2) / 2.1) ** 2 + ((yy - 6.1) / 1.6) ** 2) ** 2.5
+    funcs = [("Sersic", True), ("FlatSky", False), ("PointSource", True)]
+    params = [44.2, 40.7, 30.0, 0.2, 1.8, 4.0, 9.0, 2.0, 30.3, 52.6, 800.0]
+    if rot:
+        funcs.append(("PointSourceRot", True))
+        params += [61.4, 27.2, 25.0, 500.0]
+    else:
+        funcs.append(("PointSource", True))
+        params += [61.4, 27.2, 500.0]
+    osamps = []
+    if osamp:
+        yo, xo = np.mgrid[0:39, 0:39].astype(float)
+        pso = 1.0 / (1.0 + ((xo - 19.0) / 6.0) ** 2 + ((yo - 19.0) / 5.0) ** 2) ** 2.5
+        osamps = [(pso, 3, 21, 40, 43, 62)]
+    base = Case(funcs, params, ncols=90, nrows=84, psf=psf, osamp=osamps)
+    base.interp = method
+    o = base.build(oracle.OracleModel(threads=4))
+    truth = np.array(params, dtype=float)
+    truth[8] += 0.3
+    truth[9] -= 0.2
+    truth[10] *= 1.1
+    o.CreateModelImage(truth)
+    m = o.GetModelImageVector()
+    rng = np.random.default_rng(seed)
+    d = rng.poisson(4.0 * np.clip(m, 0, None)) / 4.0
+    c = Case(funcs, params, data=d, psf=psf, osamp=osamps, chars=(4.0, 1.0, 1.0, 1, 0.0), stat=stat)
+    c.interp = method
+    return c
