@@ -26,6 +26,8 @@ WEIGHTS_ARE_SIGMAS, WEIGHTS_ARE_VARIANCES, WEIGHTS_ARE_WEIGHTS = 100, 110, 120
 MASK_ZERO_IS_GOOD, MASK_ZERO_IS_BAD = 10, 20
 PRECISION_FP64, PRECISION_FP32 = 0, 1
 GRADIENT_ADJOINT, GRADIENT_TANGENT = 0, 1
+INTERP_BICUBIC, INTERP_LANCZOS2, INTERP_LANCZOS3 = 1, 2, 3
+INTERPOLATION = {"bicubic": INTERP_BICUBIC, "lanczos2": INTERP_LANCZOS2, "lanczos3": INTERP_LANCZOS3}
 
 _SIGS = {
     "ifg_create": ([C.c_int], _vp),
@@ -34,6 +36,7 @@ _SIGS = {
     "ifg_set_precision": ([_vp, C.c_int], C.c_int),
     "ifg_add_function": ([_vp, C.c_char_p, C.c_int], C.c_int),
     "ifg_set_subsampling": ([_vp, C.c_int], C.c_int),
+    "ifg_set_psf_interpolation": ([_vp, C.c_int], C.c_int),
     "ifg_get_nparams": ([_vp], C.c_int),
     "ifg_add_psf": ([_vp, _dp, C.c_int, C.c_int, C.c_int], C.c_int),
     "ifg_setup_model_image": ([_vp, C.c_int, C.c_int], C.c_int),
@@ -190,6 +193,11 @@ class ModelObjectGPU:
 
     def SetSubsampling(self, on):
         return self._chk(_LIB.ifg_set_subsampling(self.h, int(on)), "SetSubsampling")
+
+    def SetupPsfInterpolation(self, method=INTERP_BICUBIC):
+        """Point-source PSF interpolation: INTERP_BICUBIC, INTERP_LANCZOS2 or INTERP_LANCZOS3
+        (ModelObject::SetupPsfInterpolation, core/model_object.cpp:343-380)."""
+        return self._chk(_LIB.ifg_set_psf_interpolation(self.h, int(method)), "SetupPsfInterpolation")
 
     def GetNParams(self):
         return _LIB.ifg_get_nparams(self.h)

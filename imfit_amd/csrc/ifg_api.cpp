@@ -65,6 +65,8 @@ int NParamsOf(int type) {
     case F_GENSERSIC: return 6;
     case F_BROKENEXP: return 7;
     case F_EDGEONDISK: return 5;
+    case F_POINTSOURCE: return 1;      // I_tot (func_pointsource.cpp:48)
+    case F_POINTSOURCEROT: return 2;   // PA, I_tot (func_pointsource-rot.cpp)
   }
   return 0;
 }
@@ -80,6 +82,8 @@ int TypeOf(const char* name) {
   if (s == "Sersic_GenEllipse") return F_GENSERSIC;
   if (s == "BrokenExponential") return F_BROKENEXP;
   if (s == "EdgeOnDisk") return F_EDGEONDISK;
+  if (s == "PointSource") return F_POINTSOURCE;
+  if (s == "PointSourceRot") return F_POINTSOURCEROT;
   return 0;
 }
 
@@ -99,6 +103,18 @@ void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
   v[1] = y0;
   if (type == F_FLATSKY) {
     v[5] = p[0];
+    return;
+  }
+  if (type == F_POINTSOURCE) {   // PointSource::Setup (func_pointsource.cpp:198-203)
+    v[2] = 1.0;
+    v[5] = p[0];
+    return;
+  }
+  if (type == F_POINTSOURCEROT) {   // PointSourceRot::Setup (func_pointsource-rot.cpp:188-198)
+    const double PA_rad = (p[0] + 90.0) * DEG2RAD;
+    v[2] = cos(PA_rad);
+    v[3] = sin(PA_rad);
+    v[5] = p[1];
     return;
   }
   if (type == F_EDGEONDISK) {   // EdgeOnDisk::Setup (func_edge-on-disk.cpp:95-116): PA, L_0, h, n, z_0
@@ -317,10 +333,40 @@ struct ConvGrid {
   int rowsK1 = 1, rowsK3 = 1, C64 = 1, C32 = 1;
 };
 
+// The PSF a grid's point sources interpolate, on the device: pixels, then (bicubic)
+// the natural-spline knot derivatives d/dx, d/dy, d2/dxdy, each nC x nR FP64.
+struct PsfInterpHost {
+  int method = 0, nC = 0, nR = 0;
+  DevBuf d;
+  bool ready() const { return method != 0; }
+  PsfInterp view(int scale) const {
+    PsfInterp P;
+    std::memset(&P, 0, sizeof(P));
+    if (!ready()) return P;
+    const double* b = d.as<double>();
+    const long n = (long)nC * nR;
+    P.z = b;
+    if (method == PI_BICUBIC) {
+      P.zx = b + n;
+      P.zy = b + 2 * n;
+      P.zxy = b + 3 * n;
+    }
+    P.method = method;
+    P.nC = nC;
+    P.nR = nR;
+    P.scale = scale;
+    P.xBound = (nC - 1) / 2.0;
+    P.yBound = (nR - 1) / 2.0;
+    return P;
+  }
+};
+
 struct Region {
   int x1, y1, nbc, nbr, scale, pc, pr, rowsReplaced;
-  std::vector<double> psf;   // normalised
-  ConvGrid conv[2];          // [0] single (halves), [1] batch (pairs)
+  std::vector<double> psf;      // normalised
+  std::vector<double> psfRaw;   // as passed in: what the region's point-source interpolator uses
+  PsfInterpHost psi;
+  ConvGrid conv[2];             // [0] single (halves), [1] batch (pairs)
 };
 
 struct Plan {
@@ -344,6 +390,7 @@ struct Plan {
   DevBuf aTail_d, adjPart_d;   // adjoint plans: box pixels' adjoint values, per-slot partial sums
   int adjWidth = 0;            // nComp * IFG_SLOTS
   long adjTailBlocks = 0;
+  int adjPsBlocks = 0;         // partial slots of the point sources' sums (the row-pass blocks)
   size_t usedOff = 0, prefixOff = 0;   // staging offsets of the per-grid box pixel counts / tail prefixes
   int nSlots = 1, nTasks = 1;
   std::vector<int> nB1, nB2, nB3, nJobs;   // per chunk: blocks of the three passes, final jobs
@@ -429,6 +476,15 @@ struct ifg_model {
   std::vector<long> bootIdx;
   std::vector<double> bootCount;
   std::vector<std::unique_ptr<Region>> regions;
+  // point sources: interpolation method of the main PSF (SetupPsfInterpolation) and its
+  // device copy
+  int psMethod = PI_BICUBIC;
+  PsfInterpHost mainPsi;
+  bool hasPointSources() const {
+    for (int t : types)
+      if (IsPointSourceType(t)) return true;
+    return false;
+  }
   // device state
   bool devReady = false;
   std::map<int, std::unique_ptr<FFTLen>> ffts;
@@ -803,10 +859,105 @@ int UploadImageArrays(ifg_model* m) {
   return IFG_OK;
 }
 
+// Knot derivatives of the natural cubic spline through n values at unit spacing
+// (GSL 2.5's gsl_interp_cspline, which gsl_interp2d_bicubic runs along each axis,
+// function_objects/psf_interpolators.cpp:62-63): c = y''/2 with c = 0 at both ends
+// from the interior equations c[i-1] + 4 c[i] + c[i+1] = 3 (y[i+1] - 2 y[i] + y[i-1])
+// (symmetric tridiagonal LDL^T); the derivative at knot i is that of interval i at
+// its start, and at the last knot that of the last interval at its end.
+void SplineKnotDerivs(const double* y, long ys, int n, double* d, long ds) {
+  std::vector<double> c(n, 0.0);
+  const int m = n - 2;
+  if (m == 1) {
+    c[1] = 3.0 * ((y[2 * ys] - y[ys]) - (y[ys] - y[0])) / 4.0;
+  } else if (m > 1) {
+    std::vector<double> alpha(m), gamma(m), z(m), x(m);
+    alpha[0] = 4.0;
+    gamma[0] = 1.0 / alpha[0];
+    for (int i = 1; i < m - 1; i++) {
+      alpha[i] = 4.0 - 1.0 * gamma[i - 1];
+      gamma[i] = 1.0 / alpha[i];
+    }
+    alpha[m - 1] = 4.0 - 1.0 * gamma[m - 2];
+    for (int i = 0; i < m; i++) {
+      const double g = 3.0 * ((y[(i + 2) * ys] - y[(i + 1) * ys]) - (y[(i + 1) * ys] - y[i * ys]));
+      z[i] = (i == 0) ? g : g - gamma[i - 1] * z[i - 1];
+    }
+    x[m - 1] = z[m - 1] / alpha[m - 1];
+    for (int i = m - 2; i >= 0; i--) x[i] = z[i] / alpha[i] - gamma[i] * x[i + 1];
+    for (int i = 0; i < m; i++) c[i + 1] = x[i];
+  }
+  for (int i = 0; i < n; i++) {
+    const int k = (i < n - 1) ? i : n - 2;
+    const double delx = (i < n - 1) ? 0.0 : 1.0;
+    const double dy = y[(k + 1) * ys] - y[k * ys];
+    const double b = dy - (c[k + 1] + 2.0 * c[k]) / 3.0;
+    const double dd = (c[k + 1] - c[k]) / 3.0;
+    d[i * ds] = b + delx * (2.0 * c[k] + 3.0 * dd * delx);
+  }
+}
+
+// Device copy of a PSF for point-source interpolation (PsfInterpolator_* constructors,
+// psf_interpolators.cpp:44-66,111-133): the pixels, and for bicubic the knot
+// derivatives along x (rows), along y (columns) and the cross derivative (the x
+// derivative of the y derivatives).
+int BuildPsfInterp(ifg_model* m, PsfInterpHost& H, int method, const std::vector<double>& img, int nC, int nR) {
+  const long n = (long)nC * nR;
+  std::vector<double> buf(method == PI_BICUBIC ? 4 * n : n);
+  std::copy(img.begin(), img.end(), buf.begin());
+  if (method == PI_BICUBIC) {
+    double *z = buf.data(), *zx = z + n, *zy = z + 2 * n, *zxy = z + 3 * n;
+    for (int r = 0; r < nR; r++) SplineKnotDerivs(z + (long)r * nC, 1, nC, zx + (long)r * nC, 1);
+    for (int c = 0; c < nC; c++) SplineKnotDerivs(z + c, nC, nR, zy + c, nC);
+    for (int r = 0; r < nR; r++) SplineKnotDerivs(zy + (long)r * nC, 1, nC, zxy + (long)r * nC, 1);
+  }
+  HIPCHK(H.d.ensure(sizeof(double) * buf.size()), "alloc PSF interpolator");
+  HIPCHK(hipMemcpy(H.d.p, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice), "PSF interpolator");
+  H.method = method;
+  H.nC = nC;
+  H.nR = nR;
+  return IFG_OK;
+}
+
+// Point sources need the PSF for interpolation, at least 4 x 4 (ModelObject::
+// SetupPsfInterpolation, core/model_object.cpp:343-380); every oversampled region makes
+// a bicubic interpolator of its own PSF as passed in, before its convolver normalises
+// it (core/oversampled_region.cpp:176-185).
+int PreparePointSources(ifg_model* m) {
+  if (!m->hasPointSources()) return IFG_OK;
+  if (!m->doConv) return m->fail(IFG_ERR_STATE, "PointSource function used, but no PSF image was supplied");
+  if (m->Pc < 4 || m->Pr < 4)
+    return m->fail(IFG_ERR_ARG, "PSF image too small for interpolation with PointSource functions (needs 4 x 4)");
+  int st;
+  if (!m->mainPsi.ready() && (st = BuildPsfInterp(m, m->mainPsi, m->psMethod, m->psf, m->Pc, m->Pr)) < 0) return st;
+  for (auto& R : m->regions) {
+    if (R->pc < 4 || R->pr < 4)
+      return m->fail(IFG_ERR_UNSUPPORTED, "oversampled PSF too small for interpolation with PointSource functions");
+    if (!R->psi.ready() && (st = BuildPsfInterp(m, R->psi, PI_BICUBIC, R->psfRaw, R->pc, R->pr)) < 0) return st;
+  }
+  return IFG_OK;
+}
+
+// The interpolator of grid gi (0: main image, 1..: regions) of model m, for its tasks.
+PsfInterp GridPsi(const ifg_model* m, int gi) {
+  if (!m->hasPointSources()) {
+    PsfInterp P;
+    std::memset(&P, 0, sizeof(P));
+    return P;
+  }
+  if (gi == 0) return m->mainPsi.view(1);
+  const Region& R = *m->regions[gi - 1];
+  return R.psi.view(R.scale);
+}
+
 int PrepareDevice(ifg_model* m) {
   if (m->devReady) return IFG_OK;
   if (!m->setupDone) return m->fail(IFG_ERR_STATE, "no image: call ifg_add_image_data or ifg_setup_model_image");
   if (m->types.empty()) return m->fail(IFG_ERR_STATE, "no functions added");
+  {
+    const int st = PreparePointSources(m);
+    if (st < 0) return st;
+  }
   if (m->dataSet) {
     if (!m->finalDone) return m->fail(IFG_ERR_STATE, "call ifg_final_setup before evaluating against data");
     int st = (m->precision == IFG_PRECISION_FP64) ? UploadImageArrays<double>(m) : UploadImageArrays<float>(m);
@@ -1166,6 +1317,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
       if (Tk.kind != TK_MAIN_PLAIN)
         P.lds3 = std::max(P.lds3, InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
       FillTaskStat<T>(m, Tk.st);
+      Tk.psi = GridPsi(m, gi);
       Tk.st.devOut = (flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
       if (flags & PF_DEVKEEP) Tk.st.devOut = m->devkeep_d.as<double>();
       Tk.st.modelOut = (flags & PF_MODEL) ? P.model_d.as<double>() + (long)v0 * nData : nullptr;
@@ -1637,6 +1789,7 @@ int BuildAdjPlan(ifg_model* m, Plan& P, int flags, const std::vector<int>& minSt
       Tk.r0 = R.y1 - 1;
     }
     FillTaskStat<T>(m, Tk.st);
+    Tk.psi = GridPsi(m, gi);
     Tk.st.dfac = (flags & PF_ADJ_DFAC) ? m->dfac_d.as<double>() : nullptr;
     Tk.st.uvec = (flags & PF_ADJ_U) ? m->adjU_d.as<double>() : nullptr;
   }
@@ -1667,7 +1820,8 @@ int BuildAdjPlan(ifg_model* m, Plan& P, int flags, const std::vector<int>& minSt
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += P.strideMax[gi];
   P.adjTailBlocks = P.tailWaves > 0 ? std::min<long>((P.tailWaves + 3) / 4, 2048) : 0;
   P.adjWidth = nC * IFG_SLOTS;
-  HIPCHK(P.adjPart_d.ensure(sizeof(double) * (size_t)(P.nSlots + P.adjTailBlocks) * P.adjWidth),
+  P.adjPsBlocks = m->hasPointSources() ? P.nB1[0] : 0;   // the row-pass blocks sum the point sources
+  HIPCHK(P.adjPart_d.ensure(sizeof(double) * (size_t)(P.nSlots + P.adjTailBlocks + P.adjPsBlocks) * P.adjWidth),
          "alloc adjoint partials");
   HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * P.adjWidth + 64, hipHostMallocDefault), "pinned results");
   HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
@@ -1689,10 +1843,11 @@ int EnqueueAdj(ifg_model* m, Plan& P) {
   al.partials = P.adjPart_d.as<double>();
   al.width = P.adjWidth;
   al.tailSlot0 = P.nSlots;
+  al.psSlot0 = P.adjPsBlocks > 0 ? (int)(P.nSlots + P.adjTailBlocks) : -1;
   TIMED("k_adjoint",
         launch_adjoint<T>(s, P.tasks_d.as<Task>(), P.nTasks, P.br1[0], P.nB1[0], P.lds1, P.br2[0], P.nB2[0], P.lds2,
                           P.br3[0], P.nB3[0], P.lds3, prefix_d, P.adjTailBlocks, m->doSubsampling ? 1 : 0,
-                          HeavyVariant(m), al, (int)(P.nSlots + P.adjTailBlocks), P.outDev, 1),
+                          HeavyVariant(m), al, (int)(P.nSlots + P.adjTailBlocks + P.adjPsBlocks), P.outDev, 1),
         "k_adjoint");
   return IFG_OK;
 }
@@ -2021,6 +2176,17 @@ int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set
   return ExceptionStatus(m);
 }
 
+int ifg_set_psf_interpolation(ifg_model* m, int method) try {
+  if (!m) return IFG_ERR_ARG;
+  if (method != PI_BICUBIC && method != PI_LANCZOS2 && method != PI_LANCZOS3)
+    return m->fail(IFG_ERR_ARG, "unknown PSF interpolation method");
+  if (m->devReady) return m->fail(IFG_ERR_STATE, "PSF interpolation chosen after the first evaluation");
+  m->psMethod = method;
+  return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(m);
+}
+
 int ifg_set_subsampling(ifg_model* m, int do_subsampling) try {
   if (!m) return IFG_ERR_ARG;
   m->doSubsampling = do_subsampling != 0;
@@ -2160,6 +2326,7 @@ int ifg_add_oversampled_psf(ifg_model* m, const double* psf, int n_columns, int 
   R->pc = n_columns;
   R->pr = n_rows;
   R->psf = NormalizedPSF(psf, n, normalize != 0);
+  R->psfRaw.assign(psf, psf + n);
   // DownsampleAndReplace derives the replaced row count from the column count
   // (core/downsample.cpp:78-79); keep that, but only where it stays inside the region.
   const int nOsCols = R->nbc * scale + 2 * n_columns;
@@ -2813,6 +2980,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
       if (Tk.kind != TK_MAIN_PLAIN)
         P.lds3 = std::max(P.lds3, InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
       FillTaskStat<T>(m, Tk.st);
+      Tk.psi = GridPsi(m, gi);
       if (P.kind == BK_KEEP) {
         Tk.st.devOut = m->devkeep_d.as<double>();
         Tk.st.dfacOut = m->dfac_d.as<double>();

@@ -14,7 +14,29 @@ namespace ifg {
 
 enum FuncType {
   F_SERSIC = 1, F_EXP = 2, F_FLATSKY = 3, F_GAUSS = 4, F_MOFFAT = 5, F_EXPDISK3D = 6, F_GENSERSIC = 7, F_BROKENEXP = 8,
-  F_EDGEONDISK = 9
+  F_EDGEONDISK = 9, F_POINTSOURCE = 10, F_POINTSOURCEROT = 11
+};
+
+// Point sources are added after the PSF convolution (core/model_object.cpp:1183-1218),
+// so the per-pixel component loops skip them.
+__host__ __device__ inline bool IsPointSourceType(int type) { return type == F_POINTSOURCE || type == F_POINTSOURCEROT; }
+
+// Interpolation methods of the PSF for point sources (function_objects/psf_interpolators.h:8-10)
+enum PsfInterpMethod { PI_BICUBIC = 1, PI_LANCZOS2 = 2, PI_LANCZOS3 = 3 };
+
+// The PSF that the point sources of one grid interpolate (PsfInterpolator,
+// function_objects/psf_interpolators.cpp): pixel (col, row) is the knot at offset
+// (col - xBound, row - yBound) from the PSF centre.  Bicubic: the natural-spline knot
+// derivatives d/dx, d/dy, d2/dxdy in the same layout.  scale: the grid's oversampling
+// (PointSource multiplies its offsets by it, func_pointsource.cpp:222-223).
+// z == null: the grid has no point sources.
+struct PsfInterp {
+  const double* z;
+  const double* zx;
+  const double* zy;
+  const double* zxy;
+  int method, nC, nR, scale;
+  double xBound, yBound;
 };
 
 // S_JAC: write Jacobian columns dfac * dm/dp_j;  S_GRAD: accumulate sum(gfac * dm/dp_j)
@@ -161,6 +183,7 @@ struct Task {
   int nVec;                    // vectors (planes) of this task in the launch
   double* aTail;               // adjoint pass: the adjoint value of each sub-pixel box pixel, per plane
                                // (plane p at aTail + p * ea.tailStride, box layout as Box.off)
+  PsfInterp psi;               // point sources on this grid (psi.z null: none)
   EvalArgs ea;
   TaskStat st;
 };
@@ -172,6 +195,7 @@ struct AdjLaunch {
   double* partials;
   int width;
   int tailSlot0;
+  int psSlot0;      // point sources present: the row-pass blocks' partial slots start here (else -1)
 };
 
 // Block -> task lookup of one merged launch: with few tasks (one model's grids),

@@ -1022,6 +1022,7 @@ __device__ __forceinline__ double deriv_point(const EvalArgs& ea, int j, double 
   const int gidx = ea.freeIdx[j];
   double sum = 0.0;
   for (int n = 0; n < ea.nComp; n++) {
+    if (IsPointSourceType(ea.types[n])) continue;   // added after the convolution
     const int slot = grad_slot(ea, n, gidx);
     if (slot >= 0) sum += comp_grad<DISK>(ea.types[n], ea.recs[n].v, x, y, slot);
   }
@@ -1036,6 +1037,7 @@ __device__ __forceinline__ void deriv_point2(const EvalArgs& ea, int jA, int jB,
   const int gA = ea.freeIdx[jA], gB = ea.freeIdx[jB];
   double sA = 0.0, sB = 0.0;
   for (int n = 0; n < ea.nComp; n++) {
+    if (IsPointSourceType(ea.types[n])) continue;
     const int slA = grad_slot(ea, n, gA), slB = grad_slot(ea, n, gB);
     if (slA < 0 && slB < 0) continue;
     double gs[kSlots];
@@ -1046,6 +1048,180 @@ __device__ __forceinline__ void deriv_point2(const EvalArgs& ea, int jA, int jB,
   }
   dA = sA;
   dB = sB;
+}
+
+// ------------------------------------------------------------------ point sources
+// PointSource / PointSourceRot (func_pointsource.cpp:213-235, func_pointsource-rot.cpp:
+// 200-245): I_tot times the PSF interpolated at the offset from the centre, scaled by
+// the grid's oversampling and, for PointSourceRot, rotated into the PA frame.
+//   rec: v[0] x0  v[1] y0  v[2] cos PA  v[3] sin PA (PointSourceRot)  v[5] I_tot
+
+// Lanczos kernel with the reference's constants (psf_interpolators.cpp:30-31,296-305)
+// and its derivative (0 where the kernel is flat or outside).
+__device__ __forceinline__ double lanczos_k(double x, int n, double& dk) {
+  const double PI_T = 3.14159265358979, PI2_T = 9.86960440108936;
+  const double xa = fabs(x);
+  dk = 0.0;
+  if (xa < 1.0e-6) return 1.0;
+  if (xa > n) return 0.0;
+  const double s1 = sin(PI_T * xa), s2 = sin(PI_T * xa / n);
+  const double val = (n * s1 * s2) / (PI2_T * xa * xa);
+  const double ds = (n * (PI_T * cos(PI_T * xa) * s2 + (PI_T / n) * s1 * cos(PI_T * xa / n))) / (PI2_T * xa * xa) -
+                    2.0 * val / xa;
+  dk = x < 0.0 ? -ds : ds;
+  return val;
+}
+
+// Cell of offset x on the knots k - bound (k = 0 .. n-1): the last k <= n-2 whose knot
+// is <= x (the interval GSL's search returns for the bicubic spline).
+__device__ __forceinline__ int interp_cell(double x, double bound, int n) {
+  int i = (int)floor(x + bound);
+  i = max(0, min(i, n - 2));
+  if (i > 0 && (i - bound) > x) i--;
+  if (i < n - 2 && (i + 1 - bound) <= x) i++;
+  return i;
+}
+
+// Interpolated PSF at offset (x, y); GRAD: also its derivatives along x and y.
+template <bool GRAD>
+__device__ double psf_interp(const PsfInterp& P, double x, double y, double& gx, double& gy) {
+  gx = gy = 0.0;
+  // outside the PSF (or not a number): 0, as psf_interpolators.cpp:99-100 outside
+  if (!(x >= -P.xBound && x <= P.xBound && y >= -P.yBound && y <= P.yBound)) return 0.0;
+  if (P.method == PI_BICUBIC) {   // bicubic Hermite patch of the cell (unit knot spacing)
+    const int i = interp_cell(x, P.xBound, P.nC), j = interp_cell(y, P.yBound, P.nR);
+    const double t = x - (i - P.xBound), u = y - (j - P.yBound);
+    const double H[2] = {(1.0 + 2.0 * t) * (1.0 - t) * (1.0 - t), t * t * (3.0 - 2.0 * t)};
+    const double G[2] = {t * (1.0 - t) * (1.0 - t), t * t * (t - 1.0)};
+    const double Hu[2] = {(1.0 + 2.0 * u) * (1.0 - u) * (1.0 - u), u * u * (3.0 - 2.0 * u)};
+    const double Gu[2] = {u * (1.0 - u) * (1.0 - u), u * u * (u - 1.0)};
+    const double dH[2] = {6.0 * t * t - 6.0 * t, 6.0 * t - 6.0 * t * t};
+    const double dG[2] = {3.0 * t * t - 4.0 * t + 1.0, 3.0 * t * t - 2.0 * t};
+    const double dHu[2] = {6.0 * u * u - 6.0 * u, 6.0 * u - 6.0 * u * u};
+    const double dGu[2] = {3.0 * u * u - 4.0 * u + 1.0, 3.0 * u * u - 2.0 * u};
+    double v = 0.0;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const long k = (long)(j + b) * P.nC + (i + a);
+        const double z = P.z[k], zx = P.zx[k], zy = P.zy[k], zxy = P.zxy[k];
+        v += z * H[a] * Hu[b] + zx * G[a] * Hu[b] + zy * H[a] * Gu[b] + zxy * G[a] * Gu[b];
+        if (GRAD) {
+          gx += z * dH[a] * Hu[b] + zx * dG[a] * Hu[b] + zy * dH[a] * Gu[b] + zxy * dG[a] * Gu[b];
+          gy += z * H[a] * dHu[b] + zx * G[a] * dHu[b] + zy * H[a] * dGu[b] + zxy * G[a] * dGu[b];
+        }
+      }
+    return v;
+  }
+  // Lanczos2 / Lanczos3 (psf_interpolators.cpp:153-190, 235-272), PSF index ix * nCols + iy
+  // as the reference forms it (entries past the array's end count as 0)
+  const int n = (P.method == PI_LANCZOS2) ? 2 : 3;
+  const int midx = (int)floor(x + P.xBound), midy = (int)floor(y + P.yBound);
+  const long npix = (long)P.nC * P.nR;
+  double newVal = 0.0;
+  for (int i = -n; i <= n; i++) {
+    const int ix = midx + i;
+    if (ix < 0 || ix >= P.nC) continue;
+    double dkx;
+    const double kx = lanczos_k(x - (ix - P.xBound), n, dkx);
+    for (int jj = -n; jj <= n; jj++) {
+      const int iy = midy + jj;
+      if (iy < 0 || iy >= P.nR) continue;
+      double dky;
+      const double ky = lanczos_k(y - (iy - P.yBound), n, dky);
+      const long idx = (long)ix * P.nC + iy;
+      const double pix = idx < npix ? P.z[idx] : 0.0;
+      newVal += kx * ky * pix;
+      if (GRAD) {
+        gx += dkx * ky * pix;
+        gy += kx * dky * pix;
+      }
+    }
+  }
+  return newVal;
+}
+
+// Offsets of grid point (x, y) in the PSF frame of point source `v` (type `type`).
+__device__ __forceinline__ void ps_offsets(int type, const double* v, const PsfInterp& P, double x, double y,
+                                           double& xp, double& yp) {
+  const double x_diff = P.scale * (x - v[0]), y_diff = P.scale * (y - v[1]);
+  if (type == F_POINTSOURCEROT) {
+    xp = x_diff * v[2] + y_diff * v[3];
+    yp = -x_diff * v[3] + y_diff * v[2];
+  } else {
+    xp = x_diff;
+    yp = y_diff;
+  }
+}
+
+// Kahan sum over the point sources at grid point (x, y) (core/model_object.cpp:1198-1216).
+__device__ double ps_sum(const EvalArgs& ea, const CompRec* rec, const PsfInterp& P, double x, double y) {
+  double newValSum = 0.0, storedError = 0.0;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int type = ea.types[n];
+    if (!IsPointSourceType(type)) continue;
+    const double* v = rec[n].v;
+    double xp, yp, gx, gy;
+    ps_offsets(type, v, P, x, y, xp, yp);
+    const double adjVal = v[5] * psf_interp<false>(P, xp, yp, gx, gy) - storedError;
+    const double tempSum = newValSum + adjVal;
+    storedError = (tempSum - newValSum) - adjVal;
+    newValSum = tempSum;
+  }
+  return newValSum;
+}
+
+// Slot derivatives of point source `v` at (x, y): 0 x0, 1 y0, then PointSource: 2 I_tot;
+// PointSourceRot: 2 PA, 3 I_tot.
+__device__ void ps_grad_slots(int type, const double* v, const PsfInterp& P, double x, double y,
+                              double (&g)[kSlots]) {
+  const double DEG2RAD = 0.017453292519943295;
+#pragma unroll
+  for (int k = 0; k < kSlots; k++) g[k] = 0.0;
+  double xp, yp, gx, gy;
+  ps_offsets(type, v, P, x, y, xp, yp);
+  const double val = psf_interp<true>(P, xp, yp, gx, gy);
+  const double I = v[5], s = P.scale;
+  if (type == F_POINTSOURCEROT) {
+    const double c = v[2], sn = v[3];
+    g[0] = I * (gx * (-s * c) + gy * (s * sn));
+    g[1] = I * (gx * (-s * sn) + gy * (-s * c));
+    g[2] = I * (gx * yp - gy * xp) * DEG2RAD;
+    g[3] = val;
+  } else {
+    g[0] = -I * s * gx;
+    g[1] = -I * s * gy;
+    g[2] = val;
+  }
+}
+
+// d(point-source sum)/d(global parameter gidx) at (x, y), summed over the point sources
+// in component order.
+__device__ double ps_deriv(const EvalArgs& ea, const CompRec* rec, const PsfInterp& P, double x, double y,
+                           int gidx) {
+  double sum = 0.0;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int type = ea.types[n];
+    if (!IsPointSourceType(type)) continue;
+    const int slot = grad_slot(ea, n, gidx);
+    if (slot < 0) continue;
+    double g[kSlots];
+    ps_grad_slots(type, rec[n].v, P, x, y, g);
+    sum += pick_slot(g, slot);
+  }
+  return sum;
+}
+
+// What the point sources add to the value of grid pixel (c, r) of vector `vec`: their
+// sum, or in a derivative pass (S_JAC, S_GRAD) its derivative by the plane's parameter.
+// Out of line: the inverse pass calls it only for grids that have point sources.
+__device__ __noinline__ double ps_add(const Task& Tk, int mode, int vec, int c, int r) {
+  const EvalArgs& ea = Tk.ea;
+  const double x = Tk.g.xa + (double)c * Tk.g.frac, y = Tk.g.ya + (double)r * Tk.g.frac;
+  const CompRec* rec = ea.recs + (long)vec * ea.recStride;
+  if (mode == S_JAC || mode == S_GRAD) return ps_deriv(ea, rec, Tk.psi, x, y, ea.freeIdx[vec]);
+  return ps_sum(ea, rec, Tk.psi, x, y);
 }
 
 // Derivative planes jA and jB (jB -1: none) on K pixels of their rows ra / rb
@@ -1118,6 +1294,7 @@ __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g
     }
     for (int n = 0; n < ea.nComp; n++) {
       const int type = ea.types[n];
+      if (IsPointSourceType(type)) continue;   // added after the convolution
       const double* v = rec[n].v;
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -1228,6 +1405,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
     if (lane == 0)
       for (int j = 0; j < ea.nFree; j++) outs[(long)j * ea.tailStride] = 0.0;
     for (int n = 0; n < ea.nComp; n++) {
+      if (IsPointSourceType(ea.types[n])) continue;
       unsigned need = 0;   // this component's slots that some plane differentiates by
       for (int j = 0; j < ea.nFree; j++) {
         const int slot = grad_slot(ea, n, ea.freeIdx[j]);
@@ -1248,6 +1426,7 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
   double newValSum = 0.0, storedError = 0.0;
   for (int n = 0; n < ea.nComp; n++) {
     const int type = ea.types[n];
+    if (IsPointSourceType(type)) continue;
     const CompRec rv = rec[n];   // into registers: read once, not per sample
     const double* v = rv.v;
     double val;
@@ -1922,7 +2101,11 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
             double v = *a;
             for (int jj = 0; jj < s; jj++) {
               const cpx<T> x = out[rr * g.Nc + bj * s + jj + g.hl];
-              v += (double)(plane == 0 ? x.x : x.y);
+              double val = (double)(plane == 0 ? x.x : x.y);
+              if (Tk.psi.z && (plane == 0 || pr.z >= 0))   // point sources, after the convolution
+                val += ps_add(Tk, sl.mode, plane ? pr.z : pr.x, bj * s + jj,
+                              (plane ? pr.w : pr.y) + io0 + sr0 + rr);
+              v += val;
             }
             *a = v;
           }
@@ -1982,16 +2165,26 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
         const int c = threadIdx.x + k * kThreads;
         if (c >= g.W) continue;
         const cpx<T> v = out[c + g.hl];
-        if (ra < st.H) stat_pixel_dw<T>(st, sl, pr.x, ra, c, (double)v.x, (double)dA[k], (double)wA[k], ea, true);
-        if (pr.z >= 0 && rb < st.H) stat_pixel_dw<T>(st, sl, pr.z, rb, c, (double)v.y, (double)dB[k], (double)wB[k], eb, true);
+        double mA = (double)v.x, mB = (double)v.y;
+        if (Tk.psi.z) {   // point sources, after the convolution
+          if (ra < st.H) mA += ps_add(Tk, sl.mode, pr.x, c, ra);
+          if (pr.z >= 0 && rb < st.H) mB += ps_add(Tk, sl.mode, pr.z, c, rb);
+        }
+        if (ra < st.H) stat_pixel_dw<T>(st, sl, pr.x, ra, c, mA, (double)dA[k], (double)wA[k], ea, true);
+        if (pr.z >= 0 && rb < st.H) stat_pixel_dw<T>(st, sl, pr.z, rb, c, mB, (double)dB[k], (double)wB[k], eb, true);
       }
     } else if (Tk.kind == TK_MAIN_CONV) {
       for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++) {
         const int io = io0 + rr, ra = pr.y + io, rb = pr.w + io;
         for (int c = threadIdx.x; c < g.W; c += kThreads) {
           const cpx<T> v = out[rr * g.Nc + c + g.hl];
-          if (ra < st.H) stat_pixel<T>(st, sl, pr.x, ra, c, (double)v.x, ea, true);
-          if (pr.z >= 0 && rb < st.H) stat_pixel<T>(st, sl, pr.z, rb, c, (double)v.y, eb, true);
+          double mA = (double)v.x, mB = (double)v.y;
+          if (Tk.psi.z) {
+            if (ra < st.H) mA += ps_add(Tk, sl.mode, pr.x, c, ra);
+            if (pr.z >= 0 && rb < st.H) mB += ps_add(Tk, sl.mode, pr.z, c, rb);
+          }
+          if (ra < st.H) stat_pixel<T>(st, sl, pr.x, ra, c, mA, ea, true);
+          if (pr.z >= 0 && rb < st.H) stat_pixel<T>(st, sl, pr.z, rb, c, mB, eb, true);
         }
       }
     }
@@ -2054,34 +2247,6 @@ __device__ __forceinline__ double adj_input(const Task& Tk, int4 pr, int plane, 
 // Adjoint inputs of the output rows of both planes, placed at plane rows
 // [hb, hb + Hh) and columns [hl, hl + W), then the forward row FFT (rows1 rows per
 // block, Hh / rows1 blocks per pair).
-template <typename T>
-__global__ __launch_bounds__(256) void k_adj_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
-  extern __shared__ char smem_raw[];
-  int local;
-  const int t = block_task(br, blockIdx.x, local);
-  if (t < 0) return;
-  const Task& Tk = tasks[t];
-  const int pi = local / Tk.n1, lb = local - pi * Tk.n1;
-  const GridGeom g = Tk.g;
-  const int rows = Tk.rows1;
-  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rows * g.Nc;
-  cpx<T>* TW = B + rows * g.Nc;
-  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
-  const int4 pr = Tk.pairs[pi];
-  const int io0 = lb * rows;
-  const int total = rows * g.Nc;
-  for (int i = threadIdx.x; i < total; i += kThreads) {
-    const int rr = i / g.Nc, j = i - rr * g.Nc;
-    A[i] = cpx<T>{(T)adj_input(Tk, pr, 0, io0 + rr, j), (T)adj_input(Tk, pr, 1, io0 + rr, j)};
-  }
-  __syncthreads();
-  cpx<T>* out = fft_lds<T>(A, B, Tk.pc, rows, g.Nc, TW, +1);
-  cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
-  for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++)
-    for (int j = threadIdx.x; j < g.Nc; j += kThreads) S[(long)(g.hb + io0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
-}
-
 // Block sum of one component's kSlots accumulators into dst[0 .. kSlots) (thread 0's
 // wave order, deterministic); every thread must call it.
 __device__ __forceinline__ void block_sum_slots(double (&acc)[kSlots], double* dst) {
@@ -2101,6 +2266,76 @@ __device__ __forceinline__ void block_sum_slots(double (&acc)[kSlots], double* d
   __syncthreads();   // red is reused by the next call
 }
 
+// Point sources (added after the convolution, so not correlated with the PSF): the
+// block's sums of a * d(point source)/d(slot) over its output positions go to its
+// partial slot al.psSlot0 + block (al.psSlot0 < 0: the model has none).
+template <typename T>
+__device__ __forceinline__ void adj_point_sources(const Task& Tk, int4 pr, int io0, int rows, const AdjLaunch& al,
+                                                  int slot) {
+  if (al.psSlot0 < 0) return;
+  double* dst = al.partials + (long)slot * al.width;
+  for (int i = threadIdx.x; i < al.width; i += kThreads) dst[i] = 0.0;
+  if (!Tk.psi.z) return;   // uniform: the whole block returns
+  const GridGeom& g = Tk.g;
+  const EvalArgs& ea = Tk.ea;
+  const int total = rows * g.Nc;
+  for (int n = 0; n < ea.nComp; n++) {
+    const int type = ea.types[n];
+    if (!IsPointSourceType(type)) continue;
+    const double* v = ea.recs[n].v;
+    double acc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) acc[k] = 0.0;
+    for (int i = threadIdx.x; i < total; i += kThreads) {
+      const int rr = i / g.Nc, j = i - rr * g.Nc, io = io0 + rr;
+      if (io >= g.Hh || j < g.hl || j >= g.hl + g.W) continue;
+#pragma unroll
+      for (int pl = 0; pl < 2; pl++) {
+        const double x = g.xa + (double)(j - g.hl) * g.frac, y = g.ya + (double)((pl ? pr.w : pr.y) + io) * g.frac;
+        double xp, yp;
+        ps_offsets(type, v, Tk.psi, x, y, xp, yp);
+        if (!(fabs(xp) <= Tk.psi.xBound && fabs(yp) <= Tk.psi.yBound)) continue;
+        const double a = adj_input(Tk, pr, pl, io, j);
+        if (a == 0.0) continue;
+        double gs[kSlots];
+        ps_grad_slots(type, v, Tk.psi, x, y, gs);
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) acc[k] += a * gs[k];
+      }
+    }
+    block_sum_slots(acc, dst + n * kSlots);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_rowfft(const Task* __restrict__ tasks, BlockRanges br, AdjLaunch al) {
+  extern __shared__ char smem_raw[];
+  int local;
+  const int t = block_task(br, blockIdx.x, local);
+  if (t < 0) return;
+  const Task& Tk = tasks[t];
+  const int pi = local / Tk.n1, lb = local - pi * Tk.n1;
+  const GridGeom g = Tk.g;
+  const int rows = Tk.rows1;
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + rows * g.Nc;
+  cpx<T>* TW = B + rows * g.Nc;
+  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  const int4 pr = Tk.pairs[pi];
+  const int io0 = lb * rows;
+  const int total = rows * g.Nc;
+  adj_point_sources<T>(Tk, pr, io0, rows, al, al.psSlot0 + (int)blockIdx.x);
+  for (int i = threadIdx.x; i < total; i += kThreads) {
+    const int rr = i / g.Nc, j = i - rr * g.Nc;
+    A[i] = cpx<T>{(T)adj_input(Tk, pr, 0, io0 + rr, j), (T)adj_input(Tk, pr, 1, io0 + rr, j)};
+  }
+  __syncthreads();
+  cpx<T>* out = fft_lds<T>(A, B, Tk.pc, rows, g.Nc, TW, +1);
+  cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
+  for (int rr = 0; rr < rows && io0 + rr < g.Hh; rr++)
+    for (int j = threadIdx.x; j < g.Nc; j += kThreads) S[(long)(g.hb + io0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
+}
+
 // Slots of component n that have a parameter: x0, y0 and its own parameters.
 __device__ __forceinline__ unsigned comp_slots(const EvalArgs& ea, int n) {
   return ea.types[n] == F_FLATSKY ? 4u : ((1u << (2 + ea.compNP[n])) - 1u);
@@ -2114,6 +2349,10 @@ __device__ __forceinline__ void adj_accumulate(const EvalArgs& ea, const CompRec
   for (int n = 0; n < ea.nComp; n++) {
     const unsigned need = comp_slots(ea, n);
     const int type = ea.types[n];
+    if (IsPointSourceType(type)) {   // their sums come from the row pass; this slot's entries are 0
+      if (threadIdx.x < kSlots) dst[n * kSlots + threadIdx.x] = 0.0;
+      continue;
+    }
     double acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; k++) acc[k] = 0.0;
@@ -2288,6 +2527,7 @@ __global__ __launch_bounds__(256) void k_adj_tail(const Task* __restrict__ tasks
     if (a == 0.0) continue;
     const double x = g.xa + (double)c * g.frac, y = g.ya + (double)r * g.frac;
     for (int n = 0; n < ea.nComp; n++) {
+      if (IsPointSourceType(ea.types[n])) continue;
       const CompRec rv = ea.recs[n];
       double mean[kSlots];
       tail_grad_mean<DISK>(ea.types[n], rv.v, x, y, comp_slots(ea, n), subsampling, lane, mean);
@@ -2499,7 +2739,7 @@ hipError_t launch_adjoint(hipStream_t s, const Task* tasks, int nTasks, const Bl
                           const BlockRanges& br2, int nB2, size_t lds2, const BlockRanges& br3, int nB3, size_t lds3,
                           const long* prefix, long tailBlocks, int sub, bool disk, const AdjLaunch& al, int nSlots,
                           double* out, int hostOut) {
-  if (nB1 > 0) IFG_LAUNCH(true, false, k_adj_rowfft<T>, dim3(nB1), dim3(256), lds1, s, tasks, br1);
+  if (nB1 > 0) IFG_LAUNCH(true, false, k_adj_rowfft<T>, dim3(nB1), dim3(256), lds1, s, tasks, br1, al);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (nB2 > 0) IFG_LAUNCH(nB1 <= 0, false, (k_colconv<T, true>), dim3(nB2), dim3(256), lds2, s, tasks, br2);

@@ -24,8 +24,8 @@ no data, image size from NCOLS/NROWS or the arguments.
 """
 import numpy as np
 
-from . import (MASK_ZERO_IS_BAD, MASK_ZERO_IS_GOOD, PRECISION_FP64, WEIGHTS_ARE_SIGMAS, WEIGHTS_ARE_VARIANCES,
-               WEIGHTS_ARE_WEIGHTS, BatchGPU, ModelObjectGPU, init_genrand)
+from . import (INTERPOLATION, MASK_ZERO_IS_BAD, MASK_ZERO_IS_GOOD, PRECISION_FP64, WEIGHTS_ARE_SIGMAS,
+               WEIGHTS_ARE_VARIANCES, WEIGHTS_ARE_WEIGHTS, BatchGPU, ModelObjectGPU, init_genrand)
 from .config import ConfigError, ReadConfigFile, SplitString
 from .fits import DetermineImageOffset, FitsError, read_image, write_image
 
@@ -67,8 +67,18 @@ def _read(name, what, shape=None):
 
 
 def _add_functions(model, cfg, subsampling):
-    for name, new_set in cfg.functions():
+    method = None
+    for k, (name, new_set) in enumerate(cfg.functions()):
         model.AddFunction(name, new_set)
+        # the first PointSource's "method" chooses the model's PSF interpolation
+        # (core/model_object.cpp:294-305, func_pointsource.cpp:137-170)
+        if name in ("PointSource", "PointSourceRot") and method is None:
+            opt = cfg.optional_params[k] if k < len(cfg.optional_params) else {}
+            m = str(opt.get("method", "bicubic")).lower()
+            if m not in INTERPOLATION or (name == "PointSourceRot" and m == "lanczos3"):
+                raise SetupError("unidentified interpolation type %r for %s" % (opt.get("method"), name))
+            method = INTERPOLATION[m]
+            model.SetupPsfInterpolation(method)
     model.SetSubsampling(subsampling)
     n = model.GetNParams()
     if n != len(cfg.parameters):

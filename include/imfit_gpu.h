@@ -70,8 +70,22 @@ int ifg_set_precision(ifg_model* m, int precision);
 /* AddFunction + DefineFunctionSets (core/model_object.cpp:277-340; name -> class
  * as in core/add_functions.cpp:163-385).  Supported names: "Sersic",
  * "Exponential", "BrokenExponential", "EdgeOnDisk", "FlatSky", "Gaussian", "Moffat",
- * "ExponentialDisk3D", "Sersic_GenEllipse". */
+ * "ExponentialDisk3D", "Sersic_GenEllipse", "PointSource", "PointSourceRot".
+ * PointSource (I_tot) and PointSourceRot (PA, I_tot) are the PSF itself, interpolated
+ * at the source's position and added after the convolution (function_objects/
+ * func_pointsource.cpp, core/model_object.cpp:1183-1218); they need a PSF of at least
+ * 4 x 4 pixels (checked at the first evaluation). */
 int ifg_add_function(ifg_model* m, const char* name, int starts_new_function_set);
+/* ModelObject::SetupPsfInterpolation (core/model_object.cpp:343-380): how point sources
+ * interpolate the PSF -- IFG_INTERP_BICUBIC (default; GSL's 2-D bicubic spline in the
+ * reference), IFG_INTERP_LANCZOS2 or IFG_INTERP_LANCZOS3 (the "method" optional
+ * parameter of the first PointSource in a configuration file).  Oversampled regions
+ * always interpolate their PSF bicubically (core/oversampled_region.cpp:181-185).
+ * Before the first evaluation. */
+#define IFG_INTERP_BICUBIC 1
+#define IFG_INTERP_LANCZOS2 2
+#define IFG_INTERP_LANCZOS3 3
+int ifg_set_psf_interpolation(ifg_model* m, int method);
 /* FunctionObject::SetSubsampling for every component (imfit --no-subsampling) */
 int ifg_set_subsampling(ifg_model* m, int do_subsampling);
 /* GetNParams (core/model_object.cpp:2398) */

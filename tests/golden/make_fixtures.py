@@ -27,6 +27,8 @@ Each fixture records which reference test produced it:
   conv_n5419     do_psfconvolve_tests:11,60 -> tests/convolve_out_n5419.fits
   conv_5x5       do_psfconvolve_tests:8-9 -> tests/psfconvolve_out{1,2}.txt (padded PSF)
   multimfit      do_multimfit_tests.sh:59-85 -> multimfit_textout1a, 1d, 1e, 3a, 3b
+  ptsource       do_imfit_tests:178 -> imfit_textout8a (PointSource + fixed FlatSky on
+                 tests/psf_moffat_fwhm2.fits, PSF tests/psf_moffat_fwhm2_35x35.fits)
   bootstrap_ic3478  do_imfit_tests:133-134 -> bootstrap_output_seed10_tail.dat,
                  bootstrap_output_seed10_2_tail.dat (seed 10, whole image and section)
 """
@@ -172,6 +174,12 @@ def main():
         psf=read_fits(t("psf_moffat_35.fits")),
         psf_osamp=read_fits(t("psf_moffat_35_oversamp3.fits")),
         **flat("t4e3", fit_record(t(lin + "imfit_textout4e3"))),
+    )
+    np.savez_compressed(   # do_imfit_tests:178 (PointSource with bicubic interpolation, L-M)
+        os.path.join(OUT, "ptsource.npz"),
+        data=read_fits(t("psf_moffat_fwhm2.fits")),
+        psf=read_fits(t("psf_moffat_fwhm2_35x35.fits")),
+        **flat("t8a", fit_record(t(lin + "imfit_textout8a"))),
     )
     # --- makeimage goldens (float32 images written by SaveVectorAsImage)
     np.savez_compressed(os.path.join(OUT, "sersic_exp512.npz"),

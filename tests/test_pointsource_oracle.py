@@ -102,3 +102,13 @@ def test_pointsource_needs_a_psf_of_at_least_4x4():   # unittest_model_object.t.
     assert o.SetupPsfInterpolation(1) == -1
     o.AddPSFVector(np.ones((3, 3)))
     assert o.SetupPsfInterpolation(1) == -2
+
+
+def test_textout8a_initial_statistic():
+    """imfit_textout8a (do_imfit_tests:178): PointSource + fixed FlatSky fitted to
+    tests/psf_moffat_fwhm2.fits with PSF tests/psf_moffat_fwhm2_35x35.fits (bicubic,
+    main grid); INITIAL CHI^2 printed with 6 decimals."""
+    d = cases.golden("ptsource")
+    c = cases.Case([("PointSource", True), ("FlatSky", False)], [19, 19, 250000, 100], data=d["data"], psf=d["psf"])
+    o = c.build(oracle.OracleModel(threads=4))
+    assert abs(o.GetFitStatistic(c.params) - float(d["t8a_initial"])) <= 5e-7
