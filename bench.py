@@ -391,12 +391,27 @@ def cutout_benchmark(n=2000, n_plain=100):
             (key, n, dt, n / dt, res["batched_mean_niter_" + key]))
         if prec == imfit_amd.PRECISION_FP64:
             t0 = time.perf_counter()
-            rel = []
+            rel, c_plain_all = [], []
             for k in range(n_plain):
                 x, r = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, jacobian="fd")
                 c_plain, c_batch = models[k].GetFitStatistic(x), models[k].GetFitStatistic(xb[k])
                 rel.append(abs(c_batch - c_plain) / c_plain)
+                c_plain_all.append(c_plain)
             dtp = time.perf_counter() - t0
+            # control: mpfit itself with the exact Jacobian (same start, limits, ftol) against
+            # mpfit with forward differences, and the batch against one model's own
+            # normal-equation LM -- separates the problems' sensitivity to the Jacobian
+            # (several minima, parameters pegged at limits) from anything the batching does
+            rel_ctrl, same_single = [], 0
+            for k in range(n_plain):
+                xa, _ = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, jacobian="analytic")
+                rel_ctrl.append(abs(models[k].GetFitStatistic(xa) - c_plain_all[k]) / c_plain_all[k])
+                xn, _ = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, method="normal")
+                same_single += int(models[k].GetFitStatistic(xn) == models[k].GetFitStatistic(xb[k]))
+            res["control_mpfit_analytic_matched_chi2_1e-9"] = int(sum(v <= 1e-9 for v in rel_ctrl))
+            res["batch_equals_single_model_normal_lm"] = same_single
+            res["batch_lower_chi2_than_fd_mpfit"] = int(sum(1 for k in range(n_plain) if rel[k] > 1e-9 and
+                                                            models[k].GetFitStatistic(xb[k]) < c_plain_all[k]))
             # the normal-equation LM with batched forward differences, one cutout at a time
             t1 = time.perf_counter()
             for k in range(n_plain):
@@ -417,6 +432,10 @@ def cutout_benchmark(n=2000, n_plain=100):
             log("cfg5 plain FD mpfit (fp64, one cutout at a time): %.2f fits/s; same minimum (chi2 within 1e-9) "
                 "on %d of %d sampled cutouts (max rel diff %.2e there), max rel diff overall %.2e" %
                 (n_plain / dtp, len(matched), n_plain, max(matched) if matched else float("nan"), max(rel)))
+            log("cfg5 control: mpfit with the exact Jacobian matches FD mpfit on %d of %d; the batch equals the "
+                "single-model normal-equation LM on %d of %d; of the batch's mismatches %d end lower" %
+                (res["control_mpfit_analytic_matched_chi2_1e-9"], n_plain, same_single, n_plain,
+                 res["batch_lower_chi2_than_fd_mpfit"]))
         del b, models
     return res
 
