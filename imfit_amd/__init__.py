@@ -161,6 +161,44 @@ def init_genrand(seed):
     _LIB.ifg_init_genrand(int(seed))
 
 
+_DEVFN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, _dp, C.c_long, _dp)
+_LIB.ifg_mpfit.argtypes = [C.c_long, C.c_int, _dp, C.c_void_p, C.c_void_p, _DEVFN, C.c_void_p, C.c_void_p]
+_LIB.ifg_mpfit.restype = C.c_int
+
+
+def mpfit(deviates, params, m, limits=None, fixed=None, ftol=1e-8, xtol=0.0, maxiter=1000):
+    """mpfit (ifg_mpfit: MINPACK LMDIF as imfit runs it, forward-difference Jacobian) on
+    `deviates(p) -> m values` from the start `params`, within limits [(lo, hi) or None]
+    and fixed flags.  Returns (params, LMResult).  Exceptions raised by `deviates` end
+    the fit and are re-raised."""
+    x = np.array(params, dtype=np.float64)
+    n = x.size
+    errors = []
+
+    def cb(ctx, npar, p, mm, out):
+        try:
+            v = np.asarray(deviates(np.ctypeslib.as_array(p, shape=(npar,)).copy()), dtype=np.float64)
+            if v.size != mm:
+                raise ValueError("deviates returned %d values, expected %d" % (v.size, mm))
+            np.ctypeslib.as_array(out, shape=(mm,))[:] = v
+            return 0
+        except Exception as e:   # noqa: BLE001 -- handed back to the caller below
+            errors.append(e)
+            return -1
+
+    fn = _DEVFN(cb)
+    pinfo = ModelObjectGPU._pinfo(limits, fixed, n)
+    cfg = LMConfig(ftol, xtol, 0.0, maxiter, 0)
+    res = LMResult()
+    st = _LIB.ifg_mpfit(int(m), n, _ptr(x), C.cast(pinfo, C.c_void_p) if pinfo is not None else None,
+                        C.cast(C.pointer(cfg), C.c_void_p), fn, None, C.cast(C.pointer(res), C.c_void_p))
+    if errors:
+        raise errors[0]
+    if st < 0 and st > -16:
+        raise IfgError("mpfit failed (%d)" % st)
+    return x, res
+
+
 class ModelObjectGPU:
     """ModelObject on one MI355X: setup and evaluation through the C ABI."""
 

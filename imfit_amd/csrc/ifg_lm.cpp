@@ -638,15 +638,12 @@ struct NormalLM {
   }
 };
 
-}  // namespace
-
-extern "C" {
-
-int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
-               int jacobian_mode, ifg_lm_result* res) try {
-  if (!model) return IFG_ERR_ARG;
-  const int npar = ifg_get_nparams(model);
-  const long m = ifg_get_ndata(model);
+// mpfit (solvers/mpfit.cpp:352-1097) on deviates from `dev(x, fvec)` (status < 0:
+// error), with forward-difference columns from `dev` or, when `jac` is given, the
+// columns from `jac(x, ifree, nfree, fjac)` (mpfit's side == 3 hook).
+template <class Dev, class Jac>
+int MpfitRun(int npar, long m, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg, Dev dev,
+             const Jac* jac, ifg_lm_result* res) {
   double ftol = 1e-10, xtol = 1e-10, gtol = 1e-10, stepfactor = 100.0;   // mpfit.cpp:377-388
   int maxiter = 200, maxfev = 0;
   if (cfg) {
@@ -665,7 +662,7 @@ int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, cons
   std::vector<double> diag(nfree), wa1(nfree), wa2(nfree), wa3(nfree), wa4(m);
   std::vector<int> ipvt(nfree);
   int nfev = 0, iter = 1, info = 0;
-  if ((st = ifg_compute_deviates(model, xall, fvec.data())) < 0) return st;
+  if ((st = dev(xall, fvec.data())) < 0) return st;
   nfev++;
   double fnorm = enorm((int)m, fvec.data()), fnorm1 = -1.0, xnorm = -1.0, delta = 0.0, par = 0.0;
   const double orignorm = fnorm * fnorm;
@@ -674,8 +671,8 @@ int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, cons
   const double eps = sqrt(std::max(MACHEP0, MACHEP0));   // epsfcn = MP_MACHEP0
   for (;;) {   // outer loop: Jacobian at the current point
     for (int j = 0; j < nfree; j++) xnew[L.ifree[j]] = x[j];
-    if (jacobian_mode == IFG_LM_JACOBIAN_ANALYTIC) {
-      if ((st = ifg_jacobian(model, xnew.data(), nfree, L.ifree.data(), nullptr, fjac.data())) < 0) return st;
+    if (jac) {
+      if ((st = (*jac)(xnew.data(), L.ifree.data(), nfree, fjac.data())) < 0) return st;
       nfev++;
     } else {   // mp_fdjac2 one-sided differences (mpfit.cpp:1273-1300)
       for (int j = 0; j < nfree; j++) {
@@ -684,7 +681,7 @@ int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, cons
         if (h == 0.0) h = eps;
         if (L.qulim[j] && temp > L.ulim[j] - h) h = -h;
         xnew[L.ifree[j]] = temp + h;
-        if ((st = ifg_compute_deviates(model, xnew.data(), wa4.data())) < 0) return st;
+        if ((st = dev(xnew.data(), wa4.data())) < 0) return st;
         nfev++;
         xnew[L.ifree[j]] = temp;
         double* col = fjac.data() + (size_t)j * m;
@@ -740,7 +737,7 @@ int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, cons
       const double pnorm = enorm(nfree, wa3.data());
       if (iter == 1) delta = std::min(delta, pnorm);
       for (int j = 0; j < nfree; j++) xnew[L.ifree[j]] = wa2[j];
-      if ((st = ifg_compute_deviates(model, xnew.data(), wa4.data())) < 0) return st;
+      if ((st = dev(xnew.data(), wa4.data())) < 0) return st;
       nfev++;
       fnorm1 = enorm((int)m, wa4.data());
       actred = -1.0;
@@ -794,6 +791,21 @@ done:
   for (int j = 0; j < nfree; j++) xall[L.ifree[j]] = x[j];
   Finish(res, info, iter, nfev, nfree, fnorm, fnorm1, orignorm, L, x.data(), pinfo, npar, xall);
   return info;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ifg_fit_lm(ifg_model* model, double* xall, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+               int jacobian_mode, ifg_lm_result* res) try {
+  if (!model) return IFG_ERR_ARG;
+  auto dev = [model](const double* x, double* f) { return ifg_compute_deviates(model, x, f); };
+  auto jac = [model](const double* x, const int* ifree, int nfree, double* fjac) {
+    return ifg_jacobian(model, x, nfree, ifree, nullptr, fjac);
+  };
+  return MpfitRun(ifg_get_nparams(model), ifg_get_ndata(model), xall, pinfo, cfg, dev,
+                  jacobian_mode == IFG_LM_JACOBIAN_ANALYTIC ? &jac : (decltype(jac)*)nullptr, res);
 } catch (...) {
   return ifg::ExceptionStatus(model);
 }
@@ -864,6 +876,19 @@ int ifg_fit_lm_normal_fd(ifg_model* model, double* xall, const ifg_param_info* p
   return S.info;
 } catch (...) {
   return ifg::ExceptionStatus(model);
+}
+
+int ifg_mpfit(long m, int npar, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+              ifg_deviates_fn fn, void* ctx, ifg_lm_result* res) try {
+  if (!params || !fn || m < 1 || npar < 1) return IFG_ERR_ARG;
+  auto dev = [fn, ctx, m, npar](const double* x, double* f) {
+    const int st = fn(ctx, npar, x, m, f);
+    return st < 0 ? IFG_ERR_ARG : 0;
+  };
+  auto noJac = [](const double*, const int*, int, double*) { return 0; };
+  return MpfitRun(npar, m, params, pinfo, cfg, dev, (decltype(noJac)*)nullptr, res);
+} catch (...) {
+  return IFG_ERR_INTERNAL;
 }
 
 int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,

@@ -20,13 +20,22 @@ core/paramvector_processing.cpp:42-104):
 Image-info files (core/imageparams_file_parser.cpp): blocks opened by IMAGE_START,
 with DATA, ERROR, MASK, PSF, GAIN, READNOISE, EXPTIME, NCOMBINED, ORIGINAL_SKY,
 WEIGHT, the flags MASK_ZERO_IS_BAD / ERRORS_ARE_VARIANCES / ERRORS_ARE_WEIGHTS,
-and the image-description parameters with limits or "fixed" (ignored for the
-first image).  Per-image function blocks (FUNCTIONS_START) and oversampled PSFs
-are not supported.
+the image-description parameters with limits or "fixed" (ignored for the first
+image), oversampled PSFs (OVERSAMPLED_PSF, OVERSAMPLE_SCALE, OVERSAMPLED_REGION,
+:499-516), and a block's own functions after FUNCTIONS_START (:536-580): a function
+section of the configuration-file form whose functions exist in that image only,
+in their own function sets after the global ones, with their parameters taken as
+given (no image-description scaling, core/model_object_multimage.cpp: only global
+functions see SetImageParameters) and appended to the global vector image by
+image (core/param_holder.cpp:236-246).
 
-Fits use Levenberg-Marquardt on the normal equations: each image's J^T J and
-J^T r come from the GPU (analytic Jacobians), and are pushed through the
-derivatives of the parameter assembly.
+Fits: mpfit on the images' concatenated deviates (`FitMpfit`, multimfit's L-M:
+the same mpfit code as ifg_fit_lm, driven through ifg_mpfit), or
+Levenberg-Marquardt on the normal equations (`FitLM`), where each image's J^T J
+and J^T r come from the GPU (analytic Jacobians) and are pushed through the
+derivatives of the parameter assembly.  Bootstrap resampling draws every image's
+sample in image order from the one MT19937 stream (ModelObjectMultImage::
+MakeBootstrapSample, core/model_object_multimage.cpp:749-760).
 """
 import math
 import os
@@ -34,8 +43,9 @@ import os
 import numpy as np
 
 from . import (MASK_ZERO_IS_BAD, MASK_ZERO_IS_GOOD, PRECISION_FP64, WEIGHTS_ARE_SIGMAS, WEIGHTS_ARE_VARIANCES,
-               WEIGHTS_ARE_WEIGHTS, BatchGPU, ModelObjectGPU)
-from .config import ChopComment, ConfigError, ParamLimit, ReadConfigFile, SplitString, TrimWhitespace, strtod
+               WEIGHTS_ARE_WEIGHTS, BatchGPU, ModelObjectGPU, init_genrand, mpfit)
+from .config import (ChopComment, ConfigError, ParamLimit, ParseFunctionSection, ReadConfigFile, SplitString,
+                     TrimWhitespace, strtod)
 from .fits import DetermineImageOffset, read_image
 
 N_IMAGE_PARAMS = 5
@@ -54,9 +64,12 @@ ROLES = {
     "FlatSky": (None, (0,), (), ()),
     "ExponentialDisk3D": (0, (2,), (3, 5), ()),
     "EdgeOnDisk": (0, (1,), (2, 4), ()),
+    "PointSource": (None, (0,), (), ()),          # func_pointsource.cpp:185-191
+    "PointSourceRot": (0, (1,), (), ()),          # func_pointsource-rot.cpp:175-182
 }
 NPARAMS = {"Sersic": 5, "Sersic_GenEllipse": 6, "Exponential": 4, "BrokenExponential": 7, "Gaussian": 4,
-           "Moffat": 5, "FlatSky": 1, "ExponentialDisk3D": 6, "EdgeOnDisk": 5}
+           "Moffat": 5, "FlatSky": 1, "ExponentialDisk3D": 6, "EdgeOnDisk": 5, "PointSource": 1,
+           "PointSourceRot": 2}
 
 
 class ImageInfo:
@@ -67,6 +80,8 @@ class ImageInfo:
         self.mask_type, self.error_type = MASK_ZERO_IS_GOOD, WEIGHTS_ARE_SIGMAS
         self.params = [1.0, 0.0, 1.0, 0.0, 0.0]     # PIXEL_SCALE IMAGE_PA FLUX_SCALE X0 Y0
         self.limits = [ParamLimit() for _ in range(N_IMAGE_PARAMS)]
+        self.overpsf, self.overpsf_scale, self.overpsf_region = [], [], []
+        self.functions = None   # the block's own function section (FunctionSection), if any
 
 
 def read_image_info(path, resolve=None):
@@ -76,17 +91,25 @@ def read_image_info(path, resolve=None):
     with open(path) as f:
         lines = f.read().split("\n")
     infos = []
+    func_lines = None   # lines of the current block's function section, with their numbers
     for k, raw in enumerate(lines, start=1):
         line = TrimWhitespace(raw)
         if not line or line[0] == "#":
             continue
         if "IMAGE_START" in line:
+            if func_lines is not None:
+                _finish_functions(infos[-1], func_lines)
+                func_lines = None
             infos.append(ImageInfo())
             continue
-        if "FUNCTIONS_START" in line:
-            raise ConfigError(0, k, "per-image functions (FUNCTIONS_START) are not supported")
         if not infos:
             raise ConfigError(0, k, "image-info line before the first IMAGE_START")
+        if func_lines is not None:
+            func_lines.append((line, k))
+            continue
+        if "FUNCTIONS_START" in line:   # the rest of the block is a function section
+            func_lines = []
+            continue
         info = infos[-1]
         pieces = SplitString(ChopComment(line))
         key = pieces[0]
@@ -124,30 +147,53 @@ def read_image_info(path, resolve=None):
                     lim.limited = [1, 1]
                     lim.limits = [strtod(lo_hi[0])[0], strtod(lo_hi[1])[0]]
             info.limits[j] = lim
-        elif key in ("OVERSAMPLED_PSF", "OVERSAMPLE_SCALE", "OVERSAMPLED_REGION"):
-            raise ConfigError(0, k, "oversampled PSFs in image-info files are not supported")
+        elif key == "OVERSAMPLED_PSF":
+            info.overpsf.append(resolve(pieces[1]))
+        elif key == "OVERSAMPLE_SCALE":
+            info.overpsf_scale.append(int(strtod(pieces[1])[0]))
+        elif key == "OVERSAMPLED_REGION":
+            info.overpsf_region.append(pieces[1])
+    if func_lines is not None:
+        _finish_functions(infos[-1], func_lines)
     if not infos:
         raise ConfigError(0, -1, "no IMAGE_START block in %r" % path)
     return infos
 
 
+def _finish_functions(info, func_lines):
+    """Parse a block's FUNCTIONS_START section (core/imageparams_file_parser.cpp:620-650)."""
+    lines = [ln for ln, _ in func_lines]
+    numbers = [k for _, k in func_lines]
+    info.functions = ParseFunctionSection(lines, True, numbers, checkParameterLimits=True)
+
+
 class ModelObjectMultImageGPU:
-    """A configuration file's model on several images (one GPU ModelObject each)."""
+    """A configuration file's model on several images (one GPU ModelObject each).
+
+    stat: "chi2" (data-based errors, or the images' ERROR files), "model" (errors from
+    the model, imfit --model-errors), "cash" or "mlr"."""
 
     def __init__(self, config, infos, stat="chi2", precision=PRECISION_FP64, device=0):
         self.cfg = config if hasattr(config, "function_names") else ReadConfigFile(config)
         self.infos = infos
         self.n_images = len(infos)
+        self.stat = stat
         self.models, self.offsets = [], []
         for info in infos:
             m = ModelObjectGPU(device, precision)
             for name, new_set in self.cfg.functions():
                 m.AddFunction(name, new_set)
+            if info.functions is not None:   # the image's own functions, new function sets after the global ones
+                starts = set(info.functions.fset_starts)
+                for j, name in enumerate(info.functions.function_names):
+                    m.AddFunction(name, j in starts)
             if info.psf is not None:
                 m.AddPSFVector(read_image(info.psf))
             data = read_image(info.data)
             m.AddImageDataVector(data)
             m.AddImageCharacteristics(info.gain, info.readnoise, info.exptime, info.ncombined, info.original_sky)
+            off = DetermineImageOffset(info.data)
+            self._add_oversampled(m, info, off)
             if info.mask is not None:
                 m.AddMaskVector(read_image(info.mask), info.mask_type)
             if stat == "cash":
@@ -156,12 +202,15 @@ class ModelObjectMultImageGPU:
                 m.UsePoissonMLR()
             elif info.error is not None:
                 m.AddErrorVector(read_image(info.error), info.error_type)
+            elif stat == "model":
+                m.UseModelErrors()
             m.FinalSetupForFitting()
             self.models.append(m)
-            self.offsets.append(DetermineImageOffset(info.data))
+            self.offsets.append(off)
         self.batch = BatchGPU(self.models) if self.n_images > 1 else None
         self.n_image_params = N_IMAGE_PARAMS * (self.n_images - 1)
         self.n_model = len(self.cfg.parameters)
+        self.n_local = [0 if info.functions is None else len(info.functions.parameters) for info in infos]
         # function layout: (name, first own parameter in the model vector)
         self._funcs, off = [], 0
         for k, (name, new_set) in enumerate(self.cfg.functions()):
@@ -171,14 +220,43 @@ class ModelObjectMultImageGPU:
             off += NPARAMS[name]
         self._set_xy = [(i, j) for i, j in self.cfg.xy_index]
 
+    @staticmethod
+    def _add_oversampled(m, info, off):
+        """OVERSAMPLED_PSF / OVERSAMPLE_SCALE / OVERSAMPLED_REGION of one image: one PSF and
+        scale for every region, or one per region (core/multimfit_main.cpp, as imfit's
+        command-line options); regions in the image file's pixels, moved into an image
+        section's frame."""
+        files, scales, regions = info.overpsf, info.overpsf_scale, info.overpsf_region
+        if not files and not regions:
+            return
+        if not files or not regions or len(scales) != len(files):
+            raise ConfigError(0, -1, "oversampled PSF, scale and region must be given together")
+        if len(files) > 1 and len(files) != len(regions):
+            raise ConfigError(0, -1, "one oversampled PSF, or one per region")
+        for k, region in enumerate(regions):
+            q = 0 if len(files) == 1 else k
+            x, y = SplitString(region, ",")
+            x1, x2 = [int(v) for v in SplitString(x, ":")]
+            y1, y2 = [int(v) for v in SplitString(y, ":")]
+            m.AddOversampledPsf(read_image(files[q]), int(scales[q]), x1 - off[0], x2 - off[0], y1 - off[1],
+                                y2 - off[1])
+
     # ---- parameters
     def GetNParams(self):
-        return self.n_image_params + self.n_model
+        return self.n_image_params + self.n_model + sum(self.n_local)
+
+    def _local_start(self, k):
+        """Index of image k's first own (per-image function) parameter in the global vector."""
+        return self.n_image_params + self.n_model + sum(self.n_local[:k])
 
     def initial_parameters(self):
         """The global vector of the files' values, X0/Y0 moved into the images' frames
-        (image sections, core/utilities_multimfit.cpp:14-63)."""
+        (image sections, core/utilities_multimfit.cpp:14-63): image-description
+        parameters, the model's, then each image's own functions' parameters."""
         p = [v for info in self.infos[1:] for v in info.params] + list(self.cfg.parameters)
+        for info in self.infos:
+            if info.functions is not None:
+                p += list(info.functions.parameters)
         return self._to_data_frame(np.array(p, dtype=np.float64))
 
     def limits_and_fixed(self):
@@ -189,6 +267,12 @@ class ModelObjectMultImageGPU:
                 fixed.append(lim.fixed == 1)
         lims += self.cfg.limit_pairs()
         fixed += list(self.cfg.fixed())
+        for info in self.infos:   # core/multimfit_main.cpp:585-612
+            if info.functions is None:
+                continue
+            for lim in info.functions.limits:
+                lims.append((lim.limits[0], lim.limits[1]) if lim.limited[0] and lim.limited[1] else None)
+                fixed.append(lim.fixed == 1)
         shift = self._frame_shift()
         out = [None if l is None else (l[0] - shift[k], l[1] - shift[k]) for k, l in enumerate(lims)]
         return out, np.array(fixed, dtype=bool)
@@ -201,6 +285,13 @@ class ModelObjectMultImageGPU:
         for i, j in self._set_xy:
             s[self.n_image_params + i] = self.offsets[0][0]
             s[self.n_image_params + j] = self.offsets[0][1]
+        for k, info in enumerate(self.infos):   # own functions: in their image's frame
+            if info.functions is None:
+                continue
+            b = self._local_start(k)
+            for i, j in info.functions.xy_index:
+                s[b + i] = self.offsets[k][0]
+                s[b + j] = self.offsets[k][1]
         return s
 
     def _to_data_frame(self, p):
@@ -213,10 +304,21 @@ class ModelObjectMultImageGPU:
         """The parameter vector image k's ModelObject evaluates, and its derivative
         with respect to the global vector (n_model x n_global)."""
         p = np.asarray(p, dtype=np.float64)
+        nI, nM, nG = self.n_image_params, self.n_model, self.GetNParams()
+        M = p[nI:nI + nM].copy()
+        A = np.zeros((nM, nG))
+        A[:, nI:nI + nM] = np.eye(nM)
+        out, A = self._global_for_image(p, k, M, A)
+        nL = self.n_local[k]
+        if nL == 0:
+            return out, A
+        b = self._local_start(k)   # the image's own parameters, as given (param_holder.cpp:236-246)
+        L = np.zeros((nL, nG))
+        L[:, b:b + nL] = np.eye(nL)
+        return np.concatenate([out, p[b:b + nL]]), np.vstack([A, L])
+
+    def _global_for_image(self, p, k, M, A):
         nI, nM = self.n_image_params, self.n_model
-        M = p[nI:].copy()
-        A = np.zeros((nM, nI + nM))
-        A[:, nI:] = np.eye(nM)
         if k == 0:
             return M, A
         base = N_IMAGE_PARAMS * (k - 1)
@@ -288,6 +390,39 @@ class ModelObjectMultImageGPU:
             jtr += Af.T @ b
             stat += st
         return jtj, jtr, stat
+
+    def FitMpfit(self, p0=None, ftol=1e-8, maxiter=1000):
+        """multimfit's L-M: mpfit on the images' concatenated deviates, forward-difference
+        Jacobian, within the files' limits and fixed flags (ftol 1e-8 as multimfit's
+        default).  Returns (parameters in the files' frame, statistic, LMResult)."""
+        p = self.initial_parameters() if p0 is None else np.array(p0, dtype=np.float64)
+        lims, fixed = self.limits_and_fixed()
+        m = sum(int(mo.GetNDataValues()) for mo in self.models)
+        x, res = mpfit(self.ComputeDeviates, p, m, limits=lims, fixed=fixed, ftol=ftol, maxiter=maxiter)
+        return self.to_file_frame(x), self.GetFitStatistic(x), res
+
+    def Bootstrap(self, n, seed, best, ftol=1e-8):
+        """n bootstrap fits from the best-fit vector `best` (data frame): seed the MT19937,
+        switch every image to bootstrap samples, then per iteration draw each image's
+        sample in image order and refit with mpfit (core/bootstrap_errors.cpp:160-250).
+        Returns the successful fits' parameters in the files' frame, one row each."""
+        init_genrand(seed)
+        for mo in self.models:
+            mo.UseBootstrap()
+        lims, fixed = self.limits_and_fixed()
+        m = sum(int(mo.GetNDataValues()) for mo in self.models)
+        rows = []
+        try:
+            for _ in range(n):
+                for mo in self.models:
+                    mo.MakeBootstrapSample()
+                x, res = mpfit(self.ComputeDeviates, best, m, limits=lims, fixed=fixed, ftol=ftol)
+                if res.status > 0:
+                    rows.append(self.to_file_frame(x))
+        finally:
+            for mo in self.models:
+                mo.StopBootstrap()
+        return np.array(rows)
 
     def FitLM(self, p0=None, ftol=1e-8, maxiter=500):
         """Levenberg-Marquardt on the summed statistic within the files' limits and fixed

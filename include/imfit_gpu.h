@@ -263,6 +263,15 @@ typedef struct {
  * the host; params (nParams) is updated in place; returns the mpfit status. */
 int ifg_fit_lm(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                int jacobian_mode, ifg_lm_result* res);
+/* mpfit on deviates supplied by the caller: fn(ctx, npar, params, m, fvec) fills the m
+ * deviates at params and returns a negative value on failure.  Forward-difference
+ * Jacobian (mp_fdjac2, solvers/mpfit.cpp:1241-1300), the same code as ifg_fit_lm.  For
+ * callers whose deviates combine several models, such as ModelObjectMultImage's
+ * ComputeDeviates (core/model_object_multimage.cpp:411-438) in multimfit's L-M
+ * (core/multimfit_main.cpp calling mpfit through solvers/levmar_fit.cpp:54-60). */
+typedef int (*ifg_deviates_fn)(void* ctx, int npar, const double* params, long m, double* fvec);
+int ifg_mpfit(long m, int npar, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
+              ifg_deviates_fn fn, void* ctx, ifg_lm_result* res);
 /* Levenberg-Marquardt on the Gauss-Newton normal equations J^T J, J^T r reduced
  * on the GPU in FP64 (ifg_normal_equations); no counterpart in the reference. */
 int ifg_fit_lm_normal(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,

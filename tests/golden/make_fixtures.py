@@ -260,6 +260,12 @@ FITS_INPUTS = [
     "multimfit_reference/smallDataImage1a.fits", "multimfit_reference/smallDataImage1b.fits",
     "multimfit_reference/smallgauss_5x5_0.fits", "multimfit_reference/smallgauss_5x5_1.fits",
     "multimfit_reference/smallgauss_5x5_sigma.fits",
+    # do_multimfit_tests.sh:70 (two images with oversampled PSFs)
+    "multimfit_reference/oversamp_test4b.fits",
+    # do_imfit_tests:178 (PointSource), do_makeimage_tests:74,239 (oversampled PointSource)
+    "psf_moffat_fwhm2.fits", "psf_moffat_fwhm2_35x35.fits", "oversampled_pointsource.fits",
+    # unit_tests/unittest_psf_interpolators.t.h:20
+    "../unit_tests/psf_gauss_sigma0.5_5.fits",
     # unit_tests/unittest_image_io.t.h:17-25
     "test_emptyhdu.fits", "test_table.fits", "test_multiextension_hdu0empty.fits",
     "test_multiextension_2images.fits", "multiextension_ic3478rss_64x64_twice.fits",
@@ -287,7 +293,12 @@ CONFIG_INPUTS = [
     "multimfit_reference/config_imfit_smalldataimages_with-limits.dat",
     "multimfit_reference/config_imfit_smallgauss.dat", "multimfit_reference/imageinfo_multimfit0.txt",
     "multimfit_reference/imageinfo_multimfit0_with-limits.txt", "multimfit_reference/imageinfo_multimfit3.txt",
-    "multimfit_reference/imageinfo_multimfit3b.txt",
+    "multimfit_reference/imageinfo_multimfit3b.txt", "multimfit_reference/imageinfo_multimfit3c.txt",
+    "multimfit_reference/imageinfo_multimfit3_noerror.txt", "multimfit_reference/imageinfo_multimfit3d.txt",
+    "multimfit_reference/imageinfo_multimfit+osample.txt",
+    # do_imfit_tests:178, do_makeimage_tests:74 (PointSource)
+    "imfit_reference/config_imfit_ptsource.dat", "makeimage_reference/config_makeimage_pointsource-oversample.dat",
+    "makeimage_reference/config_imfit_pointsource-extra.dat",
     # unit_tests/unittest_config_parser.t.h:35-52
     "config_makeimage_sersictest512.dat", "config_makeimage_sersictest512_with_labels.dat",
     "config_imfit_sersictest512_badlimits4.dat",
@@ -312,14 +323,32 @@ def bootstrap_fixture():
 
 
 def multimfit_fixture():
-    """do_multimfit_tests.sh:59-85: multimfit printouts -- the statistic (1a: fit statistic
-    only; 1d, 1e: L-M fits on two 2x2 images, 1e fitting FLUX_SCALE; 3a, 3b: Nelder-Mead
-    fits of a Gaussian on two 5x5 images, 3b on image sections) and the best-fit main
-    model and image-description parameters, in printed order."""
+    """do_multimfit_tests.sh:59-95: multimfit printouts -- the statistic (1a: fit statistic
+    only; 1d, 1e: L-M fits on two 2x2 images, 1e fitting FLUX_SCALE; 3a-3e: Nelder-Mead
+    fits of a Gaussian on two 5x5 images: 3b, 3c on image sections, 3d without noise
+    images, 3e with model errors; 2a: Nelder-Mead, Poisson MLR, two 200x200 images with
+    oversampled PSFs; 3f: L-M, Poisson MLR, with 3g its five seed-10 bootstrap fits --
+    3f and 3g are archived for macOS only, x86_64 taken, whose LM trajectory matches
+    arm64's counts) and the best-fit main model and image-description parameters, in
+    printed order; the L-M runs' NITER, NFEV and initial statistic."""
     def record(name):
-        with open(t("multimfit_reference/" + name)) as f:
+        path = t("multimfit_reference/" + name)
+        with open(path) as f:
             txt = f.read()
-        out = {"chi2": grab(t("multimfit_reference/" + name), r"CHI-SQUARE = ([0-9.]+)")}
+        if "POISSON-MLR STATISTIC" in txt:
+            out = {"chi2": grab(path, r"POISSON-MLR STATISTIC = ([0-9.]+)")}
+        else:
+            out = {"chi2": grab(path, r"CHI-SQUARE = ([0-9.]+)")}
+        if "NITER" in txt:
+            out["niter"] = grab(path, r"NITER = (\d+)", int)
+        if "NFEV" in txt:
+            out["nfev"] = grab(path, r"NFEV = (\d+)", int)
+        m = re.search(r"INITIAL (?:CHI\^2|POISSON-MLR STATISTIC) = ([0-9.]+)", txt)
+        if m:
+            out["initial"] = float(m.group(1))
+        if "Statistics for parameter values from bootstrap" in txt:   # mean and std per printed parameter
+            out["boot_mean_std"] = np.array([[float(a), float(b)] for a, b in
+                                             re.findall(r"\(([-0-9.e+]+) \+/- ([-0-9.e+]+)\)", txt)])
         for key, head in (("model", "# Main model parameters"), ("image", "# Image-description parameters")):
             if head not in txt:
                 continue
@@ -334,8 +363,11 @@ def multimfit_fixture():
             out[key] = np.array(vals)
         return out
     recs = {}
-    for k in ("1a", "1d", "1e", "3a", "3b"):
+    for k in ("1a", "1d", "1e", "3a", "3b", "3c", "3d", "3e", "2a"):
         for name, v in record("multimfit_textout" + k).items():
+            recs["t%s_%s" % (k, name)] = v
+    for k in ("3f", "3g"):
+        for name, v in record("macos_x86_64/multimfit_textout" + k).items():
             recs["t%s_%s" % (k, name)] = v
     np.savez_compressed(os.path.join(OUT, "multimfit.npz"), **recs)
 
