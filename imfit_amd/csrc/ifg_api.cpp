@@ -2476,8 +2476,6 @@ static int SetBootstrapSample(ifg_model* m, std::vector<long>&& idx) {
 
 static int BootstrapReady(ifg_model* m) {
   if (!m->finalDone) return m->fail(IFG_ERR_STATE, "call ifg_final_setup before bootstrap resampling");
-  if (m->modelErrors)
-    return m->fail(IFG_ERR_UNSUPPORTED, "bootstrap resampling with weights from the model is not supported");
   return IFG_OK;
 }
 

@@ -1808,20 +1808,23 @@ __device__ __forceinline__ void stat_pixel_dw(const TaskStat& st, const StatLaun
     acc.s2 += st.dfac[z] * m;
     return;
   }
-  double w = w0;
-  if (st.modelErrors && w != 0.0) {   // UpdateWeightVector (core/model_object.cpp:1345-1381)
+  // UpdateWeightVector (core/model_object.cpp:1345-1381): 1/sigma from the model, times
+  // the uploaded weight (the mask; with bootstrap samples, sqrt of the pixel's draw count)
+  double w = w0, u = 0.0;
+  if (st.modelErrors && w != 0.0) {
     double totalFlux = m + st.sky;
     double noise_squared = totalFlux / st.effGain + st.rnTerm;
-    w = 1.0 / sqrt(noise_squared);
+    u = 1.0 / sqrt(noise_squared);
+    w = w0 * u;
   }
   if (sl.mode == S_CHI2) {
     const double dev = w * (d - m);
     en_add(acc, dev, st.agiant);
     if (st.devOut) st.devOut[(long)vec * st.nData + z] = dev;
     if (st.dfacOut) {
-      // d(dev)/dm = -w, plus the model-error weight's own dependence on m
+      // d(dev)/dm = -w, plus the model-error weight's own dependence on m (du/dm = -u^3/(2 g))
       double f = -w;
-      if (st.modelErrors && w != 0.0) f += -0.5 * w * w * w / st.effGain * (d - m);
+      if (st.modelErrors && w != 0.0) f += -0.5 * w * u * u / st.effGain * (d - m);
       st.dfacOut[z] = (sl.dfacKind == 2) ? 2.0 * dev * f : f;
     }
   } else {   // Cash / Poisson-MLR (core/model_object.cpp:1386-1401,1703-1775)

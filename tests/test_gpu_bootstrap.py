@@ -7,7 +7,8 @@ core/model_object.cpp:2036-2087; BootstrapErrors, core/bootstrap_errors.cpp:150-
     required 10000th output of the default-seeded generator) replays the draws.
   * The statistic: the reference sums over the index list; the GPU weights each
     pixel by how often it was drawn.  Against the oracle's deviates summed over the
-    same index list, within 1e-12.
+    same index list, within 1e-12 -- also with weights from the model
+    (--model-errors), where the draw count multiplies the model-based weight.
   * The archived outcome: imfit --seed 10 --bootstrap 5 on ic3478 (whole image and the
     section [10:64,10:64]) lists the five fitted parameter sets
     (tests/imfit_reference/linux_x86_64/bootstrap_output_seed10*_tail.dat).
@@ -31,7 +32,7 @@ def announce(msg):
     print("[gpu-test]", msg, flush=True)
 
 
-@pytest.mark.parametrize("stat", ["chi2", "mlr"])
+@pytest.mark.parametrize("stat", ["chi2", "mlr", "model"])
 def test_bootstrap_samples_and_statistic(stat):
     """n3073 (mask, 35^2 PSF): the library's samples follow the draw rule (masked pixels
     redrawn), and the statistic equals the oracle's deviates summed over the indices."""
@@ -72,6 +73,15 @@ def test_bootstrap_samples_and_statistic(stat):
         assert np.all(np.abs(dg ** 2 - counts * dev ** 2) <= 1e-12 * counts * scale)
     else:
         assert np.allclose(dg, dev * np.sqrt(counts), rtol=1e-12, atol=1e-12 * np.abs(dev).max())
+    if stat == "model":   # weights from the model, per drawn pixel: the Jacobian carries both factors
+        fvec, J = g.Jacobian(p, [3, 5, 9])
+        for col, j in enumerate([3, 5, 9]):
+            h = 1e-6 * max(abs(p[j]), 1.0)
+            pp, pm = p.copy(), p.copy()
+            pp[j] += h
+            pm[j] -= h
+            fd = (g.ComputeDeviates(pp) - g.ComputeDeviates(pm)) / (2 * h)
+            assert np.linalg.norm(J[:, col] - fd) <= 1e-5 * np.linalg.norm(fd), j
     # a caller's sample, and back to the full data
     g.SetBootstrapIndices(first)
     assert abs(g.GetFitStatistic(p) - float(np.sum(dev[first] ** 2))) <= 1e-12 * want
