@@ -1501,7 +1501,8 @@ int Enqueue(ifg_model* m, Plan& P) {
           "k_tail");
     TIMED("k_eval_rowfft", launch_eval_rowfft<T>(s, tk, map1, P.nB1[c], P.lds1, disk, deriv), "k_eval_rowfft");
     TIMED("k_colconv", launch_colconv<T>(s, tk, map2, P.nB2[c], P.lds2), "k_colconv");
-    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv, !m->doConv),
+    TIMED("k_inverse_stat", launch_inverse_stat<T>(s, tk, map3, P.nB3[c], P.lds3, sl, disk, deriv, !m->doConv,
+                                                   m->hasPointSources()),
           "k_inverse_stat");
     if (reduce)   // a few results go straight into the pinned results (mapped): no copy back
       TIMED("k_final",
@@ -2757,6 +2758,7 @@ struct BatchPlan {
   long tailWaves = 0;
   bool disk = false;
   bool plain = false;   // some model has no PSF: its main grid is evaluated in the inverse pass
+  bool ps = false;      // some model has point sources
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   BatchPlan() = default;
@@ -2856,6 +2858,7 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
     }
     P.disk = P.disk || HeavyVariant(m);
     P.plain = P.plain || !m->doConv;
+    P.ps = P.ps || m->hasPointSources();
     P.recOff[k] = stage;
     stage += (size_t)m->nComp() * sizeof(CompRec);
     P.boxOff[k] = stage;
@@ -3227,7 +3230,7 @@ int EnqueueBatch(ifg_batch* b, BatchPlan& P, const BatchLaunch& L) {
   if ((e = launch_eval_rowfft<T>(s, tk, L.br1, L.nB1, P.lds1, P.disk, deriv)) != hipSuccess)
     return b->hipfail(e, "k_eval_rowfft");
   if ((e = launch_colconv<T>(s, tk, L.br2, L.nB2, P.lds2)) != hipSuccess) return b->hipfail(e, "k_colconv");
-  if ((e = launch_inverse_stat<T>(s, tk, L.br3, L.nB3, P.lds3, sl, P.disk, deriv, P.plain)) != hipSuccess)
+  if ((e = launch_inverse_stat<T>(s, tk, L.br3, L.nB3, P.lds3, sl, P.disk, deriv, P.plain, P.ps)) != hipSuccess)
     return b->hipfail(e, "k_inverse_stat");
   if (!deriv)
     e = launch_final(s, P.jobs_d.as<FinalJob>(), L.jobList, L.nJobs, P.partials_d.as<double>(), mode,

@@ -2017,7 +2017,9 @@ __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs
 // PLAIN: the launch has a no-PSF main grid (TK_MAIN_PLAIN), whose pixels this kernel
 // evaluates itself; without it the kernel carries none of the profile code and needs
 // far fewer registers (more waves per SIMD for the convolved grids).
-template <typename T, bool DISK, bool DERIV, bool PLAIN>
+// PS: some grid of the launch has point sources, added after the convolution by a
+// call (ps_add); without it the call and the registers it reserves are compiled out.
+template <typename T, bool DISK, bool DERIV, bool PLAIN, bool PS>
 __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
                                                       StatLaunch sl) {
   extern __shared__ char smem_raw[];
@@ -2105,7 +2107,7 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
             for (int jj = 0; jj < s; jj++) {
               const cpx<T> x = out[rr * g.Nc + bj * s + jj + g.hl];
               double val = (double)(plane == 0 ? x.x : x.y);
-              if (Tk.psi.z && (plane == 0 || pr.z >= 0))   // point sources, after the convolution
+              if (PS && Tk.psi.z && (plane == 0 || pr.z >= 0))   // point sources, after the convolution
                 val += ps_add(Tk, sl.mode, plane ? pr.z : pr.x, bj * s + jj,
                               (plane ? pr.w : pr.y) + io0 + sr0 + rr);
               v += val;
@@ -2169,7 +2171,7 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
         if (c >= g.W) continue;
         const cpx<T> v = out[c + g.hl];
         double mA = (double)v.x, mB = (double)v.y;
-        if (Tk.psi.z) {   // point sources, after the convolution
+        if (PS && Tk.psi.z) {   // point sources, after the convolution
           if (ra < st.H) mA += ps_add(Tk, sl.mode, pr.x, c, ra);
           if (pr.z >= 0 && rb < st.H) mB += ps_add(Tk, sl.mode, pr.z, c, rb);
         }
@@ -2182,7 +2184,7 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
         for (int c = threadIdx.x; c < g.W; c += kThreads) {
           const cpx<T> v = out[rr * g.Nc + c + g.hl];
           double mA = (double)v.x, mB = (double)v.y;
-          if (Tk.psi.z) {
+          if (PS && Tk.psi.z) {
             if (ra < st.H) mA += ps_add(Tk, sl.mode, pr.x, c, ra);
             if (pr.z >= 0 && rb < st.H) mB += ps_add(Tk, sl.mode, pr.z, c, rb);
           }
@@ -2724,15 +2726,20 @@ hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& b
 
 template <typename T>
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
-                               const StatLaunch& sl, bool disk, bool deriv, bool plain) {
+                               const StatLaunch& sl, bool disk, bool deriv, bool plain, bool ps) {
   if (nBlocks <= 0) return hipSuccess;
-  if (plain) {
-    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  if (ps) {   // point sources: the general variant (plain covers the convolved grids too)
+    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  } else if (plain) {
+    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
   } else {   // convolved grids only: the profile code (the one user of DISK and DERIV) is compiled out
-    IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, false, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
   }
   return hipGetLastError();
 }
@@ -2822,7 +2829,7 @@ hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobLis
   template hipError_t launch_colconv<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t);                 \
   template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t,             \
                                              const StatLaunch&,                                                    \
-                                             bool, bool, bool);                                                    \
+                                             bool, bool, bool, bool);                                              \
   template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
   template hipError_t launch_adjoint<T>(hipStream_t, const Task*, int, const BlockRanges&, int, size_t,             \
                                         const BlockRanges&, int, size_t, const BlockRanges&, int, size_t,           \

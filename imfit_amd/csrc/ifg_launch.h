@@ -25,9 +25,10 @@ hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRange
 template <typename T>
 hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds);
 template <typename T>
-// plain: some task of the launch is a no-PSF main grid (evaluated in this pass)
+// plain: some task of the launch is a no-PSF main grid (evaluated in this pass); ps: some
+// task has point sources (added in this pass, after the convolution)
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
-                               const StatLaunch& sl, bool disk, bool deriv, bool plain);
+                               const StatLaunch& sl, bool disk, bool deriv, bool plain, bool ps);
 // Adjoint pass of one single-image plan (see k_adj_rowfft .. k_adj_final): row pass
 // (nB1 blocks over the convolved grids), column pass, inverse pass (nB3 blocks over
 // every grid, partial slot per block), tail pass (tailBlocks blocks, partial slots
