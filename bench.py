@@ -383,12 +383,27 @@ def cutout_benchmark(n=2000, n_plain=100):
         b.GetFitStatistic(starts)   # plans and graphs
         device_sync()
         t0 = time.perf_counter()
-        xb, rb = b.FitLM(starts, limits=lims, ftol=1e-10)
+        xb, rb = b.FitLM(starts, limits=lims, ftol=1e-10)   # resident on the GPU (the default)
         dt = time.perf_counter() - t0
         res["batched_fits_per_s_" + key] = n / dt
         res["batched_mean_niter_" + key] = float(np.mean([r.niter for r in rb]))
-        log("cfg5 batched normal-equation LM (%s): %d fits in %.2f s -> %.1f fits/s (mean niter %.1f)" %
+        log("cfg5 batched normal-equation LM, resident (%s): %d fits in %.2f s -> %.1f fits/s (mean niter %.1f)" %
             (key, n, dt, n / dt, res["batched_mean_niter_" + key]))
+        # the same fits driven from the host (one synchronisation and host LM step per round)
+        b.SetLMResident(False)
+        t0 = time.perf_counter()
+        xh, rh = b.FitLM(starts, limits=lims, ftol=1e-10)
+        dth = time.perf_counter() - t0
+        b.SetLMResident(True)
+        c_res, c_host = b.GetFitStatistic(xb), b.GetFitStatistic(xh)
+        rel_rh = np.abs(c_res - c_host) / c_host
+        res["host_rounds_fits_per_s_" + key] = n / dth
+        res["resident_speedup_vs_host_rounds_" + key] = dth / dt
+        res["resident_matches_host_rounds_chi2_1e-9_" + key] = int(np.sum(rel_rh <= 1e-9))
+        res["resident_lower_chi2_than_host_rounds_" + key] = int(np.sum((rel_rh > 1e-9) & (c_res < c_host)))
+        log("cfg5 host-driven rounds (%s): %.1f fits/s; resident / host rounds %.2fx; same chi^2 (1e-9) on %d of %d, "
+            "resident lower on %d" % (key, n / dth, dth / dt, res["resident_matches_host_rounds_chi2_1e-9_" + key], n,
+                                      res["resident_lower_chi2_than_host_rounds_" + key]))
         if prec == imfit_amd.PRECISION_FP64:
             t0 = time.perf_counter()
             rel, c_plain_all = [], []
@@ -407,9 +422,9 @@ def cutout_benchmark(n=2000, n_plain=100):
                 xa, _ = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, jacobian="analytic")
                 rel_ctrl.append(abs(models[k].GetFitStatistic(xa) - c_plain_all[k]) / c_plain_all[k])
                 xn, _ = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, method="normal")
-                same_single += int(models[k].GetFitStatistic(xn) == models[k].GetFitStatistic(xb[k]))
+                same_single += int(models[k].GetFitStatistic(xn) == models[k].GetFitStatistic(xh[k]))
             res["control_mpfit_analytic_matched_chi2_1e-9"] = int(sum(v <= 1e-9 for v in rel_ctrl))
-            res["batch_equals_single_model_normal_lm"] = same_single
+            res["host_rounds_equal_single_model_normal_lm"] = same_single
             res["batch_lower_chi2_than_fd_mpfit"] = int(sum(1 for k in range(n_plain) if rel[k] > 1e-9 and
                                                             models[k].GetFitStatistic(xb[k]) < c_plain_all[k]))
             # the normal-equation LM with batched forward differences, one cutout at a time
@@ -432,8 +447,8 @@ def cutout_benchmark(n=2000, n_plain=100):
             log("cfg5 plain FD mpfit (fp64, one cutout at a time): %.2f fits/s; same minimum (chi2 within 1e-9) "
                 "on %d of %d sampled cutouts (max rel diff %.2e there), max rel diff overall %.2e" %
                 (n_plain / dtp, len(matched), n_plain, max(matched) if matched else float("nan"), max(rel)))
-            log("cfg5 control: mpfit with the exact Jacobian matches FD mpfit on %d of %d; the batch equals the "
-                "single-model normal-equation LM on %d of %d; of the batch's mismatches %d end lower" %
+            log("cfg5 control: mpfit with the exact Jacobian matches FD mpfit on %d of %d; the host-driven batch "
+                "equals the single-model normal-equation LM on %d of %d; of the batch's mismatches %d end lower" %
                 (res["control_mpfit_analytic_matched_chi2_1e-9"], n_plain, same_single, n_plain,
                  res["batch_lower_chi2_than_fd_mpfit"]))
         del b, models
@@ -505,6 +520,8 @@ def main():
     ap.add_argument("--no-aux", action="store_true", help="skip the other workloads' side measurements")
     ap.add_argument("--aux-only", action="store_true",
                     help="only the 3-D disk and population measurements (profiling runs of those kernels)")
+    ap.add_argument("--cutouts-only", action="store_true",
+                    help="only the many-cutout LM fits (resident and host-driven batches, plain fits)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (profiling runs: no tuning trials in the trace)")
     ap.add_argument("--launch-shapes", default=None,
@@ -521,6 +538,9 @@ def main():
 
     if args.aux_only:
         print(json.dumps(aux_benchmarks(args.steps, fits=False)))
+        return
+    if args.cutouts_only:
+        print(json.dumps(cutout_benchmark()))
         return
     log("building the 1024^2 workload (PSFs and data on the GPU)")
     w = workloads.cfg2()

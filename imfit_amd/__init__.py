@@ -99,6 +99,7 @@ _SIGS["ifg_batch_fit_statistic"] = ([_vp, _dp, _dp], C.c_int)
 _SIGS["ifg_batch_normal_equations"] = ([_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp], C.c_int)
 _SIGS["ifg_batch_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
 _SIGS["ifg_batch_set_graph_mode"] = ([_vp, C.c_int], C.c_int)
+_SIGS["ifg_batch_set_lm_resident"] = ([_vp, C.c_int], C.c_int)
 _SIGS["ifg_autotune"] = ([_vp, _dp, C.c_int], C.c_int)
 _SIGS["ifg_get_launch_shape"] = ([_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int)
 _SIGS["ifg_init_genrand"] = ([C.c_ulong], None)
@@ -605,6 +606,12 @@ class BatchGPU:
 
     def SetGraphMode(self, on):
         return self._chk(_LIB.ifg_batch_set_graph_mode(self.h, int(on)), "SetGraphMode")
+
+    def SetLMResident(self, on):
+        """FitLM's driver: True (default) keeps the fits resident on the GPU (staging,
+        evaluation and LM steps on the device, one host wait per several rounds);
+        False runs the host-driven rounds (the LM step on the host)."""
+        return self._chk(_LIB.ifg_batch_set_lm_resident(self.h, int(on)), "SetLMResident")
 
     def Autotune(self, params):
         """One column-strip width and one rows-per-workgroup multiplier for every model,

@@ -28,48 +28,12 @@
 #include "ifg_guard.h"
 #include "ifg_internal.h"
 #include "ifg_launch.h"
+#include "ifg_lmdev.h"
+#include "ifg_setup.h"
 
 using namespace ifg;
 
 namespace {
-
-const double DEG2RAD = 0.017453292519943295;   // function_objects/func_sersic.cpp:65
-const double LOG2E = 1.4426950408889634073599246810019;
-
-// Calculate_bn (function_objects/helper_funcs.cpp:14-29)
-double Calculate_bn(double n) {
-  double n2 = n * n;
-  if (n > 0.36)
-    return 2 * n - 0.333333333333333 + 0.009876543209876543 / n + 0.0018028610621203215 / n2 +
-           0.00011409410586365319 / (n2 * n) - 7.1510122958919723e-05 / (n2 * n2);
-  return 0.01945 + -0.8902 * n + 10.95 * n2 + -19.67 * n2 * n + 13.43 * n2 * n2;
-}
-
-// d(b_n)/dn of Calculate_bn, for the analytic Sersic derivative
-double Calculate_dbn_dn(double n) {
-  double n2 = n * n;
-  if (n > 0.36)
-    return 2.0 - 0.009876543209876543 / n2 - 2.0 * 0.0018028610621203215 / (n2 * n) -
-           3.0 * 0.00011409410586365319 / (n2 * n2) + 4.0 * 7.1510122958919723e-05 / (n2 * n2 * n);
-  return -0.8902 + 2.0 * 10.95 * n + 3.0 * -19.67 * n2 + 4.0 * 13.43 * n2 * n;
-}
-
-int NParamsOf(int type) {
-  switch (type) {
-    case F_SERSIC: return 5;
-    case F_EXP: return 4;
-    case F_FLATSKY: return 1;
-    case F_GAUSS: return 4;
-    case F_MOFFAT: return 5;
-    case F_EXPDISK3D: return 6;
-    case F_GENSERSIC: return 6;
-    case F_BROKENEXP: return 7;
-    case F_EDGEONDISK: return 5;
-    case F_POINTSOURCE: return 1;      // I_tot (func_pointsource.cpp:48)
-    case F_POINTSOURCEROT: return 2;   // PA, I_tot (func_pointsource-rot.cpp)
-  }
-  return 0;
-}
 
 int TypeOf(const char* name) {
   std::string s(name ? name : "");
@@ -85,108 +49,6 @@ int TypeOf(const char* name) {
   if (s == "PointSource") return F_POINTSOURCE;
   if (s == "PointSourceRot") return F_POINTSOURCEROT;
   return 0;
-}
-
-bool Subsampling(int type) {
-  return type == F_SERSIC || type == F_EXP || type == F_GAUSS || type == F_MOFFAT || type == F_GENSERSIC ||
-         type == F_BROKENEXP || type == F_EDGEONDISK;
-}
-
-// FunctionObject::Setup for one component (func_sersic.cpp:109-127, func_exp.cpp:104-118,
-// func_broken-exp.cpp:88-111,
-// func_gaussian.cpp:104-121, func_moffat.cpp:111-132, func_flatsky.cpp:85-90,
-// func_expdisk3d.cpp:149-172, func_gen-sersic.cpp:112-133).
-void SetupRec(int type, const double* p, double x0, double y0, CompRec& R) {
-  std::memset(&R, 0, sizeof(R));
-  double* v = R.v;
-  v[0] = x0;
-  v[1] = y0;
-  if (type == F_FLATSKY) {
-    v[5] = p[0];
-    return;
-  }
-  if (type == F_POINTSOURCE) {   // PointSource::Setup (func_pointsource.cpp:198-203)
-    v[2] = 1.0;
-    v[5] = p[0];
-    return;
-  }
-  if (type == F_POINTSOURCEROT) {   // PointSourceRot::Setup (func_pointsource-rot.cpp:188-198)
-    const double PA_rad = (p[0] + 90.0) * DEG2RAD;
-    v[2] = cos(PA_rad);
-    v[3] = sin(PA_rad);
-    v[5] = p[1];
-    return;
-  }
-  if (type == F_EDGEONDISK) {   // EdgeOnDisk::Setup (func_edge-on-disk.cpp:95-116): PA, L_0, h, n, z_0
-    const double PA = p[0], L_0 = p[1], h = p[2], n = p[3], z_0 = p[4];
-    const double PA_rad = (PA + 90.0) * DEG2RAD;
-    const double alpha = 2.0 / n;
-    v[2] = cos(PA_rad); v[3] = sin(PA_rad); v[4] = 1.0; v[15] = 1.0;
-    v[5] = 2.0 * h * L_0; v[6] = h; v[7] = n; v[8] = alpha; v[9] = z_0; v[10] = alpha * z_0;
-    v[11] = pow(2.0, alpha); v[13] = L_0;
-    return;
-  }
-  if (type == F_EXPDISK3D) {   // ExponentialDisk3D::Setup (func_expdisk3d.cpp:149-172)
-    const double PA = p[0], inc = p[1], J_0 = p[2], h = p[3], n = p[4], z_0 = p[5];
-    const double PA_rad = (PA + 90.0) * DEG2RAD, inc_rad = inc * DEG2RAD;
-    const double alpha = 2.0 / n;
-    v[2] = cos(PA_rad); v[3] = sin(PA_rad);
-    v[4] = cos(inc_rad); v[7] = sin(inc_rad);
-    v[5] = J_0; v[6] = h; v[8] = alpha; v[9] = z_0; v[10] = alpha * z_0; v[11] = pow(2.0, alpha);
-    v[12] = 20.0 * h;   // INTEGRATION_MULTIPLIER * h (func_expdisk3d.cpp:95,215)
-    v[13] = n;
-    return;
-  }
-  const double PA = p[0], ell = p[1];
-  const double q = 1.0 - ell;
-  const double PA_rad = (PA + 90.0) * DEG2RAD;
-  v[2] = cos(PA_rad);
-  v[3] = sin(PA_rad);
-  v[4] = q;
-  v[15] = 1.0 / q;
-  switch (type) {
-    case F_SERSIC:
-    case F_GENSERSIC: {   // Sersic_GenEllipse: PA, ell, c0, n, I_e, r_e
-      const int o = (type == F_GENSERSIC) ? 3 : 2;
-      const double n = p[o], I_e = p[o + 1], r_e = p[o + 2];
-      const double bn = Calculate_bn(n);
-      v[5] = I_e; v[6] = r_e; v[7] = bn; v[8] = 1.0 / n;
-      v[9] = n; v[10] = Calculate_dbn_dn(n);
-      if (type == F_GENSERSIC) v[11] = p[2] + 2.0;   // ellExp = c0 + 2.0 (func_gen-sersic.cpp:130)
-      v[12] = 1.0 / (r_e * r_e); v[13] = 0.5 / n; v[14] = -bn * LOG2E;
-      break;
-    }
-    case F_EXP: {
-      const double I_0 = p[2], h = p[3];
-      v[5] = I_0; v[6] = h;
-      v[12] = -LOG2E / h;
-      break;
-    }
-    case F_BROKENEXP: {   // PA, ell, I_0, h1, h2, r_break, alpha
-      const double I_0 = p[2], h1 = p[3], h2 = p[4], r_b = p[5], alpha = p[6];
-      const double exponent = (1.0 / alpha) * (1.0 / h1 - 1.0 / h2);
-      const double S = pow((1.0 + exp(-alpha * r_b)), (-exponent));
-      v[5] = I_0 * S; v[6] = h1; v[7] = h2; v[8] = r_b; v[9] = alpha; v[10] = exponent;
-      v[11] = r_b / h2 - r_b / h1;
-      v[12] = -LOG2E / h1; v[13] = S; v[14] = log1p(exp(-alpha * r_b));
-      break;
-    }
-    case F_GAUSS: {
-      const double I_0 = p[2], sigma = p[3];
-      v[5] = I_0; v[6] = sigma; v[7] = 2.0 * sigma * sigma;
-      v[12] = -LOG2E / v[7];
-      break;
-    }
-    case F_MOFFAT: {
-      const double I_0 = p[2], fwhm = p[3], beta = p[4];
-      const double exponent = pow(2.0, 1.0 / beta);
-      const double alpha = 0.5 * fwhm / sqrt(exponent - 1.0);
-      v[5] = I_0; v[6] = alpha; v[7] = beta;
-      v[9] = fwhm; v[10] = alpha * exponent * log(2.0) / (2.0 * beta * beta * (exponent - 1.0));   // d alpha/d beta
-      v[12] = 1.0 / (alpha * alpha); v[13] = -beta;
-      break;
-    }
-  }
 }
 
 int NextSmooth(int n) {   // smallest 2^a 3^b 5^c >= n
@@ -734,93 +596,10 @@ GridGeom PlainGeom(int W, int H, double xa, double ya, double frac) {
   return g;
 }
 
-// Sub-pixel boxes of one parameter vector on one grid: every component that may
-// subsample contributes the grid pixels within 10 pixels of its centre (r < 10
-// implies |x - x0| < 10 and |y - y0| < 10 since q <= 1), plus one pixel of margin.
-// A generalized ellipse with exponent e > 2 reaches 10 * 2^(1/2 - 1/e) from the
-// centre (the corners of |x'| < 10, |y'| < 10q); with e <= 2 it lies inside the
-// ellipse.  Identical boxes are merged; beyond IFG_MAXBOX the boxes collapse to
-// their union.
+// Sub-pixel boxes of one parameter vector's records on grid g (the shared MakeBoxes).
 int MakeBoxes(const ifg_model* m, const CompRec* recs, const GridGeom& g, int cLo, int cHi, int rLo, int rHi,
               Box* out) {
-  struct B { int c0, c1, r0, r1; };
-  std::vector<B> bs;
-  if (m->doSubsampling) {
-    for (int k = 0; k < m->nComp(); k++) {
-      if (!Subsampling(m->types[k])) continue;
-      const double x0 = recs[k].v[0], y0 = recs[k].v[1];
-      if (!std::isfinite(x0) || !std::isfinite(y0)) continue;
-      if (m->types[k] == F_EDGEONDISK) {   // pixels within 10 of either axis (func_edge-on-disk.cpp:225)
-        for (int axis = 0; axis < 2; axis++) {
-          // |u x_d + w y_d| < 10 with (u, w) = (cos, sin) (along R) or (-sin, cos) (along z):
-          // per grid row, the columns where it holds; their bounding box, widened by a pixel
-          const double c = recs[k].v[2], sn = recs[k].v[3];
-          const double u = axis == 0 ? c : -sn, wv = axis == 0 ? sn : c;
-          const double L = 10.0 + 2.0 * g.frac;
-          int bc0 = cHi + 1, bc1 = cLo - 1, br0 = rHi + 1, br1 = rLo - 1;
-          for (int r = rLo; r <= rHi; r++) {
-            const double yd = g.ya + r * g.frac - y0;
-            double xlo, xhi;
-            if (std::fabs(u) < 1e-12) {
-              if (std::fabs(wv * yd) >= L) continue;
-              xlo = g.xa + cLo * g.frac; xhi = g.xa + cHi * g.frac;
-            } else {
-              const double a = (-L - wv * yd) / u + x0, b = (L - wv * yd) / u + x0;
-              xlo = std::min(a, b); xhi = std::max(a, b);
-            }
-            const double fc0 = std::max(std::floor((xlo - g.xa) / g.frac) - 1, (double)cLo);
-            const double fc1 = std::min(std::ceil((xhi - g.xa) / g.frac) + 1, (double)cHi);
-            if (fc0 > fc1) continue;
-            bc0 = std::min(bc0, (int)fc0); bc1 = std::max(bc1, (int)fc1);
-            br0 = std::min(br0, r); br1 = std::max(br1, r);
-          }
-          if (bc0 > bc1) continue;
-          B b = {bc0, bc1, std::max(br0 - 1, rLo), std::min(br1 + 1, rHi)};
-          bool dup = false;
-          for (auto& o : bs) dup = dup || (o.c0 == b.c0 && o.c1 == b.c1 && o.r0 == b.r0 && o.r1 == b.r1);
-          if (!dup) bs.push_back(b);
-        }
-        continue;
-      }
-      double rad = 10.0;
-      if (m->types[k] == F_GENSERSIC) {
-        const double e = recs[k].v[11];
-        if (!(e > 0.0)) continue;
-        if (e > 2.0) rad = 10.0 * std::pow(2.0, 0.5 - 1.0 / e);
-      }
-      double fc0 = std::floor((x0 - rad - g.xa) / g.frac) - 1, fc1 = std::ceil((x0 + rad - g.xa) / g.frac) + 1;
-      double fr0 = std::floor((y0 - rad - g.ya) / g.frac) - 1, fr1 = std::ceil((y0 + rad - g.ya) / g.frac) + 1;
-      fc0 = std::max(fc0, (double)cLo); fc1 = std::min(fc1, (double)cHi);
-      fr0 = std::max(fr0, (double)rLo); fr1 = std::min(fr1, (double)rHi);
-      if (fc0 > fc1 || fr0 > fr1) continue;
-      B b = {(int)fc0, (int)fc1, (int)fr0, (int)fr1};
-      bool dup = false;
-      for (auto& o : bs) dup = dup || (o.c0 == b.c0 && o.c1 == b.c1 && o.r0 == b.r0 && o.r1 == b.r1);
-      if (!dup) bs.push_back(b);
-    }
-  }
-  if ((int)bs.size() > IFG_MAXBOX) {
-    B u = bs[0];
-    for (auto& o : bs) {
-      u.c0 = std::min(u.c0, o.c0); u.c1 = std::max(u.c1, o.c1);
-      u.r0 = std::min(u.r0, o.r0); u.r1 = std::max(u.r1, o.r1);
-    }
-    bs.assign(1, u);
-  }
-  int off = 0;
-  for (int b = 0; b < IFG_MAXBOX; b++) {
-    Box& o = out[b];
-    std::memset(&o, 0, sizeof(o));
-    if (b < (int)bs.size()) {
-      o.c0 = bs[b].c0;
-      o.r0 = bs[b].r0;
-      o.w = bs[b].c1 - bs[b].c0 + 1;
-      o.h = bs[b].r1 - bs[b].r0 + 1;
-      o.off = off;
-      off += o.w * o.h;
-    }
-  }
-  return off;
+  return ifg::MakeBoxes(m->doSubsampling ? 1 : 0, m->nComp(), m->types.data(), recs, g, cLo, cHi, rLo, rHi, out);
 }
 
 // Weight of data pixel z as the device uses it.  A bootstrap sample sums each drawn
@@ -2761,6 +2540,10 @@ struct BatchPlan {
   bool ps = false;      // some model has point sources
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the resident LM's device view of this plan (static arrays, uploaded on first use)
+  bool lmReady = false;
+  DevBuf lmRecOff_d, lmBoxOff_d, lmFirstTask_d, lmNGrid_d, lmGeoms_d, lmStrideMax_d, lmC_d, lmJobTiles_d,
+      lmCounts_d, lmOffs_d, lmOutOff_d;
   BatchPlan() = default;
   BatchPlan(const BatchPlan&) = delete;
   BatchPlan& operator=(const BatchPlan&) = delete;
@@ -2779,6 +2562,7 @@ struct ifg_batch {
   std::vector<ifg_model*> models;
   std::vector<std::unique_ptr<BatchPlan>> plans;
   bool useGraphs = true;
+  bool residentLM = true;   // ifg_batch_fit_lm: the GPU-resident LM (else the host-driven rounds)
   ~ifg_batch() {
     plans.clear();
     if (stream) (void)hipStreamDestroy(stream);
@@ -3349,6 +3133,260 @@ int BatchDispatch(ifg_batch* b, int kind, const std::vector<std::vector<int>>* f
 
 }  // namespace
 
+namespace {
+
+// ------------------------------------------------------------------ resident batched LM
+// Grids of the looping launches of a resident round (the block counts stay on the
+// device): the three passes (a multiple of 8: the column pass's XCD assignment),
+// normal-equation tiles, and jobs.
+constexpr int kResidentPassGrid = 8192;
+constexpr int kResidentTileGrid = 4096;
+constexpr int kResidentJobGrid = 1024;
+
+template <typename V>
+int UploadVec(ifg_batch* b, DevBuf& d, const std::vector<V>& v, const char* what) {
+  BCHK(d.ensure(sizeof(V) * std::max<size_t>(1, v.size())), what);
+  if (!v.empty()) BCHK(hipMemcpy(d.p, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice), what);
+  return IFG_OK;
+}
+
+// The device view of plan P for the staging, scan and scatter kernels; the compacted
+// maps, job and tile lists use the plan's buffers for host-compacted launches, which
+// hold the full maps.
+int LMPlanView(ifg_batch* b, BatchPlan& P, LMPlanDev& D) {
+  const int nM = P.nModels, nT = P.nTasks;
+  if (!P.lmReady) {
+    std::vector<long> recOff(nM), boxOff(nM);
+    std::vector<int> c(3 * (size_t)nT);
+    for (int k = 0; k < nM; k++) {
+      recOff[k] = (long)P.recOff[k];
+      boxOff[k] = (long)P.boxOff[k];
+    }
+    for (int t = 0; t < nT; t++) {
+      const Task& Tk = P.tasksHost[t];
+      c[t] = Tk.nPairs * Tk.n1;
+      c[nT + t] = Tk.nPairs * Tk.n2;
+      c[2 * nT + t] = Tk.nPairs * Tk.n3;
+      if (c[nT + t] % 8) return b->fail(IFG_ERR_INTERNAL, "column-pass blocks of a task not a multiple of 8");
+    }
+    int st;
+    if ((st = UploadVec(b, P.lmRecOff_d, recOff, "resident LM: record offsets")) < 0 ||
+        (st = UploadVec(b, P.lmBoxOff_d, boxOff, "resident LM: box offsets")) < 0 ||
+        (st = UploadVec(b, P.lmFirstTask_d, P.firstTask, "resident LM: first tasks")) < 0 ||
+        (st = UploadVec(b, P.lmNGrid_d, P.nGridM, "resident LM: grids")) < 0 ||
+        (st = UploadVec(b, P.lmGeoms_d, P.geoms, "resident LM: geometries")) < 0 ||
+        (st = UploadVec(b, P.lmStrideMax_d, P.strideMax, "resident LM: tail strides")) < 0 ||
+        (st = UploadVec(b, P.lmC_d, c, "resident LM: blocks per task")) < 0 ||
+        (st = UploadVec(b, P.lmJobTiles_d, P.jobTiles, "resident LM: tiles per job")) < 0 ||
+        (st = UploadVec(b, P.lmOutOff_d, P.outOff, "resident LM: output offsets")) < 0)
+      return st;
+    BCHK(P.lmCounts_d.ensure(sizeof(int) * 8), "resident LM: counts");
+    BCHK(P.lmOffs_d.ensure(sizeof(int) * (3 * (size_t)nT + nM)), "resident LM: offsets");
+    P.lmReady = true;
+  }
+  std::memset(&D, 0, sizeof(D));
+  D.stage = P.stage_d.as<char>();
+  D.recOff = P.lmRecOff_d.as<long>();
+  D.boxOff = P.lmBoxOff_d.as<long>();
+  D.firstTask = P.lmFirstTask_d.as<int>();
+  D.nGrid = P.lmNGrid_d.as<int>();
+  D.geoms = P.lmGeoms_d.as<GridGeom>();
+  D.strideMax = P.lmStrideMax_d.as<int>();
+  D.c1 = P.lmC_d.as<int>();
+  D.c2 = D.c1 + nT;
+  D.c3 = D.c2 + nT;
+  D.jac = P.kind == BK_JAC ? 1 : 0;
+  D.jobTiles = D.jac ? P.lmJobTiles_d.as<int>() : nullptr;
+  D.activeOff = (long)P.activeOff;
+  D.activeTaskOff = (long)P.activeTaskOff;
+  D.usedOff = (long)P.usedOff;
+  D.prefixOff = (long)P.prefixOff;
+  D.nModels = nM;
+  D.nTasks = nT;
+  D.m1 = P.cmaps_d.as<int2>();
+  D.m2 = D.m1 + P.nB1;
+  D.m3 = D.m2 + P.nB2;
+  D.jobs = P.cjobs_d.as<int>();
+  D.tiles = D.jac ? P.ctileMap_d.as<int2>() : nullptr;
+  D.counts = P.lmCounts_d.as<int>();
+  D.offs = P.lmOffs_d.as<int>();
+  return IFG_OK;
+}
+
+// One batched evaluation of plan P over the models its staging kernel marked active:
+// every launch loops over the compacted maps whose sizes the scan kernel left on the
+// device; the results stay in the plan's device output.
+template <typename T>
+int EnqueueBatchResident(ifg_batch* b, BatchPlan& P, const LMPlanDev& D) {
+  hipStream_t s = b->stream;
+  ifg_model* m0 = b->models[0];
+  const bool deriv = P.kind == BK_JAC;
+  const int mode = deriv ? S_JAC : m0->statMode();
+  const int sub = m0->doSubsampling ? 1 : 0;
+  const long* prefix_d = reinterpret_cast<const long*>(P.stage_d.as<char>() + P.prefixOff);
+  const Task* tk = P.tasks_d.as<Task>();
+  StatLaunch sl;
+  sl.mode = mode;
+  sl.dfacKind = 1;
+  sl.partials = deriv ? nullptr : P.partials_d.as<double>();
+  BlockRanges r1 = P.br1, r2 = P.br2, r3 = P.br3;
+  r1.map = D.m1;
+  r1.count = D.counts + 0;
+  r2.map = D.m2;
+  r2.count = D.counts + 1;
+  r3.map = D.m3;
+  r3.count = D.counts + 2;
+  const int g1 = std::min(P.nB1, kResidentPassGrid), g2 = std::min(P.nB2, kResidentPassGrid),
+            g3 = std::min(P.nB3, kResidentPassGrid), gj = std::min(P.nModels, kResidentJobGrid);
+  hipError_t e;
+  if ((e = launch_tail<T>(s, tk, P.nTasks, prefix_d, P.tailWaves, sub, P.disk, deriv)) != hipSuccess)
+    return b->hipfail(e, "k_tail");
+  if ((e = launch_eval_rowfft<T>(s, tk, r1, g1, P.lds1, P.disk, deriv)) != hipSuccess)
+    return b->hipfail(e, "k_eval_rowfft");
+  if ((e = launch_colconv<T>(s, tk, r2, g2, P.lds2)) != hipSuccess) return b->hipfail(e, "k_colconv");
+  if ((e = launch_inverse_stat<T>(s, tk, r3, g3, P.lds3, sl, P.disk, deriv, P.plain, P.ps)) != hipSuccess)
+    return b->hipfail(e, "k_inverse_stat");
+  if (!deriv)
+    e = launch_final(s, P.jobs_d.as<FinalJob>(), D.jobs, gj, P.partials_d.as<double>(), mode, P.out_d.as<double>(),
+                     0, nullptr, D.counts + 3);
+  else
+    e = launch_normal(s, P.njobs_d.as<NormalJob>(), D.jobs, gj, D.tiles, std::min(P.nTiles, kResidentTileGrid),
+                      P.maxFree, P.normPart_d.as<double>(), D.counts + 4, D.counts + 3);
+  if (e != hipSuccess) return b->hipfail(e, deriv ? "k_normal" : "k_final");
+  return IFG_OK;
+}
+
+// The batch plan of `kind` (Jacobian plans: for these free parameters), built if
+// missing; plans in `keep` are not evicted to make room.
+template <typename T>
+int GetBatchPlan(ifg_batch* b, int kind, const std::vector<std::vector<int>>* freeIdx,
+                 const std::vector<BatchPlan*>& keep, BatchPlan** out) {
+  for (auto& p : b->plans)
+    if (p->kind == kind && (kind != BK_JAC || p->freeIdx == *freeIdx)) {
+      *out = p.get();
+      return IFG_OK;
+    }
+  if (b->plans.size() >= 6) {
+    BCHK(hipStreamSynchronize(b->stream), "drain before plan eviction");
+    for (size_t k = 0; k < b->plans.size(); k++)
+      if (std::find(keep.begin(), keep.end(), b->plans[k].get()) == keep.end()) {
+        b->plans.erase(b->plans.begin() + k);
+        break;
+      }
+  }
+  b->plans.emplace_back(new BatchPlan());
+  BatchPlan* P = b->plans.back().get();
+  P->kind = kind;
+  if (kind == BK_JAC) P->freeIdx = *freeIdx;
+  const int st = BuildBatchPlan<T>(b, *P);
+  if (st < 0) {
+    b->plans.pop_back();
+    return st;
+  }
+  *out = P;
+  return IFG_OK;
+}
+
+// Pinned mapped status words of a resident run.
+struct PinnedStatus {
+  int* host = nullptr;
+  int* dev = nullptr;
+  ~PinnedStatus() {
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+template <typename T>
+int ResidentRun(ifg_batch* b, const std::vector<std::vector<int>>& freeIdx, std::vector<LMState>& S,
+                std::vector<double>& dArena, const std::vector<int>& iArena, std::vector<double>& X,
+                int roundsPerSync) {
+  const int nM = (int)b->models.size();
+  hipStream_t s = b->stream;
+  BatchPlan *PK = nullptr, *PJ = nullptr, *PS = nullptr;
+  int st;
+  if ((st = GetBatchPlan<T>(b, BK_KEEP, nullptr, {}, &PK)) < 0) return st;
+  if ((st = GetBatchPlan<T>(b, BK_JAC, &freeIdx, {PK}, &PJ)) < 0) return st;
+  if ((st = GetBatchPlan<T>(b, BK_STAT, nullptr, {PK, PJ}, &PS)) < 0) return st;
+  LMPlanDev DK, DJ, DS;
+  if ((st = LMPlanView(b, *PK, DK)) < 0 || (st = LMPlanView(b, *PJ, DJ)) < 0 || (st = LMPlanView(b, *PS, DS)) < 0)
+    return st;
+  std::vector<LMModelDev> models(nM);
+  for (int k = 0; k < nM; k++) {
+    const ifg_model* m = b->models[k];
+    LMModelDev& M = models[k];
+    std::memset(&M, 0, sizeof(M));
+    M.nComp = m->nComp();
+    M.pOff = S[k].pOff;
+    for (int n = 0; n < M.nComp; n++) {
+      M.types[n] = m->types[n];
+      M.fsetStart[n] = m->fsetStart[n];
+    }
+  }
+  DevBuf S_d, dA_d, iA_d, X_d, XT_d, M_d, F_d;
+  if ((st = UploadVec(b, S_d, S, "resident LM: fit states")) < 0 ||
+      (st = UploadVec(b, dA_d, dArena, "resident LM: fit arrays")) < 0 ||
+      (st = UploadVec(b, iA_d, iArena, "resident LM: fit indexes")) < 0 ||
+      (st = UploadVec(b, X_d, X, "resident LM: points")) < 0 ||
+      (st = UploadVec(b, XT_d, X, "resident LM: trial points")) < 0 ||
+      (st = UploadVec(b, M_d, models, "resident LM: models")) < 0)
+    return st;
+  BCHK(F_d.ensure(sizeof(int) * 4), "resident LM: flags");
+  BCHK(hipMemset(F_d.p, 0, sizeof(int) * 4), "resident LM: flags");
+  PinnedStatus status;
+  BCHK(hipHostMalloc(&status.host, sizeof(int) * 4, hipHostMallocDefault), "resident LM: status");
+  BCHK(hipHostGetDevicePointer((void**)&status.dev, status.host, 0), "resident LM: map status");
+  LMBatchDev B;
+  std::memset(&B, 0, sizeof(B));
+  B.S = S_d.as<LMState>();
+  B.dArena = dA_d.as<double>();
+  B.iArena = iA_d.as<int>();
+  B.X = X_d.as<double>();
+  B.XT = XT_d.as<double>();
+  B.models = M_d.as<LMModelDev>();
+  B.nModels = nM;
+  B.doSubsampling = b->models[0]->doSubsampling ? 1 : 0;
+  B.flags = F_d.as<int>();
+  B.status = status.dev;
+  for (const LMState& t : S) B.maxFree = std::max(B.maxFree, t.n);
+  if (lm_step_lds(B.maxFree) > 65536) return b->fail(IFG_ERR_UNSUPPORTED, "resident LM: too many free parameters");
+  const int scatK = std::min(PK->nTasks, 1024), scatJ = std::min(PJ->nTasks, 1024), scatS = std::min(PS->nTasks, 1024);
+  const int K = std::max(1, roundsPerSync);
+  // a round advances every running fit by one trial; each rejected trial at least
+  // halves the step bound, so within about 55 rejections in a row the tolerance tests
+  // end the fit: a fit takes a bounded multiple of its iterations
+  long maxRounds = 1000;
+  for (const LMState& t : S) maxRounds = std::max(maxRounds, 60L * t.maxiter + 1000L);
+  hipError_t e;
+  for (long rounds = 0;; rounds += K) {
+    if (rounds > maxRounds) return b->fail(IFG_ERR_INTERNAL, "resident LM: fits do not end");
+    for (int r = 0; r < K; r++) {
+      // normal equations at the current points of the fits that want them, then the
+      // trial step of every running fit; then the trial statistics and the verdicts
+      if ((e = launch_lm_stage(s, B, DK, 0, scatK)) != hipSuccess) return b->hipfail(e, "k_lm_stage");
+      if ((e = launch_lm_stage(s, B, DJ, 0, scatJ)) != hipSuccess) return b->hipfail(e, "k_lm_stage");
+      if ((st = EnqueueBatchResident<T>(b, *PK, DK)) < 0) return st;
+      if ((st = EnqueueBatchResident<T>(b, *PJ, DJ)) < 0) return st;
+      if ((e = launch_lm_normal(s, B, PK->out_d.as<double>(), PK->lmOutOff_d.as<long>(), PJ->out_d.as<double>(),
+                                PJ->lmOutOff_d.as<long>())) != hipSuccess)
+        return b->hipfail(e, "k_lm_normal");
+      if ((e = launch_lm_stage(s, B, DS, 1, scatS)) != hipSuccess) return b->hipfail(e, "k_lm_stage");
+      if ((st = EnqueueBatchResident<T>(b, *PS, DS)) < 0) return st;
+      if ((e = launch_lm_trial(s, B, PS->out_d.as<double>(), PS->lmOutOff_d.as<long>())) != hipSuccess)
+        return b->hipfail(e, "k_lm_trial");
+    }
+    if ((e = launch_lm_status(s, B)) != hipSuccess) return b->hipfail(e, "k_lm_status");
+    BCHK(hipStreamSynchronize(s), "resident LM rounds");
+    if (status.host[1]) return 1;   // a model's boxes outgrew a plan: the host loop takes over
+    if (status.host[0] == 0) break;
+  }
+  BCHK(hipMemcpy(S.data(), S_d.p, sizeof(LMState) * S.size(), hipMemcpyDeviceToHost), "resident LM: states");
+  BCHK(hipMemcpy(dArena.data(), dA_d.p, sizeof(double) * dArena.size(), hipMemcpyDeviceToHost), "resident LM: arrays");
+  BCHK(hipMemcpy(X.data(), X_d.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost), "resident LM: points");
+  return IFG_OK;
+}
+
+}  // namespace
+
 namespace ifg {
 // Used by the batched LM driver (ifg_lm.cpp): statistics of the active models
 // (stat[k]), and -- with freeIdx -- their normal equations (packed J^T J upper
@@ -3371,6 +3409,23 @@ int BatchEvaluate(ifg_batch* b, const double* params, const int* active, double*
   *normalOff = PJ->outOff.data();
   return IFG_OK;
 }
+// The batched LM resident on the GPU (ifg_lmdev.h) from the packed fit states: on
+// return (0) S, dArena and X hold the ended fits; 1: a model's sub-pixel boxes outgrew
+// its plan (nothing is returned, the caller runs the host-driven loop instead).
+int BatchResidentLM(ifg_batch* b, const std::vector<std::vector<int>>& freeIdx, std::vector<LMState>& S,
+                    std::vector<double>& dArena, const std::vector<int>& iArena, std::vector<double>& X,
+                    int roundsPerSync) {
+  int st = BatchCheck(b);
+  if (st < 0) return st;
+  BCHK(hipSetDevice(b->device), "hipSetDevice");
+  (void)hipGetLastError();
+  if (b->models[0]->precision == IFG_PRECISION_FP64)
+    return ResidentRun<double>(b, freeIdx, S, dArena, iArena, X, roundsPerSync);
+  return ResidentRun<float>(b, freeIdx, S, dArena, iArena, X, roundsPerSync);
+}
+
+bool BatchResidentOn(const ifg_batch* b) { return b->residentLM; }
+
 int BatchSize(const ifg_batch* b) { return (int)b->models.size(); }
 ifg_model* BatchModel(ifg_batch* b, int k) { return b->models[k]; }
 void SetHandleError(const ifg_model* m, const char* msg) noexcept {
@@ -3463,8 +3518,15 @@ int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_
 
 int ifg_batch_set_graph_mode(ifg_batch* b, int on) try {
   if (!b) return IFG_ERR_ARG;
-  if (!b) return IFG_ERR_ARG;
   b->useGraphs = on != 0;
+  return IFG_OK;
+} catch (...) {
+  return ExceptionStatus(b);
+}
+
+int ifg_batch_set_lm_resident(ifg_batch* b, int on) try {
+  if (!b) return IFG_ERR_ARG;
+  b->residentLM = on != 0;
   return IFG_OK;
 } catch (...) {
   return ExceptionStatus(b);

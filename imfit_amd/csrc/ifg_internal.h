@@ -207,6 +207,8 @@ struct BlockRanges {
   int start[IFG_ARGTASKS + 1];
   const int2* map;     // (task, block within the task), task -1 for padding blocks
   const int* active;   // with map: per-task flag of this call (0: the task's model sits out); may be null
+  const int* count;    // with map: the number of map entries, held on the device (a fixed grid loops
+                       // over them); null: one entry per launched block
 };
 
 // Launch-wide statistic settings of the inverse pass.

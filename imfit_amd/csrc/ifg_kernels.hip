@@ -1544,10 +1544,10 @@ __device__ __forceinline__ void stage_pair(const Task& Tk, int4 pr, CompRec (*sR
 // Evaluates two real planes (one complex row: plane A real, plane B imaginary)
 // straight into LDS and runs the forward row FFT there.
 template <typename T, bool DISK, bool DERIV>
-__global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
+__device__ __forceinline__ void eval_rowfft_block(const Task* __restrict__ tasks, const BlockRanges& br, int bx) {
   extern __shared__ char smem_raw[];
   int local;
-  const int t = block_task(br, blockIdx.x, local);
+  const int t = block_task(br, bx, local);
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const int pi = local / Tk.n1, lb = local - pi * Tk.n1;
@@ -1600,6 +1600,19 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
     for (int j = threadIdx.x; j < g.Nc; j += kThreads) S[(long)(ir0 + rr) * g.Nc + j] = out[rr * g.Nc + j];
 }
 
+// Blocks of a launch: one per launched workgroup, or -- when the block count is held
+// on the device (br.count, set by an earlier kernel) -- a fixed grid looping over them.
+__device__ __forceinline__ int launch_blocks(const BlockRanges& br) { return br.count ? *br.count : (int)gridDim.x; }
+
+template <typename T, bool DISK, bool DERIV>
+__global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
+  const int n = launch_blocks(br);
+  for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
+    eval_rowfft_block<T, DISK, DERIV>(tasks, br, bx);
+    __syncthreads();   // the next block reuses the LDS
+  }
+}
+
 // ------------------------------------------------------------------ column pass
 // Forward column FFT of rows [0, Lr) (rest zero), multiply by the PSF spectrum,
 // inverse column FFT, store rows [hb, hb + Hh) -- for a strip of C columns per
@@ -1610,10 +1623,10 @@ __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ ta
 // ADJ: the adjoint of that map (correlation with the PSF): rows [hb, hb + Hh) in,
 // multiplied by the conjugate spectrum, rows [0, Lr) out.
 template <typename T, bool ADJ>
-__global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, BlockRanges br) {
+__device__ __forceinline__ void colconv_block(const Task* __restrict__ tasks, const BlockRanges& br, int bx) {
   extern __shared__ char smem_raw[];
   int local;
-  const int t = block_task(br, blockIdx.x, local);
+  const int t = block_task(br, bx, local);
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const GridGeom g = Tk.g;
@@ -1670,6 +1683,16 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
       for (int i = i0; i < g.Lr; i += rstep) S[(long)i * Nc + col] = X[cc * ld + i];
     else
       for (int i = i0; i < g.Hh; i += rstep) S[(long)(g.hb + i) * Nc + col] = X[cc * ld + g.hb + i];
+  }
+}
+
+// (a looping grid is a multiple of 8 blocks, so that a block's XCD is that of its index)
+template <typename T, bool ADJ>
+__global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, BlockRanges br) {
+  const int n = launch_blocks(br);
+  for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
+    colconv_block<T, ADJ>(tasks, br, bx);
+    __syncthreads();
   }
 }
 
@@ -1990,20 +2013,26 @@ __device__ __forceinline__ void write_job(const FinalJob& J, ENorm a, const ENor
 // of its pair (both planes of a single image split into halves are one vector).
 // done (launches of one job writing to mapped host memory): set to 1 after the
 // result is visible system-wide, so that the host can stop waiting on it alone.
+// jobCount (device, with jobList): the number of jobs, looped over by a fixed grid.
 __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs, const int* __restrict__ jobList,
                                                const double* __restrict__ partials, int mode,
-                                               double* __restrict__ out, int hostOut, unsigned int* done) {
-  const FinalJob& J = jobs[jobList ? jobList[blockIdx.x] : blockIdx.x];
-  if (J.active && !*J.active) return;
-  ENorm a, b;
-  merge_job(J, partials, a, b);
-  if (threadIdx.x == 0) {
-    write_job(J, a, b, mode, out);
-    if (hostOut) __threadfence_system();   // the host reads it once the stream is done
-    if (done) {
-      __hip_atomic_store(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      __threadfence_system();
+                                               double* __restrict__ out, int hostOut, unsigned int* done,
+                                               const int* __restrict__ jobCount) {
+  const int nJobs = jobCount ? *jobCount : (int)gridDim.x;
+  for (int jb = blockIdx.x; jb < nJobs; jb += gridDim.x) {
+    const FinalJob& J = jobs[jobList ? jobList[jb] : jb];
+    if (J.active && !*J.active) continue;
+    ENorm a, b;
+    merge_job(J, partials, a, b);
+    if (threadIdx.x == 0) {
+      write_job(J, a, b, mode, out);
+      if (hostOut) __threadfence_system();   // the host reads it once the stream is done
+      if (done) {
+        __hip_atomic_store(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+      }
     }
+    __syncthreads();   // the block reduction's LDS is reused by the next job
   }
 }
 
@@ -2020,11 +2049,11 @@ __global__ __launch_bounds__(256) void k_final(const FinalJob* __restrict__ jobs
 // PS: some grid of the launch has point sources, added after the convolution by a
 // call (ps_add); without it the call and the registers it reserves are compiled out.
 template <typename T, bool DISK, bool DERIV, bool PLAIN, bool PS>
-__global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
-                                                      StatLaunch sl) {
+__device__ __forceinline__ void inverse_stat_block(const Task* __restrict__ tasks, const BlockRanges& br,
+                                                   const StatLaunch& sl, int bx) {
   extern __shared__ char smem_raw[];
   int local;
-  const int t = block_task(br, blockIdx.x, local);
+  const int t = block_task(br, bx, local);
   if (t < 0) return;
   const Task& Tk = tasks[t];
   const TaskStat& st = Tk.st;
@@ -2195,6 +2224,16 @@ __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ t
     }
   }
   if (sl.partials) finish_block(sl.partials, Tk.slot0 + local, ea, eb);
+}
+
+template <typename T, bool DISK, bool DERIV, bool PLAIN, bool PS>
+__global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
+                                                      StatLaunch sl) {
+  const int n = launch_blocks(br);
+  for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
+    inverse_stat_block<T, DISK, DERIV, PLAIN, PS>(tasks, br, sl, bx);
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------ adjoint (reverse-mode) pass
@@ -2600,10 +2639,10 @@ __global__ __launch_bounds__(256) void k_fd_columns(double* __restrict__ dev, lo
 // Jacobian columns jcol[j * nData + i] and deviates r[i]: each block stages a tile of
 // TP pixels of every column of its job in LDS, each thread forms whole dot products
 // over the tile, and k_normal_final sums a job's tile partials in a fixed order.
-__global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict__ jobs, const int2* __restrict__ map,
-                                                     double* __restrict__ partials) {
+__device__ __forceinline__ void normal_tile_block(const NormalJob* __restrict__ jobs, const int2* __restrict__ map,
+                                                  double* __restrict__ partials, int bx) {
   extern __shared__ char smem_raw[];
-  const int2 e = map[blockIdx.x];   // (job, tile of the job)
+  const int2 e = map[bx];   // (job, tile of the job)
   if (e.x < 0) return;
   const NormalJob& Jb = jobs[e.x];
   if (Jb.active && !*Jb.active) return;
@@ -2643,24 +2682,40 @@ __global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict
   }
 }
 
+// tileCount (device): the number of map entries, looped over by a fixed grid; null:
+// one entry per block.
+__global__ __launch_bounds__(256) void k_normal_tile(const NormalJob* __restrict__ jobs, const int2* __restrict__ map,
+                                                     double* __restrict__ partials, const int* __restrict__ tileCount) {
+  const int n = tileCount ? *tileCount : (int)gridDim.x;
+  for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
+    normal_tile_block(jobs, map, partials, bx);
+    __syncthreads();
+  }
+}
+
 // Block (e, job): entry e of the job's normal equations, summed over its tiles.
 __global__ __launch_bounds__(256) void k_normal_final(const NormalJob* __restrict__ jobs,
                                                       const int* __restrict__ jobList,
-                                                      const double* __restrict__ partials) {
+                                                      const double* __restrict__ partials,
+                                                      const int* __restrict__ jobCount) {
   __shared__ double red[256];
-  const NormalJob& Jb = jobs[jobList ? jobList[blockIdx.y] : blockIdx.y];
-  const int nEnt = Jb.nFree * (Jb.nFree + 1) / 2 + Jb.nFree;
-  const int e = blockIdx.x;
-  if (e >= nEnt || (Jb.active && !*Jb.active)) return;
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < Jb.nTiles; b += kThreads) acc += partials[Jb.part0 + (long)b * nEnt + e];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  const int nJobs = jobCount ? *jobCount : (int)gridDim.y;
+  for (int jy = blockIdx.y; jy < nJobs; jy += gridDim.y) {
+    const NormalJob& Jb = jobs[jobList ? jobList[jy] : jy];
+    const int nEnt = Jb.nFree * (Jb.nFree + 1) / 2 + Jb.nFree;
+    const int e = blockIdx.x;
+    if (e >= nEnt || (Jb.active && !*Jb.active)) continue;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < Jb.nTiles; b += kThreads) acc += partials[Jb.part0 + (long)b * nEnt + e];
+    red[threadIdx.x] = acc;
     __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) Jb.out[e] = red[0];
+    __syncthreads();   // red is reused by the next job
   }
-  if (threadIdx.x == 0) Jb.out[e] = red[0];
 }
 
 }  // namespace ifg
@@ -2793,10 +2848,12 @@ hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, lo
 }
 
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
-                        int mode, double* out, int hostOut, unsigned int* done) {
+                        int mode, double* out, int hostOut, unsigned int* done, const int* jobCount) {
   if (nJobs <= 0) return hipSuccess;
-  if (done && (nJobs != 1 || !hostOut)) return hipErrorInvalidValue;
-  IFG_LAUNCH(true, true, k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut, done);
+  if (done && (nJobs != 1 || !hostOut || jobCount)) return hipErrorInvalidValue;
+  if (jobCount && !jobList) return hipErrorInvalidValue;
+  IFG_LAUNCH(true, true, k_final, dim3(nJobs), dim3(256), 0, s, jobs, jobList, partials, mode, out, hostOut, done,
+             jobCount);
   return hipGetLastError();
 }
 
@@ -2811,15 +2868,16 @@ hipError_t launch_fd_columns(hipStream_t s, double* dev, long nData, int nFree, 
 }
 
 hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobList, int nJobs, const int2* tileMap,
-                         int nTiles, int maxFree, double* partials) {
+                         int nTiles, int maxFree, double* partials, const int* tileCount, const int* jobCount) {
   if (nJobs <= 0 || nTiles <= 0) return hipSuccess;
+  if (jobCount && !jobList) return hipErrorInvalidValue;
   size_t lds = 0;   // the largest job's tile: (nFree + 1) * TP doubles
   for (int f = 1; f <= maxFree; f++) lds = std::max(lds, (size_t)(f + 1) * normal_tile_pixels(f) * sizeof(double));
-  IFG_LAUNCH(true, false, k_normal_tile, dim3(nTiles), dim3(256), lds, s, jobs, tileMap, partials);
+  IFG_LAUNCH(true, false, k_normal_tile, dim3(nTiles), dim3(256), lds, s, jobs, tileMap, partials, tileCount);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int maxEnt = maxFree * (maxFree + 1) / 2 + maxFree;
-  IFG_LAUNCH(false, true, k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, jobList, partials);
+  IFG_LAUNCH(false, true, k_normal_final, dim3(maxEnt, nJobs), dim3(256), 0, s, jobs, jobList, partials, jobCount);
   return hipGetLastError();
 }
 

@@ -48,13 +48,19 @@ hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPl
 hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, long n8);
 // jobList: the jobs of this launch (null: jobs 0 .. nJobs-1); hostOut: out is mapped
 // host memory (each job block ends with a system-scope fence); done (one job, hostOut
-// only, else null): mapped host word set to 1 once the result is visible
+// only, else null): mapped host word set to 1 once the result is visible.  jobCount
+// (device, with jobList): the number of jobs, held on the device; nJobs is then the
+// grid that loops over them.
 hipError_t launch_final(hipStream_t s, const FinalJob* jobs, const int* jobList, int nJobs, const double* partials,
-                        int mode, double* out, int hostOut, unsigned int* done = nullptr);
+                        int mode, double* out, int hostOut, unsigned int* done = nullptr,
+                        const int* jobCount = nullptr);
 // Jacobian columns (dev_j - dev_0) / h_j in place, from nFree + 1 deviate vectors
 hipError_t launch_fd_columns(hipStream_t s, double* dev, long nData, int nFree, const FDSteps& h);
 // pixels per tile of a normal-equations job with nFree columns
 int normal_tile_pixels(int nFree);
+// tileCount / jobCount (device, optional): the numbers of tiles and jobs held on the
+// device; nTiles / nJobs are then the grids that loop over them.
 hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobList, int nJobs, const int2* tileMap,
-                         int nTiles, int maxFree, double* partials);
+                         int nTiles, int maxFree, double* partials, const int* tileCount = nullptr,
+                         const int* jobCount = nullptr);
 }  // namespace ifg

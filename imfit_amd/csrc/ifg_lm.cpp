@@ -21,6 +21,7 @@
 
 #include "../../include/imfit_gpu.h"
 #include "ifg_guard.h"
+#include "ifg_lmdev.h"
 
 // batch access (ifg_api.cpp)
 namespace ifg {
@@ -31,6 +32,11 @@ ifg_model* BatchModel(ifg_batch* b, int k);
 // normal equations of a forward-difference Jacobian with steps h (one batched launch)
 int NormalEquationsFD(ifg_model* m, const double* x, int nFree, const int* ifree, const double* h, double* jtj,
                       double* jtr, double* stat);
+// the GPU-resident batched LM (0: done; 1: the caller runs the host-driven rounds)
+int BatchResidentLM(ifg_batch* b, const std::vector<std::vector<int>>& freeIdx, std::vector<LMState>& S,
+                    std::vector<double>& dArena, const std::vector<int>& iArena, std::vector<double>& X,
+                    int roundsPerSync);
+bool BatchResidentOn(const ifg_batch* b);
 }  // namespace ifg
 
 namespace {
@@ -891,9 +897,98 @@ int ifg_mpfit(long m, int npar, double* params, const ifg_param_info* pinfo, con
   return IFG_ERR_INTERNAL;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The batch's fits resident on the GPU, from the host-initialised NormalLM states (the
+// same configuration, limits and starting points): packs them into device fit states,
+// runs them to the end (ifg_lmdev.h) and fills params and res as the host rounds do.
+// Returns the number of fits that ended without a convergence status, an error, or 1
+// with nothing changed when the host rounds must run instead.
+int ResidentFits(ifg_batch* b, std::vector<NormalLM>& S, const std::vector<int>& status, const std::vector<long>& off,
+                 const std::vector<std::vector<int>>& freeIdx, double* params, const ifg_param_info* pinfo,
+                 ifg_lm_result* res) {
+  const int nM = (int)S.size();
+  std::vector<ifg::LMState> LS(nM);
+  std::vector<double> dA, X(params, params + off[nM]);
+  std::vector<int> iA;
+  for (int k = 0; k < nM; k++) {
+    const NormalLM& s = S[k];
+    ifg::LMState& T = LS[k];
+    std::memset(&T, 0, sizeof(T));
+    const int n = status[k] ? 0 : s.n;
+    T.ftol = s.ftol;
+    T.xtol = s.xtol;
+    T.gtol = s.gtol;
+    T.stepfactor = s.stepfactor;
+    T.maxiter = s.maxiter;
+    T.maxfev = s.maxfev;
+    T.npar = (int)(off[k + 1] - off[k]);
+    T.n = n;
+    T.iter = 1;
+    T.info = status[k];
+    T.needJ = 1;
+    T.firstStep = 1;
+    T.anyLim = (n > 0 && s.L.any) ? 1 : 0;
+    T.fnorm1 = -1.0;
+    T.alpha = 1.0;
+    T.pOff = off[k];
+    T.d = (long)dA.size();
+    T.i = (long)iA.size();
+    // doubles: jtj, Lc (n x n), then g, p, x, wa2, diag, cnorm, sc, y, llim, ulim
+    dA.resize(dA.size() + 2 * (size_t)n * n + 10 * (size_t)n, 0.0);
+    double* xs = dA.data() + T.d + 2 * (long)n * n + 2 * n;
+    double* ll = dA.data() + T.d + 2 * (long)n * n + 8 * n;
+    double* ul = ll + n;
+    for (int j = 0; j < n; j++) {
+      xs[j] = s.x[j];
+      ll[j] = s.L.llim[j];
+      ul[j] = s.L.ulim[j];
+    }
+    for (int j = 0; j < n; j++) iA.push_back(s.L.ifree[j]);
+    for (int j = 0; j < n; j++) iA.push_back(s.L.qllim[j]);
+    for (int j = 0; j < n; j++) iA.push_back(s.L.qulim[j]);
+  }
+  const int st = ifg::BatchResidentLM(b, freeIdx, LS, dA, iA, X, 8);
+  if (st != 0) return st;
+  int notConverged = 0;
+  for (int k = 0; k < nM; k++) {
+    const ifg::LMState& T = LS[k];
+    if (res) std::memset(&res[k], 0, sizeof(res[k]));
+    if (status[k] == 0) {
+      double* xall = params + off[k];
+      for (long i = 0; i < off[k + 1] - off[k]; i++) xall[i] = X[off[k] + i];
+      if (res) {
+        res[k].status = T.info;
+        res[k].niter = T.iter;
+        res[k].nfev = T.nfev;
+        res[k].nfree = T.n;
+        res[k].bestnorm = T.fnorm * T.fnorm;
+        res[k].orignorm = T.orig;
+        int npegged = 0;
+        if (pinfo)
+          for (int i = 0; i < T.npar; i++) {
+            const ifg_param_info& pi = pinfo[off[k] + i];
+            if ((pi.limited[0] && pi.limits[0] == xall[i]) || (pi.limited[1] && pi.limits[1] == xall[i])) npegged++;
+          }
+        res[k].npegged = npegged;
+      }
+    } else if (res) {
+      res[k].status = status[k];
+    }
+    const int sk = status[k] ? status[k] : T.info;
+    if (sk <= 0 || sk == IFG_LM_MAXITER) notConverged++;
+  }
+  return notConverged;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                      ifg_lm_result* res) try {
-  if (!b) return IFG_ERR_ARG;
   if (!b || !params) return IFG_ERR_ARG;
   const int nM = ifg::BatchSize(b);
   if (nM < 1) return IFG_ERR_STATE;
@@ -911,6 +1006,10 @@ int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, 
     } else {
       freeIdx[k] = S[k].L.ifree;
     }
+  }
+  if (ifg::BatchResidentOn(b)) {
+    const int st = ResidentFits(b, S, status, off, freeIdx, params, pinfo, res);
+    if (st != 1) return st;   // 1: a model's boxes outgrew the batch's layout; the host rounds take over
   }
   std::vector<double> X(params, params + off[nM]), stat(nM, 0.0);
   const double* normal = nullptr;
