@@ -1602,11 +1602,16 @@ __device__ __forceinline__ void eval_rowfft_block(const Task* __restrict__ tasks
 
 // Blocks of a launch: one per launched workgroup, or -- when the block count is held
 // on the device (br.count, set by an earlier kernel) -- a fixed grid looping over them.
-__device__ __forceinline__ int launch_blocks(const BlockRanges& br) { return br.count ? *br.count : (int)gridDim.x; }
 
-template <typename T, bool DISK, bool DERIV>
+// LOOP: the looping form (a separate variant: the loop raises the kernel's register
+// count, which the ordinary one-block-per-workgroup launches should not pay for).
+template <typename T, bool DISK, bool DERIV, bool LOOP>
 __global__ __launch_bounds__(256) void k_eval_rowfft(const Task* __restrict__ tasks, BlockRanges br) {
-  const int n = launch_blocks(br);
+  if (!LOOP) {
+    eval_rowfft_block<T, DISK, DERIV>(tasks, br, blockIdx.x);
+    return;
+  }
+  const int n = *br.count;
   for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
     eval_rowfft_block<T, DISK, DERIV>(tasks, br, bx);
     __syncthreads();   // the next block reuses the LDS
@@ -1687,9 +1692,13 @@ __device__ __forceinline__ void colconv_block(const Task* __restrict__ tasks, co
 }
 
 // (a looping grid is a multiple of 8 blocks, so that a block's XCD is that of its index)
-template <typename T, bool ADJ>
+template <typename T, bool ADJ, bool LOOP>
 __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks, BlockRanges br) {
-  const int n = launch_blocks(br);
+  if (!LOOP) {
+    colconv_block<T, ADJ>(tasks, br, blockIdx.x);
+    return;
+  }
+  const int n = *br.count;
   for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
     colconv_block<T, ADJ>(tasks, br, bx);
     __syncthreads();
@@ -2226,10 +2235,14 @@ __device__ __forceinline__ void inverse_stat_block(const Task* __restrict__ task
   if (sl.partials) finish_block(sl.partials, Tk.slot0 + local, ea, eb);
 }
 
-template <typename T, bool DISK, bool DERIV, bool PLAIN, bool PS>
+template <typename T, bool DISK, bool DERIV, bool PLAIN, bool PS, bool LOOP>
 __global__ __launch_bounds__(256) void k_inverse_stat(const Task* __restrict__ tasks, BlockRanges br,
                                                       StatLaunch sl) {
-  const int n = launch_blocks(br);
+  if (!LOOP) {
+    inverse_stat_block<T, DISK, DERIV, PLAIN, PS>(tasks, br, sl, blockIdx.x);
+    return;
+  }
+  const int n = *br.count;
   for (int bx = blockIdx.x; bx < n; bx += gridDim.x) {
     inverse_stat_block<T, DISK, DERIV, PLAIN, PS>(tasks, br, sl, bx);
     __syncthreads();
@@ -2755,6 +2768,21 @@ LaunchEvents& launch_events() {
     else IFG_LAUNCH(true, true, (KERNEL<T, false, false>), __VA_ARGS__);              \
   } while (0)
 
+#define IFG_VARIANT_LOOP(KERNEL, T, disk, deriv, loop, ...)                                   \
+  do {                                                                                        \
+    if (loop) {                                                                               \
+      if (disk && deriv) IFG_LAUNCH(true, true, (KERNEL<T, true, true, true>), __VA_ARGS__);  \
+      else if (disk) IFG_LAUNCH(true, true, (KERNEL<T, true, false, true>), __VA_ARGS__);     \
+      else if (deriv) IFG_LAUNCH(true, true, (KERNEL<T, false, true, true>), __VA_ARGS__);    \
+      else IFG_LAUNCH(true, true, (KERNEL<T, false, false, true>), __VA_ARGS__);              \
+    } else {                                                                                  \
+      if (disk && deriv) IFG_LAUNCH(true, true, (KERNEL<T, true, true, false>), __VA_ARGS__); \
+      else if (disk) IFG_LAUNCH(true, true, (KERNEL<T, true, false, false>), __VA_ARGS__);    \
+      else if (deriv) IFG_LAUNCH(true, true, (KERNEL<T, false, true, false>), __VA_ARGS__);   \
+      else IFG_LAUNCH(true, true, (KERNEL<T, false, false, false>), __VA_ARGS__);             \
+    }                                                                                         \
+  } while (0)
+
 template <typename T>
 hipError_t launch_tail(hipStream_t s, const Task* tasks, int nTasks, const long* prefix, long maxWaves, int sub,
                        bool disk, bool deriv) {
@@ -2768,34 +2796,42 @@ template <typename T>
 hipError_t launch_eval_rowfft(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
                               bool disk, bool deriv) {
   if (nBlocks <= 0) return hipSuccess;
-  IFG_VARIANT(k_eval_rowfft, T, disk, deriv, dim3(nBlocks), dim3(256), lds, s, tasks, br);
+  IFG_VARIANT_LOOP(k_eval_rowfft, T, disk, deriv, br.count != nullptr, dim3(nBlocks), dim3(256), lds, s, tasks, br);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_colconv(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds) {
   if (nBlocks <= 0) return hipSuccess;
-  IFG_LAUNCH(true, true, (k_colconv<T, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br);
+  if (br.count) IFG_LAUNCH(true, true, (k_colconv<T, false, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br);
+  else IFG_LAUNCH(true, true, (k_colconv<T, false, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br);
   return hipGetLastError();
+}
+
+template <typename T, bool LOOP>
+void launch_inverse_variant(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
+                            const StatLaunch& sl, bool disk, bool deriv, bool plain, bool ps) {
+  if (ps) {   // point sources: the general variant (plain covers the convolved grids too)
+    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true, true, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true, true, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true, true, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true, true, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  } else if (plain) {
+    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true, false, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true, false, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true, false, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true, false, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  } else {   // convolved grids only: the profile code (the one user of DISK and DERIV) is compiled out
+    IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, false, false, LOOP>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
+  }
 }
 
 template <typename T>
 hipError_t launch_inverse_stat(hipStream_t s, const Task* tasks, const BlockRanges& br, int nBlocks, size_t lds,
                                const StatLaunch& sl, bool disk, bool deriv, bool plain, bool ps) {
   if (nBlocks <= 0) return hipSuccess;
-  if (ps) {   // point sources: the general variant (plain covers the convolved grids too)
-    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true, true>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-  } else if (plain) {
-    if (disk && deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, true, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else if (disk) IFG_LAUNCH(true, true, (k_inverse_stat<T, true, false, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else if (deriv) IFG_LAUNCH(true, true, (k_inverse_stat<T, false, true, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-    else IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, true, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-  } else {   // convolved grids only: the profile code (the one user of DISK and DERIV) is compiled out
-    IFG_LAUNCH(true, true, (k_inverse_stat<T, false, false, false, false>), dim3(nBlocks), dim3(256), lds, s, tasks, br, sl);
-  }
+  if (br.count) launch_inverse_variant<T, true>(s, tasks, br, nBlocks, lds, sl, disk, deriv, plain, ps);
+  else launch_inverse_variant<T, false>(s, tasks, br, nBlocks, lds, sl, disk, deriv, plain, ps);
   return hipGetLastError();
 }
 
@@ -2807,7 +2843,7 @@ hipError_t launch_adjoint(hipStream_t s, const Task* tasks, int nTasks, const Bl
   if (nB1 > 0) IFG_LAUNCH(true, false, k_adj_rowfft<T>, dim3(nB1), dim3(256), lds1, s, tasks, br1, al);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (nB2 > 0) IFG_LAUNCH(nB1 <= 0, false, (k_colconv<T, true>), dim3(nB2), dim3(256), lds2, s, tasks, br2);
+  if (nB2 > 0) IFG_LAUNCH(nB1 <= 0, false, (k_colconv<T, true, false>), dim3(nB2), dim3(256), lds2, s, tasks, br2);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (nB3 > 0) {
     if (disk) IFG_LAUNCH(nB1 <= 0, false, (k_adj_inverse<T, true>), dim3(nB3), dim3(256), lds3, s, tasks, br3, al);

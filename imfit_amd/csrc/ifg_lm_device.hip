@@ -1,6 +1,6 @@
 // ifg_lm_device.hip -- kernels of the GPU-resident batched Levenberg-Marquardt fit
 // (ifg_lmdev.h): per-round staging and block compaction of a batch plan, and the LM
-// step of every model's fit, one thread per model.
+// step of every model's fit, one wavefront per model.
 //
 // The LM step is the normal-equation LM of the host driver (ifg_lm.cpp, NormalLM):
 // mpfit's step control (solvers/mpfit.cpp:352-1097) -- trust region on ||D p||, the

@@ -6,7 +6,7 @@
 // model's Setup records and sub-pixel boxes from its parameter vector (ifg_setup.h),
 // a scan kernel compacts the active models' blocks, tiles and jobs into the maps the
 // evaluation kernels loop over (their counts stay on the device), the model passes
-// run, and an LM kernel (one thread per model) advances every fit: the normal
+// run, and an LM kernel (one wavefront per model) advances every fit: the normal
 // equations at the current point, the Levenberg-Marquardt parameter, the trial step
 // inside the limits, and the accept/reject decision on the trial statistic.  The host
 // enqueues several rounds and looks at the fits once after them.
