@@ -313,7 +313,39 @@ def aux_benchmarks(steps, fits=True):
     if fits:
         out["cfg5_cutouts"] = cutout_benchmark()
         out["bootstrap_cfg1"] = bootstrap_benchmark()
+        out["de_cfg1"] = de_benchmark()
     return out
+
+
+def de_benchmark(seed=1):
+    """imfit's Differential Evolution (DiffEvolnFit) on the 256^2 Sersic+Exp+sky image
+    (cfg1, 12 parameters, population 96) within the LM limits: each generation's trials
+    in one batched launch, against DESolver's sequential loop (one evaluation per
+    trial); both with the same seed, compared with the LM minimum."""
+    import imfit_amd
+    from imfit_amd import workloads
+    w = workloads.cfg1()
+    w.make_data()
+    lim = w.lm_limits()
+    m = w.fit_model(imfit_amd.PRECISION_FP64)
+    x0 = w.lm_start(seed=1)
+    best, _ = m.FitLM(x0, limits=lim, ftol=1e-10, method="normal")
+    c_lm = m.GetFitStatistic(best)
+    res = {"image": "256x256, no PSF", "params": int(x0.size), "chi2_lm": c_lm}
+    for mode, key in ((imfit_amd.DE_GENERATIONAL, "generational"), (imfit_amd.DE_SEQUENTIAL, "sequential")):
+        m.FitDE(x0, lim, seed=seed, max_generations=2, mode=mode)   # plans (untimed)
+        device_sync()
+        t0 = time.perf_counter()
+        x, r = m.FitDE(x0, lim, seed=seed, mode=mode)
+        dt = time.perf_counter() - t0
+        c = m.GetFitStatistic(x)
+        res[key] = {"s_per_fit": dt, "generations": r.generations, "population": r.population,
+                    "evals_per_s": r.generations * r.population / dt, "status": r.status,
+                    "chi2_rel_vs_lm": (c - c_lm) / c_lm}
+        log("DE (%s) on cfg1: %d generations x %d in %.2f s (%.0f evals/s), chi^2 rel. to LM %.1e" %
+            (key, r.generations, r.population, dt, r.generations * r.population / dt, (c - c_lm) / c_lm))
+    res["speedup_generational"] = res["sequential"]["s_per_fit"] / res["generational"]["s_per_fit"]
+    return res
 
 
 def bootstrap_benchmark(n=256, n_serial=4, seed=10):

@@ -84,6 +84,20 @@ class LMResult(C.Structure):
                 ("nfev", C.c_int), ("nfree", C.c_int), ("npegged", C.c_int)]
 
 
+class DEConfig(C.Structure):
+    """ifg_de_config: imfit's Differential Evolution settings."""
+    _fields_ = [("ftol", C.c_double), ("seed", C.c_ulong), ("max_generations", C.c_int),
+                ("pop_per_param", C.c_int), ("mode", C.c_int)]
+
+
+class DEResult(C.Structure):
+    _fields_ = [("best", C.c_double), ("nfev", C.c_long), ("status", C.c_int), ("generations", C.c_int),
+                ("population", C.c_int)]
+
+
+DE_GENERATIONAL, DE_SEQUENTIAL = 0, 1
+DE_CONVERGED, DE_MAXGEN = 1, 5
+_SIGS["ifg_fit_de"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(DEConfig), C.POINTER(DEResult)], C.c_int)
 _SIGS["ifg_fit_lm"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.c_int, C.POINTER(LMResult)],
                        C.c_int)
 _SIGS["ifg_fit_lm_normal"] = ([_vp, _dp, C.POINTER(ParamInfo), C.POINTER(LMConfig), C.POINTER(LMResult)], C.c_int)
@@ -467,6 +481,21 @@ class ModelObjectGPU:
             st = _LIB.ifg_fit_lm(self.h, _ptr(x), pinfo, C.byref(cfg), mode, C.byref(res))
         if st < 0 and st > -16:
             self._chk(st, "FitLM")
+        return x, res
+
+    def FitDE(self, params, limits, fixed=None, ftol=1e-8, seed=1, max_generations=600,
+              mode=DE_GENERATIONAL, pop_per_param=8):
+        """imfit's Differential Evolution (DiffEvolnFit): every free parameter needs both
+        limits.  mode DE_GENERATIONAL evaluates each generation's trials in one batched
+        launch; DE_SEQUENTIAL is DESolver's own loop (one evaluation per trial).
+        Returns (best params, DEResult)."""
+        x = np.array(self._vec(params), dtype=np.float64)
+        pinfo = self._pinfo(limits, fixed, x.size)
+        cfg = DEConfig(ftol, seed, max_generations, pop_per_param, mode)
+        res = DEResult()
+        st = _LIB.ifg_fit_de(self.h, _ptr(x), pinfo, C.byref(cfg), C.byref(res))
+        if st < 0:
+            self._chk(st, "FitDE")
         return x, res
 
     # ---- instrumentation

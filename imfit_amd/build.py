@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libimfit_gpu.so")
-SOURCES = ["ifg_kernels.hip", "ifg_lm_device.hip", "ifg_api.cpp", "ifg_lm.cpp", "ifg_noalloc.cpp"]
+SOURCES = ["ifg_kernels.hip", "ifg_lm_device.hip", "ifg_api.cpp", "ifg_lm.cpp", "ifg_de.cpp", "ifg_noalloc.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 

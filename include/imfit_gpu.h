@@ -284,6 +284,38 @@ int ifg_fit_lm_normal(ifg_model* m, double* params, const ifg_param_info* pinfo,
 int ifg_fit_lm_normal_fd(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                          ifg_lm_result* res);
 
+/* ---- Differential Evolution (DiffEvolnFit, solvers/diff_evoln_fit.cpp:95-168, and
+ * DESolver, solvers/DESolver.cpp) ------------------------------------------------
+ * imfit's DE: rand-to-best/1/exp, F 0.7, CR 1, 8 x nFree population uniform within the
+ * limits (both limits required for every free parameter), MT19937 seeded with seed
+ * (0: the time), convergence when the best statistic's relative change, taken every
+ * 10 generations from generation 20, is below ftol three times in a row.
+ * IFG_DE_SEQUENTIAL: DESolver::Solve's loop, one evaluation per trial, a better trial
+ * replacing its candidate at once.  IFG_DE_GENERATIONAL (default): each generation's
+ * trials are bred from the population as the generation found it (the same draws in
+ * the same order) and evaluated together by one ifg_fit_statistic_batch launch.
+ * Returns the status (IFG_DE_CONVERGED or IFG_DE_MAXGEN) or an error; params receive
+ * the best vector. */
+#define IFG_DE_GENERATIONAL 0
+#define IFG_DE_SEQUENTIAL 1
+#define IFG_DE_CONVERGED 1
+#define IFG_DE_MAXGEN 5
+#define IFG_DE_ERR_LIMITS (-12)   /* a free parameter without both limits (DiffEvolnFit's -2) */
+typedef struct {
+  double ftol;           /* <= 0: 1e-8 */
+  unsigned long seed;    /* 0: the time */
+  int max_generations;   /* <= 0: 600 */
+  int pop_per_param;     /* <= 0: 8 */
+  int mode;              /* IFG_DE_GENERATIONAL or IFG_DE_SEQUENTIAL */
+} ifg_de_config;
+typedef struct {
+  double best;           /* the best statistic */
+  long nfev;             /* generations x population, as imfit reports it */
+  int status, generations, population;
+} ifg_de_result;
+int ifg_fit_de(ifg_model* m, double* params, const ifg_param_info* pinfo, const ifg_de_config* cfg,
+               ifg_de_result* res);
+
 /* ---- batches of independent models (no counterpart in the reference, which fits
  * one image per run) ------------------------------------------------------------
  * Many ifg_models -- e.g. cutouts of different sizes, each set up through the calls
