@@ -17,6 +17,7 @@ CPU oracle of the same evaluation, timed on this host's cores on a bounded sampl
 import argparse
 import ctypes
 import json
+import re
 import math
 import os
 import sys
@@ -118,24 +119,36 @@ def profiled_traffic(kernel, prec):
     path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2_1024.md")
     if not os.path.exists(path):
         return None
-    names = tuple("ifg::%s<%s%s>" % (kernel, prec, v) for v in ("", ",false,false", ",false,false,false"))
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] in names:
+        if len(f) == 3 and f[0] == "traffic_bytes" and plain_variant(f[1], kernel, prec):
             return float(f[2])
     return None
 
 
-def profiled_valu(kernel, prec, profile="valu_cfg2_1024.md", variants=("", ",false,false", ",false,false,false")):
+def plain_variant(name, kernel, prec, flags=None):
+    """Whether the profiled kernel name is `kernel` in precision `prec` with the given
+    compile-time flags after the precision (default: all false -- the variant of a
+    single image without the 3-D disk, derivatives, point sources or looping grid)."""
+    m = re.match(r"^ifg::(\w+)<(\w+)((?:,\w+)*)>$", name)
+    if not m or m.group(1) != kernel or m.group(2) != prec:
+        return False
+    got = [v for v in m.group(3).split(",") if v]
+    if flags is None:
+        return all(v == "false" for v in got)
+    return got[:len(flags)] == list(flags) and all(v == "false" for v in got[len(flags):])
+
+
+def profiled_valu(kernel, prec, profile="valu_cfg2_1024.md", flags=None):
     """VALU wave instructions per dispatch of `kernel` from profiles/<profile>
-    (rocprofv3 SQ_INSTS_VALU of the workload), or None."""
+    (rocprofv3 SQ_INSTS_VALU of the workload; flags: the compile-time variant after the
+    precision, as plain_variant), or None."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
-    names = tuple("ifg::%s<%s%s>" % (kernel, prec, v) for v in variants)
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "valu_insts" and f[1] in names:
+        if len(f) == 3 and f[0] == "valu_insts" and plain_variant(f[1], kernel, prec, flags):
             return float(f[2])
     return None
 
@@ -258,13 +271,14 @@ def gradient_benchmark(models, p, results, steps):
     return out
 
 
-def valu_roofline(kt, prec, profile, variants):
+def valu_roofline(kt, prec, profile, flags=None):
     """VALU-bound workloads: the kernel with the most event time, its average duration
     and its VALU issue fraction (4 cycles per wave64 instruction on one of the 1024
-    SIMDs at 2.4 GHz, over the duration) from the committed SQ_INSTS_VALU profile."""
+    SIMDs at 2.4 GHz, over the duration) from the committed SQ_INSTS_VALU profile
+    (flags: the dominant kernel's compile-time variant, e.g. ["true"] for the 3-D disk)."""
     dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
-    valu = profiled_valu(dom, prec, profile, variants)
+    valu = profiled_valu(dom, prec, profile, flags if dom in ("k_eval_rowfft", "k_tail") else None)
     return {"bound": "valu", "kernel": dom, "avg_us": avg_s * 1e6,
             "valu_issue_frac": 4.0 * valu / 1024 / 2.4e9 / avg_s if valu is not None else None,
             "kernels_us": {k: 1e3 * t / n for k, (t, n) in kt.items()}}
@@ -288,7 +302,7 @@ def aux_benchmarks(steps, fits=True):
         res4["evals_per_s_" + key] = n / dt
         kt, _ = kernel_times(m, p4, n, 1)
         res4["roofline_" + key] = valu_roofline(kt, "float" if key == "fp32" else "double", "valu_cfg3_cfg4.md",
-                                                ("", ",true,false", ",false,false,false"))
+                                                ["true"])
     out["cfg4_expdisk3d_512"] = res4
     log("cfg4 (3-D disk + bulge, 512^2, PSF 25): %.1f evals/s fp32, %.1f evals/s fp64" %
         (res4["evals_per_s_fp32"], res4["evals_per_s_fp64"]))
@@ -306,8 +320,7 @@ def aux_benchmarks(steps, fits=True):
     kt3, _ = kernel_times(m3, None, 2, 1, call=lambda: m3.GetFitStatisticBatch(pop))
     out["cfg3_population_512"] = {"vectors": int(pop.shape[0]), "s_per_batch": dt,
                                   "chi2_per_s": pop.shape[0] / dt, "finite": bool(np.isfinite(chi).all()),
-                                  "roofline_fp32": valu_roofline(kt3, "float", "valu_cfg3_cfg4.md",
-                                                                 ("", ",false,false", ",false,false,false"))}
+                                  "roofline_fp32": valu_roofline(kt3, "float", "valu_cfg3_cfg4.md")}
     log("cfg3 (4096-vector population, 512^2, PSF 35, FP32): %.3f s per batch, %.0f chi^2/s" %
         (dt, pop.shape[0] / dt))
     if fits:
@@ -393,7 +406,7 @@ def bootstrap_benchmark(n=256, n_serial=4, seed=10):
     return res
 
 
-def cutout_benchmark(n=2000, n_plain=100):
+def cutout_benchmark(n=2000, n_plain=100, precisions=("fp64", "fp32"), host_rounds=True):
     """Many-cutout fits: n cutouts (64-512 px, 1-3 components, 25^2 Moffat PSF) fitted
     all at once by the batched normal-equation LM (analytic Jacobians, J^T J and J^T r
     reduced on the GPU in FP64, every model's step in one batched evaluation), versus
@@ -409,6 +422,8 @@ def cutout_benchmark(n=2000, n_plain=100):
     lims = [w.lm_limits() for w in ws]
     res = {"cutouts": n, "sizes_px": "64-512 log-uniform per side (seed 4)"}
     for prec, key in ((imfit_amd.PRECISION_FP64, "fp64"), (imfit_amd.PRECISION_FP32, "fp32")):
+        if key not in precisions:
+            continue
         models = [w.fit_model(prec) for w in ws]
         b = imfit_amd.BatchGPU(models)
         b.Autotune(starts)
@@ -421,6 +436,9 @@ def cutout_benchmark(n=2000, n_plain=100):
         res["batched_mean_niter_" + key] = float(np.mean([r.niter for r in rb]))
         log("cfg5 batched normal-equation LM, resident (%s): %d fits in %.2f s -> %.1f fits/s (mean niter %.1f)" %
             (key, n, dt, n / dt, res["batched_mean_niter_" + key]))
+        if not host_rounds:
+            del b, models
+            continue
         # the same fits driven from the host (one synchronisation and host LM step per round)
         b.SetLMResident(False)
         t0 = time.perf_counter()
@@ -436,7 +454,7 @@ def cutout_benchmark(n=2000, n_plain=100):
         log("cfg5 host-driven rounds (%s): %.1f fits/s; resident / host rounds %.2fx; same chi^2 (1e-9) on %d of %d, "
             "resident lower on %d" % (key, n / dth, dth / dt, res["resident_matches_host_rounds_chi2_1e-9_" + key], n,
                                       res["resident_lower_chi2_than_host_rounds_" + key]))
-        if prec == imfit_amd.PRECISION_FP64:
+        if prec == imfit_amd.PRECISION_FP64 and n_plain > 0:
             t0 = time.perf_counter()
             rel, c_plain_all = [], []
             for k in range(n_plain):
@@ -550,10 +568,14 @@ def main():
     ap.add_argument("--lm-repeats", type=int, default=3)
     ap.add_argument("--copy-calibration-mb", type=int, default=0)
     ap.add_argument("--no-aux", action="store_true", help="skip the other workloads' side measurements")
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the gradient timings (counter runs of the plain evaluation alone)")
     ap.add_argument("--aux-only", action="store_true",
                     help="only the 3-D disk and population measurements (profiling runs of those kernels)")
     ap.add_argument("--cutouts-only", action="store_true",
                     help="only the many-cutout LM fits (resident and host-driven batches, plain fits)")
+    ap.add_argument("--cutouts-resident-fp64", action="store_true",
+                    help="with --cutouts-only: the resident FP64 batch fit alone (profiling runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (profiling runs: no tuning trials in the trace)")
     ap.add_argument("--launch-shapes", default=None,
@@ -572,7 +594,10 @@ def main():
         print(json.dumps(aux_benchmarks(args.steps, fits=False)))
         return
     if args.cutouts_only:
-        print(json.dumps(cutout_benchmark()))
+        if args.cutouts_resident_fp64:
+            print(json.dumps(cutout_benchmark(n_plain=0, precisions=("fp64",), host_rounds=False)))
+        else:
+            print(json.dumps(cutout_benchmark()))
         return
     log("building the 1024^2 workload (PSFs and data on the GPU)")
     w = workloads.cfg2()
@@ -609,7 +634,7 @@ def main():
     roof = roofline(w, head["kt"], args.precision)
     side_roof = roofline(w, side["kt"], other)
 
-    grad = gradient_benchmark(models, p, results, args.steps)
+    grad = None if args.no_gradient else gradient_benchmark(models, p, results, args.steps)
     lm = None if args.no_lm else lm_benchmark(w, args.lm_repeats)
     aux = None if args.no_aux else aux_benchmarks(args.steps)
 
