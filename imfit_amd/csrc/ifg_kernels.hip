@@ -328,10 +328,14 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 // ------------------------------------------------------------------ image functions
 // Point intensity at radius r (CalculateIntensity of func_sersic.cpp:135-141,
 // func_exp.cpp:127-130, func_gaussian.cpp:123-128, func_moffat.cpp:140-147).
+// Sersic's (r/r_e)^(1/n) is formed as exp(log(r/r_e) / n): the device library's pow
+// carries an extended-precision logarithm that costs more than the pixel's other
+// terms together; the plain form's relative error, |log(r/r_e) / n| ulp, stays
+// below 1e-14 of the intensity over any image.
 __device__ __forceinline__ double intensity_f64(int type, const double* v, double r) {
   switch (type) {
     case F_SERSIC:
-    case F_GENSERSIC: return v[5] * exp(-v[7] * (pow((r / v[6]), v[8]) - 1.0));
+    case F_GENSERSIC: return v[5] * exp(-v[7] * (exp(v[8] * log(r / v[6])) - 1.0));
     case F_EXP: return v[5] * exp(-r / v[6]);
     case F_BROKENEXP:   // func_broken-exp.cpp:118-131, outer form past alpha (r - r_b) = 100
       if (v[9] * (r - v[8]) > 100.0) return v[5] * exp(v[11] - r / v[7]);
