@@ -193,7 +193,31 @@ struct ConvGrid {
   FFTLen* fc = nullptr;   // row transforms (length Nc)
   DevBuf khat64, khat32;
   int rowsK1 = 1, rowsK3 = 1, C64 = 1, C32 = 1;
+  // long transforms in the model's precision: rows (columns) that do not fit two LDS
+  // buffers and the twiddle table run with the second buffer in global scratch
+  bool bigRow = false, bigCol = false;
 };
+
+// Whether length-N transforms of `n` sequences need the second buffer outside LDS (two
+// buffers and the table do not fit); false also when even one buffer does not fit.
+bool LongFFT(int N, int n, size_t elem, int maxLds) {
+  return (2 * (size_t)n * N + N) * elem > (size_t)maxLds && (size_t)n * N * elem <= (size_t)maxLds;
+}
+
+// Scratch elements of the passes of one launch sequence (tasks [first, first + count))
+// for its tasks with long transforms (longT bit 1: rows, bit 2: columns): every
+// workgroup of a pass has a slot of its task's buffer size.
+size_t LongScratchElems(const std::vector<Task>& tasks, const std::vector<char>& longT, size_t first, size_t count,
+                        int nB1, int nB2, int nB3) {
+  size_t e = 0;
+  for (size_t t = first; t < first + count; t++) {
+    const Task& Tk = tasks[t];
+    if (longT[t] & 1)
+      e = std::max(e, std::max((size_t)nB1 * Tk.rows1, (size_t)nB3 * Tk.rows3) * (size_t)Tk.g.Nc);
+    if (longT[t] & 2) e = std::max(e, (size_t)nB2 * Tk.C * (size_t)Tk.ld);
+  }
+  return e;
+}
 
 // The PSF a grid's point sources interpolate, on the device: pixels, then (bicubic)
 // the natural-spline knot derivatives d/dx, d/dy, d2/dxdy, each nC x nR FP64.
@@ -250,6 +274,7 @@ struct Plan {
   DevBuf stage_d, tail_d, pairs_d, spec_d, specR_d, partials_d, out_d, dev_d, model_d, tasks_d;
   DevBuf maps_d, jobs_d, njobs_d, tileMap_d;
   DevBuf aTail_d, adjPart_d;   // adjoint plans: box pixels' adjoint values, per-slot partial sums
+  DevBuf fft_d;                // long transforms: the passes' second FFT buffers, one slot per workgroup
   int adjWidth = 0;            // nComp * IFG_SLOTS
   long adjTailBlocks = 0;
   int adjPsBlocks = 0;         // partial slots of the point sources' sums (the row-pass blocks)
@@ -536,11 +561,12 @@ int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double
   g.frac = frac;
   int maxLds = 65536;
   (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device);
-  // the PSF spectrum is always transformed in FP64: one row (two buffers + twiddles)
-  // or one column tile must fit in a workgroup's LDS
-  const size_t need = std::max((size_t)3 * g.Nc, (size_t)(2 * (g.Nr + 1) + g.Nr)) * sizeof(cpx<double>);
+  // one FP64 row (or column) buffer must fit in a workgroup's LDS (the PSF spectrum is
+  // always transformed in FP64); longer ones than two buffers and the twiddle table
+  // allow run with the second buffer in global scratch
+  const size_t need = (size_t)std::max(g.Nc, g.Nr + 1) * sizeof(cpx<double>);
   if (need > (size_t)maxLds)
-    return m->fail(IFG_ERR_UNSUPPORTED, "padded image side too large for the in-LDS FFT (needs " +
+    return m->fail(IFG_ERR_UNSUPPORTED, "padded image side too large for the FFT (one row needs " +
                                             std::to_string(need) + " bytes of LDS per workgroup)");
   int st = 0;
   cg.fr = GetFFT(m, g.Nr, &st);
@@ -558,27 +584,42 @@ int BuildConv(ifg_model* m, ConvGrid& cg, int W, int H, const std::vector<double
   HIPCHK(cg.khat64.ensure(NN * sizeof(cpx<double>)), "alloc PSF spectrum");
   HIPCHK(cg.khat32.ensure(NN * sizeof(cpx<float>)), "alloc PSF spectrum");
   HIPCHK(hipMemcpy(cg.khat64.p, k.data(), NN * sizeof(cpx<double>), hipMemcpyHostToDevice), "upload PSF");
-  const int rows = RowsPerBlock(g.Nc, sizeof(cpx<double>));
-  HIPCHK(launch_rowfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fc->plan,
-                               cg.fc->tw64.as<cpx<double>>(), +1, rows),
-         "PSF row FFT");
-  HIPCHK(launch_colfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fr->plan,
-                               cg.fr->tw64.as<cpx<double>>(), ColsPerBlock(g.Nr, sizeof(cpx<double>))),
-         "PSF column FFT");
-  HIPCHK(hipStreamSynchronize(m->stream), "PSF spectrum");
+  {
+    const size_t e64 = sizeof(cpx<double>);
+    const bool longRow = LongFFT(g.Nc, 1, e64, maxLds), longCol = LongFFT(g.Nr + 1, 1, e64, maxLds);
+    const int rows = longRow ? 1 : RowsPerBlock(g.Nc, e64);
+    const int C = longCol ? 1 : ColsPerBlock(g.Nr, e64);
+    DevBuf scratch;   // long transforms: one buffer per workgroup
+    const size_t sb = std::max(longRow ? (size_t)((g.Nr + rows - 1) / rows) * rows * g.Nc : 0,
+                               longCol ? (size_t)((g.Nc + C - 1) / C) * C * (g.Nr + 1) : 0);
+    if (sb) HIPCHK(scratch.ensure(sb * e64), "FFT scratch");
+    HIPCHK(launch_rowfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fc->plan,
+                                 cg.fc->tw64.as<cpx<double>>(), +1, rows,
+                                 longRow ? scratch.as<cpx<double>>() : nullptr),
+           "PSF row FFT");
+    HIPCHK(launch_colfft<double>(m->stream, cg.khat64.as<cpx<double>>(), g.Nr, g.Nc, cg.fr->plan,
+                                 cg.fr->tw64.as<cpx<double>>(), C, longCol ? scratch.as<cpx<double>>() : nullptr),
+           "PSF column FFT");
+    HIPCHK(hipStreamSynchronize(m->stream), "PSF spectrum");
+  }
   HIPCHK(hipMemcpy(k.data(), cg.khat64.p, NN * sizeof(cpx<double>), hipMemcpyDeviceToHost), "PSF spectrum D2H");
   std::vector<cpx<float>> k32(NN);
   for (long z = 0; z < NN; z++) k32[z] = {(float)k[z].x, (float)k[z].y};
   HIPCHK(hipMemcpy(cg.khat32.p, k32.data(), NN * sizeof(cpx<float>), hipMemcpyHostToDevice), "PSF spectrum FP32");
   const size_t elem = m->precision == IFG_PRECISION_FP64 ? sizeof(cpx<double>) : sizeof(cpx<float>);
-  cg.rowsK1 = RowsPerBlock(g.Nc, elem);
+  cg.bigRow = LongFFT(g.Nc, 1, elem, maxLds);
+  cg.bigCol = LongFFT(g.Nr + 1, 1, elem, maxLds);
+  if ((cg.bigRow || cg.bigCol) && splitMultiple > 1)
+    return m->fail(IFG_ERR_UNSUPPORTED, "oversampled region too wide for one workgroup's LDS");
+  cg.rowsK1 = cg.bigRow ? 1 : RowsPerBlock(g.Nc, elem);
   // inverse-pass rows per block: whole groups of splitMultiple rows (regions average
   // s x s sub-pixel blocks inside one workgroup)
   cg.rowsK3 = std::max(1, cg.rowsK1 / splitMultiple) * splitMultiple;
-  if ((2 * (size_t)cg.rowsK3 + 1) * g.Nc * elem > (size_t)maxLds)
+  if (!cg.bigRow && (2 * (size_t)cg.rowsK3 + 1) * g.Nc * elem > (size_t)maxLds)
     return m->fail(IFG_ERR_UNSUPPORTED, "oversampled region too wide for one workgroup's LDS");
   cg.C64 = ColsPerBlock(g.Nr, sizeof(cpx<double>));
   cg.C32 = ColsPerBlock(g.Nr, sizeof(cpx<float>));
+  if (cg.bigCol) cg.C64 = cg.C32 = 1;
   cg.ready = true;
   return IFG_OK;
 }
@@ -757,8 +798,14 @@ int EnsureRegionConv(ifg_model* m, Region& R, int which) {
   if (cg.ready) return IFG_OK;
   const double frac = 1.0 / R.scale;
   const double off = 0.5 * frac - 0.5;   // core/oversampled_region.cpp:223-226
-  return BuildConv(m, cg, R.nbc * R.scale, R.nbr * R.scale, R.psf, R.pc, R.pr, which == 0, (double)R.x1 + off,
-                   (double)R.y1 + off, frac, R.scale);
+  const int st = BuildConv(m, cg, R.nbc * R.scale, R.nbr * R.scale, R.psf, R.pc, R.pr, which == 0,
+                           (double)R.x1 + off, (double)R.y1 + off, frac, R.scale);
+  if (st < 0) return st;
+  if (cg.bigRow || cg.bigCol) {   // a region's inverse pass averages its s x s blocks inside one workgroup's LDS
+    cg.ready = false;
+    return m->fail(IFG_ERR_UNSUPPORTED, "oversampled region too wide for one workgroup's LDS");
+  }
+  return IFG_OK;
 }
 
 // ---------------------------------------------------------------- evaluation plans
@@ -1016,6 +1063,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   P.mapOff.assign(P.nChunks, 0);
   P.jobOff.assign(P.nChunks, 0);
   int maxSlots = 0;
+  std::vector<char> longT(tasks.size(), 0);   // tasks with long row (1) / column (2) transforms
+  size_t longElems = 0;
   for (int c = 0; c < P.nChunks; c++) {
     const int v0 = c * P.chunkVecs, nv = std::min(P.chunkVecs, nVec - v0);
     for (int gi = 0; gi < P.nGrids; gi++) {
@@ -1080,8 +1129,10 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         const int unit = 8 * Tk.stripsPerLine;
         Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
         Tk.n3 = (Tk.g.Hh + Tk.rows3 - 1) / Tk.rows3;
-        P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
-        P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
+        const bool lr = cg->bigRow && !region, lc = cg->bigCol && !region;
+        longT[(size_t)c * P.nTasks + gi] = (char)((lr ? 1 : 0) | (lc ? 2 : 0));
+        P.lds1 = std::max(P.lds1, (lr ? (size_t)Tk.rows1 : 2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
+        P.lds2 = std::max(P.lds2, (lc ? (size_t)Tk.C * Tk.ld : 2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
       }
       if (region) {
         const Region& R = *m->regions[gi - 1];
@@ -1094,7 +1145,9 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
       if (region)   // within the main grid's LDS, or 52 KB (three blocks per CU with allocation granules)
         Tk.subPar = RegionSubPar(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, std::max<size_t>(P.lds3, 53248));
       if (Tk.kind != TK_MAIN_PLAIN)
-        P.lds3 = std::max(P.lds3, InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
+        P.lds3 = std::max(P.lds3, (longT[(size_t)c * P.nTasks + gi] & 1)
+                                      ? (size_t)Tk.rows3 * Tk.g.Nc * elem
+                                      : InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
       FillTaskStat<T>(m, Tk.st);
       Tk.psi = GridPsi(m, gi);
       Tk.st.devOut = (flags & (PF_DEV | PF_JAC)) ? P.dev_d.as<double>() + (long)v0 * nData : nullptr;
@@ -1110,6 +1163,8 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
     P.nB1[c] = (int)m1.size();
     P.nB2[c] = (int)m2.size();
     P.nB3[c] = (int)m3.size();
+    longElems = std::max(longElems, LongScratchElems(tasks, longT, (size_t)c * P.nTasks, P.nTasks, P.nB1[c],
+                                                     P.nB2[c], P.nB3[c]));
     P.mapOff[c] = (long)maps.size();
     maps.insert(maps.end(), m1.begin(), m1.end());
     maps.insert(maps.end(), m2.begin(), m2.end());
@@ -1159,6 +1214,13 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   }
   P.tailWaves = 0;
   for (int gi = 0; gi < P.nGrids; gi++) P.tailWaves += (long)P.strideMax[gi] * std::min(P.chunkVecs, nVec);
+  if (longElems) {   // long transforms: the second FFT buffers in global memory
+    if ((st = PlanBuf(m, wa, P.fft_d, longElems * elem, false, "FFT scratch")) < 0) return st;
+    for (size_t t = 0; t < tasks.size(); t++) {
+      if (longT[t] & 1) tasks[t].rowScratch = P.fft_d.p;
+      if (longT[t] & 2) tasks[t].colScratch = P.fft_d.p;
+    }
+  }
   if ((st = PlanBuf(m, wa, P.tasks_d, sizeof(Task) * tasks.size(), true, "task table")) < 0 ||
       (st = PlanUpload(m, wa, P.tasks_d, tasks.data(), sizeof(Task) * tasks.size(), "tasks")) < 0)
     return st;
@@ -1504,6 +1566,7 @@ int BuildAdjPlan(ifg_model* m, Plan& P, int flags, const std::vector<int>& minSt
   const int* used_d = reinterpret_cast<const int*>(P.stage_d.as<char>() + P.usedOff);
   const size_t elem = sizeof(cpx<T>);
   std::vector<Task> tasks(P.nTasks);
+  std::vector<char> longT(tasks.size(), 0);   // tasks with long row (1) / column (2) transforms
   for (int gi = 0; gi < P.nGrids; gi++) {
     Task& Tk = tasks[gi];
     std::memset(&Tk, 0, sizeof(Tk));
@@ -1556,9 +1619,11 @@ int BuildAdjPlan(ifg_model* m, Plan& P, int flags, const std::vector<int>& minSt
       const int unit = 8 * Tk.stripsPerLine;
       Tk.n2 = (Tk.nStrips + unit - 1) / unit * unit;
       Tk.n3 = (Tk.g.Lr + Tk.rows3 - 1) / Tk.rows3;
-      P.lds1 = std::max(P.lds1, (2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
-      P.lds2 = std::max(P.lds2, (2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
-      P.lds3 = std::max(P.lds3, (2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
+      const bool lr = cg->bigRow && !region, lc = cg->bigCol && !region;
+      longT[gi] = (char)((lr ? 1 : 0) | (lc ? 2 : 0));
+      P.lds1 = std::max(P.lds1, (lr ? (size_t)Tk.rows1 : 2 * (size_t)Tk.rows1 + 1) * Tk.g.Nc * elem);
+      P.lds2 = std::max(P.lds2, (lc ? (size_t)Tk.C * Tk.ld : 2 * (size_t)Tk.C * Tk.ld + Tk.g.Nr) * elem);
+      P.lds3 = std::max(P.lds3, (lr ? (size_t)Tk.rows3 : 2 * (size_t)Tk.rows3 + 1) * Tk.g.Nc * elem);
     }
     if (region) {
       const Region& R = *m->regions[gi - 1];
@@ -1605,6 +1670,13 @@ int BuildAdjPlan(ifg_model* m, Plan& P, int flags, const std::vector<int>& minSt
          "alloc adjoint partials");
   HIPCHK(hipHostMalloc(&P.pinnedOut, sizeof(double) * P.adjWidth + 64, hipHostMallocDefault), "pinned results");
   HIPCHK(hipHostGetDevicePointer((void**)&P.outDev, P.pinnedOut, 0), "map pinned results");
+  if (const size_t le = LongScratchElems(tasks, longT, 0, tasks.size(), P.nB1[0], P.nB2[0], P.nB3[0])) {
+    HIPCHK(P.fft_d.ensure(le * elem), "FFT scratch");   // long transforms: second FFT buffers
+    for (size_t t = 0; t < tasks.size(); t++) {
+      if (longT[t] & 1) tasks[t].rowScratch = P.fft_d.p;
+      if (longT[t] & 2) tasks[t].colScratch = P.fft_d.p;
+    }
+  }
   HIPCHK(P.tasks_d.ensure(sizeof(Task) * tasks.size()), "alloc task table");
   HIPCHK(hipMemcpy(P.tasks_d.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice), "tasks");
   HIPCHK(hipDeviceSynchronize(), "adjoint plan setup");
@@ -2615,6 +2687,9 @@ int BuildBatchPlan(ifg_batch* b, BatchPlan& P) {
     ifg_model* m = b->models[k];
     if ((st = PrepareDevice(m)) < 0) return BatchModelFail(b, k, st);
     if (m->doConv && (st = EnsureMainConv(m, which)) < 0) return BatchModelFail(b, k, st);
+    if (m->doConv && (m->mainConv[which].bigRow || m->mainConv[which].bigCol))
+      return b->fail(IFG_ERR_UNSUPPORTED, "model " + std::to_string(k) +
+                                              ": images whose FFT rows or columns exceed LDS are fitted one at a time");
     for (auto& R : m->regions)
       if ((st = EnsureRegionConv(m, *R, which)) < 0) return BatchModelFail(b, k, st);
     const int nG = 1 + (int)m->regions.size();
@@ -3249,9 +3324,10 @@ int EnqueueBatchResident(ifg_batch* b, BatchPlan& P, const LMPlanDev& D) {
   if (!deriv)
     e = launch_final(s, P.jobs_d.as<FinalJob>(), D.jobs, gj, P.partials_d.as<double>(), mode, P.out_d.as<double>(),
                      0, nullptr, D.counts + 3);
-  else
-    e = launch_normal(s, P.njobs_d.as<NormalJob>(), D.jobs, gj, D.tiles, std::min(P.nTiles, kResidentTileGrid),
-                      P.maxFree, P.normPart_d.as<double>(), D.counts + 4, D.counts + 3);
+  else   // normal equations: a (entries x 64) grid loops over the jobs
+    e = launch_normal(s, P.njobs_d.as<NormalJob>(), D.jobs, std::min(P.nModels, 64), D.tiles,
+                      std::min(P.nTiles, kResidentTileGrid), P.maxFree, P.normPart_d.as<double>(), D.counts + 4,
+                      D.counts + 3);
   if (e != hipSuccess) return b->hipfail(e, deriv ? "k_normal" : "k_final");
   return IFG_OK;
 }
@@ -3573,6 +3649,7 @@ int Rows3For(int rows1, int split) { return split == 1 ? rows1 : std::max(1, row
 bool ShapeFits(const ConvGrid& cg, const GridShape& s, size_t elem, int maxLds, int split) {
   const GridGeom& g = cg.g;
   if (s.C < 1 || s.C > 8 || (s.C & (s.C - 1)) || s.rows1 < 1) return false;
+  if (cg.bigRow || cg.bigCol) return s.C == 1 && s.rows1 == 1 && split == 1;   // long transforms: one shape
   if ((2 * (size_t)s.C * (g.Nr + 1) + g.Nr) * elem > (size_t)maxLds) return false;
   if ((2 * (size_t)s.rows1 + 1) * g.Nc * elem > (size_t)maxLds) return false;
   return (2 * (size_t)Rows3For(s.rows1, split) + 1) * g.Nc * elem <= (size_t)maxLds;

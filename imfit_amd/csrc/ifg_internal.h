@@ -184,6 +184,12 @@ struct Task {
   double* aTail;               // adjoint pass: the adjoint value of each sub-pixel box pixel, per plane
                                // (plane p at aTail + p * ea.tailStride, box layout as Box.off)
   PsfInterp psi;               // point sources on this grid (psi.z null: none)
+  // Long transforms: a grid whose rows (or columns) do not fit two LDS buffers and the
+  // twiddle table keeps one buffer in LDS; the other is the workgroup's slot (by its
+  // index in the launch) of a global scratch area, and the twiddles are read from
+  // global memory.  Null: both buffers and the table in LDS.
+  void* rowScratch;            // row passes: rows x Nc elements per workgroup
+  void* colScratch;            // column pass: C x ld elements per workgroup
   EvalArgs ea;
   TaskStat st;
 };

@@ -325,6 +325,24 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
   lds_fill<8>(dst, tw, N, N);
 }
 
+// The second FFT buffer B and the twiddle table TW of a workgroup whose first buffer
+// A (n elements) starts the LDS: after A in LDS, with the length-N table staged after
+// it; or, for long transforms (scratch set), the workgroup's slot of the global
+// scratch area and the table in global memory.
+template <typename T>
+__device__ __forceinline__ void fft_buffers(cpx<T>* A, int n, int N, const void* twGlobal, void* scratch,
+                                            cpx<T>*& B, const cpx<T>*& TW) {
+  if (scratch) {
+    B = reinterpret_cast<cpx<T>*>(scratch) + (long)blockIdx.x * n;
+    TW = reinterpret_cast<const cpx<T>*>(twGlobal);
+  } else {
+    B = A + n;
+    cpx<T>* t = B + n;
+    stage_twiddles(t, reinterpret_cast<const cpx<T>*>(twGlobal), N);
+    TW = t;
+  }
+}
+
 // ------------------------------------------------------------------ image functions
 // Point intensity at radius r (CalculateIntensity of func_sersic.cpp:135-141,
 // func_exp.cpp:127-130, func_gaussian.cpp:123-128, func_moffat.cpp:140-147).
@@ -1558,9 +1576,9 @@ __device__ __forceinline__ void eval_rowfft_block(const Task* __restrict__ tasks
   const GridGeom g = Tk.g;
   const int rows = Tk.rows1;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rows * g.Nc;
-  cpx<T>* TW = B + rows * g.Nc;
-  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  cpx<T>* B;
+  const cpx<T>* TW;
+  fft_buffers<T>(A, rows * g.Nc, g.Nc, Tk.twc, Tk.rowScratch, B, TW);
   const int4 pr = Tk.pairs[pi];
   __shared__ CompRec sRec[2][IFG_MAXCOMP];
   __shared__ Box sBox[2][IFG_MAXBOX];
@@ -1645,9 +1663,9 @@ __device__ __forceinline__ void colconv_block(const Task* __restrict__ tasks, co
   const int strip = (q * 8 + (rem & 7)) * G + (rem >> 3);
   if (strip >= Tk.nStrips) return;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + C * ld;
-  cpx<T>* TW = B + C * ld;
-  stage_twiddles(TW, (const cpx<T>*)Tk.twr, g.Nr);
+  cpx<T>* B;
+  const cpx<T>* TW;
+  fft_buffers<T>(A, C * ld, g.Nr, Tk.twr, Tk.colScratch, B, TW);
   const int c0 = strip * C;
   cpx<T>* S = (cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
   const cpx<T>* khat = (const cpx<T>*)Tk.khat;
@@ -1712,12 +1730,12 @@ __global__ __launch_bounds__(256) void k_colconv(const Task* __restrict__ tasks,
 // Setup-time transforms of the PSF spectrum: in-place row FFTs and column FFTs.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, FFTPlan pl, const cpx<T>* tw,
-                                                int dir, int rowsPerBlock) {
+                                                int dir, int rowsPerBlock, cpx<T>* scratch) {
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rowsPerBlock * Nc;
-  cpx<T>* TW = B + rowsPerBlock * Nc;
-  stage_twiddles(TW, tw, Nc);
+  cpx<T>* B;
+  const cpx<T>* TW;
+  fft_buffers<T>(A, rowsPerBlock * Nc, Nc, tw, scratch, B, TW);
   const int ir0 = blockIdx.x * rowsPerBlock;
   const int total = rowsPerBlock * Nc;
   for (int e = threadIdx.x; e < total; e += kThreads) {
@@ -1734,12 +1752,12 @@ __global__ __launch_bounds__(256) void k_rowfft(cpx<T>* buf, int nRows, int Nc, 
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_colfft(cpx<T>* buf, int Nr, int Nc, FFTPlan pl, const cpx<T>* tw, int C,
-                                                int ld) {
+                                                int ld, cpx<T>* scratch) {
   extern __shared__ char smem_raw[];
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + C * ld;
-  cpx<T>* TW = B + C * ld;
-  stage_twiddles(TW, tw, Nr);
+  cpx<T>* B;
+  const cpx<T>* TW;
+  fft_buffers<T>(A, C * ld, Nr, tw, scratch, B, TW);
   const int c0 = blockIdx.x * C;
   for (int e = threadIdx.x; e < Nr * C; e += kThreads) {
     const int i = e / C, cc = e - i * C, col = c0 + cc;
@@ -2120,9 +2138,9 @@ __device__ __forceinline__ void inverse_stat_block(const Task* __restrict__ task
     // then the regions' block sums
     const int bufRows = region ? Tk.subPar : rows;
     cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-    cpx<T>* B = A + bufRows * g.Nc;
-    cpx<T>* TW = B + bufRows * g.Nc;
-    stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+    cpx<T>* B;
+    const cpx<T>* TW;
+    fft_buffers<T>(A, bufRows * g.Nc, g.Nc, Tk.twc, region ? nullptr : Tk.rowScratch, B, TW);
     const int io0 = lb * rows;
     const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
     if (region) {
@@ -2133,7 +2151,7 @@ __device__ __forceinline__ void inverse_stat_block(const Task* __restrict__ task
       const int s = Tk.scale, par = Tk.subPar;
       const int nbr = rows / s;
       const int nAcc = 2 * nbr * Tk.nbc;
-      double* acc = reinterpret_cast<double*>(TW + g.Nc);
+      double* acc = reinterpret_cast<double*>(B + bufRows * g.Nc + g.Nc);   // after the LDS table
       for (int e = threadIdx.x; e < nAcc; e += kThreads) acc[e] = 0.0;
       const int nSub = max(0, min(rows, g.Hh - io0));
       for (int sr0 = 0; sr0 < nSub; sr0 += par) {
@@ -2379,9 +2397,9 @@ __global__ __launch_bounds__(256) void k_adj_rowfft(const Task* __restrict__ tas
   const GridGeom g = Tk.g;
   const int rows = Tk.rows1;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rows * g.Nc;
-  cpx<T>* TW = B + rows * g.Nc;
-  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  cpx<T>* B;
+  const cpx<T>* TW;
+  fft_buffers<T>(A, rows * g.Nc, g.Nc, Tk.twc, Tk.rowScratch, B, TW);
   const int4 pr = Tk.pairs[pi];
   const int io0 = lb * rows;
   const int total = rows * g.Nc;
@@ -2498,9 +2516,9 @@ __global__ __launch_bounds__(256) void k_adj_inverse(const Task* __restrict__ ta
   }
   const int rows = Tk.rows3;
   cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* B = A + rows * g.Nc;
-  cpx<T>* TW = B + rows * g.Nc;
-  stage_twiddles(TW, (const cpx<T>*)Tk.twc, g.Nc);
+  cpx<T>* B;
+  const cpx<T>* TW;
+  fft_buffers<T>(A, rows * g.Nc, g.Nc, Tk.twc, Tk.rowScratch, B, TW);
   const int ir0 = lb * rows;
   const int validRows = max(0, min(rows, g.Lr - ir0));
   const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
@@ -2865,17 +2883,19 @@ hipError_t launch_adjoint(hipStream_t s, const Task* tasks, int nTasks, const Bl
 
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
-                         int dir, int rows) {
-  IFG_LAUNCH(true, true, k_rowfft<T>, dim3((nRows + rows - 1) / rows), dim3(256), smem_rows(rows, Nc, sizeof(cpx<T>)),
-             s, buf, nRows, Nc, pl, tw, dir, rows);
+                         int dir, int rows, cpx<T>* scratch) {
+  const size_t lds = scratch ? (size_t)rows * Nc * sizeof(cpx<T>) : smem_rows(rows, Nc, sizeof(cpx<T>));
+  IFG_LAUNCH(true, true, k_rowfft<T>, dim3((nRows + rows - 1) / rows), dim3(256), lds, s, buf, nRows, Nc, pl, tw, dir,
+             rows, scratch);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C) {
+hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C,
+                         cpx<T>* scratch) {
   const int ld = Nr + 1;
-  IFG_LAUNCH(true, true, k_colfft<T>, dim3((Nc + C - 1) / C), dim3(256), (2 * (size_t)C * ld + Nr) * sizeof(cpx<T>),
-             s, buf, Nr, Nc, pl, tw, C, ld);
+  const size_t lds = (scratch ? (size_t)C * ld : 2 * (size_t)C * ld + Nr) * sizeof(cpx<T>);
+  IFG_LAUNCH(true, true, k_colfft<T>, dim3((Nc + C - 1) / C), dim3(256), lds, s, buf, Nr, Nc, pl, tw, C, ld, scratch);
   return hipGetLastError();
 }
 
@@ -2928,11 +2948,12 @@ hipError_t launch_normal(hipStream_t s, const NormalJob* jobs, const int* jobLis
   template hipError_t launch_inverse_stat<T>(hipStream_t, const Task*, const BlockRanges&, int, size_t,             \
                                              const StatLaunch&,                                                    \
                                              bool, bool, bool, bool);                                              \
-  template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int);    \
+  template hipError_t launch_rowfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, int,     \
+                                       cpx<T>*);                                                                   \
   template hipError_t launch_adjoint<T>(hipStream_t, const Task*, int, const BlockRanges&, int, size_t,             \
                                         const BlockRanges&, int, size_t, const BlockRanges&, int, size_t,           \
                                         const long*, long, int, bool, const AdjLaunch&, int, double*, int);         \
-  template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int);
+  template hipError_t launch_colfft<T>(hipStream_t, cpx<T>*, int, int, const FFTPlan&, const cpx<T>*, int, cpx<T>*);
 IFG_INST(double)
 IFG_INST(float)
 

@@ -39,11 +39,14 @@ hipError_t launch_adjoint(hipStream_t s, const Task* tasks, int nTasks, const Bl
                           const BlockRanges& br2, int nB2, size_t lds2, const BlockRanges& br3, int nB3, size_t lds3,
                           const long* prefix, long tailBlocks, int sub, bool disk, const AdjLaunch& al, int nSlots,
                           double* out, int hostOut);
+// scratch (long transforms): the second FFT buffer per workgroup in global memory,
+// rows x Nc (row FFT) or C x (Nr + 1) (column FFT) elements each; null: in LDS
 template <typename T>
 hipError_t launch_rowfft(hipStream_t s, cpx<T>* buf, int nRows, int Nc, const FFTPlan& pl, const cpx<T>* tw,
-                         int dir, int rows);
+                         int dir, int rows, cpx<T>* scratch = nullptr);
 template <typename T>
-hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C);
+hipError_t launch_colfft(hipStream_t s, cpx<T>* buf, int Nr, int Nc, const FFTPlan& pl, const cpx<T>* tw, int C,
+                         cpx<T>* scratch = nullptr);
 // n8 8-byte words from mapped pinned host memory into device memory
 hipError_t launch_stage_copy(hipStream_t s, void* dst, const void* srcMapped, long n8);
 // jobList: the jobs of this launch (null: jobs 0 .. nJobs-1); hostOut: out is mapped
