@@ -350,11 +350,15 @@ __device__ __forceinline__ void fft_buffers(cpx<T>* A, int n, int N, const void*
 // carries an extended-precision logarithm that costs more than the pixel's other
 // terms together; the plain form's relative error, |log(r/r_e) / n| ulp, stays
 // below 1e-14 of the intensity over any image.
+__device__ __forceinline__ double sersic_f64(const double* v, double r) {
+  return v[5] * exp(-v[7] * (exp(v[8] * log(r / v[6])) - 1.0));
+}
+__device__ __forceinline__ double exp_f64(const double* v, double r) { return v[5] * exp(-r / v[6]); }
 __device__ __forceinline__ double intensity_f64(int type, const double* v, double r) {
   switch (type) {
     case F_SERSIC:
-    case F_GENSERSIC: return v[5] * exp(-v[7] * (exp(v[8] * log(r / v[6])) - 1.0));
-    case F_EXP: return v[5] * exp(-r / v[6]);
+    case F_GENSERSIC: return sersic_f64(v, r);
+    case F_EXP: return exp_f64(v, r);
     case F_BROKENEXP:   // func_broken-exp.cpp:118-131, outer form past alpha (r - r_b) = 100
       if (v[9] * (r - v[8]) > 100.0) return v[5] * exp(v[11] - r / v[7]);
       return v[5] * exp(-r / v[6]) * pow(1.0 + exp(v[9] * (r - v[8])), v[10]);
@@ -372,14 +376,18 @@ __device__ __forceinline__ double intensity_f64(int type, const double* v, doubl
 }
 
 // FP32 intensity from the squared radius, with hardware exp2/log2 (v_exp_f32, v_log_f32).
+__device__ __forceinline__ float sersic_f32_r2(const double* v, float r2) {
+  float p = __builtin_amdgcn_exp2f((float)v[13] * __builtin_amdgcn_logf(r2 * (float)v[12]));
+  return (float)v[5] * __builtin_amdgcn_exp2f((float)v[14] * (p - 1.0f));
+}
+__device__ __forceinline__ float exp_f32_r2(const double* v, float r2) {
+  return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * __builtin_sqrtf(r2));
+}
 __device__ __forceinline__ float intensity_f32_r2(int type, const double* v, float r2) {
   switch (type) {
     case F_SERSIC:
-    case F_GENSERSIC: {
-      float p = __builtin_amdgcn_exp2f((float)v[13] * __builtin_amdgcn_logf(r2 * (float)v[12]));
-      return (float)v[5] * __builtin_amdgcn_exp2f((float)v[14] * (p - 1.0f));
-    }
-    case F_EXP: return (float)v[5] * __builtin_amdgcn_exp2f((float)v[12] * __builtin_sqrtf(r2));
+    case F_GENSERSIC: return sersic_f32_r2(v, r2);
+    case F_EXP: return exp_f32_r2(v, r2);
     case F_BROKENEXP: {   // log2 I = log2(I_0 S) - r log2e/h1 + E log2(1 + e^t), t = alpha (r - r_b)
       const float r = __builtin_sqrtf(r2);
       const float t2 = (float)v[9] * 1.44269504088896341f * (r - (float)v[8]);   // t log2(e)
@@ -438,6 +446,40 @@ __device__ __forceinline__ float radius2_f32(int type, const double* v, float xp
   const float b = __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(fabsf(yp)));
   const float r = __builtin_amdgcn_exp2f(inve * __builtin_amdgcn_logf(a + b));
   return r * r;
+}
+
+// Two sub-pixel samples of one component at (x0, y0) and (x1, y1), each equal to
+// intensity_r(type, v, comp_radius(type, v, x, y)): Sersic and Exponential are written
+// out for both points in one block so that the two chains interleave.
+__device__ __forceinline__ void sample2_f64(int type, const double* v, double x0, double y0, double x1, double y1,
+                                            double& a0, double& a1) {
+  if (type == F_SERSIC) {
+    const double r0 = ell_radius(v, x0, y0), r1 = ell_radius(v, x1, y1);
+    a0 = sersic_f64(v, r0);
+    a1 = sersic_f64(v, r1);
+  } else if (type == F_EXP) {
+    const double r0 = ell_radius(v, x0, y0), r1 = ell_radius(v, x1, y1);
+    a0 = exp_f64(v, r0);
+    a1 = exp_f64(v, r1);
+  } else {
+    a0 = intensity_f64(type, v, comp_radius(type, v, x0, y0));
+    a1 = intensity_f64(type, v, comp_radius(type, v, x1, y1));
+  }
+}
+// The same from FP32 frame coordinates, each equal to
+// intensity_f32_r2(type, v, radius2_f32(type, v, xp, yp)).
+__device__ __forceinline__ void sample2_f32(int type, const double* v, float xp0, float yp0, float xp1, float yp1,
+                                            float& a0, float& a1) {
+  if (type == F_SERSIC) {
+    a0 = sersic_f32_r2(v, xp0 * xp0 + yp0 * yp0);
+    a1 = sersic_f32_r2(v, xp1 * xp1 + yp1 * yp1);
+  } else if (type == F_EXP) {
+    a0 = exp_f32_r2(v, xp0 * xp0 + yp0 * yp0);
+    a1 = exp_f32_r2(v, xp1 * xp1 + yp1 * yp1);
+  } else {
+    a0 = intensity_f32_r2(type, v, radius2_f32(type, v, xp0, yp0));
+    a1 = intensity_f32_r2(type, v, radius2_f32(type, v, xp1, yp1));
+  }
 }
 
 // ------------------------------------------------------------------ EdgeOnDisk
@@ -1290,6 +1332,48 @@ __device__ __forceinline__ void row_derivs2(const EvalArgs& ea, const GridGeom& 
 }
 
 // ------------------------------------------------------------------ model values of grid rows
+// K point values of one component on one grid row (pixel k at x = xa + cols[k] * frac,
+// y[k]), each equal to comp_point<T, DISK>(type, v, x, y[k]).  Sersic and Exponential
+// are written out for all K pixels in one block, so that the K chains interleave.
+template <typename T, bool DISK, int K>
+__device__ __forceinline__ void comp_points(int type, const double* v, const GridGeom& g, const int (&cols)[K],
+                                            const double (&y)[K], T (&val)[K]) {
+  if (type == F_SERSIC || type == F_EXP) {
+    if constexpr (sizeof(T) == 8) {
+      double r[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) r[k] = ell_radius(v, g.xa + (double)cols[k] * g.frac, y[k]);
+      if (type == F_SERSIC) {
+#pragma unroll
+        for (int k = 0; k < K; k++) val[k] = sersic_f64(v, r[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) val[k] = exp_f64(v, r[k]);
+      }
+    } else {
+      const float c = (float)v[2], s = (float)v[3], iq = (float)v[15];
+      float r2[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {   // comp_point_2d<float>'s frame coordinates
+        const float xd = (float)(g.xa + (double)cols[k] * g.frac - v[0]), yd = (float)(y[k] - v[1]);
+        const float xp = xd * c + yd * s;
+        const float yp = (yd * c - xd * s) * iq;
+        r2[k] = xp * xp + yp * yp;
+      }
+      if (type == F_SERSIC) {
+#pragma unroll
+        for (int k = 0; k < K; k++) val[k] = sersic_f32_r2(v, r2[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) val[k] = exp_f32_r2(v, r2[k]);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) val[k] = comp_point<T, DISK>(type, v, g.xa + (double)cols[k] * g.frac, y[k]);
+}
+
 // K pixels of one grid row r of vector vec (columns cols[k], valid where ok[k]),
 // grid pixel (c, r) at x = xa + c*frac, y = ya + r*frac: the sum over components
 // with Kahan compensation in component order (core/model_object.cpp:1164-1172),
@@ -1318,10 +1402,12 @@ __device__ __forceinline__ void row_values(const EvalArgs& ea, const GridGeom& g
       const int type = ea.types[n];
       if (IsPointSourceType(type)) continue;   // added after the convolution
       const double* v = rec[n].v;
+      T val[K];
+      comp_points<T, DISK, K>(type, v, g, cols, y, val);
 #pragma unroll
       for (int k = 0; k < K; k++) {
         if (!ok[k]) continue;
-        const T adjVal = comp_point<T, DISK>(type, v, g.xa + (double)cols[k] * g.frac, y[k]) - err[k];
+        const T adjVal = val[k] - err[k];
         const T tempSum = sum[k] + adjVal;
         err[k] = (tempSum - sum[k]) - adjVal;
         sum[k] = tempSum;
@@ -1479,22 +1565,39 @@ __device__ void tail_pixel(const EvalArgs& ea, const GridGeom& g, int vec, int l
         const double y_sub_start = y - 0.5 + 0.5 * deltaSubpix;
         const int n2 = nSub * nSub;
         double part = 0.0;
+        // A lane's samples are s = lane, lane + 64, ... added in that order.  Each step
+        // evaluates two of them together (independent chains for the scheduler) and
+        // adds them in that same order, so the sum is the one-at-a-time sum.
         if (sizeof(T) == 8) {   // FP64: sample coordinates exactly as the reference forms them
-          for (int s = lane; s < n2; s += 64) {
-            const int ii = s / nSub, jj = s - ii * nSub;
-            const double x_ii = x_sub_start + ii * deltaSubpix;
-            const double y_ii = y_sub_start + jj * deltaSubpix;
-            part += (double)intensity_r<T>(type, v, comp_radius(type, v, x_ii, y_ii));
+          for (int s = lane; s < n2; s += 128) {
+            const bool two = s + 64 < n2;
+            const int s1 = two ? s + 64 : s;
+            const int ii0 = s / nSub, jj0 = s - ii0 * nSub;
+            const int ii1 = s1 / nSub, jj1 = s1 - ii1 * nSub;
+            const double x0 = x_sub_start + ii0 * deltaSubpix, y0 = y_sub_start + jj0 * deltaSubpix;
+            const double x1 = x_sub_start + ii1 * deltaSubpix, y1 = y_sub_start + jj1 * deltaSubpix;
+            double a0, a1;
+            sample2_f64(type, v, x0, y0, x1, y1, a0, a1);
+            part += a0;
+            if (two) part += a1;
           }
         } else {   // FP32 pixels: sample offsets in FP32 from the FP64 start of the pixel's grid
           const float bx = (float)(x_sub_start - v[0]), by = (float)(y_sub_start - v[1]);
           const float dl = (float)deltaSubpix, c = (float)v[2], sn = (float)v[3], iq = (float)v[15];
           const float rn = 1.0f / (float)nSub;
-          for (int s = lane; s < n2; s += 64) {
-            const int ii = (int)(((float)s + 0.5f) * rn), jj = s - ii * nSub;
-            const float xd = bx + (float)ii * dl, yd = by + (float)jj * dl;
-            const float xp = xd * c + yd * sn, yp = (yd * c - xd * sn) * iq;
-            part += (double)intensity_f32_r2(type, v, radius2_f32(type, v, xp, yp));
+          for (int s = lane; s < n2; s += 128) {
+            const bool two = s + 64 < n2;
+            const int s1 = two ? s + 64 : s;
+            const int ii0 = (int)(((float)s + 0.5f) * rn), jj0 = s - ii0 * nSub;
+            const int ii1 = (int)(((float)s1 + 0.5f) * rn), jj1 = s1 - ii1 * nSub;
+            const float xd0 = bx + (float)ii0 * dl, yd0 = by + (float)jj0 * dl;
+            const float xd1 = bx + (float)ii1 * dl, yd1 = by + (float)jj1 * dl;
+            const float xp0 = xd0 * c + yd0 * sn, yp0 = (yd0 * c - xd0 * sn) * iq;
+            const float xp1 = xd1 * c + yd1 * sn, yp1 = (yd1 * c - xd1 * sn) * iq;
+            float a0, a1;
+            sample2_f32(type, v, xp0, yp0, xp1, yp1, a0, a1);
+            part += (double)a0;
+            if (two) part += (double)a1;
           }
         }
         val = wave_sum(part) / (nSub * nSub);
