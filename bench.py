@@ -406,13 +406,15 @@ def bootstrap_benchmark(n=256, n_serial=4, seed=10):
     return res
 
 
-def cutout_benchmark(n=2000, n_plain=100, precisions=("fp64", "fp32"), host_rounds=True):
+def cutout_benchmark(n=2000, n_plain=100, precisions=("fp64", "fp32"), resident=True, resident_only=False):
     """Many-cutout fits: n cutouts (64-512 px, 1-3 components, 25^2 Moffat PSF) fitted
     all at once by the batched normal-equation LM (analytic Jacobians, J^T J and J^T r
-    reduced on the GPU in FP64, every model's step in one batched evaluation), versus
-    plain GPU fits one cutout after another on a sample of n_plain: mpfit with
-    finite-difference Jacobians (chi^2 compared), and the normal-equation LM with
-    batched finite differences."""
+    reduced on the GPU in FP64, every model's step in one batched evaluation; the
+    default host-driven rounds, and with `resident` the same fits resident on the GPU
+    beside them), versus plain GPU fits one cutout after another on a sample of
+    n_plain: mpfit with finite-difference Jacobians (chi^2 compared), and the
+    normal-equation LM with batched finite differences.  resident_only: the resident
+    fits alone."""
     import imfit_amd
     from imfit_amd import workloads
     ws = workloads.cutouts(n)
@@ -428,31 +430,36 @@ def cutout_benchmark(n=2000, n_plain=100, precisions=("fp64", "fp32"), host_roun
         b = imfit_amd.BatchGPU(models)
         b.Autotune(starts)
         b.GetFitStatistic(starts)   # plans and graphs
+        b.SetLMResident(resident_only)   # host-driven rounds: the default driver
+        b.FitLM(starts, limits=lims, ftol=1e-10, maxiter=2)   # warm-up: the batch's Jacobian and deviate plans
         device_sync()
         t0 = time.perf_counter()
-        xb, rb = b.FitLM(starts, limits=lims, ftol=1e-10)   # resident on the GPU (the default)
+        xb, rb = b.FitLM(starts, limits=lims, ftol=1e-10)
         dt = time.perf_counter() - t0
         res["batched_fits_per_s_" + key] = n / dt
         res["batched_mean_niter_" + key] = float(np.mean([r.niter for r in rb]))
-        log("cfg5 batched normal-equation LM, resident (%s): %d fits in %.2f s -> %.1f fits/s (mean niter %.1f)" %
-            (key, n, dt, n / dt, res["batched_mean_niter_" + key]))
-        if not host_rounds:
+        res["batched_driver"] = "resident" if resident_only else "host rounds"
+        log("cfg5 batched normal-equation LM, %s (%s): %d fits in %.2f s -> %.1f fits/s (mean niter %.1f)" %
+            (res["batched_driver"], key, n, dt, n / dt, res["batched_mean_niter_" + key]))
+        if resident_only or not resident:
             del b, models
             continue
-        # the same fits driven from the host (one synchronisation and host LM step per round)
-        b.SetLMResident(False)
-        t0 = time.perf_counter()
-        xh, rh = b.FitLM(starts, limits=lims, ftol=1e-10)
-        dth = time.perf_counter() - t0
+        # the same fits resident on the GPU (LM steps in kernels, one host wait per 8 rounds)
         b.SetLMResident(True)
-        c_res, c_host = b.GetFitStatistic(xb), b.GetFitStatistic(xh)
+        b.FitLM(starts, limits=lims, ftol=1e-10, maxiter=2)   # warm-up, as above
+        device_sync()
+        t0 = time.perf_counter()
+        xr, rr = b.FitLM(starts, limits=lims, ftol=1e-10)
+        dtr = time.perf_counter() - t0
+        b.SetLMResident(False)
+        c_res, c_host = b.GetFitStatistic(xr), b.GetFitStatistic(xb)
         rel_rh = np.abs(c_res - c_host) / c_host
-        res["host_rounds_fits_per_s_" + key] = n / dth
-        res["resident_speedup_vs_host_rounds_" + key] = dth / dt
+        res["resident_fits_per_s_" + key] = n / dtr
+        res["resident_speedup_vs_host_rounds_" + key] = dt / dtr
         res["resident_matches_host_rounds_chi2_1e-9_" + key] = int(np.sum(rel_rh <= 1e-9))
         res["resident_lower_chi2_than_host_rounds_" + key] = int(np.sum((rel_rh > 1e-9) & (c_res < c_host)))
-        log("cfg5 host-driven rounds (%s): %.1f fits/s; resident / host rounds %.2fx; same chi^2 (1e-9) on %d of %d, "
-            "resident lower on %d" % (key, n / dth, dth / dt, res["resident_matches_host_rounds_chi2_1e-9_" + key], n,
+        log("cfg5 resident (%s): %.1f fits/s; resident / host rounds %.2fx; same chi^2 (1e-9) on %d of %d, "
+            "resident lower on %d" % (key, n / dtr, dt / dtr, res["resident_matches_host_rounds_chi2_1e-9_" + key], n,
                                       res["resident_lower_chi2_than_host_rounds_" + key]))
         if prec == imfit_amd.PRECISION_FP64 and n_plain > 0:
             t0 = time.perf_counter()
@@ -472,7 +479,7 @@ def cutout_benchmark(n=2000, n_plain=100, precisions=("fp64", "fp32"), host_roun
                 xa, _ = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, jacobian="analytic")
                 rel_ctrl.append(abs(models[k].GetFitStatistic(xa) - c_plain_all[k]) / c_plain_all[k])
                 xn, _ = models[k].FitLM(starts[k], limits=lims[k], ftol=1e-10, method="normal")
-                same_single += int(models[k].GetFitStatistic(xn) == models[k].GetFitStatistic(xh[k]))
+                same_single += int(models[k].GetFitStatistic(xn) == models[k].GetFitStatistic(xb[k]))
             res["control_mpfit_analytic_matched_chi2_1e-9"] = int(sum(v <= 1e-9 for v in rel_ctrl))
             res["host_rounds_equal_single_model_normal_lm"] = same_single
             res["batch_lower_chi2_than_fd_mpfit"] = int(sum(1 for k in range(n_plain) if rel[k] > 1e-9 and
@@ -595,7 +602,7 @@ def main():
         return
     if args.cutouts_only:
         if args.cutouts_resident_fp64:
-            print(json.dumps(cutout_benchmark(n_plain=0, precisions=("fp64",), host_rounds=False)))
+            print(json.dumps(cutout_benchmark(n_plain=0, precisions=("fp64",), resident_only=True)))
         else:
             print(json.dumps(cutout_benchmark()))
         return

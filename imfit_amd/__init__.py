@@ -637,9 +637,10 @@ class BatchGPU:
         return self._chk(_LIB.ifg_batch_set_graph_mode(self.h, int(on)), "SetGraphMode")
 
     def SetLMResident(self, on):
-        """FitLM's driver: True (default) keeps the fits resident on the GPU (staging,
-        evaluation and LM steps on the device, one host wait per several rounds);
-        False runs the host-driven rounds (the LM step on the host)."""
+        """FitLM's driver: False (default) runs the host-driven rounds (the LM step on
+        the host, one synchronisation per round); True keeps the fits resident on the
+        GPU (staging, evaluation and LM steps on the device, one host wait per 8
+        rounds)."""
         return self._chk(_LIB.ifg_batch_set_lm_resident(self.h, int(on)), "SetLMResident")
 
     def Autotune(self, params):

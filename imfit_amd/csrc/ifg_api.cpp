@@ -2634,7 +2634,7 @@ struct ifg_batch {
   std::vector<ifg_model*> models;
   std::vector<std::unique_ptr<BatchPlan>> plans;
   bool useGraphs = true;
-  bool residentLM = true;   // ifg_batch_fit_lm: the GPU-resident LM (else the host-driven rounds)
+  bool residentLM = false;   // ifg_batch_fit_lm: the GPU-resident LM (default: the host-driven rounds)
   ~ifg_batch() {
     plans.clear();
     if (stream) (void)hipStreamDestroy(stream);

@@ -343,15 +343,19 @@ int ifg_batch_normal_equations(ifg_batch* b, const double* params, const int* n_
  * and unbounded), params updated in place, res[nModels].  Returns the number of
  * models that ended without a convergence status (0: all converged), or an error.
  * Replaces mpfit's per-model loop (solvers/mpfit.cpp:610-746) for many models.  By
- * default the fits are resident on the GPU: staging, evaluation and every LM step
- * (trust region, Levenberg-Marquardt parameter, Cholesky solve, limits) run on the
- * device, and the host waits once per several rounds (ifg_batch_set_lm_resident). */
+ * default the host drives the rounds (the LM steps on the host, one synchronisation
+ * per round); ifg_batch_set_lm_resident keeps the fits resident on the GPU instead. */
 int ifg_batch_fit_lm(ifg_batch* b, double* params, const ifg_param_info* pinfo, const ifg_lm_config* cfg,
                      ifg_lm_result* res);
-/* ifg_batch_fit_lm's driver: on != 0 (default) the GPU-resident fits, 0 the host-driven
- * rounds (the LM step on the host, one synchronisation per round).  Both take the same
+/* ifg_batch_fit_lm's driver: 0 (default) the host-driven rounds (the LM step on the
+ * host, one synchronisation per round), on != 0 the GPU-resident fits: staging,
+ * evaluation and every LM step (trust region, Levenberg-Marquardt parameter, Cholesky
+ * solve, limits) on the device, with one host wait per 8 rounds.  Both take the same
  * steps; the resident fits fall back to the host rounds when a model's sub-pixel
- * boxes outgrow the batch's layout. */
+ * boxes outgrow the batch's layout.  The resident driver's LM step runs in one
+ * wavefront per model (about 0.1 ms a round): it gains where the host's per-round
+ * work over many models outweighs that, and loses where the rounds serve a few slowly
+ * converging fits. */
 int ifg_batch_set_lm_resident(ifg_batch* b, int on);
 /* as ifg_set_graph_mode, for the batch's own launches */
 int ifg_batch_set_graph_mode(ifg_batch* b, int on);
