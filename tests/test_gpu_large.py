@@ -36,9 +36,10 @@ def case_for(w, h, psf_n, data=None):
 
 
 # (width, height, PSF side, FP32 pixels): long rows in FP64 (3645-point rows), long
-# columns in FP64, long rows in FP32 (6912 points), and a 4096^2 frame in FP64 (4320
-# points both ways)
-CASES = [(3600, 48, 35, False), (48, 3600, 35, False), (6900, 24, 25, True), (4096, 4096, 51, False)]
+# columns in FP64 (a 6900-row image is two 3450-row halves: 3600-point columns), long
+# rows in FP32 (7200 points), and a 4096^2 frame in FP64 (4320-point rows; its
+# 2048-row halves give 2160-point columns, which fit)
+CASES = [(3600, 48, 35, False), (24, 6900, 25, False), (6900, 24, 25, True), (4096, 4096, 51, False)]
 
 
 @pytest.mark.parametrize("w,h,pn,fp32", CASES, ids=lambda v: str(v))
@@ -69,14 +70,14 @@ def test_long_transforms_match_oracle(w, h, pn, fp32):
         assert abs(sg - so) <= 1e-5 * abs(so), (sg, so)
 
 
-@pytest.mark.parametrize("w,h", [(3600, 48), (48, 3600)])
-def test_long_transforms_gradient_adjoint_equals_tangent(w, h):
+@pytest.mark.parametrize("w,h,pn", [(3600, 48, 35), (24, 6900, 25)])
+def test_long_transforms_gradient_adjoint_equals_tangent(w, h, pn):
     import imfit_amd
-    announce("long transforms gradient %dx%d" % (w, h))
-    base = case_for(w, h, 35)
+    announce("long transforms gradient %dx%d psf %d" % (w, h, pn))
+    base = case_for(w, h, pn)
     o = base.build(oracle.OracleModel(threads=16))
     o.CreateModelImage(base.params * 1.02)
-    case = case_for(w, h, 35, data=o.GetModelImageVector() + 1.0)
+    case = case_for(w, h, pn, data=o.GetModelImageVector() + 1.0)
     g = case.build(imfit_amd.ModelObjectGPU(0, imfit_amd.PRECISION_FP64))
     ga, sa = g.GetFitStatisticGradient(case.params)
     g.SetGradientMethod(imfit_amd.GRADIENT_TANGENT)
