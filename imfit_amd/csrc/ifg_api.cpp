@@ -275,6 +275,7 @@ struct Plan {
   DevBuf maps_d, jobs_d, njobs_d, tileMap_d;
   DevBuf aTail_d, adjPart_d;   // adjoint plans: box pixels' adjoint values, per-slot partial sums
   DevBuf fft_d;                // long transforms: the passes' second FFT buffers, one slot per workgroup
+  DevBuf sub_d;                // regions' inverse blocks that transform all sub-rows at once: their second buffers
   int adjWidth = 0;            // nComp * IFG_SLOTS
   long adjTailBlocks = 0;
   int adjPsBlocks = 0;         // partial slots of the point sources' sums (the row-pass blocks)
@@ -1064,6 +1065,7 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
   P.jobOff.assign(P.nChunks, 0);
   int maxSlots = 0;
   std::vector<char> longT(tasks.size(), 0);   // tasks with long row (1) / column (2) transforms
+  std::vector<char> wideT(tasks.size(), 0);   // region tasks whose inverse blocks transform all sub-rows at once
   size_t longElems = 0;
   for (int c = 0; c < P.nChunks; c++) {
     const int v0 = c * P.chunkVecs, nv = std::min(P.chunkVecs, nVec - v0);
@@ -1142,11 +1144,21 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
         Tk.c0 = R.x1 - 1;
         Tk.r0 = R.y1 - 1;
       }
-      if (region)   // within the main grid's LDS, or 52 KB (three blocks per CU with allocation granules)
-        Tk.subPar = RegionSubPar(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, std::max<size_t>(P.lds3, 53248));
+      if (region) {   // within the main grid's LDS, or 52 KB (three blocks per CU with allocation granules)
+        const size_t budget = std::max<size_t>(P.lds3, 53248);
+        Tk.subPar = RegionSubPar(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, budget);
+        // fewer than all sub-rows at a time (one load, transform and sum after another):
+        // with the second buffer and the twiddles in global memory, all of them may fit
+        if (Tk.subPar < Tk.rows3 && RegionWideLdsBytes(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem) <= budget) {
+          wideT[(size_t)c * P.nTasks + gi] = 1;
+          Tk.subPar = Tk.rows3;
+        }
+      }
       if (Tk.kind != TK_MAIN_PLAIN)
         P.lds3 = std::max(P.lds3, (longT[(size_t)c * P.nTasks + gi] & 1)
                                       ? (size_t)Tk.rows3 * Tk.g.Nc * elem
+                                  : wideT[(size_t)c * P.nTasks + gi]
+                                      ? RegionWideLdsBytes(Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem)
                                       : InverseLdsBytes(Tk.kind, Tk.rows3, Tk.g.Nc, Tk.scale, Tk.nbc, elem, Tk.subPar));
       FillTaskStat<T>(m, Tk.st);
       Tk.psi = GridPsi(m, gi);
@@ -1220,6 +1232,18 @@ int BuildPlan(ifg_model* m, Plan& P, int nVec, int flags, const std::vector<int>
       if (longT[t] & 1) tasks[t].rowScratch = P.fft_d.p;
       if (longT[t] & 2) tasks[t].colScratch = P.fft_d.p;
     }
+  }
+  size_t wideElems = 0;   // regions transforming all sub-rows at once: a slot per block of the task
+  for (size_t t = 0; t < tasks.size(); t++)
+    if (wideT[t]) wideElems += (size_t)tasks[t].nPairs * tasks[t].n3 * tasks[t].rows3 * tasks[t].g.Nc;
+  if (wideElems) {
+    if ((st = PlanBuf(m, wa, P.sub_d, wideElems * elem, false, "region sub-row buffers")) < 0) return st;
+    size_t off = 0;
+    for (size_t t = 0; t < tasks.size(); t++)
+      if (wideT[t]) {
+        tasks[t].subScratch = P.sub_d.as<char>() + off * elem;
+        off += (size_t)tasks[t].nPairs * tasks[t].n3 * tasks[t].rows3 * tasks[t].g.Nc;
+      }
   }
   if ((st = PlanBuf(m, wa, P.tasks_d, sizeof(Task) * tasks.size(), true, "task table")) < 0 ||
       (st = PlanUpload(m, wa, P.tasks_d, tasks.data(), sizeof(Task) * tasks.size(), "tasks")) < 0)

@@ -155,6 +155,13 @@ __host__ __device__ inline size_t InverseLdsBytes(int kind, int rows3, int Nc, i
   return (2 * (size_t)rows3 + 1) * Nc * elem;
 }
 
+// LDS bytes of a region's inverse-pass block that transforms all its rows3 sub-rows at
+// once, with the second buffer and the twiddles in global memory: one buffer, then
+// the block sums.
+__host__ __device__ inline size_t RegionWideLdsBytes(int rows3, int Nc, int scale, int nbc, size_t elem) {
+  return (size_t)rows3 * Nc * elem + 2 * (size_t)(rows3 / scale) * nbc * sizeof(double);
+}
+
 // Sub-rows a region's inverse block transforms together: the most (up to its rows3)
 // whose LDS stays within `budget` bytes, so that region blocks do not lower the
 // occupancy of the main grid's blocks in the same launch; at least one.
@@ -190,6 +197,8 @@ struct Task {
   // global memory.  Null: both buffers and the table in LDS.
   void* rowScratch;            // row passes: rows x Nc elements per workgroup
   void* colScratch;            // column pass: C x ld elements per workgroup
+  void* subScratch;            // region, inverse pass: rows3 x Nc elements per block of the task (its
+                               // second buffer, so that all its sub-rows transform at once); null: in LDS
   EvalArgs ea;
   TaskStat st;
 };

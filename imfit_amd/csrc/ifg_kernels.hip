@@ -327,13 +327,13 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 
 // The second FFT buffer B and the twiddle table TW of a workgroup whose first buffer
 // A (n elements) starts the LDS: after A in LDS, with the length-N table staged after
-// it; or, for long transforms (scratch set), the workgroup's slot of the global
-// scratch area and the table in global memory.
+// it; or (scratch set) slot `slot` of a global scratch area (by default the
+// workgroup's index in the launch) and the table in global memory.
 template <typename T>
 __device__ __forceinline__ void fft_buffers(cpx<T>* A, int n, int N, const void* twGlobal, void* scratch,
-                                            cpx<T>*& B, const cpx<T>*& TW) {
+                                            cpx<T>*& B, const cpx<T>*& TW, long slot = -1) {
   if (scratch) {
-    B = reinterpret_cast<cpx<T>*>(scratch) + (long)blockIdx.x * n;
+    B = reinterpret_cast<cpx<T>*>(scratch) + (slot < 0 ? (long)blockIdx.x : slot) * n;
     TW = reinterpret_cast<const cpx<T>*>(twGlobal);
   } else {
     B = A + n;
@@ -2251,12 +2251,16 @@ __device__ __forceinline__ void inverse_stat_block(const Task* __restrict__ task
     const bool region = Tk.kind == TK_REGION;
     // LDS layout (InverseLdsBytes): two buffers of `rows` rows (subPar rows for
     // regions, which transform their sub-rows subPar at a time), the twiddle table,
-    // then the regions' block sums
+    // then the regions' block sums.  A region with subScratch transforms all its
+    // sub-rows at once: one buffer in LDS, then the block sums; the second buffer is
+    // the block's slot of subScratch and the twiddles are read from global memory.
     const int bufRows = region ? Tk.subPar : rows;
+    const bool wide = region && Tk.subScratch;
     cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* B;
     const cpx<T>* TW;
-    fft_buffers<T>(A, bufRows * g.Nc, g.Nc, Tk.twc, region ? nullptr : Tk.rowScratch, B, TW);
+    fft_buffers<T>(A, bufRows * g.Nc, g.Nc, Tk.twc, region ? Tk.subScratch : Tk.rowScratch, B, TW,
+                   region ? (long)local : -1L);
     const int io0 = lb * rows;
     const cpx<T>* S = (const cpx<T>*)Tk.spec + (long)pi * Tk.pairStride;
     if (region) {
@@ -2267,7 +2271,8 @@ __device__ __forceinline__ void inverse_stat_block(const Task* __restrict__ task
       const int s = Tk.scale, par = Tk.subPar;
       const int nbr = rows / s;
       const int nAcc = 2 * nbr * Tk.nbc;
-      double* acc = reinterpret_cast<double*>(B + bufRows * g.Nc + g.Nc);   // after the LDS table
+      double* acc = reinterpret_cast<double*>(wide ? A + bufRows * g.Nc         // after the LDS buffer
+                                                   : B + bufRows * g.Nc + g.Nc);  // after the LDS table
       for (int e = threadIdx.x; e < nAcc; e += kThreads) acc[e] = 0.0;
       const int nSub = max(0, min(rows, g.Hh - io0));
       for (int sr0 = 0; sr0 < nSub; sr0 += par) {
