@@ -2556,14 +2556,44 @@ __device__ __forceinline__ void adj_accumulate(const EvalArgs& ea, const CompRec
     double acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; k++) acc[k] = 0.0;
-    for (int e = threadIdx.x; e < nItems; e += kThreads) {
-      double x, y, a;
-      item(e, x, y, a);
-      if (a == 0.0) continue;
-      double gs[kSlots];
-      comp_grad_all<DISK>(type, rec[n].v, x, y, need, gs);
+    if (type == F_SERSIC || type == F_EXP) {
+      // items e and e + kThreads side by side (the type a constant in each call, so the
+      // two evaluations form one block and interleave), added in the thread's order
+      const double* v = rec[n].v;
+      for (int e = threadIdx.x; e < nItems; e += 2 * kThreads) {
+        double x0, y0, a0, x1, y1, a1 = 0.0;
+        item(e, x0, y0, a0);
+        x1 = x0;
+        y1 = y0;
+        if (e + kThreads < nItems) item(e + kThreads, x1, y1, a1);
+        if (a0 == 0.0 && a1 == 0.0) continue;
+        double g0[kSlots], g1[kSlots];
+        if (type == F_SERSIC) {
+          comp_grad_all<DISK>(F_SERSIC, v, x0, y0, need, g0);
+          comp_grad_all<DISK>(F_SERSIC, v, x1, y1, need, g1);
+        } else {
+          comp_grad_all<DISK>(F_EXP, v, x0, y0, need, g0);
+          comp_grad_all<DISK>(F_EXP, v, x1, y1, need, g1);
+        }
+        if (a0 != 0.0) {
 #pragma unroll
-      for (int k = 0; k < kSlots; k++) acc[k] += a * gs[k];
+          for (int k = 0; k < kSlots; k++) acc[k] += a0 * g0[k];
+        }
+        if (a1 != 0.0) {
+#pragma unroll
+          for (int k = 0; k < kSlots; k++) acc[k] += a1 * g1[k];
+        }
+      }
+    } else {
+      for (int e = threadIdx.x; e < nItems; e += kThreads) {
+        double x, y, a;
+        item(e, x, y, a);
+        if (a == 0.0) continue;
+        double gs[kSlots];
+        comp_grad_all<DISK>(type, rec[n].v, x, y, need, gs);
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) acc[k] += a * gs[k];
+      }
     }
     block_sum_slots(acc, dst + n * kSlots);
   }
