@@ -91,8 +91,8 @@ def _compare_resident(rr, rh, xr, xh, what):
     """The resident fits against the host-driven rounds: the same LM steps on the same
     statistics, apart from the Setup constants (cos / sin / pow of the device library
     instead of the C library's, differences of an ulp): the same end status, and the
-    minimum to 1e-9 in chi^2 and 1e-6 in the parameters (north_star's tolerance
-    against the reference LM); most fits take exactly the same path."""
+    minimum to 1e-9 in chi^2 and 1e-6 in the parameters (the tolerance the fits
+    are held to against the reference LM); most fits take exactly the same path."""
     same_path = 0
     for k in range(len(rr)):
         assert rr[k].status == rh[k].status or (rr[k].status > 0 and rh[k].status > 0), (what, k)
