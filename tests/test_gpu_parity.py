@@ -236,3 +236,21 @@ def test_population_across_chunks():
     for k in (0, 1, 1365, 1366, 1367, 2731, 2732, 4094, 4095):
         ref = o.GetFitStatistic(pop[k])
         assert abs(batch[k] - ref) <= 1e-12 * abs(ref), (k, batch[k], ref)
+
+
+def test_population_with_oversampled_region():
+    """A population on the 1024^2 workload with its 40^2 region oversampled 5x (FP64):
+    its region blocks transform all five sub-rows of each base row at once, with a
+    second buffer per block and vector pair in global scratch.  Five vectors (three
+    pairs) must each give the statistic of a single evaluation within 1e-12."""
+    import imfit_amd
+    from imfit_amd import workloads
+    announce("population with oversampled region (1024^2, 5 vectors, fp64)")
+    w = workloads.cfg2()
+    g = w.fit_model(imfit_amd.PRECISION_FP64)
+    rng = np.random.default_rng(21)
+    P = w.start_params() * rng.uniform(0.97, 1.03, size=(5, w.truth.size))
+    batch = g.GetFitStatisticBatch(P)
+    for k in range(P.shape[0]):
+        single = g.GetFitStatistic(P[k])
+        assert abs(batch[k] - single) <= 1e-12 * abs(single), (k, batch[k], single)
